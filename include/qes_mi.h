@@ -1,0 +1,155 @@
+/*
+ * qes_mi.h -- C ABI of libqes_mi, the MI355X (gfx950) implementation of the
+ * symmetric-sector spin-1/2 Hamiltonian matvec and the Lanczos ground-state
+ * solve it drives.
+ *
+ * Plain C types only: opaque handles, host pointers for small tables,
+ * device pointers (from hipMalloc or a torch tensor's data_ptr()) for
+ * vectors, and hipStream_t passed as void*.  Every function returns 0 on
+ * success and -1 on error; qes_last_error() then holds a message.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * file:line relative to the reference package root Python/QES/.
+ */
+#ifndef QES_MI_H
+#define QES_MI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qes_basis_s* qes_basis_t;
+typedef struct qes_op_s* qes_op_t;
+typedef struct qes_lanczos_s* qes_lanczos_t;
+
+/* Base-op codes of one base group element (applied in generator order). */
+#define QES_BASE_OP_NONE 0
+#define QES_BASE_OP_REFLECT 1 /* reflection.py:143, full bit reversal        */
+#define QES_BASE_OP_FLIP 2    /* parity x/z, parity.py:46-54, complement     */
+#define QES_BASE_OP_FLIP_Y 3  /* parity y, parity.py:55-63, complement, +-i  */
+
+/*
+ * Symmetry sector description (one irrep of G = translations x base).
+ * Replaces SymmetryContainer.build_group/build_compiled_group and
+ * get_character (Algebra/Symmetries/symmetry_container.py:866-1174,
+ * :1399-1442).  chi[g] uses the reference element order
+ * g = (a*nty + b)*nbase + B, with a the x power, b the y power and B the
+ * base element (Algebra/Symmetries/symmetry_container.py:943-950).
+ */
+typedef struct {
+  int32_t ns;             /* sites = bits (<= 64)                                */
+  int32_t n_up;           /* U(1) popcount sector (globals.py:164-189), -1: none */
+  int32_t z2;             /* popcount-parity sector +-1, 0: none (globals.py:336) */
+  int32_t nbase;          /* number of base elements, 1..4                        */
+  uint8_t base_ops[4][4]; /* QES_BASE_OP_* per base element                       */
+  int32_t ntx;            /* x translation powers (Lx, or 1 if absent)            */
+  int32_t nty;            /* y translation powers (Ly, or 1 if absent)            */
+  int32_t row_bits;       /* bits per lattice row = n_basis * Lx                  */
+  int32_t cell_bits;      /* bits per unit cell = n_basis                         */
+  const double* chi_re;   /* host, ntx*nty*nbase                                  */
+  const double* chi_im;   /* host, ntx*nty*nbase                                  */
+} qes_group_spec;
+
+/*
+ * Build the sorted representative list and normalisations of a sector on the
+ * GPU.  Replaces SymmetryContainer.generate_symmetric_basis
+ * (Algebra/Symmetries/symmetry_container.py:1575-1765) with its kernels
+ * scan_chunk_find_representatives / fill_representatives /
+ * compute_normalization (Algebra/Symmetries/jit/symmetry_container_jit.py:154-255).
+ * *n_out receives the reduced dimension (HilbertSpace.nh, hilbert.py:1202).
+ */
+int qes_basis_create(const qes_group_spec* spec, qes_basis_t* out, int64_t* n_out);
+
+/* Host copies of the representative list and normalisations
+ * (CompactSymmetryData.representative_list / normalization,
+ *  symmetry_container.py:1746-1752). Either pointer may be NULL. */
+int qes_basis_get(qes_basis_t b, uint64_t* reps_host, double* norms_host);
+
+/* Device pointers of the same arrays (owned by the handle). */
+int qes_basis_device_ptrs(qes_basis_t b, const uint64_t** reps_dev, const double** norms_dev);
+
+/*
+ * Entries of the reference's compact index map for arbitrary states:
+ * idx_out[i] = repr_map[state] (rep index, -1 if not in the sector) and
+ * g_out[i] = first minimising group element, from which
+ * phase_idx = g_to_pidx[g] (symmetry_container_jit.py:248-255,
+ * symmetry_container.py:184-195, :1767-1804).  Host arrays, any length.
+ */
+int qes_basis_map_states(qes_basis_t b, const uint64_t* states_host, int64_t n, int64_t* idx_out, int32_t* g_out);
+
+int qes_basis_destroy(qes_basis_t b);
+
+/*
+ * Compile a spin-1/2 instruction list bound to a basis.  The arrays are the
+ * ones SpecialOperator._finalize_instruction_arrays builds
+ * (Algebra/Operator/special_operator.py:1392-1417): codes[T] (SpinLookupCodes,
+ * operators_spin.py:1084-1118), sites[T][2] (-1 padded), coefficients
+ * (re/im; im may be NULL), evaluated with the semantics of
+ * sigma_composition_integer (operators_spin.py:2151-2354);
+ * is_complex selects the complex build (real build keeps Re of each
+ * operator value).  coupling[T] (may be NULL) tags each term with a coupling
+ * id 0..7 for qes_matvec_couplings.
+ */
+int qes_op_create(qes_basis_t b, const int32_t* codes, const int32_t* sites, const double* coeff_re,
+                  const double* coeff_im, int32_t n_terms, int32_t is_complex, const int32_t* coupling,
+                  qes_op_t* out);
+int qes_op_destroy(qes_op_t op);
+
+/* 1 if the operator needs complex128 vectors (complex values or characters). */
+int qes_op_needs_complex(qes_op_t op, int32_t* needs);
+
+/*
+ * y = H x on device vectors, x and y of shape (n, nbatch) row-major.
+ * dtype 0: float64, 1: complex128 (interleaved).  accumulate != 0 adds into y.
+ * Replaces SpecialOperator.matvec_fun (special_operator.py:593-709) and its
+ * kernel _apply_op_batch_compact_seq_jit (Algebra/Hilbert/matrix_builder.py:1347-1377),
+ * and Operator.matvec(..., hilbert_in=...) (Algebra/Operator/operator.py:1559-1602).
+ */
+int qes_matvec(qes_op_t op, const void* x_dev, void* y_dev, int64_t nbatch, int32_t dtype, int32_t accumulate,
+               void* stream);
+
+/* y[c] = H_c x for every coupling id c (y has n_couplings * n entries). */
+int qes_matvec_couplings(qes_op_t op, const void* x_dev, void* y_dev, int32_t dtype, void* stream);
+int qes_op_n_couplings(qes_op_t op, int32_t* n);
+
+/* Number of off-diagonal gathers of one matvec (E_off) -- stats pass. */
+int qes_op_count_offdiag(qes_op_t op, int64_t* e_off);
+
+/*
+ * Device Lanczos (FP64).  Replaces the eigensolver that
+ * DiagonalizationEngine.diagonalize dispatches to
+ * (Algebra/Hamil/hamil_diag_engine.py:176-300, called from
+ * Algebra/hamil.py:2063-2081 with matvec=matvec_fun).
+ * store_basis keeps all Lanczos vectors on the device (needed for full
+ * reorthogonalisation and for Ritz vectors); reorth != 0 applies classical
+ * Gram-Schmidt twice against all stored vectors each step.
+ */
+int qes_lanczos_create(qes_op_t op, int32_t max_steps, int32_t store_basis, int32_t dtype, qes_lanczos_t* out);
+int qes_lanczos_start(qes_lanczos_t L, const void* v0_dev, void* stream);
+int qes_lanczos_run(qes_lanczos_t L, int32_t nsteps, int32_t reorth, int32_t use_graph);
+int qes_lanczos_get(qes_lanczos_t L, double* alpha_host, double* beta_host, int32_t* steps_done);
+int qes_lanczos_ritz(qes_lanczos_t L, const double* coeffs_host, int32_t k, void* out_dev, void* stream);
+int qes_lanczos_destroy(qes_lanczos_t L);
+
+/*
+ * All-in-one ground state: run up to max_steps Lanczos steps from v0 (device),
+ * checking the lowest Ritz value every check_every steps; stop when it moves
+ * by less than tol*|E0| or on breakdown.  alpha/beta (host, max_steps) and
+ * *e0 receive the result.
+ */
+int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int32_t max_steps, int32_t reorth,
+                             double tol, int32_t check_every, double* alpha_host, double* beta_host,
+                             int32_t* steps_done, double* e0, void* stream);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* qes_last_error(void);
+
+/* Library version string. */
+const char* qes_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QES_MI_H */

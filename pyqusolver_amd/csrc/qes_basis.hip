@@ -1,0 +1,293 @@
+// Basis construction on the GPU: sorted representatives and normalisations.
+//
+// Replaces the 2^Ns scan of scan_chunk_find_representatives
+// (Algebra/Symmetries/jit/symmetry_container_jit.py:204-255),
+// fill_representatives (:193-199) and compute_normalization (:154-187).
+// Instead of scanning every state into a dense 2^Ns map, only states that
+// pass the global constraint are enumerated, in increasing order (unranking
+// in the combinatorial number system + next-combination stepping for U(1)),
+// each tested for "s == min_g g(s)" with an early exit, and the survivors
+// are stream-compacted in order, so the list comes out sorted without a sort.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "qes_internal.h"
+
+namespace qes {
+
+__constant__ uint64_t c_binom[65][33];   // C(n, k) for k <= 32 (symmetric use)
+
+static bool g_binom_ready = false;
+
+static uint64_t host_binom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  if (k > n - k) k = n - k;
+  unsigned __int128 r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (unsigned)(n - k + i) / (unsigned)i;
+  return (uint64_t)r;
+}
+
+static int ensure_binom() {
+  if (g_binom_ready) return 0;
+  static uint64_t tab[65][33];
+  for (int n = 0; n <= 64; ++n)
+    for (int k = 0; k <= 32; ++k) tab[n][k] = host_binom(n, k);
+  QES_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), tab, sizeof(tab)));
+  g_binom_ready = true;
+  return 0;
+}
+
+__device__ __forceinline__ uint64_t dbinom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  if (k > n - k) k = n - k;
+  return c_binom[n][k];
+}
+
+// k-th (0-based) ns-bit number with popcount m, in increasing numeric order
+__device__ __forceinline__ uint64_t unrank_combination(uint64_t k, int ns, int m) {
+  uint64_t s = 0;
+  for (int p = ns - 1; p >= 0 && m > 0; --p) {
+    uint64_t c = dbinom(p, m);   // numbers below 2^p with popcount m
+    if (k >= c) {
+      s |= 1ull << p;
+      k -= c;
+      --m;
+    }
+  }
+  return s;
+}
+
+// next larger number with the same popcount (s != 0)
+__device__ __forceinline__ uint64_t next_combination(uint64_t s) {
+  uint64_t t = s | (s - 1);
+  return (t + 1) | (((~t & (t + 1)) - 1) >> (__builtin_ctzll(s) + 1));
+}
+
+__device__ __forceinline__ bool passes_globals(const QesGroup& G, uint64_t s) {
+  if (G.z2 != 0) {
+    int par = (__popcll(s) & 1) ? -1 : 1;
+    if (par != G.z2) return false;
+  }
+  return true;
+}
+
+// Thread t owns candidates [t*per, min((t+1)*per, total)).
+template <typename W, bool WRITE>
+__global__ void __launch_bounds__(256) basis_scan_kernel(QesGroup G, const double* __restrict__ chi_re,
+                                                         const double* __restrict__ chi_im, uint64_t total,
+                                                         uint32_t per, uint32_t* __restrict__ block_counts,
+                                                         const int64_t* __restrict__ block_offsets,
+                                                         uint64_t* __restrict__ reps, double* __restrict__ norms) {
+  __shared__ uint32_t s_scan[256];
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t begin = t * per;
+  uint64_t end = begin + per;
+  if (end > total) end = total;
+  uint32_t cnt = 0;
+  const bool u1 = G.n_up >= 0;
+
+  // pass 1 (count) and pass 2 (write) run the identical enumeration
+  int64_t out = 0;
+  if (WRITE) {
+    // count first so the block scan can place this thread's outputs
+    uint64_t s = 0;
+    if (begin < end) s = u1 ? unrank_combination(begin, G.ns, G.n_up) : begin;
+    for (uint64_t r = begin; r < end; ++r) {
+      if (passes_globals(G, s)) {
+        double pre, pim;
+        if (rep_test<W>(G, chi_re, chi_im, (W)s, &pre, &pim)) {
+          if (hypot(pre, pim) > 1e-7) ++cnt;
+        }
+      }
+      s = u1 ? (r + 1 < end ? next_combination(s) : s) : s + 1;
+    }
+    s_scan[threadIdx.x] = cnt;
+    __syncthreads();
+    // exclusive scan over the block (256 entries, simple Hillis-Steele)
+    for (int off = 1; off < 256; off <<= 1) {
+      uint32_t v = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0;
+      __syncthreads();
+      s_scan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    out = block_offsets[blockIdx.x] + (int64_t)(s_scan[threadIdx.x] - cnt);
+    if (cnt == 0) return;
+    s = u1 ? unrank_combination(begin, G.ns, G.n_up) : begin;
+    for (uint64_t r = begin; r < end; ++r) {
+      if (passes_globals(G, s)) {
+        double pre, pim;
+        if (rep_test<W>(G, chi_re, chi_im, (W)s, &pre, &pim)) {
+          double a = hypot(pre, pim);
+          if (a > 1e-7) {
+            reps[out] = s;
+            norms[out] = sqrt(a);
+            ++out;
+          }
+        }
+      }
+      s = u1 ? (r + 1 < end ? next_combination(s) : s) : s + 1;
+    }
+  } else {
+    uint64_t s = 0;
+    if (begin < end) s = u1 ? unrank_combination(begin, G.ns, G.n_up) : begin;
+    for (uint64_t r = begin; r < end; ++r) {
+      if (passes_globals(G, s)) {
+        double pre, pim;
+        if (rep_test<W>(G, chi_re, chi_im, (W)s, &pre, &pim)) {
+          if (hypot(pre, pim) > 1e-7) ++cnt;
+        }
+      }
+      s = u1 ? (r + 1 < end ? next_combination(s) : s) : s + 1;
+    }
+    s_scan[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if (threadIdx.x < (unsigned)off) s_scan[threadIdx.x] += s_scan[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = s_scan[0];
+  }
+}
+
+__global__ void keys32_kernel(const uint64_t* __restrict__ reps, uint32_t* __restrict__ keys, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = (uint32_t)reps[i];
+}
+
+__device__ __forceinline__ int64_t rel_bucket(uint64_t v, int mbits, int e_min) {
+  int e;
+  uint32_t k = bucket_of(v, mbits, &e);
+  return (int64_t)k - ((int64_t)e_min << mbits);
+}
+
+// bucket[k] = first index i with key(reps[i]) >= k
+__global__ void bucket_kernel(const uint64_t* __restrict__ reps, int64_t n, int mbits, int e_min,
+                              int64_t nbucket, uint32_t* __restrict__ bucket) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  int64_t kprev = (i == 0) ? -1 : rel_bucket(reps[i - 1], mbits, e_min);
+  int64_t kcur = (i == n) ? nbucket : rel_bucket(reps[i], mbits, e_min);
+  for (int64_t k = kprev + 1; k <= kcur && k <= nbucket; ++k) bucket[k] = (uint32_t)i;
+}
+
+static int bitlen(uint64_t v) { return v ? 64 - __builtin_clzll(v) : 0; }
+
+int build_basis(Basis* B, hipStream_t st) {
+  if (ensure_binom()) return -1;
+  const QesGroup& G = B->group;
+  uint64_t total;
+  if (G.n_up >= 0) {
+    if (G.n_up > G.ns) return fail("n_up > ns");
+    total = host_binom(G.ns, G.n_up);
+  } else {
+    if (G.ns > 40) return fail("sector without U(1) limited to ns <= 40");
+    total = 1ull << G.ns;
+  }
+  const uint32_t per = total > (1ull << 28) ? 256 : (total > (1ull << 20) ? 64 : 8);
+  const uint64_t nthreads = (total + per - 1) / per;
+  const uint64_t nblocks = (nthreads + 255) / 256;
+  if (nblocks > 0x7fffffffull) return fail("basis enumeration too large");
+  uint32_t* d_counts = nullptr;
+  int64_t* d_offsets = nullptr;
+  QES_HIP(hipMalloc(&d_counts, std::max<uint64_t>(nblocks, 1) * sizeof(uint32_t)));
+  QES_HIP(hipMalloc(&d_offsets, std::max<uint64_t>(nblocks, 1) * sizeof(int64_t)));
+  const bool w32 = G.ns <= 32;
+  if (nblocks > 0) {
+    if (w32)
+      hipLaunchKernelGGL((basis_scan_kernel<uint32_t, false>), dim3((unsigned)nblocks), dim3(256), 0, st, G,
+                         B->chi_re, B->chi_im, total, per, d_counts, nullptr, nullptr, nullptr);
+    else
+      hipLaunchKernelGGL((basis_scan_kernel<uint64_t, false>), dim3((unsigned)nblocks), dim3(256), 0, st, G,
+                         B->chi_re, B->chi_im, total, per, d_counts, nullptr, nullptr, nullptr);
+    QES_HIP(hipGetLastError());
+  }
+  std::vector<uint32_t> counts(nblocks);
+  std::vector<int64_t> offsets(nblocks);
+  if (nblocks) QES_HIP(hipMemcpyAsync(counts.data(), d_counts, nblocks * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  QES_HIP(hipStreamSynchronize(st));
+  int64_t acc = 0;
+  for (uint64_t b = 0; b < nblocks; ++b) {
+    offsets[b] = acc;
+    acc += counts[b];
+  }
+  B->n = acc;
+  if (nblocks) QES_HIP(hipMemcpyAsync(d_offsets, offsets.data(), nblocks * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  const int64_t nalloc = std::max<int64_t>(acc, 1);
+  QES_HIP(hipMalloc(&B->reps, nalloc * sizeof(uint64_t)));
+  QES_HIP(hipMalloc(&B->norms, nalloc * sizeof(double)));
+  if (nblocks > 0 && acc > 0) {
+    if (w32)
+      hipLaunchKernelGGL((basis_scan_kernel<uint32_t, true>), dim3((unsigned)nblocks), dim3(256), 0, st, G,
+                         B->chi_re, B->chi_im, total, per, nullptr, d_offsets, B->reps, B->norms);
+    else
+      hipLaunchKernelGGL((basis_scan_kernel<uint64_t, true>), dim3((unsigned)nblocks), dim3(256), 0, st, G,
+                         B->chi_re, B->chi_im, total, per, nullptr, d_offsets, B->reps, B->norms);
+    QES_HIP(hipGetLastError());
+  }
+  QES_HIP(hipStreamSynchronize(st));
+  QES_HIP(hipFree(d_counts));
+  QES_HIP(hipFree(d_offsets));
+
+  // lookup structures
+  const int64_t n = B->n;
+  if (w32) {
+    QES_HIP(hipMalloc(&B->keys32, nalloc * sizeof(uint32_t)));
+    if (n) hipLaunchKernelGGL(keys32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B->reps, B->keys32, n);
+  }
+  uint64_t lo_rep = 0, hi_rep = 0;
+  if (n) {
+    QES_HIP(hipMemcpyAsync(&lo_rep, B->reps, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    QES_HIP(hipMemcpyAsync(&hi_rep, B->reps + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    QES_HIP(hipStreamSynchronize(st));
+  }
+  B->e_min = bitlen(lo_rep);
+  B->e_max = bitlen(hi_rep);
+  const int ne = B->e_max - B->e_min + 1;
+  int mbits = 0;
+  while (mbits < 24 && ((int64_t)ne << (mbits + 1)) <= std::max<int64_t>(n / 2, 1)) ++mbits;
+  B->mbits = mbits;
+  B->nbucket = (int64_t)ne << mbits;
+  QES_HIP(hipMalloc(&B->bucket, (B->nbucket + 1) * sizeof(uint32_t)));
+  hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, st, B->reps, n, mbits,
+                     B->e_min, B->nbucket, B->bucket);
+  QES_HIP(hipGetLastError());
+  QES_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+template <typename W>
+__global__ void map_states_kernel(QesGroup G, QesLookup L, const uint64_t* __restrict__ states, int64_t n,
+                                  int64_t* __restrict__ idx, int32_t* __restrict__ gmin, int32_t* __restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t s = states[i];
+  int64_t id = -1;
+  int32_t g = -1, c = 0;
+  bool ok = (s & ~G.mask) == 0 && passes_globals(G, s) && (G.n_up < 0 || __popcll(s) == G.n_up);
+  if (ok) {
+    Canon<W> cn = canonicalize<W>(G, (W)s);
+    id = lookup_rep<W>(L, cn.rep);
+    g = cn.g;
+    c = cn.count;
+  }
+  idx[i] = id;
+  gmin[i] = id >= 0 ? g : -1;
+  if (cnt) cnt[i] = id >= 0 ? c : 0;
+}
+
+int map_states(const Basis* B, const uint64_t* d_states, int64_t n, int64_t* d_idx, int32_t* d_g, int32_t* d_cnt,
+               hipStream_t st) {
+  if (n <= 0) return 0;
+  dim3 grid((unsigned)((n + 255) / 256));
+  if (B->group.ns <= 32)
+    hipLaunchKernelGGL(map_states_kernel<uint32_t>, grid, dim3(256), 0, st, B->group, B->lookup(), d_states, n,
+                       d_idx, d_g, d_cnt);
+  else
+    hipLaunchKernelGGL(map_states_kernel<uint64_t>, grid, dim3(256), 0, st, B->group, B->lookup(), d_states, n,
+                       d_idx, d_g, d_cnt);
+  QES_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace qes

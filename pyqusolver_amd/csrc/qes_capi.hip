@@ -1,0 +1,461 @@
+// extern "C" boundary of libqes_mi (declarations in include/qes_mi.h).
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/qes_mi.h"
+#include "qes_internal.h"
+#include "qes_lanczos.h"
+
+namespace qes {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(const std::string& msg) {
+  g_err = msg;
+  return -1;
+}
+int check_hip(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Spin-1/2 instruction semantics (Algebra/Operator/impl/operators_spin.py):
+// site i addresses bit ns-1-i (:153); spin value 0.5; bit 1 = up.
+// ---------------------------------------------------------------------------
+using cplx = std::complex<double>;
+
+enum {
+  SX_L = 11, SY_L = 12, SZ_L = 13, SP_L = 14, SM_L = 15,
+  SX_C = 20, SY_C = 21, SZ_C = 22, SP_C = 23, SM_C = 24,
+  SXY_C = 25, SYX_C = 26, SXZ_C = 27, SZX_C = 28, SYZ_C = 29, SZY_C = 30, SPM_C = 31, SMP_C = 32
+};
+
+// one elementary spin operator on one site: returns false if it annihilates
+static bool elem(char kind, uint64_t& s, int ns, int site, cplx& c) {
+  const int pos = ns - 1 - site;
+  const int b = (int)((s >> pos) & 1ull);
+  switch (kind) {
+    case 'x': s ^= 1ull << pos; c *= 0.5; return true;                               // :147-157
+    case 'y': c *= cplx(0.0, (2.0 * b - 1.0) * 0.5); s ^= 1ull << pos; return true;   // :269-281
+    case 'z': c *= (2.0 * b - 1.0) * 0.5; return true;                                 // :439-447
+    case 'p': if (b) { c = 0.0; return false; } s ^= 1ull << pos; c *= 0.5; return true;   // :627-645
+    case 'm': if (!b) { c = 0.0; return false; } s ^= 1ull << pos; c *= 0.5; return true;  // :698-737
+  }
+  return false;
+}
+
+// sigma_operator_composition_single_op (:2165-2313); returns is_diagonal
+static bool apply_code(int code, uint64_t& s, int ns, int s1, int s2, cplx& c, bool& ok) {
+  c = 1.0;
+  ok = true;
+  const char* k = nullptr;
+  bool diag = false;
+  switch (code) {
+    case SX_L: k = "x"; break;
+    case SY_L: k = "y"; break;
+    case SZ_L: k = "z"; diag = true; break;
+    case SP_L: k = "p"; break;
+    case SM_L: k = "m"; break;
+    case SX_C: k = "xx"; break;
+    case SY_C: k = "yy"; break;
+    case SZ_C: k = "zz"; diag = true; break;
+    case SP_C: k = "pp"; break;
+    case SM_C: k = "mm"; break;
+    case SXY_C: k = "xy"; break;
+    case SYX_C: k = "yx"; break;
+    case SXZ_C: k = "xz"; break;
+    case SZX_C: k = "zx"; break;
+    case SYZ_C: k = "yz"; break;
+    case SZY_C: k = "zy"; break;
+    case SPM_C: k = "pm"; break;
+    case SMP_C: k = "mp"; break;
+    default: ok = false; return false;
+  }
+  const int sites[2] = {s1, s2};
+  const int nk = (int)strlen(k);
+  for (int i = 0; i < nk; ++i) {
+    if (sites[i] < 0 || sites[i] >= ns) { ok = false; return false; }
+    if (!elem(k[i], s, ns, sites[i], c)) break;
+  }
+  return diag;
+}
+
+struct GroupKey {
+  uint64_t mask;
+  int p0, p1, cpl;
+  bool operator<(const GroupKey& o) const {
+    return std::tie(mask, p0, p1, cpl) < std::tie(o.mask, o.p0, o.p1, o.cpl);
+  }
+};
+
+static int compile_terms(int ns, const int32_t* codes, const int32_t* sites, const double* cre, const double* cim,
+                         int T, bool is_complex, const int32_t* coupling, std::vector<QesTermGroup>& diag,
+                         std::vector<QesTermGroup>& off, int* ncpl, bool* has_imag) {
+  std::map<GroupKey, QesTermGroup> dmap, omap;
+  std::vector<GroupKey> dorder, oorder;
+  *ncpl = 1;
+  *has_imag = false;
+  for (int t = 0; t < T; ++t) {
+    const int s1 = sites[2 * t], s2 = sites[2 * t + 1];
+    const cplx coef(cre[t], cim ? cim[t] : 0.0);
+    const int cp = coupling ? coupling[t] : 0;
+    if (cp < 0 || cp > 7) return fail("coupling id must be in 0..7");
+    if (cp + 1 > *ncpl) *ncpl = cp + 1;
+    const int p0 = (s1 >= 0 && s1 < ns) ? ns - 1 - s1 : -1;
+    const int p1 = (s2 >= 0 && s2 < ns) ? ns - 1 - s2 : -1;
+    if (p0 < 0) return fail("term " + std::to_string(t) + ": first site out of range");
+    // enumerate the bits at the read sites
+    double vre[4] = {0, 0, 0, 0}, vim[4] = {0, 0, 0, 0};
+    uint64_t mask = 0;
+    bool mask_set = false, is_diag = false;
+    for (int k = 0; k < 4; ++k) {
+      const int b0 = (k >> 1) & 1, b1 = k & 1;
+      if (p1 < 0 && b1) continue;
+      uint64_t s = ((uint64_t)b0 << p0) | (p1 >= 0 ? ((uint64_t)b1 << p1) : 0ull);
+      if (p1 >= 0 && p1 == p0 && b0 != b1) continue;   // same site: bits must agree
+      const uint64_t s_in = s;
+      cplx c;
+      bool ok;
+      is_diag = apply_code(codes[t], s, ns, s1, s2, c, ok);
+      if (!ok) return fail("term " + std::to_string(t) + ": unsupported code " + std::to_string(codes[t]));
+      cplx val = is_complex ? c * coef : cplx(c.real() * coef.real(), 0.0);
+      if (std::abs(val) < 1e-15) val = 0.0;   // operators_spin.py:2335
+      if (val != 0.0) {
+        const uint64_t m = s ^ s_in;
+        if (mask_set && m != mask) return fail("inconsistent flip mask");
+        mask = m;
+        mask_set = true;
+      }
+      vre[k] = val.real();
+      vim[k] = val.imag();
+      if (p1 < 0) {   // single read site: duplicate over the unused bit
+        vre[k | 1] = vre[k];
+        vim[k | 1] = vim[k];
+      }
+    }
+    if (!mask_set) continue;   // identically zero term
+    const bool as_diag = is_diag || mask == 0;
+    GroupKey key{as_diag ? 0ull : mask, p0, p1, cp};
+    auto& mp = as_diag ? dmap : omap;
+    auto& order = as_diag ? dorder : oorder;
+    auto it = mp.find(key);
+    if (it == mp.end()) {
+      QesTermGroup g{};
+      g.mask = key.mask;
+      g.p0 = p0;
+      g.p1 = p1;
+      g.coupling = cp;
+      for (int k = 0; k < 4; ++k) {
+        g.re[k] = 0.0;
+        g.im[k] = 0.0;
+      }
+      it = mp.emplace(key, g).first;
+      order.push_back(key);
+    }
+    for (int k = 0; k < 4; ++k) {
+      it->second.re[k] += vre[k];
+      it->second.im[k] += vim[k];
+    }
+  }
+  for (auto& k : dorder) diag.push_back(dmap[k]);
+  for (auto& k : oorder) {
+    QesTermGroup g = omap[k];
+    bool nz = false;
+    for (int q = 0; q < 4; ++q) {
+      if (std::fabs(g.re[q]) < 1e-15) g.re[q] = 0.0;
+      if (std::fabs(g.im[q]) < 1e-15) g.im[q] = 0.0;
+      nz = nz || g.re[q] != 0.0 || g.im[q] != 0.0;
+    }
+    if (nz) off.push_back(g);
+  }
+  for (auto& g : diag)
+    for (int q = 0; q < 4; ++q) *has_imag = *has_imag || g.im[q] != 0.0;
+  for (auto& g : off)
+    for (int q = 0; q < 4; ++q) *has_imag = *has_imag || g.im[q] != 0.0;
+  return 0;
+}
+
+}  // namespace qes
+
+using namespace qes;
+
+struct qes_basis_s {
+  Basis b;
+};
+struct qes_op_s {
+  Op op;
+};
+struct qes_lanczos_s {
+  Lanczos* L;
+};
+
+extern "C" {
+
+const char* qes_last_error(void) { return g_err.c_str(); }
+const char* qes_version(void) { return "qes_mi 0.1 gfx950"; }
+
+int qes_basis_create(const qes_group_spec* spec, qes_basis_t* out, int64_t* n_out) {
+  if (!spec || !out) return fail("null argument");
+  if (spec->ns < 1 || spec->ns > 64) return fail("ns must be in 1..64");
+  if (spec->nbase < 1 || spec->nbase > QES_MAX_BASE) return fail("nbase must be in 1..4");
+  if (spec->ntx < 1 || spec->nty < 1 || spec->ntx > QES_MAX_TX) return fail("bad translation extents");
+  const int gs = spec->ntx * spec->nty * spec->nbase;
+  if (gs > 256) return fail("group order above 256 is not supported");
+  if (spec->row_bits < 1 || spec->ns % spec->row_bits != 0 || (spec->nty > 1 && spec->row_bits * spec->nty != spec->ns))
+    return fail("row_bits must divide ns (and row_bits * nty == ns with y translations)");
+  if (spec->cell_bits < 1 || spec->ntx * spec->cell_bits > spec->row_bits) return fail("bad cell_bits");
+  if (spec->n_up > spec->ns) return fail("n_up > ns");
+  auto* h = new qes_basis_s();
+  Basis& B = h->b;
+  QesGroup& G = B.group;
+  memset(&G, 0, sizeof(G));
+  G.ns = spec->ns;
+  G.nbase = spec->nbase;
+  G.ntx = spec->ntx;
+  G.nty = spec->nty;
+  G.row_bits = spec->row_bits;
+  G.cell_bits = spec->cell_bits;
+  G.n_up = spec->n_up;
+  G.z2 = spec->z2;
+  G.mask = spec->ns == 64 ? ~0ull : ((1ull << spec->ns) - 1ull);
+  G.has_phase = 0;
+  for (int b = 0; b < QES_MAX_BASE; ++b)
+    for (int k = 0; k < QES_MAX_BASE_OPS; ++k) {
+      G.base_ops[b][k] = b < spec->nbase ? spec->base_ops[b][k] : 0;
+      if (G.base_ops[b][k] == QES_BOP_FLIPY) G.has_phase = 1;
+    }
+  for (int a = 0; a < spec->ntx; ++a) {
+    const int r = a * spec->cell_bits;
+    uint64_t lo = 0;
+    for (int row = 0; row < spec->ns / spec->row_bits; ++row)
+      lo |= (r ? ((1ull << r) - 1ull) : 0ull) << (row * spec->row_bits);
+    G.rowlo[a] = lo;
+  }
+  B.gsize = gs;
+  B.h_chi_re.assign(spec->chi_re, spec->chi_re + gs);
+  B.h_chi_im.assign(spec->chi_im, spec->chi_im + gs);
+  hipStream_t st = nullptr;
+  int rc = 0;
+  if (hipMalloc(&B.chi_re, gs * sizeof(double)) != hipSuccess || hipMalloc(&B.chi_im, gs * sizeof(double)) != hipSuccess)
+    rc = fail("hipMalloc failed for characters");
+  if (!rc && hipMemcpy(B.chi_re, spec->chi_re, gs * 8, hipMemcpyHostToDevice) != hipSuccess) rc = fail("copy chi");
+  if (!rc && hipMemcpy(B.chi_im, spec->chi_im, gs * 8, hipMemcpyHostToDevice) != hipSuccess) rc = fail("copy chi");
+  if (!rc) rc = build_basis(&B, st);
+  if (rc) {
+    qes_basis_destroy(h);
+    return -1;
+  }
+  *out = h;
+  if (n_out) *n_out = B.n;
+  return 0;
+}
+
+int qes_basis_get(qes_basis_t h, uint64_t* reps_host, double* norms_host) {
+  if (!h) return fail("null basis");
+  const int64_t n = h->b.n;
+  if (n == 0) return 0;
+  if (reps_host) QES_HIP(hipMemcpy(reps_host, h->b.reps, n * 8, hipMemcpyDeviceToHost));
+  if (norms_host) QES_HIP(hipMemcpy(norms_host, h->b.norms, n * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int qes_basis_device_ptrs(qes_basis_t h, const uint64_t** reps_dev, const double** norms_dev) {
+  if (!h) return fail("null basis");
+  if (reps_dev) *reps_dev = h->b.reps;
+  if (norms_dev) *norms_dev = h->b.norms;
+  return 0;
+}
+
+int qes_basis_map_states(qes_basis_t h, const uint64_t* states_host, int64_t n, int64_t* idx_out, int32_t* g_out) {
+  if (!h) return fail("null basis");
+  if (n <= 0) return 0;
+  uint64_t* ds = nullptr;
+  int64_t* di = nullptr;
+  int32_t* dg = nullptr;
+  QES_HIP(hipMalloc(&ds, n * 8));
+  QES_HIP(hipMalloc(&di, n * 8));
+  QES_HIP(hipMalloc(&dg, n * 4));
+  QES_HIP(hipMemcpy(ds, states_host, n * 8, hipMemcpyHostToDevice));
+  int rc = map_states(&h->b, ds, n, di, dg, nullptr, nullptr);
+  if (!rc) {
+    QES_HIP(hipDeviceSynchronize());
+    QES_HIP(hipMemcpy(idx_out, di, n * 8, hipMemcpyDeviceToHost));
+    if (g_out) QES_HIP(hipMemcpy(g_out, dg, n * 4, hipMemcpyDeviceToHost));
+  }
+  hipFree(ds);
+  hipFree(di);
+  hipFree(dg);
+  return rc;
+}
+
+int qes_basis_destroy(qes_basis_t h) {
+  if (!h) return 0;
+  Basis& B = h->b;
+  if (B.chi_re) hipFree(B.chi_re);
+  if (B.chi_im) hipFree(B.chi_im);
+  if (B.reps) hipFree(B.reps);
+  if (B.keys32) hipFree(B.keys32);
+  if (B.norms) hipFree(B.norms);
+  if (B.bucket) hipFree(B.bucket);
+  delete h;
+  return 0;
+}
+
+int qes_op_create(qes_basis_t bh, const int32_t* codes, const int32_t* sites, const double* coeff_re,
+                  const double* coeff_im, int32_t n_terms, int32_t is_complex, const int32_t* coupling,
+                  qes_op_t* out) {
+  if (!bh || !out) return fail("null argument");
+  if (n_terms < 0 || (n_terms > 0 && (!codes || !sites || !coeff_re))) return fail("bad term arrays");
+  std::vector<QesTermGroup> diag, off;
+  int ncpl = 1;
+  bool has_imag = false;
+  if (compile_terms(bh->b.group.ns, codes, sites, coeff_re, coeff_im, n_terms, is_complex != 0, coupling, diag, off,
+                    &ncpl, &has_imag))
+    return -1;
+  if (diag.size() + off.size() > QES_MAX_TERMS) return fail("too many term groups");
+  auto* h = new qes_op_s();
+  Op& op = h->op;
+  op.basis = &bh->b;
+  op.n_diag = (int)diag.size();
+  op.n_off = (int)off.size();
+  op.n_coupling = ncpl;
+  op.is_complex = has_imag;
+  op.chi_complex = false;
+  for (int g = 0; g < bh->b.gsize; ++g)
+    if (std::fabs(bh->b.h_chi_im[g]) > 1e-12) op.chi_complex = true;
+  op.h_groups = diag;
+  op.h_groups.insert(op.h_groups.end(), off.begin(), off.end());
+  const size_t bytes = std::max<size_t>(op.h_groups.size(), 1) * sizeof(QesTermGroup);
+  if (hipMalloc(&op.d_groups, bytes) != hipSuccess) {
+    delete h;
+    return fail("hipMalloc failed for term groups");
+  }
+  if (!op.h_groups.empty() &&
+      hipMemcpy(op.d_groups, op.h_groups.data(), op.h_groups.size() * sizeof(QesTermGroup), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    hipFree(op.d_groups);
+    delete h;
+    return fail("copy of term groups failed");
+  }
+  *out = h;
+  return 0;
+}
+
+int qes_op_destroy(qes_op_t h) {
+  if (!h) return 0;
+  if (h->op.d_groups) hipFree(h->op.d_groups);
+  delete h;
+  return 0;
+}
+
+int qes_op_needs_complex(qes_op_t h, int32_t* needs) {
+  if (!h || !needs) return fail("null argument");
+  *needs = (h->op.is_complex || h->op.chi_complex) ? 1 : 0;
+  return 0;
+}
+
+int qes_op_n_couplings(qes_op_t h, int32_t* n) {
+  if (!h || !n) return fail("null argument");
+  *n = h->op.n_coupling;
+  return 0;
+}
+
+int qes_matvec(qes_op_t h, const void* x_dev, void* y_dev, int64_t nbatch, int32_t dtype, int32_t accumulate,
+               void* stream) {
+  if (!h) return fail("null operator");
+  if (dtype != 0 && dtype != 1) return fail("dtype must be 0 (float64) or 1 (complex128)");
+  if (h->op.basis->n > 0 && (!x_dev || !y_dev)) return fail("null vector");
+  if (x_dev == y_dev && h->op.basis->n > 0) return fail("x and y must not alias");
+  return matvec(&h->op, x_dev, y_dev, nbatch, dtype, accumulate, (hipStream_t)stream);
+}
+
+int qes_matvec_couplings(qes_op_t h, const void* x_dev, void* y_dev, int32_t dtype, void* stream) {
+  if (!h) return fail("null operator");
+  return matvec_couplings(&h->op, x_dev, y_dev, dtype, (hipStream_t)stream);
+}
+
+int qes_op_count_offdiag(qes_op_t h, int64_t* e_off) {
+  if (!h || !e_off) return fail("null argument");
+  return count_offdiag(&h->op, e_off, nullptr);
+}
+
+int qes_lanczos_create(qes_op_t op, int32_t max_steps, int32_t store_basis, int32_t dtype, qes_lanczos_t* out) {
+  if (!op || !out) return fail("null argument");
+  Lanczos* L = nullptr;
+  if (lanczos_create(&op->op, max_steps, store_basis, dtype, &L)) return -1;
+  auto* h = new qes_lanczos_s();
+  h->L = L;
+  *out = h;
+  return 0;
+}
+
+int qes_lanczos_start(qes_lanczos_t h, const void* v0_dev, void* stream) {
+  if (!h || !v0_dev) return fail("null argument");
+  return lanczos_start(h->L, v0_dev, (hipStream_t)stream);
+}
+
+int qes_lanczos_run(qes_lanczos_t h, int32_t nsteps, int32_t reorth, int32_t use_graph) {
+  if (!h) return fail("null argument");
+  return lanczos_run(h->L, nsteps, reorth, use_graph);
+}
+
+int qes_lanczos_get(qes_lanczos_t h, double* alpha_host, double* beta_host, int32_t* steps_done) {
+  if (!h) return fail("null argument");
+  if (steps_done) *steps_done = lanczos_steps_done(h->L);
+  return lanczos_get(h->L, alpha_host, beta_host);
+}
+
+int qes_lanczos_ritz(qes_lanczos_t h, const double* coeffs_host, int32_t k, void* out_dev, void* stream) {
+  if (!h || !coeffs_host || !out_dev) return fail("null argument");
+  return lanczos_combine(h->L, coeffs_host, k, out_dev, (hipStream_t)stream);
+}
+
+int qes_lanczos_destroy(qes_lanczos_t h) {
+  if (!h) return 0;
+  lanczos_destroy(h->L);
+  delete h;
+  return 0;
+}
+
+int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int32_t max_steps, int32_t reorth,
+                             double tol, int32_t check_every, double* alpha_host, double* beta_host,
+                             int32_t* steps_done, double* e0, void* stream) {
+  if (!op || !v0_dev || !alpha_host || !beta_host) return fail("null argument");
+  if (check_every < 1) check_every = 10;
+  Lanczos* L = nullptr;
+  if (lanczos_create(&op->op, max_steps, reorth ? 1 : 0, dtype, &L)) return -1;
+  int rc = lanczos_start(L, v0_dev, (hipStream_t)stream);
+  double prev = 1e300, cur = 0.0;
+  std::vector<double> a(max_steps), b(max_steps);
+  while (!rc && lanczos_steps_done(L) < max_steps) {
+    rc = lanczos_run(L, check_every, reorth, 1);
+    if (rc) break;
+    const int k = lanczos_steps_done(L);
+    rc = lanczos_get(L, a.data(), b.data());
+    if (rc) break;
+    cur = tridiag_lowest(a.data(), b.data(), k);
+    bool breakdown = false;
+    for (int i = 0; i < k; ++i)
+      if (!(std::fabs(b[i]) > 1e-14 * std::max(1.0, std::fabs(cur)))) breakdown = true;
+    if (breakdown || std::fabs(cur - prev) < tol * std::max(1.0, std::fabs(cur))) break;
+    prev = cur;
+  }
+  const int k = lanczos_steps_done(L);
+  if (!rc) {
+    memcpy(alpha_host, a.data(), sizeof(double) * k);
+    memcpy(beta_host, b.data(), sizeof(double) * k);
+    if (steps_done) *steps_done = k;
+    if (e0) *e0 = cur;
+  }
+  lanczos_destroy(L);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,279 @@
+// Device-side building blocks for the symmetric-sector spin-1/2 path on gfx950.
+//
+// Group model.  Every symmetry group handled here is  G = T x B  where
+//   B  = the "base" elements: all combinations of reflection (full bit
+//        reversal) and spin flips (complement of all ns bits), applied first
+//        (Algebra/Symmetries/symmetry_container.py:918-950, base + t order),
+//   T  = translations T_x^a T_y^b on a lattice stored row-major with
+//        row_bits = n_basis * Lx bits per row (translation.py:316-437):
+//        T_x^a rotates every row left by a*cell_bits bits, T_y^b rotates the
+//        whole ns-bit word left by b*row_bits bits.
+// Element g has the reference index g = (a*nty + b)*nbase + B
+// (symmetry_container.py:943-950 with translation powers from
+// itertools.product over sorted directions, :1158-1172).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define QES_MAX_BASE 4
+#define QES_MAX_BASE_OPS 4
+#define QES_MAX_TX 64
+#define QES_MAX_TERMS 1024
+
+// base-op codes (one base element = up to QES_MAX_BASE_OPS ops in generator order)
+#define QES_BOP_NONE 0
+#define QES_BOP_REV 1      // reflection: bit i -> bit ns-1-i (reflection.py:143)
+#define QES_BOP_FLIP 2     // parity x or z: complement, phase 1 (parity.py:42-54)
+#define QES_BOP_FLIPY 3    // parity y: complement, phase +-i (parity.py:55-63)
+
+struct QesGroup {
+  int32_t ns;
+  int32_t nbase;
+  int32_t ntx, nty;
+  int32_t row_bits;
+  int32_t cell_bits;
+  int32_t has_phase;     // some base element carries an intrinsic (+-i) phase
+  int32_t n_up;          // U(1) popcount sector, -1 if none
+  int32_t z2;            // popcount-parity sector (+1/-1), 0 if none
+  int32_t pad;
+  uint8_t base_ops[QES_MAX_BASE][QES_MAX_BASE_OPS];
+  uint64_t mask;                      // (1<<ns)-1
+  uint64_t rowlo[QES_MAX_TX];         // per x-power a: bits of each row below a*cell_bits
+};
+
+// --------------------------------------------------------------------------
+// word helpers
+// --------------------------------------------------------------------------
+template <typename W> __device__ __forceinline__ int popc(W v);
+template <> __device__ __forceinline__ int popc<uint32_t>(uint32_t v) { return __popc(v); }
+template <> __device__ __forceinline__ int popc<uint64_t>(uint64_t v) { return __popcll(v); }
+
+template <typename W> __device__ __forceinline__ W brev(W v);
+template <> __device__ __forceinline__ uint32_t brev<uint32_t>(uint32_t v) { return __builtin_bitreverse32(v); }
+template <> __device__ __forceinline__ uint64_t brev<uint64_t>(uint64_t v) { return __builtin_bitreverse64(v); }
+
+// rotate left by r within the low ns bits (0 <= r < ns)
+template <typename W>
+__device__ __forceinline__ W rotl_ns(W v, int r, int ns, W mask) {
+  if (r == 0) return v;
+  return ((v << r) | (v >> (ns - r))) & mask;
+}
+
+// rotate every row_bits-wide row left by r bits (0 <= r < row_bits);
+// lo = bits of each row whose in-row position is < r
+template <typename W>
+__device__ __forceinline__ W rotl_rows(W v, int r, int row_bits, W lo, W mask) {
+  if (r == 0) return v;
+  W hi = mask ^ lo;
+  return ((v << r) & hi) | ((v >> (row_bits - r)) & lo);
+}
+
+// apply base element B; returns new state and multiplies *ph by the intrinsic
+// phase (only parity-y contributes, +i if the number of down spins is even)
+template <typename W>
+__device__ __forceinline__ W apply_base(const QesGroup& G, int B, W s, int* ph_quarter) {
+  W v = s;
+  const int ns = G.ns;
+  const W mask = (W)G.mask;
+#pragma unroll
+  for (int k = 0; k < QES_MAX_BASE_OPS; ++k) {
+    int op = G.base_ops[B][k];
+    if (op == QES_BOP_REV) {
+      v = brev<W>(v) >> (int)(8 * sizeof(W) - ns);
+    } else if (op == QES_BOP_FLIP) {
+      v = (~v) & mask;
+    } else if (op == QES_BOP_FLIPY) {
+      int downs = ns - popc<W>(v);
+      *ph_quarter += (downs & 1) ? 3 : 1;   // -i : +i  (quarter turns)
+      v = (~v) & mask;
+    }
+  }
+  return v;
+}
+
+// --------------------------------------------------------------------------
+// Canonicalisation: representative = min over G of g(s)
+// (scan_chunk_find_representatives, jit/symmetry_container_jit.py:234-246).
+// Returns the minimum, the number of minimising elements (|Stab(rep)|) and
+// the reference index of the first minimiser.
+// --------------------------------------------------------------------------
+template <typename W>
+struct Canon {
+  W rep;
+  int count;
+  int g;
+};
+
+template <typename W>
+__device__ __forceinline__ Canon<W> canonicalize(const QesGroup& G, W s) {
+  const int ns = G.ns;
+  const W mask = (W)G.mask;
+  W vb[QES_MAX_BASE];
+#pragma unroll
+  for (int B = 0; B < QES_MAX_BASE; ++B) {
+    int dummy = 0;
+    vb[B] = (B < G.nbase) ? apply_base<W>(G, B, s, &dummy) : (W)0;
+  }
+  Canon<W> c;
+  c.rep = ~(W)0;
+  c.count = 0;
+  c.g = 0;
+  for (int a = 0; a < G.ntx; ++a) {
+    const int ra = a * G.cell_bits;
+    const W lo = (W)G.rowlo[a];
+    W ta[QES_MAX_BASE];
+#pragma unroll
+    for (int B = 0; B < QES_MAX_BASE; ++B) ta[B] = rotl_rows<W>(vb[B], ra, G.row_bits, lo, mask);
+    for (int b = 0; b < G.nty; ++b) {
+      const int rb = b * G.row_bits;
+#pragma unroll
+      for (int B = 0; B < QES_MAX_BASE; ++B) {
+        if (B < G.nbase) {
+          W v = rotl_ns<W>(ta[B], rb, ns, mask);
+          int g = (a * G.nty + b) * G.nbase + B;
+          if (v < c.rep) {
+            c.rep = v;
+            c.count = 1;
+            c.g = g;
+          } else if (v == c.rep) {
+            c.count += 1;
+          }
+        }
+      }
+    }
+  }
+  return c;
+}
+
+// --------------------------------------------------------------------------
+// Representative test used by the basis scan.  Returns false as soon as some
+// g(s) < s.  Otherwise sums conj(chi_g)*phase_g over the stabiliser in the
+// reference element order (compute_normalization, jit :154-174) and returns
+// |sum| through *proj_abs2 components.
+// --------------------------------------------------------------------------
+template <typename W>
+__device__ __forceinline__ bool rep_test(const QesGroup& G, const double* __restrict__ chi_re,
+                                         const double* __restrict__ chi_im, W s, double* pre,
+                                         double* pim) {
+  const int ns = G.ns;
+  const W mask = (W)G.mask;
+  W vb[QES_MAX_BASE];
+  int phq[QES_MAX_BASE];
+#pragma unroll
+  for (int B = 0; B < QES_MAX_BASE; ++B) {
+    phq[B] = 0;
+    vb[B] = (B < G.nbase) ? apply_base<W>(G, B, s, &phq[B]) : (W)0;
+  }
+  double re = 0.0, im = 0.0;
+  for (int a = 0; a < G.ntx; ++a) {
+    const int ra = a * G.cell_bits;
+    const W lo = (W)G.rowlo[a];
+    W ta[QES_MAX_BASE];
+#pragma unroll
+    for (int B = 0; B < QES_MAX_BASE; ++B) ta[B] = rotl_rows<W>(vb[B], ra, G.row_bits, lo, mask);
+    for (int b = 0; b < G.nty; ++b) {
+      const int rb = b * G.row_bits;
+#pragma unroll
+      for (int B = 0; B < QES_MAX_BASE; ++B) {
+        if (B < G.nbase) {
+          W v = rotl_ns<W>(ta[B], rb, ns, mask);
+          if (v < s) return false;
+          if (v == s) {
+            int g = (a * G.nty + b) * G.nbase + B;
+            double cr = chi_re[g], ci = -chi_im[g];   // conj(chi)
+            // times intrinsic phase i^q (exact: swaps/negations only)
+            double tr, ti;
+            switch (phq[B] & 3) {
+              case 0: tr = cr; ti = ci; break;
+              case 1: tr = -ci; ti = cr; break;
+              case 2: tr = -cr; ti = -ci; break;
+              default: tr = ci; ti = -cr; break;
+            }
+            re += tr;
+            im += ti;
+          }
+        }
+      }
+    }
+  }
+  *pre = re;
+  *pim = im;
+  return true;
+}
+
+// --------------------------------------------------------------------------
+// Representative lookup: sorted keys + an exponent/mantissa bucket table.
+// key(v): e = bit length of v; bucket = e<<mbits | (next mbits bits after the
+// leading one), so bucket widths follow the roughly geometric density of
+// representatives (which carry their longest zero run at the top).
+// --------------------------------------------------------------------------
+struct QesLookup {
+  const uint64_t* keys64;
+  const uint32_t* keys32;   // non-null when ns <= 32
+  const uint32_t* bucket;   // ((e_max-e_min+1) << mbits) + 1 offsets
+  int64_t n;
+  int32_t mbits;
+  int32_t e_min, e_max;     // bit-length range covered by the table
+  int32_t pad;
+};
+
+__device__ __forceinline__ uint32_t bucket_of(uint64_t v, int mbits, int* e_out) {
+  int e = 64 - __clzll(v);   // 0 for v == 0
+  *e_out = e;
+  uint64_t mant;
+  if (e <= mbits) {
+    mant = v;   // small values map one-to-one
+  } else {
+    mant = (v >> (e - 1 - mbits)) & ((1ull << mbits) - 1ull);
+  }
+  return ((uint32_t)e << mbits) | (uint32_t)mant;
+}
+
+template <typename W>
+__device__ __forceinline__ int64_t lookup_rep(const QesLookup& L, W v) {
+  int e;
+  uint32_t k = bucket_of((uint64_t)v, L.mbits, &e);
+  if (e < L.e_min || e > L.e_max) return -1;
+  k -= (uint32_t)L.e_min << L.mbits;
+  int64_t lo = L.bucket[k];
+  int64_t hi = L.bucket[k + 1];   // exclusive
+  if (sizeof(W) == 4 && L.keys32 != nullptr) {
+    const uint32_t* keys = L.keys32;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      uint32_t km = keys[mid];
+      if (km < (uint32_t)v) lo = mid + 1; else hi = mid;
+    }
+    if (lo < L.n && keys[lo] == (uint32_t)v) return lo;
+    return -1;
+  } else {
+    const uint64_t* keys = L.keys64;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      uint64_t km = keys[mid];
+      if (km < (uint64_t)v) lo = mid + 1; else hi = mid;
+    }
+    if (lo < L.n && keys[lo] == (uint64_t)v) return lo;
+    return -1;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Compiled Hamiltonian terms.  One group = one flip mask and up to two read
+// sites; its value for a state is val[(bit(p0)<<1) | bit(p1)].
+// --------------------------------------------------------------------------
+struct QesTermGroup {
+  uint64_t mask;       // flip mask (0 for diagonal groups)
+  int32_t p0, p1;      // bit positions read (-1: unused -> bit 0)
+  int32_t coupling;    // coupling-group id (for per-coupling outputs)
+  int32_t pad;
+  double re[4];
+  double im[4];
+};
+
+template <typename W>
+__device__ __forceinline__ int term_bits(const QesTermGroup& t, W s) {
+  int b0 = t.p0 >= 0 ? (int)((s >> t.p0) & 1) : 0;
+  int b1 = t.p1 >= 0 ? (int)((s >> t.p1) & 1) : 0;
+  return (b0 << 1) | b1;
+}

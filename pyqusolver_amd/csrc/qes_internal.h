@@ -1,0 +1,76 @@
+// Host-side internals shared by the HIP translation units of libqes_mi.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "qes_device.h"
+
+namespace qes {
+
+void set_error(const std::string& msg);
+int fail(const std::string& msg);           // set_error + return -1
+int check_hip(hipError_t e, const char* what);
+
+#define QES_HIP(call)                                   \
+  do {                                                  \
+    hipError_t _e = (call);                             \
+    if (_e != hipSuccess) return qes::check_hip(_e, #call); \
+  } while (0)
+
+// Device basis of one symmetry sector.
+struct Basis {
+  QesGroup group{};
+  int64_t n = 0;
+  int gsize = 0;              // |G|
+  double* chi_re = nullptr;   // device, |G|
+  double* chi_im = nullptr;
+  std::vector<double> h_chi_re, h_chi_im;
+  uint64_t* reps = nullptr;   // device, n (sorted)
+  uint32_t* keys32 = nullptr; // device, n (ns <= 32)
+  double* norms = nullptr;    // device, n
+  uint32_t* bucket = nullptr; // device lookup table
+  int mbits = 0, e_min = 0, e_max = 0;
+  int64_t nbucket = 0;
+  bool owns = true;
+  QesLookup lookup() const {
+    QesLookup L;
+    L.keys64 = reps;
+    L.keys32 = keys32;
+    L.bucket = bucket;
+    L.n = n;
+    L.mbits = mbits;
+    L.e_min = e_min;
+    L.e_max = e_max;
+    L.pad = 0;
+    return L;
+  }
+};
+
+// Compiled operator (Hamiltonian) bound to a basis.
+struct Op {
+  Basis* basis = nullptr;
+  int n_diag = 0, n_off = 0;       // groups
+  int n_coupling = 1;              // distinct coupling ids
+  bool is_complex = false;         // some term value is complex
+  bool chi_complex = false;        // sector characters complex
+  QesTermGroup* d_groups = nullptr;   // device: diag groups then off groups
+  std::vector<QesTermGroup> h_groups;
+  int block = 256;
+  int xcd_remap = 1;
+};
+
+// launchers (qes_basis.hip)
+int build_basis(Basis* B, hipStream_t st);
+int map_states(const Basis* B, const uint64_t* d_states, int64_t n, int64_t* d_idx, int32_t* d_g,
+               int32_t* d_cnt, hipStream_t st);
+
+// launchers (qes_matvec.hip)
+// dtype 0: float64 vectors, 1: complex128 (interleaved re,im)
+int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int accumulate,
+           hipStream_t st);
+int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_t st);
+int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st);
+
+}  // namespace qes

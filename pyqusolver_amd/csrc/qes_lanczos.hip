@@ -1,0 +1,506 @@
+// Device Lanczos (FP64 / complex128) driven by the pull-form matvec.
+//
+// The reference hands SpecialOperator.matvec_fun to the eigensolver of its
+// (absent) general_python submodule (Algebra/Hamil/hamil_diag_engine.py:267-280,
+// called from Algebra/hamil.py:2063-2081).  Here the three-term recurrence,
+// the dot products, the axpys and the classical Gram-Schmidt
+// reorthogonalisation (applied twice) run on the device.  The step index
+// lives in device memory, so one Lanczos iteration is a fixed sequence of
+// launches that is captured once into a hipGraph and replayed.  Only alpha /
+// beta travel to the host, for the FP64 tridiagonal eigensolve.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "qes_internal.h"
+#include "qes_lanczos.h"
+
+namespace qes {
+
+static constexpr int RB = 256;        // threads per block for vector kernels
+static constexpr int VB = 8;          // Lanczos vectors per block in the reorth projection
+
+// ---------------------------------------------------------------------------
+// reductions: per-block partials then one deterministic finishing block
+// ---------------------------------------------------------------------------
+template <bool CPLX>
+__global__ void __launch_bounds__(RB) dot_partial_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                         int64_t n, double* __restrict__ part) {
+  __shared__ double sr[RB], si[RB];
+  double r = 0.0, im = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    if (CPLX) {
+      double ar = a[2 * i], ai = a[2 * i + 1], br = b[2 * i], bi = b[2 * i + 1];
+      r += ar * br + ai * bi;    // conj(a) * b
+      im += ar * bi - ai * br;
+    } else {
+      r += a[i] * b[i];
+    }
+  }
+  sr[threadIdx.x] = r;
+  si[threadIdx.x] = im;
+  __syncthreads();
+  for (int off = RB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) {
+      sr[threadIdx.x] += sr[threadIdx.x + off];
+      si[threadIdx.x] += si[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sr[0];
+    part[2 * blockIdx.x + 1] = si[0];
+  }
+}
+
+// out[0..1] = sum of nb complex partials; mode 0: plain, 1: sqrt of real part
+__global__ void __launch_bounds__(RB) finish_kernel(const double* __restrict__ part, int nb, double* __restrict__ out,
+                                                    int mode) {
+  __shared__ double sr[RB], si[RB];
+  double r = 0.0, im = 0.0;
+  for (int i = threadIdx.x; i < nb; i += RB) {
+    r += part[2 * i];
+    im += part[2 * i + 1];
+  }
+  sr[threadIdx.x] = r;
+  si[threadIdx.x] = im;
+  __syncthreads();
+  for (int off = RB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) {
+      sr[threadIdx.x] += sr[threadIdx.x + off];
+      si[threadIdx.x] += si[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (mode == 1) {
+      out[0] = sqrt(sr[0]);
+      out[1] = 0.0;
+    } else {
+      out[0] = sr[0];
+      out[1] = si[0];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos step kernels.  Scalars: alpha[j], beta[j] on device; j = *d_j.
+// ---------------------------------------------------------------------------
+
+// alpha_j = Re <v, w>   (from finished dot in scal[0..1])
+__global__ void set_alpha_kernel(const double* __restrict__ scal, double* __restrict__ alpha, const int* __restrict__ d_j) {
+  alpha[*d_j] = scal[0];
+}
+
+// w -= alpha_j v + beta_{j-1} v_prev
+template <bool CPLX>
+__global__ void __launch_bounds__(RB) recur_kernel(double* __restrict__ w, const double* __restrict__ v,
+                                                   const double* __restrict__ vp, const double* __restrict__ alpha,
+                                                   const double* __restrict__ beta, const int* __restrict__ d_j, int64_t n) {
+  const int j = *d_j;
+  const double al = alpha[j];
+  const double be = j > 0 ? beta[j - 1] : 0.0;
+  const int cw = CPLX ? 2 : 1;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n * cw; i += (int64_t)gridDim.x * RB)
+    w[i] = w[i] - al * v[i] - be * vp[i];
+}
+
+// partial projections c_i = <V_i, w> for i <= j; grid (nchunk, ceil(m/VB))
+template <bool CPLX>
+__global__ void __launch_bounds__(RB) proj_partial_kernel(const double* __restrict__ V, int64_t ldv,
+                                                          const double* __restrict__ w, int64_t n,
+                                                          const int* __restrict__ d_j, double* __restrict__ part,
+                                                          int nchunk) {
+  __shared__ double sr[VB][RB / 64], si[VB][RB / 64];
+  const int j = *d_j;
+  const int i0 = blockIdx.y * VB;
+  if (i0 > j) return;
+  double r[VB], im[VB];
+#pragma unroll
+  for (int k = 0; k < VB; ++k) {
+    r[k] = 0.0;
+    im[k] = 0.0;
+  }
+  const int cw = CPLX ? 2 : 1;
+  for (int64_t e = (int64_t)blockIdx.x * RB + threadIdx.x; e < n; e += (int64_t)nchunk * RB) {
+    double wr = w[e * cw], wi = CPLX ? w[e * cw + 1] : 0.0;
+#pragma unroll
+    for (int k = 0; k < VB; ++k) {
+      if (i0 + k <= j) {
+        const double* vp = V + (int64_t)(i0 + k) * ldv + e * cw;
+        double vr = vp[0];
+        if (CPLX) {
+          double vi = vp[1];
+          r[k] += vr * wr + vi * wi;
+          im[k] += vr * wi - vi * wr;
+        } else {
+          r[k] += vr * wr;
+        }
+      }
+    }
+  }
+  // wavefront reduce then across the 4 waves
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < VB; ++k) {
+    double a = r[k], b = im[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_down(a, off, 64);
+      b += __shfl_down(b, off, 64);
+    }
+    if (lane == 0) {
+      sr[k][wid] = a;
+      si[k][wid] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < VB) {
+    const int k = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < RB / 64; ++q) {
+      a += sr[k][q];
+      b += si[k][q];
+    }
+    // part layout: [vector][chunk][2]
+    part[((int64_t)(i0 + k) * nchunk + blockIdx.x) * 2] = a;
+    part[((int64_t)(i0 + k) * nchunk + blockIdx.x) * 2 + 1] = b;
+  }
+}
+
+// c_i = sum over chunks (one block per vector, i <= j)
+__global__ void __launch_bounds__(RB) proj_finish_kernel(const double* __restrict__ part, int nchunk,
+                                                         const int* __restrict__ d_j, double* __restrict__ coef) {
+  __shared__ double sr[RB], si[RB];
+  const int i = blockIdx.x;
+  if (i > *d_j) return;
+  double r = 0.0, im = 0.0;
+  for (int c = threadIdx.x; c < nchunk; c += RB) {
+    r += part[((int64_t)i * nchunk + c) * 2];
+    im += part[((int64_t)i * nchunk + c) * 2 + 1];
+  }
+  sr[threadIdx.x] = r;
+  si[threadIdx.x] = im;
+  __syncthreads();
+  for (int off = RB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) {
+      sr[threadIdx.x] += sr[threadIdx.x + off];
+      si[threadIdx.x] += si[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    coef[2 * i] = sr[0];
+    coef[2 * i + 1] = si[0];
+  }
+}
+
+// w -= sum_{i<=j} c_i V_i   (c complex when CPLX)
+template <bool CPLX>
+__global__ void __launch_bounds__(RB) proj_update_kernel(const double* __restrict__ V, int64_t ldv,
+                                                         double* __restrict__ w, int64_t n, const int* __restrict__ d_j,
+                                                         const double* __restrict__ coef) {
+  __shared__ double sc[2 * 1024];
+  const int j = *d_j;
+  for (int i = threadIdx.x; i <= j && i < 1024; i += RB) {
+    sc[2 * i] = coef[2 * i];
+    sc[2 * i + 1] = coef[2 * i + 1];
+  }
+  __syncthreads();
+  const int cw = CPLX ? 2 : 1;
+  for (int64_t e = (int64_t)blockIdx.x * RB + threadIdx.x; e < n; e += (int64_t)gridDim.x * RB) {
+    double ar = 0.0, ai = 0.0;
+    for (int i = 0; i <= j; ++i) {
+      const double* vp = V + (int64_t)i * ldv + e * cw;
+      double cr = sc[2 * i], ci = sc[2 * i + 1];
+      if (CPLX) {
+        double vr = vp[0], vi = vp[1];
+        ar += cr * vr - ci * vi;
+        ai += cr * vi + ci * vr;
+      } else {
+        ar += cr * vp[0];
+      }
+    }
+    w[e * cw] -= ar;
+    if (CPLX) w[e * cw + 1] -= ai;
+  }
+}
+
+// beta_j = ||w|| (already in scal[0]); v_prev <- v; v <- w / beta_j; V[j+1] <- v
+__global__ void __launch_bounds__(RB) advance_kernel(double* __restrict__ w, double* __restrict__ v,
+                                                     double* __restrict__ vp, double* __restrict__ V, int64_t ldv,
+                                                     int store, const double* __restrict__ scal,
+                                                     const int* __restrict__ d_j, int64_t len, int maxv) {
+  const int j = *d_j;
+  const double be = scal[0];
+  const double inv = be > 0.0 ? 1.0 / be : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < len; i += (int64_t)gridDim.x * RB) {
+    double nv = w[i] * inv;
+    vp[i] = v[i];
+    v[i] = nv;
+    if (store && j + 1 < maxv) V[(int64_t)(j + 1) * ldv + i] = nv;
+  }
+}
+
+__global__ void set_beta_step_kernel(const double* __restrict__ scal, double* __restrict__ beta, int* __restrict__ d_j) {
+  beta[*d_j] = scal[0];
+  *d_j = *d_j + 1;
+}
+
+__global__ void __launch_bounds__(RB) fill_kernel(double* __restrict__ p, int64_t len, double val) {
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < len; i += (int64_t)gridDim.x * RB) p[i] = val;
+}
+
+__global__ void __launch_bounds__(RB) scale_copy_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                                        double* __restrict__ dst2, int64_t len,
+                                                        const double* __restrict__ scal) {
+  const double inv = scal[0] > 0.0 ? 1.0 / scal[0] : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < len; i += (int64_t)gridDim.x * RB) {
+    double v = src[i] * inv;
+    dst[i] = v;
+    if (dst2) dst2[i] = v;
+  }
+}
+
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
+// psi = sum_i c_i V_i  (real coefficients c)
+template <bool CPLX>
+__global__ void __launch_bounds__(RB) combine_kernel(const double* __restrict__ V, int64_t ldv, int k,
+                                                     const double* __restrict__ c, double* __restrict__ out, int64_t n) {
+  const int cw = CPLX ? 2 : 1;
+  for (int64_t e = (int64_t)blockIdx.x * RB + threadIdx.x; e < n * cw; e += (int64_t)gridDim.x * RB) {
+    double acc = 0.0;
+    for (int i = 0; i < k; ++i) acc += c[i] * V[(int64_t)i * ldv + e];
+    out[e] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct Lanczos {
+  Op* op = nullptr;
+  int dtype = 0;
+  int maxv = 0;          // stored vectors (max_steps + 1) when store
+  int max_steps = 0;
+  int store = 0;
+  int64_t n = 0, len = 0;  // len = n * cw
+  double *v = nullptr, *vp = nullptr, *w = nullptr, *V = nullptr;
+  double *alpha = nullptr, *beta = nullptr, *scal = nullptr, *part = nullptr, *coef = nullptr, *ppart = nullptr;
+  int* d_j = nullptr;
+  int nred = 0, nchunk = 0, nvec_blocks = 0;
+  int steps_done = 0;
+  int reorth_graph = -1;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static int grid_for(int64_t len) {
+  int64_t g = (len + RB - 1) / RB;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+static int enqueue_dot(Lanczos* L, const double* a, const double* b, double* out, int mode, hipStream_t st) {
+  if (L->dtype == 1)
+    hipLaunchKernelGGL(dot_partial_kernel<true>, dim3(L->nred), dim3(RB), 0, st, a, b, L->n, L->part);
+  else
+    hipLaunchKernelGGL(dot_partial_kernel<false>, dim3(L->nred), dim3(RB), 0, st, a, b, L->n, L->part);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(RB), 0, st, L->part, L->nred, out, mode);
+  return 0;
+}
+
+static int enqueue_step(Lanczos* L, int reorth, hipStream_t st) {
+  const bool c = L->dtype == 1;
+  if (matvec(L->op, L->v, L->w, 1, L->dtype, 0, st)) return -1;
+  enqueue_dot(L, L->v, L->w, L->scal, 0, st);
+  hipLaunchKernelGGL(set_alpha_kernel, dim3(1), dim3(1), 0, st, L->scal, L->alpha, L->d_j);
+  const int g = grid_for(L->len);
+  if (c)
+    hipLaunchKernelGGL(recur_kernel<true>, dim3(g), dim3(RB), 0, st, L->w, L->v, L->vp, L->alpha, L->beta, L->d_j, L->n);
+  else
+    hipLaunchKernelGGL(recur_kernel<false>, dim3(g), dim3(RB), 0, st, L->w, L->v, L->vp, L->alpha, L->beta, L->d_j, L->n);
+  if (reorth && L->store) {
+    for (int pass = 0; pass < 2; ++pass) {
+      dim3 pg(L->nchunk, L->nvec_blocks);
+      if (c)
+        hipLaunchKernelGGL(proj_partial_kernel<true>, pg, dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->ppart,
+                           L->nchunk);
+      else
+        hipLaunchKernelGGL(proj_partial_kernel<false>, pg, dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->ppart,
+                           L->nchunk);
+      hipLaunchKernelGGL(proj_finish_kernel, dim3(L->maxv), dim3(RB), 0, st, L->ppart, L->nchunk, L->d_j, L->coef);
+      if (c)
+        hipLaunchKernelGGL(proj_update_kernel<true>, dim3(g), dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->coef);
+      else
+        hipLaunchKernelGGL(proj_update_kernel<false>, dim3(g), dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->coef);
+    }
+  }
+  enqueue_dot(L, L->w, L->w, L->scal, 1, st);
+  hipLaunchKernelGGL(advance_kernel, dim3(g), dim3(RB), 0, st, L->w, L->v, L->vp, L->V, L->len, L->store, L->scal,
+                     L->d_j, L->len, L->maxv);
+  hipLaunchKernelGGL(set_beta_step_kernel, dim3(1), dim3(1), 0, st, L->scal, L->beta, L->d_j);
+  QES_HIP(hipGetLastError());
+  return 0;
+}
+
+int lanczos_create(Op* op, int max_steps, int store, int dtype, Lanczos** out) {
+  if (max_steps < 1) return fail("max_steps must be >= 1");
+  if (store && max_steps + 1 > 1024) return fail("stored Lanczos basis limited to 1023 steps");
+  if (dtype != 0 && dtype != 1) return fail("dtype must be 0 (float64) or 1 (complex128)");
+  if (dtype == 0 && (op->is_complex || op->chi_complex)) return fail("complex sector/model needs complex128");
+  Lanczos* L = new Lanczos();
+  L->op = op;
+  L->dtype = dtype;
+  L->max_steps = max_steps;
+  L->store = store;
+  L->n = op->basis->n;
+  L->len = L->n * (dtype == 1 ? 2 : 1);
+  L->maxv = max_steps + 1;
+  const int64_t len = std::max<int64_t>(L->len, 1);
+  L->nred = grid_for(L->n);
+  L->nchunk = (int)std::min<int64_t>(std::max<int64_t>((L->n + RB - 1) / RB, 1), 512);
+  L->nvec_blocks = (L->maxv + VB - 1) / VB;
+  QES_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+  QES_HIP(hipMalloc(&L->v, len * 8));
+  QES_HIP(hipMalloc(&L->vp, len * 8));
+  QES_HIP(hipMalloc(&L->w, len * 8));
+  if (store) QES_HIP(hipMalloc(&L->V, (size_t)len * 8 * L->maxv));
+  QES_HIP(hipMalloc(&L->alpha, sizeof(double) * L->maxv));
+  QES_HIP(hipMalloc(&L->beta, sizeof(double) * L->maxv));
+  QES_HIP(hipMalloc(&L->scal, sizeof(double) * 4));
+  QES_HIP(hipMalloc(&L->part, sizeof(double) * 2 * L->nred));
+  QES_HIP(hipMalloc(&L->coef, sizeof(double) * 2 * L->maxv));
+  QES_HIP(hipMalloc(&L->ppart, sizeof(double) * 2 * (size_t)L->nchunk * L->nvec_blocks * VB));
+  QES_HIP(hipMalloc(&L->d_j, sizeof(int)));
+  *out = L;
+  return 0;
+}
+
+int lanczos_destroy(Lanczos* L) {
+  if (!L) return 0;
+  if (L->exec) hipGraphExecDestroy(L->exec);
+  if (L->graph) hipGraphDestroy(L->graph);
+  hipFree(L->v);
+  hipFree(L->vp);
+  hipFree(L->w);
+  if (L->V) hipFree(L->V);
+  hipFree(L->alpha);
+  hipFree(L->beta);
+  hipFree(L->scal);
+  hipFree(L->part);
+  hipFree(L->coef);
+  hipFree(L->ppart);
+  hipFree(L->d_j);
+  if (L->stream) hipStreamDestroy(L->stream);
+  delete L;
+  return 0;
+}
+
+int lanczos_start(Lanczos* L, const void* v0, hipStream_t user) {
+  hipStream_t st = L->stream;
+  hipEvent_t ev;
+  QES_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  QES_HIP(hipEventRecord(ev, user));
+  QES_HIP(hipStreamWaitEvent(st, ev, 0));
+  const int g = grid_for(L->len);
+  // v = v0 / ||v0||; V[0] = v; v_prev = 0
+  hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(RB), 0, st, L->vp, L->len, 0.0);
+  QES_HIP(hipMemcpyAsync(L->w, v0, L->len * 8, hipMemcpyDeviceToDevice, st));
+  enqueue_dot(L, L->w, L->w, L->scal, 1, st);
+  hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(RB), 0, st, L->w, L->v, L->store ? L->V : nullptr, L->len, L->scal);
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, L->d_j, 0);
+  QES_HIP(hipGetLastError());
+  L->steps_done = 0;
+  QES_HIP(hipStreamSynchronize(st));
+  QES_HIP(hipEventDestroy(ev));
+  return 0;
+}
+
+int lanczos_run(Lanczos* L, int nsteps, int reorth, int use_graph) {
+  hipStream_t st = L->stream;
+  nsteps = std::min(nsteps, L->max_steps - L->steps_done);
+  if (nsteps <= 0) return 0;
+  if (use_graph) {
+    if (!L->exec || L->reorth_graph != reorth) {
+      if (L->exec) {
+        hipGraphExecDestroy(L->exec);
+        hipGraphDestroy(L->graph);
+        L->exec = nullptr;
+        L->graph = nullptr;
+      }
+      QES_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_step(L, reorth, st);
+      hipGraph_t gph = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &gph);
+      if (rc) return rc;
+      if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
+      L->graph = gph;
+      QES_HIP(hipGraphInstantiate(&L->exec, L->graph, nullptr, nullptr, 0));
+      L->reorth_graph = reorth;
+    }
+    for (int s = 0; s < nsteps; ++s) QES_HIP(hipGraphLaunch(L->exec, st));
+  } else {
+    for (int s = 0; s < nsteps; ++s)
+      if (enqueue_step(L, reorth, st)) return -1;
+  }
+  QES_HIP(hipStreamSynchronize(st));
+  L->steps_done += nsteps;
+  return 0;
+}
+
+int lanczos_get(Lanczos* L, double* alpha, double* beta) {
+  const int k = L->steps_done;
+  if (k > 0) {
+    QES_HIP(hipMemcpy(alpha, L->alpha, sizeof(double) * k, hipMemcpyDeviceToHost));
+    QES_HIP(hipMemcpy(beta, L->beta, sizeof(double) * k, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int lanczos_combine(Lanczos* L, const double* c_host, int k, void* out, hipStream_t user) {
+  if (!L->store) return fail("Lanczos basis was not stored");
+  if (k < 1 || k > L->steps_done + 1 || k > L->maxv) return fail("bad number of Ritz coefficients");
+  double* dc = nullptr;
+  QES_HIP(hipMalloc(&dc, sizeof(double) * k));
+  QES_HIP(hipMemcpy(dc, c_host, sizeof(double) * k, hipMemcpyHostToDevice));
+  const int g = grid_for(L->len);
+  if (L->dtype == 1)
+    hipLaunchKernelGGL(combine_kernel<true>, dim3(g), dim3(RB), 0, L->stream, L->V, L->len, k, dc, (double*)out, L->n);
+  else
+    hipLaunchKernelGGL(combine_kernel<false>, dim3(g), dim3(RB), 0, L->stream, L->V, L->len, k, dc, (double*)out, L->n);
+  QES_HIP(hipGetLastError());
+  QES_HIP(hipStreamSynchronize(L->stream));
+  QES_HIP(hipFree(dc));
+  (void)user;
+  return 0;
+}
+
+int lanczos_steps_done(const Lanczos* L) { return L->steps_done; }
+
+// lowest eigenvalue of the symmetric tridiagonal (a, b) by Sturm bisection
+double tridiag_lowest(const double* a, const double* b, int k) {
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < k; ++i) {
+    double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i < k - 1 ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(lo)); ++it) {
+    double mid = 0.5 * (lo + hi);
+    int neg = 0;
+    double d = 1.0;
+    for (int i = 0; i < k; ++i) {
+      double bb = i > 0 ? b[i - 1] * b[i - 1] : 0.0;
+      d = a[i] - mid - (i > 0 ? bb / d : 0.0);
+      if (d == 0.0) d = -1e-300;
+      if (d < 0) ++neg;
+    }
+    if (neg >= 1) hi = mid; else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+}  // namespace qes

@@ -1,0 +1,153 @@
+"""Ground-state Lanczos on the device.
+
+The recurrence, dots, axpys and full reorthogonalisation run in libqes_mi
+(one hipGraph-captured step replayed per iteration); the host only sees
+alpha/beta and solves the FP64 tridiagonal problem with
+scipy.linalg.eigh_tridiagonal.  This replaces the eigensolver the
+reference dispatches from DiagonalizationEngine
+(Algebra/Hamil/hamil_diag_engine.py:176-300, called at Algebra/hamil.py:2063-2081).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+import torch
+from scipy.linalg import eigh_tridiagonal
+
+from . import _lib
+
+
+@dataclass
+class LanczosResult:
+    eigenvalues: np.ndarray
+    alpha: np.ndarray
+    beta: np.ndarray
+    steps: int
+    converged: bool
+    history: list = field(default_factory=list)     # (steps, E0) at each check
+    eigenvectors: Optional[torch.Tensor] = None
+    seconds: float = 0.0
+    residual_bound: float = float("nan")
+
+
+def start_vector(n: int, complex_: bool, seed: int = 0) -> np.ndarray:
+    """Seeded N(0,1) start vector (complex: re + 1j*im)."""
+    rng = np.random.default_rng(seed)
+    if complex_:
+        return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    return rng.standard_normal(n)
+
+
+class DeviceLanczos:
+    """Thin owner of a qes_lanczos handle."""
+
+    def __init__(self, ham, max_steps: int, store_basis: bool, complex_: bool):
+        self.ham = ham
+        self.max_steps = int(max_steps)
+        self.dtype = 1 if complex_ else 0
+        h = C.c_void_p()
+        _lib.check(_lib.lib().qes_lanczos_create(ham._device_op(), self.max_steps, 1 if store_basis else 0,
+                                                 self.dtype, C.byref(h)), "qes_lanczos_create")
+        self._h = h
+
+    def start(self, v0: torch.Tensor):
+        _lib.check(_lib.lib().qes_lanczos_start(self._h, _lib.dptr(v0), _lib.stream_ptr()), "qes_lanczos_start")
+
+    def run(self, nsteps: int, reorth: bool, use_graph: bool = True):
+        _lib.check(_lib.lib().qes_lanczos_run(self._h, int(nsteps), 1 if reorth else 0, 1 if use_graph else 0),
+                   "qes_lanczos_run")
+
+    def coefficients(self):
+        a = np.zeros(self.max_steps)
+        b = np.zeros(self.max_steps)
+        k = C.c_int32()
+        _lib.check(_lib.lib().qes_lanczos_get(self._h, a.ctypes.data_as(_lib._PD), b.ctypes.data_as(_lib._PD),
+                                              C.byref(k)), "qes_lanczos_get")
+        return a[:k.value].copy(), b[:k.value].copy()
+
+    def ritz(self, coeffs: np.ndarray, out: torch.Tensor):
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        _lib.check(_lib.lib().qes_lanczos_ritz(self._h, c.ctypes.data_as(_lib._PD), len(c), _lib.dptr(out),
+                                               _lib.stream_ptr()), "qes_lanczos_ritz")
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            _lib.lib().qes_lanczos_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tridiag_eigs(alpha, beta, k=1):
+    m = len(alpha)
+    if m == 1:
+        return np.array([alpha[0]]), np.ones((1, 1))
+    w, v = eigh_tridiagonal(alpha, beta[: m - 1])
+    return w, v
+
+
+def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300, v0=None, seed: int = 0,
+                         reorth: bool = True, store_basis: Optional[bool] = None, compute_eigvec: bool = False,
+                         check_every: int = 10, use_graph: bool = True) -> LanczosResult:
+    n = ham.nh
+    complex_ = bool(ham.needs_complex)
+    if v0 is None:
+        v0 = start_vector(n, complex_, seed)
+    if isinstance(v0, torch.Tensor):
+        v0t = v0.to("cuda")
+    else:
+        v0t = torch.from_numpy(np.ascontiguousarray(v0)).to("cuda")
+    v0t = v0t.to(torch.complex128 if complex_ else torch.float64).contiguous()
+    maxiter = int(min(maxiter, max(n, 1)))
+    if store_basis is None:
+        store_basis = bool(reorth or compute_eigvec)
+    reorth = bool(reorth and store_basis)
+    t0 = time.perf_counter()
+    L = DeviceLanczos(ham, maxiter, store_basis, complex_)
+    L.start(v0t)
+    hist = []
+    prev = None
+    converged = False
+    a = b = np.zeros(0)
+    while True:
+        a_done = len(a)
+        nsteps = min(check_every, maxiter - a_done)
+        if nsteps <= 0:
+            break
+        L.run(nsteps, reorth, use_graph)
+        a, b = L.coefficients()
+        w, _ = tridiag_eigs(a, b)
+        e0 = float(w[0])
+        hist.append((len(a), e0))
+        scale = max(1.0, abs(e0))
+        if np.any(np.abs(b) <= 1e-14 * scale):          # invariant subspace found
+            cut = int(np.argmax(np.abs(b) <= 1e-14 * scale)) + 1
+            a, b = a[:cut], b[:cut]
+            w, _ = tridiag_eigs(a, b)
+            converged = True
+            break
+        if prev is not None and abs(e0 - prev) <= tol * scale:
+            converged = True
+            break
+        prev = e0
+    torch.cuda.synchronize()
+    w, V = tridiag_eigs(a, b)
+    res_bound = float(abs(b[len(a) - 1] * V[-1, 0])) if len(a) else float("nan")
+    vecs = None
+    if compute_eigvec:
+        if not store_basis:
+            raise ValueError("eigenvectors need store_basis=True")
+        vecs = torch.empty((n,), dtype=v0t.dtype, device="cuda")
+        L.ritz(V[:, 0], vecs)
+    L.close()
+    return LanczosResult(eigenvalues=w[:k].copy(), alpha=a, beta=b, steps=len(a), converged=converged,
+                         history=hist, eigenvectors=vecs, seconds=time.perf_counter() - t0,
+                         residual_bound=res_bound)

@@ -1,0 +1,317 @@
+"""Spin-1/2 Hamiltonians on a symmetry-reduced sector, matvec on the GPU.
+
+Host-side mirror of the reference path
+  Hamiltonian.add -> SpecialOperator instruction arrays
+      (Algebra/Operator/special_operator.py:1113-1231, :1392-1417)
+  SpecialOperator.matvec_fun (special_operator.py:593-709)
+  Operator.matvec(vecs, hilbert_in=...) (Algebra/Operator/operator.py:1559-1602)
+  GeneralMatrix as scipy LinearOperator (Algebra/Operator/matrix.py:269-408)
+  Hamiltonian.diagonalize(method='lanczos') (Algebra/hamil.py:1809-2114)
+with every matvec and Lanczos step executed by libqes_mi on the device.
+
+Differences from the reference, on purpose:
+  * the LinearOperator has shape (nh, nh) and the Hilbert context of the
+    Hamiltonian (the reference leaves (0, 0) and passes hilbert=None,
+    matrix.py:334,381 and operator.py:495-503);
+  * a float64 vector with complex matrix elements (complex couplings or a
+    complex momentum sector) raises instead of silently dropping Im.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+
+# SpinLookupCodes (Algebra/Operator/impl/operators_spin.py:1084-1118)
+SIG_X_L, SIG_Y_L, SIG_Z_L, SIG_P_L, SIG_M_L = 11, 12, 13, 14, 15
+SIG_X_C, SIG_Y_C, SIG_Z_C, SIG_P_C, SIG_M_C = 20, 21, 22, 23, 24
+SIG_XY_C, SIG_YX_C, SIG_XZ_C, SIG_ZX_C, SIG_YZ_C, SIG_ZY_C = 25, 26, 27, 28, 29, 30
+SIG_PM_C, SIG_MP_C = 31, 32
+
+_LOCAL = {"x": SIG_X_L, "y": SIG_Y_L, "z": SIG_Z_L, "p": SIG_P_L, "m": SIG_M_L}
+_CORR = {"x": SIG_X_C, "y": SIG_Y_C, "z": SIG_Z_C, "p": SIG_P_C, "m": SIG_M_C, "xy": SIG_XY_C,
+         "yx": SIG_YX_C, "xz": SIG_XZ_C, "zx": SIG_ZX_C, "yz": SIG_YZ_C, "zy": SIG_ZY_C,
+         "pm": SIG_PM_C, "mp": SIG_MP_C}
+
+
+@dataclass(frozen=True)
+class SpinOperator:
+    """Lightweight stand-in for the reference's spin Operator objects: only
+    the instruction code matters on this path."""
+    name: str
+    code: int
+    type_acting: str       # "local" | "correlation"
+    modifies: bool
+
+
+def spin_op(kind: str, type_act: str = "local") -> SpinOperator:
+    t = str(getattr(type_act, "name", type_act)).lower()
+    if t.startswith("loc"):
+        code = _LOCAL[kind]
+        t = "local"
+    elif t.startswith("cor"):
+        code = _CORR[kind]
+        t = "correlation"
+    else:
+        raise ValueError("type_act must be 'local' or 'correlation'")
+    return SpinOperator(f"S{kind}/{t[0].upper()}", code, t, kind not in ("z",))
+
+
+class _SpinOps:
+    """ham.operators namespace (sig_x(ns=..., type_act=...), ...)."""
+
+    def __getattr__(self, name):
+        if not name.startswith("sig_"):
+            raise AttributeError(name)
+        kind = name[4:]
+
+        def make(ns=None, type_act="local", sites=None, **kw):
+            return spin_op(kind, type_act)
+        return make
+
+
+class Hamiltonian:
+    """Many-body spin-1/2 Hamiltonian on a HilbertSpace sector."""
+
+    def __init__(self, hilbert_space, dtype=np.float64, name: str = "Hamiltonian", lattice=None, **kwargs):
+        self._hilbert_space = hilbert_space
+        self._lattice = lattice if lattice is not None else getattr(hilbert_space, "lattice", None)
+        self._dtype = np.dtype(dtype)
+        self._name = name
+        self.ns = hilbert_space.ns
+        self._instr_codes: List[int] = []
+        self._instr_coeffs: List[complex] = []
+        self._instr_sites: List[List[int]] = []
+        self._instr_coupling: List[int] = []
+        self._op_handle = None
+        self._cached_matvec_fun = None
+        self._eig_val = None
+        self._eig_vec = None
+        self.operators = _SpinOps()
+        self.last_lanczos = None
+
+    # ---- term list ----------------------------------------------------
+    def _dtype_is_complex(self) -> bool:
+        return np.issubdtype(self._dtype, np.complexfloating)
+
+    @property
+    def dtype(self):
+        return self._dtype
+
+    @property
+    def hilbert_space(self):
+        return self._hilbert_space
+
+    @property
+    def nh(self) -> int:
+        return self._hilbert_space.nh
+
+    def add(self, operator: SpinOperator, multiplier, modifies: bool = False, sites=None, coupling: int = 0):
+        """special_operator.py:1113-1231: append (code, coeff, sites)."""
+        if abs(multiplier) < 1e-15:
+            return
+        s = list(sites) if sites else [0]
+        if len(s) > 2:
+            raise ValueError("spin-1/2 instructions act on at most two sites")
+        s = s + [-1] * (2 - len(s))
+        self._instr_codes.append(int(operator.code))
+        self._instr_coeffs.append(multiplier)
+        self._instr_sites.append(s)
+        self._instr_coupling.append(int(coupling))
+        self._invalidate()
+
+    def _invalidate(self):
+        if self._op_handle is not None:
+            try:
+                _lib.lib().qes_op_destroy(self._op_handle)
+            except Exception:
+                pass
+        self._op_handle = None
+        self._cached_matvec_fun = None
+
+    def instruction_arrays(self):
+        """(sites int32[T,2], coeffs, codes int32[T]) like
+        _finalize_instruction_arrays (special_operator.py:1392-1417)."""
+        n = len(self._instr_codes)
+        sites = np.asarray(self._instr_sites, dtype=np.int32).reshape(n, 2)
+        coeffs = np.asarray(self._instr_coeffs, dtype=np.complex128 if self._dtype_is_complex() else np.float64)
+        codes = np.asarray(self._instr_codes, dtype=np.int32)
+        return sites, coeffs, codes
+
+    def set_coupling_scale(self, scales: Sequence[float]):
+        """Rescale the coefficients of each coupling id (used by autograd)."""
+        self._coupling_scale = np.asarray(scales, dtype=np.float64)
+        self._invalidate()
+
+    # ---- device operator ----------------------------------------------
+    def _device_op(self):
+        if self._op_handle is not None:
+            return self._op_handle
+        sites, coeffs, codes = self.instruction_arrays()
+        cpl = np.asarray(self._instr_coupling, dtype=np.int32)
+        coeffs = coeffs.astype(np.complex128)
+        scale = getattr(self, "_coupling_scale", None)
+        if scale is not None and len(cpl):
+            coeffs = coeffs * scale[cpl]
+        cre = np.ascontiguousarray(coeffs.real)
+        cim = np.ascontiguousarray(coeffs.imag)
+        h = C.c_void_p()
+        T = len(codes)
+        _lib.check(_lib.lib().qes_op_create(
+            self._hilbert_space.handle,
+            codes.ctypes.data_as(_lib._PI32) if T else None,
+            np.ascontiguousarray(sites).ctypes.data_as(_lib._PI32) if T else None,
+            cre.ctypes.data_as(_lib._PD) if T else None,
+            cim.ctypes.data_as(_lib._PD) if T else None,
+            T, 1 if self._dtype_is_complex() else 0,
+            cpl.ctypes.data_as(_lib._PI32) if T else None,
+            C.byref(h)), "qes_op_create")
+        self._op_handle = h
+        return h
+
+    @property
+    def needs_complex(self) -> bool:
+        v = C.c_int32()
+        _lib.check(_lib.lib().qes_op_needs_complex(self._device_op(), C.byref(v)))
+        return bool(v.value)
+
+    @property
+    def vector_dtype(self):
+        return np.complex128 if self.needs_complex else np.float64
+
+    @property
+    def n_couplings(self) -> int:
+        v = C.c_int32()
+        _lib.check(_lib.lib().qes_op_n_couplings(self._device_op(), C.byref(v)))
+        return int(v.value)
+
+    def count_offdiag(self) -> int:
+        v = C.c_int64()
+        _lib.check(_lib.lib().qes_op_count_offdiag(self._device_op(), C.byref(v)), "qes_op_count_offdiag")
+        return int(v.value)
+
+    # ---- matvec ---------------------------------------------------------
+    def matvec_device(self, x: torch.Tensor, out: Optional[torch.Tensor] = None, accumulate: bool = False,
+                      stream=None) -> torch.Tensor:
+        """y = H x for device tensors of shape (nh,) or (nh, nb)."""
+        n = self.nh
+        if x.device.type != "cuda":
+            raise ValueError("matvec_device expects a device tensor")
+        if x.shape[0] != n or x.dim() > 2:
+            raise ValueError(f"vector shape {tuple(x.shape)} does not match nh={n}")
+        if x.dtype not in (torch.float64, torch.complex128):
+            raise TypeError("vectors must be float64 or complex128")
+        x = x.contiguous()
+        nb = 1 if x.dim() == 1 else x.shape[1]
+        y = out if out is not None else torch.empty_like(x)
+        if y.shape != x.shape or y.dtype != x.dtype or not y.is_contiguous():
+            raise ValueError("out must match x in shape/dtype and be contiguous")
+        dtype = 1 if x.dtype == torch.complex128 else 0
+        _lib.check(_lib.lib().qes_matvec(self._device_op(), _lib.dptr(x), _lib.dptr(y), nb, dtype,
+                                         1 if accumulate else 0, _lib.stream_ptr(stream)), "qes_matvec")
+        return y
+
+    def matvec_couplings_device(self, x: torch.Tensor) -> torch.Tensor:
+        """Stack of H_c x for every coupling id c: shape (n_couplings, nh)."""
+        ncpl = self.n_couplings
+        y = torch.empty((ncpl,) + tuple(x.shape), dtype=x.dtype, device=x.device)
+        dtype = 1 if x.dtype == torch.complex128 else 0
+        _lib.check(_lib.lib().qes_matvec_couplings(self._device_op(), _lib.dptr(x.contiguous()), _lib.dptr(y), dtype,
+                                                   _lib.stream_ptr()), "qes_matvec_couplings")
+        return y
+
+    def _matvec_any(self, x):
+        if isinstance(x, torch.Tensor):
+            if x.device.type == "cuda":
+                return self.matvec_device(x)
+            return self.matvec_device(x.to("cuda")).cpu()
+        xa = np.asarray(x)
+        if xa.dtype not in (np.float64, np.complex128):
+            xa = xa.astype(np.complex128 if np.iscomplexobj(xa) else np.float64)
+        xt = torch.from_numpy(np.ascontiguousarray(xa)).to("cuda")
+        y = self.matvec_device(xt)
+        return y.cpu().numpy()
+
+    @property
+    def matvec_fun(self):
+        """Callable x -> H x (special_operator.py:593-709): output has the
+        dtype and shape of x; numpy in -> numpy out, device tensor in ->
+        device tensor out."""
+        if self._cached_matvec_fun is None:
+            self._device_op()
+            self._cached_matvec_fun = self._matvec_any
+        return self._cached_matvec_fun
+
+    def matvec(self, vecs, *args, hilbert_in=None, hilbert_out=None, out=None, **kwargs):
+        """Operator.matvec (operator.py:1559-1602) for the same-sector case."""
+        if hilbert_in is not None and hilbert_in is not self._hilbert_space:
+            raise ValueError("hilbert_in differs from the Hamiltonian's sector")
+        if hilbert_out is not None and hilbert_out is not self._hilbert_space:
+            raise NotImplementedError("sector-changing (projected) matvec is not part of this build")
+        y = self._matvec_any(vecs)
+        if out is not None:
+            if isinstance(out, torch.Tensor):
+                out.copy_(y if isinstance(y, torch.Tensor) else torch.from_numpy(y))
+            else:
+                out[...] = y
+            return out
+        return y
+
+    def __matmul__(self, x):
+        return self._matvec_any(x)
+
+    def as_linear_operator(self, dtype=None):
+        """scipy LinearOperator with the right shape and adjoint = self."""
+        from scipy.sparse.linalg import LinearOperator
+        n = self.nh
+        dt = np.dtype(dtype) if dtype is not None else np.dtype(self.vector_dtype)
+        f = self._matvec_any
+        return LinearOperator((n, n), matvec=f, rmatvec=f, matmat=f, rmatmat=f, dtype=dt)
+
+    @property
+    def hamil(self):
+        return self.as_linear_operator()
+
+    # ---- diagonalisation ------------------------------------------------
+    def diagonalize(self, method: str = "lanczos", k: int = 1, tol: float = 1e-10, maxiter: int = 300,
+                    v0=None, seed: int = 0, reorth: bool = True, store_basis: Optional[bool] = None,
+                    compute_eigvec: bool = False, check_every: int = 10, **kwargs):
+        """Lowest eigenvalue(s) by device Lanczos (hamil.py:1809-2114 with
+        method='lanczos').  Only E0 is claimed to the requested tolerance;
+        the other k-1 Ritz values are returned as computed."""
+        from .lanczos import lanczos_ground_state
+        if method not in ("lanczos", "auto"):
+            raise NotImplementedError("only method='lanczos' is implemented on the device")
+        if self.nh == 0:
+            self._eig_val = np.zeros(0)
+            return self._eig_val
+        res = lanczos_ground_state(self, k=k, tol=tol, maxiter=maxiter, v0=v0, seed=seed, reorth=reorth,
+                                   store_basis=store_basis, compute_eigvec=compute_eigvec, check_every=check_every)
+        self.last_lanczos = res
+        self._eig_val = res.eigenvalues
+        self._eig_vec = res.eigenvectors
+        return self._eig_val
+
+    @property
+    def eig_val(self):
+        return self._eig_val
+
+    @property
+    def eigenvalues(self):
+        return self._eig_val
+
+    @property
+    def eig_vec(self):
+        return self._eig_vec
+
+    def __del__(self):
+        try:
+            if self._op_handle is not None:
+                _lib.lib().qes_op_destroy(self._op_handle)
+        except Exception:
+            pass

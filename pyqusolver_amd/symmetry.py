@@ -1,0 +1,266 @@
+"""Symmetry generators, sector parsing and the device group descriptor.
+
+Mirrors the reference's user-facing symmetry API:
+  * ``SymmetryGenerators`` (Algebra/Operator/operator.py:85-152),
+  * ``parse_symmetry_spec`` / ``normalize_symmetry_generators``
+    (Algebra/Symmetries/symmetry_container.py:2036-2224),
+  * the global-symmetry filter (Algebra/Symmetries/parity.py:283-321),
+  * ``build_group`` / ``get_character`` / ``build_phases_map``
+    (symmetry_container.py:866-1174, :1399-1442, :1767-1804).
+
+One deliberate deviation: reflection together with a translation at a
+generic momentum (k not 0 or L/2) is rejected.  The reference accepts it
+(its is_real_sector is always True, base.py:581-609) and then builds a
+non-Hermitian, wrong block; the characters r*exp(2 pi i k n/L) are not a
+representation of the dihedral group there.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+from dataclasses import dataclass, field
+from enum import Enum
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+SYM_NORM_THRESHOLD = 1e-7
+
+BOP_NONE, BOP_REFLECT, BOP_FLIP, BOP_FLIP_Y = 0, 1, 2, 3
+
+
+class SymmetryGenerators(Enum):
+    Translation_x = "Translation_x"
+    Translation_y = "Translation_y"
+    Translation_z = "Translation_z"
+    Reflection = "Reflection"
+    ParityX = "ParityX"
+    ParityY = "ParityY"
+    ParityZ = "ParityZ"
+    Inversion = "Inversion"
+
+
+class GlobalSymmetries(Enum):
+    U1 = 1
+    Z2_PARITY = 2
+
+
+@dataclass
+class GlobalSymmetry:
+    name: GlobalSymmetries
+    val: float
+    lat: object = None
+
+    def get_val(self):
+        return self.val
+
+
+def get_u1_sym(lat, val) -> GlobalSymmetry:
+    """U(1): popcount == val (Algebra/globals.py:164-189)."""
+    return GlobalSymmetry(GlobalSymmetries.U1, val, lat)
+
+
+def get_z2_sym(lat, val) -> GlobalSymmetry:
+    """Popcount parity (+1 even, -1 odd) (Algebra/globals.py:336-339)."""
+    return GlobalSymmetry(GlobalSymmetries.Z2_PARITY, val, lat)
+
+
+def parse_symmetry_spec(name: str, value) -> List[Tuple[SymmetryGenerators, object]]:
+    """symmetry_container.py:2036-2144 (the spin-1/2 subset)."""
+    key = name.lower().replace("_", "").replace("-", "")
+    out = []
+    if key in {"translation", "translations", "trans", "momentum"}:
+        if isinstance(value, dict):
+            for kname, gen in (("kx", SymmetryGenerators.Translation_x), ("ky", SymmetryGenerators.Translation_y),
+                               ("kz", SymmetryGenerators.Translation_z)):
+                alt = kname[0] + "_" + kname[1]
+                if kname in value or alt in value:
+                    out.append((gen, value.get(kname, value.get(alt))))
+        else:
+            out.append((SymmetryGenerators.Translation_x, value))
+    elif key in {"parity", "parityx", "parityy", "parityz", "spin"}:
+        if isinstance(value, dict):
+            for ax, gen in (("x", SymmetryGenerators.ParityX), ("y", SymmetryGenerators.ParityY),
+                            ("z", SymmetryGenerators.ParityZ)):
+                if ax in value or "p" + ax in value:
+                    out.append((gen, value.get(ax, value.get("p" + ax))))
+        else:
+            out.append((SymmetryGenerators.ParityZ, int(value)))
+    elif key in {"reflection", "reflect", "mirror"}:
+        out.append((SymmetryGenerators.Reflection, value.get("sector", 1) if isinstance(value, dict) else value))
+    elif key in {"inversion", "inv", "spatial", "spatialinversion"}:
+        raise NotImplementedError("inversion symmetry is not part of this build")
+    else:
+        raise ValueError(f"Unknown symmetry name: '{name}'")
+    return out
+
+
+def normalize_symmetry_generators(sym_gen) -> List[Tuple[SymmetryGenerators, object]]:
+    """symmetry_container.py:2147-2224."""
+    if sym_gen is None:
+        return []
+    if isinstance(sym_gen, dict):
+        out = []
+        for k, v in sym_gen.items():
+            out.extend(parse_symmetry_spec(k, v))
+        return out
+    if isinstance(sym_gen, (list, tuple)):
+        out = []
+        for item in sym_gen:
+            g, v = item
+            if isinstance(g, SymmetryGenerators):
+                out.append((g, v))
+            else:
+                out.extend(parse_symmetry_spec(str(g), v))
+        return out
+    raise ValueError(f"sym_gen must be dict, list or None, got {type(sym_gen)}")
+
+
+@dataclass
+class SectorGroup:
+    """A compiled symmetry sector: the device group descriptor plus the
+    reference-ordered characters and phase table."""
+    ns: int
+    n_up: int                      # -1 if no U(1)
+    z2: int                        # 0 if none
+    generators: List[Tuple[SymmetryGenerators, object]]
+    nbase: int
+    base_ops: List[List[int]]
+    ntx: int
+    nty: int
+    row_bits: int
+    cell_bits: int
+    chi: np.ndarray                # complex128, reference element order
+    g_to_pidx: np.ndarray = field(default=None)
+    phase_table: np.ndarray = field(default=None)
+
+    @property
+    def order(self) -> int:
+        return self.ntx * self.nty * self.nbase
+
+    @property
+    def is_real(self) -> bool:
+        return bool(np.all(np.abs(self.chi.imag) <= 1e-12))
+
+    def spec(self):
+        from ._lib import GroupSpec
+        s = GroupSpec()
+        s.ns, s.n_up, s.z2, s.nbase = self.ns, self.n_up, self.z2, self.nbase
+        for b in range(4):
+            for k in range(4):
+                s.base_ops[b][k] = self.base_ops[b][k] if b < self.nbase and k < len(self.base_ops[b]) else 0
+        s.ntx, s.nty, s.row_bits, s.cell_bits = self.ntx, self.nty, self.row_bits, self.cell_bits
+        self._chi_re = np.ascontiguousarray(self.chi.real, dtype=np.float64)
+        self._chi_im = np.ascontiguousarray(self.chi.imag, dtype=np.float64)
+        s.chi_re = self._chi_re.ctypes.data_as(C.POINTER(C.c_double))
+        s.chi_im = self._chi_im.ctypes.data_as(C.POINTER(C.c_double))
+        return s
+
+
+def _is_real_momentum(k, L) -> bool:
+    k = int(k) % L
+    return k == 0 or (L % 2 == 0 and k == L // 2)
+
+
+def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymmetry] = ()) -> SectorGroup:
+    n_up, z2 = -1, 0
+    for g in global_syms or []:
+        if g.name == GlobalSymmetries.U1:
+            n_up = int(g.val)
+        elif g.name == GlobalSymmetries.Z2_PARITY:
+            z2 = int(g.val)
+    # global-symmetry filter (parity.py:283-321) and pairwise rules
+    gens = []
+    for gen, sec in specs:
+        if gen in (SymmetryGenerators.ParityX, SymmetryGenerators.ParityY, SymmetryGenerators.ParityZ) and n_up >= 0:
+            if gen == SymmetryGenerators.ParityZ:
+                continue          # acts trivially with U(1): dropped
+            if n_up != ns // 2 or ns % 2 != 0:
+                continue          # X/Y need half filling
+        if gen in (SymmetryGenerators.Translation_z, SymmetryGenerators.Inversion):
+            raise NotImplementedError(f"{gen.name} is not part of this build")
+        if any(g == gen for g, _ in gens):
+            raise ValueError(f"duplicate generator {gen.name}")
+        gens.append((gen, sec))
+    if any(g in (SymmetryGenerators.Translation_x, SymmetryGenerators.Translation_y) for g, _ in gens) and lattice is None:
+        raise ValueError("translation symmetry requires a lattice")
+    lx = getattr(lattice, "lx", ns) if lattice is not None else ns
+    ly = getattr(lattice, "ly", 1) if lattice is not None else 1
+    nb = getattr(lattice, "n_basis", 1) if lattice is not None else 1
+    trans = {g: s for g, s in gens if g in (SymmetryGenerators.Translation_x, SymmetryGenerators.Translation_y)}
+    others = [(g, s) for g, s in gens if g not in trans]
+    has_refl = any(g == SymmetryGenerators.Reflection for g, _ in others)
+    if has_refl:
+        for g, k in trans.items():
+            L = lx if g == SymmetryGenerators.Translation_x else ly
+            if not _is_real_momentum(k, L):
+                raise ValueError(
+                    f"reflection is only a symmetry of momentum sectors k=0 or k=L/2 (got k={k}, L={L}); "
+                    "use the translation sector alone at generic k")
+    # base elements: identity + combinations of the other generators in order (:918-928)
+    base = [()]
+    for r in range(1, len(others) + 1):
+        for combo in itertools.combinations(range(len(others)), r):
+            base.append(tuple(others[i] for i in combo))
+    if len(base) > 4:
+        raise NotImplementedError("at most two non-translation generators")
+    base_ops, base_chi = [], []
+    for elem in base:
+        ops, chi = [], 1.0
+        for g, sec in elem:
+            if g == SymmetryGenerators.Reflection:
+                ops.append(BOP_REFLECT)
+            elif g == SymmetryGenerators.ParityY:
+                ops.append(BOP_FLIP_Y)
+            else:
+                ops.append(BOP_FLIP)
+            chi = chi * (sec ** 1)                    # base.py:556
+        base_ops.append(ops)
+        base_chi.append(chi)
+    ntx = lx if SymmetryGenerators.Translation_x in trans else 1
+    nty = ly if SymmetryGenerators.Translation_y in trans else 1
+    kx = trans.get(SymmetryGenerators.Translation_x, 0)
+    ky = trans.get(SymmetryGenerators.Translation_y, 0)
+    if lattice is None:
+        row_bits, cell_bits = ns, 1
+    else:
+        row_bits, cell_bits = nb * lx, nb
+        if ns % row_bits != 0:
+            raise ValueError("lattice size does not match ns")
+    # characters in reference order g = (a*nty + b)*nbase + B, i.e. element
+    # base_elem + (Tx^a, Ty^b) with character built in tuple order (:1420-1442)
+    chis = []
+    for a in range(ntx):
+        for b in range(nty):
+            for B in range(len(base)):
+                ch = 1.0
+                ch = ch * base_chi[B] if base[B] else ch
+                if a != 0:
+                    ch = ch * np.exp(2j * np.pi * kx * a / lx)   # translation.py:519
+                if b != 0:
+                    ch = ch * np.exp(2j * np.pi * ky * b / ly)
+                chis.append(np.complex128(ch))
+    chi = np.asarray(chis, dtype=np.complex128)
+    grp = SectorGroup(ns=ns, n_up=n_up, z2=z2, generators=gens, nbase=len(base), base_ops=base_ops, ntx=ntx,
+                      nty=nty, row_bits=row_bits, cell_bits=cell_bits, chi=chi)
+    grp.g_to_pidx, grp.phase_table = phases_map(chi)
+    return grp
+
+
+def phases_map(chi: np.ndarray):
+    """build_phases_map, symmetry_container.py:1767-1804."""
+    uniq: List[complex] = []
+    g_to_pidx = np.zeros(len(chi), dtype=np.uint8)
+    for g, ch in enumerate(chi):
+        found = None
+        for p, e in enumerate(uniq):
+            if abs(ch - e) < SYM_NORM_THRESHOLD:
+                found = p
+                break
+        if found is None:
+            if len(uniq) >= 255:
+                raise MemoryError("Exceeded maximum number of unique phases (255) for phase indexing.")
+            found = len(uniq)
+            uniq.append(ch)
+        g_to_pidx[g] = found
+    return g_to_pidx, np.asarray(uniq, dtype=np.complex128)
