@@ -320,6 +320,20 @@ int qes_op_create(qes_basis_t bh, const int32_t* codes, const int32_t* sites, co
                     &ncpl, &has_imag))
     return -1;
   if (diag.size() + off.size() > QES_MAX_TERMS) return fail("too many term groups");
+  {
+    // Sectors whose reference matrix is not Hermitian: parity y (its +-i
+    // intrinsic phase enters the norms but not the matrix elements) and
+    // reflection combined with a complex (generic-k) translation character.
+    const QesGroup& G = bh->b.group;
+    bool has_rev = false, chi_cplx = false;
+    for (int b = 0; b < G.nbase; ++b)
+      for (int k = 0; k < QES_MAX_BASE_OPS; ++k) has_rev = has_rev || G.base_ops[b][k] == QES_BOP_REV;
+    for (int g = 0; g < bh->b.gsize; ++g) chi_cplx = chi_cplx || std::fabs(bh->b.h_chi_im[g]) > 1e-12;
+    if (G.has_phase)
+      return fail("parity-y sectors are not supported for matvec: the reference block is not Hermitian there");
+    if (has_rev && chi_cplx)
+      return fail("reflection at a generic momentum is not a symmetry sector (non-Hermitian reference block)");
+  }
   auto* h = new qes_op_s();
   Op& op = h->op;
   op.basis = &bh->b;

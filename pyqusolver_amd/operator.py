@@ -152,6 +152,9 @@ class Hamiltonian:
     def _device_op(self):
         if self._op_handle is not None:
             return self._op_handle
+        err = getattr(self._hilbert_space.group, "matvec_error", None)
+        if err:
+            raise _lib.QesError(err)
         sites, coeffs, codes = self.instruction_arrays()
         cpl = np.asarray(self._instr_coupling, dtype=np.int32)
         coeffs = coeffs.astype(np.complex128)

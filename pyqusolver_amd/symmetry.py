@@ -8,11 +8,15 @@ Mirrors the reference's user-facing symmetry API:
   * ``build_group`` / ``get_character`` / ``build_phases_map``
     (symmetry_container.py:866-1174, :1399-1442, :1767-1804).
 
-One deliberate deviation: reflection together with a translation at a
-generic momentum (k not 0 or L/2) is rejected.  The reference accepts it
-(its is_real_sector is always True, base.py:581-609) and then builds a
-non-Hermitian, wrong block; the characters r*exp(2 pi i k n/L) are not a
-representation of the dihedral group there.
+Deliberate deviation: two sector kinds keep their (bit-exact) basis but
+refuse the matvec, because the reference block is not Hermitian there:
+  * reflection together with a translation at a generic momentum (k not 0
+    or L/2): the reference accepts it (is_real_sector is always True,
+    base.py:581-609) but r*exp(2 pi i k n/L) is not a representation of the
+    dihedral group;
+  * parity y: its +-i intrinsic phase (parity.py:55-63) enters the
+    normalisation (symmetry_container_jit.py:154-174) but not the matrix
+    elements (phase_table holds characters only, symmetry_container.py:1783-1803).
 """
 from __future__ import annotations
 
@@ -133,6 +137,7 @@ class SectorGroup:
     chi: np.ndarray                # complex128, reference element order
     g_to_pidx: np.ndarray = field(default=None)
     phase_table: np.ndarray = field(default=None)
+    matvec_error: Optional[str] = None    # sector where the reference block is not Hermitian
 
     @property
     def order(self) -> int:
@@ -190,13 +195,16 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
     trans = {g: s for g, s in gens if g in (SymmetryGenerators.Translation_x, SymmetryGenerators.Translation_y)}
     others = [(g, s) for g, s in gens if g not in trans]
     has_refl = any(g == SymmetryGenerators.Reflection for g, _ in others)
+    matvec_error = None
     if has_refl:
         for g, k in trans.items():
             L = lx if g == SymmetryGenerators.Translation_x else ly
             if not _is_real_momentum(k, L):
-                raise ValueError(
-                    f"reflection is only a symmetry of momentum sectors k=0 or k=L/2 (got k={k}, L={L}); "
-                    "use the translation sector alone at generic k")
+                matvec_error = (f"reflection is only a symmetry of momentum sectors k=0 or k=L/2 (got k={k}, L={L}); "
+                                "the reference block is not Hermitian there -- use the translation sector alone")
+    if any(g == SymmetryGenerators.ParityY for g, _ in others):
+        matvec_error = ("parity-y sectors: the reference's +-i intrinsic phase enters the normalisation but not "
+                        "the matrix elements, so its block is not Hermitian -- use parity x")
     # base elements: identity + combinations of the other generators in order (:918-928)
     base = [()]
     for r in range(1, len(others) + 1):
@@ -243,6 +251,7 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
     chi = np.asarray(chis, dtype=np.complex128)
     grp = SectorGroup(ns=ns, n_up=n_up, z2=z2, generators=gens, nbase=len(base), base_ops=base_ops, ntx=ntx,
                       nty=nty, row_bits=row_bits, cell_bits=cell_bits, chi=chi)
+    grp.matvec_error = matvec_error
     grp.g_to_pidx, grp.phase_table = phases_map(chi)
     return grp
 

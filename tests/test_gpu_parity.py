@@ -1,0 +1,274 @@
+"""GPU parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: representatives / index maps bit-exact; matvec <= 1e-12 relative L2;
+E0 <= 1e-10 relative.  Sizes the oracle finishes in seconds; larger sizes
+through size-independent properties (closed-form dimensions and energies,
+hermiticity, determinism, sampled rows)."""
+import numpy as np
+import pytest
+
+from oracle import qes_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MATVEC_TOL = 1e-12
+E0_TOL = 1e-10
+
+
+def _lat(kind, lx, ly):
+    from pyqusolver_amd import ChainLattice, HoneycombLattice, SquareLattice
+    return {"chain": lambda: ChainLattice(lx), "square": lambda: SquareLattice(lx, ly),
+            "honeycomb": lambda: HoneycombLattice(lx, ly)}[kind]()
+
+
+def _sym(gens):
+    sym = {}
+    tr = {}
+    for k, s in gens:
+        if k == "tx":
+            tr["kx"] = s
+        elif k == "ty":
+            tr["ky"] = s
+        elif k == "R":
+            sym["reflection"] = s
+        elif k in ("Px", "Py", "Pz"):
+            sym.setdefault("parity", {})[k[1]] = s
+    if tr:
+        sym = {"translation": tr, **sym}
+    return sym
+
+
+def build_pair(kind, lx, ly, gens, n_up, model, z2=None):
+    from pyqusolver_amd import get_u1_sym, get_z2_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import HeisenbergKitaev, J1J2Model, XXZ
+    lat = _lat(kind, lx, ly)
+    olat = O.OLattice(kind, lx, ly)
+    gl = []
+    if n_up is not None:
+        gl.append(get_u1_sym(lat, n_up))
+    if z2 is not None:
+        gl.append(get_z2_sym(lat, z2))
+    hs = HilbertSpace(lattice=lat, sym_gen=_sym(gens), global_syms=gl)
+    grp = O.Group(olat, [O.Generator(k, s) for k, s in gens], n_up=n_up, z2=z2)
+    b = O.scan_basis(grp)
+    if model == "xxz":
+        H = XXZ(lat, hilbert_space=hs, jxy=1.0, jz=0.7, hz=0.1)
+        t = O.terms_xxz(olat, 1.0, 0.7, hz=0.1)
+    elif model == "j1j2":
+        H = J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=hs)
+        t = O.terms_j1j2(olat, -1.0, -0.5)
+    elif model == "j1j2_field":
+        H = J1J2Model(lat, J1=-1.0, J2=-0.3, hilbert_space=hs, hx=0.2, hz=0.1)
+        t = O.terms_j1j2(olat, -1.0, -0.3, hx=0.2, hz=0.1)
+    else:
+        h = 0.05 / np.sqrt(3)
+        H = HeisenbergKitaev(lat, K=-1.0, J=0.2, hilbert_space=hs, hx=h, hy=h, hz=h)
+        t = O.terms_heisenberg_kitaev(olat, K=-1.0, J=0.2, hx=h, hy=h, hz=h)
+    return hs, H, grp, b, t
+
+
+CASES = [
+    ("chain", 16, 1, [("tx", 0)], 8, "j1j2"),
+    ("chain", 14, 1, [("tx", 3)], 7, "xxz"),
+    ("chain", 14, 1, [("tx", 7), ("R", -1)], 7, "xxz"),
+    ("chain", 14, 1, [("tx", 0), ("R", 1), ("Px", -1)], 7, "j1j2"),
+    ("chain", 14, 1, [("tx", 7), ("R", -1), ("Px", 1)], 7, "j1j2"),
+    ("chain", 13, 1, [("tx", 5)], 6, "xxz"),
+    ("chain", 12, 1, [("tx", 0), ("Py", 1)], 6, "j1j2"),
+    ("chain", 12, 1, [("tx", 2), ("Py", -1)], 6, "j1j2"),
+    ("chain", 12, 1, [("tx", 0), ("R", 1)], None, "j1j2_field"),
+    ("chain", 12, 1, [("tx", 4), ("Px", 1)], None, "j1j2_field"),
+    ("square", 4, 4, [("tx", 0), ("ty", 0), ("Px", 1)], 8, "j1j2"),
+    ("square", 4, 3, [("tx", 1), ("ty", 2)], 6, "j1j2"),
+    ("square", 4, 3, [("tx", 2), ("ty", 0), ("R", -1)], 6, "j1j2"),
+    ("square", 3, 4, [("ty", 1)], 6, "j1j2"),
+    ("honeycomb", 3, 2, [("tx", 0), ("ty", 0)], None, "hk"),
+    ("honeycomb", 2, 3, [("tx", 1), ("ty", 2)], None, "hk"),
+]
+
+
+def _ids(c):
+    return f"{c[0]}{c[1]}x{c[2]}-" + "-".join(f"{k}{s}" for k, s in c[3]) + f"-n{c[4]}-{c[5]}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_basis_bit_exact_and_matvec(case):
+    kind, lx, ly, gens, n_up, model = case
+    hs, H, grp, b, t = build_pair(kind, lx, ly, gens, n_up, model)
+    assert hs.nh == len(b.reps)
+    if not any(k == "Py" for k, _ in gens):
+        assert hs.nh == O.sector_dimension(grp)
+    np.testing.assert_array_equal(hs.representative_list, b.reps)
+    np.testing.assert_allclose(hs.normalization, b.norms, rtol=1e-14, atol=0)
+    idx, pidx = hs.map_states(b.states)
+    np.testing.assert_array_equal(idx, b.state_rep)
+    np.testing.assert_array_equal(pidx, b.state_pidx)
+    np.testing.assert_allclose(hs.group.phase_table, b.phase_table, rtol=0, atol=0)
+    if any(k == "Py" for k, _ in gens):
+        # the reference block is not Hermitian here: basis kept, matvec refused
+        from pyqusolver_amd._lib import QesError
+        with pytest.raises(QesError):
+            H.matvec_fun(np.ones(hs.nh, dtype=np.complex128))
+        return
+    c, s, co = t.arrays()
+    rng = np.random.default_rng(7)
+    n = hs.nh
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = H.matvec_fun(x)
+    yo = O.push_matvec(b, lx * ly * (2 if kind == "honeycomb" else 1), c, s, co, t.is_complex, x)
+    assert np.linalg.norm(y - yo) <= MATVEC_TOL * np.linalg.norm(yo)
+    if not H.needs_complex:
+        xr = rng.standard_normal(n)
+        yr = H.matvec_fun(xr)
+        assert yr.dtype == np.float64
+        yor = O.push_matvec(b, lx * ly * (2 if kind == "honeycomb" else 1), c, s, co, t.is_complex, xr)
+        assert np.linalg.norm(yr - yor) <= MATVEC_TOL * np.linalg.norm(yor)
+
+
+@pytest.mark.parametrize("nb", [2, 3, 5])
+def test_multi_rhs_matches_single(nb):
+    hs, H, grp, b, t = build_pair("chain", 14, 1, [("tx", 3)], 7, "xxz")
+    rng = np.random.default_rng(nb)
+    X = rng.standard_normal((hs.nh, nb)) + 1j * rng.standard_normal((hs.nh, nb))
+    Y = H.matvec_fun(X)
+    for j in range(nb):
+        yj = H.matvec_fun(np.ascontiguousarray(X[:, j]))
+        np.testing.assert_array_equal(Y[:, j], yj)
+
+
+def test_matvec_is_deterministic_and_linear():
+    import torch
+    hs, H, grp, b, t = build_pair("square", 4, 4, [("tx", 0), ("ty", 0), ("Px", 1)], 8, "j1j2")
+    x = torch.randn(hs.nh, dtype=torch.float64, device="cuda")
+    z = torch.randn(hs.nh, dtype=torch.float64, device="cuda")
+    y1 = H.matvec_device(x)
+    y2 = H.matvec_device(x)
+    assert torch.equal(y1, y2)
+    lin = H.matvec_device(2.0 * x + z) - (2.0 * y1 + H.matvec_device(z))
+    assert lin.norm().item() <= 1e-13 * y1.norm().item()
+
+
+def test_accumulate_flag():
+    import torch
+    hs, H, grp, b, t = build_pair("chain", 14, 1, [("tx", 0), ("R", 1), ("Px", -1)], 7, "j1j2")
+    x = torch.randn(hs.nh, dtype=torch.float64, device="cuda")
+    y0 = torch.randn(hs.nh, dtype=torch.float64, device="cuda")
+    y = y0.clone()
+    H.matvec_device(x, out=y, accumulate=True)
+    assert torch.allclose(y, y0 + H.matvec_device(x), rtol=1e-14, atol=1e-14)
+
+
+def test_real_vector_with_complex_sector_raises():
+    from pyqusolver_amd._lib import QesError
+    hs, H, grp, b, t = build_pair("chain", 14, 1, [("tx", 3)], 7, "xxz")
+    with pytest.raises(QesError):
+        H.matvec_fun(np.ones(hs.nh))
+
+
+def test_linear_operator_drop_in():
+    import scipy.sparse.linalg as spla
+    hs, H, grp, b, t = build_pair("chain", 16, 1, [("tx", 0)], 8, "j1j2")
+    A = H.as_linear_operator()
+    assert A.shape == (810, 810)
+    w = spla.eigsh(A, k=1, which="SA", tol=1e-12)[0][0]
+    c, s, co = t.arrays()
+    e_ref = np.linalg.eigvalsh(O.reduced_dense(b, 16, c, s, co, t.is_complex))[0]
+    assert abs(w - e_ref) <= E0_TOL * abs(e_ref)
+
+
+def test_chain16_ground_state_energy():
+    hs, H, grp, b, t = build_pair("chain", 16, 1, [("tx", 0)], 8, "j1j2")
+    H2 = __import__("pyqusolver_amd.models", fromlist=["J1J2Model"]).J1J2Model(hs.lattice, J1=-1.0, J2=0.0,
+                                                                                hilbert_space=hs)
+    e0 = H2.diagonalize(method="lanczos", k=1, tol=1e-13, maxiter=200, seed=0)[0]
+    c, s, co = O.terms_j1j2(O.OLattice("chain", 16), -1.0, 0.0).arrays()
+    e_ref = np.linalg.eigvalsh(O.reduced_dense(b, 16, c, s, co, False))[0]
+    assert abs(e0 - e_ref) <= E0_TOL * abs(e_ref)
+    assert abs(e0 - (-7.14229636061679)) <= E0_TOL * 7.15
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[1] * c[2] <= 16 and all(k != "Py" for k, _ in c[3])],
+                         ids=_ids)
+def test_lanczos_e0_against_dense(case):
+    kind, lx, ly, gens, n_up, model = case
+    hs, H, grp, b, t = build_pair(kind, lx, ly, gens, n_up, model)
+    if hs.nh < 2:
+        pytest.skip("trivial sector")
+    c, s, co = t.arrays()
+    ns = lx * ly * (2 if kind == "honeycomb" else 1)
+    e_ref = np.linalg.eigvalsh(O.reduced_dense(b, ns, c, s, co, t.is_complex))[0]
+    e0 = H.diagonalize(k=1, tol=1e-13, maxiter=min(300, hs.nh), compute_eigvec=True)[0]
+    assert abs(e0 - e_ref) <= E0_TOL * max(1.0, abs(e_ref))
+    psi = H.eig_vec
+    r = H.matvec_device(psi) - e0 * psi
+    assert r.norm().item() <= 1e-6
+
+
+def test_majumdar_ghosh_chain24_exact_energy():
+    """J1-J2 chain at J2 = J1/2: E0 = -3L/8 exactly (Majumdar-Ghosh)."""
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import J1J2Model
+    lat = ChainLattice(24)
+    hs = HilbertSpace(lattice=lat, sym_gen={"translation": 0, "parity": {"x": 1}}, global_syms=[get_u1_sym(lat, 12)])
+    H = J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=hs)
+    e0 = H.diagonalize(k=1, tol=1e-13, maxiter=300)[0]
+    assert abs(e0 - (-9.0)) <= E0_TOL * 9.0
+
+
+def test_chain32_scale_properties():
+    """L=32 J1-J2, Sz=0, k=0, flip+ (9,393,119 states): closed-form dimension,
+    sorted reps, sampled rows against the oracle, hermiticity, determinism."""
+    import torch
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import J1J2Model
+    lat = ChainLattice(32)
+    hs = HilbertSpace(lattice=lat, sym_gen={"translation": 0, "parity": {"x": 1}}, global_syms=[get_u1_sym(lat, 16)])
+    assert hs.nh == 9_393_119
+    reps = hs.representative_list
+    assert np.all(np.diff(reps) > 0)
+    H = J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=hs)
+    n = hs.nh
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    z = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    y = H.matvec_device(x)
+    assert torch.equal(y, H.matvec_device(x))
+    lhs = torch.dot(z, y).item()
+    rhs = torch.dot(H.matvec_device(z), x).item()
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    # sampled rows against the oracle (group action by permutation tables)
+    grp = O.Group(O.OLattice("chain", 32), [O.Generator("tx", 0), O.Generator("Px", 1)], n_up=16)
+    t = O.terms_j1j2(O.OLattice("chain", 32), -1.0, -0.5)
+    c, s, co = t.arrays()
+    rng = np.random.default_rng(1)
+    sample = np.sort(rng.choice(n, 300, replace=False))
+    xs = x.cpu().numpy()
+    ys = O.sampled_rows(grp, reps, hs.normalization, sample, 32, c, s, co, False, xs)
+    yg = y.cpu().numpy()[sample]
+    assert np.linalg.norm(yg - ys.real) <= MATVEC_TOL * np.linalg.norm(ys)
+    # norms: N_r^2 = |Stab(r)| on the sample
+    _, _, cnt = O.canonicalize(grp, reps[sample])
+    np.testing.assert_allclose(hs.normalization[sample] ** 2, cnt, rtol=1e-14)
+
+
+def test_index_map_on_random_states_chain32():
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    lat = ChainLattice(32)
+    hs = HilbertSpace(lattice=lat, sym_gen={"translation": 0, "parity": {"x": 1}}, global_syms=[get_u1_sym(lat, 16)])
+    reps = hs.representative_list
+    grp = O.Group(O.OLattice("chain", 32), [O.Generator("tx", 0), O.Generator("Px", 1)], n_up=16)
+    rng = np.random.default_rng(5)
+    states = []
+    while len(states) < 400:
+        bits = rng.choice(32, 16, replace=False)
+        states.append(int(np.sum(1 << bits.astype(np.int64))))
+    states = np.array(states, dtype=np.int64)
+    idx, pidx = hs.map_states(states)
+    best, bg, _ = O.canonicalize(grp, states)
+    pos = np.searchsorted(reps, best)
+    ok = (pos < len(reps)) & (reps[np.minimum(pos, len(reps) - 1)] == best)
+    np.testing.assert_array_equal(idx, np.where(ok, pos, -1))

@@ -1,0 +1,109 @@
+"""Host-side logic of pyqusolver_amd that needs no GPU: sector parsing,
+characters in reference order, model instruction lists, refused sectors."""
+import numpy as np
+import pytest
+
+from oracle import qes_oracle as O
+from pyqusolver_amd import ChainLattice, HoneycombLattice, SquareLattice
+from pyqusolver_amd.symmetry import (SymmetryGenerators, build_sector_group, get_u1_sym,
+                                     normalize_symmetry_generators)
+
+
+def _grp(lat, sym, n_up=None):
+    specs = normalize_symmetry_generators(sym)
+    gl = [get_u1_sym(lat, n_up)] if n_up is not None else []
+    return build_sector_group(lat, lat.ns, specs, gl)
+
+
+@pytest.mark.parametrize("lat,olat,sym,ogens,n_up", [
+    (ChainLattice(12), O.OLattice("chain", 12), {"translation": 5}, [("tx", 5)], 6),
+    (ChainLattice(12), O.OLattice("chain", 12), {"translation": 6, "reflection": -1, "parity": {"x": -1}},
+     [("tx", 6), ("R", -1), ("Px", -1)], 6),
+    (SquareLattice(4, 3), O.OLattice("square", 4, 3), {"translation": {"kx": 1, "ky": 2}, "parity": {"x": 1}},
+     [("tx", 1), ("ty", 2), ("Px", 1)], 6),
+    (HoneycombLattice(3, 2), O.OLattice("honeycomb", 3, 2), {"translation": {"kx": 2, "ky": 1}},
+     [("tx", 2), ("ty", 1)], None),
+])
+def test_characters_bit_exact_in_reference_order(lat, olat, sym, ogens, n_up):
+    g = _grp(lat, sym, n_up)
+    og = O.Group(olat, [O.Generator(k, s) for k, s in ogens], n_up=n_up)
+    assert g.order == og.n_group
+    np.testing.assert_array_equal(g.chi, og.chi)
+    gp, pt = O.phases_map(og)
+    np.testing.assert_array_equal(g.g_to_pidx, gp)
+    np.testing.assert_array_equal(g.phase_table, pt)
+
+
+def test_parity_z_dropped_with_u1_and_x_needs_half_filling():
+    lat = ChainLattice(10)
+    g = _grp(lat, {"translation": 0, "parity": 1}, 5)           # int -> ParityZ, dropped
+    assert g.nbase == 1
+    g = _grp(lat, {"translation": 0, "parity": {"x": 1}}, 4)    # not half filling
+    assert g.nbase == 1
+    g = _grp(lat, {"translation": 0, "parity": {"x": 1}}, 5)
+    assert g.nbase == 2
+
+
+def test_non_hermitian_reference_sectors_are_flagged():
+    lat = ChainLattice(8)
+    assert _grp(lat, {"translation": 2, "reflection": 1}, 4).matvec_error
+    assert _grp(lat, {"translation": 4, "reflection": 1}, 4).matvec_error is None
+    assert _grp(lat, {"translation": 0, "parity": {"y": 1}}, 4).matvec_error
+    # the flagged sectors are indeed non-Hermitian in the reference semantics
+    olat = O.OLattice("chain", 8)
+    t = O.terms_xxz(olat, 1.0, 0.7)
+    c, s, co = t.arrays()
+    for gens in ([O.Generator("tx", 2), O.Generator("R", 1)], [O.Generator("tx", 0), O.Generator("Py", 1)]):
+        grp = O.Group(olat, gens, n_up=4)
+        b = O.scan_basis(grp)
+        H = O.reduced_dense(b, 8, c, s, co, True)
+        assert np.abs(H - H.conj().T).max() > 1e-3
+
+
+def _dummy_space(lat):
+    class S:
+        pass
+    s = S()
+    s.ns = lat.ns
+    s.lattice = lat
+    return s
+
+
+def test_model_instruction_lists_match_oracle():
+    from pyqusolver_amd.models import HeisenbergKitaev, J1J2Model, XXZ
+    for lat, olat in ((ChainLattice(10), O.OLattice("chain", 10)), (SquareLattice(4, 3), O.OLattice("square", 4, 3))):
+        H = J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=_dummy_space(lat), hz=0.3)
+        sites, coeffs, codes = H.instruction_arrays()
+        c, s, co = O.terms_j1j2(olat, -1.0, -0.5, hz=0.3).arrays()
+        np.testing.assert_array_equal(codes, c)
+        np.testing.assert_array_equal(sites, s)
+        np.testing.assert_array_equal(coeffs, co)
+        H = XXZ(lat, hilbert_space=_dummy_space(lat), jxy=1.0, jz=0.7, hz=np.linspace(-1, 1, lat.ns))
+        sites, coeffs, codes = H.instruction_arrays()
+        c, s, co = O.terms_xxz(olat, 1.0, 0.7, hz=np.linspace(-1, 1, lat.ns)).arrays()
+        np.testing.assert_array_equal(codes, c)
+        np.testing.assert_array_equal(sites, s)
+        np.testing.assert_array_equal(coeffs, co)
+    lat, olat = HoneycombLattice(3, 2), O.OLattice("honeycomb", 3, 2)
+    h = 0.05 / np.sqrt(3)
+    H = HeisenbergKitaev(lat, K=-1.0, J=0.2, hilbert_space=_dummy_space(lat), hx=h, hy=h, hz=h)
+    sites, coeffs, codes = H.instruction_arrays()
+    c, s, co = O.terms_heisenberg_kitaev(olat, K=-1.0, J=0.2, hx=h, hy=h, hz=h).arrays()
+    np.testing.assert_array_equal(codes, c)
+    np.testing.assert_array_equal(sites, s)
+    np.testing.assert_array_equal(coeffs, co)
+
+
+def test_split_couplings_same_matrix_bruteforce():
+    """HeisenbergKitaev(split_couplings=True) gives the same full-space
+    matrix as the reference's merged instruction list."""
+    from pyqusolver_amd.models import HeisenbergKitaev
+    lat = HoneycombLattice(2, 2)
+    kw = dict(K=-1.0, J=0.2, hx=0.02, hy=0.03, hz=0.04)
+    A = HeisenbergKitaev(lat, hilbert_space=_dummy_space(lat), **kw)
+    B = HeisenbergKitaev(lat, hilbert_space=_dummy_space(lat), split_couplings=True, **kw)
+    sa, ca, ka = A.instruction_arrays()
+    sb, cb, kb = B.instruction_arrays()
+    Ha = O.full_dense_hamiltonian(8, ka, sa, ca, True)
+    Hb = O.full_dense_hamiltonian(8, kb, sb, cb, True)
+    np.testing.assert_allclose(Ha, Hb, atol=1e-14)
