@@ -34,6 +34,8 @@ def parse():
     p.add_argument("--no-lanczos", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=2000, help="columns timed for the CPU baseline (0: skip)")
     p.add_argument("--quiet", action="store_true")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="also run a plain device copy of MIB MiB (profiler counter calibration)")
     return p.parse_args()
 
 
@@ -112,6 +114,17 @@ def main():
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     log(f"matvec {ms_per_step:.3f} ms/step (kernel {kern_ms:.3f} ms), B_alg {b_alg/1e9:.3f} GB, "
         f"{achieved:.1f} GB/s", a.quiet)
+
+    if a.calibrate_copy > 0:
+        # plain copy of a known size: its FETCH_SIZE / WRITE_SIZE per dispatch
+        # calibrate the profiler's byte counters
+        nel = a.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     lanczos = None
     if not a.no_lanczos:
