@@ -118,6 +118,19 @@ int qes_op_n_couplings(qes_op_t op, int32_t* n);
 int qes_op_count_offdiag(qes_op_t op, int64_t* e_off);
 
 /*
+ * Tuning and diagnostics (no reference counterpart; the reference has one
+ * kernel).  Canonicaliser kinds: -1 automatic, 0 brute force over all group
+ * elements, 1 cyclic chains (longest-zero-run candidates), 2 lattices with
+ * rows (row-rotation lookup table).  qes_op_get_plan reports the chosen kind,
+ * the number of exchange-bond classes and of generic term groups.
+ * qes_basis_canon_check counts states on which canonicaliser `kind` differs
+ * from the brute-force one (representative, stabiliser size, character).
+ */
+int qes_op_set_canonicalizer(qes_op_t op, int32_t kind);
+int qes_op_get_plan(qes_op_t op, int32_t* canon, int32_t* n_classes, int32_t* n_generic);
+int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_host, int64_t n, int64_t* mismatches);
+
+/*
  * Device Lanczos (FP64).  Replaces the eigensolver that
  * DiagonalizationEngine.diagonalize dispatches to
  * (Algebra/Hamil/hamil_diag_engine.py:176-300, called from

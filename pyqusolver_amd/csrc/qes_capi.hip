@@ -358,12 +358,47 @@ int qes_op_create(qes_basis_t bh, const int32_t* codes, const int32_t* sites, co
     delete h;
     return fail("copy of term groups failed");
   }
+  if (build_plan(&op, -1)) {
+    hipFree(op.d_groups);
+    free_plan(&op);
+    delete h;
+    return -1;
+  }
   *out = h;
   return 0;
 }
 
+int qes_op_set_canonicalizer(qes_op_t h, int32_t kind) {
+  if (!h) return fail("null operator");
+  if (kind < -1 || kind > 2) return fail("kind must be -1 (auto), 0 (brute), 1 (cyclic) or 2 (rows)");
+  return build_plan(&h->op, kind);
+}
+
+int qes_op_get_plan(qes_op_t h, int32_t* canon, int32_t* n_classes, int32_t* n_generic) {
+  if (!h) return fail("null operator");
+  if (canon) *canon = h->op.plan.canon;
+  if (n_classes) *n_classes = h->op.plan.nclass;
+  if (n_generic) *n_generic = h->op.plan.n_gen;
+  return 0;
+}
+
+int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_host, int64_t n, int64_t* mismatches) {
+  if (!b || !mismatches) return fail("null argument");
+  if (n <= 0) {
+    *mismatches = 0;
+    return 0;
+  }
+  uint64_t* ds = nullptr;
+  QES_HIP(hipMalloc(&ds, n * 8));
+  QES_HIP(hipMemcpy(ds, states_host, n * 8, hipMemcpyHostToDevice));
+  int rc = canon_check(&b->b, kind, nullptr, ds, n, mismatches, nullptr);
+  hipFree(ds);
+  return rc;
+}
+
 int qes_op_destroy(qes_op_t h) {
   if (!h) return 0;
+  free_plan(&h->op);
   if (h->op.d_groups) hipFree(h->op.d_groups);
   delete h;
   return 0;

@@ -277,3 +277,191 @@ __device__ __forceinline__ int term_bits(const QesTermGroup& t, W s) {
   int b1 = t.p1 >= 0 ? (int)((s >> t.p1) & 1) : 0;
   return (b0 << 1) | b1;
 }
+
+// --------------------------------------------------------------------------
+// Fast canonicalisers.  All return the same (rep, count) as canonicalize();
+// the reported element g is *a* minimiser (not necessarily the first), which
+// has the same character as the first one whenever the state belongs to the
+// sector (characters of a one-dimensional irrep are constant on the coset
+// of the stabiliser), i.e. whenever the lookup succeeds.
+// --------------------------------------------------------------------------
+#define QES_CANON_BRUTE 0
+#define QES_CANON_CYCLIC 1
+#define QES_CANON_ROWS 2
+
+// rotate left within ns bits; FULL: ns == word width
+template <typename W, bool FULL>
+__device__ __forceinline__ W rotc(W v, int r, int ns, W mask) {
+  if (FULL) {
+    if (sizeof(W) == 4) return (W)__builtin_rotateleft32((uint32_t)v, (uint32_t)r);
+    return (W)__builtin_rotateleft64((uint64_t)v, (uint64_t)r);
+  }
+  return ((v << r) | (v >> (ns - r))) & mask;   // valid for 0 <= r < ns < width
+}
+
+// Cyclic group of rotations (one lattice row = the whole word, cell_bits 1)
+// times the base elements.  The minimal rotation of v starts with the longest
+// cyclic run of zeros of v, so only the starts of maximal runs are evaluated.
+// run_masks: R_k[p] = 1 iff bits p, p-1, ..., p-k+1 (cyclic) of v are all 0.
+template <typename W, bool FULL>
+__device__ __forceinline__ int max_zero_run(W v, int ns, W mask, W* starts) {
+  const W z = (~v) & mask;
+  if (z == 0) {   // all ones: every rotation is equal, all are candidates
+    *starts = mask;
+    return 0;
+  }
+  if (z == mask) {
+    *starts = mask;
+    return ns;
+  }
+  W R[6];
+  R[0] = z;
+#pragma unroll
+  for (int j = 1; j < 6; ++j) {
+    const int sh = 1 << (j - 1);
+    R[j] = (sh < ns) ? (R[j - 1] & rotc<W, FULL>(R[j - 1], sh, ns, mask)) : (W)0;
+  }
+  W cur = mask;
+  int kk = 0;
+#pragma unroll
+  for (int j = 5; j >= 0; --j) {
+    const int step = 1 << j;
+    if (step < ns) {
+      const W cand = cur & rotc<W, FULL>(R[j], kk % ns, ns, mask);
+      if (cand) {
+        cur = cand;
+        kk += step;
+      }
+    }
+  }
+  *starts = cur;
+  return kk;
+}
+
+template <typename W, bool FULL>
+__device__ __forceinline__ Canon<W> canonicalize_cyclic(const QesGroup& G, W s) {
+  const int ns = G.ns;
+  const W mask = (W)G.mask;
+  W vb[QES_MAX_BASE], st[QES_MAX_BASE];
+  int run[QES_MAX_BASE];
+  int best_run = -1;
+#pragma unroll
+  for (int B = 0; B < QES_MAX_BASE; ++B) {
+    run[B] = -1;
+    st[B] = 0;
+    vb[B] = 0;
+    if (B < G.nbase) {
+      int dummy = 0;
+      vb[B] = apply_base<W>(G, B, s, &dummy);
+      run[B] = max_zero_run<W, FULL>(vb[B], ns, mask, &st[B]);
+      best_run = run[B] > best_run ? run[B] : best_run;
+    }
+  }
+  Canon<W> c;
+  c.rep = ~(W)0;
+  c.count = 0;
+  c.g = 0;
+#pragma unroll
+  for (int B = 0; B < QES_MAX_BASE; ++B) {
+    if (B < G.nbase && run[B] == best_run) {
+      W m = st[B];
+      while (m) {
+        const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
+        m &= m - 1;
+        const int r = (ns - 1 - p);   // rotation that brings bit p to the top
+        const W v = rotc<W, FULL>(vb[B], r, ns, mask);
+        const int g = r * G.nbase + B;
+        if (v < c.rep) {
+          c.rep = v;
+          c.count = 1;
+          c.g = g;
+        } else if (v == c.rep) {
+          c.count += 1;
+        }
+      }
+    }
+  }
+  return c;
+}
+
+// Lattices with rows: the top row of the minimum must be the smallest
+// row-rotation over all (base element, row that y-translations bring to the
+// top).  lut[rv] = minval | (amask << 16) over x powers a (a*cell_bits bit
+// rotation of the row).  Only candidates whose top row attains it are
+// evaluated in full.
+template <typename W>
+__device__ __forceinline__ Canon<W> canonicalize_rows(const QesGroup& G, const uint32_t* __restrict__ lut, W s) {
+  const int ns = G.ns;
+  const W mask = (W)G.mask;
+  const int Wr = G.row_bits;
+  const int nrows = ns / Wr;
+  const W rowmask = (W)((Wr == 64) ? ~0ull : ((1ull << Wr) - 1ull));
+  const int q_lo = (G.nty > 1) ? 0 : nrows - 1;
+  W vb[QES_MAX_BASE];
+#pragma unroll
+  for (int B = 0; B < QES_MAX_BASE; ++B) {
+    int dummy = 0;
+    vb[B] = (B < G.nbase) ? apply_base<W>(G, B, s, &dummy) : (W)0;
+  }
+  uint32_t mtop = 0xffffu;
+#pragma unroll
+  for (int B = 0; B < QES_MAX_BASE; ++B) {
+    if (B < G.nbase) {
+      for (int q = q_lo; q < nrows; ++q) {
+        const uint32_t e = lut[(uint32_t)((vb[B] >> (q * Wr)) & rowmask)] & 0xffffu;
+        mtop = e < mtop ? e : mtop;
+      }
+    }
+  }
+  Canon<W> c;
+  c.rep = ~(W)0;
+  c.count = 0;
+  c.g = 0;
+#pragma unroll
+  for (int B = 0; B < QES_MAX_BASE; ++B) {
+    if (B < G.nbase) {
+      for (int q = q_lo; q < nrows; ++q) {
+        const uint32_t e = lut[(uint32_t)((vb[B] >> (q * Wr)) & rowmask)];
+        if ((e & 0xffffu) != mtop) continue;
+        uint32_t am = e >> 16;
+        const int b = nrows - 1 - q;
+        while (am) {
+          const int a = __builtin_ctz(am);
+          am &= am - 1;
+          const W t = rotl_rows<W>(vb[B], a * G.cell_bits, Wr, (W)G.rowlo[a], mask);
+          const W v = rotl_ns<W>(t, b * Wr, ns, mask);
+          const int g = (a * G.nty + (G.nty > 1 ? b : 0)) * G.nbase + B;
+          if (v < c.rep) {
+            c.rep = v;
+            c.count = 1;
+            c.g = g;
+          } else if (v == c.rep) {
+            c.count += 1;
+          }
+        }
+      }
+    }
+  }
+  return c;
+}
+
+// --------------------------------------------------------------------------
+// Operator plan: exchange bonds grouped into "shift classes" plus generic
+// term groups.  An exchange group flips bits p0 and p1 = (p0 - d) mod ns and
+// is non-zero only when they differ; for a class d the active bonds of a
+// state x are the set bits of (x ^ rot(x, d)) & pmask.
+// Value table exval[((c*ns + p0)*2 + bit(p0))*2 + {re,im}].
+// --------------------------------------------------------------------------
+#define QES_MAX_CLASSES 8
+struct QesExClass {
+  uint64_t pmask;
+  int32_t d;
+  int32_t pad;
+};
+struct QesPlan {
+  int32_t canon;     // QES_CANON_*
+  int32_t nclass;
+  int32_t n_gen;     // generic off-diagonal groups (after the diagonal ones)
+  int32_t lut_bits;  // ROWS: row LUT has 1 << lut_bits entries
+  QesExClass cls[QES_MAX_CLASSES];
+};

@@ -51,7 +51,13 @@ struct Basis {
 // Compiled operator (Hamiltonian) bound to a basis.
 struct Op {
   Basis* basis = nullptr;
-  int n_diag = 0, n_off = 0;       // groups
+  int n_diag = 0, n_off = 0;       // groups (n_off = all off-diagonal groups)
+  QesPlan plan{};                  // exchange classes + generic groups + canon kind
+  QesTermGroup* d_gen = nullptr;   // device: diag groups then generic off groups
+  double* d_exval = nullptr;       // device exchange value table
+  int32_t* d_excpl = nullptr;      // device exchange coupling ids [c][p]
+  uint32_t* d_lut = nullptr;       // device row LUT (ROWS canonicaliser)
+  size_t lds_bytes = 0;
   int n_coupling = 1;              // distinct coupling ids
   bool is_complex = false;         // some term value is complex
   bool chi_complex = false;        // sector characters complex
@@ -72,5 +78,9 @@ int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int 
            hipStream_t st);
 int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_t st);
 int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st);
+int build_plan(Op* op, int canon_override);   // -1: automatic
+void free_plan(Op* op);
+int canon_check(const Basis* B, int kind, const uint32_t* d_lut, const uint64_t* d_states, int64_t n,
+                int64_t* d_bad, hipStream_t st);
 
 }  // namespace qes

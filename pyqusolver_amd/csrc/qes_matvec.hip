@@ -11,17 +11,17 @@
 // N_b = sqrt(|Stab(r_b)|) is the number of minimising group elements found
 // during canonicalisation.  Every output is written by exactly one lane: no
 // atomics, bitwise reproducible.
+//
+// Off-diagonal terms come in two forms (see QesPlan): exchange bonds, visited
+// only where the two bits differ via (x ^ rot(x, d)) masks, and generic term
+// groups evaluated one by one.
+#include <algorithm>
+#include <map>
+#include <vector>
+
 #include "qes_internal.h"
 
 namespace qes {
-
-template <bool CPLX> struct Acc;
-template <> struct Acc<false> {
-  double v = 0.0;
-};
-template <> struct Acc<true> {
-  double r = 0.0, i = 0.0;
-};
 
 __device__ __forceinline__ double sel4(const double* t, int k) {
   double v0 = t[0], v1 = t[1], v2 = t[2], v3 = t[3];
@@ -36,29 +36,145 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks, int r
   return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-// NB = number of right-hand sides handled per lane (columns j0 .. j0+NB-1 of
-// an (n, ldx) row-major block).
-template <typename W, bool CPLX, bool LOADNORM, int NB>
-__global__ void __launch_bounds__(256) matvec_kernel(QesGroup G, QesLookup L, const double* __restrict__ chi_re,
-                                                     const double* __restrict__ chi_im,
-                                                     const QesTermGroup* __restrict__ groups, int n_diag, int n_off,
-                                                     const double* __restrict__ norms, const double* __restrict__ x,
-                                                     double* __restrict__ y, int64_t n, int64_t ldx, int64_t j0,
-                                                     int ncols, int accumulate, int remap) {
-  __shared__ double s_chr[256];
-  __shared__ double s_chi[256];
+// Everything a lane needs to enumerate the off-diagonal elements of a row.
+struct RowCtx {
+  QesGroup G;
+  QesLookup L;
+  QesPlan P;
+  const QesTermGroup* groups;   // diag then generic off
+  int n_diag;
+  const double* s_chr;          // LDS
+  const double* s_chi;
+  const double* s_ex;           // LDS exchange values
+  const uint32_t* s_lut;        // LDS row LUT
+};
+
+template <typename W, int CK>
+__device__ __forceinline__ Canon<W> canon_dispatch(const RowCtx& C, W s) {
+  if (CK == QES_CANON_CYCLIC) {
+    if (sizeof(W) * 8 == 32 || sizeof(W) * 8 == 64) {
+      if (C.G.ns == (int)(sizeof(W) * 8)) return canonicalize_cyclic<W, true>(C.G, s);
+    }
+    return canonicalize_cyclic<W, false>(C.G, s);
+  } else if (CK == QES_CANON_ROWS) {
+    return canonicalize_rows<W>(C.G, C.s_lut, s);
+  }
+  return canonicalize<W>(C.G, s);
+}
+
+// Visit every valid off-diagonal element of row r: f(b, fr, fi, coupling)
+// with (fr, fi) = conj(val) * chi(g*) * N_b (the caller divides by N_a).
+template <typename W, bool CPLX, int CK, class F>
+__device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
+  const int ns = C.G.ns;
+  const W mask = (W)C.G.mask;
+  auto emit = [&](W sp, double vr, double vi, int cpl) {
+    const Canon<W> cn = canon_dispatch<W, CK>(C, sp);
+    const int64_t b = lookup_rep<W>(C.L, cn.rep);
+    if (b < 0) return;
+    const double nb = sqrt((double)cn.count);
+    const double cr = C.s_chr[cn.g];
+    double fr, fi;
+    if (CPLX) {
+      const double ci = C.s_chi[cn.g];
+      fr = (vr * cr + vi * ci) * nb;
+      fi = (vr * ci - vi * cr) * nb;
+    } else {
+      fr = vr * cr * nb;
+      fi = 0.0;
+    }
+    f(b, fr, fi, cpl);
+  };
+  // exchange classes
+  for (int c = 0; c < C.P.nclass; ++c) {
+    const int d = C.P.cls[c].d;
+    W m = (r ^ rotl_ns<W>(r, d, ns, mask)) & (W)C.P.cls[c].pmask;
+    while (m) {
+      const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
+      m &= m - 1;
+      const int b0 = (int)((r >> p) & 1);
+      int p1 = p - d;
+      if (p1 < 0) p1 += ns;
+      const double* e = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
+      const double vr = e[0];
+      const double vi = CPLX ? e[1] : 0.0;
+      const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
+      emit(sp, vr, vi, c * ns + p);
+    }
+  }
+  // generic groups
+  for (int t = C.n_diag; t < C.n_diag + C.P.n_gen; ++t) {
+    const QesTermGroup& g = C.groups[t];
+    const int k = term_bits<W>(g, r);
+    const double vr = sel4(g.re, k);
+    const double vi = CPLX ? sel4(g.im, k) : 0.0;
+    if (vr == 0.0 && vi == 0.0) continue;
+    const W sp = r ^ (W)g.mask;
+    if (C.G.n_up >= 0 && popc<W>(sp) != C.G.n_up) continue;
+    if (C.G.z2 != 0 && ((popc<W>(sp) & 1) ? -1 : 1) != C.G.z2) continue;
+    emit(sp, vr, vi, -1 - t);
+  }
+}
+
+// Load the per-block tables into LDS; returns the context.
+template <bool CPLX>
+__device__ __forceinline__ RowCtx load_ctx(const QesGroup& G, const QesLookup& L, const QesPlan& P,
+                                           const double* __restrict__ chi_re, const double* __restrict__ chi_im,
+                                           const QesTermGroup* __restrict__ groups, int n_diag,
+                                           const double* __restrict__ exval, const uint32_t* __restrict__ lut,
+                                           char* smem) {
   const int gs = G.ntx * G.nty * G.nbase;
-  for (int i = threadIdx.x; i < gs && i < 256; i += blockDim.x) {
+  double* s_chr = (double*)smem;
+  double* s_chi = s_chr + gs;
+  double* s_ex = s_chi + gs;
+  const int nex = P.nclass * G.ns * 4;
+  uint32_t* s_lut = (uint32_t*)(s_ex + nex);
+  const int nlut = (P.canon == QES_CANON_ROWS) ? (1 << P.lut_bits) : 0;
+  for (int i = threadIdx.x; i < gs; i += blockDim.x) {
     s_chr[i] = chi_re[i];
     s_chi[i] = CPLX ? chi_im[i] : 0.0;
   }
+  for (int i = threadIdx.x; i < nex; i += blockDim.x) s_ex[i] = exval[i];
+  for (int i = threadIdx.x; i < nlut; i += blockDim.x) s_lut[i] = lut[i];
   __syncthreads();
+  RowCtx C;
+  C.G = G;
+  C.L = L;
+  C.P = P;
+  C.groups = groups;
+  C.n_diag = n_diag;
+  C.s_chr = s_chr;
+  C.s_chi = s_chi;
+  C.s_ex = s_ex;
+  C.s_lut = s_lut;
+  return C;
+}
+
+template <typename W>
+__device__ __forceinline__ W load_rep(const QesLookup& L, int64_t a) {
+  if (sizeof(W) == 4 && L.keys32) return (W)L.keys32[a];
+  return (W)L.keys64[a];
+}
+
+// NB = number of right-hand sides per lane (columns j0 .. j0+NB-1 of an
+// (n, ldx) row-major block).
+template <typename W, bool CPLX, int CK, int NB>
+__global__ void __launch_bounds__(256) matvec_kernel(QesGroup G, QesLookup L, QesPlan P,
+                                                     const double* __restrict__ chi_re,
+                                                     const double* __restrict__ chi_im,
+                                                     const QesTermGroup* __restrict__ groups, int n_diag,
+                                                     const double* __restrict__ exval,
+                                                     const uint32_t* __restrict__ lut,
+                                                     const double* __restrict__ norms, const double* __restrict__ x,
+                                                     double* __restrict__ y, int64_t n, int64_t ldx, int64_t j0,
+                                                     int ncols, int accumulate, int remap) {
+  extern __shared__ char smem[];
+  const RowCtx C = load_ctx<CPLX>(G, L, P, chi_re, chi_im, groups, n_diag, exval, lut, smem);
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, remap);
   const int64_t a = blk * blockDim.x + threadIdx.x;
   if (a >= n) return;
-  const W r = (W)L.keys64[a];
-  const double na = norms[a];
-  const double inv_na = 1.0 / na;
+  const W r = load_rep<W>(L, a);
+  const double inv_na = 1.0 / norms[a];
   const int cw = CPLX ? 2 : 1;
 
   double acc_r[NB], acc_i[NB];
@@ -67,8 +183,7 @@ __global__ void __launch_bounds__(256) matvec_kernel(QesGroup G, QesLookup L, co
     acc_r[j] = 0.0;
     acc_i[j] = 0.0;
   }
-
-  // diagonal (sum of all mask-0 groups), H_aa
+  // diagonal
   {
     double dr = 0.0, di = 0.0;
     for (int t = 0; t < n_diag; ++t) {
@@ -91,31 +206,9 @@ __global__ void __launch_bounds__(256) matvec_kernel(QesGroup G, QesLookup L, co
       }
     }
   }
-
-  for (int t = n_diag; t < n_diag + n_off; ++t) {
-    const QesTermGroup& g = groups[t];
-    const int k = term_bits<W>(g, r);
-    const double vr = sel4(g.re, k);
-    const double vi = CPLX ? sel4(g.im, k) : 0.0;
-    if (vr == 0.0 && vi == 0.0) continue;
-    const W sp = r ^ (W)g.mask;
-    if (G.n_up >= 0 && popc<W>(sp) != G.n_up) continue;
-    if (G.z2 != 0 && ((popc<W>(sp) & 1) ? -1 : 1) != G.z2) continue;
-    const Canon<W> c = canonicalize<W>(G, sp);
-    const int64_t b = lookup_rep<W>(L, c.rep);
-    if (b < 0) continue;
-    const double nb = LOADNORM ? norms[b] : sqrt((double)c.count);
-    const double sc = nb * inv_na;
-    // factor = conj(v) * chi[g*] * N_b/N_a
-    const double cr = s_chr[c.g], ci = s_chi[c.g];
-    double fr, fi;
-    if (CPLX) {
-      fr = (vr * cr + vi * ci) * sc;
-      fi = (vr * ci - vi * cr) * sc;
-    } else {
-      fr = vr * cr * sc;
-      fi = 0.0;
-    }
+  visit_row<W, CPLX, CK>(C, r, [&](int64_t b, double fr, double fi, int) {
+    fr *= inv_na;
+    fi *= inv_na;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (j < ncols) {
@@ -129,8 +222,7 @@ __global__ void __launch_bounds__(256) matvec_kernel(QesGroup G, QesLookup L, co
         }
       }
     }
-  }
-
+  });
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     if (j < ncols) {
@@ -150,73 +242,24 @@ __global__ void __launch_bounds__(256) matvec_kernel(QesGroup G, QesLookup L, co
   }
 }
 
-template <typename W, bool CPLX, bool LOADNORM>
-static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatch, int accumulate, hipStream_t st) {
-  const Basis* B = op->basis;
-  const int64_t n = B->n;
-  if (n == 0) return 0;
-  const int block = 256;
-  const int64_t nblocks = (n + block - 1) / block;
-  if (nblocks > 0x7fffffffll) return fail("matvec grid too large");
-  for (int64_t j0 = 0; j0 < nbatch; j0 += 4) {
-    int ncols = (int)std::min<int64_t>(4, nbatch - j0);
-    if (ncols == 1)
-      hipLaunchKernelGGL((matvec_kernel<W, CPLX, LOADNORM, 1>), dim3((unsigned)nblocks), dim3(block), 0, st, B->group,
-                         B->lookup(), B->chi_re, B->chi_im, op->d_groups, op->n_diag, op->n_off, B->norms, x, y, n,
-                         nbatch, j0, ncols, accumulate, op->xcd_remap);
-    else
-      hipLaunchKernelGGL((matvec_kernel<W, CPLX, LOADNORM, 4>), dim3((unsigned)nblocks), dim3(block), 0, st, B->group,
-                         B->lookup(), B->chi_re, B->chi_im, op->d_groups, op->n_diag, op->n_off, B->norms, x, y, n,
-                         nbatch, j0, ncols, accumulate, op->xcd_remap);
-    QES_HIP(hipGetLastError());
-  }
-  return 0;
-}
-
-int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int accumulate, hipStream_t st) {
-  const Basis* B = op->basis;
-  const bool cplx = dtype == 1;
-  if (!cplx && (op->is_complex || op->chi_complex))
-    return fail("complex matrix elements (complex couplings or momentum sector) need complex128 vectors");
-  if (nbatch < 1) return fail("nbatch must be >= 1");
-  const bool w32 = B->group.ns <= 32;
-  const bool ln = B->group.has_phase != 0;
-  const double* xd = (const double*)x;
-  double* yd = (double*)y;
-#define QES_DISPATCH(W_, C_, L_) return launch_matvec<W_, C_, L_>(op, xd, yd, nbatch, accumulate, st)
-  if (w32) {
-    if (cplx) { if (ln) QES_DISPATCH(uint32_t, true, true); else QES_DISPATCH(uint32_t, true, false); }
-    else { if (ln) QES_DISPATCH(uint32_t, false, true); else QES_DISPATCH(uint32_t, false, false); }
-  } else {
-    if (cplx) { if (ln) QES_DISPATCH(uint64_t, true, true); else QES_DISPATCH(uint64_t, true, false); }
-    else { if (ln) QES_DISPATCH(uint64_t, false, true); else QES_DISPATCH(uint64_t, false, false); }
-  }
-#undef QES_DISPATCH
-}
-
-// ---------------------------------------------------------------------------
-// Per-coupling outputs: y[c, a] = sum over terms of coupling c (single RHS).
-// Used by the autograd backward (grad_J_c = Re <g, H_c v>).
-// ---------------------------------------------------------------------------
+// Per-coupling outputs y[c, a] (single RHS), c = coupling id of each term.
 #define QES_MAX_COUPLINGS 8
-template <typename W, bool CPLX, bool LOADNORM>
-__global__ void __launch_bounds__(256) matvec_couplings_kernel(QesGroup G, QesLookup L, const double* __restrict__ chi_re,
+template <typename W, bool CPLX, int CK>
+__global__ void __launch_bounds__(256) matvec_couplings_kernel(QesGroup G, QesLookup L, QesPlan P,
+                                                               const double* __restrict__ chi_re,
                                                                const double* __restrict__ chi_im,
                                                                const QesTermGroup* __restrict__ groups, int n_diag,
-                                                               int n_off, int ncpl, const double* __restrict__ norms,
+                                                               const double* __restrict__ exval,
+                                                               const int32_t* __restrict__ excpl,
+                                                               const uint32_t* __restrict__ lut, int ncpl,
+                                                               const double* __restrict__ norms,
                                                                const double* __restrict__ x, double* __restrict__ y,
                                                                int64_t n) {
-  __shared__ double s_chr[256];
-  __shared__ double s_chi[256];
-  const int gs = G.ntx * G.nty * G.nbase;
-  for (int i = threadIdx.x; i < gs && i < 256; i += blockDim.x) {
-    s_chr[i] = chi_re[i];
-    s_chi[i] = CPLX ? chi_im[i] : 0.0;
-  }
-  __syncthreads();
+  extern __shared__ char smem[];
+  const RowCtx C = load_ctx<CPLX>(G, L, P, chi_re, chi_im, groups, n_diag, exval, lut, smem);
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
-  const W r = (W)L.keys64[a];
+  const W r = load_rep<W>(L, a);
   const double inv_na = 1.0 / norms[a];
   const int cw = CPLX ? 2 : 1;
   double ar[QES_MAX_COUPLINGS], ai[QES_MAX_COUPLINGS];
@@ -238,32 +281,22 @@ __global__ void __launch_bounds__(256) matvec_couplings_kernel(QesGroup G, QesLo
         ai[c] += pi;
       }
   }
-  for (int t = n_diag; t < n_diag + n_off; ++t) {
-    const QesTermGroup& g = groups[t];
-    const int k = term_bits<W>(g, r);
-    const double vr = sel4(g.re, k);
-    const double vi = CPLX ? sel4(g.im, k) : 0.0;
-    if (vr == 0.0 && vi == 0.0) continue;
-    const W sp = r ^ (W)g.mask;
-    if (G.n_up >= 0 && popc<W>(sp) != G.n_up) continue;
-    if (G.z2 != 0 && ((popc<W>(sp) & 1) ? -1 : 1) != G.z2) continue;
-    const Canon<W> cn = canonicalize<W>(G, sp);
-    const int64_t b = lookup_rep<W>(L, cn.rep);
-    if (b < 0) continue;
-    const double nb = LOADNORM ? norms[b] : sqrt((double)cn.count);
-    const double sc = nb * inv_na;
-    const double cr = s_chr[cn.g], ci = s_chi[cn.g];
-    double fr = (vr * cr + vi * ci) * sc;
-    double fi = CPLX ? (vr * ci - vi * cr) * sc : 0.0;
+  const int ns = G.ns;
+  // tag >= 0: exchange entry (class * ns + bit position); tag < 0: generic group -1-tag
+  visit_row<W, CPLX, CK>(C, r, [&](int64_t b, double fr, double fi, int tag) {
+    const int cp = tag >= 0 ? excpl[tag] : groups[-1 - tag].coupling;
+    fr *= inv_na;
+    fi *= inv_na;
     const double xr = x[b * cw], xi = CPLX ? x[b * cw + 1] : 0.0;
-    double pr = fr * xr - fi * xi, pi = fr * xi + fi * xr;
+    const double pr = fr * xr - fi * xi, pi = fr * xi + fi * xr;
 #pragma unroll
     for (int c = 0; c < QES_MAX_COUPLINGS; ++c)
-      if (c == g.coupling) {
+      if (c == cp) {
         ar[c] += pr;
         ai[c] += pi;
       }
-  }
+  });
+  (void)ns;
   for (int c = 0; c < ncpl && c < QES_MAX_COUPLINGS; ++c) {
     double* yp = y + ((int64_t)c * n + a) * cw;
     yp[0] = ar[c];
@@ -271,54 +304,22 @@ __global__ void __launch_bounds__(256) matvec_couplings_kernel(QesGroup G, QesLo
   }
 }
 
-int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_t st) {
-  const Basis* B = op->basis;
-  const bool cplx = dtype == 1;
-  if (!cplx && (op->is_complex || op->chi_complex))
-    return fail("complex matrix elements need complex128 vectors");
-  if (op->n_coupling > QES_MAX_COUPLINGS) return fail("too many coupling groups (max 8)");
-  const int64_t n = B->n;
-  if (n == 0) return 0;
-  dim3 grid((unsigned)((n + 255) / 256));
-  const bool w32 = B->group.ns <= 32, ln = B->group.has_phase != 0;
-  const double* xd = (const double*)x;
-  double* yd = (double*)y;
-#define QES_LC(W_, C_, L_)                                                                                       \
-  hipLaunchKernelGGL((matvec_couplings_kernel<W_, C_, L_>), grid, dim3(256), 0, st, B->group, B->lookup(),        \
-                     B->chi_re, B->chi_im, op->d_groups, op->n_diag, op->n_off, op->n_coupling, B->norms, xd, yd, n)
-  if (w32) {
-    if (cplx) { if (ln) QES_LC(uint32_t, true, true); else QES_LC(uint32_t, true, false); }
-    else { if (ln) QES_LC(uint32_t, false, true); else QES_LC(uint32_t, false, false); }
-  } else {
-    if (cplx) { if (ln) QES_LC(uint64_t, true, true); else QES_LC(uint64_t, true, false); }
-    else { if (ln) QES_LC(uint64_t, false, true); else QES_LC(uint64_t, false, false); }
-  }
-#undef QES_LC
-  QES_HIP(hipGetLastError());
-  return 0;
-}
-
-// ---------------------------------------------------------------------------
-// E_off: number of off-diagonal gathers per matvec (stats pass)
-// ---------------------------------------------------------------------------
-template <typename W>
-__global__ void count_offdiag_kernel(QesGroup G, QesLookup L, const QesTermGroup* __restrict__ groups, int n_diag,
-                                     int n_off, int64_t n, unsigned long long* __restrict__ out) {
+template <typename W, int CK>
+__global__ void __launch_bounds__(256) count_offdiag_kernel(QesGroup G, QesLookup L, QesPlan P,
+                                                            const double* __restrict__ chi_re,
+                                                            const double* __restrict__ chi_im,
+                                                            const QesTermGroup* __restrict__ groups, int n_diag,
+                                                            const double* __restrict__ exval,
+                                                            const uint32_t* __restrict__ lut, int64_t n,
+                                                            unsigned long long* __restrict__ out) {
+  extern __shared__ char smem[];
+  const RowCtx C = load_ctx<true>(G, L, P, chi_re, chi_im, groups, n_diag, exval, lut, smem);
   __shared__ unsigned long long s_sum[256];
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long cnt = 0;
   if (a < n) {
-    const W r = (W)L.keys64[a];
-    for (int t = n_diag; t < n_diag + n_off; ++t) {
-      const QesTermGroup& g = groups[t];
-      const int k = term_bits<W>(g, r);
-      if (sel4(g.re, k) == 0.0 && sel4(g.im, k) == 0.0) continue;
-      const W sp = r ^ (W)g.mask;
-      if (G.n_up >= 0 && popc<W>(sp) != G.n_up) continue;
-      if (G.z2 != 0 && ((popc<W>(sp) & 1) ? -1 : 1) != G.z2) continue;
-      const Canon<W> c = canonicalize<W>(G, sp);
-      if (lookup_rep<W>(L, c.rep) >= 0) ++cnt;
-    }
+    const W r = load_rep<W>(L, a);
+    visit_row<W, true, CK>(C, r, [&](int64_t, double, double, int) { ++cnt; });
   }
   s_sum[threadIdx.x] = cnt;
   __syncthreads();
@@ -327,6 +328,248 @@ __global__ void count_offdiag_kernel(QesGroup G, QesLookup L, const QesTermGroup
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicAdd(out, s_sum[0]);
+}
+
+// Fast canonicalisers against the brute-force one on given states.
+template <typename W, int CK>
+__global__ void canon_check_kernel(QesGroup G, const uint32_t* __restrict__ lut, int lut_n,
+                                   const double* __restrict__ chi_re, const double* __restrict__ chi_im,
+                                   const uint64_t* __restrict__ states, int64_t n,
+                                   unsigned long long* __restrict__ bad) {
+  extern __shared__ char smem[];
+  uint32_t* s_lut = (uint32_t*)smem;
+  for (int i = threadIdx.x; i < lut_n; i += blockDim.x) s_lut[i] = lut[i];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const W s = (W)states[i];
+  const Canon<W> a = canonicalize<W>(G, s);
+  Canon<W> b;
+  if (CK == QES_CANON_CYCLIC) {
+    b = (G.ns == (int)(sizeof(W) * 8)) ? canonicalize_cyclic<W, true>(G, s) : canonicalize_cyclic<W, false>(G, s);
+  } else {
+    b = canonicalize_rows<W>(G, s_lut, s);
+  }
+  // same representative and stabiliser size; same character of the reported
+  // element whenever the state belongs to the sector (outside it the
+  // character varies over the minimisers and the lookup rejects the state)
+  double pr = 0.0, pim = 0.0;
+  const bool in_sector = rep_test<W>(G, chi_re, chi_im, a.rep, &pr, &pim) && hypot(pr, pim) > 1e-7;
+  // (characters of distinct elements that are equal in exact arithmetic,
+  // e.g. exp(i*pi*n) for even n, differ in the last bits of the FP64 table)
+  const bool ok = a.rep == b.rep && a.count == b.count &&
+                  (!in_sector || (fabs(chi_re[a.g] - chi_re[b.g]) <= 1e-12 && fabs(chi_im[a.g] - chi_im[b.g]) <= 1e-12));
+  if (!ok) atomicAdd(bad, 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// host: plan construction and launches
+// ---------------------------------------------------------------------------
+static int pick_canon(const QesGroup& G) {
+  const bool cyclic = G.nty == 1 && G.row_bits == G.ns && G.cell_bits == 1 && G.ntx == G.ns && G.ntx > 1;
+  if (cyclic) return QES_CANON_CYCLIC;
+  if ((G.ntx > 1 || G.nty > 1) && G.row_bits <= 12 && G.ns % G.row_bits == 0) return QES_CANON_ROWS;
+  return QES_CANON_BRUTE;
+}
+
+static std::vector<uint32_t> row_lut(const QesGroup& G) {
+  const int Wr = G.row_bits;
+  const uint32_t nr = 1u << Wr;
+  const uint32_t rm = nr - 1;
+  std::vector<uint32_t> lut(nr);
+  for (uint32_t v = 0; v < nr; ++v) {
+    uint32_t best = 0xffffffffu, am = 0;
+    for (int a = 0; a < G.ntx; ++a) {
+      const int r = a * G.cell_bits;
+      const uint32_t t = r ? (((v << r) | (v >> (Wr - r))) & rm) : v;
+      if (t < best) {
+        best = t;
+        am = 1u << a;
+      } else if (t == best) {
+        am |= 1u << a;
+      }
+    }
+    lut[v] = best | (am << 16);
+  }
+  return lut;
+}
+
+void free_plan(Op* op) {
+  if (op->d_exval) hipFree(op->d_exval);
+  if (op->d_excpl) hipFree(op->d_excpl);
+  if (op->d_lut) hipFree(op->d_lut);
+  if (op->d_gen) hipFree(op->d_gen);
+  op->d_exval = nullptr;
+  op->d_excpl = nullptr;
+  op->d_lut = nullptr;
+  op->d_gen = nullptr;
+}
+
+int build_plan(Op* op, int canon_override) {
+  free_plan(op);
+  const Basis* B = op->basis;
+  const QesGroup& G = B->group;
+  const int ns = G.ns;
+  QesPlan P{};
+  P.canon = canon_override >= 0 ? canon_override : pick_canon(G);
+  if (P.canon == QES_CANON_CYCLIC &&
+      !(G.nty == 1 && G.row_bits == ns && G.cell_bits == 1 && G.ntx == ns && G.ntx > 1))
+    return fail("cyclic canonicaliser needs a chain translation group");
+  if (P.canon == QES_CANON_ROWS && !(G.row_bits <= 12 && G.ns % G.row_bits == 0 && G.ntx <= 16))
+    return fail("row canonicaliser needs rows of at most 12 bits");
+  // classify off-diagonal groups
+  std::vector<QesTermGroup> gen;
+  std::map<int, int> cls_of;   // d -> class
+  std::vector<std::vector<double>> exv;
+  std::vector<std::vector<int32_t>> exc;
+  std::vector<uint64_t> pm;
+  std::vector<int> ds;
+  for (int i = op->n_diag; i < (int)op->h_groups.size(); ++i) {
+    const QesTermGroup& g = op->h_groups[i];
+    const bool ex = g.p0 >= 0 && g.p1 >= 0 && g.p0 != g.p1 &&
+                    g.mask == ((1ull << g.p0) | (1ull << g.p1)) && g.re[0] == 0.0 && g.im[0] == 0.0 &&
+                    g.re[3] == 0.0 && g.im[3] == 0.0;
+    int c = -1;
+    if (ex) {
+      const int d = ((g.p0 - g.p1) % ns + ns) % ns;
+      auto it = cls_of.find(d);
+      if (it != cls_of.end()) {
+        c = it->second;
+        if ((pm[c] >> g.p0) & 1ull) c = -2;   // same (d, p0) twice: keep generic
+      } else if ((int)ds.size() < QES_MAX_CLASSES) {
+        c = (int)ds.size();
+        cls_of[d] = c;
+        ds.push_back(d);
+        pm.push_back(0);
+        exv.emplace_back(ns * 4, 0.0);
+        exc.emplace_back(ns, 0);
+      }
+    }
+    if (c >= 0) {
+      pm[c] |= 1ull << g.p0;
+      // bit(p0) = 0 -> table entry 1 (b0=0,b1=1); bit(p0) = 1 -> entry 2
+      exv[c][(g.p0 * 2 + 0) * 2 + 0] = g.re[1];
+      exv[c][(g.p0 * 2 + 0) * 2 + 1] = g.im[1];
+      exv[c][(g.p0 * 2 + 1) * 2 + 0] = g.re[2];
+      exv[c][(g.p0 * 2 + 1) * 2 + 1] = g.im[2];
+      exc[c][g.p0] = g.coupling;
+    } else {
+      gen.push_back(g);
+    }
+  }
+  P.nclass = (int)ds.size();
+  for (int c = 0; c < P.nclass; ++c) {
+    P.cls[c].d = ds[c];
+    P.cls[c].pmask = pm[c];
+  }
+  P.n_gen = (int)gen.size();
+  std::vector<QesTermGroup> dg(op->h_groups.begin(), op->h_groups.begin() + op->n_diag);
+  dg.insert(dg.end(), gen.begin(), gen.end());
+  QES_HIP(hipMalloc(&op->d_gen, std::max<size_t>(dg.size(), 1) * sizeof(QesTermGroup)));
+  if (!dg.empty()) QES_HIP(hipMemcpy(op->d_gen, dg.data(), dg.size() * sizeof(QesTermGroup), hipMemcpyHostToDevice));
+  std::vector<double> exflat;
+  std::vector<int32_t> cpflat;
+  for (int c = 0; c < P.nclass; ++c) {
+    exflat.insert(exflat.end(), exv[c].begin(), exv[c].end());
+    cpflat.insert(cpflat.end(), exc[c].begin(), exc[c].end());
+  }
+  QES_HIP(hipMalloc(&op->d_exval, std::max<size_t>(exflat.size(), 1) * sizeof(double)));
+  QES_HIP(hipMalloc(&op->d_excpl, std::max<size_t>(cpflat.size(), 1) * sizeof(int32_t)));
+  if (!exflat.empty()) {
+    QES_HIP(hipMemcpy(op->d_exval, exflat.data(), exflat.size() * sizeof(double), hipMemcpyHostToDevice));
+    QES_HIP(hipMemcpy(op->d_excpl, cpflat.data(), cpflat.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  int nlut = 0;
+  if (P.canon == QES_CANON_ROWS) {
+    std::vector<uint32_t> lut = row_lut(G);
+    P.lut_bits = G.row_bits;
+    nlut = (int)lut.size();
+    QES_HIP(hipMalloc(&op->d_lut, lut.size() * sizeof(uint32_t)));
+    QES_HIP(hipMemcpy(op->d_lut, lut.data(), lut.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  } else {
+    QES_HIP(hipMalloc(&op->d_lut, sizeof(uint32_t)));
+  }
+  op->plan = P;
+  op->lds_bytes = (size_t)B->gsize * 2 * sizeof(double) + exflat.size() * sizeof(double) + nlut * sizeof(uint32_t);
+  return 0;
+}
+
+#define QES_CK_SWITCH(CK_VAR, MACRO)                       \
+  switch (CK_VAR) {                                        \
+    case QES_CANON_CYCLIC: MACRO(QES_CANON_CYCLIC); break; \
+    case QES_CANON_ROWS: MACRO(QES_CANON_ROWS); break;     \
+    default: MACRO(QES_CANON_BRUTE); break;                \
+  }
+
+template <typename W, bool CPLX>
+static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatch, int accumulate, hipStream_t st) {
+  const Basis* B = op->basis;
+  const int64_t n = B->n;
+  if (n == 0) return 0;
+  const int block = 256;
+  const int64_t nblocks = (n + block - 1) / block;
+  if (nblocks > 0x7fffffffll) return fail("matvec grid too large");
+  const QesPlan& P = op->plan;
+  for (int64_t j0 = 0; j0 < nbatch; j0 += 4) {
+    const int ncols = (int)std::min<int64_t>(4, nbatch - j0);
+#define QES_LAUNCH(CK)                                                                                           \
+  if (ncols == 1)                                                                                                \
+    hipLaunchKernelGGL((matvec_kernel<W, CPLX, CK, 1>), dim3((unsigned)nblocks), dim3(block), op->lds_bytes, st, \
+                       B->group, B->lookup(), P, B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval,        \
+                       op->d_lut, B->norms, x, y, n, nbatch, j0, ncols, accumulate, op->xcd_remap);              \
+  else                                                                                                           \
+    hipLaunchKernelGGL((matvec_kernel<W, CPLX, CK, 4>), dim3((unsigned)nblocks), dim3(block), op->lds_bytes, st, \
+                       B->group, B->lookup(), P, B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval,        \
+                       op->d_lut, B->norms, x, y, n, nbatch, j0, ncols, accumulate, op->xcd_remap)
+    QES_CK_SWITCH(P.canon, QES_LAUNCH)
+#undef QES_LAUNCH
+    QES_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int accumulate, hipStream_t st) {
+  const Basis* B = op->basis;
+  const bool cplx = dtype == 1;
+  if (!cplx && (op->is_complex || op->chi_complex))
+    return fail("complex matrix elements (complex couplings or momentum sector) need complex128 vectors");
+  if (nbatch < 1) return fail("nbatch must be >= 1");
+  const bool w32 = B->group.ns <= 32;
+  const double* xd = (const double*)x;
+  double* yd = (double*)y;
+  if (w32) return cplx ? launch_matvec<uint32_t, true>(op, xd, yd, nbatch, accumulate, st)
+                       : launch_matvec<uint32_t, false>(op, xd, yd, nbatch, accumulate, st);
+  return cplx ? launch_matvec<uint64_t, true>(op, xd, yd, nbatch, accumulate, st)
+              : launch_matvec<uint64_t, false>(op, xd, yd, nbatch, accumulate, st);
+}
+
+int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_t st) {
+  const Basis* B = op->basis;
+  const bool cplx = dtype == 1;
+  if (!cplx && (op->is_complex || op->chi_complex)) return fail("complex matrix elements need complex128 vectors");
+  if (op->n_coupling > QES_MAX_COUPLINGS) return fail("too many coupling groups (max 8)");
+  const int64_t n = B->n;
+  if (n == 0) return 0;
+  dim3 grid((unsigned)((n + 255) / 256));
+  const double* xd = (const double*)x;
+  double* yd = (double*)y;
+  const QesPlan& P = op->plan;
+#define QES_LC(W_, C_, CK)                                                                                          \
+  hipLaunchKernelGGL((matvec_couplings_kernel<W_, C_, CK>), grid, dim3(256), op->lds_bytes, st, B->group,          \
+                     B->lookup(), P, B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_excpl,         \
+                     op->d_lut, op->n_coupling, B->norms, xd, yd, n)
+#define QES_LC32R(CK) QES_LC(uint32_t, false, CK)
+#define QES_LC32C(CK) QES_LC(uint32_t, true, CK)
+#define QES_LC64R(CK) QES_LC(uint64_t, false, CK)
+#define QES_LC64C(CK) QES_LC(uint64_t, true, CK)
+  if (B->group.ns <= 32) {
+    if (cplx) { QES_CK_SWITCH(P.canon, QES_LC32C) } else { QES_CK_SWITCH(P.canon, QES_LC32R) }
+  } else {
+    if (cplx) { QES_CK_SWITCH(P.canon, QES_LC64C) } else { QES_CK_SWITCH(P.canon, QES_LC64R) }
+  }
+#undef QES_LC
+  QES_HIP(hipGetLastError());
+  return 0;
 }
 
 int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st) {
@@ -338,18 +581,76 @@ int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st) {
   QES_HIP(hipMalloc(&d, sizeof(unsigned long long)));
   QES_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
   dim3 grid((unsigned)((n + 255) / 256));
-  if (B->group.ns <= 32)
-    hipLaunchKernelGGL(count_offdiag_kernel<uint32_t>, grid, dim3(256), 0, st, B->group, B->lookup(), op->d_groups,
-                       op->n_diag, op->n_off, n, d);
-  else
-    hipLaunchKernelGGL(count_offdiag_kernel<uint64_t>, grid, dim3(256), 0, st, B->group, B->lookup(), op->d_groups,
-                       op->n_diag, op->n_off, n, d);
+  const QesPlan& P = op->plan;
+#define QES_CO(W_, CK)                                                                                              \
+  hipLaunchKernelGGL((count_offdiag_kernel<W_, CK>), grid, dim3(256), op->lds_bytes, st, B->group, B->lookup(), P, \
+                     B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_lut, n, d)
+#define QES_CO32(CK) QES_CO(uint32_t, CK)
+#define QES_CO64(CK) QES_CO(uint64_t, CK)
+  if (B->group.ns <= 32) {
+    QES_CK_SWITCH(P.canon, QES_CO32)
+  } else {
+    QES_CK_SWITCH(P.canon, QES_CO64)
+  }
+#undef QES_CO
   QES_HIP(hipGetLastError());
   unsigned long long h = 0;
   QES_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, st));
   QES_HIP(hipStreamSynchronize(st));
   QES_HIP(hipFree(d));
   *e_off = (int64_t)h;
+  return 0;
+}
+
+int canon_check(const Basis* B, int kind, const uint32_t* d_lut_in, const uint64_t* d_states, int64_t n,
+                int64_t* bad_out, hipStream_t st) {
+  const QesGroup& G = B->group;
+  *bad_out = 0;
+  if (n <= 0) return 0;
+  std::vector<uint32_t> lut;
+  uint32_t* d_lut = nullptr;
+  int nlut = 0;
+  if (kind == QES_CANON_ROWS) {
+    if (!(G.row_bits <= 12 && G.ns % G.row_bits == 0 && G.ntx <= 16)) return fail("rows canonicaliser not applicable");
+    lut = row_lut(G);
+    nlut = (int)lut.size();
+    QES_HIP(hipMalloc(&d_lut, lut.size() * 4));
+    QES_HIP(hipMemcpy(d_lut, lut.data(), lut.size() * 4, hipMemcpyHostToDevice));
+  } else if (kind == QES_CANON_CYCLIC) {
+    if (!(G.nty == 1 && G.row_bits == G.ns && G.cell_bits == 1 && G.ntx == G.ns && G.ntx > 1))
+      return fail("cyclic canonicaliser not applicable");
+    QES_HIP(hipMalloc(&d_lut, 4));
+  } else {
+    return fail("unknown canonicaliser");
+  }
+  (void)d_lut_in;
+  unsigned long long* d_bad = nullptr;
+  QES_HIP(hipMalloc(&d_bad, sizeof(unsigned long long)));
+  QES_HIP(hipMemsetAsync(d_bad, 0, sizeof(unsigned long long), st));
+  dim3 grid((unsigned)((n + 255) / 256));
+  const size_t lds = nlut * 4;
+  if (G.ns <= 32) {
+    if (kind == QES_CANON_CYCLIC)
+      hipLaunchKernelGGL((canon_check_kernel<uint32_t, QES_CANON_CYCLIC>), grid, dim3(256), lds, st, G, d_lut, nlut,
+                         B->chi_re, B->chi_im, d_states, n, d_bad);
+    else
+      hipLaunchKernelGGL((canon_check_kernel<uint32_t, QES_CANON_ROWS>), grid, dim3(256), lds, st, G, d_lut, nlut,
+                         B->chi_re, B->chi_im, d_states, n, d_bad);
+  } else {
+    if (kind == QES_CANON_CYCLIC)
+      hipLaunchKernelGGL((canon_check_kernel<uint64_t, QES_CANON_CYCLIC>), grid, dim3(256), lds, st, G, d_lut, nlut,
+                         B->chi_re, B->chi_im, d_states, n, d_bad);
+    else
+      hipLaunchKernelGGL((canon_check_kernel<uint64_t, QES_CANON_ROWS>), grid, dim3(256), lds, st, G, d_lut, nlut,
+                         B->chi_re, B->chi_im, d_states, n, d_bad);
+  }
+  QES_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  QES_HIP(hipMemcpyAsync(&h, d_bad, sizeof(h), hipMemcpyDeviceToHost, st));
+  QES_HIP(hipStreamSynchronize(st));
+  hipFree(d_bad);
+  hipFree(d_lut);
+  *bad_out = (int64_t)h;
   return 0;
 }
 

@@ -112,6 +112,16 @@ class HilbertSpace:
         pidx = np.where(idx >= 0, self.group.g_to_pidx[np.maximum(g, 0)], 255).astype(np.uint8)
         return idx, pidx
 
+    def canon_check(self, states, kind: str) -> int:
+        """Number of states where the fast canonicaliser ``kind`` ('cyclic' or
+        'rows') disagrees with the brute-force one."""
+        st = np.ascontiguousarray(np.asarray(states, dtype=np.uint64))
+        bad = C.c_int64()
+        k = {"cyclic": 1, "rows": 2}[kind]
+        _lib.check(_lib.lib().qes_basis_canon_check(self._h, k, st.ctypes.data_as(_lib._PU64), st.size,
+                                                    C.byref(bad)), "qes_basis_canon_check")
+        return int(bad.value)
+
     def device_arrays(self):
         """Device pointers (reps uint64, norms float64) owned by this space."""
         r = C.c_void_p()

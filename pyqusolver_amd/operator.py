@@ -193,6 +193,20 @@ class Hamiltonian:
         _lib.check(_lib.lib().qes_op_n_couplings(self._device_op(), C.byref(v)))
         return int(v.value)
 
+    CANON_KINDS = {"auto": -1, "brute": 0, "cyclic": 1, "rows": 2}
+
+    def set_canonicalizer(self, kind: str = "auto"):
+        """Select the canonicalisation kernel (tuning / cross-checks)."""
+        _lib.check(_lib.lib().qes_op_set_canonicalizer(self._device_op(), self.CANON_KINDS[kind]),
+                   "qes_op_set_canonicalizer")
+
+    @property
+    def plan(self) -> dict:
+        c, nc, ng = C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib().qes_op_get_plan(self._device_op(), C.byref(c), C.byref(nc), C.byref(ng)))
+        names = {v: k for k, v in self.CANON_KINDS.items()}
+        return {"canonicalizer": names[c.value], "exchange_classes": nc.value, "generic_groups": ng.value}
+
     def count_offdiag(self) -> int:
         v = C.c_int64()
         _lib.check(_lib.lib().qes_op_count_offdiag(self._device_op(), C.byref(v)), "qes_op_count_offdiag")
