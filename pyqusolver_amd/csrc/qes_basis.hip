@@ -80,6 +80,9 @@ __global__ void __launch_bounds__(256) basis_scan_kernel(QesGroup G, const doubl
                                                          const int64_t* __restrict__ block_offsets,
                                                          uint64_t* __restrict__ reps, double* __restrict__ norms) {
   __shared__ uint32_t s_scan[256];
+  __shared__ uint64_t s_rowlo[QES_MAX_TX];
+  const GV gv = make_gv(G, s_rowlo);
+  __syncthreads();
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t begin = t * per;
   uint64_t end = begin + per;
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(256) basis_scan_kernel(QesGroup G, const doubl
     for (uint64_t r = begin; r < end; ++r) {
       if (passes_globals(G, s)) {
         double pre, pim;
-        if (rep_test<W>(G, chi_re, chi_im, (W)s, &pre, &pim)) {
+        if (rep_test<W>(gv, chi_re, chi_im, (W)s, &pre, &pim)) {
           if (hypot(pre, pim) > 1e-7) ++cnt;
         }
       }
@@ -117,7 +120,7 @@ __global__ void __launch_bounds__(256) basis_scan_kernel(QesGroup G, const doubl
     for (uint64_t r = begin; r < end; ++r) {
       if (passes_globals(G, s)) {
         double pre, pim;
-        if (rep_test<W>(G, chi_re, chi_im, (W)s, &pre, &pim)) {
+        if (rep_test<W>(gv, chi_re, chi_im, (W)s, &pre, &pim)) {
           double a = hypot(pre, pim);
           if (a > 1e-7) {
             reps[out] = s;
@@ -134,7 +137,7 @@ __global__ void __launch_bounds__(256) basis_scan_kernel(QesGroup G, const doubl
     for (uint64_t r = begin; r < end; ++r) {
       if (passes_globals(G, s)) {
         double pre, pim;
-        if (rep_test<W>(G, chi_re, chi_im, (W)s, &pre, &pim)) {
+        if (rep_test<W>(gv, chi_re, chi_im, (W)s, &pre, &pim)) {
           if (hypot(pre, pim) > 1e-7) ++cnt;
         }
       }
@@ -259,6 +262,9 @@ int build_basis(Basis* B, hipStream_t st) {
 template <typename W>
 __global__ void map_states_kernel(QesGroup G, QesLookup L, const uint64_t* __restrict__ states, int64_t n,
                                   int64_t* __restrict__ idx, int32_t* __restrict__ gmin, int32_t* __restrict__ cnt) {
+  __shared__ uint64_t s_rowlo[QES_MAX_TX];
+  const GV gv = make_gv(G, s_rowlo);
+  __syncthreads();
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint64_t s = states[i];
@@ -266,7 +272,7 @@ __global__ void map_states_kernel(QesGroup G, QesLookup L, const uint64_t* __res
   int32_t g = -1, c = 0;
   bool ok = (s & ~G.mask) == 0 && passes_globals(G, s) && (G.n_up < 0 || __popcll(s) == G.n_up);
   if (ok) {
-    Canon<W> cn = canonicalize<W>(G, (W)s);
+    Canon<W> cn = canonicalize<W>(gv, (W)s);
     id = lookup_rep<W>(L, cn.rep);
     g = cn.g;
     c = cn.count;

@@ -41,6 +41,43 @@ struct QesGroup {
   uint64_t rowlo[QES_MAX_TX];         // per x-power a: bits of each row below a*cell_bits
 };
 
+// Register-sized view of a QesGroup used inside kernels.  Kernel arguments
+// indexed with run-time indices would be copied to private (scratch) memory,
+// so the per-x-power row masks live in LDS and the base ops are packed into
+// one 32-bit word (2 bits per op).
+struct GV {
+  int ns, nbase, ntx, nty, row_bits, cell_bits, has_phase, n_up, z2;
+  uint32_t bops;
+  uint64_t mask;
+  const uint64_t* rowlo;   // LDS copy, ntx entries
+  __device__ __forceinline__ int bop(int B, int k) const { return (int)((bops >> ((B * 4 + k) * 2)) & 3u); }
+};
+
+// Stage the row masks in LDS (s_rowlo must hold QES_MAX_TX entries); every
+// thread of the block must call this before __syncthreads().
+__device__ __forceinline__ GV make_gv(const QesGroup& G, uint64_t* s_rowlo) {
+  for (int i = threadIdx.x; i < G.ntx; i += blockDim.x) s_rowlo[i] = G.rowlo[i];
+  GV v;
+  v.ns = G.ns;
+  v.nbase = G.nbase;
+  v.ntx = G.ntx;
+  v.nty = G.nty;
+  v.row_bits = G.row_bits;
+  v.cell_bits = G.cell_bits;
+  v.has_phase = G.has_phase;
+  v.n_up = G.n_up;
+  v.z2 = G.z2;
+  v.mask = G.mask;
+  uint32_t b = 0;
+#pragma unroll
+  for (int B = 0; B < QES_MAX_BASE; ++B)
+#pragma unroll
+    for (int k = 0; k < QES_MAX_BASE_OPS; ++k) b |= ((uint32_t)G.base_ops[B][k] & 3u) << ((B * 4 + k) * 2);
+  v.bops = b;
+  v.rowlo = s_rowlo;
+  return v;
+}
+
 // --------------------------------------------------------------------------
 // word helpers
 // --------------------------------------------------------------------------
@@ -71,13 +108,13 @@ __device__ __forceinline__ W rotl_rows(W v, int r, int row_bits, W lo, W mask) {
 // apply base element B; returns new state and multiplies *ph by the intrinsic
 // phase (only parity-y contributes, +i if the number of down spins is even)
 template <typename W>
-__device__ __forceinline__ W apply_base(const QesGroup& G, int B, W s, int* ph_quarter) {
+__device__ __forceinline__ W apply_base(const GV& G, int B, W s, int* ph_quarter) {
   W v = s;
   const int ns = G.ns;
   const W mask = (W)G.mask;
 #pragma unroll
   for (int k = 0; k < QES_MAX_BASE_OPS; ++k) {
-    int op = G.base_ops[B][k];
+    int op = G.bop(B, k);
     if (op == QES_BOP_REV) {
       v = brev<W>(v) >> (int)(8 * sizeof(W) - ns);
     } else if (op == QES_BOP_FLIP) {
@@ -105,7 +142,7 @@ struct Canon {
 };
 
 template <typename W>
-__device__ __forceinline__ Canon<W> canonicalize(const QesGroup& G, W s) {
+__device__ __forceinline__ Canon<W> canonicalize(const GV& G, W s) {
   const int ns = G.ns;
   const W mask = (W)G.mask;
   W vb[QES_MAX_BASE];
@@ -152,7 +189,7 @@ __device__ __forceinline__ Canon<W> canonicalize(const QesGroup& G, W s) {
 // |sum| through *proj_abs2 components.
 // --------------------------------------------------------------------------
 template <typename W>
-__device__ __forceinline__ bool rep_test(const QesGroup& G, const double* __restrict__ chi_re,
+__device__ __forceinline__ bool rep_test(const GV& G, const double* __restrict__ chi_re,
                                          const double* __restrict__ chi_im, W s, double* pre,
                                          double* pim) {
   const int ns = G.ns;
@@ -339,7 +376,7 @@ __device__ __forceinline__ int max_zero_run(W v, int ns, W mask, W* starts) {
 }
 
 template <typename W, bool FULL>
-__device__ __forceinline__ Canon<W> canonicalize_cyclic(const QesGroup& G, W s) {
+__device__ __forceinline__ Canon<W> canonicalize_cyclic(const GV& G, W s) {
   const int ns = G.ns;
   const W mask = (W)G.mask;
   W vb[QES_MAX_BASE], st[QES_MAX_BASE];
@@ -390,7 +427,7 @@ __device__ __forceinline__ Canon<W> canonicalize_cyclic(const QesGroup& G, W s) 
 // rotation of the row).  Only candidates whose top row attains it are
 // evaluated in full.
 template <typename W>
-__device__ __forceinline__ Canon<W> canonicalize_rows(const QesGroup& G, const uint32_t* __restrict__ lut, W s) {
+__device__ __forceinline__ Canon<W> canonicalize_rows(const GV& G, const uint32_t* __restrict__ lut, W s) {
   const int ns = G.ns;
   const W mask = (W)G.mask;
   const int Wr = G.row_bits;

@@ -37,10 +37,13 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks, int r
 }
 
 // Everything a lane needs to enumerate the off-diagonal elements of a row.
+// All run-time-indexed tables live in LDS (see make_gv).
 struct RowCtx {
-  QesGroup G;
+  GV G;
   QesLookup L;
-  QesPlan P;
+  int nclass, n_gen;
+  const int32_t* s_cls_d;       // LDS, nclass
+  const uint64_t* s_cls_pm;     // LDS, nclass
   const QesTermGroup* groups;   // diag then generic off
   int n_diag;
   const double* s_chr;          // LDS
@@ -48,6 +51,9 @@ struct RowCtx {
   const double* s_ex;           // LDS exchange values
   const uint32_t* s_lut;        // LDS row LUT
 };
+
+// bytes of dynamic LDS used by load_ctx before the per-op tables
+#define QES_CTX_FIXED_LDS (QES_MAX_TX * 8 + QES_MAX_CLASSES * 8 + QES_MAX_CLASSES * 4 + 32)
 
 template <typename W, int CK>
 __device__ __forceinline__ Canon<W> canon_dispatch(const RowCtx& C, W s) {
@@ -86,9 +92,9 @@ __device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
     f(b, fr, fi, cpl);
   };
   // exchange classes
-  for (int c = 0; c < C.P.nclass; ++c) {
-    const int d = C.P.cls[c].d;
-    W m = (r ^ rotl_ns<W>(r, d, ns, mask)) & (W)C.P.cls[c].pmask;
+  for (int c = 0; c < C.nclass; ++c) {
+    const int d = C.s_cls_d[c];
+    W m = (r ^ rotl_ns<W>(r, d, ns, mask)) & (W)C.s_cls_pm[c];
     while (m) {
       const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
       m &= m - 1;
@@ -103,7 +109,7 @@ __device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
     }
   }
   // generic groups
-  for (int t = C.n_diag; t < C.n_diag + C.P.n_gen; ++t) {
+  for (int t = C.n_diag; t < C.n_diag + C.n_gen; ++t) {
     const QesTermGroup& g = C.groups[t];
     const int k = term_bits<W>(g, r);
     const double vr = sel4(g.re, k);
@@ -123,13 +129,26 @@ __device__ __forceinline__ RowCtx load_ctx(const QesGroup& G, const QesLookup& L
                                            const QesTermGroup* __restrict__ groups, int n_diag,
                                            const double* __restrict__ exval, const uint32_t* __restrict__ lut,
                                            char* smem) {
+  uint64_t* s_rowlo = (uint64_t*)smem;
+  uint64_t* s_cls_pm = s_rowlo + QES_MAX_TX;
+  int32_t* s_cls_d = (int32_t*)(s_cls_pm + QES_MAX_CLASSES);
+  double* s_chr = (double*)(smem + QES_CTX_FIXED_LDS);
   const int gs = G.ntx * G.nty * G.nbase;
-  double* s_chr = (double*)smem;
   double* s_chi = s_chr + gs;
   double* s_ex = s_chi + gs;
   const int nex = P.nclass * G.ns * 4;
   uint32_t* s_lut = (uint32_t*)(s_ex + nex);
   const int nlut = (P.canon == QES_CANON_ROWS) ? (1 << P.lut_bits) : 0;
+  RowCtx C;
+  C.G = make_gv(G, s_rowlo);
+  if (threadIdx.x < QES_MAX_CLASSES) {
+#pragma unroll
+    for (int c = 0; c < QES_MAX_CLASSES; ++c)
+      if ((int)threadIdx.x == c) {
+        s_cls_pm[c] = P.cls[c].pmask;
+        s_cls_d[c] = P.cls[c].d;
+      }
+  }
   for (int i = threadIdx.x; i < gs; i += blockDim.x) {
     s_chr[i] = chi_re[i];
     s_chi[i] = CPLX ? chi_im[i] : 0.0;
@@ -137,10 +156,11 @@ __device__ __forceinline__ RowCtx load_ctx(const QesGroup& G, const QesLookup& L
   for (int i = threadIdx.x; i < nex; i += blockDim.x) s_ex[i] = exval[i];
   for (int i = threadIdx.x; i < nlut; i += blockDim.x) s_lut[i] = lut[i];
   __syncthreads();
-  RowCtx C;
-  C.G = G;
   C.L = L;
-  C.P = P;
+  C.nclass = P.nclass;
+  C.n_gen = P.n_gen;
+  C.s_cls_d = s_cls_d;
+  C.s_cls_pm = s_cls_pm;
   C.groups = groups;
   C.n_diag = n_diag;
   C.s_chr = s_chr;
@@ -337,24 +357,26 @@ __global__ void canon_check_kernel(QesGroup G, const uint32_t* __restrict__ lut,
                                    const uint64_t* __restrict__ states, int64_t n,
                                    unsigned long long* __restrict__ bad) {
   extern __shared__ char smem[];
+  __shared__ uint64_t s_rowlo[QES_MAX_TX];
   uint32_t* s_lut = (uint32_t*)smem;
   for (int i = threadIdx.x; i < lut_n; i += blockDim.x) s_lut[i] = lut[i];
+  const GV gv = make_gv(G, s_rowlo);
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const W s = (W)states[i];
-  const Canon<W> a = canonicalize<W>(G, s);
+  const Canon<W> a = canonicalize<W>(gv, s);
   Canon<W> b;
   if (CK == QES_CANON_CYCLIC) {
-    b = (G.ns == (int)(sizeof(W) * 8)) ? canonicalize_cyclic<W, true>(G, s) : canonicalize_cyclic<W, false>(G, s);
+    b = (gv.ns == (int)(sizeof(W) * 8)) ? canonicalize_cyclic<W, true>(gv, s) : canonicalize_cyclic<W, false>(gv, s);
   } else {
-    b = canonicalize_rows<W>(G, s_lut, s);
+    b = canonicalize_rows<W>(gv, s_lut, s);
   }
   // same representative and stabiliser size; same character of the reported
   // element whenever the state belongs to the sector (outside it the
   // character varies over the minimisers and the lookup rejects the state)
   double pr = 0.0, pim = 0.0;
-  const bool in_sector = rep_test<W>(G, chi_re, chi_im, a.rep, &pr, &pim) && hypot(pr, pim) > 1e-7;
+  const bool in_sector = rep_test<W>(gv, chi_re, chi_im, a.rep, &pr, &pim) && hypot(pr, pim) > 1e-7;
   // (characters of distinct elements that are equal in exact arithmetic,
   // e.g. exp(i*pi*n) for even n, differ in the last bits of the FP64 table)
   const bool ok = a.rep == b.rep && a.count == b.count &&
@@ -490,7 +512,8 @@ int build_plan(Op* op, int canon_override) {
     QES_HIP(hipMalloc(&op->d_lut, sizeof(uint32_t)));
   }
   op->plan = P;
-  op->lds_bytes = (size_t)B->gsize * 2 * sizeof(double) + exflat.size() * sizeof(double) + nlut * sizeof(uint32_t);
+  op->lds_bytes = QES_CTX_FIXED_LDS + (size_t)B->gsize * 2 * sizeof(double) + exflat.size() * sizeof(double) +
+                  nlut * sizeof(uint32_t);
   return 0;
 }
 
