@@ -127,8 +127,14 @@ int qes_op_count_offdiag(qes_op_t op, int64_t* e_off);
  * from the brute-force one (representative, stabiliser size, character).
  */
 int qes_op_set_canonicalizer(qes_op_t op, int32_t kind);
+/* Kernel options: "bond_order_uniform" (0/1), "direct_lookup" (0/1), "xcd_remap" (0/1). */
+int qes_op_set_option(qes_op_t op, const char* name, int64_t value);
 int qes_op_get_plan(qes_op_t op, int32_t* canon, int32_t* n_classes, int32_t* n_generic);
 int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_host, int64_t n, int64_t* mismatches);
+
+/* Representative lookup used by the matvec: 0 sorted keys + bucket table,
+ * 1 open-addressing hash table (default when the index fits beside the key). */
+int qes_basis_set_lookup(qes_basis_t b, int32_t mode);
 
 /*
  * Device Lanczos (FP64).  Replaces the eigensolver that

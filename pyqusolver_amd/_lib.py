@@ -59,8 +59,10 @@ SIGNATURES = {
     "qes_op_n_couplings": [_P, _PI32],
     "qes_op_count_offdiag": [_P, _PI64],
     "qes_op_set_canonicalizer": [_P, _I32],
+    "qes_op_set_option": [_P, C.c_char_p, _I64],
     "qes_op_get_plan": [_P, _PI32, _PI32, _PI32],
     "qes_basis_canon_check": [_P, _I32, _PU64, _I64, _PI64],
+    "qes_basis_set_lookup": [_P, _I32],
     "qes_lanczos_create": [_P, _I32, _I32, _I32, C.POINTER(_P)],
     "qes_lanczos_start": [_P, _P, _P],
     "qes_lanczos_run": [_P, _I32, _I32, _I32],

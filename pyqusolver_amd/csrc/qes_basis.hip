@@ -176,6 +176,9 @@ __global__ void bucket_kernel(const uint64_t* __restrict__ reps, int64_t n, int 
 
 static int bitlen(uint64_t v) { return v ? 64 - __builtin_clzll(v) : 0; }
 
+__global__ void keyc_kernel(const uint64_t* __restrict__ reps, const double* __restrict__ norms, int64_t n,
+                            uint64_t* __restrict__ keyc);
+
 int build_basis(Basis* B, hipStream_t st) {
   if (ensure_binom()) return -1;
   const QesGroup& G = B->group;
@@ -256,6 +259,70 @@ int build_basis(Basis* B, hipStream_t st) {
                      B->e_min, B->nbucket, B->bucket);
   QES_HIP(hipGetLastError());
   QES_HIP(hipStreamSynchronize(st));
+  if (G.ns <= QES_KEY_BITS && !G.has_phase && n > 0) {
+    QES_HIP(hipMalloc(&B->keyc, n * sizeof(uint64_t)));
+    hipLaunchKernelGGL(keyc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B->reps, B->norms, n,
+                       B->keyc);
+    QES_HIP(hipGetLastError());
+    QES_HIP(hipStreamSynchronize(st));
+  }
+  B->use_hash = false;
+  return 0;
+}
+
+// packed key: representative | (|Stab| - 1) << 56 with |Stab| = N^2 (no
+// intrinsic phases: the normalisation is sqrt of the stabiliser size)
+__global__ void keyc_kernel(const uint64_t* __restrict__ reps, const double* __restrict__ norms, int64_t n,
+                            uint64_t* __restrict__ keyc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double nn = norms[i];
+  const uint64_t cnt = (uint64_t)llrint(nn * nn);
+  keyc[i] = reps[i] | ((cnt - 1ull) << QES_KEY_BITS);
+}
+
+__global__ void hash_fill_kernel(uint64_t* __restrict__ t, uint64_t nslots) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += (uint64_t)gridDim.x * blockDim.x)
+    t[i] = QES_HASH_EMPTY;
+}
+
+// Insert every representative (linear probing, atomic claim of empty slots).
+// Slot contents depend on insertion order only for colliding keys; every
+// lookup returns the same index regardless.
+__global__ void hash_insert_kernel(const uint64_t* __restrict__ reps, int64_t n, int ns, int hbits,
+                                   unsigned long long* __restrict__ t) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t key = reps[i];
+  const unsigned long long entry = key | ((unsigned long long)i << ns);
+  const uint64_t hmask = (1ull << hbits) - 1ull;
+  uint64_t slot = qes_hash_slot(key, hbits);
+  for (uint64_t probe = 0; probe <= hmask; ++probe) {   // bounded: table is never full
+    const unsigned long long prev = atomicCAS(&t[slot], (unsigned long long)QES_HASH_EMPTY, entry);
+    if (prev == (unsigned long long)QES_HASH_EMPTY) return;
+    slot = (slot + 1) & hmask;
+  }
+}
+
+// Hash table representative -> index, load factor <= 0.6, when the index
+// fits beside the key in 64 bits.
+int build_hash(Basis* B, hipStream_t st) {
+  const int64_t n = B->n;
+  const int ns = B->group.ns;
+  B->use_hash = false;
+  if (n == 0 || ns >= 64 || (ns > 0 && (uint64_t)n > (1ull << (64 - ns)) - 1ull)) return 0;
+  int hbits = 4;
+  while ((double)n > 0.6 * (double)(1ull << hbits)) ++hbits;
+  const uint64_t nslots = 1ull << hbits;
+  QES_HIP(hipMalloc(&B->hash, nslots * sizeof(uint64_t)));
+  B->hbits = hbits;
+  hipLaunchKernelGGL(hash_fill_kernel, dim3((unsigned)std::min<uint64_t>((nslots + 255) / 256, 65536)), dim3(256), 0,
+                     st, B->hash, nslots);
+  hipLaunchKernelGGL(hash_insert_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B->reps, n, ns, hbits,
+                     (unsigned long long*)B->hash);
+  QES_HIP(hipGetLastError());
+  QES_HIP(hipStreamSynchronize(st));
+  B->use_hash = true;
   return 0;
 }
 

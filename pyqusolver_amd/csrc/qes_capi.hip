@@ -304,6 +304,8 @@ int qes_basis_destroy(qes_basis_t h) {
   if (B.keys32) hipFree(B.keys32);
   if (B.norms) hipFree(B.norms);
   if (B.bucket) hipFree(B.bucket);
+  if (B.hash) hipFree(B.hash);
+  if (B.keyc) hipFree(B.keyc);
   delete h;
   return 0;
 }
@@ -374,11 +376,33 @@ int qes_op_set_canonicalizer(qes_op_t h, int32_t kind) {
   return build_plan(&h->op, kind);
 }
 
+int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
+  if (!h || !name) return fail("null argument");
+  Op& op = h->op;
+  const std::string n(name);
+  if (n == "bond_order_uniform") op.bond_order_uniform = value ? 1 : 0;
+  else if (n == "direct_lookup") op.direct_lookup = value ? 1 : 0;
+  else if (n == "xcd_remap") op.xcd_remap = value ? 1 : 0;
+  else return fail("unknown option " + n + " (bond_order_uniform, direct_lookup, xcd_remap)");
+  return build_plan(&op, op.plan.canon == pick_canon_public(op.basis->group) ? -1 : op.plan.canon);
+}
+
 int qes_op_get_plan(qes_op_t h, int32_t* canon, int32_t* n_classes, int32_t* n_generic) {
   if (!h) return fail("null operator");
   if (canon) *canon = h->op.plan.canon;
   if (n_classes) *n_classes = h->op.plan.nclass;
   if (n_generic) *n_generic = h->op.plan.n_gen;
+  return 0;
+}
+
+int qes_basis_set_lookup(qes_basis_t b, int32_t mode) {
+  if (!b) return fail("null basis");
+  if (mode == 1 && !b->b.hash) {
+    if (build_hash(&b->b, nullptr)) return -1;
+    if (!b->b.hash) return fail("no hash table for this basis (index does not fit beside the key)");
+  }
+  if (mode != 0 && mode != 1) return fail("mode must be 0 (sorted keys + bucket table) or 1 (hash table)");
+  b->b.use_hash = mode == 1;
   return 0;
 }
 

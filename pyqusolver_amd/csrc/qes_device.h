@@ -251,8 +251,46 @@ struct QesLookup {
   int64_t n;
   int32_t mbits;
   int32_t e_min, e_max;     // bit-length range covered by the table
+  int32_t ns;
+  // open-addressing hash table (used when non-null): entry = key | index << ns,
+  // empty slots are all ones; 2^hbits slots, linear probing
+  const uint64_t* hash;
+  int32_t hbits;
   int32_t pad;
+  // packed keys (ns <= 56): key | (|Stab(rep)| - 1) << 56, sorted by key;
+  // a hit on the packed array also yields the stabiliser size
+  const uint64_t* keyc;
 };
+
+#define QES_KEY_BITS 56
+#define QES_KEY_MASK ((1ull << QES_KEY_BITS) - 1ull)
+
+// index of key v in the packed sorted array (bucket table narrows the range);
+// returns -1 if absent, else the index and *count = |Stab|
+__device__ __forceinline__ int64_t lookup_packed(const QesLookup& L, uint64_t v, int* count);
+
+#define QES_HASH_EMPTY (~0ull)
+__device__ __host__ __forceinline__ uint64_t qes_hash_slot(uint64_t key, int hbits) {
+  return (key * 0x9E3779B97F4A7C15ull) >> (64 - hbits);
+}
+
+// first probe of a hash lookup (independent loads can be issued for several
+// keys before any result is inspected)
+__device__ __forceinline__ uint64_t hash_first(const QesLookup& L, uint64_t key, uint64_t* slot) {
+  *slot = qes_hash_slot(key, L.hbits);
+  return L.hash[*slot];
+}
+
+__device__ __forceinline__ int64_t hash_resolve(const QesLookup& L, uint64_t key, uint64_t slot, uint64_t e) {
+  const uint64_t kmask = (L.ns >= 64) ? ~0ull : ((1ull << L.ns) - 1ull);
+  const uint64_t hmask = (1ull << L.hbits) - 1ull;
+  while (e != QES_HASH_EMPTY) {
+    if ((e & kmask) == key) return (int64_t)(e >> L.ns);
+    slot = (slot + 1) & hmask;
+    e = L.hash[slot];
+  }
+  return -1;
+}
 
 __device__ __forceinline__ uint32_t bucket_of(uint64_t v, int mbits, int* e_out) {
   int e = 64 - __clzll(v);   // 0 for v == 0
@@ -266,8 +304,34 @@ __device__ __forceinline__ uint32_t bucket_of(uint64_t v, int mbits, int* e_out)
   return ((uint32_t)e << mbits) | (uint32_t)mant;
 }
 
+__device__ __forceinline__ int64_t lookup_packed(const QesLookup& L, uint64_t v, int* count) {
+  int e;
+  uint32_t k = bucket_of(v, L.mbits, &e);
+  if (e < L.e_min || e > L.e_max) return -1;
+  k -= (uint32_t)L.e_min << L.mbits;
+  uint32_t lo = L.bucket[k];
+  uint32_t hi = L.bucket[k + 1];   // exclusive
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if ((L.keyc[mid] & QES_KEY_MASK) < v) lo = mid + 1; else hi = mid;
+  }
+  if (lo < (uint64_t)L.n) {
+    const uint64_t e0 = L.keyc[lo];
+    if ((e0 & QES_KEY_MASK) == v) {
+      *count = (int)(e0 >> QES_KEY_BITS) + 1;
+      return (int64_t)lo;
+    }
+  }
+  return -1;
+}
+
 template <typename W>
 __device__ __forceinline__ int64_t lookup_rep(const QesLookup& L, W v) {
+  if (L.hash) {
+    uint64_t slot;
+    const uint64_t e0 = hash_first(L, (uint64_t)v, &slot);
+    return hash_resolve(L, (uint64_t)v, slot, e0);
+  }
   int e;
   uint32_t k = bucket_of((uint64_t)v, L.mbits, &e);
   if (e < L.e_min || e > L.e_max) return -1;
@@ -322,6 +386,9 @@ __device__ __forceinline__ int term_bits(const QesTermGroup& t, W s) {
 // sector (characters of a one-dimensional irrep are constant on the coset
 // of the stabiliser), i.e. whenever the lookup succeeds.
 // --------------------------------------------------------------------------
+#ifndef QES_BATCH
+#define QES_BATCH 4
+#endif
 #define QES_CANON_BRUTE 0
 #define QES_CANON_CYCLIC 1
 #define QES_CANON_ROWS 2
@@ -500,5 +567,7 @@ struct QesPlan {
   int32_t nclass;
   int32_t n_gen;     // generic off-diagonal groups (after the diagonal ones)
   int32_t lut_bits;  // ROWS: row LUT has 1 << lut_bits entries
+  int32_t uniform;   // 1: bonds in wavefront-uniform order, 0: each lane's active bonds
+  int32_t direct;    // 1: look flipped states up directly before canonicalising
   QesExClass cls[QES_MAX_CLASSES];
 };

@@ -43,9 +43,17 @@ struct Basis {
     L.mbits = mbits;
     L.e_min = e_min;
     L.e_max = e_max;
+    L.ns = group.ns;
+    L.hash = use_hash ? hash : nullptr;
+    L.hbits = hbits;
     L.pad = 0;
+    L.keyc = keyc;
     return L;
   }
+  uint64_t* keyc = nullptr;   // device packed keys (key | (|Stab|-1) << 56), ns <= 56
+  uint64_t* hash = nullptr;   // device hash table (representative -> index)
+  int hbits = 0;
+  bool use_hash = false;
 };
 
 // Compiled operator (Hamiltonian) bound to a basis.
@@ -64,11 +72,17 @@ struct Op {
   QesTermGroup* d_groups = nullptr;   // device: diag groups then off groups
   std::vector<QesTermGroup> h_groups;
   int block = 256;
-  int xcd_remap = 1;
+  // measured on MI355X (J1-J2 chain L=32 and 6x5 square): consecutive row
+  // blocks spread over all XCDs keep one compact row window hot in the
+  // shared MALL, faster than giving each XCD a contiguous slice
+  int xcd_remap = 0;
+  int bond_order_uniform = 0;
+  int direct_lookup = 0;
 };
 
 // launchers (qes_basis.hip)
 int build_basis(Basis* B, hipStream_t st);
+int build_hash(Basis* B, hipStream_t st);
 int map_states(const Basis* B, const uint64_t* d_states, int64_t n, int64_t* d_idx, int32_t* d_g,
                int32_t* d_cnt, hipStream_t st);
 
@@ -79,6 +93,7 @@ int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int 
 int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_t st);
 int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st);
 int build_plan(Op* op, int canon_override);   // -1: automatic
+int pick_canon_public(const QesGroup& G);
 void free_plan(Op* op);
 int canon_check(const Basis* B, int kind, const uint32_t* d_lut, const uint64_t* d_states, int64_t n,
                 int64_t* d_bad, hipStream_t st);

@@ -50,6 +50,8 @@ struct RowCtx {
   const double* s_chi;
   const double* s_ex;           // LDS exchange values
   const uint32_t* s_lut;        // LDS row LUT
+  int uniform;                  // bond order of the direct-lookup pass
+  int direct;                   // run the direct-lookup pass
 };
 
 // bytes of dynamic LDS used by load_ctx before the per-op tables
@@ -76,7 +78,8 @@ __device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
   const W mask = (W)C.G.mask;
   auto emit = [&](W sp, double vr, double vi, int cpl) {
     const Canon<W> cn = canon_dispatch<W, CK>(C, sp);
-    const int64_t b = lookup_rep<W>(C.L, cn.rep);
+    int cnt_unused = 0;
+    const int64_t b = C.L.keyc ? lookup_packed(C.L, (uint64_t)cn.rep, &cnt_unused) : lookup_rep<W>(C.L, cn.rep);
     if (b < 0) return;
     const double nb = sqrt((double)cn.count);
     const double cr = C.s_chr[cn.g];
@@ -91,21 +94,48 @@ __device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
     }
     f(b, fr, fi, cpl);
   };
-  // exchange classes
+  // Exchange classes.  Pass 1 walks the bonds in wave-uniform order (all
+  // lanes of a wavefront handle the same bond together, so their targets and
+  // key probes cluster in a few cache lines) and looks the flipped state up
+  // directly: most flipped states are representatives already, and a hit on
+  // the packed key array gives the stabiliser size (character of the
+  // identity = 1).  Pass 2 canonicalises only the flipped states that missed.
   for (int c = 0; c < C.nclass; ++c) {
     const int d = C.s_cls_d[c];
-    W m = (r ^ rotl_ns<W>(r, d, ns, mask)) & (W)C.s_cls_pm[c];
-    while (m) {
-      const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
-      m &= m - 1;
-      const int b0 = (int)((r >> p) & 1);
+    const W pm = (W)C.s_cls_pm[c];
+    const W act = (r ^ rotl_ns<W>(r, d, ns, mask)) & pm;
+    const bool direct = C.L.keyc && C.direct;
+    W fail = direct ? (W)0 : act;
+    if (direct) {
+      W m = C.uniform ? pm : act;   // uniform order, or each lane's active bonds
+      while (m) {
+        const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
+        m &= m - 1;
+        if (!((act >> p) & 1)) continue;
+        int p1 = p - d;
+        if (p1 < 0) p1 += ns;
+        const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
+        int cnt = 0;
+        const int64_t b = lookup_packed(C.L, (uint64_t)sp, &cnt);
+        if (b < 0) {
+          fail |= (W)1 << p;
+          continue;
+        }
+        const int b0 = (int)((r >> p) & 1);
+        const double* e = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
+        const double nb = sqrt((double)cnt);
+        f(b, e[0] * nb, CPLX ? -e[1] * nb : 0.0, c * ns + p);
+      }
+    }
+    while (fail) {
+      const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
+      fail &= fail - 1;
       int p1 = p - d;
       if (p1 < 0) p1 += ns;
-      const double* e = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
-      const double vr = e[0];
-      const double vi = CPLX ? e[1] : 0.0;
       const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
-      emit(sp, vr, vi, c * ns + p);
+      const int b0 = (int)((r >> p) & 1);
+      const double* e = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
+      emit(sp, e[0], CPLX ? e[1] : 0.0, c * ns + p);
     }
   }
   // generic groups
@@ -159,6 +189,8 @@ __device__ __forceinline__ RowCtx load_ctx(const QesGroup& G, const QesLookup& L
   C.L = L;
   C.nclass = P.nclass;
   C.n_gen = P.n_gen;
+  C.uniform = P.uniform;
+  C.direct = P.direct;
   C.s_cls_d = s_cls_d;
   C.s_cls_pm = s_cls_pm;
   C.groups = groups;
@@ -387,6 +419,9 @@ __global__ void canon_check_kernel(QesGroup G, const uint32_t* __restrict__ lut,
 // ---------------------------------------------------------------------------
 // host: plan construction and launches
 // ---------------------------------------------------------------------------
+static int pick_canon(const QesGroup& G);
+int pick_canon_public(const QesGroup& G) { return pick_canon(G); }
+
 static int pick_canon(const QesGroup& G) {
   const bool cyclic = G.nty == 1 && G.row_bits == G.ns && G.cell_bits == 1 && G.ntx == G.ns && G.ntx > 1;
   if (cyclic) return QES_CANON_CYCLIC;
@@ -434,6 +469,8 @@ int build_plan(Op* op, int canon_override) {
   const int ns = G.ns;
   QesPlan P{};
   P.canon = canon_override >= 0 ? canon_override : pick_canon(G);
+  P.uniform = op->bond_order_uniform;
+  P.direct = op->direct_lookup;
   if (P.canon == QES_CANON_CYCLIC &&
       !(G.nty == 1 && G.row_bits == ns && G.cell_bits == 1 && G.ntx == ns && G.ntx > 1))
     return fail("cyclic canonicaliser needs a chain translation group");

@@ -122,6 +122,10 @@ class HilbertSpace:
                                                     C.byref(bad)), "qes_basis_canon_check")
         return int(bad.value)
 
+    def set_lookup(self, mode: str = "hash"):
+        """Representative lookup of the matvec: 'hash' or 'sorted'."""
+        _lib.check(_lib.lib().qes_basis_set_lookup(self._h, {"sorted": 0, "hash": 1}[mode]), "qes_basis_set_lookup")
+
     def device_arrays(self):
         """Device pointers (reps uint64, norms float64) owned by this space."""
         r = C.c_void_p()

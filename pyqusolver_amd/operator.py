@@ -200,6 +200,10 @@ class Hamiltonian:
         _lib.check(_lib.lib().qes_op_set_canonicalizer(self._device_op(), self.CANON_KINDS[kind]),
                    "qes_op_set_canonicalizer")
 
+    def set_option(self, name: str, value: int):
+        """Kernel tuning knob (bond_order_uniform, direct_lookup, xcd_remap)."""
+        _lib.check(_lib.lib().qes_op_set_option(self._device_op(), name.encode(), int(value)), "qes_op_set_option")
+
     @property
     def plan(self) -> dict:
         c, nc, ng = C.c_int32(), C.c_int32(), C.c_int32()
