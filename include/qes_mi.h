@@ -114,6 +114,9 @@ int qes_matvec(qes_op_t op, const void* x_dev, void* y_dev, int64_t nbatch, int3
 int qes_matvec_couplings(qes_op_t op, const void* x_dev, void* y_dev, int32_t dtype, void* stream);
 int qes_op_n_couplings(qes_op_t op, int32_t* n);
 
+/* Diagonal H_aa of every row into device arrays (imaginary part optional). */
+int qes_op_diagonal(qes_op_t op, void* re_dev, void* im_dev, void* stream);
+
 /* Number of off-diagonal gathers of one matvec (E_off) -- stats pass. */
 int qes_op_count_offdiag(qes_op_t op, int64_t* e_off);
 
@@ -161,6 +164,52 @@ int qes_lanczos_destroy(qes_lanczos_t L);
 int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int32_t max_steps, int32_t reorth,
                              double tol, int32_t check_every, double* alpha_host, double* beta_host,
                              int32_t* steps_done, double* e0, void* stream);
+
+/*
+ * Batched Lanczos over many independent operators (sectors or disorder
+ * realisations).  Replaces the reference's sequential sector sweep
+ * (HilbertSpace.iter_momentum_sectors / iter_symmetry_sectors,
+ * Algebra/hilbert.py:412-652, one diagonalize() per sector) with one
+ * persistent matvec launch over a queue of row tiles of all problems and one
+ * launch per vector step, captured into a hipGraph.  Vectors of all problems
+ * are concatenated (problem p occupies elements offsets[p]..offsets[p+1]).
+ * Three-term recurrence without reorthogonalisation; converged problems are
+ * masked out.
+ */
+typedef struct qes_batch_s* qes_batch_t;
+int qes_batch_create(const qes_op_t* ops, int32_t nprob, int32_t max_steps, int32_t dtype, qes_batch_t* out);
+/*
+ * Same, plus nblock "block" problems: block i is R_i = block_nrhs[i] (1..256)
+ * disorder realisations that share the basis and every off-diagonal term of
+ * block_ops[i] and differ only in their diagonal, given on the device as
+ * block_diag_dev[i] (float64, n x R_i row-major: H_aa of realisation r at
+ * a*R_i + r, e.g. from qes_op_diagonal of each realisation).  Its vectors are
+ * stored n x R_i row-major after the ordinary problems.  Each realisation is
+ * its own Lanczos run (one virtual problem); virtual problems are numbered
+ * ordinary problems first, then block 0 columns 0..R_0-1, and so on -- the
+ * layout of alpha/beta/steps/active/e0.  Replaces one diagonalize() per
+ * realisation of a disordered model (Algebra/hamil.py:2063-2081 called in a
+ * loop over random fields).
+ */
+int qes_batch_create_blocks(const qes_op_t* ops, int32_t nprob, const qes_op_t* block_ops, const int32_t* block_nrhs,
+                            const void* const* block_diag_dev, int32_t nblock, int32_t max_steps, int32_t dtype,
+                            qes_batch_t* out);
+/* element offsets of every real problem (ordinary, then blocks) + total */
+int qes_batch_offsets(qes_batch_t b, int64_t* offsets /* nprob + nblock + 1 */);
+int qes_batch_start(qes_batch_t b, const void* v0_dev, void* stream);
+int qes_batch_run(qes_batch_t b, int32_t nsteps, int32_t use_graph);
+int qes_batch_get(qes_batch_t b, double* alpha_host, double* beta_host, int32_t* steps_host);
+int qes_batch_set_active(qes_batch_t b, const int32_t* active_host);
+int qes_batch_matvec(qes_batch_t b, const void* x_dev, void* y_dev, void* stream);
+/* Schedule options: "tile_order" (0 problem-contiguous, 1 largest-first
+ * round-robin), "grid_mult" (persistent matvec blocks per CU), "block_rows"
+ * (rows per tile of a block problem), "block_table" (1, default: block
+ * problems read a neighbour table built once at creation -- column index and
+ * pull-form value of every off-diagonal element, shared by all R columns and
+ * all steps; 0: canonicalise and look up on the fly like qes_matvec). */
+int qes_batch_set_option(qes_batch_t b, const char* name, int64_t value);
+int qes_batch_ground_states(qes_batch_t b, double tol, int32_t check_every, double* e0_host, int32_t* steps_host);
+int qes_batch_destroy(qes_batch_t b);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* qes_last_error(void);

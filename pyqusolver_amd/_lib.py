@@ -58,6 +58,7 @@ SIGNATURES = {
     "qes_matvec_couplings": [_P, _P, _P, _I32, _P],
     "qes_op_n_couplings": [_P, _PI32],
     "qes_op_count_offdiag": [_P, _PI64],
+    "qes_op_diagonal": [_P, _P, _P, _P],
     "qes_op_set_canonicalizer": [_P, _I32],
     "qes_op_set_option": [_P, C.c_char_p, _I64],
     "qes_op_get_plan": [_P, _PI32, _PI32, _PI32],
@@ -70,11 +71,17 @@ SIGNATURES = {
     "qes_lanczos_ritz": [_P, _PD, _I32, _P, _P],
     "qes_lanczos_destroy": [_P],
     "qes_lanczos_ground_state": [_P, _P, _I32, _I32, _I32, C.c_double, _I32, _PD, _PD, _PI32, _PD, _P],
-    "qes_batch_create": [_P, _I32, _I32, _I32, C.POINTER(_P)],
+    "qes_batch_create": [C.POINTER(_P), _I32, _I32, _I32, C.POINTER(_P)],
+    "qes_batch_create_blocks": [C.POINTER(_P), _I32, C.POINTER(_P), _PI32, C.POINTER(_P), _I32, _I32, _I32,
+                                C.POINTER(_P)],
+    "qes_batch_offsets": [_P, _PI64],
     "qes_batch_start": [_P, _P, _P],
     "qes_batch_run": [_P, _I32, _I32],
     "qes_batch_get": [_P, _PD, _PD, _PI32],
+    "qes_batch_set_active": [_P, _PI32],
     "qes_batch_matvec": [_P, _P, _P, _P],
+    "qes_batch_set_option": [_P, C.c_char_p, _I64],
+    "qes_batch_ground_states": [_P, C.c_double, _I32, _PD, _PI32],
     "qes_batch_destroy": [_P],
     "qes_last_error": [],
     "qes_version": [],

@@ -1,0 +1,236 @@
+"""Batched ground states of many sectors / disorder realisations in one go.
+
+The reference sweeps symmetry sectors sequentially
+(HilbertSpace.iter_momentum_sectors / iter_symmetry_sectors,
+Algebra/hilbert.py:412-652) and diagonalises each one separately.  Here all
+problems advance together on the device (libqes_mi batched Lanczos): one
+persistent matvec launch over a tile queue of every problem and one launch per
+vector step, replayed from a captured hipGraph, with converged problems masked
+out.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .lanczos import start_vector
+
+
+@dataclass
+class BatchResult:
+    e0: np.ndarray
+    steps: np.ndarray
+    seconds: float
+
+
+_DIAGONAL_CODES = (13, 22)   # sigma_z local / sigma_z correlation (operators_spin.py:1084-1118)
+
+
+def _offdiag_signature(h):
+    """Key of everything but the diagonal: the basis object, the build type
+    and the off-diagonal instructions with their effective coefficients."""
+    sites, coeffs, codes = h.instruction_arrays()
+    coeffs = np.asarray(coeffs, dtype=np.complex128)
+    scale = getattr(h, "_coupling_scale", None)
+    if scale is not None and len(codes):
+        coeffs = coeffs * scale[np.asarray(h._instr_coupling, dtype=np.int64)]
+    off = [(int(c), int(s[0]), int(s[1]), complex(v)) for c, s, v in zip(codes, sites, coeffs)
+           if int(c) not in _DIAGONAL_CODES]
+    return (id(h.hilbert_space), h._dtype_is_complex(), tuple(off))
+
+
+class BatchedLanczos:
+    """Owner of a qes_batch handle for a list of Hamiltonians.
+
+    Hamiltonians on the same HilbertSpace object whose off-diagonal
+    instructions coincide (disorder realisations of a random field) are
+    advanced as multi-RHS blocks (up to 256 columns each) unless
+    ``blocks=False``; the others are ordinary problems.  Results are always
+    reported in the order of ``hams``.
+    """
+
+    MAX_RHS = 256
+
+    def __init__(self, hams: Sequence, max_steps: int = 300, complex_: Optional[bool] = None,
+                 blocks: bool = True, min_block: int = 2):
+        if not hams:
+            raise ValueError("empty batch")
+        self.hams = list(hams)
+        cplx = any(h.needs_complex for h in self.hams) if complex_ is None else bool(complex_)
+        self.dtype = 1 if cplx else 0
+        self.max_steps = int(max_steps)
+        groups = {}
+        if blocks:
+            for i, h in enumerate(self.hams):
+                groups.setdefault(_offdiag_signature(h), []).append(i)
+        regular, blocks_ = [], []
+        for idx in (groups.values() if blocks else [[i] for i in range(len(self.hams))]):
+            if len(idx) < max(2, min_block):
+                regular += idx
+                continue
+            for s in range(0, len(idx), self.MAX_RHS):
+                blocks_.append(idx[s:s + self.MAX_RHS])
+        regular.sort()
+        self._diag = []
+        block_ops, block_r, block_d = [], [], []
+        for members in blocks_:
+            d = self._block_diagonal(members)
+            if d is None:                      # complex diagonal: ordinary problems
+                regular += members
+                continue
+            self._diag.append(d)
+            block_ops.append(self.hams[members[0]]._device_op().value)
+            block_r.append(len(members))
+            block_d.append(d.data_ptr())
+        self.regular = regular
+        self.blocks = [b for b in blocks_ if not set(b) & set(regular)]
+        # virtual problem number of every Hamiltonian
+        order = list(regular) + [i for b in self.blocks for i in b]
+        self.vp_of = np.empty(len(self.hams), dtype=np.int64)
+        self.vp_of[np.asarray(order, dtype=np.int64)] = np.arange(len(order))
+        ops = (C.c_void_p * max(len(regular), 1))(*[self.hams[i]._device_op().value for i in regular])
+        nb = len(self.blocks)
+        bops = (C.c_void_p * max(nb, 1))(*block_ops)
+        br = (C.c_int32 * max(nb, 1))(*block_r)
+        bd = (C.c_void_p * max(nb, 1))(*block_d)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().qes_batch_create_blocks(ops, len(regular), bops, br, bd, nb, self.max_steps, self.dtype,
+                                                      C.byref(h)), "qes_batch_create_blocks")
+        self._h = h
+        offs = np.zeros(len(regular) + nb + 1, dtype=np.int64)
+        _lib.check(_lib.lib().qes_batch_offsets(self._h, offs.ctypes.data_as(_lib._PI64)))
+        self.offsets = offs
+
+    def _block_diagonal(self, members):
+        """(n, R) float64 device array of H_aa of every member, or None when a
+        diagonal has an imaginary part."""
+        n = self.hams[members[0]].nh
+        re = torch.empty((len(members), n), dtype=torch.float64, device="cuda")
+        im = torch.empty((len(members), n), dtype=torch.float64, device="cuda")
+        for r, i in enumerate(members):
+            _lib.check(_lib.lib().qes_op_diagonal(self.hams[i]._device_op(), _lib.dptr(re[r]), _lib.dptr(im[r]),
+                                                  _lib.stream_ptr()), "qes_op_diagonal")
+        if bool((im != 0).any()):
+            return None
+        return re.t().contiguous()
+
+    @property
+    def torch_dtype(self):
+        return torch.complex128 if self.dtype == 1 else torch.float64
+
+    def set_option(self, name: str, value: int):
+        """Schedule knob: tile_order (0/1), grid_mult (blocks per CU),
+        block_rows (rows per tile of a block problem) or block_table (0:
+        matrix-free block rows, 1: cached neighbour table)."""
+        _lib.check(_lib.lib().qes_batch_set_option(self._h, name.encode(), int(value)), "qes_batch_set_option")
+
+    def concat(self, vecs: Sequence) -> torch.Tensor:
+        """Device vector in the batch layout from one vector (numpy or torch)
+        per Hamiltonian: ordinary problems concatenated, then each block stored
+        (n, R) row-major."""
+        dt = self.torch_dtype
+
+        def dev(v):
+            t = v if isinstance(v, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(v))
+            return t.to(device="cuda", dtype=dt)
+        parts = [dev(vecs[i]) for i in self.regular]
+        for b in self.blocks:
+            parts.append(torch.stack([dev(vecs[i]) for i in b], dim=1).reshape(-1))
+        return torch.cat(parts).contiguous()
+
+    def split(self, x: torch.Tensor) -> List[torch.Tensor]:
+        """Inverse of concat: one device vector per Hamiltonian."""
+        out = [None] * len(self.hams)
+        k = 0
+        for i in self.regular:
+            out[i] = x[int(self.offsets[k]):int(self.offsets[k + 1])]
+            k += 1
+        for b in self.blocks:
+            m = x[int(self.offsets[k]):int(self.offsets[k + 1])].reshape(-1, len(b))
+            for r, i in enumerate(b):
+                out[i] = m[:, r]
+            k += 1
+        return out
+
+    def matvec(self, x: torch.Tensor) -> torch.Tensor:
+        """y = H_p x_p for every problem, on vectors in the batch layout."""
+        y = torch.empty_like(x)
+        _lib.check(_lib.lib().qes_batch_matvec(self._h, _lib.dptr(x), _lib.dptr(y), _lib.stream_ptr()),
+                   "qes_batch_matvec")
+        return y
+
+    def start(self, v0: Optional[torch.Tensor] = None, seeds: Optional[Sequence[int]] = None):
+        if v0 is None:
+            seeds = list(range(len(self.hams))) if seeds is None else list(seeds)
+            v0 = self.concat([start_vector(h.nh, self.dtype == 1, s) for h, s in zip(self.hams, seeds)])
+        v0 = v0.to(self.torch_dtype).contiguous()
+        _lib.check(_lib.lib().qes_batch_start(self._h, _lib.dptr(v0), _lib.stream_ptr()), "qes_batch_start")
+
+    def ground_states(self, tol: float = 1e-10, check_every: int = 10, v0=None, seeds=None) -> BatchResult:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.start(v0, seeds)
+        V = len(self.hams)
+        e0 = np.zeros(V)
+        steps = np.zeros(V, dtype=np.int32)
+        _lib.check(_lib.lib().qes_batch_ground_states(self._h, float(tol), int(check_every),
+                                                      e0.ctypes.data_as(_lib._PD), steps.ctypes.data_as(_lib._PI32)),
+                   "qes_batch_ground_states")
+        return BatchResult(e0=e0[self.vp_of], steps=steps[self.vp_of], seconds=time.perf_counter() - t0)
+
+    def coefficients(self):
+        """alpha, beta (one row per Hamiltonian, in the order of hams) and steps."""
+        P, M = len(self.hams), self.max_steps
+        a = np.zeros(P * M)
+        b = np.zeros(P * M)
+        k = np.zeros(P, dtype=np.int32)
+        _lib.check(_lib.lib().qes_batch_get(self._h, a.ctypes.data_as(_lib._PD), b.ctypes.data_as(_lib._PD),
+                                            k.ctypes.data_as(_lib._PI32)), "qes_batch_get")
+        return a.reshape(P, M)[self.vp_of], b.reshape(P, M)[self.vp_of], k[self.vp_of]
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().qes_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def momentum_sectors(L: int, reflection: bool = True):
+    """Sector specs of a chain in the order of HilbertSpace.iter_momentum_sectors
+    (hilbert.py:546-652): every k, with reflection parities only where they
+    form a symmetry sector (k = 0 and k = L/2)."""
+    out = []
+    for k in range(L):
+        real = k == 0 or (L % 2 == 0 and k == L // 2)
+        if reflection and real:
+            out += [{"translation": k, "reflection": 1}, {"translation": k, "reflection": -1}]
+        else:
+            out.append({"translation": k})
+    return out
+
+
+def sector_by_sector(hams: Sequence, tol: float = 1e-10, max_steps: int = 300, check_every: int = 10, seeds=None):
+    """Reference-style sweep on the same GPU: one device Lanczos per problem,
+    same recurrence (no reorthogonalisation) and convergence rule."""
+    from .lanczos import lanczos_ground_state
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0 = []
+    steps = []
+    for i, h in enumerate(hams):
+        r = lanczos_ground_state(h, k=1, tol=tol, maxiter=max_steps, seed=(seeds[i] if seeds else i), reorth=False,
+                                 store_basis=False, check_every=check_every)
+        e0.append(r.eigenvalues[0])
+        steps.append(r.steps)
+    return BatchResult(e0=np.asarray(e0), steps=np.asarray(steps), seconds=time.perf_counter() - t0)

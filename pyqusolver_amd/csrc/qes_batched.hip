@@ -1,0 +1,1113 @@
+// Batched Lanczos over many independent sectors / disorder realisations.
+//
+// The reference sweeps sectors one HilbertSpace at a time
+// (HilbertSpace.iter_momentum_sectors / iter_symmetry_sectors,
+// Algebra/hilbert.py:412-652), each followed by its own diagonalize().
+// Here all problems of a batch advance together: one persistent matvec
+// launch walks a queue of row tiles drawn from every problem, and each
+// vector step (dot, three-term update, norm, normalise) is one launch over
+// chunks of every problem.  One Lanczos iteration of the whole batch is a
+// fixed launch sequence, captured once into a hipGraph and replayed.
+//
+// Problems that share a basis and every off-diagonal term and differ only in
+// their diagonal (disorder realisations of a random field) form a "block":
+// R right-hand sides stored row-major (element (a, r) at a*R + r) that are
+// advanced by one wavefront per row with lanes over the realisations, so a
+// single canonicalisation + lookup serves all R columns and the gathers are
+// contiguous.  Every column is its own Lanczos ("virtual problem") with its
+// own alpha/beta; converged virtual problems are masked out.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/qes_mi.h"
+#include "qes_internal.h"
+#include "qes_lanczos.h"
+#include "qes_rows.h"
+
+namespace qes {
+
+Op* qes_op_internal(qes_op_t h);   // qes_capi.hip
+
+static constexpr int BRB = 256;        // threads per block
+static constexpr int CHUNK = 4096;     // vector elements per chunk
+static constexpr int MAX_RHS = 256;    // columns of a block problem (4 per lane)
+
+struct QesProb {
+  QesGroup G;
+  QesLookup L;
+  QesPlan P;
+  const double* chi_re;
+  const double* chi_im;
+  const QesTermGroup* groups;
+  int32_t n_diag;
+  int32_t nrhs;         // 1: ordinary problem; R > 1: block of R realisations
+  const double* exval;
+  const uint32_t* lut;
+  const double* norms;
+  const double* D;      // block: per-realisation diagonal, (n, R) row-major
+  int64_t n;            // rows
+  int64_t off;          // element offset into the concatenated vectors
+  int32_t vp0;          // first virtual problem
+  int32_t blk;          // 1: block problem (diagonal from D)
+  // block problems: off-diagonal neighbour table (pull form, value already
+  // divided by N_a), built once at batch creation; null = matrix-free rows
+  const int64_t* rowptr;
+  const int32_t* col;
+  const double* val;    // 1 (real) or 2 (complex) doubles per entry
+};
+
+struct Tile {
+  int32_t prob;
+  int32_t rows;         // rows of this tile
+  int64_t row0;
+};
+
+struct Chunk {
+  int32_t prob;         // real problem
+  int32_t rows;         // ordinary: elements; block: rows (x R columns)
+  int64_t start;        // element offset
+  int32_t nrhs;         // 1 or R
+  int32_t vp0;          // virtual problem of column 0
+  int32_t slot;         // partial-sum slot of column 0
+  int32_t sstride;      // slot stride between columns (block)
+};
+
+// ---------------------------------------------------------------------------
+// batched matvec: persistent grid over the tile queue
+// ---------------------------------------------------------------------------
+template <typename W, bool CPLX, int CK>
+__device__ __forceinline__ void tile_rows(const RowCtx& C, const QesProb& P, int64_t a, const double* __restrict__ x,
+                                          double* __restrict__ y) {
+  const int cw = CPLX ? 2 : 1;
+  const W r = load_rep<W>(C.L, a);
+  const double inv_na = 1.0 / P.norms[a];
+  const double* xp0 = x + P.off * cw;
+  double* yp0 = y + P.off * cw;
+  double dr = 0.0, di = 0.0;
+  for (int t = 0; t < C.n_diag; ++t) {
+    const QesTermGroup& g = C.groups[t];
+    int k = term_bits<W>(g, r);
+    dr += sel4(g.re, k);
+    if (CPLX) di += sel4(g.im, k);
+  }
+  double ar, ai = 0.0;
+  if (CPLX) {
+    const double xr = xp0[a * 2], xi = xp0[a * 2 + 1];
+    ar = dr * xr - di * xi;
+    ai = dr * xi + di * xr;
+  } else {
+    ar = dr * xp0[a];
+  }
+  visit_row<W, CPLX, CK>(C, r, [&](int64_t b, double fr, double fi, int) {
+    fr *= inv_na;
+    fi *= inv_na;
+    if (CPLX) {
+      const double xr = xp0[b * 2], xi = xp0[b * 2 + 1];
+      ar += fr * xr - fi * xi;
+      ai += fr * xi + fi * xr;
+    } else {
+      ar += fr * xp0[b];
+    }
+  });
+  if (CPLX) {
+    yp0[a * 2] = ar;
+    yp0[a * 2 + 1] = ai;
+  } else {
+    yp0[a] = ar;
+  }
+}
+
+// Rp = lanes per row: R rounded up to a power of two, at most 64.  A
+// wavefront holds 64/Rp rows; lane c0 of a row owns columns c0, c0+Rp, ...
+__host__ __device__ __forceinline__ int block_lanes(int R) {
+  int rp = 1;
+  while (rp < R && rp < 64) rp <<= 1;
+  return rp;
+}
+
+__device__ __forceinline__ unsigned block_colmask(const QesProb& P, int c0, int Rp, const int* __restrict__ active) {
+  unsigned m = 0;
+#pragma unroll
+  for (int k = 0; k < MAX_RHS / 64; ++k) {
+    const int c = c0 + Rp * k;
+    if (c < P.nrhs && (!active || active[P.vp0 + c])) m |= 1u << k;
+  }
+  return m;
+}
+
+template <typename W, bool CPLX, int CK>
+__device__ __forceinline__ void block_row(const RowCtx& C, const QesProb& P, int64_t a, int c0, int Rp,
+                                          unsigned cm, const double* __restrict__ x, double* __restrict__ y) {
+  const int cw = CPLX ? 2 : 1;
+  const int R = P.nrhs;
+  const W r = load_rep<W>(C.L, a);
+  const double inv_na = 1.0 / P.norms[a];
+  const double* xp0 = x + P.off * cw;
+  double* yp0 = y + P.off * cw;
+  double ar[MAX_RHS / 64], ai[MAX_RHS / 64];
+#pragma unroll
+  for (int k = 0; k < MAX_RHS / 64; ++k) {
+    const int c = c0 + Rp * k;
+    ar[k] = 0.0;
+    ai[k] = 0.0;
+    if ((cm >> k) & 1) {
+      const int64_t e = a * R + c;
+      const double d = P.D[e];
+      ar[k] = d * xp0[e * cw];
+      if (CPLX) ai[k] = d * xp0[e * cw + 1];
+    }
+  }
+  visit_row<W, CPLX, CK>(C, r, [&](int64_t b, double fr, double fi, int) {
+    fr *= inv_na;
+    fi *= inv_na;
+#pragma unroll
+    for (int k = 0; k < MAX_RHS / 64; ++k) {
+      const int c = c0 + Rp * k;
+      if ((cm >> k) & 1) {
+        const int64_t e = b * R + c;
+        if (CPLX) {
+          const double xr = xp0[e * 2], xi = xp0[e * 2 + 1];
+          ar[k] += fr * xr - fi * xi;
+          ai[k] += fr * xi + fi * xr;
+        } else {
+          ar[k] += fr * xp0[e];
+        }
+      }
+    }
+  });
+#pragma unroll
+  for (int k = 0; k < MAX_RHS / 64; ++k) {
+    const int c = c0 + Rp * k;
+    if ((cm >> k) & 1) {
+      const int64_t e = a * R + c;
+      yp0[e * cw] = ar[k];
+      if (CPLX) yp0[e * cw + 1] = ai[k];
+    }
+  }
+}
+
+// Same row from the neighbour table: the lanes of a row read the same entry
+// (one broadcast request), entries are consumed four at a time so that the
+// table loads and the contiguous column gathers of four neighbours overlap.
+template <bool CPLX>
+__device__ __forceinline__ void block_row_table(const QesProb& P, int64_t a, int c0, int Rp, unsigned cm,
+                                                const double* __restrict__ x, double* __restrict__ y) {
+  const int cw = CPLX ? 2 : 1;
+  const int R = P.nrhs;
+  const double* xp0 = x + P.off * cw;
+  double* yp0 = y + P.off * cw;
+  double ar[MAX_RHS / 64], ai[MAX_RHS / 64];
+#pragma unroll
+  for (int k = 0; k < MAX_RHS / 64; ++k) {
+    const int c = c0 + Rp * k;
+    ar[k] = 0.0;
+    ai[k] = 0.0;
+    if ((cm >> k) & 1) {
+      const int64_t e = a * R + c;
+      const double d = P.D[e];
+      ar[k] = d * xp0[e * cw];
+      if (CPLX) ai[k] = d * xp0[e * cw + 1];
+    }
+  }
+  const int64_t e0 = P.rowptr[a], e1 = P.rowptr[a + 1];
+  auto acc = [&](int64_t b, double fr, double fi) {
+#pragma unroll
+    for (int k = 0; k < MAX_RHS / 64; ++k) {
+      const int c = c0 + Rp * k;
+      if ((cm >> k) & 1) {
+        const int64_t e = b * R + c;
+        if (CPLX) {
+          const double xr = xp0[e * 2], xi = xp0[e * 2 + 1];
+          ar[k] += fr * xr - fi * xi;
+          ai[k] += fr * xi + fi * xr;
+        } else {
+          ar[k] += fr * xp0[e];
+        }
+      }
+    }
+  };
+  int64_t e = e0;
+  for (; e + 4 <= e1; e += 4) {
+    int64_t b[4];
+    double fr[4], fi[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      b[u] = P.col[e + u];
+      fr[u] = P.val[(e + u) * cw];
+      fi[u] = CPLX ? P.val[(e + u) * cw + 1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc(b[u], fr[u], fi[u]);
+  }
+  for (; e < e1; ++e) acc(P.col[e], P.val[e * cw], CPLX ? P.val[e * cw + 1] : 0.0);
+#pragma unroll
+  for (int k = 0; k < MAX_RHS / 64; ++k) {
+    const int c = c0 + Rp * k;
+    if ((cm >> k) & 1) {
+      const int64_t e = a * R + c;
+      yp0[e * cw] = ar[k];
+      if (CPLX) yp0[e * cw + 1] = ai[k];
+    }
+  }
+}
+
+template <typename W, bool CPLX>
+__global__ void __launch_bounds__(BRB) batch_matvec_kernel(const QesProb* __restrict__ probs,
+                                                           const Tile* __restrict__ tiles, int ntiles,
+                                                           const int* __restrict__ prob_active,
+                                                           const int* __restrict__ vp_active,
+                                                           const double* __restrict__ x, double* __restrict__ y) {
+  extern __shared__ char smem[];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile T = tiles[t];
+    if (prob_active && !prob_active[T.prob]) continue;   // uniform per block
+    const QesProb& P = probs[T.prob];
+    if (P.rowptr) {   // tabulated block problem: no operator context needed
+      const int Rp = block_lanes(P.nrhs);
+      const int c0 = threadIdx.x % Rp;
+      const unsigned cm = block_colmask(P, c0, Rp, vp_active);
+      for (int64_t a = T.row0 + threadIdx.x / Rp; a < (T.row0 + T.rows < P.n ? T.row0 + T.rows : P.n); a += BRB / Rp)
+        block_row_table<CPLX>(P, a, c0, Rp, cm, x, y);
+      continue;
+    }
+    __syncthreads();                                     // previous tile done with LDS
+    const RowCtx C = load_ctx<CPLX>(P.G, P.L, P.P, P.chi_re, P.chi_im, P.groups, P.n_diag, P.exval, P.lut, smem);
+    if (!P.blk) {
+      const int64_t a = T.row0 + threadIdx.x;
+      if (a < P.n) {
+        switch (P.P.canon) {
+          case QES_CANON_CYCLIC: tile_rows<W, CPLX, QES_CANON_CYCLIC>(C, P, a, x, y); break;
+          case QES_CANON_ROWS: tile_rows<W, CPLX, QES_CANON_ROWS>(C, P, a, x, y); break;
+          default: tile_rows<W, CPLX, QES_CANON_BRUTE>(C, P, a, x, y); break;
+        }
+      }
+    } else {
+      const int Rp = block_lanes(P.nrhs);
+      const int c0 = threadIdx.x % Rp;
+      const unsigned cm = block_colmask(P, c0, Rp, vp_active);
+      for (int64_t a = T.row0 + threadIdx.x / Rp; a < (T.row0 + T.rows < P.n ? T.row0 + T.rows : P.n); a += BRB / Rp) {
+        switch (P.P.canon) {
+          case QES_CANON_CYCLIC: block_row<W, CPLX, QES_CANON_CYCLIC>(C, P, a, c0, Rp, cm, x, y); break;
+          case QES_CANON_ROWS: block_row<W, CPLX, QES_CANON_ROWS>(C, P, a, c0, Rp, cm, x, y); break;
+          default: block_row<W, CPLX, QES_CANON_BRUTE>(C, P, a, c0, Rp, cm, x, y); break;
+        }
+      }
+    }
+  }
+}
+
+// neighbour table of a block problem: count pass, then fill pass in the
+// exact order (and arithmetic) of the matrix-free visitor
+template <typename W, bool CPLX, bool FILL>
+__global__ void __launch_bounds__(BRB) table_kernel(const QesProb* __restrict__ probs, int p,
+                                                    int64_t* __restrict__ cnt, const int64_t* __restrict__ rowptr,
+                                                    int32_t* __restrict__ col, double* __restrict__ val) {
+  extern __shared__ char smem[];
+  const QesProb& P = probs[p];
+  const RowCtx C = load_ctx<CPLX>(P.G, P.L, P.P, P.chi_re, P.chi_im, P.groups, P.n_diag, P.exval, P.lut, smem);
+  const int64_t a = (int64_t)blockIdx.x * BRB + threadIdx.x;
+  if (a >= P.n) return;
+  const W r = load_rep<W>(C.L, a);
+  const double inv_na = 1.0 / P.norms[a];
+  int64_t k = FILL ? rowptr[a] : 0;
+  auto f = [&](int64_t b, double fr, double fi, int) {
+    if (FILL) {
+      col[k] = (int32_t)b;
+      if (CPLX) {
+        val[2 * k] = fr * inv_na;
+        val[2 * k + 1] = fi * inv_na;
+      } else {
+        val[k] = fr * inv_na;
+      }
+    }
+    ++k;
+  };
+  switch (P.P.canon) {
+    case QES_CANON_CYCLIC: visit_row<W, CPLX, QES_CANON_CYCLIC>(C, r, f); break;
+    case QES_CANON_ROWS: visit_row<W, CPLX, QES_CANON_ROWS>(C, r, f); break;
+    default: visit_row<W, CPLX, QES_CANON_BRUTE>(C, r, f); break;
+  }
+  if (!FILL) cnt[a] = k;
+}
+
+// ---------------------------------------------------------------------------
+// batched vector kernels.  Ordinary chunks: contiguous elements of one virtual
+// problem, block-wide reduction.  Block chunks: rows x R columns; thread t
+// works on column t % Rq (one virtual problem) of every (BRB/Rq)-th row, and
+// the per-column partial sums are reduced across the BRB/Rq row groups.
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int block_cols(int R) {
+  int rq = 1;
+  while (rq < R) rq <<= 1;
+  return rq;   // R <= MAX_RHS == BRB
+}
+
+// mode 0: <a, b>, mode 1: <b, b>
+template <bool CPLX>
+__global__ void __launch_bounds__(BRB) bdot_kernel(const Chunk* __restrict__ chunks, int nchunks,
+                                                   const int* __restrict__ active, const double* __restrict__ a,
+                                                   const double* __restrict__ b, double* __restrict__ part) {
+  __shared__ double sr[BRB], si[BRB];
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const Chunk K = chunks[c];
+    if (K.nrhs > 1) {
+      // Rq threads per row (R rounded up to a power of two), BRB/Rq rows at once
+      const int Rq = block_cols(K.nrhs);
+      const int col = threadIdx.x % Rq;
+      double r = 0.0, im = 0.0;
+      if (col < K.nrhs && (!active || active[K.vp0 + col])) {
+        for (int i = threadIdx.x / Rq; i < K.rows; i += BRB / Rq) {
+          const int64_t e = K.start + (int64_t)i * K.nrhs + col;
+          if (CPLX) {
+            const double ar = a[2 * e], ai = a[2 * e + 1], br = b[2 * e], bi = b[2 * e + 1];
+            r += ar * br + ai * bi;
+            im += ar * bi - ai * br;
+          } else {
+            r += a[e] * b[e];
+          }
+        }
+      }
+      __syncthreads();
+      sr[threadIdx.x] = r;
+      si[threadIdx.x] = im;
+      __syncthreads();
+      for (int off = BRB / 2; off >= Rq; off >>= 1) {
+        if (threadIdx.x < (unsigned)off) {
+          sr[threadIdx.x] += sr[threadIdx.x + off];
+          si[threadIdx.x] += si[threadIdx.x + off];
+        }
+        __syncthreads();
+      }
+      if ((int)threadIdx.x < K.nrhs) {
+        part[2 * (K.slot + threadIdx.x * K.sstride)] = sr[threadIdx.x];
+        part[2 * (K.slot + threadIdx.x * K.sstride) + 1] = si[threadIdx.x];
+      }
+      continue;
+    }
+    double r = 0.0, im = 0.0;
+    if (!active || active[K.vp0]) {
+      for (int i = threadIdx.x; i < K.rows; i += BRB) {
+        const int64_t e = K.start + i;
+        if (CPLX) {
+          const double ar = a[2 * e], ai = a[2 * e + 1], br = b[2 * e], bi = b[2 * e + 1];
+          r += ar * br + ai * bi;
+          im += ar * bi - ai * br;
+        } else {
+          r += a[e] * b[e];
+        }
+      }
+    }
+    __syncthreads();
+    sr[threadIdx.x] = r;
+    si[threadIdx.x] = im;
+    __syncthreads();
+    for (int off = BRB / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < (unsigned)off) {
+        sr[threadIdx.x] += sr[threadIdx.x + off];
+        si[threadIdx.x] += si[threadIdx.x + off];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      part[2 * K.slot] = sr[0];
+      part[2 * K.slot + 1] = si[0];
+    }
+  }
+}
+
+// one block per virtual problem: scal[v] = sum of its partial slots (mode 1: sqrt)
+__global__ void __launch_bounds__(BRB) bfinish_kernel(const int* __restrict__ vslot0, const int* __restrict__ vnslot,
+                                                      int nvp, const int* __restrict__ active,
+                                                      const double* __restrict__ part, double* __restrict__ scal,
+                                                      int mode) {
+  __shared__ double sr[BRB], si[BRB];
+  for (int p = blockIdx.x; p < nvp; p += gridDim.x) {
+    double r = 0.0, im = 0.0;
+    const bool on = !active || active[p];
+    if (on) {
+      for (int c = threadIdx.x; c < vnslot[p]; c += BRB) {
+        r += part[2 * (vslot0[p] + c)];
+        im += part[2 * (vslot0[p] + c) + 1];
+      }
+    }
+    __syncthreads();
+    sr[threadIdx.x] = r;
+    si[threadIdx.x] = im;
+    __syncthreads();
+    for (int off = BRB / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < (unsigned)off) {
+        sr[threadIdx.x] += sr[threadIdx.x + off];
+        si[threadIdx.x] += si[threadIdx.x + off];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0 && on) {
+      scal[2 * p] = mode == 1 ? sqrt(sr[0]) : sr[0];
+      scal[2 * p + 1] = mode == 1 ? 0.0 : si[0];
+    }
+  }
+}
+
+__global__ void bset_alpha_kernel(int nvp, const int* __restrict__ active, const double* __restrict__ scal,
+                                  double* __restrict__ alpha, const int* __restrict__ jstep, int maxs) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nvp && (!active || active[p]) && jstep[p] < maxs) alpha[(int64_t)p * maxs + jstep[p]] = scal[2 * p];
+}
+
+__global__ void bset_beta_kernel(int nvp, const int* __restrict__ active, const double* __restrict__ scal,
+                                 double* __restrict__ beta, int* __restrict__ jstep, int maxs) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nvp && (!active || active[p]) && jstep[p] < maxs) {
+    beta[(int64_t)p * maxs + jstep[p]] = scal[2 * p];
+    jstep[p] = jstep[p] + 1;
+  }
+}
+
+// element-wise step over a chunk: fn(e, vp) for every element e (in doubles
+// units handled by the caller) whose virtual problem vp is active
+template <bool CPLX, class F>
+__device__ __forceinline__ void for_chunk(const Chunk& K, const int* __restrict__ active, F&& fn) {
+  const int cw = CPLX ? 2 : 1;
+  if (K.nrhs > 1) {
+    const int Rq = block_cols(K.nrhs);
+    const int col = threadIdx.x % Rq;
+    if (col >= K.nrhs) return;
+    const int vp = K.vp0 + col;
+    if (active && !active[vp]) return;
+    for (int i = threadIdx.x / Rq; i < K.rows; i += BRB / Rq) {
+      const int64_t e = K.start + (int64_t)i * K.nrhs + col;
+      fn(e * cw, vp);
+      if (CPLX) fn(e * cw + 1, vp);
+    }
+  } else {
+    if (active && !active[K.vp0]) return;
+    for (int i = threadIdx.x; i < K.rows * cw; i += BRB) fn(K.start * cw + i, K.vp0);
+  }
+}
+
+template <bool CPLX>
+__global__ void __launch_bounds__(BRB) brecur_kernel(const Chunk* __restrict__ chunks, int nchunks,
+                                                     const int* __restrict__ active, double* __restrict__ w,
+                                                     const double* __restrict__ v, const double* __restrict__ vp,
+                                                     const double* __restrict__ alpha, const double* __restrict__ beta,
+                                                     const int* __restrict__ jstep, int maxs) {
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const Chunk K = chunks[c];
+    for_chunk<CPLX>(K, active, [&](int64_t e, int p) {
+      const int j = jstep[p];
+      if (j >= maxs) return;   // step budget of this problem exhausted
+      const double al = alpha[(int64_t)p * maxs + j];
+      const double be = j > 0 ? beta[(int64_t)p * maxs + j - 1] : 0.0;
+      w[e] = w[e] - al * v[e] - be * vp[e];
+    });
+  }
+}
+
+// v_prev <- v ; v <- w / scal
+template <bool CPLX>
+__global__ void __launch_bounds__(BRB) badvance_kernel(const Chunk* __restrict__ chunks, int nchunks,
+                                                       const int* __restrict__ active, const double* __restrict__ w,
+                                                       double* __restrict__ v, double* __restrict__ vp,
+                                                       const double* __restrict__ scal, int first) {
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const Chunk K = chunks[c];
+    for_chunk<CPLX>(K, active, [&](int64_t e, int p) {
+      const double be = scal[2 * p];
+      const double inv = be > 0.0 ? 1.0 / be : 0.0;
+      vp[e] = first ? 0.0 : v[e];
+      v[e] = w[e] * inv;
+    });
+  }
+}
+
+__global__ void bzero_int_kernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+struct BlockSpec {
+  Op* op;
+  int nrhs;
+  const double* D;
+};
+
+struct Batch {
+  std::vector<Op*> ops;           // real problems (ordinary then blocks)
+  std::vector<int> nrhs, vp0, blk;
+  int nprob = 0, nvp = 0, maxs = 0, dtype = 0;
+  bool w32 = true;
+  int64_t total = 0;               // elements
+  std::vector<int64_t> offs;
+  QesProb* d_probs = nullptr;
+  Tile* d_tiles = nullptr;
+  Chunk* d_chunks = nullptr;
+  int* d_vslot0 = nullptr;
+  int* d_vnslot = nullptr;
+  int ntiles = 0, nchunks = 0, nslots = 0;
+  int* d_active = nullptr;         // per virtual problem
+  int* d_pactive = nullptr;        // per real problem (any column active)
+  int* d_j = nullptr;
+  double *v = nullptr, *vp = nullptr, *w = nullptr;
+  double *alpha = nullptr, *beta = nullptr, *scal = nullptr, *part = nullptr;
+  size_t lds = 0;
+  int grid_matvec = 0, grid_vec = 0;
+  int tile_order = 1;     // 0: problem-contiguous tiles, 1: largest problems first, round-robin
+  int grid_mult = 8;      // persistent matvec blocks per CU
+  int block_rows = 16;    // rows per tile of a block problem (at least one per lane group)
+  int block_table = 1;    // block problems use their neighbour table
+  std::vector<QesProb> h_probs;
+  std::vector<int64_t*> t_rowptr;   // per real problem (null for ordinary problems)
+  std::vector<int32_t*> t_col;
+  std::vector<double*> t_val;
+  int64_t table_entries = 0;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> h_active;
+};
+
+static int batch_build_tiles(Batch* B) {
+  std::vector<std::vector<Tile>> per(B->nprob);
+  for (int p = 0; p < B->nprob; ++p) {
+    const int rows_per_tile = B->blk[p] ? std::max(BRB / block_lanes(B->nrhs[p]), B->block_rows) : BRB;
+    for (int64_t r = 0; r < B->ops[p]->basis->n; r += rows_per_tile) per[p].push_back(Tile{p, rows_per_tile, r});
+  }
+  std::vector<Tile> tiles;
+  if (B->tile_order == 0) {
+    for (auto& v : per) tiles.insert(tiles.end(), v.begin(), v.end());
+  } else {
+    // largest problems first, then round-robin across problems so that every
+    // wave of the persistent grid mixes problems of all sizes
+    std::vector<int> order(B->nprob);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return per[a].size() > per[b].size(); });
+    size_t maxlen = 0;
+    for (auto& v : per) maxlen = std::max(maxlen, v.size());
+    for (size_t i = 0; i < maxlen; ++i)
+      for (int p : order)
+        if (i < per[p].size()) tiles.push_back(per[p][i]);
+  }
+  if (B->d_tiles) hipFree(B->d_tiles);
+  B->d_tiles = nullptr;
+  B->ntiles = (int)tiles.size();
+  QES_HIP(hipMalloc(&B->d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(Tile)));
+  if (!tiles.empty()) QES_HIP(hipMemcpy(B->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  int dev = 0, ncu = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  B->grid_matvec = std::max(1, std::min(B->ntiles, ncu * B->grid_mult));
+  return 0;
+}
+
+// Build the neighbour tables of the block problems (h_probs must be on the
+// device without table pointers), then upload the table pointers if enabled.
+static int batch_build_tables(Batch* B) {
+  B->t_rowptr.assign(B->nprob, nullptr);
+  B->t_col.assign(B->nprob, nullptr);
+  B->t_val.assign(B->nprob, nullptr);
+  B->table_entries = 0;
+  const int cw = B->dtype == 1 ? 2 : 1;
+  for (int p = 0; p < B->nprob; ++p) {
+    if (!B->blk[p]) continue;
+    const int64_t n = B->h_probs[p].n;
+    if (n >= (int64_t(1) << 31)) return fail("block problem too large for 32-bit neighbour indices");
+    int64_t* d_cnt = nullptr;
+    QES_HIP(hipMalloc(&d_cnt, sizeof(int64_t) * (n + 1)));
+    B->t_rowptr[p] = d_cnt;   // reused as rowptr after the scan
+    const dim3 grid((unsigned)((n + BRB - 1) / BRB));
+    const size_t lds = B->ops[p]->lds_bytes;
+    const bool c = B->dtype == 1;
+#define QES_TABLE_LAUNCH(FILL, ...)                                                                               \
+    do {                                                                                                          \
+      if (B->w32) {                                                                                               \
+        if (c) hipLaunchKernelGGL((table_kernel<uint32_t, true, FILL>), grid, dim3(BRB), lds, B->stream, __VA_ARGS__); \
+        else hipLaunchKernelGGL((table_kernel<uint32_t, false, FILL>), grid, dim3(BRB), lds, B->stream, __VA_ARGS__); \
+      } else {                                                                                                    \
+        if (c) hipLaunchKernelGGL((table_kernel<uint64_t, true, FILL>), grid, dim3(BRB), lds, B->stream, __VA_ARGS__); \
+        else hipLaunchKernelGGL((table_kernel<uint64_t, false, FILL>), grid, dim3(BRB), lds, B->stream, __VA_ARGS__); \
+      }                                                                                                           \
+    } while (0)
+    QES_TABLE_LAUNCH(false, B->d_probs, p, d_cnt, nullptr, nullptr, nullptr);
+    QES_HIP(hipGetLastError());
+    std::vector<int64_t> h(n + 1);
+    QES_HIP(hipMemcpyAsync(h.data(), d_cnt, sizeof(int64_t) * n, hipMemcpyDeviceToHost, B->stream));
+    QES_HIP(hipStreamSynchronize(B->stream));
+    int64_t acc = 0;
+    for (int64_t a = 0; a < n; ++a) {
+      const int64_t k = h[a];
+      h[a] = acc;
+      acc += k;
+    }
+    h[n] = acc;
+    QES_HIP(hipMemcpy(d_cnt, h.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    QES_HIP(hipMalloc(&B->t_col[p], sizeof(int32_t) * std::max<int64_t>(acc, 1)));
+    QES_HIP(hipMalloc(&B->t_val[p], sizeof(double) * cw * std::max<int64_t>(acc, 1)));
+    QES_TABLE_LAUNCH(true, B->d_probs, p, nullptr, d_cnt, B->t_col[p], B->t_val[p]);
+#undef QES_TABLE_LAUNCH
+    QES_HIP(hipGetLastError());
+    QES_HIP(hipStreamSynchronize(B->stream));
+    B->table_entries += acc;
+  }
+  return 0;
+}
+
+static int batch_upload_probs(Batch* B) {
+  std::vector<QesProb> h = B->h_probs;
+  for (int p = 0; p < B->nprob; ++p) {
+    const bool on = B->block_table && B->t_rowptr.size() == (size_t)B->nprob && B->t_rowptr[p];
+    h[p].rowptr = on ? B->t_rowptr[p] : nullptr;
+    h[p].col = on ? B->t_col[p] : nullptr;
+    h[p].val = on ? B->t_val[p] : nullptr;
+  }
+  QES_HIP(hipMemcpy(B->d_probs, h.data(), B->nprob * sizeof(QesProb), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static void batch_free(Batch* B) {
+  for (auto p : B->t_rowptr) hipFree(p);
+  for (auto p : B->t_col) hipFree(p);
+  for (auto p : B->t_val) hipFree(p);
+  if (B->exec) hipGraphExecDestroy(B->exec);
+  if (B->graph) hipGraphDestroy(B->graph);
+  hipFree(B->d_probs);
+  hipFree(B->d_tiles);
+  hipFree(B->d_chunks);
+  hipFree(B->d_vslot0);
+  hipFree(B->d_vnslot);
+  hipFree(B->v);
+  hipFree(B->vp);
+  hipFree(B->w);
+  hipFree(B->alpha);
+  hipFree(B->beta);
+  hipFree(B->scal);
+  hipFree(B->part);
+  hipFree(B->d_active);
+  hipFree(B->d_pactive);
+  hipFree(B->d_j);
+  if (B->stream) hipStreamDestroy(B->stream);
+}
+
+int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, int max_steps, int dtype,
+                 Batch** out) {
+  if (nreg + nblock < 1) return fail("empty batch");
+  if (max_steps < 1) return fail("max_steps must be >= 1");
+  if (dtype != 0 && dtype != 1) return fail("dtype must be 0 or 1");
+  Batch* B = new Batch();
+  B->maxs = max_steps;
+  B->dtype = dtype;
+  for (int i = 0; i < nreg; ++i) {
+    B->ops.push_back(ops[i]);
+    B->nrhs.push_back(1);
+    B->blk.push_back(0);
+  }
+  std::vector<const double*> D(nreg, nullptr);
+  for (int i = 0; i < nblock; ++i) {
+    if (blocks[i].nrhs < 1 || blocks[i].nrhs > MAX_RHS) {
+      delete B;
+      return fail("block problems hold 1..256 right-hand sides");
+    }
+    if (!blocks[i].D) {
+      delete B;
+      return fail("block problem without diagonal array");
+    }
+    B->ops.push_back(blocks[i].op);
+    B->nrhs.push_back(blocks[i].nrhs);
+    B->blk.push_back(1);
+    D.push_back(blocks[i].D);
+  }
+  B->nprob = (int)B->ops.size();
+  std::vector<QesProb> h(B->nprob);
+  std::vector<Chunk> chunks;
+  std::vector<int> vslot0, vnslot;
+  int64_t off = 0;
+  int slot = 0, vp = 0;
+  for (int p = 0; p < B->nprob; ++p) {
+    const Op* op = B->ops[p];
+    if (dtype == 0 && (op->is_complex || op->chi_complex)) {
+      delete B;
+      return fail("problem " + std::to_string(p) + " needs complex128 vectors");
+    }
+    const Basis* Bs = op->basis;
+    if (Bs->group.ns > 32) B->w32 = false;
+    const int R = B->nrhs[p];
+    QesProb& q = h[p];
+    memset(&q, 0, sizeof(q));
+    q.G = Bs->group;
+    q.L = Bs->lookup();
+    q.P = op->plan;
+    q.chi_re = Bs->chi_re;
+    q.chi_im = Bs->chi_im;
+    q.groups = op->d_gen;
+    q.n_diag = op->n_diag;   // blocks take their diagonal from D (visit_row skips these groups)
+    q.nrhs = R;
+    q.blk = B->blk[p];
+    q.exval = op->d_exval;
+    q.lut = op->d_lut;
+    q.norms = Bs->norms;
+    q.D = D[p];
+    q.n = Bs->n;
+    q.off = off;
+    q.vp0 = vp;
+    B->vp0.push_back(vp);
+    if (R == 1) {
+      vslot0.push_back(slot);
+      int cnt = 0;
+      for (int64_t s = 0; s < Bs->n; s += CHUNK, ++cnt)
+        chunks.push_back(Chunk{p, (int)std::min<int64_t>(CHUNK, Bs->n - s), off + s, 1, vp, slot + cnt, 0});
+      vnslot.push_back(cnt);
+      slot += cnt;
+      vp += 1;
+    } else {
+      const int rows = std::max(1, CHUNK / R);
+      const int nbc = (int)((Bs->n + rows - 1) / rows);
+      for (int k = 0; k < nbc; ++k) {
+        const int64_t a0 = (int64_t)k * rows;
+        chunks.push_back(Chunk{p, (int)std::min<int64_t>(rows, Bs->n - a0), off + a0 * R, R, vp, slot + k, nbc});
+      }
+      for (int r = 0; r < R; ++r) {
+        vslot0.push_back(slot + r * nbc);
+        vnslot.push_back(nbc);
+      }
+      slot += R * nbc;
+      vp += R;
+    }
+    B->offs.push_back(off);
+    off += Bs->n * R;
+    B->lds = std::max(B->lds, op->lds_bytes);
+  }
+  B->offs.push_back(off);
+  B->total = off;
+  B->nvp = vp;
+  B->nslots = slot;
+  B->nchunks = (int)chunks.size();
+  const int cw = dtype == 1 ? 2 : 1;
+  const size_t vb = std::max<int64_t>(off, 1) * cw * sizeof(double);
+  int rc = 0;
+#define QES_BALLOC(ptr, bytes) \
+  if (!rc && hipMalloc(&(ptr), (bytes)) != hipSuccess) rc = fail("hipMalloc failed in batch_create")
+  QES_BALLOC(B->d_probs, B->nprob * sizeof(QesProb));
+  QES_BALLOC(B->d_chunks, std::max<size_t>(chunks.size(), 1) * sizeof(Chunk));
+  QES_BALLOC(B->d_vslot0, std::max(B->nvp, 1) * sizeof(int));
+  QES_BALLOC(B->d_vnslot, std::max(B->nvp, 1) * sizeof(int));
+  QES_BALLOC(B->v, vb);
+  QES_BALLOC(B->vp, vb);
+  QES_BALLOC(B->w, vb);
+  QES_BALLOC(B->alpha, sizeof(double) * B->nvp * max_steps);
+  QES_BALLOC(B->beta, sizeof(double) * B->nvp * max_steps);
+  QES_BALLOC(B->scal, sizeof(double) * 2 * B->nvp);
+  QES_BALLOC(B->part, sizeof(double) * 2 * std::max(B->nslots, 1));
+  QES_BALLOC(B->d_active, sizeof(int) * B->nvp);
+  QES_BALLOC(B->d_pactive, sizeof(int) * B->nprob);
+  QES_BALLOC(B->d_j, sizeof(int) * B->nvp);
+#undef QES_BALLOC
+  B->h_probs = h;
+  if (!rc) {
+    hipMemcpy(B->d_probs, h.data(), B->nprob * sizeof(QesProb), hipMemcpyHostToDevice);
+    if (!chunks.empty()) hipMemcpy(B->d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice);
+    hipMemcpy(B->d_vslot0, vslot0.data(), B->nvp * sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpy(B->d_vnslot, vnslot.data(), B->nvp * sizeof(int), hipMemcpyHostToDevice);
+    B->h_active.assign(B->nvp, 1);
+    std::vector<int> pa(B->nprob, 1);
+    hipMemcpy(B->d_active, B->h_active.data(), sizeof(int) * B->nvp, hipMemcpyHostToDevice);
+    hipMemcpy(B->d_pactive, pa.data(), sizeof(int) * B->nprob, hipMemcpyHostToDevice);
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) rc = fail("stream creation failed");
+  }
+  B->grid_vec = std::max(1, std::min(B->nchunks, 4096));
+  if (!rc) rc = batch_build_tables(B);
+  if (!rc) rc = batch_upload_probs(B);
+  if (!rc) rc = batch_build_tiles(B);
+  if (rc) {
+    batch_free(B);
+    delete B;
+    return -1;
+  }
+  *out = B;
+  return 0;
+}
+
+static void batch_drop_graph(Batch* B) {
+  if (B->exec) hipGraphExecDestroy(B->exec);
+  if (B->graph) hipGraphDestroy(B->graph);
+  B->exec = nullptr;
+  B->graph = nullptr;
+}
+
+int batch_destroy(Batch* B) {
+  if (!B) return 0;
+  batch_free(B);
+  delete B;
+  return 0;
+}
+
+static int enqueue_bmatvec(Batch* B, const double* x, double* y, const int* pactive, const int* vactive,
+                           hipStream_t st) {
+  const bool c = B->dtype == 1;
+  if (B->w32) {
+    if (c) hipLaunchKernelGGL((batch_matvec_kernel<uint32_t, true>), dim3(B->grid_matvec), dim3(BRB), B->lds, st,
+                               B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y);
+    else hipLaunchKernelGGL((batch_matvec_kernel<uint32_t, false>), dim3(B->grid_matvec), dim3(BRB), B->lds, st,
+                            B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y);
+  } else {
+    if (c) hipLaunchKernelGGL((batch_matvec_kernel<uint64_t, true>), dim3(B->grid_matvec), dim3(BRB), B->lds, st,
+                               B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y);
+    else hipLaunchKernelGGL((batch_matvec_kernel<uint64_t, false>), dim3(B->grid_matvec), dim3(BRB), B->lds, st,
+                            B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y);
+  }
+  QES_HIP(hipGetLastError());
+  return 0;
+}
+
+static void enqueue_bdot(Batch* B, const double* a, const double* b, int mode, const int* active, hipStream_t st) {
+  if (B->dtype == 1)
+    hipLaunchKernelGGL(bdot_kernel<true>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks, active, a, b,
+                       B->part);
+  else
+    hipLaunchKernelGGL(bdot_kernel<false>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks, active, a, b,
+                       B->part);
+  hipLaunchKernelGGL(bfinish_kernel, dim3(std::min(B->nvp, 4096)), dim3(BRB), 0, st, B->d_vslot0, B->d_vnslot, B->nvp,
+                     active, B->part, B->scal, mode);
+}
+
+static int enqueue_bstep(Batch* B, hipStream_t st) {
+  const bool c = B->dtype == 1;
+  const int pg = (B->nvp + 255) / 256;
+  if (enqueue_bmatvec(B, B->v, B->w, B->d_pactive, B->d_active, st)) return -1;
+  enqueue_bdot(B, B->v, B->w, 0, B->d_active, st);
+  hipLaunchKernelGGL(bset_alpha_kernel, dim3(pg), dim3(256), 0, st, B->nvp, B->d_active, B->scal, B->alpha, B->d_j,
+                     B->maxs);
+  if (c)
+    hipLaunchKernelGGL(brecur_kernel<true>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks, B->d_active,
+                       B->w, B->v, B->vp, B->alpha, B->beta, B->d_j, B->maxs);
+  else
+    hipLaunchKernelGGL(brecur_kernel<false>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks,
+                       B->d_active, B->w, B->v, B->vp, B->alpha, B->beta, B->d_j, B->maxs);
+  enqueue_bdot(B, B->w, B->w, 1, B->d_active, st);
+  if (c)
+    hipLaunchKernelGGL(badvance_kernel<true>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks,
+                       B->d_active, B->w, B->v, B->vp, B->scal, 0);
+  else
+    hipLaunchKernelGGL(badvance_kernel<false>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks,
+                       B->d_active, B->w, B->v, B->vp, B->scal, 0);
+  hipLaunchKernelGGL(bset_beta_kernel, dim3(pg), dim3(256), 0, st, B->nvp, B->d_active, B->scal, B->beta, B->d_j,
+                     B->maxs);
+  QES_HIP(hipGetLastError());
+  return 0;
+}
+
+int batch_set_active(Batch* B, const int* active) {
+  B->h_active.assign(active, active + B->nvp);
+  std::vector<int> pa(B->nprob, 0);
+  for (int p = 0; p < B->nprob; ++p)
+    for (int r = 0; r < B->nrhs[p]; ++r) pa[p] |= active[B->vp0[p] + r] ? 1 : 0;
+  QES_HIP(hipMemcpy(B->d_active, active, sizeof(int) * B->nvp, hipMemcpyHostToDevice));
+  QES_HIP(hipMemcpy(B->d_pactive, pa.data(), sizeof(int) * B->nprob, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int batch_start(Batch* B, const void* v0, hipStream_t user) {
+  hipStream_t st = B->stream;
+  hipEvent_t ev;
+  QES_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  QES_HIP(hipEventRecord(ev, user));
+  QES_HIP(hipStreamWaitEvent(st, ev, 0));
+  const int cw = B->dtype == 1 ? 2 : 1;
+  QES_HIP(hipMemcpyAsync(B->w, v0, B->total * cw * sizeof(double), hipMemcpyDeviceToDevice, st));
+  QES_HIP(hipStreamSynchronize(st));
+  std::vector<int> all(B->nvp, 1);
+  if (batch_set_active(B, all.data())) return -1;
+  hipLaunchKernelGGL(bzero_int_kernel, dim3((B->nvp + 255) / 256), dim3(256), 0, st, B->d_j, B->nvp);
+  enqueue_bdot(B, B->w, B->w, 1, B->d_active, st);
+  if (B->dtype == 1)
+    hipLaunchKernelGGL(badvance_kernel<true>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks,
+                       B->d_active, B->w, B->v, B->vp, B->scal, 1);
+  else
+    hipLaunchKernelGGL(badvance_kernel<false>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks,
+                       B->d_active, B->w, B->v, B->vp, B->scal, 1);
+  QES_HIP(hipGetLastError());
+  QES_HIP(hipStreamSynchronize(st));
+  QES_HIP(hipEventDestroy(ev));
+  return 0;
+}
+
+int batch_run(Batch* B, int nsteps, int use_graph) {
+  hipStream_t st = B->stream;
+  if (nsteps <= 0) return 0;
+  if (use_graph) {
+    if (!B->exec) {
+      QES_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_bstep(B, st);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
+      B->graph = g;
+      QES_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+    }
+    for (int s = 0; s < nsteps; ++s) QES_HIP(hipGraphLaunch(B->exec, st));
+  } else {
+    for (int s = 0; s < nsteps; ++s)
+      if (enqueue_bstep(B, st)) return -1;
+  }
+  QES_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+int batch_get(Batch* B, double* alpha, double* beta, int* steps) {
+  if (alpha) QES_HIP(hipMemcpy(alpha, B->alpha, sizeof(double) * B->nvp * B->maxs, hipMemcpyDeviceToHost));
+  if (beta) QES_HIP(hipMemcpy(beta, B->beta, sizeof(double) * B->nvp * B->maxs, hipMemcpyDeviceToHost));
+  if (steps) QES_HIP(hipMemcpy(steps, B->d_j, sizeof(int) * B->nvp, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int batch_matvec(Batch* B, const void* x, void* y, hipStream_t st) {
+  return enqueue_bmatvec(B, (const double*)x, (double*)y, nullptr, nullptr, st);
+}
+
+int batch_set_option(Batch* B, const std::string& name, int64_t value) {
+  if (name == "tile_order") B->tile_order = (int)value;
+  else if (name == "grid_mult") B->grid_mult = (int)std::max<int64_t>(1, value);
+  else if (name == "block_rows") B->block_rows = (int)std::max<int64_t>(1, std::min<int64_t>(value, 1 << 20));
+  else if (name == "block_table") B->block_table = value ? 1 : 0;
+  else return fail("unknown batch option " + name + " (tile_order, grid_mult, block_rows, block_table)");
+  batch_drop_graph(B);
+  if (batch_upload_probs(B)) return -1;
+  return batch_build_tiles(B);
+}
+
+int batch_ground_states(Batch* B, double tol, int check_every, double* e0, int* steps_out) {
+  const int V = B->nvp;
+  std::vector<double> a((size_t)V * B->maxs), b((size_t)V * B->maxs);
+  std::vector<int> steps(V), active(V, 1);
+  std::vector<double> prev(V, 1e300), cur(V, 0.0);
+  std::vector<int64_t> dim(V);
+  for (int p = 0; p < B->nprob; ++p)
+    for (int r = 0; r < B->nrhs[p]; ++r) dim[B->vp0[p] + r] = B->ops[p]->basis->n;
+  if (check_every < 1) check_every = 10;
+  int done_steps = 0;
+  while (done_steps < B->maxs) {
+    const int ns = std::min(check_every, B->maxs - done_steps);
+    if (batch_run(B, ns, 1)) return -1;
+    done_steps += ns;
+    if (batch_get(B, a.data(), b.data(), steps.data())) return -1;
+    bool any = false;
+    for (int p = 0; p < V; ++p) {
+      if (!active[p]) continue;
+      const int k = steps[p];
+      const double* ap = a.data() + (size_t)p * B->maxs;
+      const double* bp = b.data() + (size_t)p * B->maxs;
+      cur[p] = tridiag_lowest(ap, bp, k);
+      bool brk = false;
+      for (int i = 0; i < k; ++i)
+        if (!(std::fabs(bp[i]) > 1e-14 * std::max(1.0, std::fabs(cur[p])))) brk = true;
+      if (brk || k >= dim[p] || std::fabs(cur[p] - prev[p]) < tol * std::max(1.0, std::fabs(cur[p])) ||
+          k >= B->maxs) {
+        active[p] = 0;
+      } else {
+        any = true;
+      }
+      prev[p] = cur[p];
+    }
+    if (batch_set_active(B, active.data())) return -1;
+    if (!any) break;
+  }
+  if (batch_get(B, nullptr, nullptr, steps.data())) return -1;
+  for (int p = 0; p < V; ++p) {
+    if (e0) e0[p] = cur[p];
+    if (steps_out) steps_out[p] = steps[p];
+  }
+  return 0;
+}
+
+}  // namespace qes
+
+using namespace qes;
+
+struct qes_batch_s {
+  Batch* B;
+};
+
+extern "C" {
+
+int qes_batch_create(const qes_op_t* ops, int32_t nprob, int32_t max_steps, int32_t dtype, qes_batch_t* out) {
+  return qes_batch_create_blocks(ops, nprob, nullptr, nullptr, nullptr, 0, max_steps, dtype, out);
+}
+
+int qes_batch_create_blocks(const qes_op_t* ops, int32_t nprob, const qes_op_t* block_ops, const int32_t* block_nrhs,
+                            const void* const* block_diag_dev, int32_t nblock, int32_t max_steps, int32_t dtype,
+                            qes_batch_t* out) {
+  if (!out || (nprob > 0 && !ops) || (nblock > 0 && (!block_ops || !block_nrhs || !block_diag_dev)))
+    return fail("null argument");
+  std::vector<Op*> v(nprob);
+  for (int i = 0; i < nprob; ++i) {
+    if (!ops[i]) return fail("null operator in batch");
+    v[i] = qes_op_internal(ops[i]);
+  }
+  std::vector<BlockSpec> bl(nblock);
+  for (int i = 0; i < nblock; ++i) {
+    if (!block_ops[i]) return fail("null block operator in batch");
+    bl[i] = BlockSpec{qes_op_internal(block_ops[i]), block_nrhs[i], (const double*)block_diag_dev[i]};
+  }
+  Batch* B = nullptr;
+  if (batch_create(v.data(), nprob, bl.data(), nblock, max_steps, dtype, &B)) return -1;
+  auto* h = new qes_batch_s();
+  h->B = B;
+  *out = h;
+  return 0;
+}
+
+int qes_batch_offsets(qes_batch_t b, int64_t* offsets) {
+  if (!b || !offsets) return fail("null argument");
+  for (size_t i = 0; i < b->B->offs.size(); ++i) offsets[i] = b->B->offs[i];
+  return 0;
+}
+
+int qes_batch_start(qes_batch_t b, const void* v0_dev, void* stream) {
+  if (!b || !v0_dev) return fail("null argument");
+  return batch_start(b->B, v0_dev, (hipStream_t)stream);
+}
+
+int qes_batch_run(qes_batch_t b, int32_t nsteps, int32_t use_graph) {
+  if (!b) return fail("null argument");
+  return batch_run(b->B, nsteps, use_graph);
+}
+
+int qes_batch_get(qes_batch_t b, double* alpha_host, double* beta_host, int32_t* steps_host) {
+  if (!b) return fail("null argument");
+  return batch_get(b->B, alpha_host, beta_host, steps_host);
+}
+
+int qes_batch_set_active(qes_batch_t b, const int32_t* active_host) {
+  if (!b || !active_host) return fail("null argument");
+  return batch_set_active(b->B, active_host);
+}
+
+int qes_batch_matvec(qes_batch_t b, const void* x_dev, void* y_dev, void* stream) {
+  if (!b || !x_dev || !y_dev) return fail("null argument");
+  return batch_matvec(b->B, x_dev, y_dev, (hipStream_t)stream);
+}
+
+int qes_batch_set_option(qes_batch_t b, const char* name, int64_t value) {
+  if (!b || !name) return fail("null argument");
+  return batch_set_option(b->B, name, value);
+}
+
+int qes_batch_ground_states(qes_batch_t b, double tol, int32_t check_every, double* e0_host, int32_t* steps_host) {
+  if (!b) return fail("null argument");
+  return batch_ground_states(b->B, tol, check_every, e0_host, steps_host);
+}
+
+int qes_batch_destroy(qes_batch_t b) {
+  if (!b) return 0;
+  batch_destroy(b->B);
+  delete b;
+  return 0;
+}
+
+}  // extern "C"

@@ -196,6 +196,10 @@ struct qes_lanczos_s {
   Lanczos* L;
 };
 
+namespace qes {
+Op* qes_op_internal(qes_op_t h) { return &h->op; }
+}  // namespace qes
+
 extern "C" {
 
 const char* qes_last_error(void) { return g_err.c_str(); }
@@ -452,6 +456,11 @@ int qes_matvec(qes_op_t h, const void* x_dev, void* y_dev, int64_t nbatch, int32
 int qes_matvec_couplings(qes_op_t h, const void* x_dev, void* y_dev, int32_t dtype, void* stream) {
   if (!h) return fail("null operator");
   return matvec_couplings(&h->op, x_dev, y_dev, dtype, (hipStream_t)stream);
+}
+
+int qes_op_diagonal(qes_op_t h, void* re_dev, void* im_dev, void* stream) {
+  if (!h || !re_dev) return fail("null argument");
+  return op_diagonal(&h->op, (double*)re_dev, (double*)im_dev, (hipStream_t)stream);
 }
 
 int qes_op_count_offdiag(qes_op_t h, int64_t* e_off) {

@@ -92,6 +92,7 @@ int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int 
            hipStream_t st);
 int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_t st);
 int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st);
+int op_diagonal(const Op* op, double* dre, double* dim, hipStream_t st);
 int build_plan(Op* op, int canon_override);   // -1: automatic
 int pick_canon_public(const QesGroup& G);
 void free_plan(Op* op);

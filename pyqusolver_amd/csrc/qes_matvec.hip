@@ -20,193 +20,9 @@
 #include <vector>
 
 #include "qes_internal.h"
+#include "qes_rows.h"
 
 namespace qes {
-
-__device__ __forceinline__ double sel4(const double* t, int k) {
-  double v0 = t[0], v1 = t[1], v2 = t[2], v3 = t[3];
-  return k == 0 ? v0 : (k == 1 ? v1 : (k == 2 ? v2 : v3));
-}
-
-__device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks, int remap) {
-  if (!remap) return bid;
-  const int64_t nx = 8;
-  int64_t per = nblocks / nx, rem = nblocks % nx;
-  int64_t xcd = bid % nx, idx = bid / nx;
-  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-}
-
-// Everything a lane needs to enumerate the off-diagonal elements of a row.
-// All run-time-indexed tables live in LDS (see make_gv).
-struct RowCtx {
-  GV G;
-  QesLookup L;
-  int nclass, n_gen;
-  const int32_t* s_cls_d;       // LDS, nclass
-  const uint64_t* s_cls_pm;     // LDS, nclass
-  const QesTermGroup* groups;   // diag then generic off
-  int n_diag;
-  const double* s_chr;          // LDS
-  const double* s_chi;
-  const double* s_ex;           // LDS exchange values
-  const uint32_t* s_lut;        // LDS row LUT
-  int uniform;                  // bond order of the direct-lookup pass
-  int direct;                   // run the direct-lookup pass
-};
-
-// bytes of dynamic LDS used by load_ctx before the per-op tables
-#define QES_CTX_FIXED_LDS (QES_MAX_TX * 8 + QES_MAX_CLASSES * 8 + QES_MAX_CLASSES * 4 + 32)
-
-template <typename W, int CK>
-__device__ __forceinline__ Canon<W> canon_dispatch(const RowCtx& C, W s) {
-  if (CK == QES_CANON_CYCLIC) {
-    if (sizeof(W) * 8 == 32 || sizeof(W) * 8 == 64) {
-      if (C.G.ns == (int)(sizeof(W) * 8)) return canonicalize_cyclic<W, true>(C.G, s);
-    }
-    return canonicalize_cyclic<W, false>(C.G, s);
-  } else if (CK == QES_CANON_ROWS) {
-    return canonicalize_rows<W>(C.G, C.s_lut, s);
-  }
-  return canonicalize<W>(C.G, s);
-}
-
-// Visit every valid off-diagonal element of row r: f(b, fr, fi, coupling)
-// with (fr, fi) = conj(val) * chi(g*) * N_b (the caller divides by N_a).
-template <typename W, bool CPLX, int CK, class F>
-__device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
-  const int ns = C.G.ns;
-  const W mask = (W)C.G.mask;
-  auto emit = [&](W sp, double vr, double vi, int cpl) {
-    const Canon<W> cn = canon_dispatch<W, CK>(C, sp);
-    int cnt_unused = 0;
-    const int64_t b = C.L.keyc ? lookup_packed(C.L, (uint64_t)cn.rep, &cnt_unused) : lookup_rep<W>(C.L, cn.rep);
-    if (b < 0) return;
-    const double nb = sqrt((double)cn.count);
-    const double cr = C.s_chr[cn.g];
-    double fr, fi;
-    if (CPLX) {
-      const double ci = C.s_chi[cn.g];
-      fr = (vr * cr + vi * ci) * nb;
-      fi = (vr * ci - vi * cr) * nb;
-    } else {
-      fr = vr * cr * nb;
-      fi = 0.0;
-    }
-    f(b, fr, fi, cpl);
-  };
-  // Exchange classes.  Pass 1 walks the bonds in wave-uniform order (all
-  // lanes of a wavefront handle the same bond together, so their targets and
-  // key probes cluster in a few cache lines) and looks the flipped state up
-  // directly: most flipped states are representatives already, and a hit on
-  // the packed key array gives the stabiliser size (character of the
-  // identity = 1).  Pass 2 canonicalises only the flipped states that missed.
-  for (int c = 0; c < C.nclass; ++c) {
-    const int d = C.s_cls_d[c];
-    const W pm = (W)C.s_cls_pm[c];
-    const W act = (r ^ rotl_ns<W>(r, d, ns, mask)) & pm;
-    const bool direct = C.L.keyc && C.direct;
-    W fail = direct ? (W)0 : act;
-    if (direct) {
-      W m = C.uniform ? pm : act;   // uniform order, or each lane's active bonds
-      while (m) {
-        const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
-        m &= m - 1;
-        if (!((act >> p) & 1)) continue;
-        int p1 = p - d;
-        if (p1 < 0) p1 += ns;
-        const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
-        int cnt = 0;
-        const int64_t b = lookup_packed(C.L, (uint64_t)sp, &cnt);
-        if (b < 0) {
-          fail |= (W)1 << p;
-          continue;
-        }
-        const int b0 = (int)((r >> p) & 1);
-        const double* e = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
-        const double nb = sqrt((double)cnt);
-        f(b, e[0] * nb, CPLX ? -e[1] * nb : 0.0, c * ns + p);
-      }
-    }
-    while (fail) {
-      const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
-      fail &= fail - 1;
-      int p1 = p - d;
-      if (p1 < 0) p1 += ns;
-      const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
-      const int b0 = (int)((r >> p) & 1);
-      const double* e = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
-      emit(sp, e[0], CPLX ? e[1] : 0.0, c * ns + p);
-    }
-  }
-  // generic groups
-  for (int t = C.n_diag; t < C.n_diag + C.n_gen; ++t) {
-    const QesTermGroup& g = C.groups[t];
-    const int k = term_bits<W>(g, r);
-    const double vr = sel4(g.re, k);
-    const double vi = CPLX ? sel4(g.im, k) : 0.0;
-    if (vr == 0.0 && vi == 0.0) continue;
-    const W sp = r ^ (W)g.mask;
-    if (C.G.n_up >= 0 && popc<W>(sp) != C.G.n_up) continue;
-    if (C.G.z2 != 0 && ((popc<W>(sp) & 1) ? -1 : 1) != C.G.z2) continue;
-    emit(sp, vr, vi, -1 - t);
-  }
-}
-
-// Load the per-block tables into LDS; returns the context.
-template <bool CPLX>
-__device__ __forceinline__ RowCtx load_ctx(const QesGroup& G, const QesLookup& L, const QesPlan& P,
-                                           const double* __restrict__ chi_re, const double* __restrict__ chi_im,
-                                           const QesTermGroup* __restrict__ groups, int n_diag,
-                                           const double* __restrict__ exval, const uint32_t* __restrict__ lut,
-                                           char* smem) {
-  uint64_t* s_rowlo = (uint64_t*)smem;
-  uint64_t* s_cls_pm = s_rowlo + QES_MAX_TX;
-  int32_t* s_cls_d = (int32_t*)(s_cls_pm + QES_MAX_CLASSES);
-  double* s_chr = (double*)(smem + QES_CTX_FIXED_LDS);
-  const int gs = G.ntx * G.nty * G.nbase;
-  double* s_chi = s_chr + gs;
-  double* s_ex = s_chi + gs;
-  const int nex = P.nclass * G.ns * 4;
-  uint32_t* s_lut = (uint32_t*)(s_ex + nex);
-  const int nlut = (P.canon == QES_CANON_ROWS) ? (1 << P.lut_bits) : 0;
-  RowCtx C;
-  C.G = make_gv(G, s_rowlo);
-  if (threadIdx.x < QES_MAX_CLASSES) {
-#pragma unroll
-    for (int c = 0; c < QES_MAX_CLASSES; ++c)
-      if ((int)threadIdx.x == c) {
-        s_cls_pm[c] = P.cls[c].pmask;
-        s_cls_d[c] = P.cls[c].d;
-      }
-  }
-  for (int i = threadIdx.x; i < gs; i += blockDim.x) {
-    s_chr[i] = chi_re[i];
-    s_chi[i] = CPLX ? chi_im[i] : 0.0;
-  }
-  for (int i = threadIdx.x; i < nex; i += blockDim.x) s_ex[i] = exval[i];
-  for (int i = threadIdx.x; i < nlut; i += blockDim.x) s_lut[i] = lut[i];
-  __syncthreads();
-  C.L = L;
-  C.nclass = P.nclass;
-  C.n_gen = P.n_gen;
-  C.uniform = P.uniform;
-  C.direct = P.direct;
-  C.s_cls_d = s_cls_d;
-  C.s_cls_pm = s_cls_pm;
-  C.groups = groups;
-  C.n_diag = n_diag;
-  C.s_chr = s_chr;
-  C.s_chi = s_chi;
-  C.s_ex = s_ex;
-  C.s_lut = s_lut;
-  return C;
-}
-
-template <typename W>
-__device__ __forceinline__ W load_rep(const QesLookup& L, int64_t a) {
-  if (sizeof(W) == 4 && L.keys32) return (W)L.keys32[a];
-  return (W)L.keys64[a];
-}
 
 // NB = number of right-hand sides per lane (columns j0 .. j0+NB-1 of an
 // (n, ldx) row-major block).
@@ -659,6 +475,38 @@ int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st) {
   QES_HIP(hipStreamSynchronize(st));
   QES_HIP(hipFree(d));
   *e_off = (int64_t)h;
+  return 0;
+}
+
+// H_aa for every row (real part; imaginary part into dim when non-null)
+template <typename W>
+__global__ void diagonal_kernel(const QesTermGroup* __restrict__ groups, int n_diag, const uint64_t* __restrict__ reps,
+                                int64_t n, double* __restrict__ dre, double* __restrict__ dim) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const W r = (W)reps[a];
+  double vr = 0.0, vi = 0.0;
+  for (int t = 0; t < n_diag; ++t) {
+    const QesTermGroup& g = groups[t];
+    const int k = term_bits<W>(g, r);
+    vr += sel4(g.re, k);
+    vi += sel4(g.im, k);
+  }
+  dre[a] = vr;
+  if (dim) dim[a] = vi;
+}
+
+int op_diagonal(const Op* op, double* dre, double* dim, hipStream_t st) {
+  const Basis* B = op->basis;
+  if (B->n == 0) return 0;
+  dim3 grid((unsigned)((B->n + 255) / 256));
+  if (B->group.ns <= 32)
+    hipLaunchKernelGGL(diagonal_kernel<uint32_t>, grid, dim3(256), 0, st, op->d_gen, op->n_diag, B->reps, B->n, dre,
+                       dim);
+  else
+    hipLaunchKernelGGL(diagonal_kernel<uint64_t>, grid, dim3(256), 0, st, op->d_gen, op->n_diag, B->reps, B->n, dre,
+                       dim);
+  QES_HIP(hipGetLastError());
   return 0;
 }
 

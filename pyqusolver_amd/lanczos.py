@@ -34,12 +34,14 @@ class LanczosResult:
     residual_bound: float = float("nan")
 
 
-def start_vector(n: int, complex_: bool, seed: int = 0) -> np.ndarray:
-    """Seeded N(0,1) start vector (complex: re + 1j*im)."""
-    rng = np.random.default_rng(seed)
+def start_vector(n: int, complex_: bool, seed: int = 0) -> torch.Tensor:
+    """Seeded N(0,1) start vector generated on the device (complex: re + 1j*im)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(int(seed))
+    re = torch.randn(n, generator=g, dtype=torch.float64, device="cuda")
     if complex_:
-        return rng.standard_normal(n) + 1j * rng.standard_normal(n)
-    return rng.standard_normal(n)
+        return torch.complex(re, torch.randn(n, generator=g, dtype=torch.float64, device="cuda"))
+    return re
 
 
 class DeviceLanczos:
