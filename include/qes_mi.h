@@ -135,9 +135,14 @@ int qes_op_set_option(qes_op_t op, const char* name, int64_t value);
 int qes_op_get_plan(qes_op_t op, int32_t* canon, int32_t* n_classes, int32_t* n_generic);
 int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_host, int64_t n, int64_t* mismatches);
 
-/* Representative lookup used by the matvec: 0 sorted keys + bucket table,
- * 1 open-addressing hash table (default when the index fits beside the key). */
+/* Representative lookup used by the matvec: 0 sorted keys (32-bit when
+ * ns <= 32; default there) + bucket table, 1 open-addressing hash table,
+ * 2 packed 64-bit keys (key | stabiliser size) + bucket table (default for
+ * 33..56 sites without intrinsic phases). */
 int qes_basis_set_lookup(qes_basis_t b, int32_t mode);
+/* Rebuild the bucket table for about 2^rows_log2 representatives per bucket
+ * (-4..20; negative: several buckets per representative; default -3). */
+int qes_basis_set_bucket(qes_basis_t b, int32_t rows_log2);
 
 /*
  * Device Lanczos (FP64).  Replaces the eigensolver that

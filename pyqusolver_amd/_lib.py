@@ -64,6 +64,7 @@ SIGNATURES = {
     "qes_op_get_plan": [_P, _PI32, _PI32, _PI32],
     "qes_basis_canon_check": [_P, _I32, _PU64, _I64, _PI64],
     "qes_basis_set_lookup": [_P, _I32],
+    "qes_basis_set_bucket": [_P, _I32],
     "qes_lanczos_create": [_P, _I32, _I32, _I32, C.POINTER(_P)],
     "qes_lanczos_start": [_P, _P, _P],
     "qes_lanczos_run": [_P, _I32, _I32, _I32],

@@ -249,9 +249,32 @@ int build_basis(Basis* B, hipStream_t st) {
   }
   B->e_min = bitlen(lo_rep);
   B->e_max = bitlen(hi_rep);
+  if (build_bucket(B, B->bucket_rows_log2, st)) return -1;
+  if (G.ns <= QES_KEY_BITS && !G.has_phase && n > 0) {
+    QES_HIP(hipMalloc(&B->keyc, n * sizeof(uint64_t)));
+    B->use_packed = G.ns > 32;   // 32-bit keys are narrower below 33 sites
+    hipLaunchKernelGGL(keyc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B->reps, B->norms, n,
+                       B->keyc);
+    QES_HIP(hipGetLastError());
+    QES_HIP(hipStreamSynchronize(st));
+  }
+  B->use_hash = false;
+  return 0;
+}
+
+// Bucket table over (bit length, next mbits bits) of the representatives,
+// sized for about 2^rows_log2 representatives per bucket.
+int build_bucket(Basis* B, int rows_log2, hipStream_t st) {
+  const int64_t n = B->n;
   const int ne = B->e_max - B->e_min + 1;
+  // rows_log2 < 0: more buckets than representatives (2^-rows_log2 per rep)
+  const int64_t target = rows_log2 >= 0 ? std::max<int64_t>(n >> std::min(rows_log2, 40), 1)
+                                        : std::max<int64_t>(n << std::min(-rows_log2, 4), 1);
   int mbits = 0;
-  while (mbits < 24 && ((int64_t)ne << (mbits + 1)) <= std::max<int64_t>(n / 2, 1)) ++mbits;
+  while (mbits < 24 && ((int64_t)ne << (mbits + 1)) <= target) ++mbits;   // bucket_of keys stay 32-bit
+  if (B->bucket) hipFree(B->bucket);
+  B->bucket = nullptr;
+  B->bucket_rows_log2 = rows_log2;
   B->mbits = mbits;
   B->nbucket = (int64_t)ne << mbits;
   QES_HIP(hipMalloc(&B->bucket, (B->nbucket + 1) * sizeof(uint32_t)));
@@ -259,14 +282,6 @@ int build_basis(Basis* B, hipStream_t st) {
                      B->e_min, B->nbucket, B->bucket);
   QES_HIP(hipGetLastError());
   QES_HIP(hipStreamSynchronize(st));
-  if (G.ns <= QES_KEY_BITS && !G.has_phase && n > 0) {
-    QES_HIP(hipMalloc(&B->keyc, n * sizeof(uint64_t)));
-    hipLaunchKernelGGL(keyc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B->reps, B->norms, n,
-                       B->keyc);
-    QES_HIP(hipGetLastError());
-    QES_HIP(hipStreamSynchronize(st));
-  }
-  B->use_hash = false;
   return 0;
 }
 

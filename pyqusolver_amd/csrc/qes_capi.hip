@@ -401,13 +401,22 @@ int qes_op_get_plan(qes_op_t h, int32_t* canon, int32_t* n_classes, int32_t* n_g
 
 int qes_basis_set_lookup(qes_basis_t b, int32_t mode) {
   if (!b) return fail("null basis");
+  if (mode < 0 || mode > 2)
+    return fail("mode must be 0 (sorted keys + bucket table), 1 (hash table) or 2 (packed keys + bucket table)");
   if (mode == 1 && !b->b.hash) {
     if (build_hash(&b->b, nullptr)) return -1;
     if (!b->b.hash) return fail("no hash table for this basis (index does not fit beside the key)");
   }
-  if (mode != 0 && mode != 1) return fail("mode must be 0 (sorted keys + bucket table) or 1 (hash table)");
+  if (mode == 2 && !b->b.keyc) return fail("no packed keys for this basis (ns > 56 or intrinsic phases)");
   b->b.use_hash = mode == 1;
+  b->b.use_packed = mode == 2;
   return 0;
+}
+
+int qes_basis_set_bucket(qes_basis_t b, int32_t rows_log2) {
+  if (!b) return fail("null basis");
+  if (rows_log2 < -4 || rows_log2 > 20) return fail("rows_log2 must be in -4..20");
+  return build_bucket(&b->b, rows_log2, nullptr);
 }
 
 int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_host, int64_t n, int64_t* mismatches) {

@@ -256,7 +256,7 @@ struct QesLookup {
   // empty slots are all ones; 2^hbits slots, linear probing
   const uint64_t* hash;
   int32_t hbits;
-  int32_t pad;
+  int32_t packed;           // off-diagonal lookups probe keyc instead of keys32/keys64
   // packed keys (ns <= 56): key | (|Stab(rep)| - 1) << 56, sorted by key;
   // a hit on the packed array also yields the stabiliser size
   const uint64_t* keyc;

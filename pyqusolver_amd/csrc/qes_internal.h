@@ -46,7 +46,7 @@ struct Basis {
     L.ns = group.ns;
     L.hash = use_hash ? hash : nullptr;
     L.hbits = hbits;
-    L.pad = 0;
+    L.packed = (use_packed && keyc) ? 1 : 0;
     L.keyc = keyc;
     return L;
   }
@@ -54,6 +54,11 @@ struct Basis {
   uint64_t* hash = nullptr;   // device hash table (representative -> index)
   int hbits = 0;
   bool use_hash = false;
+  // Defaults from the MI355X lookup sweep (J1-J2 chain L=32, 6x6 square,
+  // honeycomb 3x4): a bucket table with ~8 buckets per representative (most
+  // lookups resolve with no search step) and the narrowest key array.
+  bool use_packed = false;    // probe the packed keys (ns <= 56, no phases)
+  int bucket_rows_log2 = -3;  // about 2^k representatives per bucket
 };
 
 // Compiled operator (Hamiltonian) bound to a basis.
@@ -83,6 +88,7 @@ struct Op {
 // launchers (qes_basis.hip)
 int build_basis(Basis* B, hipStream_t st);
 int build_hash(Basis* B, hipStream_t st);
+int build_bucket(Basis* B, int rows_log2, hipStream_t st);
 int map_states(const Basis* B, const uint64_t* d_states, int64_t n, int64_t* d_idx, int32_t* d_g,
                int32_t* d_cnt, hipStream_t st);
 

@@ -62,7 +62,8 @@ __device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
   auto emit = [&](W sp, double vr, double vi, int cpl) {
     const Canon<W> cn = canon_dispatch<W, CK>(C, sp);
     int cnt_unused = 0;
-    const int64_t b = C.L.keyc ? lookup_packed(C.L, (uint64_t)cn.rep, &cnt_unused) : lookup_rep<W>(C.L, cn.rep);
+    const int64_t b = (C.L.packed && !C.L.hash) ? lookup_packed(C.L, (uint64_t)cn.rep, &cnt_unused)
+                                                : lookup_rep<W>(C.L, cn.rep);
     if (b < 0) return;
     const double nb = sqrt((double)cn.count);
     const double cr = C.s_chr[cn.g];

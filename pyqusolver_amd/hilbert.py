@@ -122,9 +122,15 @@ class HilbertSpace:
                                                     C.byref(bad)), "qes_basis_canon_check")
         return int(bad.value)
 
-    def set_lookup(self, mode: str = "hash"):
-        """Representative lookup of the matvec: 'hash' or 'sorted'."""
-        _lib.check(_lib.lib().qes_basis_set_lookup(self._h, {"sorted": 0, "hash": 1}[mode]), "qes_basis_set_lookup")
+    LOOKUP_MODES = {"sorted": 0, "hash": 1, "packed": 2}
+
+    def set_lookup(self, mode: str = "packed", bucket_rows_log2: Optional[int] = None):
+        """Representative lookup of the matvec: 'sorted' (32/64-bit keys),
+        'hash' or 'packed' (64-bit key | stabiliser size); optionally rebuild
+        the bucket table for about 2^bucket_rows_log2 representatives per bucket."""
+        _lib.check(_lib.lib().qes_basis_set_lookup(self._h, self.LOOKUP_MODES[mode]), "qes_basis_set_lookup")
+        if bucket_rows_log2 is not None:
+            _lib.check(_lib.lib().qes_basis_set_bucket(self._h, int(bucket_rows_log2)), "qes_basis_set_bucket")
 
     def device_arrays(self):
         """Device pointers (reps uint64, norms float64) owned by this space."""

@@ -77,6 +77,15 @@ def test_matvec_plan_variants_agree(case):
     y_brute = H.matvec_device(x)
     assert torch.equal(y_fast, y_brute)
     H.set_canonicalizer("auto")
+    # every representative lookup finds the same index
+    from pyqusolver_amd._lib import QesError
+    for mode in ("sorted", "hash", "packed"):
+        for rows_log2 in (-4, 0, 3, 6):
+            try:
+                hs.set_lookup(mode, bucket_rows_log2=rows_log2)
+            except QesError:
+                continue   # mode not available for this basis
+            assert torch.equal(H.matvec_device(x), y_fast), (mode, rows_log2)
 
 
 def test_per_coupling_outputs_sum_to_matvec():
