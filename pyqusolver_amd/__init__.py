@@ -22,4 +22,10 @@ def __getattr__(name):
     if name in ("J1J2Model", "XXZ", "HeisenbergKitaev"):
         from . import models
         return getattr(models, name)
+    if name in ("DifferentiableHamiltonian", "HamiltonianMatvec"):
+        from . import autograd
+        return getattr(autograd, name)
+    if name in ("BatchedLanczos",):
+        from . import batched
+        return getattr(batched, name)
     raise AttributeError(name)

@@ -31,6 +31,18 @@ def _arr(v, ns):
     return np.asarray(v, dtype=float)
 
 
+def _cv(v) -> float:
+    """Reference value of one coupling id for the differentiable path: the
+    scalar (or uniform array) value it was compiled at; 1.0 (relative scale)
+    for site-dependent arrays and for absent or zero couplings."""
+    if v is None:
+        return 1.0
+    a = np.atleast_1d(np.asarray(v, dtype=float))
+    if a.size and np.all(a == a.flat[0]) and a.flat[0] != 0.0:
+        return float(a.flat[0])
+    return 1.0
+
+
 def _nz(v, i=None) -> bool:
     """_ADD_CONDITION (Algebra/hamil.py:123-206) for scalars/arrays."""
     if v is None:
@@ -79,6 +91,7 @@ class J1J2Model(Hamiltonian):
                     self.add(sxx, jj, True, [i, int(j)], coupling=cpl)
                     self.add(syy, jj, True, [i, int(j)], coupling=cpl)
                     self.add(szz, jj, False, [i, int(j)], coupling=cpl)
+        self.coupling_values = np.array([_cv(J1), _cv(J2), _cv(hx), _cv(hy), _cv(hz)])[: self.n_coupling_ids]
 
 
 class XXZ(Hamiltonian):
@@ -115,6 +128,7 @@ class XXZ(Hamiltonian):
                 cz = self._jz[i] * phase
                 if _nz(cz):
                     self.add(szz, -cz, False, [i, int(j)], coupling=1)
+        self.coupling_values = np.array([_cv(self._jxy), _cv(self._jz), _cv(hx), _cv(hz)])[: self.n_coupling_ids]
 
 
 class HeisenbergKitaev(Hamiltonian):
@@ -177,3 +191,10 @@ class HeisenbergKitaev(Hamiltonian):
                         self.add(syy, phase * cs * J, True, [i, nei], coupling=1)
                     if _nz(kk[1]):
                         self.add(kk[0], -phase * cs * kk[1], kk[0] is not szz, [i, nei], coupling=0)
+        hmag = float(np.sqrt(sum(float(np.real(c)) ** 2 for c in (hx, hy, hz))))
+        kval = _cv(kx) if (kx == ky == kz) else 1.0
+        if split_couplings:
+            vals = [kval, _cv(J), hmag if hmag > 0 else 1.0]
+        else:
+            vals = [1.0, 1.0, hmag if hmag > 0 else 1.0]   # K and J share the bond instructions
+        self.coupling_values = np.array(vals)[: self.n_coupling_ids]

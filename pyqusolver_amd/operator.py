@@ -143,6 +143,11 @@ class Hamiltonian:
         codes = np.asarray(self._instr_codes, dtype=np.int32)
         return sites, coeffs, codes
 
+    @property
+    def n_coupling_ids(self) -> int:
+        """Max coupling id + 1 over the instructions (= qes_op_n_couplings)."""
+        return max(self._instr_coupling, default=0) + 1
+
     def set_coupling_scale(self, scales: Sequence[float]):
         """Rescale the coefficients of each coupling id (used by autograd)."""
         self._coupling_scale = np.asarray(scales, dtype=np.float64)

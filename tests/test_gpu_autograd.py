@@ -1,0 +1,145 @@
+"""Differentiable matvec y = H(J) v and the gradients built on it (GPU).
+
+Pins: torch.autograd.gradcheck (numerical Jacobian) for the hand-written
+backward; linearity in J; dE0/dJ through the Lanczos recurrence against
+Hellmann-Feynman and central differences (step 1e-5); the Krylov propagator
+against the dense matrix exponential of the oracle's reduced matrix; the
+gradient of |<psi0|exp(-iHt)|phi>|^2 against central differences.
+"""
+import numpy as np
+import pytest
+
+from oracle import qes_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _honeycomb(lx=2, ly=3, hdir=(1.0, 1.0, 1.0), h=0.05, K=-1.0, J=0.2):
+    from pyqusolver_amd import HoneycombLattice
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import HeisenbergKitaev
+    lat = HoneycombLattice(lx, ly)
+    hs = HilbertSpace(lattice=lat, sym_gen={"translation": {"kx": 0, "ky": 0}})
+    n = np.asarray(hdir, dtype=float)
+    hx, hy, hz = h * n / np.linalg.norm(n)
+    ham = HeisenbergKitaev(lat, K=K, J=J, hilbert_space=hs, hx=hx, hy=hy, hz=hz, split_couplings=True)
+    return lat, hs, ham
+
+
+def _dh(ham):
+    from pyqusolver_amd.autograd import DifferentiableHamiltonian
+    return DifferentiableHamiltonian(ham)
+
+
+def test_forward_matches_matvec_and_is_linear_in_J():
+    import torch
+    lat, hs, ham = _honeycomb()
+    H = _dh(ham)
+    J0 = H.J0.clone()
+    assert np.allclose(J0.numpy(), [-1.0, 0.2, 0.05])
+    v = torch.randn(hs.nh, dtype=torch.complex128, device="cuda")
+    y = H(J0, v)
+    ref = ham.matvec_device(v)
+    assert torch.allclose(y, ref, rtol=1e-13, atol=1e-13 * ref.abs().max().item())
+    y2 = H(2.0 * J0, v)
+    assert torch.allclose(y2, 2.0 * y, rtol=1e-13, atol=1e-13 * ref.abs().max().item())
+    # only the field coupling switched on: field instructions alone
+    y_h = H(torch.tensor([0.0, 0.0, 0.05], dtype=torch.float64), v)
+    Y = ham.matvec_couplings_device(v)
+    assert torch.allclose(y_h, Y[2], rtol=1e-13, atol=1e-15)
+
+
+def test_backward_gradcheck():
+    import torch
+    from pyqusolver_amd.autograd import HamiltonianMatvec
+    lat, hs, ham = _honeycomb()
+    g = torch.Generator(device="cuda").manual_seed(3)
+    s = torch.tensor([1.0, 1.3, 0.7], dtype=torch.float64, requires_grad=True)
+    v = (torch.randn(hs.nh, dtype=torch.complex128, device="cuda", generator=g)).requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda s_, v_: HamiltonianMatvec.apply(s_, v_, ham), (s, v),
+                                    eps=1e-6, atol=1e-7, rtol=1e-6, fast_mode=True)
+
+
+def test_backward_gradcheck_real_chain():
+    import torch
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.autograd import HamiltonianMatvec
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import J1J2Model
+    lat = ChainLattice(12)
+    hs = HilbertSpace(lattice=lat, sym_gen={"translation": 0, "parity": {"x": 1}}, global_syms=[get_u1_sym(lat, 6)])
+    ham = J1J2Model(lat, J1=-1.0, J2=-0.4, hilbert_space=hs)
+    s = torch.tensor([0.9, 1.2], dtype=torch.float64, requires_grad=True)
+    v = torch.randn(hs.nh, dtype=torch.float64, device="cuda").requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda s_, v_: HamiltonianMatvec.apply(s_, v_, ham), (s, v),
+                                    eps=1e-6, atol=1e-7, rtol=1e-6)
+
+
+def test_dE0_dJ_lanczos_vs_hellmann_feynman_vs_central_difference():
+    import torch
+    from pyqusolver_amd.autograd import central_difference, hellmann_feynman, lanczos_energy
+    from pyqusolver_amd.lanczos import lanczos_ground_state, start_vector
+    lat, hs, ham = _honeycomb()
+    H = _dh(ham)
+    J = H.J0.clone().requires_grad_(True)
+    v0 = start_vector(hs.nh, True, seed=0)
+    m = 90
+    e0 = lanczos_energy(H, J, v0, m)
+    e0.backward()
+    g_lanczos = J.grad.clone()
+    # reference E0 from the oracle's reduced matrix
+    olat = O.OLattice("honeycomb", 2, 3)
+    b = O.scan_basis(O.Group(olat, [O.Generator("tx", 0), O.Generator("ty", 0)]))
+    c, s, co = O.terms_heisenberg_kitaev(olat, -1.0, 0.2, *(0.05 * np.ones(3) / np.sqrt(3))).arrays()
+    e_ref = np.linalg.eigvalsh(O.reduced_dense(b, lat.ns, c, s, co, True))[0]
+    assert abs(e0.item() - e_ref) <= 1e-10 * abs(e_ref)
+    res = lanczos_ground_state(ham, tol=1e-14, maxiter=300, seed=1, reorth=True, compute_eigvec=True)
+    g_hf = hellmann_feynman(H, res.eigenvectors)
+    g_fd = central_difference(lambda Jp: lanczos_energy(H, Jp, v0, m), J.detach(), h=1e-5)
+    for name, g in (("HF", g_hf), ("FD", g_fd)):
+        np.testing.assert_allclose(g_lanczos.numpy(), g.numpy(), rtol=2e-6, atol=1e-8, err_msg=name)
+    # Hellmann-Feynman sum rule: H is homogeneous of degree 1 in J
+    assert abs(float((g_hf * J.detach()).sum()) - e_ref) <= 1e-9 * abs(e_ref)
+
+
+def test_krylov_propagator_against_dense_expm():
+    import torch
+    from scipy.linalg import expm
+    from pyqusolver_amd.autograd import krylov_propagate
+    lat, hs, ham = _honeycomb()
+    H = _dh(ham)
+    rng = np.random.default_rng(7)
+    phi = rng.standard_normal(hs.nh) + 1j * rng.standard_normal(hs.nh)
+    phi /= np.linalg.norm(phi)
+    olat = O.OLattice("honeycomb", 2, 3)
+    b = O.scan_basis(O.Group(olat, [O.Generator("tx", 0), O.Generator("ty", 0)]))
+    c, s, co = O.terms_heisenberg_kitaev(olat, -1.0, 0.2, *(0.05 * np.ones(3) / np.sqrt(3))).arrays()
+    Hd = O.reduced_dense(b, lat.ns, c, s, co, True)
+    exact = expm(-1j * Hd) @ phi
+    # m = 30 (the differentiable workload) carries the Krylov truncation
+    # error; it falls super-exponentially with m
+    for m, tol in ((30, 1e-7), (50, 1e-11)):
+        out = krylov_propagate(H, H.J0, torch.from_numpy(phi).cuda(), t=1.0, m=m).cpu().numpy()
+        assert np.linalg.norm(out - exact) <= tol, m
+        assert abs(np.linalg.norm(out) - 1.0) <= 1e-12
+
+
+def test_krylov_overlap_gradient_vs_central_difference():
+    import torch
+    from pyqusolver_amd.autograd import central_difference, krylov_overlap
+    from pyqusolver_amd.lanczos import lanczos_ground_state
+    lat, hs, ham = _honeycomb()
+    H = _dh(ham)
+    res = lanczos_ground_state(ham, tol=1e-13, maxiter=300, seed=1, reorth=True, compute_eigvec=True)
+    psi0 = res.eigenvectors / torch.linalg.vector_norm(res.eigenvectors)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    phi = torch.randn(hs.nh, dtype=torch.complex128, device="cuda", generator=g)
+    phi = phi / torch.linalg.vector_norm(phi) + 0.5 * psi0    # non-trivial overlap
+    J = H.J0.clone().requires_grad_(True)
+    for seg in (None, 1, 30):
+        J.grad = None
+        f = krylov_overlap(H, J, psi0, phi, t=1.0, m=30, segment=seg)
+        f.backward()
+        g_ad = J.grad.clone()
+        g_fd = central_difference(lambda Jp: krylov_overlap(H, Jp, psi0, phi, t=1.0, m=30), J.detach(), h=1e-5)
+        np.testing.assert_allclose(g_ad.numpy(), g_fd.numpy(), rtol=1e-6, atol=1e-9)
