@@ -1,14 +1,20 @@
 #!/usr/bin/env python
 """Benchmark: symmetric-sector H.v matvec throughput and time-to-E0 on one MI355X.
 
-Workload (default): J1-J2 Heisenberg chain L=32, PBC, J1=1, J2=0.5 (reference
-constructor J1J2Model(J1=-1, J2=-0.5)), Sz=0, k=0, spin flip +1:
-N = 9,393,119 reduced states, float64.  A "step" is one H.v on a seeded
-N(0,1) vector already resident in HBM.  After the timed steps a Lanczos
-ground-state solve (full FP64 reorthogonalisation, up to 300 steps) reports
-time-to-E0 against the exact Majumdar-Ghosh energy E0 = -3L/8.
+Workload (default, --config plain): J1-J2 Heisenberg chain L=32, PBC, J1=1,
+J2=0.5 (reference constructor J1J2Model(J1=-1, J2=-0.5)), Sz=0, k=0, spin
+flip +1: N = 9,393,119 reduced states, float64.  A "step" is one H.v on a
+seeded N(0,1) vector already resident in HBM.  After the timed steps a
+Lanczos ground-state solve (full FP64 reorthogonalisation, up to 300 steps)
+reports time-to-E0 against the exact Majumdar-Ghosh energy E0 = -3L/8.
 
-Prints one JSON line (rank 0).  Only one GPU is used; --gpus N > 1 exits.
+Other workloads: --config large (J1-J2 6x6 square, k=(0,0), flip +1,
+N = 126,056,625; Lanczos without reorthogonalisation), --config batched
+(XXZ L=24 all momentum sectors + 256 random-field L=20 realisations, one
+batched Lanczos against a sector-by-sector sweep), --config grad
+(Kitaev-Heisenberg honeycomb 24 sites: dE0/dJ and Krylov gradients).
+
+Prints one JSON line.  Only one GPU is used; --gpus N > 1 exits.
 """
 from __future__ import annotations
 
@@ -21,6 +27,14 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per matvec dispatch from rocprofv3 FETCH_SIZE + WRITE_SIZE
+# (KiB per dispatch, FETCH scaled x2.0 and WRITE x1.0 by a 1 GiB device copy
+# measured in the same runs); filled from the committed profiles
+TRAFFIC = {
+    # profiles/matvec_kernel_chain32_j1j2.md: 9,992,541 KiB x 2.0 + 73,384 KiB x 1.0
+    "plain": 9992541 * 1024 * 2 + 73384 * 1024,
+}
 
 
 def parse():
@@ -36,10 +50,11 @@ def parse():
     p.add_argument("--quiet", action="store_true")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="also run a plain device copy of MIB MiB (profiler counter calibration)")
-    p.add_argument("--config", default="plain", choices=["plain", "batched", "grad"],
+    p.add_argument("--config", default="plain", choices=["plain", "batched", "grad", "large"],
                    help="plain: L=32 J1-J2 chain matvec + Lanczos (default); batched: XXZ L=24 all momentum "
                         "sectors + random-field L=20 realisations, batched vs sector-by-sector; grad: "
-                        "Kitaev-Heisenberg honeycomb 24 sites, dE0/dJ and Krylov time-evolution gradients")
+                        "Kitaev-Heisenberg honeycomb 24 sites, dE0/dJ and Krylov time-evolution gradients; large: "
+                        "J1-J2 6x6 square k=(0,0) flip+ matvec + Lanczos (no reorthogonalisation)")
     p.add_argument("--realisations", type=int, default=256)
     return p.parse_args()
 
@@ -181,13 +196,13 @@ def log(msg, quiet=False):
         print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(L, J2, reps, norms, sample, n):
+def cpu_baseline(wl, J2, reps, norms, sample, n):
     """CPU port of the reference push-form kernel (oracle/qes_oracle.c) on the
     first ``sample`` columns, single thread; extrapolated to matvecs/s."""
     from oracle import qes_oracle as O
     from oracle import qes_oracle_c as OC
-    lat = O.OLattice("chain", L)
-    grp = O.Group(lat, [O.Generator("tx", 0), O.Generator("Px", 1)], n_up=L // 2)
+    lat = O.OLattice(wl["kind"], wl["lx"], wl["ly"])
+    grp = O.Group(lat, [O.Generator(g, k) for g, k in wl["oracle_gens"]], n_up=wl["n_up"])
     t = O.terms_j1j2(lat, -1.0, -J2)
     c, s, co = t.arrays()
     x = np.random.default_rng(0).standard_normal(len(reps))
@@ -198,44 +213,54 @@ def cpu_baseline(L, J2, reps, norms, sample, n):
     dt = time.perf_counter() - t0
     cols_per_s = k1 / dt
     return {"value": cols_per_s / n, "unit": "matvec/s", "cores": 1, "kind": "port",
-            "sample": f"push-form columns 0..{k1} of the same L={L} sector ({dt:.1f} s single-thread, "
-                      f"{cols_per_s:.0f} columns/s); reference group action per element and binary search "
-                      f"in place of the 2^{L}-entry dense map"}
+            "sample": f"push-form columns 0..{k1} of the same {wl['name']} sector ({dt:.1f} s single-thread, "
+                      f"{cols_per_s:.0f} columns/s, extrapolated to N={n}); reference group action per element and "
+                      f"binary search in place of the 2^{wl['ns']}-entry dense map"}
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU sharding is described in "
-              "DESIGN.md and not run.", file=sys.stderr)
-        sys.exit(2)
-    if a.config == "batched":
-        return bench_batched(a)
-    if a.config == "grad":
-        return bench_grad(a)
+WORKLOADS = {
+    # J1-J2 chain, k=0, flip +1 (the plain-at-scale workload)
+    "plain": lambda a: {"kind": "chain", "lx": a.L, "ly": 1, "ns": a.L, "n_up": a.L // 2,
+                        "sym": {"translation": 0, "parity": {"x": 1}},
+                        "oracle_gens": [("tx", 0), ("Px", 1)],
+                        "name": f"J1-J2 chain L={a.L} PBC J1=1 J2={a.J2} Sz=0 k=0 flip+",
+                        "e_exact": (-3.0 * a.L / 8.0) if abs(a.J2 - 0.5) < 1e-15 else None,
+                        "reorth": True},
+    # J1-J2 6x6 square, k=(0,0), flip +1: no full reorthogonalisation (one
+    # basis vector is 1 GB; 300 of them do not fit next to the operator)
+    "large": lambda a: {"kind": "square", "lx": 6, "ly": 6, "ns": 36, "n_up": 18,
+                        "sym": {"translation": {"kx": 0, "ky": 0}, "parity": {"x": 1}},
+                        "oracle_gens": [("tx", 0), ("ty", 0), ("Px", 1)],
+                        "name": f"J1-J2 square 6x6 PBC J1=1 J2={a.J2} Sz=0 k=(0,0) flip+",
+                        "e_exact": None, "reorth": False},
+}
+
+
+def bench_plain(a):
     import torch
-    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd import ChainLattice, SquareLattice, get_u1_sym
     from pyqusolver_amd.hilbert import HilbertSpace
     from pyqusolver_amd.models import J1J2Model
 
-    L, J2 = a.L, a.J2
-    lat = ChainLattice(L)
+    wl = WORKLOADS[a.config](a)
+    J2 = a.J2
+    lat = ChainLattice(wl["lx"]) if wl["kind"] == "chain" else SquareLattice(wl["lx"], wl["ly"])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hs = HilbertSpace(lattice=lat, sym_gen={"translation": 0, "parity": {"x": 1}},
-                      global_syms=[get_u1_sym(lat, L // 2)])
+    hs = HilbertSpace(lattice=lat, sym_gen=wl["sym"], global_syms=[get_u1_sym(lat, wl["n_up"])])
     torch.cuda.synchronize()
     t_basis = time.perf_counter() - t0
     n = hs.nh
     H = J1J2Model(lat, J1=-1.0, J2=-J2, hilbert_space=hs)
     e_off = H.count_offdiag()
-    log(f"basis N={n} built in {t_basis:.3f}s, E_off={e_off}", a.quiet)
+    log(f"{wl['name']}: basis N={n} built in {t_basis:.3f}s, E_off={e_off}", a.quiet)
 
     # seeded N(0,1) inputs resident on the device before the timed region
-    xs = [torch.from_numpy(np.random.default_rng(i).standard_normal(n)).to("cuda") for i in range(max(a.steps, 1))]
+    nx = min(max(a.steps, 1), 10 if n < 5e7 else 2)
+    xs = [torch.from_numpy(np.random.default_rng(i).standard_normal(n)).to("cuda") for i in range(nx)]
     y = torch.empty(n, dtype=torch.float64, device="cuda")
     for i in range(a.warmup):
-        H.matvec_device(xs[i % len(xs)], out=y)
+        H.matvec_device(xs[i % nx], out=y)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
@@ -244,7 +269,7 @@ def main():
     start.record(stream)
     for i in range(a.steps):
         evs[i][0].record(stream)
-        H.matvec_device(xs[i], out=y)
+        H.matvec_device(xs[i % nx], out=y)
         evs[i][1].record(stream)
     stop.record(stream)
     torch.cuda.synchronize()
@@ -255,6 +280,7 @@ def main():
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     log(f"matvec {ms_per_step:.3f} ms/step (kernel {kern_ms:.3f} ms), B_alg {b_alg/1e9:.3f} GB, "
         f"{achieved:.1f} GB/s", a.quiet)
+    del xs
 
     if a.calibrate_copy > 0:
         # plain copy of a known size: its FETCH_SIZE / WRITE_SIZE per dispatch
@@ -269,29 +295,33 @@ def main():
 
     lanczos = None
     if not a.no_lanczos:
-        e_exact = -3.0 * L / 8.0 if abs(J2 - 0.5) < 1e-15 else None
+        e_exact = wl["e_exact"]
+        reorth = wl["reorth"]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        e0 = H.diagonalize(method="lanczos", k=1, tol=1e-12, maxiter=a.lanczos_steps, seed=0, reorth=True)[0]
+        e0 = H.diagonalize(method="lanczos", k=1, tol=1e-12, maxiter=a.lanczos_steps, seed=0, reorth=reorth,
+                           store_basis=reorth)[0]
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         r = H.last_lanczos
+        target = e_exact if e_exact is not None else float(e0)
         hit = None
-        if e_exact is not None:
-            for steps, e in r.history:
-                if abs(e - e_exact) <= 1e-10 * abs(e_exact):
-                    hit = steps
-                    break
+        for steps, e in r.history:
+            if abs(e - target) <= 1e-10 * abs(target):
+                hit = steps
+                break
         lanczos = {"E0": float(e0), "E0_exact": e_exact,
                    "rel_err": (abs(e0 - e_exact) / abs(e_exact)) if e_exact else None,
-                   "steps": int(r.steps), "steps_to_1e-10": hit, "seconds": round(dt, 4),
-                   "reorth": "full FP64, classical Gram-Schmidt x2"}
-        log(f"lanczos E0={e0:.14f} steps={r.steps} {dt:.2f}s", a.quiet)
+                   "steps": int(r.steps), "steps_to_1e-10": hit,
+                   "steps_to_1e-10_reference": "exact E0" if e_exact is not None else "final E0 (self)",
+                   "residual_bound": r.residual_bound, "converged": bool(r.converged), "seconds": round(dt, 4),
+                   "reorth": "full FP64, classical Gram-Schmidt x2" if reorth else "none (three-term recurrence)"}
+        log(f"lanczos E0={e0:.14f} steps={r.steps} {dt:.2f}s residual<={r.residual_bound:.2e}", a.quiet)
 
     cpu = None
     if a.cpu_sample > 0:
         try:
-            cpu = cpu_baseline(L, J2, hs.representative_list, hs.normalization, a.cpu_sample, n)
+            cpu = cpu_baseline(wl, J2, hs.representative_list, hs.normalization, a.cpu_sample, n)
         except Exception as ex:  # baseline is reported, never required
             cpu = {"value": None, "error": str(ex)}
 
@@ -308,15 +338,28 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"J1-J2 chain L={L} PBC J1=1 J2={J2} Sz=0 k=0 flip+ (plain at scale)",
-                   "N": n, "E_off": e_off, "group_order": hs.group.order, "basis_build_s": round(t_basis, 4)},
+        "config": {"workload": wl["name"], "N": n, "E_off": e_off, "group_order": hs.group.order,
+                   "basis_build_s": round(t_basis, 4)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(a.config),
                      "kernel": "matvec_kernel", "kernel_ms": kern_ms, "bytes_alg": b_alg},
         "cpu_baseline": cpu,
         "lanczos": lanczos,
     }
     print(json.dumps(out), flush=True)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); this build runs no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+    if a.config == "batched":
+        return bench_batched(a)
+    if a.config == "grad":
+        return bench_grad(a)
+    return bench_plain(a)
 
 
 if __name__ == "__main__":

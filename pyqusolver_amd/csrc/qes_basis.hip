@@ -164,14 +164,18 @@ __device__ __forceinline__ int64_t rel_bucket(uint64_t v, int mbits, int e_min) 
   return (int64_t)k - ((int64_t)e_min << mbits);
 }
 
-// bucket[k] = first index i with key(reps[i]) >= k
+// bucket[k] = first index i with key(reps[i]) >= k (key is monotone in the
+// representative, so a lower-bound search per bucket; k = nbucket gives n)
 __global__ void bucket_kernel(const uint64_t* __restrict__ reps, int64_t n, int mbits, int e_min,
                               int64_t nbucket, uint32_t* __restrict__ bucket) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > n) return;
-  int64_t kprev = (i == 0) ? -1 : rel_bucket(reps[i - 1], mbits, e_min);
-  int64_t kcur = (i == n) ? nbucket : rel_bucket(reps[i], mbits, e_min);
-  for (int64_t k = kprev + 1; k <= kcur && k <= nbucket; ++k) bucket[k] = (uint32_t)i;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > nbucket) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (rel_bucket(reps[mid], mbits, e_min) < k) lo = mid + 1; else hi = mid;
+  }
+  bucket[k] = (uint32_t)lo;
 }
 
 static int bitlen(uint64_t v) { return v ? 64 - __builtin_clzll(v) : 0; }
@@ -278,8 +282,8 @@ int build_bucket(Basis* B, int rows_log2, hipStream_t st) {
   B->mbits = mbits;
   B->nbucket = (int64_t)ne << mbits;
   QES_HIP(hipMalloc(&B->bucket, (B->nbucket + 1) * sizeof(uint32_t)));
-  hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, st, B->reps, n, mbits,
-                     B->e_min, B->nbucket, B->bucket);
+  hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((B->nbucket + 1 + 255) / 256)), dim3(256), 0, st, B->reps, n,
+                     mbits, B->e_min, B->nbucket, B->bucket);
   QES_HIP(hipGetLastError());
   QES_HIP(hipStreamSynchronize(st));
   return 0;
