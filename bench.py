@@ -280,6 +280,17 @@ def bench_plain(a):
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     log(f"matvec {ms_per_step:.3f} ms/step (kernel {kern_ms:.3f} ms), B_alg {b_alg/1e9:.3f} GB, "
         f"{achieved:.1f} GB/s", a.quiet)
+    # host-buffer path (matvec_fun on numpy arrays): PCIe copies in and out
+    host_ms = None
+    if n < 5e7:
+        xh = xs[0].cpu().numpy()
+        H.matvec_fun(xh)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            H.matvec_fun(xh)
+        host_ms = (time.perf_counter() - t0) / 3 * 1e3
+        log(f"matvec_fun on host numpy arrays: {host_ms:.2f} ms/call (PCIe-inclusive)", a.quiet)
     del xs
 
     if a.calibrate_copy > 0:
@@ -340,6 +351,7 @@ def bench_plain(a):
         "data": "synthetic",
         "config": {"workload": wl["name"], "N": n, "E_off": e_off, "group_order": hs.group.order,
                    "basis_build_s": round(t_basis, 4)},
+        "host_buffer_ms_per_matvec": host_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(a.config),
                      "kernel": "matvec_kernel", "kernel_ms": kern_ms, "bytes_alg": b_alg},

@@ -58,6 +58,9 @@ def build_pair(kind, lx, ly, gens, n_up, model, z2=None):
     elif model == "j1j2":
         H = J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=hs)
         t = O.terms_j1j2(olat, -1.0, -0.5)
+    elif model == "heis":
+        H = J1J2Model(lat, J1=-1.0, J2=0.0, hilbert_space=hs)
+        t = O.terms_j1j2(olat, -1.0, 0.0)
     elif model == "j1j2_field":
         H = J1J2Model(lat, J1=-1.0, J2=-0.3, hilbert_space=hs, hx=0.2, hz=0.1)
         t = O.terms_j1j2(olat, -1.0, -0.3, hx=0.2, hz=0.1)
@@ -272,3 +275,22 @@ def test_index_map_on_random_states_chain32():
     pos = np.searchsorted(reps, best)
     ok = (pos < len(reps)) & (reps[np.minimum(pos, len(reps) - 1)] == best)
     np.testing.assert_array_equal(idx, np.where(ok, pos, -1))
+
+
+def test_golden_fixtures():
+    """The HIP path against the committed fixtures: representatives bit-exact,
+    normalisations to 1e-14, H x to 1e-12 relative L2, E0 (device Lanczos with
+    full reorthogonalisation) to 1e-10 relative."""
+    from test_golden import CASES as GCASES, DATA
+    for case in GCASES:
+        name = case["name"]
+        hs, H, grp, b, t = build_pair(case["kind"], case["lx"], case["ly"], [tuple(g) for g in case["gens"]],
+                                      case["n_up"], {"heis": "heis"}.get(case["model"], case["model"]))
+        np.testing.assert_array_equal(hs.representative_list.astype(np.uint64), DATA[f"{name}/reps"], err_msg=name)
+        np.testing.assert_allclose(hs.normalization, DATA[f"{name}/norms"], rtol=1e-14, atol=0, err_msg=name)
+        yg = DATA[f"{name}/y"]
+        y = H.matvec_fun(DATA[f"{name}/x"])
+        assert np.linalg.norm(y - yg) <= MATVEC_TOL * np.linalg.norm(yg), name
+        e_ref = DATA[f"{name}/e0"][0]
+        e0 = H.diagonalize(method="lanczos", k=1, tol=1e-14, maxiter=min(300, hs.nh), seed=0, reorth=True)[0]
+        assert abs(e0 - e_ref) <= E0_TOL * abs(e_ref), (name, e0, e_ref)
