@@ -46,6 +46,7 @@ def parse():
     p.add_argument("--J2", type=float, default=0.5)
     p.add_argument("--lanczos-steps", type=int, default=300)
     p.add_argument("--no-lanczos", action="store_true")
+    p.add_argument("--no-assembled", action="store_true", help="skip the assembled-operator timing")
     p.add_argument("--cpu-sample", type=int, default=2000, help="columns timed for the CPU baseline (0: skip)")
     p.add_argument("--quiet", action="store_true")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -280,6 +281,41 @@ def bench_plain(a):
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     log(f"matvec {ms_per_step:.3f} ms/step (kernel {kern_ms:.3f} ms), B_alg {b_alg/1e9:.3f} GB, "
         f"{achieved:.1f} GB/s", a.quiet)
+    # assembled operator (table built once, streamed per matvec): same H.v,
+    # reported beside the matrix-free value
+    assembled = None
+    if not a.no_assembled:
+        free, _ = torch.cuda.mem_get_info()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        try:
+            tb = H.assemble(True, max_gb=min(64.0, 0.25 * free / 1e9))
+        except Exception as ex:   # table larger than the budget: reported, not required
+            assembled = {"error": str(ex)}
+        else:
+            torch.cuda.synchronize()
+            t_build = time.perf_counter() - t0
+            for i in range(a.warmup):
+                H.matvec_device(xs[i % nx], out=y)
+            ev2 = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+            torch.cuda.synchronize()
+            for i in range(a.steps):
+                ev2[i][0].record(stream)
+                H.matvec_device(xs[i % nx], out=y)
+                ev2[i][1].record(stream)
+            torch.cuda.synchronize()
+            asm_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev2]))
+            H.assemble(False)
+            b_asm = tb + e_off * 8 + n * 16
+            assembled = {"ms_per_matvec": asm_ms, "matvec_per_s": 1000.0 / asm_ms, "table_bytes": tb,
+                         "build_s": round(t_build, 4),
+                         "achieved_alg_GBs": b_alg / (asm_ms * 1e-3) / 1e9,
+                         "frac_alg": b_alg / (asm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_streamed": b_asm,
+                         "frac_streamed": b_asm / (asm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "kernel": "sell_matvec_kernel"}
+            log(f"assembled: table {tb/1e9:.2f} GB built in {t_build:.3f}s, {asm_ms:.3f} ms/matvec", a.quiet)
+
     # host-buffer path (matvec_fun on numpy arrays): PCIe copies in and out
     host_ms = None
     if n < 5e7:
@@ -326,6 +362,7 @@ def bench_plain(a):
                    "steps": int(r.steps), "steps_to_1e-10": hit,
                    "steps_to_1e-10_reference": "exact E0" if e_exact is not None else "final E0 (self)",
                    "residual_bound": r.residual_bound, "converged": bool(r.converged), "seconds": round(dt, 4),
+                   "assembled_operator": bool(r.assembled),
                    "reorth": "full FP64, classical Gram-Schmidt x2" if reorth else "none (three-term recurrence)"}
         log(f"lanczos E0={e0:.14f} steps={r.steps} {dt:.2f}s residual<={r.residual_bound:.2e}", a.quiet)
 
@@ -352,6 +389,7 @@ def bench_plain(a):
         "config": {"workload": wl["name"], "N": n, "E_off": e_off, "group_order": hs.group.order,
                    "basis_build_s": round(t_basis, 4)},
         "host_buffer_ms_per_matvec": host_ms,
+        "assembled": assembled,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(a.config),
                      "kernel": "matvec_kernel", "kernel_ms": kern_ms, "bytes_alg": b_alg},

@@ -117,6 +117,20 @@ int qes_op_n_couplings(qes_op_t op, int32_t* n);
 /* Diagonal H_aa of every row into device arrays (imaginary part optional). */
 int qes_op_diagonal(qes_op_t op, void* re_dev, void* im_dev, void* stream);
 
+/*
+ * Assembled operator: tabulate every row of H once on the device (sliced
+ * ELLPACK: 64 rows per slice, column index + value per entry, diagonal
+ * first), after which qes_matvec with nbatch == 1 streams the table instead
+ * of canonicalising and looking up neighbours.  The device counterpart of
+ * the reference's explicit sparse build (Algebra/Hilbert/matrix_builder.py:151-206,
+ * :911-1030), for repeated application (Lanczos).  enable = 0 frees it; the
+ * build fails if the table would exceed max_gb (<= 0: 64 GB default).  Any
+ * change of the operator's plan (qes_op_set_option / set_canonicalizer)
+ * drops the table.
+ */
+int qes_op_assemble(qes_op_t op, int32_t enable, double max_gb, void* stream);
+int qes_op_assembly_info(qes_op_t op, int64_t* entries, int64_t* bytes);
+
 /* Number of off-diagonal gathers of one matvec (E_off) -- stats pass. */
 int qes_op_count_offdiag(qes_op_t op, int64_t* e_off);
 

@@ -391,6 +391,21 @@ int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
   return build_plan(&op, op.plan.canon == pick_canon_public(op.basis->group) ? -1 : op.plan.canon);
 }
 
+int qes_op_assemble(qes_op_t h, int32_t enable, double max_gb, void* stream) {
+  if (!h) return fail("null operator");
+  if (max_gb > 0) h->op.asm_max_bytes = max_gb * 1e9;
+  return op_assemble(&h->op, enable, (hipStream_t)stream);
+}
+
+int qes_op_assembly_info(qes_op_t h, int64_t* entries, int64_t* bytes) {
+  if (!h) return fail("null operator");
+  const Op& op = h->op;
+  const int64_t e = op.asm_col ? op.asm_entries : 0;
+  if (entries) *entries = e;
+  if (bytes) *bytes = e * (4 + 8 * (op.asm_vc ? 2 : 1)) + (op.asm_col ? op.asm_slices * 12 : 0);
+  return 0;
+}
+
 int qes_op_get_plan(qes_op_t h, int32_t* canon, int32_t* n_classes, int32_t* n_generic) {
   if (!h) return fail("null operator");
   if (canon) *canon = h->op.plan.canon;

@@ -83,6 +83,14 @@ struct Op {
   int xcd_remap = 0;
   int bond_order_uniform = 0;
   int direct_lookup = 0;
+  // assembled (sliced ELLPACK) table, see qes_assemble.hip
+  int64_t* asm_off = nullptr;
+  int32_t* asm_len = nullptr;
+  int32_t* asm_col = nullptr;
+  double* asm_val = nullptr;
+  int64_t asm_entries = 0, asm_slices = 0;
+  int asm_vc = 0;
+  double asm_max_bytes = 64e9;
 };
 
 // launchers (qes_basis.hip)
@@ -100,6 +108,10 @@ int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_
 int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st);
 int op_diagonal(const Op* op, double* dre, double* dim, hipStream_t st);
 int build_plan(Op* op, int canon_override);   // -1: automatic
+// assembled operator (qes_assemble.hip)
+int op_assemble(Op* op, int enable, hipStream_t st);
+void free_assembly(Op* op);
+int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int accumulate, hipStream_t st);
 int pick_canon_public(const QesGroup& G);
 void free_plan(Op* op);
 int canon_check(const Basis* B, int kind, const uint32_t* d_lut, const uint64_t* d_states, int64_t n,

@@ -268,6 +268,7 @@ static std::vector<uint32_t> row_lut(const QesGroup& G) {
 }
 
 void free_plan(Op* op) {
+  free_assembly(op);   // the table follows the plan's visit order
   if (op->d_exval) hipFree(op->d_exval);
   if (op->d_excpl) hipFree(op->d_excpl);
   if (op->d_lut) hipFree(op->d_lut);
@@ -410,6 +411,7 @@ int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int 
   if (!cplx && (op->is_complex || op->chi_complex))
     return fail("complex matrix elements (complex couplings or momentum sector) need complex128 vectors");
   if (nbatch < 1) return fail("nbatch must be >= 1");
+  if (op->asm_col && nbatch == 1) return assembled_matvec(op, (const double*)x, (double*)y, dtype, accumulate, st);
   const bool w32 = B->group.ns <= 32;
   const double* xd = (const double*)x;
   double* yd = (double*)y;

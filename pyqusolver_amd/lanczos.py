@@ -32,6 +32,7 @@ class LanczosResult:
     eigenvectors: Optional[torch.Tensor] = None
     seconds: float = 0.0
     residual_bound: float = float("nan")
+    assembled: bool = False                          # the solve streamed an assembled table
 
 
 def start_vector(n: int, complex_: bool, seed: int = 0) -> torch.Tensor:

@@ -216,6 +216,21 @@ class Hamiltonian:
         names = {v: k for k, v in self.CANON_KINDS.items()}
         return {"canonicalizer": names[c.value], "exchange_classes": nc.value, "generic_groups": ng.value}
 
+    def assemble(self, enable: bool = True, max_gb: float = 0.0):
+        """Tabulate H on the device (sliced ELLPACK, built once by the same
+        row visitor) so that single-vector matvecs stream the table instead
+        of canonicalising; ``enable=False`` frees it.  Returns the table size
+        in bytes."""
+        _lib.check(_lib.lib().qes_op_assemble(self._device_op(), 1 if enable else 0, float(max_gb),
+                                              _lib.stream_ptr()), "qes_op_assemble")
+        return self.assembly_bytes
+
+    @property
+    def assembly_bytes(self) -> int:
+        e, b = C.c_int64(), C.c_int64()
+        _lib.check(_lib.lib().qes_op_assembly_info(self._device_op(), C.byref(e), C.byref(b)))
+        return int(b.value)
+
     def count_offdiag(self) -> int:
         v = C.c_int64()
         _lib.check(_lib.lib().qes_op_count_offdiag(self._device_op(), C.byref(v)), "qes_op_count_offdiag")
@@ -306,18 +321,38 @@ class Hamiltonian:
     # ---- diagonalisation ------------------------------------------------
     def diagonalize(self, method: str = "lanczos", k: int = 1, tol: float = 1e-10, maxiter: int = 300,
                     v0=None, seed: int = 0, reorth: bool = True, store_basis: Optional[bool] = None,
-                    compute_eigvec: bool = False, check_every: int = 10, **kwargs):
+                    compute_eigvec: bool = False, check_every: int = 10, assemble: Optional[bool] = None,
+                    **kwargs):
         """Lowest eigenvalue(s) by device Lanczos (hamil.py:1809-2114 with
         method='lanczos').  Only E0 is claimed to the requested tolerance;
-        the other k-1 Ritz values are returned as computed."""
+        the other k-1 Ritz values are returned as computed.
+
+        ``assemble``: tabulate H for the solve (see ``assemble``); None picks
+        it when the table fits in a quarter of the free device memory (at
+        most 64 GB) and N >= 65536.  A table built here is freed afterwards."""
         from .lanczos import lanczos_ground_state
         if method not in ("lanczos", "auto"):
             raise NotImplementedError("only method='lanczos' is implemented on the device")
         if self.nh == 0:
             self._eig_val = np.zeros(0)
             return self._eig_val
-        res = lanczos_ground_state(self, k=k, tol=tol, maxiter=maxiter, v0=v0, seed=seed, reorth=reorth,
-                                   store_basis=store_basis, compute_eigvec=compute_eigvec, check_every=check_every)
+        built = False
+        if self.assembly_bytes == 0 and (assemble or (assemble is None and self.nh >= 65536)):
+            free, _ = torch.cuda.mem_get_info()
+            try:
+                self.assemble(True, max_gb=min(64.0, 0.25 * free / 1e9))
+                built = True
+            except _lib.QesError:
+                if assemble:
+                    raise
+        try:
+            res = lanczos_ground_state(self, k=k, tol=tol, maxiter=maxiter, v0=v0, seed=seed, reorth=reorth,
+                                       store_basis=store_basis, compute_eigvec=compute_eigvec,
+                                       check_every=check_every)
+        finally:
+            if built:
+                self.assemble(False)
+        res.assembled = built or self.assembly_bytes > 0
         self.last_lanczos = res
         self._eig_val = res.eigenvalues
         self._eig_vec = res.eigenvectors
