@@ -363,7 +363,9 @@ def bench_plain(a):
                    "steps_to_1e-10_reference": "exact E0" if e_exact is not None else "final E0 (self)",
                    "residual_bound": r.residual_bound, "converged": bool(r.converged), "seconds": round(dt, 4),
                    "assembled_operator": bool(r.assembled),
-                   "reorth": "full FP64, classical Gram-Schmidt x2" if reorth else "none (three-term recurrence)"}
+                   "reorth": (f"full FP64 classical Gram-Schmidt against all stored vectors, DGKS second pass "
+                              f"({r.second_passes} of {r.steps} steps)") if reorth else
+                             "none (three-term recurrence)"}
         log(f"lanczos E0={e0:.14f} steps={r.steps} {dt:.2f}s residual<={r.residual_bound:.2e}", a.quiet)
 
     cpu = None

@@ -164,8 +164,10 @@ int qes_basis_set_bucket(qes_basis_t b, int32_t rows_log2);
  * (Algebra/Hamil/hamil_diag_engine.py:176-300, called from
  * Algebra/hamil.py:2063-2081 with matvec=matvec_fun).
  * store_basis keeps all Lanczos vectors on the device (needed for full
- * reorthogonalisation and for Ritz vectors); reorth != 0 applies classical
- * Gram-Schmidt twice against all stored vectors each step.
+ * reorthogonalisation and for Ritz vectors).  reorth 1: classical
+ * Gram-Schmidt against all stored vectors every step, with a second pass
+ * when the first removed more than half of |w|^2 (DGKS criterion); reorth 2:
+ * always two passes; 0: plain three-term recurrence.
  */
 int qes_lanczos_create(qes_op_t op, int32_t max_steps, int32_t store_basis, int32_t dtype, qes_lanczos_t* out);
 int qes_lanczos_start(qes_lanczos_t L, const void* v0_dev, void* stream);
@@ -173,6 +175,8 @@ int qes_lanczos_run(qes_lanczos_t L, int32_t nsteps, int32_t reorth, int32_t use
 int qes_lanczos_get(qes_lanczos_t L, double* alpha_host, double* beta_host, int32_t* steps_done);
 int qes_lanczos_ritz(qes_lanczos_t L, const double* coeffs_host, int32_t k, void* out_dev, void* stream);
 int qes_lanczos_destroy(qes_lanczos_t L);
+/* Number of second Gram-Schmidt passes run since the last start. */
+int qes_lanczos_stats(qes_lanczos_t L, int32_t* second_passes);
 
 /*
  * All-in-one ground state: run up to max_steps Lanczos steps from v0 (device),

@@ -73,6 +73,7 @@ SIGNATURES = {
     "qes_lanczos_get": [_P, _PD, _PD, _PI32],
     "qes_lanczos_ritz": [_P, _PD, _I32, _P, _P],
     "qes_lanczos_destroy": [_P],
+    "qes_lanczos_stats": [_P, _PI32],
     "qes_lanczos_ground_state": [_P, _P, _I32, _I32, _I32, C.c_double, _I32, _PD, _PD, _PI32, _PD, _P],
     "qes_batch_create": [C.POINTER(_P), _I32, _I32, _I32, C.POINTER(_P)],
     "qes_batch_create_blocks": [C.POINTER(_P), _I32, C.POINTER(_P), _PI32, C.POINTER(_P), _I32, _I32, _I32,

@@ -512,6 +512,14 @@ int qes_lanczos_run(qes_lanczos_t h, int32_t nsteps, int32_t reorth, int32_t use
   return lanczos_run(h->L, nsteps, reorth, use_graph);
 }
 
+int qes_lanczos_stats(qes_lanczos_t h, int32_t* second_passes) {
+  if (!h || !second_passes) return fail("null argument");
+  int c = 0;
+  if (lanczos_second_passes(h->L, &c)) return -1;
+  *second_passes = c;
+  return 0;
+}
+
 int qes_lanczos_get(qes_lanczos_t h, double* alpha_host, double* beta_host, int32_t* steps_done) {
   if (!h) return fail("null argument");
   if (steps_done) *steps_done = lanczos_steps_done(h->L);

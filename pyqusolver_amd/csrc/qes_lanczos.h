@@ -9,6 +9,7 @@ int lanczos_destroy(Lanczos* L);
 int lanczos_start(Lanczos* L, const void* v0, hipStream_t user);
 int lanczos_run(Lanczos* L, int nsteps, int reorth, int use_graph);
 int lanczos_get(Lanczos* L, double* alpha, double* beta);
+int lanczos_second_passes(Lanczos* L, int* count);
 int lanczos_combine(Lanczos* L, const double* c_host, int k, void* out, hipStream_t user);
 int lanczos_steps_done(const Lanczos* L);
 double tridiag_lowest(const double* a, const double* b, int k);

@@ -110,8 +110,9 @@ template <bool CPLX>
 __global__ void __launch_bounds__(RB) proj_partial_kernel(const double* __restrict__ V, int64_t ldv,
                                                           const double* __restrict__ w, int64_t n,
                                                           const int* __restrict__ d_j, double* __restrict__ part,
-                                                          int nchunk) {
+                                                          int nchunk, const int* __restrict__ gate) {
   __shared__ double sr[VB][RB / 64], si[VB][RB / 64];
+  if (gate && *gate == 0) return;
   const int j = *d_j;
   const int i0 = blockIdx.y * VB;
   if (i0 > j) return;
@@ -169,8 +170,10 @@ __global__ void __launch_bounds__(RB) proj_partial_kernel(const double* __restri
 
 // c_i = sum over chunks (one block per vector, i <= j)
 __global__ void __launch_bounds__(RB) proj_finish_kernel(const double* __restrict__ part, int nchunk,
-                                                         const int* __restrict__ d_j, double* __restrict__ coef) {
+                                                         const int* __restrict__ d_j, double* __restrict__ coef,
+                                                         const int* __restrict__ gate) {
   __shared__ double sr[RB], si[RB];
+  if (gate && *gate == 0) return;
   const int i = blockIdx.x;
   if (i > *d_j) return;
   double r = 0.0, im = 0.0;
@@ -198,8 +201,9 @@ __global__ void __launch_bounds__(RB) proj_finish_kernel(const double* __restric
 template <bool CPLX>
 __global__ void __launch_bounds__(RB) proj_update_kernel(const double* __restrict__ V, int64_t ldv,
                                                          double* __restrict__ w, int64_t n, const int* __restrict__ d_j,
-                                                         const double* __restrict__ coef) {
+                                                         const double* __restrict__ coef, const int* __restrict__ gate) {
   __shared__ double sc[2 * 1024];
+  if (gate && *gate == 0) return;
   const int j = *d_j;
   for (int i = threadIdx.x; i <= j && i < 1024; i += RB) {
     sc[2 * i] = coef[2 * i];
@@ -263,6 +267,15 @@ __global__ void __launch_bounds__(RB) scale_copy_kernel(const double* __restrict
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
+// DGKS criterion: a second Gram-Schmidt pass only when the first removed
+// more than half of |w|^2 (|w1| < |w0| / sqrt 2); s0, s1 = squared norms
+__global__ void dgks_flag_kernel(const double* __restrict__ s0, const double* __restrict__ s1, int* __restrict__ flag,
+                                 int* __restrict__ count) {
+  const int f = s1[0] < 0.5 * s0[0] ? 1 : 0;
+  *flag = f;
+  *count += f;
+}
+
 // psi = sum_i c_i V_i  (real coefficients c)
 template <bool CPLX>
 __global__ void __launch_bounds__(RB) combine_kernel(const double* __restrict__ V, int64_t ldv, int k,
@@ -288,6 +301,8 @@ struct Lanczos {
   double *v = nullptr, *vp = nullptr, *w = nullptr, *V = nullptr;
   double *alpha = nullptr, *beta = nullptr, *scal = nullptr, *part = nullptr, *coef = nullptr, *ppart = nullptr;
   int* d_j = nullptr;
+  double* dg = nullptr;    // squared norms before / after the first Gram-Schmidt pass
+  int* flag = nullptr;     // [0] second pass needed, [1] number of second passes run
   int nred = 0, nchunk = 0, nvec_blocks = 0;
   int steps_done = 0;
   int reorth_graph = -1;
@@ -323,19 +338,30 @@ static int enqueue_step(Lanczos* L, int reorth, hipStream_t st) {
   else
     hipLaunchKernelGGL(recur_kernel<false>, dim3(g), dim3(RB), 0, st, L->w, L->v, L->vp, L->alpha, L->beta, L->d_j, L->n);
   if (reorth && L->store) {
+    // reorth 1: second pass by the DGKS criterion; 2: always twice
+    if (reorth == 1) enqueue_dot(L, L->w, L->w, L->dg, 0, st);
     for (int pass = 0; pass < 2; ++pass) {
+      const int* gate = nullptr;
+      if (pass == 1 && reorth == 1) {
+        enqueue_dot(L, L->w, L->w, L->dg + 2, 0, st);
+        hipLaunchKernelGGL(dgks_flag_kernel, dim3(1), dim3(1), 0, st, L->dg, L->dg + 2, L->flag, L->flag + 1);
+        gate = L->flag;
+      }
       dim3 pg(L->nchunk, L->nvec_blocks);
       if (c)
         hipLaunchKernelGGL(proj_partial_kernel<true>, pg, dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->ppart,
-                           L->nchunk);
+                           L->nchunk, gate);
       else
         hipLaunchKernelGGL(proj_partial_kernel<false>, pg, dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->ppart,
-                           L->nchunk);
-      hipLaunchKernelGGL(proj_finish_kernel, dim3(L->maxv), dim3(RB), 0, st, L->ppart, L->nchunk, L->d_j, L->coef);
+                           L->nchunk, gate);
+      hipLaunchKernelGGL(proj_finish_kernel, dim3(L->maxv), dim3(RB), 0, st, L->ppart, L->nchunk, L->d_j, L->coef,
+                         gate);
       if (c)
-        hipLaunchKernelGGL(proj_update_kernel<true>, dim3(g), dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->coef);
+        hipLaunchKernelGGL(proj_update_kernel<true>, dim3(g), dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->coef,
+                           gate);
       else
-        hipLaunchKernelGGL(proj_update_kernel<false>, dim3(g), dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->coef);
+        hipLaunchKernelGGL(proj_update_kernel<false>, dim3(g), dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j,
+                           L->coef, gate);
     }
   }
   enqueue_dot(L, L->w, L->w, L->scal, 1, st);
@@ -375,6 +401,8 @@ int lanczos_create(Op* op, int max_steps, int store, int dtype, Lanczos** out) {
   QES_HIP(hipMalloc(&L->coef, sizeof(double) * 2 * L->maxv));
   QES_HIP(hipMalloc(&L->ppart, sizeof(double) * 2 * (size_t)L->nchunk * L->nvec_blocks * VB));
   QES_HIP(hipMalloc(&L->d_j, sizeof(int)));
+  QES_HIP(hipMalloc(&L->dg, sizeof(double) * 4));
+  QES_HIP(hipMalloc(&L->flag, sizeof(int) * 2));
   *out = L;
   return 0;
 }
@@ -394,6 +422,8 @@ int lanczos_destroy(Lanczos* L) {
   hipFree(L->coef);
   hipFree(L->ppart);
   hipFree(L->d_j);
+  hipFree(L->dg);
+  hipFree(L->flag);
   if (L->stream) hipStreamDestroy(L->stream);
   delete L;
   return 0;
@@ -412,6 +442,7 @@ int lanczos_start(Lanczos* L, const void* v0, hipStream_t user) {
   enqueue_dot(L, L->w, L->w, L->scal, 1, st);
   hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(RB), 0, st, L->w, L->v, L->store ? L->V : nullptr, L->len, L->scal);
   hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, L->d_j, 0);
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, L->flag + 1, 0);
   QES_HIP(hipGetLastError());
   L->steps_done = 0;
   QES_HIP(hipStreamSynchronize(st));
@@ -448,6 +479,11 @@ int lanczos_run(Lanczos* L, int nsteps, int reorth, int use_graph) {
   }
   QES_HIP(hipStreamSynchronize(st));
   L->steps_done += nsteps;
+  return 0;
+}
+
+int lanczos_second_passes(Lanczos* L, int* count) {
+  QES_HIP(hipMemcpy(count, L->flag + 1, sizeof(int), hipMemcpyDeviceToHost));
   return 0;
 }
 

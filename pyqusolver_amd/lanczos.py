@@ -33,6 +33,7 @@ class LanczosResult:
     seconds: float = 0.0
     residual_bound: float = float("nan")
     assembled: bool = False                          # the solve streamed an assembled table
+    second_passes: int = 0                           # Gram-Schmidt second passes (DGKS) run
 
 
 def start_vector(n: int, complex_: bool, seed: int = 0) -> torch.Tensor:
@@ -60,9 +61,18 @@ class DeviceLanczos:
     def start(self, v0: torch.Tensor):
         _lib.check(_lib.lib().qes_lanczos_start(self._h, _lib.dptr(v0), _lib.stream_ptr()), "qes_lanczos_start")
 
-    def run(self, nsteps: int, reorth: bool, use_graph: bool = True):
-        _lib.check(_lib.lib().qes_lanczos_run(self._h, int(nsteps), 1 if reorth else 0, 1 if use_graph else 0),
+    def run(self, nsteps: int, reorth, use_graph: bool = True):
+        """reorth: False/0 none, True/1 Gram-Schmidt with the DGKS second pass,
+        2 or 'twice': always two passes."""
+        mode = 2 if reorth in (2, "twice") else (1 if reorth else 0)
+        _lib.check(_lib.lib().qes_lanczos_run(self._h, int(nsteps), mode, 1 if use_graph else 0),
                    "qes_lanczos_run")
+
+    @property
+    def second_passes(self) -> int:
+        c = C.c_int32()
+        _lib.check(_lib.lib().qes_lanczos_stats(self._h, C.byref(c)), "qes_lanczos_stats")
+        return int(c.value)
 
     def coefficients(self):
         a = np.zeros(self.max_steps)
@@ -112,7 +122,7 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
     maxiter = int(min(maxiter, max(n, 1)))
     if store_basis is None:
         store_basis = bool(reorth or compute_eigvec)
-    reorth = bool(reorth and store_basis)
+    reorth = (2 if reorth in (2, "twice") else bool(reorth)) if store_basis else False
     t0 = time.perf_counter()
     L = DeviceLanczos(ham, maxiter, store_basis, complex_)
     L.start(v0t)
@@ -150,7 +160,8 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
             raise ValueError("eigenvectors need store_basis=True")
         vecs = torch.empty((n,), dtype=v0t.dtype, device="cuda")
         L.ritz(V[:, 0], vecs)
+    second = L.second_passes if reorth else 0
     L.close()
     return LanczosResult(eigenvalues=w[:k].copy(), alpha=a, beta=b, steps=len(a), converged=converged,
                          history=hist, eigenvectors=vecs, seconds=time.perf_counter() - t0,
-                         residual_bound=res_bound)
+                         residual_bound=res_bound, second_passes=second)

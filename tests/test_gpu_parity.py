@@ -208,6 +208,26 @@ def test_lanczos_e0_against_dense(case):
     assert r.norm().item() <= 1e-6
 
 
+@pytest.mark.parametrize("reorth", [True, "twice"])
+def test_full_reorth_keeps_spectrum_ghost_free(reorth):
+    """Many more Lanczos steps than needed for E0 (N = 805 states, 400
+    steps): with full reorthogonalisation (DGKS second pass, or always
+    twice) the lowest Ritz values are the lowest eigenvalues of the dense
+    reduced matrix, with no spurious copies."""
+    from pyqusolver_amd.lanczos import lanczos_ground_state
+    hs, H, grp, b, t = build_pair("chain", 16, 1, [("tx", 2)], 8, "xxz")
+    c, s, co = t.arrays()
+    ev = np.linalg.eigvalsh(O.reduced_dense(b, 16, c, s, co, True))
+    res = lanczos_ground_state(H, k=4, tol=0.0, maxiter=400, seed=3, reorth=reorth, check_every=400)
+    assert res.steps == 400
+    w = np.linalg.eigvalsh(np.diag(res.alpha) + np.diag(res.beta[:-1], 1) + np.diag(res.beta[:-1], -1))
+    np.testing.assert_allclose(w[:4], ev[:4], rtol=0, atol=1e-9 * abs(ev[0]))
+    if reorth == "twice":
+        assert res.second_passes == 0     # counted only by the DGKS variant
+    else:
+        assert 0 <= res.second_passes <= 400
+
+
 def test_majumdar_ghosh_chain24_exact_energy():
     """J1-J2 chain at J2 = J1/2: E0 = -3L/8 exactly (Majumdar-Ghosh)."""
     from pyqusolver_amd import ChainLattice, get_u1_sym
