@@ -119,17 +119,23 @@ int qes_op_diagonal(qes_op_t op, void* re_dev, void* im_dev, void* stream);
 
 /*
  * Assembled operator: tabulate every row of H once on the device (sliced
- * ELLPACK: 64 rows per slice, column index + value per entry, diagonal
- * first), after which qes_matvec with nbatch == 1 streams the table instead
- * of canonicalising and looking up neighbours.  The device counterpart of
- * the reference's explicit sparse build (Algebra/Hilbert/matrix_builder.py:151-206,
- * :911-1030), for repeated application (Lanczos).  enable = 0 frees it; the
- * build fails if the table would exceed max_gb (<= 0: 64 GB default).  Any
- * change of the operator's plan (qes_op_set_option / set_canonicalizer)
- * drops the table.
+ * ELLPACK: 64 rows per slice, column-major inside a slice), after which
+ * qes_matvec with nbatch == 1 streams the table instead of canonicalising and
+ * looking up neighbours.  The device counterpart of the reference's explicit
+ * sparse build (Algebra/Hilbert/matrix_builder.py:151-206, :911-1030), for
+ * repeated application (Lanczos).  mode 0 frees it; 1 picks the format: real
+ * operators whose off-diagonal values fit a dictionary (< 2^16 distinct
+ * values) store a value index beside the column (packed into 32 bits when
+ * index and column bits fit, format 1; else int32 + uint16, format 2) and
+ * the diagonal separately; otherwise column + f64/c128 value with the
+ * diagonal as entry 0 (format 0); mode 2 forces format 0.  The build fails
+ * if the table would exceed max_gb (<= 0: 64 GB default).  Any change of the
+ * operator's plan (qes_op_set_option / set_canonicalizer) drops the table.
+ * qes_op_assembly_info: entries per table, bytes, format (-1: none) and
+ * dictionary size.
  */
-int qes_op_assemble(qes_op_t op, int32_t enable, double max_gb, void* stream);
-int qes_op_assembly_info(qes_op_t op, int64_t* entries, int64_t* bytes);
+int qes_op_assemble(qes_op_t op, int32_t mode, double max_gb, void* stream);
+int qes_op_assembly_info(qes_op_t op, int64_t* entries, int64_t* bytes, int32_t* format, int32_t* n_values);
 
 /* Number of off-diagonal gathers of one matvec (E_off) -- stats pass. */
 int qes_op_count_offdiag(qes_op_t op, int64_t* e_off);

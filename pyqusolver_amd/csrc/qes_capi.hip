@@ -391,18 +391,21 @@ int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
   return build_plan(&op, op.plan.canon == pick_canon_public(op.basis->group) ? -1 : op.plan.canon);
 }
 
-int qes_op_assemble(qes_op_t h, int32_t enable, double max_gb, void* stream) {
+int qes_op_assemble(qes_op_t h, int32_t mode, double max_gb, void* stream) {
   if (!h) return fail("null operator");
+  if (mode < 0 || mode > 2) return fail("mode must be 0 (free), 1 (auto format) or 2 (plain values)");
   if (max_gb > 0) h->op.asm_max_bytes = max_gb * 1e9;
-  return op_assemble(&h->op, enable, (hipStream_t)stream);
+  return op_assemble(&h->op, mode != 0, mode == 1, (hipStream_t)stream);
 }
 
-int qes_op_assembly_info(qes_op_t h, int64_t* entries, int64_t* bytes) {
+int qes_op_assembly_info(qes_op_t h, int64_t* entries, int64_t* bytes, int32_t* format, int32_t* n_values) {
   if (!h) return fail("null operator");
   const Op& op = h->op;
-  const int64_t e = op.asm_col ? op.asm_entries : 0;
-  if (entries) *entries = e;
-  if (bytes) *bytes = e * (4 + 8 * (op.asm_vc ? 2 : 1)) + (op.asm_col ? op.asm_slices * 12 : 0);
+  const bool on = op.asm_fmt >= 0;
+  if (entries) *entries = on ? op.asm_entries : 0;
+  if (bytes) *bytes = on ? op.asm_bytes : 0;
+  if (format) *format = op.asm_fmt;
+  if (n_values) *n_values = op.asm_ndict;
   return 0;
 }
 

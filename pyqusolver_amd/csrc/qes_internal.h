@@ -88,8 +88,12 @@ struct Op {
   int32_t* asm_len = nullptr;
   int32_t* asm_col = nullptr;
   double* asm_val = nullptr;
-  int64_t asm_entries = 0, asm_slices = 0;
-  int asm_vc = 0;
+  uint32_t* asm_packed = nullptr;
+  uint16_t* asm_vidx = nullptr;
+  double* asm_dict = nullptr;
+  double* asm_diag = nullptr;
+  int64_t asm_entries = 0, asm_slices = 0, asm_bytes = 0;
+  int asm_vc = 0, asm_fmt = -1, asm_colbits = 32, asm_ndict = 0;
   double asm_max_bytes = 64e9;
 };
 
@@ -109,7 +113,7 @@ int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st);
 int op_diagonal(const Op* op, double* dre, double* dim, hipStream_t st);
 int build_plan(Op* op, int canon_override);   // -1: automatic
 // assembled operator (qes_assemble.hip)
-int op_assemble(Op* op, int enable, hipStream_t st);
+int op_assemble(Op* op, int enable, int want_dict, hipStream_t st);
 void free_assembly(Op* op);
 int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int accumulate, hipStream_t st);
 int pick_canon_public(const QesGroup& G);

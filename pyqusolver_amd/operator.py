@@ -216,20 +216,29 @@ class Hamiltonian:
         names = {v: k for k, v in self.CANON_KINDS.items()}
         return {"canonicalizer": names[c.value], "exchange_classes": nc.value, "generic_groups": ng.value}
 
-    def assemble(self, enable: bool = True, max_gb: float = 0.0):
+    def assemble(self, enable: bool = True, max_gb: float = 0.0, dictionary: bool = True):
         """Tabulate H on the device (sliced ELLPACK, built once by the same
         row visitor) so that single-vector matvecs stream the table instead
-        of canonicalising; ``enable=False`` frees it.  Returns the table size
-        in bytes."""
-        _lib.check(_lib.lib().qes_op_assemble(self._device_op(), 1 if enable else 0, float(max_gb),
-                                              _lib.stream_ptr()), "qes_op_assemble")
+        of canonicalising; ``enable=False`` frees it.  ``dictionary`` lets
+        real operators store value indices instead of values.  Returns the
+        table size in bytes."""
+        mode = 0 if not enable else (1 if dictionary else 2)
+        _lib.check(_lib.lib().qes_op_assemble(self._device_op(), mode, float(max_gb), _lib.stream_ptr()),
+                   "qes_op_assemble")
         return self.assembly_bytes
 
     @property
+    def assembly(self) -> dict:
+        e, b, f, nv = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib().qes_op_assembly_info(self._device_op(), C.byref(e), C.byref(b), C.byref(f),
+                                                   C.byref(nv)))
+        names = {-1: None, 0: "plain", 1: "packed32", 2: "index16"}
+        return {"entries": int(e.value), "bytes": int(b.value), "format": names[int(f.value)],
+                "n_values": int(nv.value)}
+
+    @property
     def assembly_bytes(self) -> int:
-        e, b = C.c_int64(), C.c_int64()
-        _lib.check(_lib.lib().qes_op_assembly_info(self._device_op(), C.byref(e), C.byref(b)))
-        return int(b.value)
+        return self.assembly["bytes"]
 
     def count_offdiag(self) -> int:
         v = C.c_int64()
