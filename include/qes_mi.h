@@ -226,6 +226,7 @@ int qes_batch_create_blocks(const qes_op_t* ops, int32_t nprob, const qes_op_t* 
 /* element offsets of every real problem (ordinary, then blocks) + total */
 int qes_batch_offsets(qes_batch_t b, int64_t* offsets /* nprob + nblock + 1 */);
 int qes_batch_start(qes_batch_t b, const void* v0_dev, void* stream);
+/* nsteps == 0 with use_graph != 0 captures the step graph without running it. */
 int qes_batch_run(qes_batch_t b, int32_t nsteps, int32_t use_graph);
 int qes_batch_get(qes_batch_t b, double* alpha_host, double* beta_host, int32_t* steps_host);
 int qes_batch_set_active(qes_batch_t b, const int32_t* active_host);
@@ -235,7 +236,10 @@ int qes_batch_matvec(qes_batch_t b, const void* x_dev, void* y_dev, void* stream
  * (rows per tile of a block problem), "block_table" (1, default: block
  * problems read a neighbour table built once at creation -- column index and
  * pull-form value of every off-diagonal element, shared by all R columns and
- * all steps; 0: canonicalise and look up on the fly like qes_matvec). */
+ * all steps; 0: canonicalise and look up on the fly like qes_matvec),
+ * "op_tables" (1, default: ordinary problems whose operator was assembled with
+ * qes_op_assemble before qes_batch_create stream that table; the batch keeps
+ * pointers to it, so the assembly must outlive the batch). */
 int qes_batch_set_option(qes_batch_t b, const char* name, int64_t value);
 int qes_batch_ground_states(qes_batch_t b, double tol, int32_t check_every, double* e0_host, int32_t* steps_host);
 int qes_batch_destroy(qes_batch_t b);

@@ -58,7 +58,7 @@ class BatchedLanczos:
     MAX_RHS = 256
 
     def __init__(self, hams: Sequence, max_steps: int = 300, complex_: Optional[bool] = None,
-                 blocks: bool = True, min_block: int = 2):
+                 blocks: bool = True, min_block: int = 2, assemble: bool = True):
         if not hams:
             raise ValueError("empty batch")
         self.hams = list(hams)
@@ -77,6 +77,22 @@ class BatchedLanczos:
             for s in range(0, len(idx), self.MAX_RHS):
                 blocks_.append(idx[s:s + self.MAX_RHS])
         regular.sort()
+        # ordinary problems stream an assembled table of their operator (built
+        # here when it fits a quarter of the free memory, freed by close())
+        self._assembled_here = []
+        if assemble and regular:
+            free, _ = torch.cuda.mem_get_info()
+            budget = 0.25 * free
+            for i in regular:
+                h = self.hams[i]
+                if h.assembly_bytes:
+                    continue
+                try:
+                    nb = h.assemble(True, max_gb=budget / 1e9)
+                except _lib.QesError:
+                    continue
+                budget -= nb
+                self._assembled_here.append(h)
         self._diag = []
         block_ops, block_r, block_d = [], [], []
         for members in blocks_:
@@ -126,8 +142,9 @@ class BatchedLanczos:
 
     def set_option(self, name: str, value: int):
         """Schedule knob: tile_order (0/1), grid_mult (blocks per CU),
-        block_rows (rows per tile of a block problem) or block_table (0:
-        matrix-free block rows, 1: cached neighbour table)."""
+        block_rows (rows per tile of a block problem), block_table (0:
+        matrix-free block rows, 1: cached neighbour table) or op_tables (0:
+        matrix-free ordinary problems, 1: their assembled tables)."""
         _lib.check(_lib.lib().qes_batch_set_option(self._h, name.encode(), int(value)), "qes_batch_set_option")
 
     def concat(self, vecs: Sequence) -> torch.Tensor:
@@ -172,8 +189,16 @@ class BatchedLanczos:
         v0 = v0.to(self.torch_dtype).contiguous()
         _lib.check(_lib.lib().qes_batch_start(self._h, _lib.dptr(v0), _lib.stream_ptr()), "qes_batch_start")
 
-    def ground_states(self, tol: float = 1e-10, check_every: int = 10, v0=None, seeds=None) -> BatchResult:
-        torch.cuda.synchronize()
+    def capture(self):
+        """Capture the step graph now (no steps run)."""
+        _lib.check(_lib.lib().qes_batch_run(self._h, 0, 1), "qes_batch_run")
+
+    def ground_states(self, tol: float = 1e-10, check_every: int = 10, v0=None, seeds=None,
+                      sync_device: bool = True) -> BatchResult:
+        if sync_device:
+            torch.cuda.synchronize()
+        else:
+            torch.cuda.current_stream().synchronize()
         t0 = time.perf_counter()
         self.start(v0, seeds)
         V = len(self.hams)
@@ -198,6 +223,12 @@ class BatchedLanczos:
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib.lib().qes_batch_destroy(self._h)
             self._h = None
+        for h in getattr(self, "_assembled_here", []):
+            try:
+                h.assemble(False)
+            except Exception:
+                pass
+        self._assembled_here = []
 
     def __del__(self):
         try:
@@ -220,17 +251,98 @@ def momentum_sectors(L: int, reflection: bool = True):
     return out
 
 
-def sector_by_sector(hams: Sequence, tol: float = 1e-10, max_steps: int = 300, check_every: int = 10, seeds=None):
+def sector_by_sector(hams: Sequence, tol: float = 1e-10, max_steps: int = 300, check_every: int = 10, seeds=None,
+                     assemble: bool = True):
     """Reference-style sweep on the same GPU: one device Lanczos per problem,
-    same recurrence (no reorthogonalisation) and convergence rule."""
+    same recurrence (no reorthogonalisation) and convergence rule; with
+    ``assemble`` each operator is tabulated for its own solve (and freed
+    after), as a sequential caller of this library would do."""
     from .lanczos import lanczos_ground_state
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0 = []
     steps = []
     for i, h in enumerate(hams):
+        built = False
+        if assemble and h.assembly_bytes == 0:
+            try:
+                h.assemble(True)
+                built = True
+            except _lib.QesError:
+                pass
         r = lanczos_ground_state(h, k=1, tol=tol, maxiter=max_steps, seed=(seeds[i] if seeds else i), reorth=False,
                                  store_basis=False, check_every=check_every)
+        if built:
+            h.assemble(False)
         e0.append(r.eigenvalues[0])
         steps.append(r.steps)
     return BatchResult(e0=np.asarray(e0), steps=np.asarray(steps), seconds=time.perf_counter() - t0)
+
+
+def batched_ground_states(hams: Sequence, tol: float = 1e-10, max_steps: int = 300, check_every: int = 10,
+                          seeds=None, split_dtypes: bool = True, concurrent: bool = True, warmup: bool = False):
+    """Ground states of many problems with batched Lanczos.
+
+    Problems that need complex128 vectors (generic momenta, complex models)
+    and the real ones go to two batches, so the real problems (and their
+    realisation blocks) move half the bytes; the two batches run on their
+    own streams from two host threads and overlap on the device.  Seeds
+    default to the problem's position, as in ``sector_by_sector``.  Returns
+    (BatchResult in the order of ``hams``, seconds spent creating the
+    batches: tables, diagonals)."""
+    import threading
+    seeds = list(range(len(hams))) if seeds is None else list(seeds)
+    groups = {}
+    for i, h in enumerate(hams):
+        groups.setdefault(bool(h.needs_complex) if split_dtypes else True, []).append(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batches = [(idx, BatchedLanczos([hams[i] for i in idx], max_steps=max_steps, complex_=c))
+               for c, idx in sorted(groups.items(), key=lambda kv: -len(kv[1]))]
+    torch.cuda.synchronize()
+    t_create = time.perf_counter() - t0
+    results = [None] * len(batches)
+    errors = []
+
+    streams = [torch.cuda.Stream() for _ in batches]
+    for _, B in batches:          # graphs captured up front, one batch at a time
+        B.capture()
+
+    def run(k):
+        idx, B = batches[k]
+        try:
+            # each thread feeds its batch from its own stream: no legacy-stream
+            # work while another batch's graph runs
+            with torch.cuda.stream(streams[k]):
+                results[k] = B.ground_states(tol=tol, check_every=check_every, seeds=[seeds[i] for i in idx],
+                                             sync_device=False)
+        except Exception as ex:   # re-raised on the calling thread
+            errors.append(ex)
+
+    def once():
+        if concurrent and len(batches) > 1:
+            th = [threading.Thread(target=run, args=(k,)) for k in range(len(batches))]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+        else:
+            for k in range(len(batches)):
+                run(k)
+        if errors:
+            raise errors[0]
+
+    if warmup:
+        once()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    once()
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    e0 = np.zeros(len(hams))
+    steps = np.zeros(len(hams), dtype=np.int32)
+    for (idx, B), r in zip(batches, results):
+        e0[idx] = r.e0
+        steps[idx] = r.steps
+        B.close()
+    return BatchResult(e0=e0, steps=steps, seconds=seconds), t_create

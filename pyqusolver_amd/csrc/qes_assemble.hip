@@ -23,10 +23,10 @@
 
 #include "qes_internal.h"
 #include "qes_rows.h"
+#include "qes_sell.h"
 
 namespace qes {
 
-static constexpr int SELL_C = 64;
 
 // Formats.  PLAIN: int32 column + f64 (or c128) value per entry, the
 // diagonal as entry 0.  Real operators whose off-diagonal values take few
@@ -34,7 +34,6 @@ static constexpr int SELL_C = 64;
 // ratio) store an index into a value dictionary instead: P4 packs column and
 // index into one uint32 (column in the low bits), P6 keeps an int32 column
 // and a uint16 index; the diagonal then lives in its own f64 array.
-enum { ASM_PLAIN = 0, ASM_P4 = 1, ASM_P6 = 2 };
 enum { PASS_COUNT = 0, PASS_DICT = 1, PASS_FILL = 2 };
 
 static constexpr int DICT_HBITS = 16;                  // dictionary hash slots
@@ -165,93 +164,14 @@ __global__ void __launch_bounds__(256) asm_rows_kernel(QesGroup G, QesLookup L, 
   }
 }
 
-// y = A x with the sliced table; VC: complex values, XC: complex vectors
-template <bool VC, bool XC>
-__global__ void __launch_bounds__(256) sell_matvec_kernel(const int64_t* __restrict__ off,
-                                                          const int32_t* __restrict__ slen,
-                                                          const int32_t* __restrict__ col,
-                                                          const double* __restrict__ val,
-                                                          const double* __restrict__ x, double* __restrict__ y,
-                                                          int64_t n, int accumulate) {
+// y = A x with the sliced table; XC: complex vectors
+template <bool XC>
+__global__ void __launch_bounds__(256) sell_matvec_kernel(AsmTable T, const double* __restrict__ x,
+                                                          double* __restrict__ y, int64_t n, int accumulate) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
-  const int64_t s = a / SELL_C;
-  const int l = (int)(a % SELL_C);
-  const int64_t base = off[s] + l;
-  const int len = slen[s];
-  const int vw = VC ? 2 : 1;
-  double ar = 0.0, ai = 0.0;
-  for (int k = 0; k < len; ++k) {
-    const int64_t e = base + (int64_t)k * SELL_C;
-    const int64_t b = col[e];
-    const double vr = val[e * vw];
-    const double vi = VC ? val[e * vw + 1] : 0.0;
-    if (XC) {
-      const double xr = x[2 * b], xi = x[2 * b + 1];
-      ar += vr * xr - vi * xi;
-      ai += vr * xi + vi * xr;
-    } else {
-      ar += vr * x[b];
-    }
-  }
-  if (XC) {
-    if (accumulate) {
-      y[2 * a] += ar;
-      y[2 * a + 1] += ai;
-    } else {
-      y[2 * a] = ar;
-      y[2 * a + 1] = ai;
-    }
-  } else {
-    if (accumulate) y[a] += ar; else y[a] = ar;
-  }
-}
-
-// dictionary formats (real values); XC: complex vectors with real values
-template <int FMT, bool XC>
-__global__ void __launch_bounds__(256) sell_dict_matvec_kernel(const int64_t* __restrict__ off,
-                                                               const int32_t* __restrict__ slen,
-                                                               const uint32_t* __restrict__ packed,
-                                                               const int32_t* __restrict__ col,
-                                                               const uint16_t* __restrict__ vidx,
-                                                               const double* __restrict__ dict, int colbits,
-                                                               const double* __restrict__ diag,
-                                                               const double* __restrict__ x, double* __restrict__ y,
-                                                               int64_t n, int accumulate) {
-  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
-  const int64_t s = a / SELL_C;
-  const int l = (int)(a % SELL_C);
-  const int64_t base = off[s] + l;
-  const int len = slen[s];
-  const uint32_t cmask = (colbits >= 32) ? ~0u : ((1u << colbits) - 1u);
-  const double d = diag[a];
-  double ar, ai = 0.0;
-  if (XC) {
-    ar = d * x[2 * a];
-    ai = d * x[2 * a + 1];
-  } else {
-    ar = d * x[a];
-  }
-  for (int k = 0; k < len; ++k) {
-    const int64_t e = base + (int64_t)k * SELL_C;
-    int64_t b;
-    double v;
-    if (FMT == ASM_P4) {
-      const uint32_t p = packed[e];
-      b = p & cmask;
-      v = dict[p >> colbits];
-    } else {
-      b = col[e];
-      v = dict[vidx[e]];
-    }
-    if (XC) {
-      ar += v * x[2 * b];
-      ai += v * x[2 * b + 1];
-    } else {
-      ar += v * x[b];
-    }
-  }
+  double ar, ai;
+  sell_row<XC>(T, a, x, ar, ai);
   if (XC) {
     if (accumulate) {
       y[2 * a] += ar;
@@ -459,27 +379,9 @@ int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int ac
   const dim3 grid((unsigned)((n + 255) / 256));
   const bool xc = dtype == 1;
   if (op->asm_vc && !xc) return fail("complex matrix elements need complex128 vectors");
-  if (op->asm_fmt == ASM_P4 || op->asm_fmt == ASM_P6) {
-#define QES_DM(F, X)                                                                                             \
-  hipLaunchKernelGGL((sell_dict_matvec_kernel<F, X>), grid, dim3(256), 0, st, op->asm_off, op->asm_len,          \
-                     op->asm_packed, op->asm_col, op->asm_vidx, op->asm_dict, op->asm_colbits, op->asm_diag, x, y, \
-                     n, accumulate)
-    if (op->asm_fmt == ASM_P4) {
-      if (xc) QES_DM(ASM_P4, true); else QES_DM(ASM_P4, false);
-    } else {
-      if (xc) QES_DM(ASM_P6, true); else QES_DM(ASM_P6, false);
-    }
-#undef QES_DM
-  } else if (op->asm_vc) {
-    hipLaunchKernelGGL((sell_matvec_kernel<true, true>), grid, dim3(256), 0, st, op->asm_off, op->asm_len,
-                       op->asm_col, op->asm_val, x, y, n, accumulate);
-  } else if (xc) {
-    hipLaunchKernelGGL((sell_matvec_kernel<false, true>), grid, dim3(256), 0, st, op->asm_off, op->asm_len,
-                       op->asm_col, op->asm_val, x, y, n, accumulate);
-  } else {
-    hipLaunchKernelGGL((sell_matvec_kernel<false, false>), grid, dim3(256), 0, st, op->asm_off, op->asm_len,
-                       op->asm_col, op->asm_val, x, y, n, accumulate);
-  }
+  const AsmTable T = asm_table_of(op);
+  if (xc) hipLaunchKernelGGL((sell_matvec_kernel<true>), grid, dim3(256), 0, st, T, x, y, n, accumulate);
+  else hipLaunchKernelGGL((sell_matvec_kernel<false>), grid, dim3(256), 0, st, T, x, y, n, accumulate);
   QES_HIP(hipGetLastError());
   return 0;
 }

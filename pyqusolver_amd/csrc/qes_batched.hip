@@ -27,6 +27,7 @@
 #include "qes_internal.h"
 #include "qes_lanczos.h"
 #include "qes_rows.h"
+#include "qes_sell.h"
 
 namespace qes {
 
@@ -58,6 +59,8 @@ struct QesProb {
   const int64_t* rowptr;
   const int32_t* col;
   const double* val;    // 1 (real) or 2 (complex) doubles per entry
+  // ordinary problems: the operator's assembled table (fmt < 0: none)
+  AsmTable T;
 };
 
 struct Tile {
@@ -266,6 +269,18 @@ __global__ void __launch_bounds__(BRB) batch_matvec_kernel(const QesProb* __rest
     const Tile T = tiles[t];
     if (prob_active && !prob_active[T.prob]) continue;   // uniform per block
     const QesProb& P = probs[T.prob];
+    if (!P.blk && P.T.fmt >= 0) {   // assembled ordinary problem: stream its table
+      const int64_t a = T.row0 + threadIdx.x;
+      if (a < P.n) {
+        const int cw = CPLX ? 2 : 1;
+        double ar, ai;
+        sell_row<CPLX>(P.T, a, x + P.off * cw, ar, ai);
+        double* yp = y + (P.off + a) * cw;
+        yp[0] = ar;
+        if (CPLX) yp[1] = ai;
+      }
+      continue;
+    }
     if (P.rowptr) {   // tabulated block problem: no operator context needed
       const int Rp = block_lanes(P.nrhs);
       const int c0 = threadIdx.x % Rp;
@@ -562,6 +577,7 @@ struct Batch {
   int grid_mult = 8;      // persistent matvec blocks per CU
   int block_rows = 16;    // rows per tile of a block problem (at least one per lane group)
   int block_table = 1;    // block problems use their neighbour table
+  bool use_tables = true; // ordinary problems stream their operator's assembled table when it has one
   std::vector<QesProb> h_probs;
   std::vector<int64_t*> t_rowptr;   // per real problem (null for ordinary problems)
   std::vector<int32_t*> t_col;
@@ -572,6 +588,14 @@ struct Batch {
   hipGraphExec_t exec = nullptr;
   std::vector<int> h_active;
 };
+
+// Copies ordered on the batch's own stream (never the legacy stream), so a
+// batch can run while another thread captures or replays its own graph.
+static hipError_t batch_copy(Batch* B, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, B->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(B->stream);
+}
 
 static int batch_build_tiles(Batch* B) {
   std::vector<std::vector<Tile>> per(B->nprob);
@@ -598,7 +622,7 @@ static int batch_build_tiles(Batch* B) {
   B->d_tiles = nullptr;
   B->ntiles = (int)tiles.size();
   QES_HIP(hipMalloc(&B->d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(Tile)));
-  if (!tiles.empty()) QES_HIP(hipMemcpy(B->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  if (!tiles.empty()) QES_HIP(batch_copy(B, B->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
   int dev = 0, ncu = 256;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -646,7 +670,7 @@ static int batch_build_tables(Batch* B) {
       acc += k;
     }
     h[n] = acc;
-    QES_HIP(hipMemcpy(d_cnt, h.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    QES_HIP(batch_copy(B, d_cnt, h.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
     QES_HIP(hipMalloc(&B->t_col[p], sizeof(int32_t) * std::max<int64_t>(acc, 1)));
     QES_HIP(hipMalloc(&B->t_val[p], sizeof(double) * cw * std::max<int64_t>(acc, 1)));
     QES_TABLE_LAUNCH(true, B->d_probs, p, nullptr, d_cnt, B->t_col[p], B->t_val[p]);
@@ -665,8 +689,9 @@ static int batch_upload_probs(Batch* B) {
     h[p].rowptr = on ? B->t_rowptr[p] : nullptr;
     h[p].col = on ? B->t_col[p] : nullptr;
     h[p].val = on ? B->t_val[p] : nullptr;
+    if (!B->use_tables) h[p].T.fmt = -1;
   }
-  QES_HIP(hipMemcpy(B->d_probs, h.data(), B->nprob * sizeof(QesProb), hipMemcpyHostToDevice));
+  QES_HIP(batch_copy(B, B->d_probs, h.data(), B->nprob * sizeof(QesProb), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -754,6 +779,8 @@ int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, 
     q.D = D[p];
     q.n = Bs->n;
     q.off = off;
+    q.T = asm_table_of(op);
+    if (R > 1) q.T.fmt = -1;
     q.vp0 = vp;
     B->vp0.push_back(vp);
     if (R == 1) {
@@ -906,8 +933,8 @@ int batch_set_active(Batch* B, const int* active) {
   std::vector<int> pa(B->nprob, 0);
   for (int p = 0; p < B->nprob; ++p)
     for (int r = 0; r < B->nrhs[p]; ++r) pa[p] |= active[B->vp0[p] + r] ? 1 : 0;
-  QES_HIP(hipMemcpy(B->d_active, active, sizeof(int) * B->nvp, hipMemcpyHostToDevice));
-  QES_HIP(hipMemcpy(B->d_pactive, pa.data(), sizeof(int) * B->nprob, hipMemcpyHostToDevice));
+  QES_HIP(batch_copy(B, B->d_active, active, sizeof(int) * B->nvp, hipMemcpyHostToDevice));
+  QES_HIP(batch_copy(B, B->d_pactive, pa.data(), sizeof(int) * B->nprob, hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -938,8 +965,11 @@ int batch_start(Batch* B, const void* v0, hipStream_t user) {
 
 int batch_run(Batch* B, int nsteps, int use_graph) {
   hipStream_t st = B->stream;
-  if (nsteps <= 0) return 0;
+  if (nsteps < 0) return 0;
+  if (nsteps == 0 && !use_graph) return 0;
   if (use_graph) {
+    // nsteps == 0 with a graph: capture only (lets a caller capture every
+    // batch before running several of them from different threads)
     if (!B->exec) {
       QES_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       int rc = enqueue_bstep(B, st);
@@ -960,9 +990,9 @@ int batch_run(Batch* B, int nsteps, int use_graph) {
 }
 
 int batch_get(Batch* B, double* alpha, double* beta, int* steps) {
-  if (alpha) QES_HIP(hipMemcpy(alpha, B->alpha, sizeof(double) * B->nvp * B->maxs, hipMemcpyDeviceToHost));
-  if (beta) QES_HIP(hipMemcpy(beta, B->beta, sizeof(double) * B->nvp * B->maxs, hipMemcpyDeviceToHost));
-  if (steps) QES_HIP(hipMemcpy(steps, B->d_j, sizeof(int) * B->nvp, hipMemcpyDeviceToHost));
+  if (alpha) QES_HIP(batch_copy(B, alpha, B->alpha, sizeof(double) * B->nvp * B->maxs, hipMemcpyDeviceToHost));
+  if (beta) QES_HIP(batch_copy(B, beta, B->beta, sizeof(double) * B->nvp * B->maxs, hipMemcpyDeviceToHost));
+  if (steps) QES_HIP(batch_copy(B, steps, B->d_j, sizeof(int) * B->nvp, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -975,7 +1005,8 @@ int batch_set_option(Batch* B, const std::string& name, int64_t value) {
   else if (name == "grid_mult") B->grid_mult = (int)std::max<int64_t>(1, value);
   else if (name == "block_rows") B->block_rows = (int)std::max<int64_t>(1, std::min<int64_t>(value, 1 << 20));
   else if (name == "block_table") B->block_table = value ? 1 : 0;
-  else return fail("unknown batch option " + name + " (tile_order, grid_mult, block_rows, block_table)");
+  else if (name == "op_tables") B->use_tables = value != 0;
+  else return fail("unknown batch option " + name + " (tile_order, grid_mult, block_rows, block_table, op_tables)");
   batch_drop_graph(B);
   if (batch_upload_probs(B)) return -1;
   return batch_build_tiles(B);

@@ -38,6 +38,14 @@ def test_batched_matvec_equals_individual():
             B.set_option("tile_order", order)
             B.set_option("grid_mult", gm)
             np.testing.assert_array_equal(B.matvec(x).cpu().numpy(), y)
+    # sector tables (assembled operators) against on-the-fly rows: same
+    # elements in the same order; complex multiply-adds may contract
+    # differently in the two kernels
+    assert all(h.assembly_bytes > 0 for h in hams)
+    B.set_option("op_tables", 0)
+    np.testing.assert_allclose(B.matvec(x).cpu().numpy(), y, rtol=0, atol=1e-13 * np.abs(y).max())
+    B.close()
+    assert all(h.assembly_bytes == 0 for h in hams)
 
 
 def test_batched_momentum_sweep_e0_against_oracle():
