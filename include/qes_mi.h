@@ -79,6 +79,20 @@ int qes_basis_device_ptrs(qes_basis_t b, const uint64_t** reps_dev, const double
  */
 int qes_basis_map_states(qes_basis_t b, const uint64_t* states_host, int64_t n, int64_t* idx_out, int32_t* g_out);
 
+/*
+ * Reduced <-> full Hilbert space on the device (vectors of 2^ns entries on the
+ * full side; dtype 0 float64, 1 complex128).  qes_basis_expand replaces
+ * HilbertSpace.expand_state / expand_to_full_state_jit
+ * (Algebra/hilbert.py:1659-1715, Algebra/Symmetries/jit/symmetry_container_jit.py:261-315):
+ * psi[g r_k] += x_k conj(chi_g) / (N_k sqrt|G|), written in pull form (one
+ * lane per full state, no atomics).  qes_basis_reduce replaces
+ * transform_to_reduced_space (Algebra/Hilbert/hilbert_base.py:485-563):
+ * red[k] = sum_{s in orbit k} conj(phase(s)) psi[s] / N_k (one lane per
+ * representative).  Not for sectors with parity-y phases; ns <= 36.
+ */
+int qes_basis_expand(qes_basis_t b, const void* x_red_dev, void* full_dev, int32_t dtype, void* stream);
+int qes_basis_reduce(qes_basis_t b, const void* full_dev, void* x_red_dev, int32_t dtype, void* stream);
+
 int qes_basis_destroy(qes_basis_t b);
 
 /*

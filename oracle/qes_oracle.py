@@ -676,6 +676,31 @@ def rotation_matrix(group: Group, basis: Basis) -> np.ndarray:
     return U
 
 
+def expand_state(group: Group, basis: Basis, x: np.ndarray) -> np.ndarray:
+    """expand_to_full_state_jit (jit/symmetry_container_jit.py:261-315),
+    push form over every group element."""
+    ns = group.lat.ns
+    out = np.zeros(1 << ns, dtype=np.complex128)
+    G = group.n_group
+    for g in range(G):
+        t, ph = group.apply(basis.reps, g)
+        w = np.conj(group.chi[g]) * ph / (basis.norms * np.sqrt(G))
+        np.add.at(out, t, w * x)
+    return out
+
+
+def transform_to_reduced(basis: Basis, ns: int, psi: np.ndarray) -> np.ndarray:
+    """transform_to_reduced_space (Algebra/Hilbert/hilbert_base.py:485-563):
+    red[k] = sum over states s of the sector with repr_map[s] = k of
+    conj(phase_table[phase_idx[s]]) psi[s], then divided by N_k."""
+    states = np.arange(1 << ns, dtype=np.int64)
+    idx, pidx = basis.lookup(states)
+    ok = idx >= 0
+    red = np.zeros(len(basis.reps), dtype=np.complex128)
+    np.add.at(red, idx[ok], np.conj(basis.phase_table[pidx[ok]]) * psi[ok])
+    return red / basis.norms
+
+
 # ---------------------------------------------------------------------------
 # Model term lists (instruction arrays as SpecialOperator.add builds them,
 # special_operator.py:1113-1231 and _finalize_instruction_arrays :1392-1417)

@@ -51,6 +51,8 @@ SIGNATURES = {
     "qes_basis_device_ptrs": [_P, C.POINTER(_P), C.POINTER(_P)],
     "qes_basis_map_states": [_P, _PU64, _I64, _PI64, _PI32],
     "qes_basis_destroy": [_P],
+    "qes_basis_expand": [_P, _P, _P, _I32, _P],
+    "qes_basis_reduce": [_P, _P, _P, _I32, _P],
     "qes_op_create": [_P, _PI32, _PI32, _PD, _PD, _I32, _I32, _PI32, C.POINTER(_P)],
     "qes_op_destroy": [_P],
     "qes_op_needs_complex": [_P, _PI32],

@@ -10,6 +10,7 @@
 // are stream-compacted in order, so the list comes out sorted without a sort.
 #include <algorithm>
 #include <cmath>
+#include <cmath>
 #include <vector>
 
 #include "qes_internal.h"
@@ -366,6 +367,132 @@ __global__ void map_states_kernel(QesGroup G, QesLookup L, const uint64_t* __res
   idx[i] = id;
   gmin[i] = id >= 0 ? g : -1;
   if (cnt) cnt[i] = id >= 0 ? c : 0;
+}
+
+// --------------------------------------------------------------------------
+// Reduced <-> full space (no intrinsic phases).
+//
+// expand_to_full_state_jit (jit/symmetry_container_jit.py:261-315) pushes
+//     psi[g r_k] += x_k conj(chi_g) / (N_k sqrt|G|)   for every g.
+// The g with g r_k = s form a coset g_s Stab(r_k) on which chi is constant
+// (chi is trivial on the stabiliser of a representative with N_k > 0), and
+// |Stab| = N_k^2, so in pull form, one lane per full state s:
+//     psi[s] = x_k chi(g*) N_k / sqrt|G|,  g* = first minimiser (g* s = r_k).
+// transform_to_reduced_space (Algebra/Hilbert/hilbert_base.py:485-563):
+//     red[k] = sum_{s -> k} conj(phase(s)) psi[s] / N_k,  phase(s) = chi(g*_s)
+// in pull form over the orbit, one lane per representative: the orbit
+// element g r_k has conj(chi(g*)) = chi(g), each distinct state is met |Stab|
+// times, so red[k] = sum_g chi(g) psi[g r_k] / (N_k |Stab|).
+// --------------------------------------------------------------------------
+template <typename W>
+__device__ __forceinline__ W apply_element(const GV& G, int g, W s) {
+  const int B = g % G.nbase;
+  const int ab = g / G.nbase;
+  const int b = ab % G.nty, a = ab / G.nty;
+  int dummy = 0;
+  const W mask = (W)G.mask;
+  W v = apply_base<W>(G, B, s, &dummy);
+  v = rotl_rows<W>(v, a * G.cell_bits, G.row_bits, (W)G.rowlo[a], mask);
+  return rotl_ns<W>(v, b * G.row_bits, G.ns, mask);
+}
+
+template <typename W, bool CPLX>
+__global__ void expand_kernel(QesGroup G, QesLookup L, const double* __restrict__ chi_re,
+                              const double* __restrict__ chi_im, const double* __restrict__ norms, double inv_sqrt_g,
+                              const double* __restrict__ x, double* __restrict__ full, uint64_t nfull) {
+  __shared__ uint64_t s_rowlo[QES_MAX_TX];
+  const GV gv = make_gv(G, s_rowlo);
+  __syncthreads();
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nfull) return;
+  double vr = 0.0, vi = 0.0;
+  const bool ok = passes_globals(G, s) && (G.n_up < 0 || __popcll(s) == G.n_up);
+  if (ok) {
+    const Canon<W> cn = canonicalize<W>(gv, (W)s);
+    const int64_t k = lookup_rep<W>(L, cn.rep);
+    if (k >= 0) {
+      const double w = norms[k] * inv_sqrt_g;
+      const double cr = chi_re[cn.g] * w, ci = chi_im[cn.g] * w;
+      if (CPLX) {
+        const double xr = x[2 * k], xi = x[2 * k + 1];
+        vr = xr * cr - xi * ci;
+        vi = xr * ci + xi * cr;
+      } else {
+        vr = x[k] * cr;
+      }
+    }
+  }
+  if (CPLX) {
+    full[2 * s] = vr;
+    full[2 * s + 1] = vi;
+  } else {
+    full[s] = vr;
+  }
+}
+
+template <typename W, bool CPLX>
+__global__ void reduce_kernel(QesGroup G, const double* __restrict__ chi_re, const double* __restrict__ chi_im,
+                              const uint64_t* __restrict__ reps, const double* __restrict__ norms, int64_t n,
+                              int gsize, const double* __restrict__ full, double* __restrict__ red) {
+  __shared__ uint64_t s_rowlo[QES_MAX_TX];
+  const GV gv = make_gv(G, s_rowlo);
+  __syncthreads();
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const W r = (W)reps[k];
+  double ar = 0.0, ai = 0.0;
+  for (int g = 0; g < gsize; ++g) {
+    const uint64_t s = (uint64_t)apply_element<W>(gv, g, r);
+    const double cr = chi_re[g], ci = chi_im[g];
+    if (CPLX) {
+      const double pr = full[2 * s], pi = full[2 * s + 1];
+      ar += cr * pr - ci * pi;
+      ai += cr * pi + ci * pr;
+    } else {
+      ar += cr * full[s];
+    }
+  }
+  const double nk = norms[k];
+  const double inv = 1.0 / (nk * nk * nk);
+  if (CPLX) {
+    red[2 * k] = ar * inv;
+    red[2 * k + 1] = ai * inv;
+  } else {
+    red[k] = ar * inv;
+  }
+}
+
+int expand_reduce(const Basis* B, int reduce, const double* x_red, double* full, int dtype, hipStream_t st) {
+  const QesGroup& G = B->group;
+  if (G.has_phase) return fail("expansion with parity-y intrinsic phases is not supported");
+  if (G.ns > 36) return fail("full-space vectors are limited to 36 sites");
+  bool chi_cplx = false;
+  for (int g = 0; g < B->gsize; ++g) chi_cplx = chi_cplx || std::fabs(B->h_chi_im[g]) > 1e-12;
+  if (dtype == 0 && chi_cplx) return fail("complex characters need complex128 vectors");
+  const bool c = dtype == 1;
+  const bool w32 = G.ns <= 32;
+  if (!reduce) {
+    const uint64_t nfull = 1ull << G.ns;
+    const dim3 grid((unsigned)((nfull + 255) / 256));
+    const double isg = 1.0 / std::sqrt((double)B->gsize);
+#define QES_EX(W_, C_) \
+  hipLaunchKernelGGL((expand_kernel<W_, C_>), grid, dim3(256), 0, st, G, B->lookup(), B->chi_re, B->chi_im, \
+                     B->norms, isg, x_red, full, nfull)
+    if (w32) { if (c) QES_EX(uint32_t, true); else QES_EX(uint32_t, false); }
+    else { if (c) QES_EX(uint64_t, true); else QES_EX(uint64_t, false); }
+#undef QES_EX
+  } else {
+    if (B->n == 0) return 0;
+    const dim3 grid((unsigned)((B->n + 255) / 256));
+#define QES_RD(W_, C_) \
+  hipLaunchKernelGGL((reduce_kernel<W_, C_>), grid, dim3(256), 0, st, G, B->chi_re, B->chi_im, B->reps, B->norms, \
+                     B->n, B->gsize, full, (double*)x_red)
+    if (w32) { if (c) QES_RD(uint32_t, true); else QES_RD(uint32_t, false); }
+    else { if (c) QES_RD(uint64_t, true); else QES_RD(uint64_t, false); }
+#undef QES_RD
+  }
+  QES_HIP(hipGetLastError());
+  return 0;
 }
 
 int map_states(const Basis* B, const uint64_t* d_states, int64_t n, int64_t* d_idx, int32_t* d_g, int32_t* d_cnt,

@@ -431,6 +431,16 @@ int qes_basis_set_lookup(qes_basis_t b, int32_t mode) {
   return 0;
 }
 
+int qes_basis_expand(qes_basis_t b, const void* x_red_dev, void* full_dev, int32_t dtype, void* stream) {
+  if (!b || !x_red_dev || !full_dev) return fail("null argument");
+  return expand_reduce(&b->b, 0, (const double*)x_red_dev, (double*)full_dev, dtype, (hipStream_t)stream);
+}
+
+int qes_basis_reduce(qes_basis_t b, const void* full_dev, void* x_red_dev, int32_t dtype, void* stream) {
+  if (!b || !x_red_dev || !full_dev) return fail("null argument");
+  return expand_reduce(&b->b, 1, (const double*)x_red_dev, (double*)full_dev, dtype, (hipStream_t)stream);
+}
+
 int qes_basis_set_bucket(qes_basis_t b, int32_t rows_log2) {
   if (!b) return fail("null basis");
   if (rows_log2 < -4 || rows_log2 > 20) return fail("rows_log2 must be in -4..20");

@@ -101,6 +101,8 @@ struct Op {
 int build_basis(Basis* B, hipStream_t st);
 int build_hash(Basis* B, hipStream_t st);
 int build_bucket(Basis* B, int rows_log2, hipStream_t st);
+// reduce == 0: full = expand(x_red) (2^ns entries); 1: x_red = reduce(full)
+int expand_reduce(const Basis* B, int reduce, const double* x_red, double* full, int dtype, hipStream_t st);
 int map_states(const Basis* B, const uint64_t* d_states, int64_t n, int64_t* d_idx, int32_t* d_g,
                int32_t* d_cnt, hipStream_t st);
 

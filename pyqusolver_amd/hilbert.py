@@ -132,6 +132,43 @@ class HilbertSpace:
         if bucket_rows_log2 is not None:
             _lib.check(_lib.lib().qes_basis_set_bucket(self._h, int(bucket_rows_log2)), "qes_basis_set_bucket")
 
+    # ---- reduced <-> full -----------------------------------------------
+    def _full_op(self, vec, reduce: bool):
+        import torch
+        as_np = not isinstance(vec, torch.Tensor)
+        t = torch.as_tensor(np.ascontiguousarray(vec)) if as_np else vec
+        if t.dim() != 1:
+            raise ValueError("one vector at a time")
+        cplx = t.is_complex() or not self.group.is_real
+        dt = torch.complex128 if cplx else torch.float64
+        t = t.to(device="cuda", dtype=dt).contiguous()
+        n_out = self._nh if reduce else (1 << self._ns)
+        n_in = (1 << self._ns) if reduce else self._nh
+        if t.shape[0] != n_in:
+            raise ValueError(f"vector length {t.shape[0]} does not match {n_in}")
+        out = torch.empty(n_out, dtype=dt, device="cuda")
+        L = _lib.lib()
+        if reduce:
+            _lib.check(L.qes_basis_reduce(self._h, _lib.dptr(t), _lib.dptr(out), 1 if cplx else 0, _lib.stream_ptr()),
+                       "qes_basis_reduce")
+        else:
+            _lib.check(L.qes_basis_expand(self._h, _lib.dptr(t), _lib.dptr(out), 1 if cplx else 0, _lib.stream_ptr()),
+                       "qes_basis_expand")
+        return out.cpu().numpy() if as_np else out
+
+    def expand_state(self, vec_reduced, vec_full=None):
+        """Reduced -> full space (hilbert.py:1659-1715): a 2^ns vector,
+        numpy in -> numpy out, device tensor in -> device tensor out."""
+        out = self._full_op(vec_reduced, reduce=False)
+        if vec_full is not None:
+            vec_full[...] = out
+            return vec_full
+        return out
+
+    def transform_to_reduced_space(self, vec_full):
+        """Full -> reduced space (hilbert_base.py:485-563)."""
+        return self._full_op(vec_full, reduce=True)
+
     def device_arrays(self):
         """Device pointers (reps uint64, norms float64) owned by this space."""
         r = C.c_void_p()

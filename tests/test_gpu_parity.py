@@ -314,3 +314,42 @@ def test_golden_fixtures():
         e_ref = DATA[f"{name}/e0"][0]
         e0 = H.diagonalize(method="lanczos", k=1, tol=1e-14, maxiter=min(300, hs.nh), seed=0, reorth=True)[0]
         assert abs(e0 - e_ref) <= E0_TOL * abs(e_ref), (name, e0, e_ref)
+
+
+FULL_CASES = [
+    ("chain", 12, 1, [("tx", 3)], 6, "xxz"),
+    ("chain", 12, 1, [("tx", 0), ("R", 1), ("Px", 1)], 6, "j1j2"),
+    ("chain", 10, 1, [("tx", 5), ("Px", -1)], None, "j1j2_field"),
+    ("square", 4, 3, [("tx", 1), ("ty", 2)], 6, "j1j2"),
+    ("honeycomb", 3, 2, [("tx", 0), ("ty", 0)], None, "hk"),
+]
+
+
+@pytest.mark.parametrize("case", FULL_CASES, ids=_ids)
+def test_expand_and_reduce_against_oracle(case):
+    """expand_state / transform_to_reduced_space on the device against the
+    oracle restatements; pins independent of them: reduce(expand(x)) =
+    sqrt|G| / N_k^2 x_k (each orbit state carries chi(g*) N_k / sqrt|G|), and
+    the expanded ground state is an eigenvector of the full-space H."""
+    kind, lx, ly, gens, n_up, model = case
+    hs, H, grp, b, t = build_pair(kind, lx, ly, gens, n_up, model)
+    ns = lx * ly * (2 if kind == "honeycomb" else 1)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(hs.nh) + 1j * rng.standard_normal(hs.nh)
+    full = hs.expand_state(x)
+    ref = O.expand_state(grp, b, x)
+    assert np.linalg.norm(full - ref) <= 1e-13 * np.linalg.norm(ref)
+    psi = rng.standard_normal(1 << ns) + 1j * rng.standard_normal(1 << ns)
+    red = hs.transform_to_reduced_space(psi)
+    red_ref = O.transform_to_reduced(b, ns, psi)
+    assert np.linalg.norm(red - red_ref) <= 1e-13 * np.linalg.norm(red_ref)
+    back = hs.transform_to_reduced_space(full)
+    np.testing.assert_allclose(back, x * np.sqrt(grp.n_group) / b.norms ** 2, rtol=1e-12, atol=0)
+    # eigenvector of the full-space Hamiltonian
+    c, s, co = t.arrays()
+    e0 = H.diagonalize(k=1, tol=1e-13, maxiter=min(300, hs.nh), compute_eigvec=True)[0]
+    v = H.eig_vec.cpu().numpy()
+    vf = hs.expand_state(v)
+    Hf = O.full_sparse_hamiltonian(ns, c, s, co, t.is_complex)
+    r = Hf @ vf - e0 * vf
+    assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(vf)

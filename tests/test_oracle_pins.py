@@ -192,3 +192,20 @@ def test_survey_sector_dimensions_closed_form():
     assert tot == 2_704_156
     assert dim("honeycomb", 3, 4, [("tx", 0), ("ty", 0)], None) == 1_398_500
     assert dim("square", 6, 6, [("tx", 0), ("ty", 0), ("Px", 1)], 18) == 126_056_625
+
+
+def test_expand_and_transform_to_reduced_closed_forms():
+    """Oracle expand_state is an isometry whose columns are the rotation
+    matrix (pinned by U^dagger H U above); transform_to_reduced composed with
+    it is the diagonal sqrt|G| / N_k^2 (the reference's reduction divides by
+    N_k once, with no 1/sqrt|G|)."""
+    lat = O.OLattice("chain", 10)
+    grp = O.Group(lat, [O.Generator("tx", 3)], n_up=5)
+    b = O.scan_basis(grp)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(len(b.reps)) + 1j * rng.standard_normal(len(b.reps))
+    full = O.expand_state(grp, b, x)
+    np.testing.assert_allclose(np.linalg.norm(full), np.linalg.norm(x), rtol=1e-13)
+    np.testing.assert_allclose(full, O.rotation_matrix(grp, b) @ x, rtol=0, atol=1e-14)
+    back = O.transform_to_reduced(b, 10, full)
+    np.testing.assert_allclose(back, x * np.sqrt(grp.n_group) / b.norms ** 2, rtol=1e-12)
