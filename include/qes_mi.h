@@ -150,6 +150,14 @@ int qes_op_diagonal(qes_op_t op, void* re_dev, void* im_dev, void* stream);
  */
 int qes_op_assemble(qes_op_t op, int32_t mode, double max_gb, void* stream);
 int qes_op_assembly_info(qes_op_t op, int64_t* entries, int64_t* bytes, int32_t* format, int32_t* n_values);
+/* Coordinate list (row, col, value) of the assembled table on the device --
+ * the explicit sparse matrix of the reference's Hamiltonian.build
+ * (Algebra/hamil.py:1434-1557 via matrix_builder.py:151-206).  Padding slots
+ * carry value 0 at (row, row); slots of rows past n keep the caller's fill
+ * (pre-fill rows with -1).  values: 1 (real) or 2 (complex) doubles per
+ * entry.  rows_dev == NULL: only *count (entries needed) is returned. */
+int qes_op_table_coo(qes_op_t op, int32_t* rows_dev, int32_t* cols_dev, double* vals_dev, int64_t capacity,
+                     int64_t* count, void* stream);
 
 /* Number of off-diagonal gathers of one matvec (E_off) -- stats pass. */
 int qes_op_count_offdiag(qes_op_t op, int64_t* e_off);

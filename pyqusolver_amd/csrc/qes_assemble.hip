@@ -373,6 +373,55 @@ int op_assemble(Op* op, int enable, int want_dict, hipStream_t st) {
   return rc;
 }
 
+// Coordinate list of the assembled table: slot e of row a -> (a, col, value);
+// dictionary formats append the diagonal at entries + a.  Slots of rows past
+// n in the last slice are left untouched (callers pre-fill rows with -1).
+__global__ void __launch_bounds__(256) asm_coo_kernel(AsmTable T, int64_t n, int64_t entries, int32_t* __restrict__ rows,
+                                                      int32_t* __restrict__ cols, double* __restrict__ vals) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int64_t s = a / SELL_C;
+  const int l = (int)(a % SELL_C);
+  const int64_t base = T.off[s] + l;
+  const int len = T.len[s];
+  const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
+  const int vw = T.vc ? 2 : 1;
+  for (int k = 0; k < len; ++k) {
+    const int64_t e = base + (int64_t)k * SELL_C;
+    rows[e] = (int32_t)a;
+    if (T.fmt == ASM_PLAIN) {
+      cols[e] = T.col[e];
+      vals[e * vw] = T.val[e * vw];
+      if (T.vc) vals[e * vw + 1] = T.val[e * vw + 1];
+    } else if (T.fmt == ASM_P4) {
+      const uint32_t p = T.packed[e];
+      cols[e] = (int32_t)(p & cmask);
+      vals[e] = T.dict[p >> T.colbits];
+    } else {
+      cols[e] = T.col[e];
+      vals[e] = T.dict[T.vidx[e]];
+    }
+  }
+  if (T.fmt != ASM_PLAIN) {
+    rows[entries + a] = (int32_t)a;
+    cols[entries + a] = (int32_t)a;
+    vals[entries + a] = T.diag[a];
+  }
+}
+
+int assembled_coo(const Op* op, int32_t* rows, int32_t* cols, double* vals, int64_t capacity, hipStream_t st) {
+  if (op->asm_fmt < 0) return fail("operator is not assembled");
+  const int64_t n = op->basis->n;
+  const int64_t need = op->asm_entries + (op->asm_fmt == ASM_PLAIN ? 0 : n);
+  if (capacity < need) return fail("coordinate buffers hold " + std::to_string(capacity) + " entries, need " +
+                                   std::to_string(need));
+  if (n == 0) return 0;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  hipLaunchKernelGGL(asm_coo_kernel, grid, dim3(256), 0, st, asm_table_of(op), n, op->asm_entries, rows, cols, vals);
+  QES_HIP(hipGetLastError());
+  return 0;
+}
+
 int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int accumulate, hipStream_t st) {
   const int64_t n = op->basis->n;
   if (n == 0) return 0;

@@ -409,6 +409,16 @@ int qes_op_assembly_info(qes_op_t h, int64_t* entries, int64_t* bytes, int32_t* 
   return 0;
 }
 
+int qes_op_table_coo(qes_op_t h, int32_t* rows_dev, int32_t* cols_dev, double* vals_dev, int64_t capacity,
+                     int64_t* count, void* stream) {
+  if (!h) return fail("null operator");
+  const Op& op = h->op;
+  const int64_t need = op.asm_fmt < 0 ? 0 : op.asm_entries + (op.asm_fmt == 0 ? 0 : op.basis->n);
+  if (count) *count = need;
+  if (!rows_dev) return op.asm_fmt < 0 ? fail("operator is not assembled") : 0;   // size query
+  return assembled_coo(&op, rows_dev, cols_dev, vals_dev, capacity, (hipStream_t)stream);
+}
+
 int qes_op_get_plan(qes_op_t h, int32_t* canon, int32_t* n_classes, int32_t* n_generic) {
   if (!h) return fail("null operator");
   if (canon) *canon = h->op.plan.canon;

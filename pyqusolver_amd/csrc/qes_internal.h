@@ -118,6 +118,7 @@ int build_plan(Op* op, int canon_override);   // -1: automatic
 int op_assemble(Op* op, int enable, int want_dict, hipStream_t st);
 void free_assembly(Op* op);
 int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int accumulate, hipStream_t st);
+int assembled_coo(const Op* op, int32_t* rows, int32_t* cols, double* vals, int64_t capacity, hipStream_t st);
 int pick_canon_public(const QesGroup& G);
 void free_plan(Op* op);
 int canon_check(const Basis* B, int kind, const uint32_t* d_lut, const uint64_t* d_states, int64_t n,

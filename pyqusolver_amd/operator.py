@@ -240,6 +240,40 @@ class Hamiltonian:
     def assembly_bytes(self) -> int:
         return self.assembly["bytes"]
 
+    def build(self, use_numpy: bool = False, max_gb: float = 0.0, **kwargs):
+        """Explicit sparse matrix of the reduced H (Hamiltonian.build,
+        Algebra/hamil.py:1434-1557): the assembled table is built on the
+        device and converted to scipy CSR (duplicate entries summed, padding
+        dropped).  ``use_numpy`` returns a dense array instead."""
+        import scipy.sparse as sp
+        built = False
+        if self.assembly_bytes == 0:
+            self.assemble(True, max_gb=max_gb)
+            built = True
+        try:
+            L = _lib.lib()
+            cnt = C.c_int64()
+            _lib.check(L.qes_op_table_coo(self._device_op(), None, None, None, 0, C.byref(cnt), None),
+                       "qes_op_table_coo")
+            m = int(cnt.value)
+            cplx = self.needs_complex
+            rows = torch.full((m,), -1, dtype=torch.int32, device="cuda")
+            cols = torch.zeros((m,), dtype=torch.int32, device="cuda")
+            vals = torch.zeros((m, 2) if cplx else (m,), dtype=torch.float64, device="cuda")
+            _lib.check(L.qes_op_table_coo(self._device_op(), _lib.dptr(rows), _lib.dptr(cols), _lib.dptr(vals), m,
+                                          C.byref(cnt), _lib.stream_ptr()), "qes_op_table_coo")
+            v = torch.view_as_complex(vals) if cplx else vals
+            keep = (rows >= 0) & (v != 0)
+            r, c_, v = rows[keep].cpu().numpy(), cols[keep].cpu().numpy(), v[keep].cpu().numpy()
+        finally:
+            if built:
+                self.assemble(False)
+        n = self.nh
+        A = sp.csr_matrix((v, (r, c_)), shape=(n, n))
+        A.sum_duplicates()
+        self._hamil_sparse = A
+        return A.toarray() if use_numpy else A
+
     def count_offdiag(self) -> int:
         v = C.c_int64()
         _lib.check(_lib.lib().qes_op_count_offdiag(self._device_op(), C.byref(v)), "qes_op_count_offdiag")

@@ -353,3 +353,21 @@ def test_expand_and_reduce_against_oracle(case):
     Hf = O.full_sparse_hamiltonian(ns, c, s, co, t.is_complex)
     r = Hf @ vf - e0 * vf
     assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(vf)
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if all(k != "Py" for k, _ in c[3]) and c[1] * c[2] <= 14],
+                         ids=_ids)
+def test_explicit_sparse_build_matches_oracle_matrix(case):
+    """Hamiltonian.build(): the device table as a scipy CSR matrix equals the
+    oracle's reduced matrix (push-form columns) and reproduces the matvec."""
+    kind, lx, ly, gens, n_up, model = case
+    hs, H, grp, b, t = build_pair(kind, lx, ly, gens, n_up, model)
+    c, s, co = t.arrays()
+    ns = lx * ly * (2 if kind == "honeycomb" else 1)
+    M = O.reduced_dense(b, ns, c, s, co, t.is_complex)
+    A = H.build()
+    assert A.shape == (hs.nh, hs.nh)
+    np.testing.assert_allclose(A.toarray(), M, rtol=0, atol=1e-13 * np.abs(M).max())
+    x = np.random.default_rng(2).standard_normal(hs.nh) + 0j
+    np.testing.assert_allclose(A @ x, H.matvec_fun(x), rtol=0, atol=1e-12 * np.abs(A @ x).max())
+    assert H.assembly_bytes == 0     # the table built for the export is freed
