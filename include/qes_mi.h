@@ -124,6 +124,18 @@ int qes_op_needs_complex(qes_op_t op, int32_t* needs);
 int qes_matvec(qes_op_t op, const void* x_dev, void* y_dev, int64_t nbatch, int32_t dtype, int32_t accumulate,
                void* stream);
 
+/*
+ * Sector-changing product y_out = U_out^dagger O U_in x_in (complex128 in and
+ * out): O compiled on the input basis (qes_op_create), the output basis a
+ * sector of the same symmetry group with other quantum numbers (momentum,
+ * parity, U(1) filling).  Replaces Operator.matvec(..., hilbert_out=...)
+ * (Algebra/Operator/operator.py:1559-1602 -> adapters.py:193-206) and its
+ * kernel _apply_op_batch_projected_compact_seq_jit
+ * (Algebra/Symmetries/jit/matrix_builder_jit.py:51-108); pull form, one lane per
+ * output representative, no atomics.
+ */
+int qes_projected_matvec(qes_op_t op, qes_basis_t out, const void* x_dev, void* y_dev, void* stream);
+
 /* y[c] = H_c x for every coupling id c (y has n_couplings * n entries). */
 int qes_matvec_couplings(qes_op_t op, const void* x_dev, void* y_dev, int32_t dtype, void* stream);
 int qes_op_n_couplings(qes_op_t op, int32_t* n);

@@ -676,6 +676,38 @@ def rotation_matrix(group: Group, basis: Basis) -> np.ndarray:
     return U
 
 
+def projected_matvec(group_in: Group, basis_in: Basis, group_out: Group, basis_out: Basis, ns: int, codes, sites,
+                     coeffs, is_complex: bool, x: np.ndarray) -> np.ndarray:
+    """_apply_op_batch_projected_compact_seq_jit
+    (Algebra/Symmetries/jit/matrix_builder_jit.py:51-108): for every input
+    representative k and group element g, apply the operator to g r_k, then
+    for every h with h(s') a representative of the output sector add
+        val * conj(chi_in(g)) ph_g / N_k * conj(chi_out(h)) conj(ph_h) / (|G| N_out)
+    (both sectors share the group elements; only the characters differ)."""
+    G = group_in.n_group
+    out = np.zeros(len(basis_out.reps), dtype=np.complex128)
+    x = np.asarray(x, dtype=np.complex128)
+    for g in range(G):
+        s, ph_g = group_in.apply(basis_in.reps, g)
+        w_g = np.conj(group_in.chi[g]) * ph_g / basis_in.norms
+        for st, val, keep in vm_apply(s, ns, codes, sites, coeffs, is_complex):
+            m = keep & (np.abs(val) >= 1e-15)
+            k = np.nonzero(m)[0]
+            if len(k) == 0:
+                continue
+            vw = val[m] * w_g[m]
+            for h in range(G):
+                so, ph_h = group_out.apply(st[m], h)
+                pos = np.searchsorted(basis_out.reps, so)
+                pos = np.minimum(pos, len(basis_out.reps) - 1)
+                hit = (len(basis_out.reps) > 0) & (basis_out.reps[pos] == so)
+                if not np.any(hit):
+                    continue
+                w_h = np.conj(group_out.chi[h]) * np.conj(ph_h[hit]) / (G * basis_out.norms[pos[hit]])
+                np.add.at(out, pos[hit], vw[hit] * w_h * x[k[hit]])
+    return out
+
+
 def expand_state(group: Group, basis: Basis, x: np.ndarray) -> np.ndarray:
     """expand_to_full_state_jit (jit/symmetry_container_jit.py:261-315),
     push form over every group element."""

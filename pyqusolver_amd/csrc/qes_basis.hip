@@ -384,18 +384,6 @@ __global__ void map_states_kernel(QesGroup G, QesLookup L, const uint64_t* __res
 // element g r_k has conj(chi(g*)) = chi(g), each distinct state is met |Stab|
 // times, so red[k] = sum_g chi(g) psi[g r_k] / (N_k |Stab|).
 // --------------------------------------------------------------------------
-template <typename W>
-__device__ __forceinline__ W apply_element(const GV& G, int g, W s) {
-  const int B = g % G.nbase;
-  const int ab = g / G.nbase;
-  const int b = ab % G.nty, a = ab / G.nty;
-  int dummy = 0;
-  const W mask = (W)G.mask;
-  W v = apply_base<W>(G, B, s, &dummy);
-  v = rotl_rows<W>(v, a * G.cell_bits, G.row_bits, (W)G.rowlo[a], mask);
-  return rotl_ns<W>(v, b * G.row_bits, G.ns, mask);
-}
-
 template <typename W, bool CPLX>
 __global__ void expand_kernel(QesGroup G, QesLookup L, const double* __restrict__ chi_re,
                               const double* __restrict__ chi_im, const double* __restrict__ norms, double inv_sqrt_g,

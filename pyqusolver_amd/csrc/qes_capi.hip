@@ -391,6 +391,11 @@ int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
   return build_plan(&op, op.plan.canon == pick_canon_public(op.basis->group) ? -1 : op.plan.canon);
 }
 
+int qes_projected_matvec(qes_op_t op, qes_basis_t out, const void* x_dev, void* y_dev, void* stream) {
+  if (!op || !out || !x_dev || !y_dev) return fail("null argument");
+  return projected_matvec(&op->op, &out->b, (const double*)x_dev, (double*)y_dev, (hipStream_t)stream);
+}
+
 int qes_op_assemble(qes_op_t h, int32_t mode, double max_gb, void* stream) {
   if (!h) return fail("null operator");
   if (mode < 0 || mode > 2) return fail("mode must be 0 (free), 1 (auto format) or 2 (plain values)");

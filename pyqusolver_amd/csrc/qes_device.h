@@ -128,6 +128,20 @@ __device__ __forceinline__ W apply_base(const GV& G, int B, W s, int* ph_quarter
   return v;
 }
 
+// apply group element g = (a * nty + b) * nbase + B (reference order): base
+// element B first, then a cell shifts along x and b row shifts along y
+template <typename W>
+__device__ __forceinline__ W apply_element(const GV& G, int g, W s) {
+  const int B = g % G.nbase;
+  const int ab = g / G.nbase;
+  const int b = ab % G.nty, a = ab / G.nty;
+  int dummy = 0;
+  const W mask = (W)G.mask;
+  W v = apply_base<W>(G, B, s, &dummy);
+  v = rotl_rows<W>(v, a * G.cell_bits, G.row_bits, (W)G.rowlo[a], mask);
+  return rotl_ns<W>(v, b * G.row_bits, G.ns, mask);
+}
+
 // --------------------------------------------------------------------------
 // Canonicalisation: representative = min over G of g(s)
 // (scan_chunk_find_representatives, jit/symmetry_container_jit.py:234-246).

@@ -113,6 +113,7 @@ int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int 
 int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_t st);
 int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st);
 int op_diagonal(const Op* op, double* dre, double* dim, hipStream_t st);
+int projected_matvec(const Op* op, const Basis* out, const double* x, double* y, hipStream_t st);
 int build_plan(Op* op, int canon_override);   // -1: automatic
 // assembled operator (qes_assemble.hip)
 int op_assemble(Op* op, int enable, int want_dict, hipStream_t st);

@@ -16,6 +16,7 @@
 // only where the two bits differ via (x ^ rot(x, d)) masks, and generic term
 // groups evaluated one by one.
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -478,6 +479,89 @@ int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st) {
   QES_HIP(hipStreamSynchronize(st));
   QES_HIP(hipFree(d));
   *e_off = (int64_t)h;
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// Sector-changing (projected) product.  The reference pushes every input
+// representative through every group element, the operator and every output
+// element (_apply_op_batch_projected_compact_seq_jit,
+// Algebra/Symmetries/jit/matrix_builder_jit.py:51-108), which is
+//     y = U_out^dagger O U_in x
+// with U the symmetric-basis isometries.  Pull form, one lane per output
+// representative r: for every g, s' = g r, and every term t of O,
+// u = s' ^ mask_t is an input state with <s'|O|u> = val_t(u), and
+//     y[r] = sum_g chi_out(g) sum_t val_t(u) chi_in(g*_u) sqrt|Stab(u)| x[k(u)]
+//            / (|G| N_r)
+// (g*_u: first minimiser of u, k(u): index of its representative).
+// No atomics; no intrinsic phases (parity-y sectors are refused).
+// --------------------------------------------------------------------------
+template <typename W>
+__global__ void __launch_bounds__(256) projected_kernel(QesGroup G, QesLookup Lin, const uint64_t* __restrict__ reps_out,
+                                                        const double* __restrict__ norms_out,
+                                                        const double* __restrict__ chi_out_re,
+                                                        const double* __restrict__ chi_out_im,
+                                                        const double* __restrict__ chi_in_re,
+                                                        const double* __restrict__ chi_in_im, int n_up_in, int z2_in,
+                                                        const QesTermGroup* __restrict__ groups, int nterms, int gsize,
+                                                        const double* __restrict__ x, double* __restrict__ y,
+                                                        int64_t n_out) {
+  __shared__ uint64_t s_rowlo[QES_MAX_TX];
+  const GV gv = make_gv(G, s_rowlo);
+  __syncthreads();
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_out) return;
+  const W r = (W)reps_out[a];
+  double ar = 0.0, ai = 0.0;
+  for (int g = 0; g < gsize; ++g) {
+    const W sp = apply_element<W>(gv, g, r);
+    const double cor = chi_out_re[g], coi = chi_out_im[g];
+    for (int t = 0; t < nterms; ++t) {
+      const QesTermGroup& tg = groups[t];
+      const W u = sp ^ (W)tg.mask;
+      const int kb = term_bits<W>(tg, u);
+      const double vr = sel4(tg.re, kb), vi = sel4(tg.im, kb);
+      if (vr * vr + vi * vi < 1e-30) continue;
+      if (n_up_in >= 0 && popc<W>(u) != n_up_in) continue;
+      if (z2_in != 0 && ((popc<W>(u) & 1) ? -1 : 1) != z2_in) continue;
+      const Canon<W> cn = canonicalize<W>(gv, u);
+      const int64_t k = lookup_rep<W>(Lin, cn.rep);
+      if (k < 0) continue;
+      const double nb = sqrt((double)cn.count);
+      // w = chi_out(g) * val * chi_in(g*) * sqrt|Stab|
+      const double cir = chi_in_re[cn.g], cii = chi_in_im[cn.g];
+      const double pr = cor * vr - coi * vi, pi = cor * vi + coi * vr;
+      const double wr = (pr * cir - pi * cii) * nb, wi = (pr * cii + pi * cir) * nb;
+      const double xr = x[2 * k], xi = x[2 * k + 1];
+      ar += wr * xr - wi * xi;
+      ai += wr * xi + wi * xr;
+    }
+  }
+  const double inv = 1.0 / ((double)gsize * norms_out[a]);
+  y[2 * a] = ar * inv;
+  y[2 * a + 1] = ai * inv;
+}
+
+int projected_matvec(const Op* op, const Basis* out, const double* x, double* y, hipStream_t st) {
+  const Basis* in = op->basis;
+  const QesGroup &Gi = in->group, &Go = out->group;
+  if (Gi.ns != Go.ns || Gi.nbase != Go.nbase || Gi.ntx != Go.ntx || Gi.nty != Go.nty || Gi.row_bits != Go.row_bits ||
+      Gi.cell_bits != Go.cell_bits || memcmp(Gi.base_ops, Go.base_ops, sizeof(Gi.base_ops)) != 0)
+    return fail("input and output sectors must share the group elements (same generators); only the sector "
+                "quantum numbers may differ");
+  if (Gi.has_phase || Go.has_phase) return fail("projected products with parity-y phases are not supported");
+  if (out->n == 0) return 0;
+  const int nterms = op->n_diag + op->n_off;
+  const dim3 grid((unsigned)((out->n + 255) / 256));
+  if (Gi.ns <= 32)
+    hipLaunchKernelGGL(projected_kernel<uint32_t>, grid, dim3(256), 0, st, Go, in->lookup(), out->reps, out->norms,
+                       out->chi_re, out->chi_im, in->chi_re, in->chi_im, Gi.n_up, Gi.z2, op->d_groups, nterms,
+                       in->gsize, x, y, out->n);
+  else
+    hipLaunchKernelGGL(projected_kernel<uint64_t>, grid, dim3(256), 0, st, Go, in->lookup(), out->reps, out->norms,
+                       out->chi_re, out->chi_im, in->chi_re, in->chi_im, Gi.n_up, Gi.z2, op->d_groups, nterms,
+                       in->gsize, x, y, out->n);
+  QES_HIP(hipGetLastError());
   return 0;
 }
 

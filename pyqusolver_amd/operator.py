@@ -331,13 +331,30 @@ class Hamiltonian:
             self._cached_matvec_fun = self._matvec_any
         return self._cached_matvec_fun
 
+    def projected_matvec(self, vecs, hilbert_out):
+        """y = U_out^dagger O U_in x into another sector of the same symmetry
+        group (adapters.py:193-206, matrix_builder_jit.py:51-108); complex128
+        result, numpy in -> numpy out, device tensor in -> device tensor out."""
+        as_np = not isinstance(vecs, torch.Tensor)
+        x = torch.as_tensor(np.ascontiguousarray(vecs)) if as_np else vecs
+        if x.dim() != 1 or x.shape[0] != self.nh:
+            raise ValueError(f"vector shape {tuple(x.shape)} does not match nh={self.nh}")
+        x = x.to(device="cuda", dtype=torch.complex128).contiguous()
+        y = torch.empty(hilbert_out.nh, dtype=torch.complex128, device="cuda")
+        if hilbert_out.nh:
+            _lib.check(_lib.lib().qes_projected_matvec(self._device_op(), hilbert_out.handle, _lib.dptr(x), _lib.dptr(y),
+                                                       _lib.stream_ptr()), "qes_projected_matvec")
+        return y.cpu().numpy() if as_np else y
+
     def matvec(self, vecs, *args, hilbert_in=None, hilbert_out=None, out=None, **kwargs):
-        """Operator.matvec (operator.py:1559-1602) for the same-sector case."""
+        """Operator.matvec (operator.py:1559-1602): same sector, or into
+        ``hilbert_out`` (projected product) when it is another sector."""
         if hilbert_in is not None and hilbert_in is not self._hilbert_space:
-            raise ValueError("hilbert_in differs from the Hamiltonian's sector")
+            raise ValueError("hilbert_in differs from the operator's sector")
         if hilbert_out is not None and hilbert_out is not self._hilbert_space:
-            raise NotImplementedError("sector-changing (projected) matvec is not part of this build")
-        y = self._matvec_any(vecs)
+            y = self.projected_matvec(vecs, hilbert_out)
+        else:
+            y = self._matvec_any(vecs)
         if out is not None:
             if isinstance(out, torch.Tensor):
                 out.copy_(y if isinstance(y, torch.Tensor) else torch.from_numpy(y))
@@ -419,3 +436,11 @@ class Hamiltonian:
                 _lib.lib().qes_op_destroy(self._op_handle)
         except Exception:
             pass
+
+
+class Operator(Hamiltonian):
+    """A general spin-1/2 operator (term list on a sector), e.g. a
+    momentum-resolved S^z_q for sector-changing products; same API."""
+
+    def __init__(self, hilbert_space, dtype=np.complex128, name: str = "Operator", **kwargs):
+        super().__init__(hilbert_space, dtype=dtype, name=name, **kwargs)

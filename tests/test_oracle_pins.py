@@ -209,3 +209,29 @@ def test_expand_and_transform_to_reduced_closed_forms():
     np.testing.assert_allclose(full, O.rotation_matrix(grp, b) @ x, rtol=0, atol=1e-14)
     back = O.transform_to_reduced(b, 10, full)
     np.testing.assert_allclose(back, x * np.sqrt(grp.n_group) / b.norms ** 2, rtol=1e-12)
+
+
+def test_projected_matvec_is_projected_operator():
+    """The restated projected kernel equals U_out^dagger O U_in built from the
+    pinned rotation matrices and the Kronecker full-space operator, and in
+    its own sector it is the ordinary push-form matvec."""
+    L = 8
+    lat = O.OLattice("chain", L)
+    for kin, kout, nin, nout, code in ((0, 3, 4, 4, 13), (1, 2, 4, 5, 14)):
+        gi = O.Group(lat, [O.Generator("tx", kin)], n_up=nin)
+        go = O.Group(lat, [O.Generator("tx", kout)], n_up=nout)
+        bi, bo = O.scan_basis(gi), O.scan_basis(go)
+        q = 2 * np.pi * (kout - kin) / L
+        codes, sites = [code] * L, [[j, -1] for j in range(L)]
+        coeffs = [np.exp(1j * q * j) for j in range(L)]
+        x = np.random.default_rng(kin).standard_normal(len(bi.reps)) + 0.5j
+        y = O.projected_matvec(gi, bi, go, bo, L, codes, sites, coeffs, True, x)
+        Of = O.full_dense_hamiltonian(L, codes, sites, coeffs, True)
+        yb = O.rotation_matrix(go, bo).conj().T @ Of @ O.rotation_matrix(gi, bi) @ x
+        np.testing.assert_allclose(y, yb, rtol=0, atol=1e-13 * max(1.0, np.abs(yb).max()))
+    gi = O.Group(lat, [O.Generator("tx", 2)], n_up=4)
+    bi = O.scan_basis(gi)
+    c, s, co = O.terms_xxz(lat, 1.0, 0.7).arrays()
+    x = np.random.default_rng(1).standard_normal(len(bi.reps)) + 0j
+    np.testing.assert_allclose(O.projected_matvec(gi, bi, gi, bi, L, c, s, co, True, x),
+                               O.push_matvec(bi, L, c, s, co, True, x), rtol=0, atol=1e-13)
