@@ -143,3 +143,40 @@ def test_krylov_overlap_gradient_vs_central_difference():
         g_ad = J.grad.clone()
         g_fd = central_difference(lambda Jp: krylov_overlap(H, Jp, psi0, phi, t=1.0, m=30), J.detach(), h=1e-5)
         np.testing.assert_allclose(g_ad.numpy(), g_fd.numpy(), rtol=1e-6, atol=1e-9)
+
+
+def test_structure_factor_continued_fraction_against_dense():
+    """S^z_q S^z_-q dynamical correlation of the L=12 Heisenberg chain through
+    the projected product (k=0 -> k=q) and a continued fraction run to the
+    full Krylov dimension: equal to the dense spectral sum
+    sum_n |<n|phi>|^2 / (z - E_n) in the output sector."""
+    import torch
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import J1J2Model
+    from pyqusolver_amd.operator import Operator, SpinOperator
+    from pyqusolver_amd.spectral import continued_fraction, structure_factor
+    L, kq = 12, 3
+    lat = ChainLattice(L)
+    hs0 = HilbertSpace(lattice=lat, sym_gen={"translation": 0}, global_syms=[get_u1_sym(lat, 6)])
+    hsq = HilbertSpace(lattice=lat, sym_gen={"translation": kq}, global_syms=[get_u1_sym(lat, 6)])
+    H0 = J1J2Model(lat, J1=-1.0, J2=0.0, hilbert_space=hs0)
+    Hq = J1J2Model(lat, J1=-1.0, J2=0.0, hilbert_space=hsq, dtype=np.complex128)
+    e0 = H0.diagonalize(k=1, tol=1e-14, maxiter=hs0.nh, compute_eigvec=True)[0]
+    psi0 = H0.eig_vec.cpu().numpy().astype(np.complex128)
+    psi0 /= np.linalg.norm(psi0)
+    q = 2 * np.pi * kq / L
+    op = Operator(hs0)
+    for j in range(L):
+        op.add(SpinOperator("Sz", 13, "local", False), np.exp(1j * q * j), sites=[j])
+    phi = op.matvec(psi0, hilbert_out=hsq)
+    cf = continued_fraction(Hq, phi, m=hsq.nh)
+    Hd = Hq.build().toarray()
+    w, V = np.linalg.eigh(Hd)
+    amp = np.abs(V.conj().T @ phi) ** 2
+    z = np.linspace(0.0, 4.0, 41) + e0 + 0.1j
+    g_dense = (amp[None, :] / (z[:, None] - w[None, :])).sum(axis=1)
+    np.testing.assert_allclose(cf.green(z), g_dense, rtol=1e-9, atol=1e-12)
+    s = structure_factor(Hq, op, psi0, e0, np.linspace(0.0, 4.0, 41), eta=0.1, m=hsq.nh, hilbert_out=hsq)
+    np.testing.assert_allclose(s, -g_dense.imag / np.pi, rtol=1e-9, atol=1e-12)
+    assert abs(cf.weight - np.sum(amp)) <= 1e-12 * cf.weight
