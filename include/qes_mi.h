@@ -149,19 +149,39 @@ int qes_op_diagonal(qes_op_t op, void* re_dev, void* im_dev, void* stream);
  * qes_matvec with nbatch == 1 streams the table instead of canonicalising and
  * looking up neighbours.  The device counterpart of the reference's explicit
  * sparse build (Algebra/Hilbert/matrix_builder.py:151-206, :911-1030), for
- * repeated application (Lanczos).  mode 0 frees it; 1 picks the format: real
- * operators whose off-diagonal values fit a dictionary (< 2^16 distinct
- * values) store a value index beside the column (packed into 32 bits when
- * index and column bits fit, format 1; else int32 + uint16, format 2) and
- * the diagonal separately; otherwise column + f64/c128 value with the
- * diagonal as entry 0 (format 0); mode 2 forces format 0.  The build fails
- * if the table would exceed max_gb (<= 0: 64 GB default).  Any change of the
- * operator's plan (qes_op_set_option / set_canonicalizer) drops the table.
- * qes_op_assembly_info: entries per table, bytes, format (-1: none) and
- * dictionary size.
+ * repeated application (Lanczos).  mode 0 frees it; 1 picks the format:
+ * operators whose off-diagonal values fit a dictionary (< 2^16 distinct real
+ * or complex values) store a value index beside the column (packed into 32
+ * bits when index and column bits fit, format 1; else int32 + uint16, format
+ * 2) and the diagonal separately; otherwise column + f64/c128 value with the
+ * diagonal as entry 0 (format 0); mode 2 forces format 0; mode 3 builds a
+ * coupling-tagged dictionary table (values keyed by coupling id, one
+ * diagonal per coupling) for qes_op_matvec_scaled / qes_op_coupling_dots,
+ * and fails when the values do not fit a dictionary or there are more than
+ * 8 coupling ids.  The build fails if the table would exceed max_gb (<= 0:
+ * 64 GB default).  Any change of the operator's plan (qes_op_set_option /
+ * set_canonicalizer) drops the table.  qes_op_assembly_info: entries per
+ * table, bytes, format (-1: none), dictionary size and 1 for a
+ * coupling-tagged table (any output pointer may be NULL).
  */
 int qes_op_assemble(qes_op_t op, int32_t mode, double max_gb, void* stream);
-int qes_op_assembly_info(qes_op_t op, int64_t* entries, int64_t* bytes, int32_t* format, int32_t* n_values);
+int qes_op_assembly_info(qes_op_t op, int64_t* entries, int64_t* bytes, int32_t* format, int32_t* n_values,
+                         int32_t* tagged);
+/* With a coupling-tagged table (mode 3): y = sum_c scale[c] H_c x, one pass
+ * over the table, for H(J) = sum_c (J_c / J0_c) H_c (the differentiable
+ * forward; the reference builds a fresh operator per coupling set,
+ * Algebra/hamil.py:1434-1557).  nscale = qes_op_n_couplings.  The scaled
+ * value dictionary is kept on the device and re-uploaded (blocking the host
+ * for the copy) only when the scales change; calls with different scales on
+ * different streams race. */
+int qes_op_matvec_scaled(qes_op_t op, const double* scale, int32_t nscale, const void* x_dev, void* y_dev,
+                         int32_t dtype, void* stream);
+/* With a coupling-tagged table: out[2c], out[2c+1] = Re, Im of
+ * sum_a conj(g_a) (H_c v)_a for every coupling id c (host array of
+ * 2 * n_couplings doubles), one pass over the table -- the coupling
+ * gradient of <g, H(J) v>.  Blocks the host until done. */
+int qes_op_coupling_dots(qes_op_t op, const void* v_dev, const void* g_dev, int32_t dtype, double* out,
+                         void* stream);
 /* Coordinate list (row, col, value) of the assembled table on the device --
  * the explicit sparse matrix of the reference's Hamiltonian.build
  * (Algebra/hamil.py:1434-1557 via matrix_builder.py:151-206).  Padding slots
@@ -220,13 +240,23 @@ int qes_lanczos_stats(qes_lanczos_t L, int32_t* second_passes);
 
 /*
  * All-in-one ground state: run up to max_steps Lanczos steps from v0 (device),
- * checking the lowest Ritz value every check_every steps; stop when it moves
- * by less than tol*|E0| or on breakdown.  alpha/beta (host, max_steps) and
- * *e0 receive the result.
+ * checking the lowest Ritz value theta every check_every steps; stop on
+ * breakdown or when the residual norm of the Ritz pair r = |beta_k s_k|
+ * (s: unit eigenvector of T_k) is <= tol*max(1,|theta|) -- the stopping test
+ * of the reference's eigsh/ARPACK path (Algebra/Hamil/hamil_diag_engine.py:
+ * 176-300); |theta - E0| <= r.  alpha/beta (host, max_steps) and *e0
+ * receive the result.
  */
 int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int32_t max_steps, int32_t reorth,
                              double tol, int32_t check_every, double* alpha_host, double* beta_host,
                              int32_t* steps_done, double* e0, void* stream);
+/* Host-only: lowest eigenvalue *theta of T_k (alpha[0..k-1], beta[0..k-2]),
+ * the residual norm *resid of its Ritz pair used by the stopping test above
+ * (beta[k-1] = beta_k) and, if quad is not NULL, the quadratic estimate
+ * min(r, r^2/gap) of |theta - E0| (gap to the second Ritz value; not a bound
+ * while a near-degenerate pair is unresolved).  Needs no device. */
+int qes_tridiag_ritz(const double* alpha_host, const double* beta_host, int32_t k, double* theta, double* resid,
+                     double* quad);
 
 /*
  * Batched Lanczos over many independent operators (sectors or disorder
@@ -275,6 +305,8 @@ int qes_batch_matvec(qes_batch_t b, const void* x_dev, void* y_dev, void* stream
  * qes_op_assemble before qes_batch_create stream that table; the batch keeps
  * pointers to it, so the assembly must outlive the batch). */
 int qes_batch_set_option(qes_batch_t b, const char* name, int64_t value);
+/* Lanczos on every problem of the batch; a problem stops on breakdown, at
+ * its dimension, or under the residual test of qes_lanczos_ground_state. */
 int qes_batch_ground_states(qes_batch_t b, double tol, int32_t check_every, double* e0_host, int32_t* steps_host);
 int qes_batch_destroy(qes_batch_t b);
 

@@ -7,15 +7,23 @@ models.py): with the operator compiled at couplings J0,
 
     H(J) = sum_c (J_c / J0_c) H_c ,   H_c = instructions tagged c at J0,
 
-so dH/dJ_c = H_c / J0_c.  One call of ``qes_matvec_couplings`` returns every
-H_c v in a single pass over the rows (one canonicalisation per neighbour for
-all couplings); forward and backward are built from it:
+so dH/dJ_c = H_c / J0_c.  Forward and backward are
 
     forward   y       = sum_c s_c H_c v                  (s = J / J0)
     backward  grad_v  = H(J)^dagger grad_y = H(J) grad_y (H Hermitian)
               grad_s_c = Re <grad_y, H_c v>
 
-with torch's convention grad = dL/dRe + i dL/dIm for complex tensors.
+with torch's convention grad = dL/dRe + i dL/dIm for complex tensors.  Two
+device paths compute them:
+
+  * a coupling-tagged assembled table (``Hamiltonian.assemble(couplings=
+    True)``: every tabulated value keyed by its coupling id, one diagonal
+    per coupling): y is one pass with a scaled value dictionary
+    (``qes_op_matvec_scaled``) and all Re <grad_y, H_c v> are one more pass
+    (``qes_op_coupling_dots``);
+  * matrix-free: ``qes_matvec_couplings`` returns every H_c v in a single
+    pass over the rows (one canonicalisation per neighbour for all
+    couplings) and the sums are taken with torch.
 
 On top of it:
   * ``lanczos_energy``: E0(J) from an m-step Lanczos recurrence written with
@@ -61,12 +69,14 @@ class HamiltonianMatvec(torch.autograd.Function):
     def forward(ctx, s: torch.Tensor, v: torch.Tensor, ham):
         if v.device.type != "cuda":
             raise ValueError("HamiltonianMatvec works on device tensors")
+        ctx.ham = ham
+        ctx.tagged = ham.assembly["couplings"]
+        ctx.save_for_backward(s, v)
+        if ctx.tagged:
+            return ham.matvec_scaled_device(s.detach().double().cpu().numpy(), v)
         Y = ham.matvec_couplings_device(v.contiguous())          # (n_c, N)
         sc = s.detach().to(device=v.device, dtype=torch.float64)
-        y = torch.tensordot(sc.to(Y.dtype), Y, dims=([0], [0]))
-        ctx.ham = ham
-        ctx.save_for_backward(s, v)
-        return y
+        return torch.tensordot(sc.to(Y.dtype), Y, dims=([0], [0]))
 
     @staticmethod
     def backward(ctx, g: torch.Tensor):
@@ -74,6 +84,13 @@ class HamiltonianMatvec(torch.autograd.Function):
         ham = ctx.ham
         grad_s = grad_v = None
         g = g.contiguous()
+        if ctx.tagged:
+            if ctx.needs_input_grad[1]:
+                grad_v = ham.matvec_scaled_device(s.detach().double().cpu().numpy(), g)
+            if ctx.needs_input_grad[0]:
+                d = ham.coupling_dots_device(v.contiguous(), g)
+                grad_s = torch.as_tensor(d.real, dtype=s.dtype, device=s.device)
+            return grad_s, grad_v, None
         if ctx.needs_input_grad[1]:
             Yg = ham.matvec_couplings_device(g)
             sc = s.detach().to(device=g.device, dtype=torch.float64)
@@ -90,14 +107,27 @@ class HamiltonianMatvec(torch.autograd.Function):
 
 
 class DifferentiableHamiltonian:
-    """H(J) as a differentiable linear map: ``dh(J, v)`` = H(J) v."""
+    """H(J) as a differentiable linear map: ``dh(J, v)`` = H(J) v.
 
-    def __init__(self, ham, coupling_values: Optional[Sequence[float]] = None):
+    ``assemble``: True builds a coupling-tagged table for the operator (an
+    error if its values do not fit one), False keeps whatever the operator
+    has, None (default) tries the table for spaces of 65536 states or more
+    and keeps the matrix-free path when the values do not fit."""
+
+    def __init__(self, ham, coupling_values: Optional[Sequence[float]] = None, assemble: Optional[bool] = None,
+                 max_gb: float = 0.0):
+        from . import _lib
         self.ham = ham
         if coupling_values is not None:
             ham.coupling_values = np.asarray(coupling_values, dtype=np.float64)
         self.j0 = _couplings_of(ham)
         self.n_couplings = len(self.j0)
+        if assemble or (assemble is None and ham.nh >= 65536 and not ham.assembly["couplings"]):
+            try:
+                ham.assemble(True, max_gb=max_gb, couplings=True)
+            except _lib.QesError:
+                if assemble:
+                    raise
 
     @property
     def J0(self) -> torch.Tensor:
@@ -172,10 +202,13 @@ def lanczos_energy(H: DifferentiableHamiltonian, J: torch.Tensor, v0: torch.Tens
 
 def hellmann_feynman(H: DifferentiableHamiltonian, psi: torch.Tensor) -> torch.Tensor:
     """dE0/dJ_c = <psi|H_c|psi> / (<psi|psi> J0_c) for an eigenvector psi."""
-    Y = H.ham.matvec_couplings_device(psi.contiguous())
     nrm = _dot(psi, psi).real
-    vals = torch.stack([_dot(psi, Y[c]).real for c in range(Y.shape[0])]) / nrm
-    return vals.cpu() / torch.as_tensor(H.j0, dtype=torch.float64)
+    if H.ham.assembly["couplings"]:
+        vals = torch.as_tensor(H.ham.coupling_dots_device(psi, psi).real) / nrm.cpu()
+    else:
+        Y = H.ham.matvec_couplings_device(psi.contiguous())
+        vals = torch.stack([_dot(psi, Y[c]).real for c in range(Y.shape[0])]).cpu() / nrm.cpu()
+    return vals / torch.as_tensor(H.j0, dtype=torch.float64)
 
 
 def krylov_propagate(H: DifferentiableHamiltonian, J: torch.Tensor, phi: torch.Tensor, t: float, m: int,

@@ -7,7 +7,7 @@
 // Algebra/Hilbert/matrix_builder.py:151-206, :911-1030) next to the
 // matrix-free matvec_fun.  Here the table is built on the device by the same
 // row visitor the matrix-free kernel uses, in the same order and with the
-// same arithmetic (entry 0 is the diagonal, then the off-diagonal elements in
+// same arithmetic (the diagonal first, then the off-diagonal elements in
 // visit order with their value already divided by N_a), so the assembled
 // product is the matrix-free one up to the contraction of the final adds.
 //
@@ -15,9 +15,21 @@
 // entry k of row 64s+l sits at off[s] + 64k + l.  Short rows are padded with
 // (col = own row, value 0).  A matvec then streams the table with perfectly
 // coalesced loads and gathers x[col]: no canonicalisation, no lookup.
+//
+// Formats.  PLAIN: int32 column + f64 (or c128) value per entry, the
+// diagonal as entry 0.  When the off-diagonal values take fewer than 2^16
+// distinct values (products of a coupling, a character and a norm ratio) an
+// entry stores an index into a value dictionary instead: P4 packs column and
+// index into one uint32 (column in the low bits), P6 keeps an int32 column
+// and a uint16 index; the diagonal lives in its own array.  A
+// coupling-tagged table keys its dictionary by (value, coupling id) and keeps
+// one diagonal per coupling, which gives H(J) x for any scale of each coupling
+// and the per-coupling products Re<g, H_c v> from one pass over the table.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -27,54 +39,83 @@
 
 namespace qes {
 
-
-// Formats.  PLAIN: int32 column + f64 (or c128) value per entry, the
-// diagonal as entry 0.  Real operators whose off-diagonal values take few
-// distinct values (they are products of a coupling, a character and a norm
-// ratio) store an index into a value dictionary instead: P4 packs column and
-// index into one uint32 (column in the low bits), P6 keeps an int32 column
-// and a uint16 index; the diagonal then lives in its own f64 array.
 enum { PASS_COUNT = 0, PASS_DICT = 1, PASS_FILL = 2 };
 
 static constexpr int DICT_HBITS = 16;                  // dictionary hash slots
-static constexpr uint64_t DICT_EMPTY = ~0ull;          // a NaN bit pattern, never a value
+static constexpr uint64_t DICT_EMPTY = ~0ull;
 
-__host__ __device__ __forceinline__ uint32_t dict_slot(uint64_t bits) {
-  return (uint32_t)((bits * 0x9E3779B97F4A7C15ull) >> (64 - DICT_HBITS));
+// fingerprint of a dictionary key (value re, value im, coupling id)
+__host__ __device__ __forceinline__ uint64_t dict_key(double vr, double vi, int cpl) {
+  uint64_t a, b;
+  memcpy(&a, &vr, 8);
+  memcpy(&b, &vi, 8);
+  uint64_t h = a ^ ((b << 29) | (b >> 35)) ^ ((uint64_t)(cpl + 1) * 0xD6E8FEB86659FD93ull);
+  h ^= h >> 31;
+  h *= 0x7FB5D329728EA185ull;
+  h ^= h >> 27;
+  h *= 0x81DADEF4BC2DD44Dull;
+  h ^= h >> 33;
+  return h == DICT_EMPTY ? 0x9E3779B97F4A7C15ull : h;
 }
 
-// insert (PASS_DICT) or find (PASS_FILL) a value's bit pattern
-__device__ __forceinline__ uint32_t dict_insert(unsigned long long* tab, uint64_t bits) {
-  uint32_t h = dict_slot(bits);
+__host__ __device__ __forceinline__ uint32_t dict_slot(uint64_t key) {
+  return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - DICT_HBITS));
+}
+
+struct DictBuf {
+  unsigned long long* key;   // slots
+  double* vr;                // value of the slot (written by the inserting lane)
+  double* vi;
+  int32_t* cpl;
+  const uint16_t* idx;       // PASS_FILL: dictionary index of each slot
+  int32_t* bad;              // PASS_FILL: set when a probe finds another value under the key
+};
+
+__device__ __forceinline__ void dict_insert(const DictBuf& D, double vr, double vi, int cpl) {
+  const uint64_t key = dict_key(vr, vi, cpl);
+  uint32_t h = dict_slot(key);
   const uint32_t mask = (1u << DICT_HBITS) - 1u;
   for (uint32_t probe = 0; probe <= mask; ++probe) {
-    const unsigned long long cur = tab[h];
-    if (cur == bits) return h;
+    const unsigned long long cur = D.key[h];
+    if (cur == key) return;
     if (cur == DICT_EMPTY) {
-      const unsigned long long prev = atomicCAS(&tab[h], DICT_EMPTY, (unsigned long long)bits);
-      if (prev == DICT_EMPTY || prev == bits) return h;
+      const unsigned long long prev = atomicCAS(&D.key[h], DICT_EMPTY, (unsigned long long)key);
+      if (prev == DICT_EMPTY) {
+        D.vr[h] = vr;
+        D.vi[h] = vi;
+        D.cpl[h] = cpl;
+        return;
+      }
+      if (prev == key) return;
     }
     h = (h + 1) & mask;
   }
-  return ~0u;   // table full: the host sees more than 2^16 distinct values and falls back
+  // table full: the host sees a full table and does not use a dictionary
 }
 
-// every value met by the fill pass was inserted by the dictionary pass (same
-// visitor, same arithmetic), so the probe ends at its slot; the bound only
-// guards the loop
-__device__ __forceinline__ uint32_t dict_find(const unsigned long long* __restrict__ tab, uint64_t bits) {
-  uint32_t h = dict_slot(bits);
+// every key met by the fill pass was inserted by the dictionary pass (same
+// visitor, same arithmetic); the bound only guards the loop, and the stored
+// value is compared so that two values sharing a fingerprint are caught
+__device__ __forceinline__ uint32_t dict_find(const DictBuf& D, double vr, double vi, int cpl) {
+  const uint64_t key = dict_key(vr, vi, cpl);
+  uint32_t h = dict_slot(key);
   const uint32_t mask = (1u << DICT_HBITS) - 1u;
-  for (uint32_t probe = 0; probe <= mask && tab[h] != bits; ++probe) h = (h + 1) & mask;
-  return h;
+  for (uint32_t probe = 0; probe <= mask && D.key[h] != key; ++probe) h = (h + 1) & mask;
+  if (D.key[h] != key || D.vr[h] != vr || D.vi[h] != vi || D.cpl[h] != cpl) {
+    atomicOr(D.bad, 1);
+    return 0;
+  }
+  return D.idx[h];
 }
 
 struct AsmOut {
   int fmt;
   int colbits;                          // P4: bits of the column field
+  int tagged;                           // dictionary keyed by coupling id, per-coupling diagonals
+  int ncpl;                             // coupling ids of the operator
   int32_t* rowlen;                      // PASS_COUNT
-  unsigned long long* dict_tab;         // PASS_DICT / PASS_FILL: hash slots
-  const uint16_t* slot_idx;             // PASS_FILL: dictionary index of each slot
+  DictBuf D;                            // PASS_DICT / PASS_FILL
+  const int32_t* excpl;                 // coupling id of exchange element (class, site)
   const int64_t* off;                   // slice offsets
   const int32_t* slen;                  // slice lengths (entries per row)
   int32_t* col;                         // PLAIN / P6
@@ -82,6 +123,7 @@ struct AsmOut {
   uint32_t* packed;                     // P4
   uint16_t* vidx;                       // P6
   double* diag;                         // P4 / P6
+  double* diag_c;                       // tagged: [c][row]
 };
 
 template <typename W, bool CPLX, int CK, int PASS>
@@ -106,33 +148,58 @@ __global__ void __launch_bounds__(256) asm_rows_kernel(QesGroup G, QesLookup L, 
   int k = 0;
   if (PASS == PASS_FILL) {
     double dr = 0.0, di = 0.0;
+    double cr[ASM_MAX_CPL], ci[ASM_MAX_CPL];
+#pragma unroll
+    for (int c = 0; c < ASM_MAX_CPL; ++c) cr[c] = ci[c] = 0.0;
     for (int t = 0; t < n_diag; ++t) {
       const QesTermGroup& g = groups[t];
       const int kk = term_bits<W>(g, r);
-      dr += sel4(g.re, kk);
-      if (CPLX) di += sel4(g.im, kk);
+      const double vr = sel4(g.re, kk), vi = CPLX ? sel4(g.im, kk) : 0.0;
+      dr += vr;
+      if (CPLX) di += vi;
+      if (O.tagged) {
+#pragma unroll
+        for (int c = 0; c < ASM_MAX_CPL; ++c)
+          if (c == g.coupling) {
+            cr[c] += vr;
+            ci[c] += vi;
+          }
+      }
     }
     if (plain) {
       O.col[base] = (int32_t)a;
       O.val[base * cw] = dr;
       if (CPLX) O.val[base * cw + 1] = di;
     } else {
-      O.diag[a] = dr;
+      O.diag[a * cw] = dr;
+      if (CPLX) O.diag[a * cw + 1] = di;
+      if (O.tagged) {
+#pragma unroll
+        for (int c = 0; c < ASM_MAX_CPL; ++c) {
+          if (c < O.ncpl) {
+            O.diag_c[((int64_t)c * n + a) * cw] = cr[c];
+            if (CPLX) O.diag_c[((int64_t)c * n + a) * cw + 1] = ci[c];
+          }
+        }
+      }
     }
   }
   if (plain) ++k;   // entry 0 is the diagonal
-  visit_row<W, CPLX, CK>(C, r, [&](int64_t b, double fr, double fi, int) {
+  visit_row<W, CPLX, CK>(C, r, [&](int64_t b, double fr, double fi, int tag) {
     const double vr = fr * inv_na;
+    const double vi = CPLX ? fi * inv_na : 0.0;
+    int cpl = 0;
+    if (O.tagged) cpl = tag >= 0 ? O.excpl[tag] : groups[-1 - tag].coupling;
     if (PASS == PASS_DICT) {
-      dict_insert(O.dict_tab, (uint64_t)__double_as_longlong(vr));
+      dict_insert(O.D, vr, vi, cpl);
     } else if (PASS == PASS_FILL) {
       const int64_t e = base + (int64_t)k * SELL_C;
       if (plain) {
         O.col[e] = (int32_t)b;
         O.val[e * cw] = vr;
-        if (CPLX) O.val[e * cw + 1] = fi * inv_na;
+        if (CPLX) O.val[e * cw + 1] = vi;
       } else {
-        const uint32_t idx = O.slot_idx[dict_find(O.dict_tab, (uint64_t)__double_as_longlong(vr))];
+        const uint32_t idx = dict_find(O.D, vr, vi, cpl);
         if (O.fmt == ASM_P4) {
           O.packed[e] = (uint32_t)b | (idx << O.colbits);
         } else {
@@ -147,7 +214,7 @@ __global__ void __launch_bounds__(256) asm_rows_kernel(QesGroup G, QesLookup L, 
     O.rowlen[a] = k;
   } else if (PASS == PASS_FILL) {
     // pad to the slice length with (own row, value 0)
-    const uint32_t zidx = plain ? 0u : O.slot_idx[dict_find(O.dict_tab, 0ull)];
+    const uint32_t zidx = plain ? 0u : dict_find(O.D, 0.0, 0.0, 0);
     for (; k < O.slen[s]; ++k) {
       const int64_t e = base + (int64_t)k * SELL_C;
       if (plain) {
@@ -185,9 +252,94 @@ __global__ void __launch_bounds__(256) sell_matvec_kernel(AsmTable T, const doub
   }
 }
 
+// d_c = sum_a conj(g_a) (H_c v)_a for every coupling c of a tagged table, in
+// one pass: per row, the table entries are split by the coupling of their
+// dictionary value; per block partial sums part[block][c][re, im].
+template <bool XC>
+__global__ void __launch_bounds__(256) sell_coupling_dots_kernel(AsmTable T, const uint8_t* __restrict__ dict_cpl,
+                                                                 const double* __restrict__ v,
+                                                                 const double* __restrict__ g, int64_t n,
+                                                                 double* __restrict__ part) {
+  __shared__ double sr[4][ASM_MAX_CPL], si[4][ASM_MAX_CPL];
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double hr[ASM_MAX_CPL], hi[ASM_MAX_CPL];
+#pragma unroll
+  for (int c = 0; c < ASM_MAX_CPL; ++c) hr[c] = hi[c] = 0.0;
+  double gr = 0.0, gi = 0.0;
+  if (a < n) {
+    const int64_t s = a / SELL_C;
+    const int l = (int)(a % SELL_C);
+    const int64_t base = T.off[s] + l;
+    const int len = T.len[s];
+    const int dw = T.dvc ? 2 : 1;
+    double vr_a, vi_a = 0.0;
+    if (XC) {
+      vr_a = v[2 * a];
+      vi_a = v[2 * a + 1];
+      gr = g[2 * a];
+      gi = g[2 * a + 1];
+    } else {
+      vr_a = v[a];
+      gr = g[a];
+    }
+#pragma unroll
+    for (int c = 0; c < ASM_MAX_CPL; ++c) {
+      if (c < T.ncpl) {
+        const double* dc = T.diag_c + ((int64_t)c * T.nrows + a) * dw;
+        const double dr = dc[0], di = T.dvc ? dc[1] : 0.0;
+        hr[c] = dr * vr_a - di * vi_a;
+        hi[c] = dr * vi_a + di * vr_a;
+      }
+    }
+    for (int k = 0; k < len; ++k) {
+      const int64_t e = base + (int64_t)k * SELL_C;
+      int64_t b;
+      uint32_t idx;
+      sell_entry(T, e, b, idx);
+      const double wr = T.dict[idx * dw], wi = T.dvc ? T.dict[idx * dw + 1] : 0.0;
+      const double xr = XC ? v[2 * b] : v[b], xi = XC ? v[2 * b + 1] : 0.0;
+      const double pr = wr * xr - wi * xi, pi = wr * xi + wi * xr;
+      const int cc = dict_cpl[idx];
+#pragma unroll
+      for (int c = 0; c < ASM_MAX_CPL; ++c)
+        if (c == cc) {
+          hr[c] += pr;
+          hi[c] += pi;
+        }
+    }
+  }
+  // conj(g_a) * (H_c v)_a, reduced over the block
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < ASM_MAX_CPL; ++c) {
+    double pr = gr * hr[c] + gi * hi[c];
+    double pi = gr * hi[c] - gi * hr[c];
+    for (int o = 32; o > 0; o >>= 1) {
+      pr += __shfl_down(pr, o, 64);
+      pi += __shfl_down(pi, o, 64);
+    }
+    if (lane == 0) {
+      sr[wid][c] = pr;
+      si[wid][c] = pi;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < ASM_MAX_CPL) {
+    const int c = threadIdx.x;
+    double pr = 0.0, pi = 0.0;
+    for (int w = 0; w < 4; ++w) {
+      pr += sr[w][c];
+      pi += si[w][c];
+    }
+    part[((int64_t)blockIdx.x * ASM_MAX_CPL + c) * 2] = pr;
+    part[((int64_t)blockIdx.x * ASM_MAX_CPL + c) * 2 + 1] = pi;
+  }
+}
+
 void free_assembly(Op* op) {
   for (void* p : {(void*)op->asm_off, (void*)op->asm_len, (void*)op->asm_col, (void*)op->asm_val,
-                  (void*)op->asm_packed, (void*)op->asm_vidx, (void*)op->asm_dict, (void*)op->asm_diag})
+                  (void*)op->asm_packed, (void*)op->asm_vidx, (void*)op->asm_dict, (void*)op->asm_diag,
+                  (void*)op->asm_diag_c, (void*)op->asm_dict_cpl, (void*)op->asm_dict_s, (void*)op->asm_part})
     if (p) hipFree(p);
   op->asm_off = nullptr;
   op->asm_len = nullptr;
@@ -197,10 +349,20 @@ void free_assembly(Op* op) {
   op->asm_vidx = nullptr;
   op->asm_dict = nullptr;
   op->asm_diag = nullptr;
+  op->asm_diag_c = nullptr;
+  op->asm_dict_cpl = nullptr;
+  op->asm_dict_s = nullptr;
+  op->asm_part = nullptr;
   op->asm_entries = 0;
   op->asm_vc = 0;
+  op->asm_dvc = 0;
   op->asm_fmt = -1;
   op->asm_bytes = 0;
+  op->asm_tagged = 0;
+  op->asm_ndict = 0;
+  op->asm_h_dict.clear();
+  op->asm_h_dict_cpl.clear();
+  op->asm_h_scale.clear();
 }
 
 #define QES_ASM_CK_SWITCH(CK_VAR, MACRO)                   \
@@ -216,13 +378,19 @@ static int bits_for(int64_t v) {   // bits needed to hold 0 .. v-1
   return b;
 }
 
+// want_dict: 0 plain values, 1 dictionary when possible, 2 coupling-tagged dictionary (required)
 template <typename W, bool CPLX>
 static int assemble_t(Op* op, int want_dict, hipStream_t st) {
   const Basis* B = op->basis;
   const int64_t n = B->n;
   const QesPlan& P = op->plan;
   const dim3 grid((unsigned)((n + 255) / 256));
+  const bool tagged = want_dict == 2;
+  if (tagged && op->n_coupling > ASM_MAX_CPL) return fail("a coupling-tagged table holds at most 8 coupling ids");
   AsmOut O{};
+  O.tagged = tagged ? 1 : 0;
+  O.ncpl = op->n_coupling;
+  O.excpl = op->d_excpl;
 #define QES_ASM(CK, PASS)                                                                                           \
   hipLaunchKernelGGL((asm_rows_kernel<W, CPLX, CK, PASS>), grid, dim3(256), op->lds_bytes, st, B->group,          \
                      B->lookup(), P, B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_lut, B->norms, \
@@ -240,36 +408,51 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
   hipFree(O.rowlen);
   O.rowlen = nullptr;
 
-  // pass B (real operators): value dictionary of the off-diagonal elements
+  // pass B: value dictionary of the off-diagonal elements
   int fmt = ASM_PLAIN;
-  std::vector<double> dict;
+  std::vector<double> dict;          // 1 or 2 doubles per value
+  std::vector<uint8_t> dict_cpl;
   std::vector<uint16_t> slot_idx;
-  if (!CPLX && want_dict) {
-    const size_t nslot = size_t(1) << DICT_HBITS;
-    QES_HIP(hipMalloc(&O.dict_tab, sizeof(unsigned long long) * nslot));
-    std::vector<unsigned long long> tab(nslot, DICT_EMPTY);
-    tab[dict_slot(0ull)] = 0ull;   // padding value 0.0 is always present
-    QES_HIP(hipMemcpy(O.dict_tab, tab.data(), sizeof(unsigned long long) * nslot, hipMemcpyHostToDevice));
+  const size_t nslot = size_t(1) << DICT_HBITS;
+  void* dbuf = nullptr;
+  if (want_dict) {
+    QES_HIP(hipMalloc(&dbuf, nslot * (8 + 8 + 8 + 4 + 2) + 64));
+    char* p = (char*)dbuf;
+    O.D.key = (unsigned long long*)p;
+    O.D.vr = (double*)(p + nslot * 8);
+    O.D.vi = (double*)(p + nslot * 16);
+    O.D.cpl = (int32_t*)(p + nslot * 24);
+    O.D.idx = (const uint16_t*)(p + nslot * 28);
+    O.D.bad = (int32_t*)(p + nslot * 30);
+    std::vector<unsigned long long> key(nslot, DICT_EMPTY);
+    std::vector<double> vr(nslot, 0.0), vi(nslot, 0.0);
+    std::vector<int32_t> vc(nslot, 0);
+    const uint64_t zkey = dict_key(0.0, 0.0, 0);   // padding value 0.0 is always present
+    key[dict_slot(zkey)] = zkey;
+    QES_HIP(hipMemcpy(O.D.key, key.data(), nslot * 8, hipMemcpyHostToDevice));
+    QES_HIP(hipMemcpy(O.D.vr, vr.data(), nslot * 8, hipMemcpyHostToDevice));
+    QES_HIP(hipMemcpy(O.D.vi, vi.data(), nslot * 8, hipMemcpyHostToDevice));
+    QES_HIP(hipMemcpy(O.D.cpl, vc.data(), nslot * 4, hipMemcpyHostToDevice));
+    QES_HIP(hipMemsetAsync(O.D.bad, 0, sizeof(int32_t), st));
     QES_ASM_CK_SWITCH(P.canon, QES_ASM_DICT)
     QES_HIP(hipGetLastError());
-    QES_HIP(hipMemcpyAsync(tab.data(), O.dict_tab, sizeof(unsigned long long) * nslot, hipMemcpyDeviceToHost, st));
+    QES_HIP(hipMemcpyAsync(key.data(), O.D.key, nslot * 8, hipMemcpyDeviceToHost, st));
+    QES_HIP(hipMemcpyAsync(vr.data(), O.D.vr, nslot * 8, hipMemcpyDeviceToHost, st));
+    QES_HIP(hipMemcpyAsync(vi.data(), O.D.vi, nslot * 8, hipMemcpyDeviceToHost, st));
+    QES_HIP(hipMemcpyAsync(vc.data(), O.D.cpl, nslot * 4, hipMemcpyDeviceToHost, st));
     QES_HIP(hipStreamSynchronize(st));
-    std::vector<std::pair<double, uint32_t>> vals;
+    std::vector<std::tuple<int, double, double, uint32_t>> vals;
     for (uint32_t h = 0; h < nslot; ++h)
-      if (tab[h] != DICT_EMPTY) {
-        double v;
-        memcpy(&v, &tab[h], sizeof(double));
-        vals.push_back({v, h});
-      }
+      if (key[h] != DICT_EMPTY) vals.emplace_back(vc[h], vr[h], vi[h], h);
     const int64_t nd = (int64_t)vals.size();
     if (nd < (int64_t)nslot * 3 / 4) {   // a full table means the insert gave up: no dictionary
-      std::sort(vals.begin(), vals.end(), [](const std::pair<double, uint32_t>& x, const std::pair<double, uint32_t>& y) {
-        return x.first < y.first || (x.first == y.first && x.second < y.second);
-      });
+      std::sort(vals.begin(), vals.end());
       slot_idx.assign(nslot, 0);
       for (int64_t i = 0; i < nd; ++i) {
-        dict.push_back(vals[i].first);
-        slot_idx[vals[i].second] = (uint16_t)i;
+        dict.push_back(std::get<1>(vals[i]));
+        if (CPLX) dict.push_back(std::get<2>(vals[i]));
+        dict_cpl.push_back((uint8_t)std::get<0>(vals[i]));
+        slot_idx[std::get<3>(vals[i])] = (uint16_t)i;
       }
       const int ib = bits_for(nd), cb = bits_for(n);
       if (ib + cb <= 32) {
@@ -278,6 +461,10 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
       } else if (nd <= 65536) {
         fmt = ASM_P6;
       }
+    }
+    if (fmt == ASM_PLAIN && tagged) {
+      hipFree(dbuf);
+      return fail("the operator's values do not fit a dictionary: no coupling-tagged table");
     }
   }
 
@@ -296,10 +483,11 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
   }
   off[ns] = acc;
   const int cw = CPLX ? 2 : 1;
-  double per_entry = fmt == ASM_PLAIN ? 4.0 + 8.0 * cw : (fmt == ASM_P4 ? 4.0 : 6.0);
-  const double bytes = (double)acc * per_entry + (fmt == ASM_PLAIN ? 0.0 : 8.0 * n) + 12.0 * ns;
+  const double per_entry = fmt == ASM_PLAIN ? 4.0 + 8.0 * cw : (fmt == ASM_P4 ? 4.0 : 6.0);
+  const double diag_bytes = fmt == ASM_PLAIN ? 0.0 : 8.0 * cw * n * (1 + (tagged ? op->n_coupling : 0));
+  const double bytes = (double)acc * per_entry + diag_bytes + 12.0 * ns;
   if (bytes > op->asm_max_bytes) {
-    if (O.dict_tab) hipFree(O.dict_tab);
+    if (dbuf) hipFree(dbuf);
     return fail("assembled table would take " + std::to_string(bytes / 1e9) + " GB (limit " +
                 std::to_string(op->asm_max_bytes / 1e9) + " GB)");
   }
@@ -316,13 +504,13 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
     QES_HIP(hipMemsetAsync(op->asm_col, 0, sizeof(int32_t) * ne, st));
     QES_HIP(hipMemsetAsync(op->asm_val, 0, sizeof(double) * cw * ne, st));
   } else {
-    QES_HIP(hipMalloc(&op->asm_diag, sizeof(double) * n));
+    QES_HIP(hipMalloc(&op->asm_diag, sizeof(double) * cw * n));
     QES_HIP(hipMalloc(&op->asm_dict, sizeof(double) * dict.size()));
     QES_HIP(hipMemcpy(op->asm_dict, dict.data(), sizeof(double) * dict.size(), hipMemcpyHostToDevice));
-    uint16_t* d_slot_idx = nullptr;
-    QES_HIP(hipMalloc(&d_slot_idx, sizeof(uint16_t) * slot_idx.size()));
-    QES_HIP(hipMemcpy(d_slot_idx, slot_idx.data(), sizeof(uint16_t) * slot_idx.size(), hipMemcpyHostToDevice));
-    O.slot_idx = d_slot_idx;
+    QES_HIP(hipMalloc(&op->asm_dict_cpl, dict_cpl.size()));
+    QES_HIP(hipMemcpy(op->asm_dict_cpl, dict_cpl.data(), dict_cpl.size(), hipMemcpyHostToDevice));
+    QES_HIP(hipMemcpy((void*)O.D.idx, slot_idx.data(), sizeof(uint16_t) * nslot, hipMemcpyHostToDevice));
+    if (tagged) QES_HIP(hipMalloc(&op->asm_diag_c, sizeof(double) * cw * n * op->n_coupling));
     if (fmt == ASM_P4) {
       QES_HIP(hipMalloc(&op->asm_packed, sizeof(uint32_t) * ne));
       QES_HIP(hipMemsetAsync(op->asm_packed, 0, sizeof(uint32_t) * ne, st));
@@ -340,22 +528,29 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
   O.packed = op->asm_packed;
   O.vidx = op->asm_vidx;
   O.diag = op->asm_diag;
+  O.diag_c = op->asm_diag_c;
   // pass C: the table
   QES_ASM_CK_SWITCH(P.canon, QES_ASM_FILL)
   QES_HIP(hipGetLastError());
+  int32_t bad = 0;
+  if (dbuf) QES_HIP(hipMemcpyAsync(&bad, O.D.bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   QES_HIP(hipStreamSynchronize(st));
 #undef QES_ASM_COUNT
 #undef QES_ASM_DICT
 #undef QES_ASM_FILL
 #undef QES_ASM
-  if (O.dict_tab) hipFree(O.dict_tab);
-  if (O.slot_idx) hipFree((void*)O.slot_idx);
+  if (dbuf) hipFree(dbuf);
+  if (bad) return fail("dictionary fingerprint collision");   // caller retries without a dictionary
   op->asm_entries = acc;
   op->asm_slices = ns;
-  op->asm_vc = CPLX ? 1 : 0;
+  op->asm_vc = (fmt == ASM_PLAIN && CPLX) ? 1 : 0;
+  op->asm_dvc = (fmt != ASM_PLAIN && CPLX) ? 1 : 0;
   op->asm_fmt = fmt;
   op->asm_colbits = O.colbits;
-  op->asm_ndict = (int)dict.size();
+  op->asm_ndict = (int)dict_cpl.size();
+  op->asm_tagged = tagged ? 1 : 0;
+  op->asm_h_dict = dict;
+  op->asm_h_dict_cpl = dict_cpl;
   op->asm_bytes = (int64_t)bytes + (int64_t)dict.size() * 8;
   return 0;
 }
@@ -366,9 +561,15 @@ int op_assemble(Op* op, int enable, int want_dict, hipStream_t st) {
   if (op->basis->n >= (int64_t(1) << 31)) return fail("assembled operator needs fewer than 2^31 rows");
   const bool vc = op->is_complex || op->chi_complex;
   const bool w32 = op->basis->group.ns <= 32;
-  int rc;
-  if (w32) rc = vc ? assemble_t<uint32_t, true>(op, want_dict, st) : assemble_t<uint32_t, false>(op, want_dict, st);
-  else rc = vc ? assemble_t<uint64_t, true>(op, want_dict, st) : assemble_t<uint64_t, false>(op, want_dict, st);
+  auto run = [&](int wd) {
+    if (w32) return vc ? assemble_t<uint32_t, true>(op, wd, st) : assemble_t<uint32_t, false>(op, wd, st);
+    return vc ? assemble_t<uint64_t, true>(op, wd, st) : assemble_t<uint64_t, false>(op, wd, st);
+  };
+  int rc = run(want_dict);
+  if (rc && want_dict == 1) {   // dictionary path failed (collision): plain values
+    free_assembly(op);
+    rc = run(0);
+  }
   if (rc) free_assembly(op);
   return rc;
 }
@@ -384,28 +585,29 @@ __global__ void __launch_bounds__(256) asm_coo_kernel(AsmTable T, int64_t n, int
   const int l = (int)(a % SELL_C);
   const int64_t base = T.off[s] + l;
   const int len = T.len[s];
-  const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
-  const int vw = T.vc ? 2 : 1;
+  const bool cv = T.fmt == ASM_PLAIN ? T.vc : T.dvc;
+  const int vw = cv ? 2 : 1;
   for (int k = 0; k < len; ++k) {
     const int64_t e = base + (int64_t)k * SELL_C;
     rows[e] = (int32_t)a;
     if (T.fmt == ASM_PLAIN) {
       cols[e] = T.col[e];
       vals[e * vw] = T.val[e * vw];
-      if (T.vc) vals[e * vw + 1] = T.val[e * vw + 1];
-    } else if (T.fmt == ASM_P4) {
-      const uint32_t p = T.packed[e];
-      cols[e] = (int32_t)(p & cmask);
-      vals[e] = T.dict[p >> T.colbits];
+      if (cv) vals[e * vw + 1] = T.val[e * vw + 1];
     } else {
-      cols[e] = T.col[e];
-      vals[e] = T.dict[T.vidx[e]];
+      int64_t b;
+      uint32_t idx;
+      sell_entry(T, e, b, idx);
+      cols[e] = (int32_t)b;
+      vals[e * vw] = T.dict[idx * vw];
+      if (cv) vals[e * vw + 1] = T.dict[idx * vw + 1];
     }
   }
   if (T.fmt != ASM_PLAIN) {
     rows[entries + a] = (int32_t)a;
     cols[entries + a] = (int32_t)a;
-    vals[entries + a] = T.diag[a];
+    vals[(entries + a) * vw] = T.diag[a * vw];
+    if (cv) vals[(entries + a) * vw + 1] = T.diag[a * vw + 1];
   }
 }
 
@@ -422,16 +624,79 @@ int assembled_coo(const Op* op, int32_t* rows, int32_t* cols, double* vals, int6
   return 0;
 }
 
-int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int accumulate, hipStream_t st) {
-  const int64_t n = op->basis->n;
-  if (n == 0) return 0;
+static int launch_sell(const AsmTable& T, const double* x, double* y, int64_t n, bool xc, int accumulate,
+                       hipStream_t st) {
   const dim3 grid((unsigned)((n + 255) / 256));
-  const bool xc = dtype == 1;
-  if (op->asm_vc && !xc) return fail("complex matrix elements need complex128 vectors");
-  const AsmTable T = asm_table_of(op);
   if (xc) hipLaunchKernelGGL((sell_matvec_kernel<true>), grid, dim3(256), 0, st, T, x, y, n, accumulate);
   else hipLaunchKernelGGL((sell_matvec_kernel<false>), grid, dim3(256), 0, st, T, x, y, n, accumulate);
   QES_HIP(hipGetLastError());
+  return 0;
+}
+
+int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int accumulate, hipStream_t st) {
+  const int64_t n = op->basis->n;
+  if (n == 0) return 0;
+  const bool xc = dtype == 1;
+  if ((op->asm_vc || op->asm_dvc) && !xc) return fail("complex matrix elements need complex128 vectors");
+  return launch_sell(asm_table_of(op), x, y, n, xc, accumulate, st);
+}
+
+int assembled_matvec_scaled(Op* op, const double* scale, int nscale, const double* x, double* y, int dtype,
+                            hipStream_t st) {
+  if (op->asm_fmt < 0 || !op->asm_tagged) return fail("operator has no coupling-tagged assembled table");
+  if (nscale != op->n_coupling) return fail("one scale per coupling id needed");
+  const int64_t n = op->basis->n;
+  if (n == 0) return 0;
+  const bool xc = dtype == 1;
+  if (op->asm_dvc && !xc) return fail("complex matrix elements need complex128 vectors");
+  // scaled dictionary value * scale[coupling of the value], uploaded when the
+  // scales differ from the last call's (a recurrence at fixed J uploads once)
+  const std::vector<double> sc(scale, scale + nscale);
+  if (!op->asm_dict_s || sc != op->asm_h_scale) {
+    const int dw = op->asm_dvc ? 2 : 1;
+    std::vector<double> ds(op->asm_h_dict.size());
+    for (size_t i = 0; i < op->asm_h_dict_cpl.size(); ++i)
+      for (int w = 0; w < dw; ++w) ds[i * dw + w] = op->asm_h_dict[i * dw + w] * scale[op->asm_h_dict_cpl[i]];
+    if (!op->asm_dict_s) QES_HIP(hipMalloc(&op->asm_dict_s, sizeof(double) * std::max<size_t>(ds.size(), 1)));
+    QES_HIP(hipMemcpyAsync(op->asm_dict_s, ds.data(), sizeof(double) * ds.size(), hipMemcpyHostToDevice, st));
+    // the host copy of the scaled values must outlive the asynchronous upload
+    QES_HIP(hipStreamSynchronize(st));
+    op->asm_h_scale = sc;
+  }
+  AsmTable T = asm_table_of(op);
+  T.dict = op->asm_dict_s;
+  T.ncpl = op->n_coupling;
+  for (int c = 0; c < op->n_coupling; ++c) T.dscale[c] = scale[c];
+  return launch_sell(T, x, y, n, xc, 0, st);
+}
+
+int assembled_coupling_dots(Op* op, const double* v, const double* g, int dtype, double* out_host, hipStream_t st) {
+  if (op->asm_fmt < 0 || !op->asm_tagged) return fail("operator has no coupling-tagged assembled table");
+  const int64_t n = op->basis->n;
+  const int nc = op->n_coupling;
+  for (int c = 0; c < 2 * nc; ++c) out_host[c] = 0.0;
+  if (n == 0) return 0;
+  const bool xc = dtype == 1;
+  if (op->asm_dvc && !xc) return fail("complex matrix elements need complex128 vectors");
+  const int64_t nb = (n + 255) / 256;
+  if (!op->asm_part) QES_HIP(hipMalloc(&op->asm_part, sizeof(double) * nb * ASM_MAX_CPL * 2));
+  AsmTable T = asm_table_of(op);
+  T.ncpl = nc;
+  if (xc)
+    hipLaunchKernelGGL((sell_coupling_dots_kernel<true>), dim3((unsigned)nb), dim3(256), 0, st, T, op->asm_dict_cpl, v,
+                       g, n, op->asm_part);
+  else
+    hipLaunchKernelGGL((sell_coupling_dots_kernel<false>), dim3((unsigned)nb), dim3(256), 0, st, T, op->asm_dict_cpl,
+                       v, g, n, op->asm_part);
+  QES_HIP(hipGetLastError());
+  std::vector<double> part((size_t)nb * ASM_MAX_CPL * 2);
+  QES_HIP(hipMemcpyAsync(part.data(), op->asm_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+  QES_HIP(hipStreamSynchronize(st));
+  for (int64_t b = 0; b < nb; ++b)
+    for (int c = 0; c < nc; ++c) {
+      out_host[2 * c] += part[((size_t)b * ASM_MAX_CPL + c) * 2];
+      out_host[2 * c + 1] += part[((size_t)b * ASM_MAX_CPL + c) * 2 + 1];
+    }
   return 0;
 }
 

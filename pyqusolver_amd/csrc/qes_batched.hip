@@ -1016,7 +1016,7 @@ int batch_ground_states(Batch* B, double tol, int check_every, double* e0, int* 
   const int V = B->nvp;
   std::vector<double> a((size_t)V * B->maxs), b((size_t)V * B->maxs);
   std::vector<int> steps(V), active(V, 1);
-  std::vector<double> prev(V, 1e300), cur(V, 0.0);
+  std::vector<double> cur(V, 0.0);
   std::vector<int64_t> dim(V);
   for (int p = 0; p < B->nprob; ++p)
     for (int r = 0; r < B->nrhs[p]; ++r) dim[B->vp0[p] + r] = B->ops[p]->basis->n;
@@ -1037,13 +1037,12 @@ int batch_ground_states(Batch* B, double tol, int check_every, double* e0, int* 
       bool brk = false;
       for (int i = 0; i < k; ++i)
         if (!(std::fabs(bp[i]) > 1e-14 * std::max(1.0, std::fabs(cur[p])))) brk = true;
-      if (brk || k >= dim[p] || std::fabs(cur[p] - prev[p]) < tol * std::max(1.0, std::fabs(cur[p])) ||
+      if (brk || k >= dim[p] || ritz_error(ap, bp, k, cur[p]) <= tol * std::max(1.0, std::fabs(cur[p])) ||
           k >= B->maxs) {
         active[p] = 0;
       } else {
         any = true;
       }
-      prev[p] = cur[p];
     }
     if (batch_set_active(B, active.data())) return -1;
     if (!any) break;

@@ -398,12 +398,31 @@ int qes_projected_matvec(qes_op_t op, qes_basis_t out, const void* x_dev, void* 
 
 int qes_op_assemble(qes_op_t h, int32_t mode, double max_gb, void* stream) {
   if (!h) return fail("null operator");
-  if (mode < 0 || mode > 2) return fail("mode must be 0 (free), 1 (auto format) or 2 (plain values)");
+  if (mode < 0 || mode > 3)
+    return fail("mode must be 0 (free), 1 (auto format), 2 (plain values) or 3 (coupling-tagged dictionary)");
   if (max_gb > 0) h->op.asm_max_bytes = max_gb * 1e9;
-  return op_assemble(&h->op, mode != 0, mode == 1, (hipStream_t)stream);
+  const int want_dict = mode == 1 ? 1 : (mode == 3 ? 2 : 0);
+  return op_assemble(&h->op, mode != 0, want_dict, (hipStream_t)stream);
 }
 
-int qes_op_assembly_info(qes_op_t h, int64_t* entries, int64_t* bytes, int32_t* format, int32_t* n_values) {
+int qes_op_matvec_scaled(qes_op_t h, const double* scale, int32_t nscale, const void* x_dev, void* y_dev,
+                         int32_t dtype, void* stream) {
+  if (!h || !scale || !x_dev || !y_dev) return fail("null argument");
+  if (dtype != 0 && dtype != 1) return fail("dtype must be 0 (float64) or 1 (complex128)");
+  return assembled_matvec_scaled(&h->op, scale, nscale, (const double*)x_dev, (double*)y_dev, dtype,
+                                 (hipStream_t)stream);
+}
+
+int qes_op_coupling_dots(qes_op_t h, const void* v_dev, const void* g_dev, int32_t dtype, double* out,
+                         void* stream) {
+  if (!h || !v_dev || !g_dev || !out) return fail("null argument");
+  if (dtype != 0 && dtype != 1) return fail("dtype must be 0 (float64) or 1 (complex128)");
+  return assembled_coupling_dots(&h->op, (const double*)v_dev, (const double*)g_dev, dtype, out,
+                                 (hipStream_t)stream);
+}
+
+int qes_op_assembly_info(qes_op_t h, int64_t* entries, int64_t* bytes, int32_t* format, int32_t* n_values,
+                         int32_t* tagged) {
   if (!h) return fail("null operator");
   const Op& op = h->op;
   const bool on = op.asm_fmt >= 0;
@@ -411,6 +430,7 @@ int qes_op_assembly_info(qes_op_t h, int64_t* entries, int64_t* bytes, int32_t* 
   if (bytes) *bytes = on ? op.asm_bytes : 0;
   if (format) *format = op.asm_fmt;
   if (n_values) *n_values = op.asm_ndict;
+  if (tagged) *tagged = on ? op.asm_tagged : 0;
   return 0;
 }
 
@@ -566,6 +586,15 @@ int qes_lanczos_destroy(qes_lanczos_t h) {
   return 0;
 }
 
+int qes_tridiag_ritz(const double* alpha_host, const double* beta_host, int32_t k, double* theta, double* resid,
+                     double* quad) {
+  if (!alpha_host || !beta_host || !theta || !resid) return fail("null argument");
+  if (k < 1) return fail("k must be >= 1");
+  *theta = tridiag_lowest(alpha_host, beta_host, k);
+  *resid = ritz_error(alpha_host, beta_host, k, *theta, quad);
+  return 0;
+}
+
 int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int32_t max_steps, int32_t reorth,
                              double tol, int32_t check_every, double* alpha_host, double* beta_host,
                              int32_t* steps_done, double* e0, void* stream) {
@@ -574,7 +603,7 @@ int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int
   Lanczos* L = nullptr;
   if (lanczos_create(&op->op, max_steps, reorth ? 1 : 0, dtype, &L)) return -1;
   int rc = lanczos_start(L, v0_dev, (hipStream_t)stream);
-  double prev = 1e300, cur = 0.0;
+  double cur = 0.0;
   std::vector<double> a(max_steps), b(max_steps);
   while (!rc && lanczos_steps_done(L) < max_steps) {
     rc = lanczos_run(L, check_every, reorth, 1);
@@ -586,8 +615,7 @@ int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int
     bool breakdown = false;
     for (int i = 0; i < k; ++i)
       if (!(std::fabs(b[i]) > 1e-14 * std::max(1.0, std::fabs(cur)))) breakdown = true;
-    if (breakdown || std::fabs(cur - prev) < tol * std::max(1.0, std::fabs(cur))) break;
-    prev = cur;
+    if (breakdown || ritz_error(a.data(), b.data(), k, cur) <= tol * std::max(1.0, std::fabs(cur))) break;
   }
   const int k = lanczos_steps_done(L);
   if (!rc) {

@@ -92,9 +92,16 @@ struct Op {
   uint16_t* asm_vidx = nullptr;
   double* asm_dict = nullptr;
   double* asm_diag = nullptr;
+  double* asm_diag_c = nullptr;      // coupling-tagged: diagonal of each coupling [c][row]
+  uint8_t* asm_dict_cpl = nullptr;   // coupling id of each dictionary value
+  double* asm_dict_s = nullptr;      // scaled dictionary of the last assembled_matvec_scaled
+  double* asm_part = nullptr;        // per-block partial sums of assembled_coupling_dots
   int64_t asm_entries = 0, asm_slices = 0, asm_bytes = 0;
-  int asm_vc = 0, asm_fmt = -1, asm_colbits = 32, asm_ndict = 0;
+  int asm_vc = 0, asm_dvc = 0, asm_fmt = -1, asm_colbits = 32, asm_ndict = 0, asm_tagged = 0;
   double asm_max_bytes = 64e9;
+  std::vector<double> asm_h_dict;          // host copy of the dictionary
+  std::vector<uint8_t> asm_h_dict_cpl;
+  std::vector<double> asm_h_scale;         // scales of the dictionary in asm_dict_s
 };
 
 // launchers (qes_basis.hip)
@@ -120,6 +127,10 @@ int op_assemble(Op* op, int enable, int want_dict, hipStream_t st);
 void free_assembly(Op* op);
 int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int accumulate, hipStream_t st);
 int assembled_coo(const Op* op, int32_t* rows, int32_t* cols, double* vals, int64_t capacity, hipStream_t st);
+// coupling-tagged table: y = sum_c scale[c] H_c x; out_host[2c..2c+1] = <g, H_c v>
+int assembled_matvec_scaled(Op* op, const double* scale, int nscale, const double* x, double* y, int dtype,
+                            hipStream_t st);
+int assembled_coupling_dots(Op* op, const double* v, const double* g, int dtype, double* out_host, hipStream_t st);
 int pick_canon_public(const QesGroup& G);
 void free_plan(Op* op);
 int canon_check(const Basis* B, int kind, const uint32_t* d_lut, const uint64_t* d_states, int64_t n,

@@ -516,8 +516,9 @@ int lanczos_combine(Lanczos* L, const double* c_host, int k, void* out, hipStrea
 
 int lanczos_steps_done(const Lanczos* L) { return L->steps_done; }
 
-// lowest eigenvalue of the symmetric tridiagonal (a, b) by Sturm bisection
-double tridiag_lowest(const double* a, const double* b, int k) {
+// eigenvalue idx (0 = lowest) of the symmetric tridiagonal T_k with
+// diagonal a[0..k-1] and off-diagonal b[0..k-2], by Sturm-count bisection
+double tridiag_eig(const double* a, const double* b, int k, int idx) {
   double lo = 1e300, hi = -1e300;
   for (int i = 0; i < k; ++i) {
     double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i < k - 1 ? std::fabs(b[i]) : 0.0);
@@ -534,9 +535,83 @@ double tridiag_lowest(const double* a, const double* b, int k) {
       if (d == 0.0) d = -1e-300;
       if (d < 0) ++neg;
     }
-    if (neg >= 1) hi = mid; else lo = mid;
+    if (neg >= idx + 1) hi = mid; else lo = mid;
   }
   return 0.5 * (lo + hi);
+}
+
+double tridiag_lowest(const double* a, const double* b, int k) { return tridiag_eig(a, b, k, 0); }
+
+// Residual norm r = ||H y - theta y|| = |beta_k s_k| of the lowest Ritz pair
+// after k Lanczos steps (b[k-1] = beta_k, the coefficient of the next basis
+// vector; s = unit eigenvector of T_k for theta).  |theta - lambda| <= r for
+// some eigenvalue lambda of H, and, once theta is separated from the rest of
+// the spectrum, |theta - lambda| <~ r^2 / gap; *quad (optional) receives
+// that quadratic estimate with gap = theta_2 - theta from the Ritz values.
+// s comes from two steps of inverse iteration with T_k - theta I factorised
+// by Gaussian elimination with row interchanges (stable when theta is an
+// eigenvalue to working precision).
+double ritz_error(const double* a, const double* b, int k, double theta, double* quad) {
+  if (quad) *quad = 1e300;
+  if (k <= 0) return 1e300;
+  const double beta = std::fabs(b[k - 1]);
+  if (k == 1) {
+    if (quad) *quad = beta;
+    return beta;
+  }
+  double tnorm = 0.0;
+  for (int i = 0; i < k; ++i)
+    tnorm = std::max(tnorm, std::fabs(a[i]) + (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i < k - 1 ? std::fabs(b[i]) : 0.0));
+  const double small = std::max(tnorm, 1e-300) * 1e-16;
+  std::vector<double> d(k), du(k, 0.0), du2(k, 0.0), dl(k, 0.0), x(k, 1.0), rhs(k);
+  std::vector<char> sw(k, 0);
+  for (int i = 0; i < k; ++i) d[i] = a[i] - theta;
+  for (int i = 0; i < k - 1; ++i) du[i] = dl[i] = b[i];
+  for (int i = 0; i < k - 1; ++i) {
+    if (std::fabs(d[i]) >= std::fabs(dl[i])) {
+      if (std::fabs(d[i]) < small) d[i] = d[i] < 0 ? -small : small;
+      const double f = dl[i] / d[i];
+      dl[i] = f;
+      d[i + 1] -= f * du[i];
+    } else {
+      sw[i] = 1;
+      const double f = d[i] / dl[i];
+      d[i] = dl[i];
+      dl[i] = f;
+      const double t = du[i];
+      du[i] = d[i + 1];
+      d[i + 1] = t - f * d[i + 1];
+      if (i < k - 2) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -f * du[i + 1];
+      }
+    }
+  }
+  if (std::fabs(d[k - 1]) < small) d[k - 1] = d[k - 1] < 0 ? -small : small;
+  for (int pass = 0; pass < 2; ++pass) {
+    rhs = x;
+    for (int i = 0; i < k - 1; ++i) {
+      if (sw[i]) std::swap(rhs[i], rhs[i + 1]);
+      rhs[i + 1] -= dl[i] * rhs[i];
+    }
+    for (int i = k - 1; i >= 0; --i) {
+      double v = rhs[i];
+      if (i + 1 < k) v -= du[i] * x[i + 1];
+      if (i + 2 < k) v -= du2[i] * x[i + 2];
+      x[i] = v / d[i];
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < k; ++i) nrm = std::max(nrm, std::fabs(x[i]));
+    for (int i = 0; i < k; ++i) x[i] /= nrm;
+  }
+  double nrm2 = 0.0;
+  for (int i = 0; i < k; ++i) nrm2 += x[i] * x[i];
+  const double r = beta * std::fabs(x[k - 1]) / std::sqrt(nrm2);
+  if (quad) {
+    const double gap = tridiag_eig(a, b, k, 1) - theta;
+    *quad = gap > 0.0 ? std::min(r, r * r / gap) : r;
+  }
+  return r;
 }
 
 }  // namespace qes

@@ -8,22 +8,41 @@
 namespace qes {
 
 static constexpr int SELL_C = 64;
+static constexpr int ASM_MAX_CPL = 8;   // coupling ids of a coupling-tagged table
 enum { ASM_PLAIN = 0, ASM_P4 = 1, ASM_P6 = 2 };
 
 struct AsmTable {
   int32_t fmt;          // -1 none, ASM_PLAIN / ASM_P4 / ASM_P6
-  int32_t vc;           // complex values (PLAIN only)
+  int32_t vc;           // PLAIN: complex values
+  int32_t dvc;          // P4 / P6: complex dictionary values and diagonal
   int32_t colbits;      // P4 column field width
+  int32_t ncpl;         // > 0: diagonal = sum_c dscale[c] * diag_c[c] (per-coupling diagonals)
   int32_t pad;
+  int64_t nrows;        // row count (stride of diag_c)
   const int64_t* off;
   const int32_t* len;
-  const int32_t* col;   // PLAIN / P6
-  const double* val;    // PLAIN (1 or 2 doubles per entry)
+  const int32_t* col;       // PLAIN / P6
+  const double* val;        // PLAIN (1 or 2 doubles per entry)
   const uint32_t* packed;   // P4
   const uint16_t* vidx;     // P6
-  const double* dict;       // P4 / P6
-  const double* diag;       // P4 / P6
+  const double* dict;       // P4 / P6 (1 or 2 doubles per value)
+  const double* diag;       // P4 / P6 total diagonal (1 or 2 doubles per row)
+  const double* diag_c;     // per-coupling diagonals [c][row] (1 or 2 doubles per row)
+  double dscale[ASM_MAX_CPL];
 };
+
+// entry e of a dictionary table -> (column, dictionary index)
+__device__ __forceinline__ void sell_entry(const AsmTable& T, int64_t e, int64_t& b, uint32_t& idx) {
+  if (T.fmt == ASM_P4) {
+    const uint32_t p = T.packed[e];
+    const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
+    b = p & cmask;
+    idx = p >> T.colbits;
+  } else {
+    b = T.col[e];
+    idx = T.vidx[e];
+  }
+}
 
 // (ar, ai) = row a of A times x; XC: complex x (interleaved)
 template <bool XC>
@@ -52,31 +71,42 @@ __device__ __forceinline__ void sell_row(const AsmTable& T, int64_t a, const dou
     }
     return;
   }
-  const double d = T.diag[a];
-  if (XC) {
-    ar = d * x[2 * a];
-    ai = d * x[2 * a + 1];
+  const int dw = T.dvc ? 2 : 1;
+  double dr, di = 0.0;
+  if (T.ncpl > 0) {
+    dr = 0.0;
+#pragma unroll
+    for (int c = 0; c < ASM_MAX_CPL; ++c) {
+      if (c < T.ncpl) {
+        const double* dc = T.diag_c + ((int64_t)c * T.nrows + a) * dw;
+        dr += T.dscale[c] * dc[0];
+        if (T.dvc) di += T.dscale[c] * dc[1];
+      }
+    }
   } else {
-    ar = d * x[a];
+    dr = T.diag[a * dw];
+    if (T.dvc) di = T.diag[a * dw + 1];
   }
-  const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
+  if (XC) {
+    const double xr = x[2 * a], xi = x[2 * a + 1];
+    ar = dr * xr - di * xi;
+    ai = dr * xi + di * xr;
+  } else {
+    ar = dr * x[a];
+  }
   for (int k = 0; k < len; ++k) {
     const int64_t e = base + (int64_t)k * SELL_C;
     int64_t b;
-    double v;
-    if (T.fmt == ASM_P4) {
-      const uint32_t p = T.packed[e];
-      b = p & cmask;
-      v = T.dict[p >> T.colbits];
-    } else {
-      b = T.col[e];
-      v = T.dict[T.vidx[e]];
-    }
+    uint32_t idx;
+    sell_entry(T, e, b, idx);
     if (XC) {
-      ar += v * x[2 * b];
-      ai += v * x[2 * b + 1];
+      const double vr = T.dict[idx * dw];
+      const double vi = T.dvc ? T.dict[idx * dw + 1] : 0.0;
+      const double xr = x[2 * b], xi = x[2 * b + 1];
+      ar += vr * xr - vi * xi;
+      ai += vr * xi + vi * xr;
     } else {
-      ar += v * x[b];
+      ar += T.dict[idx] * x[b];
     }
   }
 }
@@ -85,7 +115,11 @@ inline AsmTable asm_table_of(const Op* op) {
   AsmTable T{};
   T.fmt = op->asm_fmt;
   T.vc = op->asm_vc;
+  T.dvc = op->asm_dvc;
   T.colbits = op->asm_colbits;
+  T.ncpl = 0;
+  T.pad = 0;
+  T.nrows = op->basis->n;
   T.off = op->asm_off;
   T.len = op->asm_len;
   T.col = op->asm_col;
@@ -94,6 +128,7 @@ inline AsmTable asm_table_of(const Op* op) {
   T.vidx = op->asm_vidx;
   T.dict = op->asm_dict;
   T.diag = op->asm_diag;
+  T.diag_c = op->asm_diag_c;
   return T;
 }
 

@@ -10,6 +10,7 @@ reference dispatches from DiagonalizationEngine
 from __future__ import annotations
 
 import ctypes as C
+import math
 import time
 from dataclasses import dataclass, field
 from typing import Optional
@@ -107,6 +108,26 @@ def tridiag_eigs(alpha, beta, k=1):
     return w, v
 
 
+def ritz_residual(alpha, beta):
+    """(r, quad) for the lowest Ritz value after m = len(alpha) steps
+    (beta[m-1] = beta_m, the coefficient of the next basis vector):
+    r = |beta_m s_m| the residual norm ||H y - theta y|| of the Ritz pair
+    (s: unit eigenvector of T_m; |theta - lambda| <= r for some eigenvalue
+    lambda of H) -- the stopping test, as in eigsh/ARPACK; quad =
+    min(r, r^2 / gap), gap = second Ritz value - theta, the usual estimate of
+    |theta - E0| once theta is separated from the next eigenvalue (an
+    unresolved near-degenerate pair makes it too small)."""
+    m = len(alpha)
+    if m == 0:
+        return math.inf, math.inf
+    wv, V = tridiag_eigs(alpha, beta)
+    r = abs(float(beta[m - 1]) * float(V[-1, 0]))
+    if m < 2:
+        return r, r
+    gap = float(wv[1] - wv[0])
+    return r, (min(r, r * r / gap) if gap > 0 else r)
+
+
 def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300, v0=None, seed: int = 0,
                          reorth: bool = True, store_basis: Optional[bool] = None, compute_eigvec: bool = False,
                          check_every: int = 10, use_graph: bool = True) -> LanczosResult:
@@ -127,7 +148,6 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
     L = DeviceLanczos(ham, maxiter, store_basis, complex_)
     L.start(v0t)
     hist = []
-    prev = None
     converged = False
     a = b = np.zeros(0)
     while True:
@@ -147,10 +167,9 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
             w, _ = tridiag_eigs(a, b)
             converged = True
             break
-        if prev is not None and abs(e0 - prev) <= tol * scale:
+        if ritz_residual(a, b)[0] <= tol * scale:
             converged = True
             break
-        prev = e0
     torch.cuda.synchronize()
     w, V = tridiag_eigs(a, b)
     res_bound = float(abs(b[len(a) - 1] * V[-1, 0])) if len(a) else float("nan")

@@ -216,25 +216,29 @@ class Hamiltonian:
         names = {v: k for k, v in self.CANON_KINDS.items()}
         return {"canonicalizer": names[c.value], "exchange_classes": nc.value, "generic_groups": ng.value}
 
-    def assemble(self, enable: bool = True, max_gb: float = 0.0, dictionary: bool = True):
+    def assemble(self, enable: bool = True, max_gb: float = 0.0, dictionary: bool = True, couplings: bool = False):
         """Tabulate H on the device (sliced ELLPACK, built once by the same
         row visitor) so that single-vector matvecs stream the table instead
         of canonicalising; ``enable=False`` frees it.  ``dictionary`` lets
-        real operators store value indices instead of values.  Returns the
-        table size in bytes."""
-        mode = 0 if not enable else (1 if dictionary else 2)
+        the table store value indices instead of values.  ``couplings``
+        keys the dictionary by coupling id as well (one diagonal per
+        coupling), which ``matvec_scaled_device`` and
+        ``coupling_dots_device`` (the differentiable H(J)) need; it raises
+        when the values do not fit a dictionary.  Returns the table size in
+        bytes."""
+        mode = 0 if not enable else (3 if couplings else (1 if dictionary else 2))
         _lib.check(_lib.lib().qes_op_assemble(self._device_op(), mode, float(max_gb), _lib.stream_ptr()),
                    "qes_op_assemble")
         return self.assembly_bytes
 
     @property
     def assembly(self) -> dict:
-        e, b, f, nv = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        e, b, f, nv, tg = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32()
         _lib.check(_lib.lib().qes_op_assembly_info(self._device_op(), C.byref(e), C.byref(b), C.byref(f),
-                                                   C.byref(nv)))
+                                                   C.byref(nv), C.byref(tg)))
         names = {-1: None, 0: "plain", 1: "packed32", 2: "index16"}
         return {"entries": int(e.value), "bytes": int(b.value), "format": names[int(f.value)],
-                "n_values": int(nv.value)}
+                "n_values": int(nv.value), "couplings": bool(tg.value)}
 
     @property
     def assembly_bytes(self) -> int:
@@ -308,6 +312,30 @@ class Hamiltonian:
         _lib.check(_lib.lib().qes_matvec_couplings(self._device_op(), _lib.dptr(x.contiguous()), _lib.dptr(y), dtype,
                                                    _lib.stream_ptr()), "qes_matvec_couplings")
         return y
+
+    def matvec_scaled_device(self, scale, x: torch.Tensor) -> torch.Tensor:
+        """sum_c scale[c] H_c x in one pass over a coupling-tagged table
+        (``assemble(couplings=True)``)."""
+        sc = np.ascontiguousarray(np.asarray(scale, dtype=np.float64).reshape(-1))
+        x = x.contiguous()
+        y = torch.empty_like(x)
+        dtype = 1 if x.dtype == torch.complex128 else 0
+        _lib.check(_lib.lib().qes_op_matvec_scaled(self._device_op(), sc.ctypes.data_as(_lib._PD), len(sc),
+                                                   _lib.dptr(x), _lib.dptr(y), dtype, _lib.stream_ptr()),
+                   "qes_op_matvec_scaled")
+        return y
+
+    def coupling_dots_device(self, v: torch.Tensor, g: torch.Tensor) -> np.ndarray:
+        """<g, H_c v> = sum_a conj(g_a) (H_c v)_a for every coupling id c
+        (complex numpy array), one pass over a coupling-tagged table."""
+        ncpl = self.n_couplings
+        out = np.zeros(2 * ncpl, dtype=np.float64)
+        v, g = v.contiguous(), g.contiguous()
+        dtype = 1 if v.dtype == torch.complex128 else 0
+        _lib.check(_lib.lib().qes_op_coupling_dots(self._device_op(), _lib.dptr(v), _lib.dptr(g), dtype,
+                                                   out.ctypes.data_as(_lib._PD), _lib.stream_ptr()),
+                   "qes_op_coupling_dots")
+        return out[0::2] + 1j * out[1::2]
 
     def _matvec_any(self, x):
         if isinstance(x, torch.Tensor):

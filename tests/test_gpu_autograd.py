@@ -180,3 +180,63 @@ def test_structure_factor_continued_fraction_against_dense():
     s = structure_factor(Hq, op, psi0, e0, np.linspace(0.0, 4.0, 41), eta=0.1, m=hsq.nh, hilbert_out=hsq)
     np.testing.assert_allclose(s, -g_dense.imag / np.pi, rtol=1e-9, atol=1e-12)
     assert abs(cf.weight - np.sum(amp)) <= 1e-12 * cf.weight
+
+
+@pytest.mark.parametrize("kind", ["honeycomb", "chain"])
+def test_coupling_tagged_table_autograd_matches_matrix_free(kind):
+    """The one-pass forward/backward over a coupling-tagged assembled table
+    gives the matrix-free path's products and gradients, passes gradcheck,
+    and its Lanczos dE0/dJ equals Hellmann-Feynman."""
+    import torch
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.autograd import DifferentiableHamiltonian, HamiltonianMatvec, hellmann_feynman, lanczos_energy
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.lanczos import lanczos_ground_state, start_vector
+    from pyqusolver_amd.models import J1J2Model
+    if kind == "honeycomb":
+        lat, hs, ham = _honeycomb()
+        dt = torch.complex128
+    else:
+        lat = ChainLattice(16)
+        hs = HilbertSpace(lattice=lat, sym_gen={"translation": 0, "parity": {"x": 1}},
+                          global_syms=[get_u1_sym(lat, 8)])
+        ham = J1J2Model(lat, J1=-1.0, J2=-0.4, hilbert_space=hs)
+        dt = torch.float64
+    Hf = DifferentiableHamiltonian(ham, assemble=False)
+    assert not ham.assembly["couplings"]
+    g = torch.Generator(device="cuda").manual_seed(5)
+    v = torch.randn(hs.nh, dtype=dt, device="cuda", generator=g)
+    w = torch.randn(hs.nh, dtype=dt, device="cuda", generator=g)
+    J = (Hf.J0 * 1.1).requires_grad_(True)
+    vv = v.clone().requires_grad_(True)
+
+    def loss(H):
+        J.grad = None
+        vv.grad = None
+        y = H(J, vv)
+        f = (torch.vdot(w, y) if dt == torch.complex128 else torch.dot(w, y)).real
+        f.backward()
+        return y.detach(), J.grad.clone(), vv.grad.clone()
+    y_f, gJ_f, gv_f = loss(Hf)
+    v0 = start_vector(hs.nh, dt == torch.complex128, seed=0)
+    e_f = lanczos_energy(Hf, Hf.J0, v0, 90).item()
+    Ht = DifferentiableHamiltonian(ham, assemble=True)
+    assert ham.assembly["couplings"]
+    y_t, gJ_t, gv_t = loss(Ht)
+    tol = 1e-13 * y_f.abs().max().item()
+    assert torch.allclose(y_t, y_f, rtol=1e-13, atol=tol)
+    assert torch.allclose(gv_t, gv_f, rtol=1e-13, atol=1e-13 * gv_f.abs().max().item())
+    np.testing.assert_allclose(gJ_t.numpy(), gJ_f.numpy(), rtol=1e-12, atol=1e-12 * gJ_f.abs().max().item())
+    s = torch.tensor(np.linspace(0.8, 1.3, ham.n_couplings), dtype=torch.float64, requires_grad=True)
+    vg = v.clone().requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda s_, v_: HamiltonianMatvec.apply(s_, v_, ham), (s, vg),
+                                    eps=1e-6, atol=1e-7, rtol=1e-6, fast_mode=True)
+    Jg = Ht.J0.clone().requires_grad_(True)
+    e_t = lanczos_energy(Ht, Jg, v0, 90)
+    e_t.backward()
+    assert abs(e_t.item() - e_f) <= 1e-11 * abs(e_f)
+    res = lanczos_ground_state(ham, tol=1e-14, maxiter=300, seed=1, reorth=True, compute_eigvec=True)
+    g_hf = hellmann_feynman(Ht, res.eigenvectors)
+    np.testing.assert_allclose(Jg.grad.numpy(), g_hf.numpy(), rtol=2e-6, atol=1e-8)
+    assert abs(float((g_hf * Jg.detach()).sum()) - e_t.item()) <= 1e-9 * abs(e_t.item())
+    ham.assemble(False)

@@ -135,3 +135,53 @@ def test_assembled_operator_matches_matrix_free(case):
     H.set_option("xcd_remap", 0)
     assert H.assembly_bytes == 0
     H.assemble(False)
+
+
+@pytest.mark.parametrize("case", [SPACES[0], SPACES[1], SPACES[9], SPACES[11], SPACES[12]],
+                         ids=lambda c: f"{c[0]}{c[1]}x{c[2]}-{c[3]}")
+def test_coupling_tagged_table(case):
+    """A coupling-tagged table (values keyed by coupling id, one diagonal per
+    coupling): its plain product is the matrix-free one, the scaled product
+    is sum_c s_c H_c x with H_c from the matrix-free per-coupling kernel, and
+    the coupling dots are <g, H_c v>.  Complex operators and complex
+    characters take the complex value dictionary."""
+    import torch
+    from pyqusolver_amd.models import HeisenbergKitaev, J1J2Model
+    kind, lx, ly, sym, n_up, ck = case
+    lat, hs = _space(kind, lx, ly, sym, n_up)
+    if kind == "honeycomb":
+        H = HeisenbergKitaev(lat, K=-1.0, J=0.2, hilbert_space=hs, hx=0.03, hy=0.02, hz=0.05, split_couplings=True)
+    else:
+        H = J1J2Model(lat, J1=-1.0, J2=-0.4, hilbert_space=hs)
+    nc = H.n_couplings
+    assert nc >= 2
+    dt = torch.complex128 if H.needs_complex else torch.float64
+    g = torch.Generator(device="cuda").manual_seed(21)
+    x = torch.randn(hs.nh, dtype=dt, device="cuda", generator=g)
+    w = torch.randn(hs.nh, dtype=dt, device="cuda", generator=g)
+    y_free = H.matvec_device(x)
+    Y = H.matvec_couplings_device(x)
+    H.assemble(True, couplings=True)
+    info = H.assembly
+    assert info["couplings"] and info["format"] in ("packed32", "index16"), info
+    scale_tol = 1e-13 * y_free.abs().max().item()
+    y_asm = H.matvec_device(x)
+    if dt == torch.float64:
+        assert torch.equal(y_asm, y_free)
+    else:
+        assert torch.allclose(y_asm, y_free, rtol=1e-14, atol=1e-14 * y_free.abs().max().item())
+    for s in (np.ones(nc), np.linspace(0.5, 1.7, nc), np.eye(nc)[nc - 1]):
+        ys = H.matvec_scaled_device(s, x)
+        ref = torch.tensordot(torch.as_tensor(s, device="cuda").to(dt), Y, dims=([0], [0]))
+        assert torch.allclose(ys, ref, rtol=1e-13, atol=scale_tol), s
+    # unchanged scales reuse the uploaded dictionary; the next call still sees new ones
+    s = np.linspace(0.5, 1.7, nc)
+    assert torch.equal(H.matvec_scaled_device(s, x), H.matvec_scaled_device(s.copy(), x))
+    d = H.coupling_dots_device(x, w)
+    ref = np.array([torch.vdot(w, Y[c]).item() if dt == torch.complex128 else torch.dot(w, Y[c]).item()
+                    for c in range(nc)])
+    assert np.allclose(d, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    # the plain table product is unchanged by a scaled call in between
+    assert torch.equal(H.matvec_device(x), y_asm)
+    H.assemble(False)
+    assert not H.assembly["couplings"]
