@@ -85,6 +85,29 @@ class OLattice:
     def cell_site(self, x: int, y: int, sub: int) -> int:
         return self.n_basis * ((x % self.lx) + self.lx * (y % self.ly)) + sub
 
+    # Geometry used by the inversion permutation (inversion.py:135-230 reads
+    # lat.rvectors, a1/a2/a3, basis, fracs, subs).  Site s = n_basis*cell +
+    # sub; honeycomb: a1 = (sqrt3, 0), a2 = (sqrt3/2, 3/2), B at (a1+a2)/3,
+    # so the three forward bonds of oracle_bonds have unit length.
+    def geometry(self):
+        if self.kind == "honeycomb":
+            a1 = np.array([np.sqrt(3.0), 0.0, 0.0])
+            a2 = np.array([np.sqrt(3.0) / 2, 1.5, 0.0])
+            basis = np.array([np.zeros(3), (a1 + a2) / 3.0])
+        else:
+            a1, a2 = np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0])
+            basis = np.zeros((1, 3))
+        a3 = np.array([0.0, 0.0, 1.0])
+        nb = self.n_basis
+        fracs, subs, r = [], [], []
+        for s in range(self.ns):
+            cell, sub = s // nb, s % nb
+            x, y = cell % self.lx, (cell // self.lx) % self.ly
+            fracs.append((x, y, 0))
+            subs.append(sub)
+            r.append(x * a1 + y * a2 + basis[sub])
+        return a1, a2, a3, basis, np.array(fracs), np.array(subs), np.array(r)
+
 
 def oracle_bonds(lat: OLattice, order: int = 1) -> List[Tuple[int, int, int]]:
     """Forward bond list ``(i, j, nn_index)`` in site order.
@@ -165,9 +188,40 @@ def reflection_perm(ns: int) -> np.ndarray:
     return np.arange(ns)[::-1].copy()
 
 
+def inversion_perm(lat: OLattice, tol: float = 1e-7) -> np.ndarray:
+    """Algebra/Symmetries/inversion.py:135-230 (``_build_inversion_permutation``,
+    center "bbox"): perm[i] = site whose position is 2 r0 - r_i, r0 the centre
+    of the bounding box of all positions, matched over sublattices by
+    rounding the fractional Bravais coordinates and wrapping periodic
+    directions."""
+    a1, a2, a3, basis, fracs, subs, r = lat.geometry()
+    ns = lat.ns
+    r0 = 0.5 * (r.min(axis=0) + r.max(axis=0))
+    Ainv = np.linalg.inv(np.column_stack([a1, a2, a3]))
+    dims = (lat.lx, max(lat.ly, 1), max(lat.lz, 1))
+    lookup = {(int(f[0]), int(f[1]), int(f[2]), int(sb)): i for i, (f, sb) in enumerate(zip(fracs, subs))}
+    perm = np.full(ns, -1, dtype=np.int64)
+    for i in range(ns):
+        target = 2.0 * r0 - r[i]
+        best = None
+        for sb in range(len(basis)):
+            fr = Ainv @ (target - basis[sb])
+            n = [int(np.rint(fr[0])) % dims[0], int(np.rint(fr[1])) % dims[1], int(np.rint(fr[2])) % dims[2]]
+            err = np.linalg.norm(n[0] * a1 + n[1] * a2 + n[2] * a3 + basis[sb] - target)
+            if best is None or err < best[0]:
+                best = (err, n[0], n[1], n[2], sb)
+        err, nx, ny, nz, sb = best
+        if err > tol:
+            raise ValueError(f"inversion target of site {i} matches no site (err {err})")
+        perm[i] = lookup[(nx, ny, nz, sb)]
+    if len(np.unique(perm)) != ns:
+        raise ValueError("inversion permutation is not a bijection")
+    return perm
+
+
 @dataclass
 class Generator:
-    kind: str      # "tx" | "ty" | "tz" | "R" | "Px" | "Py" | "Pz"
+    kind: str      # "tx" | "ty" | "tz" | "R" | "I" | "Px" | "Py" | "Pz"
     sector: object
 
 
@@ -188,6 +242,7 @@ class Group:
     chi: np.ndarray = None
     trans_perm: list = field(default_factory=list)
     refl_perm: list = field(default_factory=list)
+    inv_perm: list = field(default_factory=list)
     parity_axis: list = field(default_factory=list)
 
     def __post_init__(self):
@@ -225,7 +280,7 @@ class Group:
         else:
             full = base
         # tables, build_compiled_group :981-1045
-        tidx, ridx, pidx = {}, {}, {}
+        tidx, ridx, pidx, iidx = {}, {}, {}, {}
         for g in gens:
             if g.kind in ("tx", "ty", "tz"):
                 tidx[id(g)] = len(self.trans_perm)
@@ -233,6 +288,9 @@ class Group:
             elif g.kind == "R":
                 ridx[id(g)] = len(self.refl_perm)
                 self.refl_perm.append(reflection_perm(ns))
+            elif g.kind == "I":
+                iidx[id(g)] = len(self.inv_perm)
+                self.inv_perm.append(inversion_perm(self.lat))
             else:
                 pidx[id(g)] = len(self.parity_axis)
                 self.parity_axis.append("xyz".index(g.kind[1]))
@@ -254,6 +312,9 @@ class Group:
                 elif g.kind == "R":
                     ops.append((OP_REFLECTION, ridx[id(g)], power))
                     chi = chi * (g.sector ** count)                        # base.py:556
+                elif g.kind == "I":
+                    ops.append((OP_INVERSION, iidx[id(g)], power))
+                    chi = chi * (g.sector ** (count % 2))                  # inversion.py:500-523
                 else:
                     ops.append((OP_PARITY, pidx[id(g)], power))
                     chi = chi * (g.sector ** count)
@@ -280,6 +341,8 @@ class Group:
                     cur = _apply_perm(cur, ns, self.trans_perm[a0])
             elif code == OP_REFLECTION:
                 cur = _apply_perm(cur, ns, self.refl_perm[a0])
+            elif code == OP_INVERSION:    # _apply_inversion_permutation, inversion.py:53-86 (base 2)
+                cur = _apply_perm(cur, ns, self.inv_perm[a0])
             elif code == OP_PARITY:
                 cur, ph = _apply_parity(cur, ns, self.parity_axis[a0])
                 phase = phase * ph
@@ -443,6 +506,8 @@ def sector_dimension(group: Group) -> int:
                     perm = group.trans_perm[a0][perm]
             elif code == OP_REFLECTION:
                 perm = group.refl_perm[a0][perm]
+            elif code == OP_INVERSION:
+                perm = group.inv_perm[a0][perm]
             elif code == OP_PARITY:
                 if group.parity_axis[a0] == 1:
                     raise ValueError("closed form excludes parity y")
@@ -826,11 +891,24 @@ def terms_xxz(lat: OLattice, jxy=1.0, jz=1.0, hx=0.0, hz=0.0) -> Terms:
     return t
 
 
-def terms_heisenberg_kitaev(lat: OLattice, K, J, hx=0.0, hy=0.0, hz=0.0, dlt=1.0) -> Terms:
+def honeycomb_bond_flux_phase(lat: OLattice, i: int, n: int, flux) -> float:
+    """Hermitian boundary phase Re exp(i w.phi) of honeycomb forward bond n
+    from A site i (heisenberg_kitaev.py:560-571 with use_forward=True): the x
+    bond (to cell x-1) winds -1 in x when x = 0, the y bond (to cell y-1)
+    winds -1 in y when y = 0, the z bond stays in its cell."""
+    phx, phy = flux
+    x, y = (i // 2) % lat.lx, (i // 2) // lat.lx
+    w = (-1 if (n == 0 and x == 0) else 0, -1 if (n == 1 and y == 0) else 0)
+    return float(np.real(np.exp(1j * (w[0] * phx + w[1] * phy))))
+
+
+def terms_heisenberg_kitaev(lat: OLattice, K, J, hx=0.0, hy=0.0, hz=0.0, dlt=1.0, flux=None) -> Terms:
     """Model/Interacting/Spin/heisenberg_kitaev.py:378-639 (spin-1/2):
     fields scaled by 2 (SINGLE_TERM_MULT, :74) in order z, y, x per site;
     per forward bond SzSz, SxSx, SySy scaled by 4 (CORR_TERM_MULT, :76) with
-    the Kitaev coupling subtracted on its bond direction (:581-586)."""
+    the Kitaev coupling subtracted on its bond direction (:581-586); with a
+    boundary flux (phi_x, phi_y) each bond's couplings carry the Hermitian part
+    of its boundary phase (:487-571)."""
     ns = lat.ns
     Kx, Ky, Kz = (K, K, K) if np.isscalar(K) else K
     is_c = not _isclose0(hy)
@@ -846,15 +924,16 @@ def terms_heisenberg_kitaev(lat: OLattice, K, J, hx=0.0, hy=0.0, hz=0.0, dlt=1.0
         if not _isclose0(hx):
             t.add(SIG_X_L, -2.0 * hx, [i])
         for (j, n) in bonds.get(i, []):
-            szsz = 4.0 * J * dlt
-            sxsx = 4.0 * J
-            sysy = 4.0 * J
+            ph = 1.0 if flux is None else honeycomb_bond_flux_phase(lat, i, n, flux)
+            szsz = ph * 4.0 * J * dlt
+            sxsx = ph * 4.0 * J
+            sysy = ph * 4.0 * J
             if n == 2:
-                szsz -= 4.0 * Kz
+                szsz -= ph * 4.0 * Kz
             elif n == 1:
-                sysy -= 4.0 * Ky
+                sysy -= ph * 4.0 * Ky
             elif n == 0:
-                sxsx -= 4.0 * Kx
+                sxsx -= ph * 4.0 * Kx
             if not _isclose0(szsz):
                 t.add(SIG_Z_C, szsz, [i, j])
             if not _isclose0(sxsx):

@@ -22,9 +22,13 @@ from .symmetry import (GlobalSymmetry, SectorGroup, build_sector_group, get_u1_s
 
 class HilbertSpace:
     def __init__(self, lattice=None, ns: Optional[int] = None, sym_gen=None, global_syms=None,
-                 is_manybody: bool = True, gen_mapping: bool = True, **kwargs):
+                 is_manybody: bool = True, gen_mapping: bool = True, boundary_flux=None, **kwargs):
         if lattice is None and ns is None:
             raise ValueError("HilbertSpace needs a lattice or ns")
+        if boundary_flux is not None:           # hilbert_base.py:106-108: set on the lattice
+            if lattice is None:
+                raise ValueError("boundary_flux needs a lattice")
+            lattice.flux = boundary_flux
         self.lattice = lattice
         self._ns = int(ns if ns is not None else lattice.ns)
         if lattice is not None and lattice.ns != self._ns:

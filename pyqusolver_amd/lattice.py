@@ -16,7 +16,17 @@ Conventions (fixed here, since the submodule is absent):
     (x,y+1) (NN) and (x+1,y+1), (x-1,y+1) (NNN); honeycomb: the A site of cell
     (x,y) bonds to B(x-1,y) [x bond, nn index 0], B(x,y-1) [y bond, 1] and
     B(x,y) [z bond, 2].
-  * No flux: boundary phases are 1.
+  * Geometry (read by the inversion permutation, Algebra/Symmetries/
+    inversion.py:135-230): Bravais a1 = x, a2 = y; honeycomb a1 = (sqrt3, 0),
+    a2 = (sqrt3/2, 3/2) with B at (a1 + a2)/3 (unit-length bonds).
+  * Boundary flux ``flux = {"x": phi_x, "y": phi_y}`` (radians): a bond that
+    winds w_d times around periodic direction d carries
+    exp(i sum_d w_d phi_d) (``boundary_phase_from_winding``), and
+    ``boundary_phases()[d][n] = exp(i phi_d n)`` for n particles crossing the
+    boundary (the table the reference's translation reads,
+    symmetry_container.py:1047-1051, translation.py:667-668).  Forward bonds
+    that leave the cell range in the negative direction wind -1, in the
+    positive direction +1.
 """
 from __future__ import annotations
 
@@ -31,14 +41,41 @@ class Lattice:
     kind = "lattice"
     n_basis = 1
 
-    def __init__(self, lx: int, ly: int = 1, lz: int = 1, bc: str = "pbc"):
+    def __init__(self, lx: int, ly: int = 1, lz: int = 1, bc: str = "pbc", flux=None):
         if bc.lower() != "pbc":
             raise ValueError("only periodic boundary conditions are supported")
         self.lx, self.ly, self.lz = int(lx), int(ly), int(lz)
         self.bc = "pbc"
         self._basis = [np.zeros(3)] * self.n_basis
+        self._winding = {}
         self._nn = self._build_forward(1)
         self._nnn = self._build_forward(2)
+        self.flux = flux
+
+    # ---- boundary flux ----------------------------------------------------
+    @property
+    def flux(self) -> dict:
+        return dict(self._flux)
+
+    @flux.setter
+    def flux(self, value):
+        """None, a number (direction x) or {"x"|"y"|"z": phi} in radians."""
+        f = {"x": 0.0, "y": 0.0, "z": 0.0}
+        if value is None:
+            pass
+        elif np.isscalar(value):
+            f["x"] = float(value)
+        else:
+            for k, v in dict(value).items():
+                key = str(getattr(k, "name", k)).lower()[-1]
+                if key not in f:
+                    raise ValueError(f"unknown flux direction {k!r}")
+                f[key] = float(v)
+        self._flux = f
+
+    @property
+    def has_flux(self) -> bool:
+        return any(abs(v) > 0.0 for v in self._flux.values())
 
     # ---- geometry -------------------------------------------------------
     @property
@@ -50,6 +87,35 @@ class Lattice:
     @property
     def dim(self) -> int:
         return 1 if self.ly == 1 and self.lz == 1 else (2 if self.lz == 1 else 3)
+
+    # ---- positions ------------------------------------------------------
+    @property
+    def a1(self) -> np.ndarray:
+        return np.array([1.0, 0.0, 0.0])
+
+    @property
+    def a2(self) -> np.ndarray:
+        return np.array([0.0, 1.0, 0.0])
+
+    @property
+    def a3(self) -> np.ndarray:
+        return np.array([0.0, 0.0, 1.0])
+
+    @property
+    def basis(self) -> np.ndarray:
+        return np.zeros((1, 3))
+
+    @property
+    def rvectors(self) -> np.ndarray:
+        """Position of every site (ns, 3)."""
+        out = np.zeros((self.ns, 3))
+        for s in range(self.ns):
+            x, y, z = self.get_coordinates(s)
+            out[s] = x * self.a1 + y * self.a2 + z * self.a3 + self.basis[s % self.n_basis]
+        return out
+
+    def periodic_flags(self):
+        return (True, self.ly > 1, self.lz > 1)
 
     def get_coordinates(self, site: int) -> Tuple[int, int, int]:
         cell = site // self.n_basis
@@ -63,14 +129,34 @@ class Lattice:
     def _forward(self, site: int, order: int) -> List[int]:
         raise NotImplementedError
 
+    def _forward_cells(self, site: int, order: int):
+        """Forward neighbours as unwrapped (x, y, sub) of the target."""
+        raise NotImplementedError
+
+    def _forward(self, site: int, order: int) -> List[int]:
+        out = []
+        for xx, yy, sb in self._forward_cells(site, order):
+            j = self.site_index(xx, yy, sub=sb)
+            if j != site:
+                out.append(j)
+        return out
+
     def _build_forward(self, order: int):
         out = []
         for i in range(self.ns):
             try:
-                js = self._forward(i, order)
+                cells = self._forward_cells(i, order)
             except NotImplementedError:
-                js = []
-            out.append([j for j in js])
+                cells = []
+            js = []
+            for xx, yy, sb in cells:
+                j = self.site_index(xx, yy, sub=sb)
+                if j == i:
+                    continue
+                js.append(j)
+                w = (xx // self.lx, yy // self.ly if self.ly > 1 else 0, 0)
+                self._winding.setdefault((i, j), w)
+            out.append(js)
         return out
 
     def get_nn_forward_num(self, i: int) -> int:
@@ -89,10 +175,20 @@ class Lattice:
         return j is None or int(j) < 0
 
     def bond_winding(self, i: int, j: int):
-        return (0, 0, 0)
+        """Times the forward bond i -> j winds around each periodic direction."""
+        return self._winding.get((int(i), int(j)), (0, 0, 0))
 
-    def boundary_phase_from_winding(self, wx, wy, wz) -> float:
-        return 1.0
+    def boundary_phase_from_winding(self, wx, wy, wz):
+        """exp(i (wx phi_x + wy phi_y + wz phi_z)); a real 1.0 without flux."""
+        ang = wx * self._flux["x"] + wy * self._flux["y"] + wz * self._flux["z"]
+        if ang == 0.0:
+            return 1.0
+        return complex(np.exp(1j * ang))
+
+    def boundary_phases(self) -> np.ndarray:
+        """(3, ns + 1) table: exp(i phi_d n) for n particles crossing direction d."""
+        n = np.arange(self.ns + 1)
+        return np.stack([np.exp(1j * self._flux[d] * n) for d in "xyz"])
 
     def bonds(self, order: int = 1):
         """Forward bonds (i, j, nn_index) in site order."""
@@ -103,36 +199,42 @@ class Lattice:
 class ChainLattice(Lattice):
     kind = "chain"
 
-    def __init__(self, L: int, bc: str = "pbc"):
-        super().__init__(L, 1, 1, bc)
+    def __init__(self, L: int, bc: str = "pbc", flux=None):
+        super().__init__(L, 1, 1, bc, flux=flux)
 
-    def _forward(self, site, order):
-        j = (site + order) % self.lx
-        return [j] if j != site else []
+    def _forward_cells(self, site, order):
+        return [(site + order, 0, 0)]
 
 
 class SquareLattice(Lattice):
     kind = "square"
 
-    def _forward(self, site, order):
+    def _forward_cells(self, site, order):
         x, y, _ = self.get_coordinates(site)
         cand = [(x + 1, y), (x, y + 1)] if order == 1 else [(x + 1, y + 1), (x - 1, y + 1)]
-        out = []
-        for xx, yy in cand:
-            j = self.site_index(xx, yy)
-            if j != site:
-                out.append(j)
-        return out
+        return [(xx, yy, 0) for xx, yy in cand]
 
 
 class HoneycombLattice(Lattice):
     kind = "honeycomb"
     n_basis = 2
 
-    def _forward(self, site, order):
+    @property
+    def a1(self) -> np.ndarray:
+        return np.array([np.sqrt(3.0), 0.0, 0.0])
+
+    @property
+    def a2(self) -> np.ndarray:
+        return np.array([np.sqrt(3.0) / 2.0, 1.5, 0.0])
+
+    @property
+    def basis(self) -> np.ndarray:
+        return np.array([np.zeros(3), (self.a1 + self.a2) / 3.0])
+
+    def _forward_cells(self, site, order):
         if order != 1:
             raise NotImplementedError
         if site % 2 == 1:
             return []
         x, y, _ = self.get_coordinates(site)
-        return [self.site_index(x - 1, y, sub=1), self.site_index(x, y - 1, sub=1), self.site_index(x, y, sub=1)]
+        return [(x - 1, y, 1), (x, y - 1, 1), (x, y, 1)]

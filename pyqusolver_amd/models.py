@@ -43,6 +43,24 @@ def _cv(v) -> float:
     return 1.0
 
 
+def _bond_phase(lattice, i, j, hermitize: bool):
+    """Boundary phase of the forward bond i -> j (lattice.boundary_phase_from_winding
+    of its winding).  ``hermitize``: its Hermitian part Re exp(i phi), what the
+    reference uses for two-site spin bilinears listed once per bond
+    (heisenberg_kitaev.py:487-571).  Otherwise a complex phase would make the
+    bond's real spin bilinear non-Hermitian (the reference multiplies the
+    couplings by it, xxz.py:393-431, j1j2.py:261-291), which is refused."""
+    ph = lattice.boundary_phase_from_winding(*lattice.bond_winding(i, j))
+    if np.iscomplexobj(ph) or isinstance(ph, complex):
+        if hermitize:
+            return float(np.real(0.5 * (ph + np.conj(ph))))
+        if abs(np.imag(ph)) > 1e-14:
+            raise ValueError(f"bond ({i}, {j}) carries the complex boundary phase {ph}: this model's bond would not "
+                             "be Hermitian (use a flux of 0 or pi along that direction)")
+        return float(np.real(ph))
+    return ph
+
+
 def _nz(v, i=None) -> bool:
     """_ADD_CONDITION (Algebra/hamil.py:123-206) for scalars/arrays."""
     if v is None:
@@ -85,7 +103,7 @@ class J1J2Model(Hamiltonian):
                     j = getj(i, num=nidx)
                     if lattice.wrong_nei(j) or int(j) == i:
                         continue
-                    jj = -lattice.boundary_phase_from_winding(*lattice.bond_winding(i, j)) * J[i]
+                    jj = -_bond_phase(lattice, i, int(j), False) * J[i]
                     if not _nz(jj):
                         continue
                     self.add(sxx, jj, True, [i, int(j)], coupling=cpl)
@@ -120,7 +138,7 @@ class XXZ(Hamiltonian):
                 j = lattice.get_nn_forward(i, num=nidx)
                 if lattice.wrong_nei(j):
                     continue
-                phase = lattice.boundary_phase_from_winding(*lattice.bond_winding(i, j))
+                phase = _bond_phase(lattice, i, int(j), False)
                 cxy = self._jxy[i] * phase
                 if _nz(cxy):
                     self.add(sxx, -cxy, True, [i, int(j)], coupling=0)
@@ -166,7 +184,8 @@ class HeisenbergKitaev(Hamiltonian):
                 if lattice.wrong_nei(nei):
                     continue
                 nei = int(nei)
-                phase = lattice.boundary_phase_from_winding(*lattice.bond_winding(i, nei))
+                # forward bonds only: Hermitian part of a flux phase (heisenberg_kitaev.py:487-571)
+                phase = _bond_phase(lattice, i, nei, bool(getattr(lattice, "has_flux", False)))
                 if not split_couplings:
                     szsz = phase * cs * J * dlt
                     sxsx = phase * cs * J

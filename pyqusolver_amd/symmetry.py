@@ -93,7 +93,7 @@ def parse_symmetry_spec(name: str, value) -> List[Tuple[SymmetryGenerators, obje
     elif key in {"reflection", "reflect", "mirror"}:
         out.append((SymmetryGenerators.Reflection, value.get("sector", 1) if isinstance(value, dict) else value))
     elif key in {"inversion", "inv", "spatial", "spatialinversion"}:
-        raise NotImplementedError("inversion symmetry is not part of this build")
+        out.append((SymmetryGenerators.Inversion, value.get("sector", 1) if isinstance(value, dict) else value))
     else:
         raise ValueError(f"Unknown symmetry name: '{name}'")
     return out
@@ -162,6 +162,38 @@ class SectorGroup:
         return s
 
 
+def inversion_permutation(lattice, tol: float = 1e-7) -> np.ndarray:
+    """Site map of spatial inversion through the centre of the bounding box
+    of the site positions (InversionSymmetry, Algebra/Symmetries/
+    inversion.py:135-230): perm[i] = the site at 2 r0 - r_i, periodic images
+    included.  The reference applies it to bit positions (bit i -> bit
+    perm[i], inversion.py:53-86)."""
+    r = np.asarray(lattice.rvectors, dtype=float)
+    ns = r.shape[0]
+    r0 = 0.5 * (r.min(axis=0) + r.max(axis=0))
+    periods = [lattice.lx * lattice.a1, lattice.ly * lattice.a2, lattice.lz * lattice.a3]
+    flags = lattice.periodic_flags()
+    shifts = [np.zeros(3)]
+    for d in range(3):
+        if flags[d]:
+            shifts = [s + m * periods[d] for s in shifts for m in (-1, 0, 1)]
+    perm = np.empty(ns, dtype=np.int64)
+    for i in range(ns):
+        t = 2.0 * r0 - r[i]
+        best_j, best_d = -1, np.inf
+        for s in shifts:
+            d = np.linalg.norm(r - (t + s), axis=1)
+            j = int(np.argmin(d))
+            if d[j] < best_d:
+                best_j, best_d = j, float(d[j])
+        if best_d > tol:
+            raise ValueError(f"inversion image of site {i} is not a lattice site (distance {best_d:.3g})")
+        perm[i] = best_j
+    if len(np.unique(perm)) != ns:
+        raise ValueError("inversion is not a bijection of the lattice sites")
+    return perm
+
+
 def _is_real_momentum(k, L) -> bool:
     k = int(k) % L
     return k == 0 or (L % 2 == 0 and k == L // 2)
@@ -182,8 +214,13 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
                 continue          # acts trivially with U(1): dropped
             if n_up != ns // 2 or ns % 2 != 0:
                 continue          # X/Y need half filling
-        if gen in (SymmetryGenerators.Translation_z, SymmetryGenerators.Inversion):
+        if gen == SymmetryGenerators.Translation_z:
             raise NotImplementedError(f"{gen.name} is not part of this build")
+        if gen == SymmetryGenerators.Inversion:
+            if lattice is None:
+                raise ValueError("inversion symmetry requires a lattice")
+            if int(sec) not in (1, -1):
+                raise ValueError(f"Inversion sector must be +1 or -1, got {sec}")
         if any(g == gen for g, _ in gens):
             raise ValueError(f"duplicate generator {gen.name}")
         gens.append((gen, sec))
@@ -193,14 +230,24 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
     ly = getattr(lattice, "ly", 1) if lattice is not None else 1
     nb = getattr(lattice, "n_basis", 1) if lattice is not None else 1
     trans = {g: s for g, s in gens if g in (SymmetryGenerators.Translation_x, SymmetryGenerators.Translation_y)}
+    flux = getattr(lattice, "flux", None) or {}
+    for g in trans:
+        d = "x" if g == SymmetryGenerators.Translation_x else "y"
+        if abs(float(flux.get(d, 0.0))) > 0.0:
+            raise NotImplementedError(
+                f"translation along {d} with a boundary flux along {d}: the flux phase on the boundary bonds breaks "
+                "the translation invariance of the couplings; the twisted translation (boundary_phases table in the "
+                "group action) is not supported -- drop that translation")
     others = [(g, s) for g, s in gens if g not in trans]
-    has_refl = any(g == SymmetryGenerators.Reflection for g, _ in others)
+    spatial = [g for g, _ in others if g in (SymmetryGenerators.Reflection, SymmetryGenerators.Inversion)]
     matvec_error = None
-    if has_refl:
+    if spatial:
+        # P T P^-1 = T^-1: chi(T^n P) = exp(ikn) p is a representation only at k = 0, pi
         for g, k in trans.items():
             L = lx if g == SymmetryGenerators.Translation_x else ly
             if not _is_real_momentum(k, L):
-                matvec_error = (f"reflection is only a symmetry of momentum sectors k=0 or k=L/2 (got k={k}, L={L}); "
+                name = spatial[0].name.lower()
+                matvec_error = (f"{name} is only a symmetry of momentum sectors k=0 or k=L/2 (got k={k}, L={L}); "
                                 "the reference block is not Hermitian there -- use the translation sector alone")
     if any(g == SymmetryGenerators.ParityY for g, _ in others):
         matvec_error = ("parity-y sectors: the reference's +-i intrinsic phase enters the normalisation but not "
@@ -218,6 +265,12 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
         for g, sec in elem:
             if g == SymmetryGenerators.Reflection:
                 ops.append(BOP_REFLECT)
+            elif g == SymmetryGenerators.Inversion:
+                perm = inversion_permutation(lattice)
+                if not np.array_equal(perm, np.arange(ns)[::-1]):
+                    raise NotImplementedError("inversion permutations other than the full site reversal "
+                                              "(chain, square and honeycomb lattices here) are not supported")
+                ops.append(BOP_REFLECT)      # bit i -> bit perm[i] = ns-1-i
             elif g == SymmetryGenerators.ParityY:
                 ops.append(BOP_FLIP_Y)
             else:

@@ -31,6 +31,8 @@ def _sym(gens):
             tr["ky"] = s
         elif k == "R":
             sym["reflection"] = s
+        elif k == "I":
+            sym["inversion"] = s
         elif k in ("Px", "Py", "Pz"):
             sym.setdefault("parity", {})[k[1]] = s
     if tr:
@@ -88,6 +90,11 @@ CASES = [
     ("square", 3, 4, [("ty", 1)], 6, "j1j2"),
     ("honeycomb", 3, 2, [("tx", 0), ("ty", 0)], None, "hk"),
     ("honeycomb", 2, 3, [("tx", 1), ("ty", 2)], None, "hk"),
+    ("chain", 14, 1, [("tx", 7), ("I", -1), ("Px", 1)], 7, "j1j2"),
+    ("square", 4, 4, [("tx", 0), ("ty", 2), ("I", -1)], 8, "j1j2"),
+    ("square", 4, 3, [("tx", 2), ("ty", 0), ("I", 1)], None, "j1j2_field"),
+    ("honeycomb", 3, 2, [("tx", 0), ("ty", 0), ("I", 1)], None, "hk"),
+    ("honeycomb", 2, 3, [("tx", 1), ("ty", 0), ("I", -1)], 6, "hk"),
 ]
 
 
@@ -160,6 +167,21 @@ def test_accumulate_flag():
     y = y0.clone()
     H.matvec_device(x, out=y, accumulate=True)
     assert torch.allclose(y, y0 + H.matvec_device(x), rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("spatial", ["R", "I"])
+def test_spatial_symmetry_at_generic_momentum_keeps_basis_refuses_matvec(spatial):
+    """Reflection / inversion with a translation at k not in {0, pi}: the
+    reference accepts the sector (translation commutes with both by its
+    compatibility table, translation.py:137-148) but exp(ikn) p is not a
+    representation there (P T P^-1 = T^-1), so its block is not Hermitian:
+    the basis is built bit-exactly, the matvec refused."""
+    from pyqusolver_amd._lib import QesError
+    hs, H, grp, b, t = build_pair("chain", 12, 1, [("tx", 2), (spatial, 1)], 6, "xxz")
+    np.testing.assert_array_equal(hs.representative_list, b.reps)
+    np.testing.assert_allclose(hs.normalization, b.norms, rtol=1e-14, atol=0)
+    with pytest.raises(QesError):
+        H.matvec_fun(np.ones(hs.nh, dtype=np.complex128))
 
 
 def test_real_vector_with_complex_sector_raises():
@@ -443,3 +465,38 @@ def test_projected_matvec_same_sector_and_2d():
     bi, bo = O.scan_basis(gi), O.scan_basis(go)
     yo = O.projected_matvec(gi, bi, go, bo, 12, codes, sites, coeffs, True, xin)
     assert np.linalg.norm(y - yo) <= MATVEC_TOL * np.linalg.norm(yo)
+
+
+@pytest.mark.parametrize("gens", [[], [("tx", 1)], [("tx", 0), ("I", 1)]], ids=["none", "tx1", "tx0-I+"])
+def test_boundary_flux_kitaev_sector_parity(gens):
+    """Kitaev-Heisenberg honeycomb 3x2 with a flux 2 pi/3 along y (bonds
+    carry the Hermitian part of the boundary phase, heisenberg_kitaev.py:
+    487-571): reduced matvec and E0 against the oracle in sectors that keep
+    the couplings invariant (translation along x, inversion), the explicit
+    build Hermitian, and translation along y refused."""
+    from pyqusolver_amd import HoneycombLattice
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import HeisenbergKitaev
+    phi = 2 * np.pi / 3
+    lat = HoneycombLattice(3, 2, flux={"y": phi})
+    olat = O.OLattice("honeycomb", 3, 2)
+    h = (0.03, 0.02, 0.05)
+    hs = HilbertSpace(lattice=lat, sym_gen=_sym(gens))
+    H = HeisenbergKitaev(lat, K=-1.0, J=0.3, hilbert_space=hs, hx=h[0], hy=h[1], hz=h[2])
+    grp = O.Group(olat, [O.Generator(k, s) for k, s in gens])
+    b = O.scan_basis(grp)
+    np.testing.assert_array_equal(hs.representative_list, b.reps)
+    c, s, co = O.terms_heisenberg_kitaev(olat, K=-1.0, J=0.3, hx=h[0], hy=h[1], hz=h[2], flux=(0.0, phi)).arrays()
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(hs.nh) + 1j * rng.standard_normal(hs.nh)
+    y = H.matvec_fun(x)
+    yo = O.push_matvec(b, lat.ns, c, s, co, True, x)
+    assert np.linalg.norm(y - yo) <= MATVEC_TOL * np.linalg.norm(yo)
+    A = H.build(use_numpy=True)
+    assert np.abs(A - A.conj().T).max() <= 1e-12
+    Hd = O.reduced_dense(b, lat.ns, c, s, co, True)
+    e_ref = np.linalg.eigvalsh(Hd)[0]
+    e0 = H.diagonalize(k=1, tol=1e-12, maxiter=400)[0]
+    assert abs(e0 - e_ref) <= E0_TOL * abs(e_ref)
+    with pytest.raises(NotImplementedError):
+        HilbertSpace(lattice=lat, sym_gen={"translation": {"ky": 0}})

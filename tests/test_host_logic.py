@@ -107,3 +107,86 @@ def test_split_couplings_same_matrix_bruteforce():
     Ha = O.full_dense_hamiltonian(8, ka, sa, ca, True)
     Hb = O.full_dense_hamiltonian(8, kb, sb, cb, True)
     np.testing.assert_allclose(Ha, Hb, atol=1e-14)
+
+
+# ------------------------------------------------------------- inversion
+
+@pytest.mark.parametrize("lat,olat", [(ChainLattice(9), O.OLattice("chain", 9)),
+                                      (SquareLattice(4, 3), O.OLattice("square", 4, 3)),
+                                      (HoneycombLattice(3, 2), O.OLattice("honeycomb", 3, 2))])
+def test_inversion_permutation_and_characters_match_oracle(lat, olat):
+    from pyqusolver_amd.symmetry import inversion_permutation
+    np.testing.assert_array_equal(inversion_permutation(lat), O.inversion_perm(olat))
+    tr = {"kx": 0, "ky": 0} if lat.ly > 1 else 0
+    otr = [("tx", 0), ("ty", 0)] if lat.ly > 1 else [("tx", 0)]
+    for sec in (1, -1):
+        g = _grp(lat, {"translation": tr, "inversion": sec})
+        og = O.Group(olat, [O.Generator(k, s) for k, s in otr + [("I", sec)]])
+        assert g.order == og.n_group and g.matvec_error is None
+        np.testing.assert_array_equal(g.chi, og.chi)
+    assert _grp(lat, {"translation": {"kx": 1, "ky": 0} if lat.ly > 1 else 1, "inversion": 1}).matvec_error
+
+
+# ------------------------------------------------------------- boundary flux
+
+def test_bond_windings_and_flux_phases():
+    """Forward bonds leaving the cell range wind -1 (negative step) or +1;
+    with flux phi along d they carry exp(i w phi); a chain's closing bond is
+    the only winding one; every honeycomb row is crossed by lx y bonds."""
+    ch = ChainLattice(6, flux=np.pi)
+    w = {(i, j): ch.bond_winding(i, j) for i, j, _ in ch.bonds(1)}
+    assert [k for k, v in w.items() if v != (0, 0, 0)] == [(5, 0)] and w[(5, 0)] == (1, 0, 0)
+    assert ch.has_flux and ch.boundary_phase_from_winding(1, 0, 0) == pytest.approx(-1.0)
+    hc = HoneycombLattice(3, 2, flux={"y": 2 * np.pi / 3})
+    ws = [hc.bond_winding(i, j) for i, j, _ in hc.bonds(1)]
+    assert sum(1 for v in ws if v == (0, -1, 0)) == 3 and sum(1 for v in ws if v == (-1, 0, 0)) == 2
+    assert hc.boundary_phase_from_winding(0, -1, 0) == pytest.approx(np.exp(-2j * np.pi / 3))
+    assert hc.boundary_phase_from_winding(-1, 0, 0) == 1.0
+    tab = hc.boundary_phases()
+    assert tab.shape == (3, hc.ns + 1)
+    np.testing.assert_allclose(tab[1, 2], np.exp(2j * 2 * np.pi / 3))
+    assert not HoneycombLattice(2, 2).has_flux
+
+
+def test_kitaev_flux_terms_match_oracle_and_are_hermitian():
+    """HeisenbergKitaev with a boundary flux: forward bonds carry the
+    Hermitian part of their phase (heisenberg_kitaev.py:487-571), so the
+    reference's own check holds (tests/models/test_heisenberg_kitaev_flux.py:
+    honeycomb 2x2, flux_y = 2 pi/3, J = 1, K = 0: H Hermitian); the
+    instruction list equals the oracle's; fluxes phi, -phi and phi + 2 pi
+    give the same matrix."""
+    from pyqusolver_amd.models import HeisenbergKitaev
+    lat, olat = HoneycombLattice(2, 2, flux={"x": 0.0, "y": 2 * np.pi / 3}), O.OLattice("honeycomb", 2, 2)
+    H = HeisenbergKitaev(lat, K=(0.0, 0.0, 0.0), J=1.0, hilbert_space=_dummy_space(lat), dtype=np.complex128)
+    s, c, k = H.instruction_arrays()
+    Hf = O.full_dense_hamiltonian(8, k, s, c, True)
+    assert np.linalg.norm(Hf - Hf.conj().T) < 1e-10
+    oc, os_, oco = O.terms_heisenberg_kitaev(olat, K=0.0, J=1.0, flux=(0.0, 2 * np.pi / 3)).arrays()
+    np.testing.assert_array_equal(k, oc)
+    np.testing.assert_array_equal(s, os_)
+    np.testing.assert_allclose(c, oco, rtol=1e-15, atol=0)
+    ref = O.full_dense_hamiltonian(8, oc, os_, oco, True)
+    for phi in (-2 * np.pi / 3, 2 * np.pi / 3 + 2 * np.pi):
+        o2 = O.terms_heisenberg_kitaev(olat, K=0.0, J=1.0, flux=(0.0, phi)).arrays()
+        np.testing.assert_allclose(O.full_dense_hamiltonian(8, *o2, True), ref, atol=1e-13)
+    # the flux changes the spectrum (it is not a no-op)
+    o0 = O.terms_heisenberg_kitaev(olat, K=0.0, J=1.0).arrays()
+    assert abs(np.linalg.eigvalsh(ref)[0] - np.linalg.eigvalsh(O.full_dense_hamiltonian(8, *o0, True))[0]) > 1e-3
+
+
+def test_flux_refusals():
+    """A complex bond phase would make an XXZ / J1-J2 bond non-Hermitian;
+    translation along a fluxed direction is not a symmetry of the couplings."""
+    from pyqusolver_amd.models import XXZ, J1J2Model
+    lat = ChainLattice(8, flux=0.3)
+    with pytest.raises(ValueError):
+        XXZ(lat, hilbert_space=_dummy_space(lat), jxy=1.0, jz=0.5)
+    with pytest.raises(ValueError):
+        J1J2Model(lat, J1=1.0, J2=0.0, hilbert_space=_dummy_space(lat))
+    XXZ(ChainLattice(8, flux=np.pi), hilbert_space=_dummy_space(lat), jxy=1.0, jz=0.5)   # real phase -1
+    with pytest.raises(NotImplementedError):
+        _grp(ChainLattice(8, flux=np.pi), {"translation": 0})
+    hc = HoneycombLattice(3, 2, flux={"y": 1.0})
+    with pytest.raises(NotImplementedError):
+        _grp(hc, {"translation": {"kx": 0, "ky": 0}})
+    assert _grp(hc, {"translation": {"kx": 1}}).order == 3

@@ -87,6 +87,11 @@ CASES = [
     ("square", 3, 3, [("tx", 1), ("ty", 2)], None),
     ("honeycomb", 2, 2, [("tx", 1), ("ty", 0)], None),
     ("honeycomb", 3, 2, [("tx", 0), ("ty", 1)], 6),
+    ("chain", 10, 1, [("tx", 5), ("I", -1), ("Px", 1)], 5),
+    ("square", 4, 2, [("tx", 0), ("ty", 1), ("I", 1)], 4),
+    ("square", 3, 3, [("tx", 0), ("ty", 0), ("I", -1)], None),
+    ("honeycomb", 2, 2, [("tx", 0), ("ty", 0), ("I", -1)], None),
+    ("honeycomb", 3, 2, [("tx", 0), ("ty", 0), ("I", 1)], 6),
 ]
 
 
@@ -235,3 +240,60 @@ def test_projected_matvec_is_projected_operator():
     x = np.random.default_rng(1).standard_normal(len(bi.reps)) + 0j
     np.testing.assert_allclose(O.projected_matvec(gi, bi, gi, bi, L, c, s, co, True, x),
                                O.push_matvec(bi, L, c, s, co, True, x), rtol=0, atol=1e-13)
+
+
+# ------------------------------------------------------------- inversion
+
+@pytest.mark.parametrize("kind,lx,ly", [("chain", 9, 1), ("chain", 10, 1), ("square", 4, 3), ("square", 3, 3),
+                                        ("honeycomb", 2, 3), ("honeycomb", 3, 3)])
+def test_inversion_perm_is_a_point_inversion_and_lattice_automorphism(kind, lx, ly):
+    """inversion_perm (inversion.py:135-230): an involution; r_i + r_perm[i]
+    equals twice the bounding-box centre up to lattice periods; it maps the
+    bond set onto itself with each honeycomb bond keeping its x/y/z type;
+    a chain reduces to the closed form i -> L-1-i."""
+    lat = O.OLattice(kind, lx, ly)
+    p = O.inversion_perm(lat)
+    ns = lat.ns
+    np.testing.assert_array_equal(p[p], np.arange(ns))
+    a1, a2, a3, basis, fracs, subs, r = lat.geometry()
+    r0 = 0.5 * (r.min(axis=0) + r.max(axis=0))
+    A = np.column_stack([lx * a1, ly * a2, a3])
+    for i in range(ns):
+        d = np.linalg.solve(A, r[i] + r[p[i]] - 2 * r0)
+        np.testing.assert_allclose(d[:2], np.rint(d[:2]), atol=1e-9)
+    orders = (1,) if kind == "honeycomb" else (1, 2)
+    for order in orders:
+        bonds = O.oracle_bonds(lat, order)
+        typed = {(min(i, j), max(i, j)): n for i, j, n in bonds}
+        for i, j, n in bonds:
+            key = (min(p[i], p[j]), max(p[i], p[j]))
+            assert key in typed
+            if kind == "honeycomb":
+                assert typed[key] == n
+    if kind == "chain":
+        np.testing.assert_array_equal(p, np.arange(ns)[::-1])
+
+
+@pytest.mark.parametrize("kind,lx,ly,n_up", [("square", 4, 2, 4), ("honeycomb", 2, 2, None), ("chain", 10, 1, 5)])
+def test_inversion_sectors_split_the_momentum_sector(kind, lx, ly, n_up):
+    """The k = 0 block's spectrum is the union of the I = +1 and I = -1
+    blocks (dimensions add up, eigenvalues match)."""
+    lat = O.OLattice(kind, lx, ly)
+    if kind == "honeycomb":
+        t = O.terms_heisenberg_kitaev(lat, K=-1.0, J=0.2, hx=0.03, hy=0.02, hz=0.05)
+    elif kind == "square":
+        t = O.terms_j1j2(lat, -1.0, -0.4)
+    else:
+        t = O.terms_xxz(lat, 1.0, 0.7, hz=0.0)
+    c, s, co = t.arrays()
+    tr = [("tx", 0)] + ([("ty", 0)] if kind != "chain" else [])
+
+    def block(gens):
+        grp = O.Group(lat, [O.Generator(k, v) for k, v in gens], n_up=n_up)
+        b = O.scan_basis(grp)
+        return np.linalg.eigvalsh(O.reduced_dense(b, lat.ns, c, s, co, t.is_complex)) if len(b.reps) else np.zeros(0)
+    e_all = block(tr)
+    e_pm = np.concatenate([block(tr + [("I", 1)]), block(tr + [("I", -1)])])
+    assert len(e_pm) == len(e_all)
+    np.testing.assert_allclose(np.sort(e_pm), e_all, atol=1e-12)
+
