@@ -231,14 +231,15 @@ __global__ void __launch_bounds__(256) asm_rows_kernel(QesGroup G, QesLookup L, 
   }
 }
 
-// y = A x with the sliced table; XC: complex vectors
-template <bool XC>
+// y = A x with the sliced table; XC: complex vectors, FMT / CV: table format
+// and complex values
+template <bool XC, int FMT, bool CV>
 __global__ void __launch_bounds__(256) sell_matvec_kernel(AsmTable T, const double* __restrict__ x,
                                                           double* __restrict__ y, int64_t n, int accumulate) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
   double ar, ai;
-  sell_row<XC>(T, a, x, ar, ai);
+  sell_row_t<FMT, XC, CV>(T, a, x, ar, ai);
   if (XC) {
     if (accumulate) {
       y[2 * a] += ar;
@@ -624,11 +625,26 @@ int assembled_coo(const Op* op, int32_t* rows, int32_t* cols, double* vals, int6
   return 0;
 }
 
+template <bool XC>
+static void launch_sell_x(const AsmTable& T, const double* x, double* y, int64_t n, int accumulate, hipStream_t st) {
+  const dim3 grid((unsigned)((n + 255) / 256));
+  const bool cv = T.fmt == ASM_PLAIN ? T.vc != 0 : T.dvc != 0;
+#define QES_SELL(F, C) \
+  hipLaunchKernelGGL((sell_matvec_kernel<XC, F, C>), grid, dim3(256), 0, st, T, x, y, n, accumulate)
+  if (T.fmt == ASM_P4) {
+    if (cv) QES_SELL(ASM_P4, true); else QES_SELL(ASM_P4, false);
+  } else if (T.fmt == ASM_P6) {
+    if (cv) QES_SELL(ASM_P6, true); else QES_SELL(ASM_P6, false);
+  } else {
+    if (cv) QES_SELL(ASM_PLAIN, true); else QES_SELL(ASM_PLAIN, false);
+  }
+#undef QES_SELL
+}
+
 static int launch_sell(const AsmTable& T, const double* x, double* y, int64_t n, bool xc, int accumulate,
                        hipStream_t st) {
-  const dim3 grid((unsigned)((n + 255) / 256));
-  if (xc) hipLaunchKernelGGL((sell_matvec_kernel<true>), grid, dim3(256), 0, st, T, x, y, n, accumulate);
-  else hipLaunchKernelGGL((sell_matvec_kernel<false>), grid, dim3(256), 0, st, T, x, y, n, accumulate);
+  if (xc) launch_sell_x<true>(T, x, y, n, accumulate, st);
+  else launch_sell_x<false>(T, x, y, n, accumulate, st);
   QES_HIP(hipGetLastError());
   return 0;
 }

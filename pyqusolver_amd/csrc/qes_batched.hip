@@ -274,7 +274,18 @@ __global__ void __launch_bounds__(BRB) batch_matvec_kernel(const QesProb* __rest
       if (a < P.n) {
         const int cw = CPLX ? 2 : 1;
         double ar, ai;
-        sell_row<CPLX>(P.T, a, x + P.off * cw, ar, ai);
+        const double* xp = x + P.off * cw;
+        const bool cv = P.T.fmt == ASM_PLAIN ? P.T.vc != 0 : P.T.dvc != 0;
+        if (P.T.fmt == ASM_P4) {
+          if (cv) sell_row_t<ASM_P4, CPLX, true>(P.T, a, xp, ar, ai);
+          else sell_row_t<ASM_P4, CPLX, false>(P.T, a, xp, ar, ai);
+        } else if (P.T.fmt == ASM_P6) {
+          if (cv) sell_row_t<ASM_P6, CPLX, true>(P.T, a, xp, ar, ai);
+          else sell_row_t<ASM_P6, CPLX, false>(P.T, a, xp, ar, ai);
+        } else {
+          if (cv) sell_row_t<ASM_PLAIN, CPLX, true>(P.T, a, xp, ar, ai);
+          else sell_row_t<ASM_PLAIN, CPLX, false>(P.T, a, xp, ar, ai);
+        }
         double* yp = y + (P.off + a) * cw;
         yp[0] = ar;
         if (CPLX) yp[1] = ai;

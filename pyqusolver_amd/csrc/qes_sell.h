@@ -8,6 +8,9 @@
 namespace qes {
 
 static constexpr int SELL_C = 64;
+#ifndef QES_SELL_UNROLL
+#define QES_SELL_UNROLL 4   // slots per lane in flight in sell_row_t
+#endif
 static constexpr int ASM_MAX_CPL = 8;   // coupling ids of a coupling-tagged table
 enum { ASM_PLAIN = 0, ASM_P4 = 1, ASM_P6 = 2 };
 
@@ -44,70 +47,101 @@ __device__ __forceinline__ void sell_entry(const AsmTable& T, int64_t e, int64_t
   }
 }
 
-// (ar, ai) = row a of A times x; XC: complex x (interleaved)
-template <bool XC>
-__device__ __forceinline__ void sell_row(const AsmTable& T, int64_t a, const double* __restrict__ x, double& ar,
-                                         double& ai) {
+// Row product specialised on the table format (FMT), complex vectors (XC)
+// and complex values (CV: complex dictionary / plain values).  Slots are
+// consumed four at a time: the four entry loads, then the four gathers and
+// value loads, then the four multiply-adds in slot order, so each lane keeps
+// several independent gathers in flight; the sums are formed in the same
+// order as sell_row, so the results are identical.
+template <int FMT, bool XC, bool CV>
+__device__ __forceinline__ void sell_row_t(const AsmTable& T, int64_t a, const double* __restrict__ x, double& ar,
+                                           double& ai) {
+  constexpr int U = QES_SELL_UNROLL;
+  constexpr int VW = CV ? 2 : 1;
   const int64_t s = a / SELL_C;
   const int l = (int)(a % SELL_C);
   const int64_t base = T.off[s] + l;
   const int len = T.len[s];
   ar = 0.0;
   ai = 0.0;
-  if (T.fmt == ASM_PLAIN) {
-    const int vw = T.vc ? 2 : 1;
-    for (int k = 0; k < len; ++k) {
-      const int64_t e = base + (int64_t)k * SELL_C;
-      const int64_t b = T.col[e];
-      const double vr = T.val[e * vw];
-      if (XC) {
-        const double vi = T.vc ? T.val[e * vw + 1] : 0.0;
-        const double xr = x[2 * b], xi = x[2 * b + 1];
-        ar += vr * xr - vi * xi;
-        ai += vr * xi + vi * xr;
-      } else {
-        ar += vr * x[b];
-      }
-    }
-    return;
-  }
-  const int dw = T.dvc ? 2 : 1;
-  double dr, di = 0.0;
-  if (T.ncpl > 0) {
-    dr = 0.0;
+  if (FMT != ASM_PLAIN) {
+    double dr, di = 0.0;
+    if (T.ncpl > 0) {
+      dr = 0.0;
 #pragma unroll
-    for (int c = 0; c < ASM_MAX_CPL; ++c) {
-      if (c < T.ncpl) {
-        const double* dc = T.diag_c + ((int64_t)c * T.nrows + a) * dw;
-        dr += T.dscale[c] * dc[0];
-        if (T.dvc) di += T.dscale[c] * dc[1];
+      for (int c = 0; c < ASM_MAX_CPL; ++c) {
+        if (c < T.ncpl) {
+          const double* dc = T.diag_c + ((int64_t)c * T.nrows + a) * VW;
+          dr += T.dscale[c] * dc[0];
+          if (CV) di += T.dscale[c] * dc[1];
+        }
       }
+    } else {
+      dr = T.diag[a * VW];
+      if (CV) di = T.diag[a * VW + 1];
     }
-  } else {
-    dr = T.diag[a * dw];
-    if (T.dvc) di = T.diag[a * dw + 1];
-  }
-  if (XC) {
-    const double xr = x[2 * a], xi = x[2 * a + 1];
-    ar = dr * xr - di * xi;
-    ai = dr * xi + di * xr;
-  } else {
-    ar = dr * x[a];
-  }
-  for (int k = 0; k < len; ++k) {
-    const int64_t e = base + (int64_t)k * SELL_C;
-    int64_t b;
-    uint32_t idx;
-    sell_entry(T, e, b, idx);
     if (XC) {
-      const double vr = T.dict[idx * dw];
-      const double vi = T.dvc ? T.dict[idx * dw + 1] : 0.0;
+      const double xr = x[2 * a], xi = x[2 * a + 1];
+      ar = dr * xr - di * xi;
+      ai = dr * xi + di * xr;
+    } else {
+      ar = dr * x[a];
+    }
+  }
+  const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
+  auto entry = [&](int64_t e, int64_t& b, double& vr, double& vi) {
+    if (FMT == ASM_P4) {
+      const uint32_t p = T.packed[e];
+      b = p & cmask;
+      const uint32_t idx = p >> T.colbits;
+      vr = T.dict[idx * VW];
+      vi = CV ? T.dict[idx * VW + 1] : 0.0;
+    } else if (FMT == ASM_P6) {
+      b = T.col[e];
+      const uint32_t idx = T.vidx[e];
+      vr = T.dict[idx * VW];
+      vi = CV ? T.dict[idx * VW + 1] : 0.0;
+    } else {
+      b = T.col[e];
+      vr = T.val[e * VW];
+      vi = CV ? T.val[e * VW + 1] : 0.0;
+    }
+  };
+  auto accumulate = [&](int64_t b, double vr, double vi) {
+    if (XC) {
       const double xr = x[2 * b], xi = x[2 * b + 1];
       ar += vr * xr - vi * xi;
       ai += vr * xi + vi * xr;
     } else {
-      ar += T.dict[idx] * x[b];
+      ar += vr * x[b];
     }
+  };
+  int k = 0;
+  for (; k + U <= len; k += U) {
+    int64_t b[U];
+    double vr[U], vi[U], xr[U], xi[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) entry(base + (int64_t)(k + u) * SELL_C, b[u], vr[u], vi[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      xr[u] = XC ? x[2 * b[u]] : x[b[u]];
+      xi[u] = XC ? x[2 * b[u] + 1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (XC) {
+        ar += vr[u] * xr[u] - vi[u] * xi[u];
+        ai += vr[u] * xi[u] + vi[u] * xr[u];
+      } else {
+        ar += vr[u] * xr[u];
+      }
+    }
+  }
+  for (; k < len; ++k) {
+    int64_t b;
+    double vr, vi;
+    entry(base + (int64_t)k * SELL_C, b, vr, vi);
+    accumulate(b, vr, vi);
   }
 }
 
