@@ -105,7 +105,10 @@ __global__ void __launch_bounds__(RB) recur_kernel(double* __restrict__ w, const
     w[i] = w[i] - al * v[i] - be * vp[i];
 }
 
-// partial projections c_i = <V_i, w> for i <= j; grid (nchunk, ceil(m/VB))
+// partial projections c_i = <V_i, w> for i <= j; grid (nchunk, ceil(m/VB)).
+// The vectors are read as 16-byte pairs of doubles (two real entries or one
+// complex entry; V's leading dimension is even); an odd real length leaves
+// one trailing entry, added by the first block column.
 template <bool CPLX>
 __global__ void __launch_bounds__(RB) proj_partial_kernel(const double* __restrict__ V, int64_t ldv,
                                                           const double* __restrict__ w, int64_t n,
@@ -122,23 +125,29 @@ __global__ void __launch_bounds__(RB) proj_partial_kernel(const double* __restri
     r[k] = 0.0;
     im[k] = 0.0;
   }
-  const int cw = CPLX ? 2 : 1;
-  for (int64_t e = (int64_t)blockIdx.x * RB + threadIdx.x; e < n; e += (int64_t)nchunk * RB) {
-    double wr = w[e * cw], wi = CPLX ? w[e * cw + 1] : 0.0;
+  const int64_t len = CPLX ? 2 * n : n;
+  const int64_t m2 = len >> 1;
+  const double2* __restrict__ w2 = reinterpret_cast<const double2*>(w);
+  for (int64_t q = (int64_t)blockIdx.x * RB + threadIdx.x; q < m2; q += (int64_t)nchunk * RB) {
+    const double2 wq = w2[q];
 #pragma unroll
     for (int k = 0; k < VB; ++k) {
       if (i0 + k <= j) {
-        const double* vp = V + (int64_t)(i0 + k) * ldv + e * cw;
-        double vr = vp[0];
+        const double2 vq = reinterpret_cast<const double2*>(V + (int64_t)(i0 + k) * ldv)[q];
         if (CPLX) {
-          double vi = vp[1];
-          r[k] += vr * wr + vi * wi;
-          im[k] += vr * wi - vi * wr;
+          r[k] += vq.x * wq.x + vq.y * wq.y;
+          im[k] += vq.x * wq.y - vq.y * wq.x;
         } else {
-          r[k] += vr * wr;
+          r[k] += vq.x * wq.x;
+          r[k] += vq.y * wq.y;
         }
       }
     }
+  }
+  if (!CPLX && (len & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < VB; ++k)
+      if (i0 + k <= j) r[k] += V[(int64_t)(i0 + k) * ldv + len - 1] * w[len - 1];
   }
   // wavefront reduce then across the 4 waves
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -197,7 +206,9 @@ __global__ void __launch_bounds__(RB) proj_finish_kernel(const double* __restric
   }
 }
 
-// w -= sum_{i<=j} c_i V_i   (c complex when CPLX)
+// w -= sum_{i<=j} c_i V_i   (c complex when CPLX).  16-byte pairs as in
+// proj_partial_kernel, four stored vectors per loop trip (their loads issue
+// together; the sums keep the order i = 0, 1, ..., j).
 template <bool CPLX>
 __global__ void __launch_bounds__(RB) proj_update_kernel(const double* __restrict__ V, int64_t ldv,
                                                          double* __restrict__ w, int64_t n, const int* __restrict__ d_j,
@@ -210,22 +221,48 @@ __global__ void __launch_bounds__(RB) proj_update_kernel(const double* __restric
     sc[2 * i + 1] = coef[2 * i + 1];
   }
   __syncthreads();
-  const int cw = CPLX ? 2 : 1;
-  for (int64_t e = (int64_t)blockIdx.x * RB + threadIdx.x; e < n; e += (int64_t)gridDim.x * RB) {
-    double ar = 0.0, ai = 0.0;
-    for (int i = 0; i <= j; ++i) {
-      const double* vp = V + (int64_t)i * ldv + e * cw;
-      double cr = sc[2 * i], ci = sc[2 * i + 1];
-      if (CPLX) {
-        double vr = vp[0], vi = vp[1];
-        ar += cr * vr - ci * vi;
-        ai += cr * vi + ci * vr;
-      } else {
-        ar += cr * vp[0];
+  const int64_t len = CPLX ? 2 * n : n;
+  const int64_t m2 = len >> 1;
+  double2* __restrict__ w2 = reinterpret_cast<double2*>(w);
+  for (int64_t q = (int64_t)blockIdx.x * RB + threadIdx.x; q < m2; q += (int64_t)gridDim.x * RB) {
+    double ax = 0.0, ay = 0.0;   // real: entries 2q, 2q+1; complex: re, im of entry q
+    int i = 0;
+    for (; i + 4 <= j + 1; i += 4) {
+      double2 vq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) vq[u] = reinterpret_cast<const double2*>(V + (int64_t)(i + u) * ldv)[q];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double cr = sc[2 * (i + u)], ci = sc[2 * (i + u) + 1];
+        if (CPLX) {
+          ax += cr * vq[u].x - ci * vq[u].y;
+          ay += cr * vq[u].y + ci * vq[u].x;
+        } else {
+          ax += cr * vq[u].x;
+          ay += cr * vq[u].y;
+        }
       }
     }
-    w[e * cw] -= ar;
-    if (CPLX) w[e * cw + 1] -= ai;
+    for (; i <= j; ++i) {
+      const double2 vq = reinterpret_cast<const double2*>(V + (int64_t)i * ldv)[q];
+      const double cr = sc[2 * i], ci = sc[2 * i + 1];
+      if (CPLX) {
+        ax += cr * vq.x - ci * vq.y;
+        ay += cr * vq.y + ci * vq.x;
+      } else {
+        ax += cr * vq.x;
+        ay += cr * vq.y;
+      }
+    }
+    double2 wq = w2[q];
+    wq.x -= ax;
+    wq.y -= ay;
+    w2[q] = wq;
+  }
+  if (!CPLX && (len & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    double a = 0.0;
+    for (int i = 0; i <= j; ++i) a += sc[2 * i] * V[(int64_t)i * ldv + len - 1];
+    w[len - 1] -= a;
   }
 }
 
@@ -298,6 +335,7 @@ struct Lanczos {
   int max_steps = 0;
   int store = 0;
   int64_t n = 0, len = 0;  // len = n * cw
+  int64_t ldv = 0;         // leading dimension of V: len rounded up to even (16-byte rows)
   double *v = nullptr, *vp = nullptr, *w = nullptr, *V = nullptr;
   double *alpha = nullptr, *beta = nullptr, *scal = nullptr, *part = nullptr, *coef = nullptr, *ppart = nullptr;
   int* d_j = nullptr;
@@ -349,23 +387,23 @@ static int enqueue_step(Lanczos* L, int reorth, hipStream_t st) {
       }
       dim3 pg(L->nchunk, L->nvec_blocks);
       if (c)
-        hipLaunchKernelGGL(proj_partial_kernel<true>, pg, dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->ppart,
+        hipLaunchKernelGGL(proj_partial_kernel<true>, pg, dim3(RB), 0, st, L->V, L->ldv, L->w, L->n, L->d_j, L->ppart,
                            L->nchunk, gate);
       else
-        hipLaunchKernelGGL(proj_partial_kernel<false>, pg, dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->ppart,
+        hipLaunchKernelGGL(proj_partial_kernel<false>, pg, dim3(RB), 0, st, L->V, L->ldv, L->w, L->n, L->d_j, L->ppart,
                            L->nchunk, gate);
       hipLaunchKernelGGL(proj_finish_kernel, dim3(L->maxv), dim3(RB), 0, st, L->ppart, L->nchunk, L->d_j, L->coef,
                          gate);
       if (c)
-        hipLaunchKernelGGL(proj_update_kernel<true>, dim3(g), dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j, L->coef,
+        hipLaunchKernelGGL(proj_update_kernel<true>, dim3(g), dim3(RB), 0, st, L->V, L->ldv, L->w, L->n, L->d_j, L->coef,
                            gate);
       else
-        hipLaunchKernelGGL(proj_update_kernel<false>, dim3(g), dim3(RB), 0, st, L->V, L->len, L->w, L->n, L->d_j,
+        hipLaunchKernelGGL(proj_update_kernel<false>, dim3(g), dim3(RB), 0, st, L->V, L->ldv, L->w, L->n, L->d_j,
                            L->coef, gate);
     }
   }
   enqueue_dot(L, L->w, L->w, L->scal, 1, st);
-  hipLaunchKernelGGL(advance_kernel, dim3(g), dim3(RB), 0, st, L->w, L->v, L->vp, L->V, L->len, L->store, L->scal,
+  hipLaunchKernelGGL(advance_kernel, dim3(g), dim3(RB), 0, st, L->w, L->v, L->vp, L->V, L->ldv, L->store, L->scal,
                      L->d_j, L->len, L->maxv);
   hipLaunchKernelGGL(set_beta_step_kernel, dim3(1), dim3(1), 0, st, L->scal, L->beta, L->d_j);
   QES_HIP(hipGetLastError());
@@ -384,6 +422,7 @@ int lanczos_create(Op* op, int max_steps, int store, int dtype, Lanczos** out) {
   L->store = store;
   L->n = op->basis->n;
   L->len = L->n * (dtype == 1 ? 2 : 1);
+  L->ldv = (L->len + 1) & ~int64_t(1);
   L->maxv = max_steps + 1;
   const int64_t len = std::max<int64_t>(L->len, 1);
   L->nred = grid_for(L->n);
@@ -393,7 +432,7 @@ int lanczos_create(Op* op, int max_steps, int store, int dtype, Lanczos** out) {
   QES_HIP(hipMalloc(&L->v, len * 8));
   QES_HIP(hipMalloc(&L->vp, len * 8));
   QES_HIP(hipMalloc(&L->w, len * 8));
-  if (store) QES_HIP(hipMalloc(&L->V, (size_t)len * 8 * L->maxv));
+  if (store) QES_HIP(hipMalloc(&L->V, (size_t)std::max<int64_t>(L->ldv, 2) * 8 * L->maxv));
   QES_HIP(hipMalloc(&L->alpha, sizeof(double) * L->maxv));
   QES_HIP(hipMalloc(&L->beta, sizeof(double) * L->maxv));
   QES_HIP(hipMalloc(&L->scal, sizeof(double) * 4));
@@ -504,9 +543,9 @@ int lanczos_combine(Lanczos* L, const double* c_host, int k, void* out, hipStrea
   QES_HIP(hipMemcpy(dc, c_host, sizeof(double) * k, hipMemcpyHostToDevice));
   const int g = grid_for(L->len);
   if (L->dtype == 1)
-    hipLaunchKernelGGL(combine_kernel<true>, dim3(g), dim3(RB), 0, L->stream, L->V, L->len, k, dc, (double*)out, L->n);
+    hipLaunchKernelGGL(combine_kernel<true>, dim3(g), dim3(RB), 0, L->stream, L->V, L->ldv, k, dc, (double*)out, L->n);
   else
-    hipLaunchKernelGGL(combine_kernel<false>, dim3(g), dim3(RB), 0, L->stream, L->V, L->len, k, dc, (double*)out, L->n);
+    hipLaunchKernelGGL(combine_kernel<false>, dim3(g), dim3(RB), 0, L->stream, L->V, L->ldv, k, dc, (double*)out, L->n);
   QES_HIP(hipGetLastError());
   QES_HIP(hipStreamSynchronize(L->stream));
   QES_HIP(hipFree(dc));
