@@ -287,6 +287,19 @@ int qes_batch_create(const qes_op_t* ops, int32_t nprob, int32_t max_steps, int3
 int qes_batch_create_blocks(const qes_op_t* ops, int32_t nprob, const qes_op_t* block_ops, const int32_t* block_nrhs,
                             const void* const* block_diag_dev, int32_t nblock, int32_t max_steps, int32_t dtype,
                             qes_batch_t* out);
+/*
+ * Coupling sweep: problem i is block_ops-style block of ncol columns over
+ * ops[i], whose operator must be assembled with coupling tags
+ * (qes_op_assemble mode 3); column r is its own Lanczos run on
+ * H_r = sum_c scales[(i*ncol + r)*8 + c] H_c (c < qes_op_n_couplings; the
+ * stride is 8 whatever the coupling count).  The tagged table is streamed
+ * once per step for all columns and every column has its own scaled value
+ * dictionary and diagonal.  Replaces one diagonalize() per coupling set
+ * (e.g. J2 in {0.4, 0.5, 0.6}) of the same sector.  The operators' tables
+ * must outlive the batch.  Vectors are n x ncol row-major per problem.
+ */
+int qes_batch_create_scaled(const qes_op_t* ops, int32_t nprob, int32_t ncol, const double* scales, int32_t max_steps,
+                            int32_t dtype, qes_batch_t* out);
 /* element offsets of every real problem (ordinary, then blocks) + total */
 int qes_batch_offsets(qes_batch_t b, int64_t* offsets /* nprob + nblock + 1 */);
 int qes_batch_start(qes_batch_t b, const void* v0_dev, void* stream);

@@ -25,7 +25,7 @@ def __getattr__(name):
     if name in ("DifferentiableHamiltonian", "HamiltonianMatvec"):
         from . import autograd
         return getattr(autograd, name)
-    if name in ("BatchedLanczos",):
+    if name in ("BatchedLanczos", "CouplingSweep"):
         from . import batched
         return getattr(batched, name)
     raise AttributeError(name)

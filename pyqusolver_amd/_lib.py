@@ -65,6 +65,7 @@ SIGNATURES = {
     "qes_op_assembly_info": [_P, _PI64, _PI64, _PI32, _PI32, _PI32],
     "qes_op_table_coo": [_P, _P, _P, _P, _I64, _PI64, _P],
     "qes_tridiag_ritz": [_PD, _PD, _I32, _PD, _PD, _PD],
+    "qes_batch_create_scaled": [C.POINTER(_P), _I32, _I32, _PD, _I32, _I32, C.POINTER(_P)],
     "qes_op_matvec_scaled": [_P, _PD, _I32, _P, _P, _I32, _P],
     "qes_op_coupling_dots": [_P, _P, _P, _I32, _PD, _P],
     "qes_op_diagonal": [_P, _P, _P, _P],

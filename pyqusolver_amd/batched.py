@@ -237,6 +237,71 @@ class BatchedLanczos:
             pass
 
 
+class CouplingSweep(BatchedLanczos):
+    """Ground states of every sector in ``hams`` at several coupling sets at
+    once: column r of sector p is its own Lanczos run on
+    H_r = sum_c scales[r][c] * H_c (H_c = the instructions tagged with
+    coupling id c, at the values the model was built with).  Each sector's
+    coupling-tagged table is streamed once per step for all columns.
+
+    ``scales``: (ncol, n_couplings) shared by all sectors, or (nprob, ncol,
+    n_couplings).  Results (``ground_states``) are in the order
+    sector 0 columns 0..ncol-1, sector 1, ... (``e0.reshape(nprob, ncol)``).
+    Tagged tables are built here when missing and freed by ``close``."""
+
+    MAX_CPL = 8
+
+    def __init__(self, hams: Sequence, scales, max_steps: int = 300, complex_: Optional[bool] = None,
+                 max_gb: float = 0.0):
+        if not hams:
+            raise ValueError("empty sweep")
+        hams = list(hams)
+        sc = np.asarray(scales, dtype=np.float64)
+        if sc.ndim == 2:
+            sc = np.broadcast_to(sc, (len(hams),) + sc.shape)
+        if sc.ndim != 3 or sc.shape[0] != len(hams):
+            raise ValueError("scales must be (ncol, n_couplings) or (nprob, ncol, n_couplings)")
+        ncol = sc.shape[1]
+        cplx = any(h.needs_complex for h in hams) if complex_ is None else bool(complex_)
+        self.dtype = 1 if cplx else 0
+        self.max_steps = int(max_steps)
+        self.ncol = ncol
+        self.sectors = hams
+        self._assembled_here = []
+        packed = np.zeros((len(hams), ncol, self.MAX_CPL))
+        for p, h in enumerate(hams):
+            nc = h.n_couplings
+            if sc.shape[2] < nc:
+                raise ValueError(f"sector {p} has {nc} coupling ids, scales give {sc.shape[2]}")
+            packed[p, :, :nc] = sc[p, :, :nc]
+            if not h.assembly["couplings"]:
+                h.assemble(True, max_gb=max_gb, couplings=True)
+                self._assembled_here.append(h)
+        self.scales = packed
+        self.hams = [h for h in hams for _ in range(ncol)]          # one entry per virtual problem
+        self.regular = []
+        self.blocks = [[p * ncol + r for r in range(ncol)] for p in range(len(hams))]
+        self.vp_of = np.arange(len(self.hams), dtype=np.int64)
+        ops = (C.c_void_p * len(hams))(*[h._device_op().value for h in hams])
+        flat = np.ascontiguousarray(packed.reshape(-1))
+        h_ = C.c_void_p()
+        _lib.check(_lib.lib().qes_batch_create_scaled(ops, len(hams), ncol, flat.ctypes.data_as(_lib._PD),
+                                                      self.max_steps, self.dtype, C.byref(h_)),
+                   "qes_batch_create_scaled")
+        self._h = h_
+        offs = np.zeros(len(hams) + 1, dtype=np.int64)
+        _lib.check(_lib.lib().qes_batch_offsets(self._h, offs.ctypes.data_as(_lib._PI64)))
+        self.offsets = offs
+
+    def ground_states(self, tol: float = 1e-10, check_every: int = 10, v0=None, seeds=None,
+                      sync_device: bool = True) -> BatchResult:
+        """Seeds default to sector index (the same start vector for every
+        column of a sector, as separate solves with seed = sector would use)."""
+        if seeds is None and v0 is None:
+            seeds = [p for p in range(len(self.sectors)) for _ in range(self.ncol)]
+        return super().ground_states(tol=tol, check_every=check_every, v0=v0, seeds=seeds, sync_device=sync_device)
+
+
 def momentum_sectors(L: int, reflection: bool = True):
     """Sector specs of a chain in the order of HilbertSpace.iter_momentum_sectors
     (hilbert.py:546-652): every k, with reflection parities only where they

@@ -59,8 +59,14 @@ struct QesProb {
   const int64_t* rowptr;
   const int32_t* col;
   const double* val;    // 1 (real) or 2 (complex) doubles per entry
-  // ordinary problems: the operator's assembled table (fmt < 0: none)
+  // ordinary problems: the operator's assembled table (fmt < 0: none).
+  // Coupling-scaled block problems: its coupling-tagged table, streamed once
+  // for all R columns; column r is H_r = sum_c cscale[r][c] H_c.
   AsmTable T;
+  const double* cscale;   // [R][ASM_MAX_CPL] (null: not a coupling-scaled block)
+  const double* dcols;    // [R][ndict] dictionary values scaled for each column (1 or 2 doubles each)
+  int32_t ndict;
+  int32_t pad2;
 };
 
 struct Tile {
@@ -258,6 +264,115 @@ __device__ __forceinline__ void block_row_table(const QesProb& P, int64_t a, int
   }
 }
 
+// Row of a coupling-scaled block problem from the operator's coupling-tagged
+// table: every slot is read once for all R columns (the lanes of a row read
+// the same entry) and the column's own scaled dictionary gives the value;
+// the diagonal of column r is sum_c cscale[r][c] diag_c[a].  Slots are taken
+// four at a time as in sell_row_t.
+template <int FMT, bool CPLX, bool CV>
+__device__ __forceinline__ void block_row_sell(const QesProb& P, int64_t a, int c0, int Rp, unsigned cm,
+                                               const double* __restrict__ x, double* __restrict__ y) {
+  constexpr int VW = CV ? 2 : 1;
+  const AsmTable& T = P.T;
+  const int cw = CPLX ? 2 : 1;
+  const int R = P.nrhs;
+  const double* xp0 = x + P.off * cw;
+  double* yp0 = y + P.off * cw;
+  double dcr[ASM_MAX_CPL], dci[ASM_MAX_CPL];
+#pragma unroll
+  for (int c = 0; c < ASM_MAX_CPL; ++c) {
+    dcr[c] = dci[c] = 0.0;
+    if (c < T.ncpl) {
+      const double* dc = T.diag_c + ((int64_t)c * T.nrows + a) * VW;
+      dcr[c] = dc[0];
+      if (CV) dci[c] = dc[1];
+    }
+  }
+  double ar[MAX_RHS / 64], ai[MAX_RHS / 64];
+#pragma unroll
+  for (int k = 0; k < MAX_RHS / 64; ++k) {
+    const int col = c0 + Rp * k;
+    ar[k] = 0.0;
+    ai[k] = 0.0;
+    if ((cm >> k) & 1) {
+      double dr = 0.0, di = 0.0;
+#pragma unroll
+      for (int c = 0; c < ASM_MAX_CPL; ++c) {
+        if (c < T.ncpl) {
+          const double s = P.cscale[col * ASM_MAX_CPL + c];
+          dr += s * dcr[c];
+          if (CV) di += s * dci[c];
+        }
+      }
+      const int64_t e = a * R + col;
+      if (CPLX) {
+        const double xr = xp0[e * 2], xi = xp0[e * 2 + 1];
+        ar[k] = dr * xr - di * xi;
+        ai[k] = dr * xi + di * xr;
+      } else {
+        ar[k] = dr * xp0[e];
+      }
+    }
+  }
+  const int64_t s = a / SELL_C;
+  const int l = (int)(a % SELL_C);
+  const int64_t base = T.off[s] + l;
+  const int len = T.len[s];
+  const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
+  auto entry = [&](int64_t e, int64_t& b, uint32_t& idx) {
+    if (FMT == ASM_P4) {
+      const uint32_t p = T.packed[e];
+      b = p & cmask;
+      idx = p >> T.colbits;
+    } else {
+      b = T.col[e];
+      idx = T.vidx[e];
+    }
+  };
+  auto acc = [&](int64_t b, uint32_t idx) {
+#pragma unroll
+    for (int k = 0; k < MAX_RHS / 64; ++k) {
+      const int col = c0 + Rp * k;
+      if ((cm >> k) & 1) {
+        const double* dv = P.dcols + ((int64_t)col * P.ndict + idx) * VW;
+        const double vr = dv[0], vi = CV ? dv[1] : 0.0;
+        const int64_t e = b * R + col;
+        if (CPLX) {
+          const double xr = xp0[e * 2], xi = xp0[e * 2 + 1];
+          ar[k] += vr * xr - vi * xi;
+          ai[k] += vr * xi + vi * xr;
+        } else {
+          ar[k] += vr * xp0[e];
+        }
+      }
+    }
+  };
+  int k = 0;
+  for (; k + 4 <= len; k += 4) {
+    int64_t b[4];
+    uint32_t idx[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) entry(base + (int64_t)(k + u) * SELL_C, b[u], idx[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc(b[u], idx[u]);
+  }
+  for (; k < len; ++k) {
+    int64_t b;
+    uint32_t idx;
+    entry(base + (int64_t)k * SELL_C, b, idx);
+    acc(b, idx);
+  }
+#pragma unroll
+  for (int kk = 0; kk < MAX_RHS / 64; ++kk) {
+    const int col = c0 + Rp * kk;
+    if ((cm >> kk) & 1) {
+      const int64_t e = a * R + col;
+      yp0[e * cw] = ar[kk];
+      if (CPLX) yp0[e * cw + 1] = ai[kk];
+    }
+  }
+}
+
 template <typename W, bool CPLX>
 __global__ void __launch_bounds__(BRB) batch_matvec_kernel(const QesProb* __restrict__ probs,
                                                            const Tile* __restrict__ tiles, int ntiles,
@@ -289,6 +404,22 @@ __global__ void __launch_bounds__(BRB) batch_matvec_kernel(const QesProb* __rest
         double* yp = y + (P.off + a) * cw;
         yp[0] = ar;
         if (CPLX) yp[1] = ai;
+      }
+      continue;
+    }
+    if (P.cscale) {   // coupling-scaled block: stream the tagged table once for all columns
+      const int Rp = block_lanes(P.nrhs);
+      const int c0 = threadIdx.x % Rp;
+      const unsigned cm = block_colmask(P, c0, Rp, vp_active);
+      const bool cv = P.T.dvc != 0;
+      for (int64_t a = T.row0 + threadIdx.x / Rp; a < (T.row0 + T.rows < P.n ? T.row0 + T.rows : P.n); a += BRB / Rp) {
+        if (P.T.fmt == ASM_P4) {
+          if (cv) block_row_sell<ASM_P4, CPLX, true>(P, a, c0, Rp, cm, x, y);
+          else block_row_sell<ASM_P4, CPLX, false>(P, a, c0, Rp, cm, x, y);
+        } else {
+          if (cv) block_row_sell<ASM_P6, CPLX, true>(P, a, c0, Rp, cm, x, y);
+          else block_row_sell<ASM_P6, CPLX, false>(P, a, c0, Rp, cm, x, y);
+        }
       }
       continue;
     }
@@ -562,6 +693,7 @@ struct BlockSpec {
   Op* op;
   int nrhs;
   const double* D;
+  const double* cscale;   // host [nrhs][ASM_MAX_CPL]: coupling-scaled block (D unused), or null
 };
 
 struct Batch {
@@ -593,6 +725,7 @@ struct Batch {
   std::vector<int64_t*> t_rowptr;   // per real problem (null for ordinary problems)
   std::vector<int32_t*> t_col;
   std::vector<double*> t_val;
+  std::vector<double*> s_buf;       // per real problem: device cscale + scaled dictionaries (coupling-scaled blocks)
   int64_t table_entries = 0;
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -650,7 +783,7 @@ static int batch_build_tables(Batch* B) {
   B->table_entries = 0;
   const int cw = B->dtype == 1 ? 2 : 1;
   for (int p = 0; p < B->nprob; ++p) {
-    if (!B->blk[p]) continue;
+    if (!B->blk[p] || B->h_probs[p].cscale) continue;
     const int64_t n = B->h_probs[p].n;
     if (n >= (int64_t(1) << 31)) return fail("block problem too large for 32-bit neighbour indices");
     int64_t* d_cnt = nullptr;
@@ -710,6 +843,7 @@ static void batch_free(Batch* B) {
   for (auto p : B->t_rowptr) hipFree(p);
   for (auto p : B->t_col) hipFree(p);
   for (auto p : B->t_val) hipFree(p);
+  for (auto p : B->s_buf) hipFree(p);
   if (B->exec) hipGraphExecDestroy(B->exec);
   if (B->graph) hipGraphDestroy(B->graph);
   hipFree(B->d_probs);
@@ -744,12 +878,19 @@ int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, 
     B->blk.push_back(0);
   }
   std::vector<const double*> D(nreg, nullptr);
+  std::vector<const double*> CS(nreg, nullptr);
   for (int i = 0; i < nblock; ++i) {
     if (blocks[i].nrhs < 1 || blocks[i].nrhs > MAX_RHS) {
       delete B;
       return fail("block problems hold 1..256 right-hand sides");
     }
-    if (!blocks[i].D) {
+    if (blocks[i].cscale) {
+      const Op* op = blocks[i].op;
+      if (op->asm_fmt < 0 || !op->asm_tagged) {
+        delete B;
+        return fail("a coupling-scaled block needs its operator assembled with coupling tags (qes_op_assemble mode 3)");
+      }
+    } else if (!blocks[i].D) {
       delete B;
       return fail("block problem without diagonal array");
     }
@@ -757,6 +898,7 @@ int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, 
     B->nrhs.push_back(blocks[i].nrhs);
     B->blk.push_back(1);
     D.push_back(blocks[i].D);
+    CS.push_back(blocks[i].cscale);
   }
   B->nprob = (int)B->ops.size();
   std::vector<QesProb> h(B->nprob);
@@ -791,7 +933,31 @@ int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, 
     q.n = Bs->n;
     q.off = off;
     q.T = asm_table_of(op);
-    if (R > 1) q.T.fmt = -1;
+    if (R > 1 && !CS[p]) q.T.fmt = -1;
+    if (CS[p]) {
+      // device: cscale [R][8], then the dictionary scaled per column [R][ndict][vw]
+      const int nd = op->asm_ndict, vw = op->asm_dvc ? 2 : 1;
+      std::vector<double> buf((size_t)R * ASM_MAX_CPL + (size_t)R * nd * vw, 0.0);
+      for (int r = 0; r < R; ++r)
+        for (int c = 0; c < ASM_MAX_CPL; ++c) buf[(size_t)r * ASM_MAX_CPL + c] = c < op->n_coupling ? CS[p][r * ASM_MAX_CPL + c] : 0.0;
+      double* dc = buf.data() + (size_t)R * ASM_MAX_CPL;
+      for (int r = 0; r < R; ++r)
+        for (int i = 0; i < nd; ++i)
+          for (int w = 0; w < vw; ++w)
+            dc[((size_t)r * nd + i) * vw + w] = op->asm_h_dict[(size_t)i * vw + w] * CS[p][r * ASM_MAX_CPL + op->asm_h_dict_cpl[i]];
+      double* d = nullptr;
+      if (hipMalloc(&d, buf.size() * sizeof(double)) != hipSuccess) {
+        for (auto b : B->s_buf) hipFree(b);
+        delete B;
+        return fail("hipMalloc failed in batch_create");
+      }
+      hipMemcpy(d, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice);
+      B->s_buf.push_back(d);
+      q.cscale = d;
+      q.dcols = d + (size_t)R * ASM_MAX_CPL;
+      q.ndict = nd;
+      q.T.ncpl = op->n_coupling;
+    }
     q.vp0 = vp;
     B->vp0.push_back(vp);
     if (R == 1) {
@@ -1093,10 +1259,28 @@ int qes_batch_create_blocks(const qes_op_t* ops, int32_t nprob, const qes_op_t* 
   std::vector<BlockSpec> bl(nblock);
   for (int i = 0; i < nblock; ++i) {
     if (!block_ops[i]) return fail("null block operator in batch");
-    bl[i] = BlockSpec{qes_op_internal(block_ops[i]), block_nrhs[i], (const double*)block_diag_dev[i]};
+    bl[i] = BlockSpec{qes_op_internal(block_ops[i]), block_nrhs[i], (const double*)block_diag_dev[i], nullptr};
   }
   Batch* B = nullptr;
   if (batch_create(v.data(), nprob, bl.data(), nblock, max_steps, dtype, &B)) return -1;
+  auto* h = new qes_batch_s();
+  h->B = B;
+  *out = h;
+  return 0;
+}
+
+int qes_batch_create_scaled(const qes_op_t* ops, int32_t nprob, int32_t ncol, const double* scales, int32_t max_steps,
+                            int32_t dtype, qes_batch_t* out) {
+  if (!out || (nprob > 0 && (!ops || !scales))) return fail("null argument");
+  if (nprob < 1) return fail("empty batch");
+  if (ncol < 1 || ncol > MAX_RHS) return fail("ncol must be 1..256");
+  std::vector<BlockSpec> bl(nprob);
+  for (int i = 0; i < nprob; ++i) {
+    if (!ops[i]) return fail("null operator in batch");
+    bl[i] = BlockSpec{qes_op_internal(ops[i]), ncol, nullptr, scales + (size_t)i * ncol * ASM_MAX_CPL};
+  }
+  Batch* B = nullptr;
+  if (batch_create(nullptr, 0, bl.data(), nprob, max_steps, dtype, &B)) return -1;
   auto* h = new qes_batch_s();
   h->B = B;
   *out = h;

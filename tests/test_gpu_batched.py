@@ -177,3 +177,86 @@ def test_block_complex_vectors():
     for r, h in enumerate(hams):
         ref = h.matvec_fun(xs[r].real) + 1j * h.matvec_fun(xs[r].imag)
         np.testing.assert_allclose(y[r].cpu().numpy(), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def _j1j2_chain_sectors(L, J2=0.5):
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import J1J2Model
+    lat = ChainLattice(L)
+    spaces = [HilbertSpace(lattice=lat, sym_gen={"translation": k}, global_syms=[get_u1_sym(lat, L // 2)])
+              for k in range(L)]
+    hams = [J1J2Model(lat, J1=-1.0, J2=-J2, hilbert_space=hs) for hs in spaces]
+    return lat, spaces, hams
+
+
+def test_coupling_sweep_matvec_equals_scaled_products():
+    """Column r of a coupling-scaled block is H_r x_r with H_r = sum_c s_rc H_c:
+    equal to the single-operator scaled product over the same tagged table."""
+    import torch
+    from pyqusolver_amd.batched import CouplingSweep
+    lat, spaces, hams = _j1j2_chain_sectors(12)
+    scales = np.array([[1.0, 0.8], [1.0, 1.0], [1.0, 1.2], [0.5, 0.0]])
+    S = CouplingSweep(hams[:5], scales, max_steps=40)
+    rng = np.random.default_rng(2)
+    xs = []
+    for h in S.hams:
+        v = rng.standard_normal(h.nh) + (1j * rng.standard_normal(h.nh) if S.dtype == 1 else 0.0)
+        xs.append(v)
+    y = S.split(S.matvec(S.concat(xs)))
+    dt = torch.complex128 if S.dtype == 1 else torch.float64
+    for p, h in enumerate(S.sectors):
+        for r in range(len(scales)):
+            v = p * len(scales) + r
+            ref = h.matvec_scaled_device(scales[r], torch.as_tensor(xs[v], dtype=dt, device="cuda"))
+            assert torch.allclose(y[v], ref, rtol=1e-13, atol=1e-13 * ref.abs().max().item()), (p, r)
+    S.close()
+    assert not any(h.assembly["couplings"] for h in S.sectors)
+
+
+def test_coupling_sweep_ground_states_against_oracle():
+    """All 12 momentum sectors of the J1-J2 chain L=12 (Sz=0) at J2 in
+    {0.3, 0.5, 0.7} in one sweep (complex sectors included): E0 of every
+    (k, J2) within 1e-10 of the dense reduced matrix of the oracle."""
+    from pyqusolver_amd.batched import CouplingSweep
+    L = 12
+    lat, spaces, hams = _j1j2_chain_sectors(L, J2=0.5)
+    j2s = [0.3, 0.5, 0.7]
+    scales = np.array([[1.0, j / 0.5] for j in j2s])
+    S = CouplingSweep(hams, scales, max_steps=200)
+    res = S.ground_states(tol=1e-12)
+    e = res.e0.reshape(L, len(j2s))
+    olat = O.OLattice("chain", L)
+    for k in range(L):
+        grp = O.Group(olat, [O.Generator("tx", k)], n_up=L // 2)
+        b = O.scan_basis(grp)
+        for r, j2 in enumerate(j2s):
+            c, s, co = O.terms_j1j2(olat, -1.0, -j2).arrays()
+            ref = np.linalg.eigvalsh(O.reduced_dense(b, L, c, s, co, True))[0]
+            assert abs(e[k, r] - ref) <= 1e-10 * abs(ref), (k, j2, e[k, r], ref)
+    # k and -k sectors agree (time reversal)
+    for k in range(1, L // 2):
+        np.testing.assert_allclose(e[k], e[L - k], rtol=1e-11)
+    S.close()
+
+
+def test_coupling_sweep_square_against_separate_solves():
+    """4x4 J1-J2 square, four momentum sectors x three J2 values in one
+    sweep, against a separate Lanczos solve of a model built at each J2."""
+    from pyqusolver_amd import SquareLattice, get_u1_sym
+    from pyqusolver_amd.batched import CouplingSweep
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import J1J2Model
+    lat = SquareLattice(4, 4)
+    ks = [(0, 0), (1, 0), (2, 2), (1, 3)]
+    spaces = [HilbertSpace(lattice=lat, sym_gen={"translation": {"kx": kx, "ky": ky}},
+                           global_syms=[get_u1_sym(lat, 8)]) for kx, ky in ks]
+    hams = [J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=hs) for hs in spaces]
+    j2s = [0.4, 0.5, 0.6]
+    S = CouplingSweep(hams, [[1.0, j / 0.5] for j in j2s], max_steps=300)
+    e = S.ground_states(tol=1e-11).e0.reshape(len(ks), len(j2s))
+    S.close()
+    for p, hs in enumerate(spaces):
+        for r, j2 in enumerate(j2s):
+            ref = J1J2Model(lat, J1=-1.0, J2=-j2, hilbert_space=hs).diagonalize(k=1, tol=1e-12, maxiter=300)[0]
+            assert abs(e[p, r] - ref) <= 1e-10 * abs(ref), (ks[p], j2)
