@@ -51,12 +51,18 @@ def parse():
     p.add_argument("--quiet", action="store_true")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="also run a plain device copy of MIB MiB (profiler counter calibration)")
-    p.add_argument("--config", default="plain", choices=["plain", "batched", "grad", "large"],
+    p.add_argument("--config", default="plain", choices=["plain", "batched", "grad", "large", "sweep"],
                    help="plain: L=32 J1-J2 chain matvec + Lanczos (default); batched: XXZ L=24 all momentum "
                         "sectors + random-field L=20 realisations, batched vs sector-by-sector; grad: "
                         "Kitaev-Heisenberg honeycomb 24 sites, dE0/dJ and Krylov time-evolution gradients; large: "
-                        "J1-J2 6x6 square k=(0,0) flip+ matvec + Lanczos (no reorthogonalisation)")
+                        "J1-J2 6x6 square k=(0,0) flip+ matvec + Lanczos (no reorthogonalisation); sweep: "
+                        "J1-J2 6x6 square, every momentum sector (flip+) at J2 in {0.4, 0.5, 0.6}, one coupling "
+                        "sweep per sector (several minutes)")
     p.add_argument("--realisations", type=int, default=256)
+    p.add_argument("--sectors", type=int, default=36, help="sweep: momentum sectors to run (of 36)")
+    p.add_argument("--sector-start", type=int, default=0, help="sweep: first sector (row-major (kx, ky) order)")
+    p.add_argument("--baseline-sectors", type=int, default=1,
+                   help="sweep: sectors also solved once per J2 value (the sequential baseline)")
     return p.parse_args()
 
 
@@ -199,6 +205,83 @@ def bench_grad(a):
                    "seconds_autograd": t_l, "seconds_hf": t_hf},
         "krylov_overlap": {"t": 1.0, "m": 30, "value": f_val, "autograd": g_k.tolist(),
                            "central_difference_1e-5": g_fd_k.tolist(), "max_rel_dev": rel(g_k, g_fd_k)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_sweep(a):
+    """J1-J2 6x6 square, Sz=0, spin flip +1, momentum sectors (kx, ky) in
+    row-major order, each at J2 in {0.4, 0.5, 0.6}: one coupling sweep per
+    sector (its coupling-tagged table built once, streamed once per step for
+    the three J2 columns; E0 of each column to the 1e-10 residual test).
+    The sequential baseline solves the first ``--baseline-sectors`` sectors
+    once per J2 value with an assembled operator each, the way a caller of
+    diagonalize would.  value = (sector, J2) ground states per second."""
+    import torch
+    from pyqusolver_amd import SquareLattice, get_u1_sym
+    from pyqusolver_amd.batched import CouplingSweep
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import J1J2Model
+    lat = SquareLattice(6, 6)
+    j2s = [0.4, 0.5, 0.6]
+    scales = [[1.0, j / 0.5] for j in j2s]
+    ks = [(kx, ky) for kx in range(6) for ky in range(6)][a.sector_start: a.sector_start + max(1, a.sectors)]
+    results, t_total, t_tables, steps_max = [], 0.0, 0.0, 0
+    for kx, ky in ks:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hs = HilbertSpace(lattice=lat, sym_gen={"translation": {"kx": kx, "ky": ky}, "parity": {"x": 1}},
+                          global_syms=[get_u1_sym(lat, 18)])
+        H = J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=hs)
+        n_sec = hs.nh
+        t1 = time.perf_counter()
+        S = CouplingSweep([H], scales, max_steps=a.lanczos_steps, max_gb=160.0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        res = S.ground_states(tol=1e-10)
+        S.close()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        del H, hs
+        t_total += t3 - t0
+        t_tables += t2 - t1
+        steps_max = max(steps_max, int(res.steps.max()))
+        results.append({"k": [kx, ky], "N": int(n_sec), "e0": [float(v) for v in res.e0],
+                        "steps": [int(v) for v in res.steps], "seconds": round(t3 - t0, 3)})
+        log(f"sector ({kx},{ky}): E0 {' '.join(f'{v:.12f}' for v in res.e0)} steps {list(res.steps)} "
+            f"{t3 - t0:.2f}s (table {t2 - t1:.2f}s, solve {t3 - t2:.2f}s)", a.quiet)
+    base = None
+    nb = min(a.baseline_sectors, len(ks))
+    if nb > 0:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev = 0.0
+        for i, (kx, ky) in enumerate(ks[:nb]):
+            hs = HilbertSpace(lattice=lat, sym_gen={"translation": {"kx": kx, "ky": ky}, "parity": {"x": 1}},
+                              global_syms=[get_u1_sym(lat, 18)])
+            for r, j2 in enumerate(j2s):
+                H = J1J2Model(lat, J1=-1.0, J2=-j2, hilbert_space=hs)
+                e = H.diagonalize(method="lanczos", k=1, tol=1e-10, maxiter=a.lanczos_steps, seed=i, reorth=False,
+                                  store_basis=False, assemble=True)[0]
+                dev = max(dev, abs(e - results[i]["e0"][r]) / abs(e))
+                del H
+            del hs
+        torch.cuda.synchronize()
+        t_seq = time.perf_counter() - t0
+        t_sweep = sum(r["seconds"] for r in results[:nb])
+        base = {"sectors": nb, "seconds_sequential": round(t_seq, 3), "seconds_sweep": round(t_sweep, 3),
+                "speedup": t_seq / t_sweep, "max_rel_dev_sweep_vs_sequential": dev}
+        log(f"sequential ({nb} sector(s) x 3 J2): {t_seq:.2f}s vs sweep {t_sweep:.2f}s", a.quiet)
+    n_gs = len(ks) * len(j2s)
+    out = {
+        "metric": "H·v matvecs/s + HBM GB/s frac; time-to-E0 at 1e-10 (batched/plain)",
+        "value": n_gs / t_total, "unit": "ground states/s ((sector, J2) pairs, 6x6 J1-J2, E0 at 1e-10)",
+        "n_gpus": 1, "steps": steps_max, "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+c128", "data": "synthetic",
+        "config": {"workload": "J1-J2 square 6x6 PBC J1=1 Sz=0 flip+, momentum sectors x J2 in {0.4, 0.5, 0.6}, "
+                               "one coupling sweep per sector", "sectors": len(ks), "first_sector": a.sector_start},
+        "seconds_total": t_total, "seconds_tables": t_tables, "sectors": results, "sequential_baseline": base,
+        "stopping": "per (sector, J2): Ritz residual |beta_m s_m| <= 1e-10 max(1,|E0|), checked every 10 steps",
     }
     print(json.dumps(out), flush=True)
 
@@ -434,6 +517,8 @@ def main():
         return bench_batched(a)
     if a.config == "grad":
         return bench_grad(a)
+    if a.config == "sweep":
+        return bench_sweep(a)
     return bench_plain(a)
 
 

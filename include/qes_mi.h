@@ -288,13 +288,14 @@ int qes_batch_create_blocks(const qes_op_t* ops, int32_t nprob, const qes_op_t* 
                             const void* const* block_diag_dev, int32_t nblock, int32_t max_steps, int32_t dtype,
                             qes_batch_t* out);
 /*
- * Coupling sweep: problem i is block_ops-style block of ncol columns over
+ * Coupling sweep: problem i is a block of ncol (1..4) columns over
  * ops[i], whose operator must be assembled with coupling tags
  * (qes_op_assemble mode 3); column r is its own Lanczos run on
  * H_r = sum_c scales[(i*ncol + r)*8 + c] H_c (c < qes_op_n_couplings; the
  * stride is 8 whatever the coupling count).  The tagged table is streamed
- * once per step for all columns and every column has its own scaled value
- * dictionary and diagonal.  Replaces one diagonalize() per coupling set
+ * once per step for all columns (one lane per row, the row's columns in
+ * registers) and every column has its own scaled value dictionary and
+ * diagonal.  Replaces one diagonalize() per coupling set
  * (e.g. J2 in {0.4, 0.5, 0.6}) of the same sector.  The operators' tables
  * must outlive the batch.  Vectors are n x ncol row-major per problem.
  */

@@ -245,7 +245,7 @@ class CouplingSweep(BatchedLanczos):
     coupling-tagged table is streamed once per step for all columns.
 
     ``scales``: (ncol, n_couplings) shared by all sectors, or (nprob, ncol,
-    n_couplings).  Results (``ground_states``) are in the order
+    n_couplings), ncol <= 4.  Results (``ground_states``) are in the order
     sector 0 columns 0..ncol-1, sector 1, ... (``e0.reshape(nprob, ncol)``).
     Tagged tables are built here when missing and freed by ``close``."""
 
@@ -262,6 +262,8 @@ class CouplingSweep(BatchedLanczos):
         if sc.ndim != 3 or sc.shape[0] != len(hams):
             raise ValueError("scales must be (ncol, n_couplings) or (nprob, ncol, n_couplings)")
         ncol = sc.shape[1]
+        if not 1 <= ncol <= 4:
+            raise ValueError("a coupling sweep runs 1..4 coupling sets per sector")
         cplx = any(h.needs_complex for h in hams) if complex_ is None else bool(complex_)
         self.dtype = 1 if cplx else 0
         self.max_steps = int(max_steps)

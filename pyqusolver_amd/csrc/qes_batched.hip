@@ -265,17 +265,18 @@ __device__ __forceinline__ void block_row_table(const QesProb& P, int64_t a, int
 }
 
 // Row of a coupling-scaled block problem from the operator's coupling-tagged
-// table: every slot is read once for all R columns (the lanes of a row read
-// the same entry) and the column's own scaled dictionary gives the value;
-// the diagonal of column r is sum_c cscale[r][c] diag_c[a].  Slots are taken
-// four at a time as in sell_row_t.
-template <int FMT, bool CPLX, bool CV>
-__device__ __forceinline__ void block_row_sell(const QesProb& P, int64_t a, int c0, int Rp, unsigned cm,
-                                               const double* __restrict__ x, double* __restrict__ y) {
+// table, one lane per row (the wavefront reads 64 consecutive slots of a
+// slice, as the single-operator product does) and the row's RC <= 4 columns
+// in registers: every slot is read once for all columns, its neighbour's RC
+// values are contiguous, and column r takes its value from its own scaled
+// dictionary; the diagonal of column r is sum_c cscale[r][c] diag_c[a].
+// Slots are taken four at a time as in sell_row_t.
+template <int FMT, bool CPLX, bool CV, int RC>
+__device__ __forceinline__ void block_row_sell(const QesProb& P, int64_t a, unsigned cm, const double* __restrict__ x,
+                                               double* __restrict__ y) {
   constexpr int VW = CV ? 2 : 1;
   const AsmTable& T = P.T;
   const int cw = CPLX ? 2 : 1;
-  const int R = P.nrhs;
   const double* xp0 = x + P.off * cw;
   double* yp0 = y + P.off * cw;
   double dcr[ASM_MAX_CPL], dci[ASM_MAX_CPL];
@@ -288,30 +289,26 @@ __device__ __forceinline__ void block_row_sell(const QesProb& P, int64_t a, int 
       if (CV) dci[c] = dc[1];
     }
   }
-  double ar[MAX_RHS / 64], ai[MAX_RHS / 64];
+  double ar[RC], ai[RC];
 #pragma unroll
-  for (int k = 0; k < MAX_RHS / 64; ++k) {
-    const int col = c0 + Rp * k;
-    ar[k] = 0.0;
-    ai[k] = 0.0;
-    if ((cm >> k) & 1) {
-      double dr = 0.0, di = 0.0;
+  for (int r = 0; r < RC; ++r) {
+    double dr = 0.0, di = 0.0;
 #pragma unroll
-      for (int c = 0; c < ASM_MAX_CPL; ++c) {
-        if (c < T.ncpl) {
-          const double s = P.cscale[col * ASM_MAX_CPL + c];
-          dr += s * dcr[c];
-          if (CV) di += s * dci[c];
-        }
+    for (int c = 0; c < ASM_MAX_CPL; ++c) {
+      if (c < T.ncpl) {
+        const double s = P.cscale[r * ASM_MAX_CPL + c];
+        dr += s * dcr[c];
+        if (CV) di += s * dci[c];
       }
-      const int64_t e = a * R + col;
-      if (CPLX) {
-        const double xr = xp0[e * 2], xi = xp0[e * 2 + 1];
-        ar[k] = dr * xr - di * xi;
-        ai[k] = dr * xi + di * xr;
-      } else {
-        ar[k] = dr * xp0[e];
-      }
+    }
+    const int64_t e = a * RC + r;
+    if (CPLX) {
+      const double xr = xp0[e * 2], xi = xp0[e * 2 + 1];
+      ar[r] = dr * xr - di * xi;
+      ai[r] = dr * xi + di * xr;
+    } else {
+      ar[r] = dr * xp0[e];
+      ai[r] = 0.0;
     }
   }
   const int64_t s = a / SELL_C;
@@ -331,19 +328,16 @@ __device__ __forceinline__ void block_row_sell(const QesProb& P, int64_t a, int 
   };
   auto acc = [&](int64_t b, uint32_t idx) {
 #pragma unroll
-    for (int k = 0; k < MAX_RHS / 64; ++k) {
-      const int col = c0 + Rp * k;
-      if ((cm >> k) & 1) {
-        const double* dv = P.dcols + ((int64_t)col * P.ndict + idx) * VW;
-        const double vr = dv[0], vi = CV ? dv[1] : 0.0;
-        const int64_t e = b * R + col;
-        if (CPLX) {
-          const double xr = xp0[e * 2], xi = xp0[e * 2 + 1];
-          ar[k] += vr * xr - vi * xi;
-          ai[k] += vr * xi + vi * xr;
-        } else {
-          ar[k] += vr * xp0[e];
-        }
+    for (int r = 0; r < RC; ++r) {
+      const double* dv = P.dcols + ((int64_t)r * P.ndict + idx) * VW;
+      const double vr = dv[0], vi = CV ? dv[1] : 0.0;
+      const int64_t e = b * RC + r;
+      if (CPLX) {
+        const double xr = xp0[e * 2], xi = xp0[e * 2 + 1];
+        ar[r] += vr * xr - vi * xi;
+        ai[r] += vr * xi + vi * xr;
+      } else {
+        ar[r] += vr * xp0[e];
       }
     }
   };
@@ -363,13 +357,23 @@ __device__ __forceinline__ void block_row_sell(const QesProb& P, int64_t a, int 
     acc(b, idx);
   }
 #pragma unroll
-  for (int kk = 0; kk < MAX_RHS / 64; ++kk) {
-    const int col = c0 + Rp * kk;
-    if ((cm >> kk) & 1) {
-      const int64_t e = a * R + col;
-      yp0[e * cw] = ar[kk];
-      if (CPLX) yp0[e * cw + 1] = ai[kk];
+  for (int r = 0; r < RC; ++r) {
+    if ((cm >> r) & 1) {
+      const int64_t e = a * RC + r;
+      yp0[e * cw] = ar[r];
+      if (CPLX) yp0[e * cw + 1] = ai[r];
     }
+  }
+}
+
+template <int FMT, bool CPLX, bool CV>
+__device__ __forceinline__ void block_row_sell_r(const QesProb& P, int64_t a, unsigned cm, const double* __restrict__ x,
+                                                 double* __restrict__ y) {
+  switch (P.nrhs) {
+    case 1: block_row_sell<FMT, CPLX, CV, 1>(P, a, cm, x, y); break;
+    case 2: block_row_sell<FMT, CPLX, CV, 2>(P, a, cm, x, y); break;
+    case 3: block_row_sell<FMT, CPLX, CV, 3>(P, a, cm, x, y); break;
+    default: block_row_sell<FMT, CPLX, CV, 4>(P, a, cm, x, y); break;
   }
 }
 
@@ -407,18 +411,19 @@ __global__ void __launch_bounds__(BRB) batch_matvec_kernel(const QesProb* __rest
       }
       continue;
     }
-    if (P.cscale) {   // coupling-scaled block: stream the tagged table once for all columns
-      const int Rp = block_lanes(P.nrhs);
-      const int c0 = threadIdx.x % Rp;
-      const unsigned cm = block_colmask(P, c0, Rp, vp_active);
+    if (P.cscale) {   // coupling-scaled block: stream the tagged table once for all columns, lane per row
+      unsigned cm = 0;
+      for (int r = 0; r < P.nrhs; ++r)
+        if (!vp_active || vp_active[P.vp0 + r]) cm |= 1u << r;
       const bool cv = P.T.dvc != 0;
-      for (int64_t a = T.row0 + threadIdx.x / Rp; a < (T.row0 + T.rows < P.n ? T.row0 + T.rows : P.n); a += BRB / Rp) {
+      const int64_t a = T.row0 + threadIdx.x;
+      if (a < P.n && a < T.row0 + T.rows) {
         if (P.T.fmt == ASM_P4) {
-          if (cv) block_row_sell<ASM_P4, CPLX, true>(P, a, c0, Rp, cm, x, y);
-          else block_row_sell<ASM_P4, CPLX, false>(P, a, c0, Rp, cm, x, y);
+          if (cv) block_row_sell_r<ASM_P4, CPLX, true>(P, a, cm, x, y);
+          else block_row_sell_r<ASM_P4, CPLX, false>(P, a, cm, x, y);
         } else {
-          if (cv) block_row_sell<ASM_P6, CPLX, true>(P, a, c0, Rp, cm, x, y);
-          else block_row_sell<ASM_P6, CPLX, false>(P, a, c0, Rp, cm, x, y);
+          if (cv) block_row_sell_r<ASM_P6, CPLX, true>(P, a, cm, x, y);
+          else block_row_sell_r<ASM_P6, CPLX, false>(P, a, cm, x, y);
         }
       }
       continue;
@@ -744,7 +749,8 @@ static hipError_t batch_copy(Batch* B, void* dst, const void* src, size_t bytes,
 static int batch_build_tiles(Batch* B) {
   std::vector<std::vector<Tile>> per(B->nprob);
   for (int p = 0; p < B->nprob; ++p) {
-    const int rows_per_tile = B->blk[p] ? std::max(BRB / block_lanes(B->nrhs[p]), B->block_rows) : BRB;
+    const bool lane_rows = !B->blk[p] || B->h_probs[p].cscale;   // one lane per row
+    const int rows_per_tile = lane_rows ? BRB : std::max(BRB / block_lanes(B->nrhs[p]), B->block_rows);
     for (int64_t r = 0; r < B->ops[p]->basis->n; r += rows_per_tile) per[p].push_back(Tile{p, rows_per_tile, r});
   }
   std::vector<Tile> tiles;
@@ -1273,7 +1279,7 @@ int qes_batch_create_scaled(const qes_op_t* ops, int32_t nprob, int32_t ncol, co
                             int32_t dtype, qes_batch_t* out) {
   if (!out || (nprob > 0 && (!ops || !scales))) return fail("null argument");
   if (nprob < 1) return fail("empty batch");
-  if (ncol < 1 || ncol > MAX_RHS) return fail("ncol must be 1..256");
+  if (ncol < 1 || ncol > 4) return fail("ncol must be 1..4 (coupling sets per sector)");
   std::vector<BlockSpec> bl(nprob);
   for (int i = 0; i < nprob; ++i) {
     if (!ops[i]) return fail("null operator in batch");
