@@ -455,7 +455,9 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
         dict_cpl.push_back((uint8_t)std::get<0>(vals[i]));
         slot_idx[std::get<3>(vals[i])] = (uint16_t)i;
       }
-      const int ib = bits_for(nd), cb = bits_for(n);
+      // at least one index bit, so that the column field is narrower than 32
+      // bits and the shift that extracts the index stays defined
+      const int ib = std::max(1, bits_for(nd)), cb = bits_for(n);
       if (ib + cb <= 32) {
         fmt = ASM_P4;
         O.colbits = 32 - ib;
