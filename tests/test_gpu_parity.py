@@ -500,3 +500,26 @@ def test_boundary_flux_kitaev_sector_parity(gens):
     assert abs(e0 - e_ref) <= E0_TOL * abs(e_ref)
     with pytest.raises(NotImplementedError):
         HilbertSpace(lattice=lat, sym_gen={"translation": {"ky": 0}})
+
+
+def test_reference_workflow_checks_tfim_build_and_reduction():
+    """The reference's own workflow checks (tests/algebra/
+    test_custom_hamiltonian_workflows.py): a translation sector is smaller
+    than the full space (L=6, :37-50) and the sparse build equals the dense
+    one for a transverse-field Ising chain (L=4, :53-60; here XXZ with
+    Jxy=0, Jz=1, hx=0.5), also equal to the oracle's full-space matrix."""
+    from pyqusolver_amd import ChainLattice
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import XXZ
+    lat6 = ChainLattice(6)
+    hs6 = HilbertSpace(lattice=lat6, sym_gen={"translation": 0})
+    assert hs6.nh < hs6.nhfull == 64
+    lat = ChainLattice(4)
+    hs = HilbertSpace(lattice=lat)
+    H = XXZ(lat, hilbert_space=hs, jxy=0.0, jz=1.0, hx=0.5)
+    A = H.build()
+    D = H.build(use_numpy=True)
+    np.testing.assert_allclose(A.toarray(), D, atol=1e-10)
+    c, s, co = O.terms_xxz(O.OLattice("chain", 4), 0.0, 1.0, hx=0.5).arrays()
+    ref = O.full_dense_hamiltonian(4, c, s, co, True)
+    np.testing.assert_allclose(D, ref, atol=1e-14)
