@@ -391,6 +391,89 @@ class Hamiltonian:
             return out
         return y
 
+    # ---- overlaps and matrix elements (operator.py:926-1075) --------------
+    @staticmethod
+    def _as_columns(vec, name: str):
+        """(dim, ncol) view of a vector or column block and whether it was a vector."""
+        if isinstance(vec, (list, tuple)):
+            vec = np.asarray(vec)
+        if not hasattr(vec, "ndim"):
+            raise TypeError(f"{name} must be array-like, got {type(vec)!r}.")
+        if vec.ndim == 1:
+            return vec[:, None], True
+        if vec.ndim == 2:
+            return vec, False
+        raise ValueError(f"{name} must be 1D or 2D, got shape {tuple(vec.shape)}.")
+
+    @staticmethod
+    def _column_overlap(left, right, left_is_vec: bool, right_is_vec: bool, pairwise: bool):
+        """<left|right> with the reference's column conventions: pairwise
+        column overlaps when both sides hold several columns (pairwise=True),
+        else the full left^dagger right, squeezed where an input was a vector."""
+        is_t = isinstance(left, torch.Tensor)
+        nl, nr = int(left.shape[1]), int(right.shape[1])
+        if pairwise and nl > 1 and nr > 1:
+            if nl != nr:
+                raise ValueError(f"Pairwise overlap requires equal column counts, got {nl} and {nr}.")
+            return (left.conj() * right).sum(dim=0) if is_t else (left.conj() * right).sum(axis=0)
+        out = left.conj().transpose(0, 1) @ right if is_t else left.conj().T @ right
+        if left_is_vec and right_is_vec:
+            return out[0, 0]
+        if left_is_vec:
+            return out[0, :]
+        if right_is_vec:
+            return out[:, 0]
+        return out
+
+    def overlap(self, vec1, vec2=None, *, pairwise: bool = True):
+        """<vec1|vec2> for vectors or column blocks (operator.py:975-1011)."""
+        if vec2 is None:
+            vec2 = vec1
+        l, lv = self._as_columns(vec1, "vec1")
+        r, rv = self._as_columns(vec2, "vec2")
+        if l.shape[0] != r.shape[0]:
+            raise ValueError(f"State size mismatch: {l.shape[0]} (vec1) vs {r.shape[0]} (vec2).")
+        return self._overlap_cols(l, lv, r, rv, pairwise)
+
+    def expectation_value(self, state, other_state=None, *, hilbert_in=None, hilbert_out=None, pairwise: bool = True,
+                          hilbert=None, **kwargs):
+        """<state|O|other_state> (operator.py:1013-1075): O applied to
+        other_state by the device matvec (projected into ``hilbert_out`` when
+        that is another sector; then ``state`` lives there), overlaps with the
+        reference's column conventions.  numpy in -> numpy out, device tensors
+        -> device tensors."""
+        if other_state is None:
+            other_state = state
+        left, lv = self._as_columns(state, "state")
+        right, rv = self._as_columns(other_state, "other_state")
+        hin = hilbert_in if hilbert_in is not None else hilbert
+        if hilbert_out is None and left.shape[0] != right.shape[0]:
+            raise ValueError(f"State size mismatch: {left.shape[0]} (state) vs {right.shape[0]} (other_state).")
+        if hilbert_out is not None and hilbert_out is not self._hilbert_space:
+            cols = [self.matvec(right[:, j], hilbert_in=hin, hilbert_out=hilbert_out) for j in range(right.shape[1])]
+            operated = (torch.stack(cols, dim=1) if isinstance(cols[0], torch.Tensor) else np.stack(cols, axis=1))
+        else:
+            src = right[:, 0] if rv else right
+            if isinstance(src, torch.Tensor):
+                src = src.contiguous()
+            elif not src.flags.c_contiguous:
+                src = np.ascontiguousarray(src)
+            operated = self.matvec(src, hilbert_in=hin)
+            if operated.ndim == 1:
+                operated = operated[:, None]
+        return self._overlap_cols(left, lv, operated, rv, pairwise)
+
+    def _overlap_cols(self, left, lv, right, rv, pairwise):
+        if isinstance(left, torch.Tensor) != isinstance(right, torch.Tensor):
+            raise TypeError("both sides must be numpy arrays or both torch tensors")
+        if not isinstance(left, torch.Tensor):
+            dt = np.result_type(left.dtype, right.dtype)
+            left, right = left.astype(dt, copy=False), right.astype(dt, copy=False)
+        else:
+            dt = torch.promote_types(left.dtype, right.dtype)
+            left, right = left.to(dt), right.to(dt)
+        return self._column_overlap(left, right, lv, rv, pairwise)
+
     def __matmul__(self, x):
         return self._matvec_any(x)
 

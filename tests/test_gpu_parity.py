@@ -523,3 +523,47 @@ def test_reference_workflow_checks_tfim_build_and_reduction():
     c, s, co = O.terms_xxz(O.OLattice("chain", 4), 0.0, 1.0, hx=0.5).arrays()
     ref = O.full_dense_hamiltonian(4, c, s, co, True)
     np.testing.assert_allclose(D, ref, atol=1e-14)
+
+
+def test_expectation_value_and_overlap():
+    """Operator.expectation_value / overlap (operator.py:926-1075): <x|H|y>
+    against the oracle's reduced matrix (numpy and device tensors), pairwise
+    and full column blocks, E0 as the expectation in the Ritz vector, and a
+    sector-changing matrix element <out|S^z_q|in> against the oracle's
+    projected product."""
+    import torch
+    hs, H, grp, b, t = build_pair("chain", 14, 1, [("tx", 3)], 7, "xxz")
+    c, s, co = t.arrays()
+    Hd = O.reduced_dense(b, 14, c, s, co, True)
+    rng = np.random.default_rng(12)
+    n = hs.nh
+    X = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    Y = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    ev = H.expectation_value(X[:, 0], Y[:, 0])
+    ref = np.vdot(X[:, 0], Hd @ Y[:, 0])
+    assert abs(ev - ref) <= 1e-12 * abs(ref)
+    np.testing.assert_allclose(H.expectation_value(X, Y), np.einsum("ij,ij->j", X.conj(), Hd @ Y), rtol=1e-12)
+    np.testing.assert_allclose(H.expectation_value(X, Y, pairwise=False), X.conj().T @ Hd @ Y, rtol=1e-12)
+    Xt, Yt = torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda()
+    evt = H.expectation_value(Xt[:, 0], Yt[:, 0])
+    assert isinstance(evt, torch.Tensor) and abs(evt.item() - ref) <= 1e-12 * abs(ref)
+    np.testing.assert_allclose(H.overlap(X, Y), np.einsum("ij,ij->j", X.conj(), Y), rtol=1e-13)
+    H.diagonalize(k=1, tol=1e-12, maxiter=300, compute_eigvec=True)
+    psi = H.eig_vec
+    e = (H.expectation_value(psi) / H.overlap(psi)).real.item()
+    e_ref = np.linalg.eigvalsh(Hd)[0]
+    assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    L = 12
+    lat = ChainLattice(L)
+    hs_in = HilbertSpace(lattice=lat, sym_gen={"translation": 1}, global_syms=[get_u1_sym(lat, 6)])
+    hs_out = HilbertSpace(lattice=lat, sym_gen={"translation": 4}, global_syms=[get_u1_sym(lat, 6)])
+    op, codes, sites, coeffs = _sz_q(hs_in, L, 2 * np.pi * 3 / L, 13)
+    xin = rng.standard_normal(hs_in.nh) + 1j * rng.standard_normal(hs_in.nh)
+    xout = rng.standard_normal(hs_out.nh) + 1j * rng.standard_normal(hs_out.nh)
+    me = op.expectation_value(xout, xin, hilbert_out=hs_out)
+    olat = O.OLattice("chain", L)
+    gi, go = O.Group(olat, [O.Generator("tx", 1)], n_up=6), O.Group(olat, [O.Generator("tx", 4)], n_up=6)
+    yo = O.projected_matvec(gi, O.scan_basis(gi), go, O.scan_basis(go), L, codes, sites, coeffs, True, xin)
+    assert abs(me - np.vdot(xout, yo)) <= 1e-12 * abs(np.vdot(xout, yo))

@@ -190,3 +190,26 @@ def test_flux_refusals():
     with pytest.raises(NotImplementedError):
         _grp(hc, {"translation": {"kx": 0, "ky": 0}})
     assert _grp(hc, {"translation": {"kx": 1}}).order == 3
+
+
+def test_overlap_column_conventions():
+    """Operator.overlap follows the reference's column rules
+    (operator.py:926-973): vectors give a scalar, two blocks of equal width
+    give pairwise overlaps (or the full product with pairwise=False), a
+    vector against a block gives a row; numpy and torch alike."""
+    import torch
+    from pyqusolver_amd.operator import Hamiltonian
+    H = Hamiltonian.__new__(Hamiltonian)
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((6, 3)) + 1j * rng.standard_normal((6, 3))
+    b = rng.standard_normal((6, 3))
+    np.testing.assert_allclose(H.overlap(a, b), np.einsum("ij,ij->j", a.conj(), b))
+    np.testing.assert_allclose(H.overlap(a, b, pairwise=False), a.conj().T @ b)
+    assert np.isclose(H.overlap(a[:, 0], b[:, 1]), np.vdot(a[:, 0], b[:, 1]))
+    np.testing.assert_allclose(H.overlap(a[:, 0], b), a[:, 0].conj() @ b)
+    np.testing.assert_allclose(H.overlap(torch.from_numpy(a), torch.from_numpy(b)).numpy(),
+                               np.einsum("ij,ij->j", a.conj(), b))
+    with pytest.raises(ValueError):
+        H.overlap(a, b[:4])
+    with pytest.raises(ValueError):
+        H.overlap(a, rng.standard_normal((6, 2)))
