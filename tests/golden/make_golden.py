@@ -30,6 +30,9 @@ CASES = [
     ("chain12_j1j2field_k4_Px", "chain", 12, 1, [("tx", 4), ("Px", 1)], None, "j1j2_field"),
     ("square4x3_j1j2_k1_2", "square", 4, 3, [("tx", 1), ("ty", 2)], 6, "j1j2"),
     ("honeycomb3x2_hk_k0", "honeycomb", 3, 2, [("tx", 0), ("ty", 0)], None, "hk"),
+    ("square4x4_j1j2_k0_2_Im", "square", 4, 4, [("tx", 0), ("ty", 2), ("I", -1)], 8, "j1j2"),
+    ("honeycomb3x2_hk_k0_Ip", "honeycomb", 3, 2, [("tx", 0), ("ty", 0), ("I", 1)], None, "hk"),
+    ("chain14_j1j2_kpi_Im_Px", "chain", 14, 1, [("tx", 7), ("I", -1), ("Px", 1)], 7, "j1j2"),
 ]
 
 
