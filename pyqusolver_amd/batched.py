@@ -247,7 +247,8 @@ class CouplingSweep(BatchedLanczos):
     ``scales``: (ncol, n_couplings) shared by all sectors, or (nprob, ncol,
     n_couplings), ncol <= 4.  Results (``ground_states``) are in the order
     sector 0 columns 0..ncol-1, sector 1, ... (``e0.reshape(nprob, ncol)``).
-    Tagged tables are built here when missing and freed by ``close``."""
+    Tagged tables are built here when missing (replacing an untagged one)
+    and freed by ``close``."""
 
     MAX_CPL = 8
 

@@ -839,7 +839,7 @@ static int batch_upload_probs(Batch* B) {
     h[p].rowptr = on ? B->t_rowptr[p] : nullptr;
     h[p].col = on ? B->t_col[p] : nullptr;
     h[p].val = on ? B->t_val[p] : nullptr;
-    if (!B->use_tables) h[p].T.fmt = -1;
+    if (!B->use_tables && !h[p].cscale) h[p].T.fmt = -1;   // coupling-scaled blocks always stream their table
   }
   QES_HIP(batch_copy(B, B->d_probs, h.data(), B->nprob * sizeof(QesProb), hipMemcpyHostToDevice));
   return 0;
