@@ -128,9 +128,24 @@ def ritz_residual(alpha, beta):
     return r, (min(r, r * r / gap) if gap > 0 else r)
 
 
+def _k_residuals(alpha, beta, k: int):
+    """Residual norms |beta_m S[m-1, i]| of the k lowest Ritz pairs and their values."""
+    w, S = tridiag_eigs(alpha, beta)
+    m = len(alpha)
+    kk = min(k, m)
+    return w[:kk], np.abs(float(beta[m - 1]) * S[-1, :kk])
+
+
 def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300, v0=None, seed: int = 0,
                          reorth: bool = True, store_basis: Optional[bool] = None, compute_eigvec: bool = False,
                          check_every: int = 10, use_graph: bool = True) -> LanczosResult:
+    """Lowest eigenvalue(s) by device Lanczos.  Stops when the Ritz residual
+    |beta_m s_m| of the lowest pair is <= tol*max(1,|theta|); with k > 1 and
+    the basis kept (reorthogonalised), all k lowest pairs must pass.  A
+    single start vector finds one copy of a degenerate eigenvalue; without
+    reorthogonalisation only the lowest value is reliable.  With
+    ``compute_eigvec`` the Ritz vectors are formed on the device: shape (n,)
+    for k = 1, (n, k) otherwise."""
     n = ham.nh
     complex_ = bool(ham.needs_complex)
     if v0 is None:
@@ -167,18 +182,31 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
             w, _ = tridiag_eigs(a, b)
             converged = True
             break
-        if ritz_residual(a, b)[0] <= tol * scale:
+        if k > 1 and reorth:
+            if len(a) >= k:
+                wk, rk = _k_residuals(a, b, k)
+                if np.all(rk <= tol * np.maximum(1.0, np.abs(wk))):
+                    converged = True
+                    break
+        elif ritz_residual(a, b)[0] <= tol * scale:
             converged = True
             break
     torch.cuda.synchronize()
     w, V = tridiag_eigs(a, b)
-    res_bound = float(abs(b[len(a) - 1] * V[-1, 0])) if len(a) else float("nan")
+    kk = min(k, len(a))
+    res_bound = float(np.max(np.abs(b[len(a) - 1] * V[-1, :kk]))) if len(a) else float("nan")
     vecs = None
     if compute_eigvec:
         if not store_basis:
             raise ValueError("eigenvectors need store_basis=True")
-        vecs = torch.empty((n,), dtype=v0t.dtype, device="cuda")
-        L.ritz(V[:, 0], vecs)
+        if k == 1:
+            vecs = torch.empty((n,), dtype=v0t.dtype, device="cuda")
+            L.ritz(V[:, 0], vecs)
+        else:
+            cols = torch.empty((kk, n), dtype=v0t.dtype, device="cuda")
+            for i in range(kk):
+                L.ritz(V[:, i], cols[i])
+            vecs = cols.transpose(0, 1).contiguous()
     second = L.second_passes if reorth else 0
     L.close()
     return LanczosResult(eigenvalues=w[:k].copy(), alpha=a, beta=b, steps=len(a), converged=converged,

@@ -495,8 +495,10 @@ class Hamiltonian:
                     compute_eigvec: bool = False, check_every: int = 10, assemble: Optional[bool] = None,
                     **kwargs):
         """Lowest eigenvalue(s) by device Lanczos (hamil.py:1809-2114 with
-        method='lanczos').  Only E0 is claimed to the requested tolerance;
-        the other k-1 Ritz values are returned as computed.
+        method='lanczos').  With the basis kept (reorth, the default) all k
+        lowest Ritz pairs are converged to the residual test; without it only
+        E0 is.  One start vector finds one copy of a degenerate level.
+        ``compute_eigvec``: eig_vec is (n,) for k = 1, (n, k) otherwise.
 
         ``assemble``: tabulate H for the solve (see ``assemble``); None picks
         it when the table fits in a quarter of the free device memory (at

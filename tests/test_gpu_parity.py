@@ -567,3 +567,24 @@ def test_expectation_value_and_overlap():
     gi, go = O.Group(olat, [O.Generator("tx", 1)], n_up=6), O.Group(olat, [O.Generator("tx", 4)], n_up=6)
     yo = O.projected_matvec(gi, O.scan_basis(gi), go, O.scan_basis(go), L, codes, sites, coeffs, True, xin)
     assert abs(me - np.vdot(xout, yo)) <= 1e-12 * abs(np.vdot(xout, yo))
+
+
+def test_lowest_k_eigenpairs_with_reorthogonalisation():
+    """diagonalize(k=4) with the basis kept: the four lowest Ritz values of a
+    sector whose lowest levels are non-degenerate match the dense reduced
+    matrix to 1e-10, each has a residual below the tolerance, and the (n, 4)
+    Ritz vectors satisfy H v = E v."""
+    import torch
+    hs, H, grp, b, t = build_pair("chain", 14, 1, [("tx", 0), ("R", 1), ("Px", -1)], 7, "j1j2")
+    c, s, co = t.arrays()
+    ev = np.linalg.eigvalsh(O.reduced_dense(b, 14, c, s, co, True))
+    assert np.min(np.diff(ev[:5])) > 1e-3          # lowest levels non-degenerate here
+    w = H.diagonalize(k=4, tol=1e-11, maxiter=min(300, hs.nh), reorth=True, compute_eigvec=True)
+    np.testing.assert_allclose(w, ev[:4], rtol=0, atol=1e-10 * abs(ev[0]))
+    assert H.last_lanczos.converged and H.last_lanczos.residual_bound <= 1e-11 * max(1.0, abs(ev[3]))
+    V = H.eig_vec
+    assert tuple(V.shape) == (hs.nh, 4)
+    for i in range(4):
+        v = V[:, i].contiguous()
+        r = H.matvec_device(v) - float(w[i]) * v
+        assert torch.linalg.vector_norm(r).item() <= 1e-8 * torch.linalg.vector_norm(v).item()
