@@ -37,3 +37,29 @@ def test_error_path_without_gpu_work():
     L = _lib.lib()
     assert L.qes_basis_create(None, None, None) == -1
     assert "null" in L.qes_last_error().decode()
+
+
+def test_every_entry_point_rejects_null_arguments():
+    """Every entry point of the C ABI called with null handles / pointers and
+    zero sizes returns an error status (destroy functions accept null) and
+    leaves a message, without touching a device."""
+    import ctypes as C
+    from pyqusolver_amd import _lib
+    L = _lib.lib()
+    for name, argtypes in _lib.SIGNATURES.items():
+        if name in ("qes_version", "qes_last_error"):
+            continue
+        args = []
+        for t in argtypes:
+            if t in (C.c_int32, C.c_int64):
+                args.append(0)
+            elif t is C.c_double:
+                args.append(0.0)
+            else:
+                args.append(None)
+        rc = getattr(L, name)(*args)
+        if name.endswith("_destroy"):
+            assert rc == 0, name
+        else:
+            assert rc == -1, name
+            assert L.qes_last_error().decode(), name
