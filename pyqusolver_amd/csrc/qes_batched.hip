@@ -629,60 +629,104 @@ __global__ void bset_beta_kernel(int nvp, const int* __restrict__ active, const 
   }
 }
 
-// element-wise step over a chunk: fn(e, vp) for every element e (in doubles
-// units handled by the caller) whose virtual problem vp is active
-template <bool CPLX, class F>
-__device__ __forceinline__ void for_chunk(const Chunk& K, const int* __restrict__ active, F&& fn) {
-  const int cw = CPLX ? 2 : 1;
-  if (K.nrhs > 1) {
-    const int Rq = block_cols(K.nrhs);
-    const int col = threadIdx.x % Rq;
-    if (col >= K.nrhs) return;
-    const int vp = K.vp0 + col;
-    if (active && !active[vp]) return;
-    for (int i = threadIdx.x / Rq; i < K.rows; i += BRB / Rq) {
-      const int64_t e = K.start + (int64_t)i * K.nrhs + col;
-      fn(e * cw, vp);
-      if (CPLX) fn(e * cw + 1, vp);
-    }
-  } else {
-    if (active && !active[K.vp0]) return;
-    for (int i = threadIdx.x; i < K.rows * cw; i += BRB) fn(K.start * cw + i, K.vp0);
-  }
-}
-
+// w <- w - alpha v - beta v_prev for every active problem, fused with the
+// partial sums of |w|^2 (the slots bdot_kernel would write for <w, w>), so
+// the norm needs no second pass over w.  The coefficients are loaded once
+// per chunk (ordinary problems) or once per column (block problems).
 template <bool CPLX>
-__global__ void __launch_bounds__(BRB) brecur_kernel(const Chunk* __restrict__ chunks, int nchunks,
-                                                     const int* __restrict__ active, double* __restrict__ w,
-                                                     const double* __restrict__ v, const double* __restrict__ vp,
-                                                     const double* __restrict__ alpha, const double* __restrict__ beta,
-                                                     const int* __restrict__ jstep, int maxs) {
+__global__ void __launch_bounds__(BRB) brecur_norm_kernel(const Chunk* __restrict__ chunks, int nchunks,
+                                                          const int* __restrict__ active, double* __restrict__ w,
+                                                          const double* __restrict__ v, const double* __restrict__ vp,
+                                                          const double* __restrict__ alpha,
+                                                          const double* __restrict__ beta,
+                                                          const int* __restrict__ jstep, int maxs,
+                                                          double* __restrict__ part) {
+  __shared__ double sr[BRB];
+  const int cw = CPLX ? 2 : 1;
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const Chunk K = chunks[c];
-    for_chunk<CPLX>(K, active, [&](int64_t e, int p) {
+    const bool blk = K.nrhs > 1;
+    const int Rq = blk ? block_cols(K.nrhs) : 1;
+    const int col = blk ? (int)(threadIdx.x % Rq) : 0;
+    const int p = K.vp0 + col;
+    double r = 0.0;
+    if (col < K.nrhs && (!active || active[p])) {
       const int j = jstep[p];
-      if (j >= maxs) return;   // step budget of this problem exhausted
-      const double al = alpha[(int64_t)p * maxs + j];
-      const double be = j > 0 ? beta[(int64_t)p * maxs + j - 1] : 0.0;
-      w[e] = w[e] - al * v[e] - be * vp[e];
-    });
+      if (j < maxs) {
+        const double al = alpha[(int64_t)p * maxs + j];
+        const double be = j > 0 ? beta[(int64_t)p * maxs + j - 1] : 0.0;
+        if (blk) {
+          for (int i = threadIdx.x / Rq; i < K.rows; i += BRB / Rq) {
+            const int64_t e = (K.start + (int64_t)i * K.nrhs + col) * cw;
+#pragma unroll
+            for (int q = 0; q < cw; ++q) {
+              const double t = w[e + q] - al * v[e + q] - be * vp[e + q];
+              w[e + q] = t;
+              r += t * t;
+            }
+          }
+        } else {
+          for (int i = threadIdx.x; i < K.rows * cw; i += BRB) {
+            const int64_t e = K.start * cw + i;
+            const double t = w[e] - al * v[e] - be * vp[e];
+            w[e] = t;
+            r += t * t;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    sr[threadIdx.x] = r;
+    __syncthreads();
+    const int stop = blk ? Rq : 1;
+    for (int off = BRB / 2; off >= stop; off >>= 1) {
+      if (threadIdx.x < (unsigned)off) sr[threadIdx.x] += sr[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (blk) {
+      if ((int)threadIdx.x < K.nrhs) {
+        part[2 * (K.slot + threadIdx.x * K.sstride)] = sr[threadIdx.x];
+        part[2 * (K.slot + threadIdx.x * K.sstride) + 1] = 0.0;
+      }
+    } else if (threadIdx.x == 0) {
+      part[2 * K.slot] = sr[0];
+      part[2 * K.slot + 1] = 0.0;
+    }
   }
 }
 
-// v_prev <- v ; v <- w / scal
+// v_prev <- v ; v <- w / scal (the scale loaded once per chunk or column)
 template <bool CPLX>
 __global__ void __launch_bounds__(BRB) badvance_kernel(const Chunk* __restrict__ chunks, int nchunks,
                                                        const int* __restrict__ active, const double* __restrict__ w,
                                                        double* __restrict__ v, double* __restrict__ vp,
                                                        const double* __restrict__ scal, int first) {
+  const int cw = CPLX ? 2 : 1;
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const Chunk K = chunks[c];
-    for_chunk<CPLX>(K, active, [&](int64_t e, int p) {
-      const double be = scal[2 * p];
-      const double inv = be > 0.0 ? 1.0 / be : 0.0;
-      vp[e] = first ? 0.0 : v[e];
-      v[e] = w[e] * inv;
-    });
+    const bool blk = K.nrhs > 1;
+    const int Rq = blk ? block_cols(K.nrhs) : 1;
+    const int col = blk ? (int)(threadIdx.x % Rq) : 0;
+    const int p = K.vp0 + col;
+    if (col >= K.nrhs || (active && !active[p])) continue;
+    const double be = scal[2 * p];
+    const double inv = be > 0.0 ? 1.0 / be : 0.0;
+    if (blk) {
+      for (int i = threadIdx.x / Rq; i < K.rows; i += BRB / Rq) {
+        const int64_t e = (K.start + (int64_t)i * K.nrhs + col) * cw;
+#pragma unroll
+        for (int q = 0; q < cw; ++q) {
+          vp[e + q] = first ? 0.0 : v[e + q];
+          v[e + q] = w[e + q] * inv;
+        }
+      }
+    } else {
+      for (int i = threadIdx.x; i < K.rows * cw; i += BRB) {
+        const int64_t e = K.start * cw + i;
+        vp[e] = first ? 0.0 : v[e];
+        v[e] = w[e] * inv;
+      }
+    }
   }
 }
 
@@ -1093,12 +1137,13 @@ static int enqueue_bstep(Batch* B, hipStream_t st) {
   hipLaunchKernelGGL(bset_alpha_kernel, dim3(pg), dim3(256), 0, st, B->nvp, B->d_active, B->scal, B->alpha, B->d_j,
                      B->maxs);
   if (c)
-    hipLaunchKernelGGL(brecur_kernel<true>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks, B->d_active,
-                       B->w, B->v, B->vp, B->alpha, B->beta, B->d_j, B->maxs);
+    hipLaunchKernelGGL(brecur_norm_kernel<true>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks,
+                       B->d_active, B->w, B->v, B->vp, B->alpha, B->beta, B->d_j, B->maxs, B->part);
   else
-    hipLaunchKernelGGL(brecur_kernel<false>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks,
-                       B->d_active, B->w, B->v, B->vp, B->alpha, B->beta, B->d_j, B->maxs);
-  enqueue_bdot(B, B->w, B->w, 1, B->d_active, st);
+    hipLaunchKernelGGL(brecur_norm_kernel<false>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks,
+                       B->d_active, B->w, B->v, B->vp, B->alpha, B->beta, B->d_j, B->maxs, B->part);
+  hipLaunchKernelGGL(bfinish_kernel, dim3(std::min(B->nvp, 4096)), dim3(BRB), 0, st, B->d_vslot0, B->d_vnslot, B->nvp,
+                     B->d_active, B->part, B->scal, 1);
   if (c)
     hipLaunchKernelGGL(badvance_kernel<true>, dim3(B->grid_vec), dim3(BRB), 0, st, B->d_chunks, B->nchunks,
                        B->d_active, B->w, B->v, B->vp, B->scal, 0);
