@@ -182,7 +182,7 @@ def test_chain16_e0_against_full_space_sparse():
     assert abs(e_sector - (-7.14229636061679)) < 1e-11
 
 
-def test_survey_sector_dimensions_closed_form():
+def test_workload_sector_dimensions_closed_form():
     """Closed-form sector dimensions of the benchmark configurations."""
     def dim(kind, lx, ly, gens, n_up):
         return O.sector_dimension(O.Group(O.OLattice(kind, lx, ly), [O.Generator(k, s) for k, s in gens], n_up=n_up))
