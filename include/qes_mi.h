@@ -146,8 +146,9 @@ int qes_op_diagonal(qes_op_t op, void* re_dev, void* im_dev, void* stream);
 /*
  * Assembled operator: tabulate every row of H once on the device (sliced
  * ELLPACK: 64 rows per slice, column-major inside a slice), after which
- * qes_matvec with nbatch == 1 streams the table instead of canonicalising and
- * looking up neighbours.  The device counterpart of the reference's explicit
+ * qes_matvec streams the table instead of canonicalising and looking up
+ * neighbours (several right-hand sides: four columns per pass over the
+ * table, each column bitwise equal to its single-vector product).  The device counterpart of the reference's explicit
  * sparse build (Algebra/Hilbert/matrix_builder.py:151-206, :911-1030), for
  * repeated application (Lanczos).  mode 0 frees it; 1 picks the format:
  * operators whose off-diagonal values fit a dictionary (< 2^16 distinct real

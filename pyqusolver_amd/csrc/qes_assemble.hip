@@ -253,6 +253,87 @@ __global__ void __launch_bounds__(256) sell_matvec_kernel(AsmTable T, const doub
   }
 }
 
+// Y[:, c0:c0+RC] = A X[:, c0:c0+RC] for (n, nb) row-major X, Y: each slot is
+// read once for the RC columns, whose values at the neighbour are contiguous;
+// per column the sums run in the order of the single-vector product, so each
+// column equals it bit for bit.
+template <int FMT, bool XC, bool CV, int RC>
+__global__ void __launch_bounds__(256) sell_matvec_multi_kernel(AsmTable T, const double* __restrict__ x,
+                                                                double* __restrict__ y, int64_t n, int nb, int c0,
+                                                                int accumulate) {
+  constexpr int VW = CV ? 2 : 1;
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int cw = XC ? 2 : 1;
+  double ar[RC], ai[RC];
+  if (FMT != ASM_PLAIN) {
+    const double dr = T.diag[a * VW], di = CV ? T.diag[a * VW + 1] : 0.0;
+#pragma unroll
+    for (int r = 0; r < RC; ++r) {
+      const int64_t e = (a * nb + c0 + r) * cw;
+      if (XC) {
+        const double xr = x[e], xi = x[e + 1];
+        ar[r] = dr * xr - di * xi;
+        ai[r] = dr * xi + di * xr;
+      } else {
+        ar[r] = dr * x[e];
+        ai[r] = 0.0;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < RC; ++r) ar[r] = ai[r] = 0.0;
+  }
+  const int64_t s = a / SELL_C;
+  const int l = (int)(a % SELL_C);
+  const int64_t base = T.off[s] + l;
+  const int len = T.len[s];
+  const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
+  for (int k = 0; k < len; ++k) {
+    const int64_t e = base + (int64_t)k * SELL_C;
+    int64_t b;
+    double vr, vi = 0.0;
+    if (FMT == ASM_P4) {
+      const uint32_t p = T.packed[e];
+      b = p & cmask;
+      const uint32_t idx = p >> T.colbits;
+      vr = T.dict[idx * VW];
+      if (CV) vi = T.dict[idx * VW + 1];
+    } else if (FMT == ASM_P6) {
+      b = T.col[e];
+      const uint32_t idx = T.vidx[e];
+      vr = T.dict[idx * VW];
+      if (CV) vi = T.dict[idx * VW + 1];
+    } else {
+      b = T.col[e];
+      vr = T.val[e * VW];
+      if (CV) vi = T.val[e * VW + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < RC; ++r) {
+      const int64_t f = (b * nb + c0 + r) * cw;
+      if (XC) {
+        const double xr = x[f], xi = x[f + 1];
+        ar[r] += vr * xr - vi * xi;
+        ai[r] += vr * xi + vi * xr;
+      } else {
+        ar[r] += vr * x[f];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RC; ++r) {
+    const int64_t e = (a * nb + c0 + r) * cw;
+    if (accumulate) {
+      y[e] += ar[r];
+      if (XC) y[e + 1] += ai[r];
+    } else {
+      y[e] = ar[r];
+      if (XC) y[e + 1] = ai[r];
+    }
+  }
+}
+
 // d_c = sum_a conj(g_a) (H_c v)_a for every coupling c of a tagged table, in
 // one pass: per row, the table entries are split by the coupling of their
 // dictionary value; per block partial sums part[block][c][re, im].
@@ -657,6 +738,52 @@ int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int ac
   const bool xc = dtype == 1;
   if ((op->asm_vc || op->asm_dvc) && !xc) return fail("complex matrix elements need complex128 vectors");
   return launch_sell(asm_table_of(op), x, y, n, xc, accumulate, st);
+}
+
+template <bool XC, int FMT, bool CV>
+static void launch_multi_fmt(const AsmTable& T, const double* x, double* y, int64_t n, int nb, int accumulate,
+                             hipStream_t st) {
+  const dim3 grid((unsigned)((n + 255) / 256));
+  for (int c0 = 0; c0 < nb; c0 += 4) {
+    const int rc = std::min(4, nb - c0);
+#define QES_MULTI(R) \
+  hipLaunchKernelGGL((sell_matvec_multi_kernel<FMT, XC, CV, R>), grid, dim3(256), 0, st, T, x, y, n, nb, c0, accumulate)
+    if (rc == 4) QES_MULTI(4);
+    else if (rc == 3) QES_MULTI(3);
+    else if (rc == 2) QES_MULTI(2);
+    else QES_MULTI(1);
+#undef QES_MULTI
+  }
+}
+
+template <bool XC>
+static void launch_multi_x(const AsmTable& T, const double* x, double* y, int64_t n, int nb, int accumulate,
+                           hipStream_t st) {
+  const bool cv = T.fmt == ASM_PLAIN ? T.vc != 0 : T.dvc != 0;
+  if (T.fmt == ASM_P4) {
+    if (cv) launch_multi_fmt<XC, ASM_P4, true>(T, x, y, n, nb, accumulate, st);
+    else launch_multi_fmt<XC, ASM_P4, false>(T, x, y, n, nb, accumulate, st);
+  } else if (T.fmt == ASM_P6) {
+    if (cv) launch_multi_fmt<XC, ASM_P6, true>(T, x, y, n, nb, accumulate, st);
+    else launch_multi_fmt<XC, ASM_P6, false>(T, x, y, n, nb, accumulate, st);
+  } else {
+    if (cv) launch_multi_fmt<XC, ASM_PLAIN, true>(T, x, y, n, nb, accumulate, st);
+    else launch_multi_fmt<XC, ASM_PLAIN, false>(T, x, y, n, nb, accumulate, st);
+  }
+}
+
+int assembled_matvec_multi(const Op* op, const double* x, double* y, int64_t nbatch, int dtype, int accumulate,
+                           hipStream_t st) {
+  const int64_t n = op->basis->n;
+  if (n == 0) return 0;
+  if (nbatch > (int64_t(1) << 30)) return fail("nbatch too large");
+  const bool xc = dtype == 1;
+  if ((op->asm_vc || op->asm_dvc) && !xc) return fail("complex matrix elements need complex128 vectors");
+  const AsmTable T = asm_table_of(op);
+  if (xc) launch_multi_x<true>(T, x, y, n, (int)nbatch, accumulate, st);
+  else launch_multi_x<false>(T, x, y, n, (int)nbatch, accumulate, st);
+  QES_HIP(hipGetLastError());
+  return 0;
 }
 
 int assembled_matvec_scaled(Op* op, const double* scale, int nscale, const double* x, double* y, int dtype,

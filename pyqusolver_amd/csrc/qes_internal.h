@@ -126,6 +126,9 @@ int build_plan(Op* op, int canon_override);   // -1: automatic
 int op_assemble(Op* op, int enable, int want_dict, hipStream_t st);
 void free_assembly(Op* op);
 int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int accumulate, hipStream_t st);
+// (n, nbatch) row-major vectors, columns in groups of four per table pass
+int assembled_matvec_multi(const Op* op, const double* x, double* y, int64_t nbatch, int dtype, int accumulate,
+                           hipStream_t st);
 int assembled_coo(const Op* op, int32_t* rows, int32_t* cols, double* vals, int64_t capacity, hipStream_t st);
 // coupling-tagged table: y = sum_c scale[c] H_c x; out_host[2c..2c+1] = <g, H_c v>
 int assembled_matvec_scaled(Op* op, const double* scale, int nscale, const double* x, double* y, int dtype,

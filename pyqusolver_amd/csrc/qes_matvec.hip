@@ -414,6 +414,7 @@ int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int 
   if (nbatch < 1) return fail("nbatch must be >= 1");
   if (op->asm_fmt >= 0 && nbatch == 1)
     return assembled_matvec(op, (const double*)x, (double*)y, dtype, accumulate, st);
+  if (op->asm_fmt >= 0) return assembled_matvec_multi(op, (const double*)x, (double*)y, nbatch, dtype, accumulate, st);
   const bool w32 = B->group.ns <= 32;
   const double* xd = (const double*)x;
   double* yd = (double*)y;
