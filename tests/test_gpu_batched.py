@@ -260,3 +260,26 @@ def test_coupling_sweep_square_against_separate_solves():
         for r, j2 in enumerate(j2s):
             ref = J1J2Model(lat, J1=-1.0, J2=-j2, hilbert_space=hs).diagonalize(k=1, tol=1e-12, maxiter=300)[0]
             assert abs(e[p, r] - ref) <= 1e-10 * abs(ref), (ks[p], j2)
+
+
+def test_coupling_sweep_field_scan_against_separate_models():
+    """A field scan through the coupling sweep: XXZ L=10 (no U(1)) with
+    Jxy, Jz and a uniform field hz tagged separately, four hz values per
+    sector and Jz rescaled in one of them, against models built at each value."""
+    from pyqusolver_amd import ChainLattice
+    from pyqusolver_amd.batched import CouplingSweep
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import XXZ
+    lat = ChainLattice(10)
+    spaces = [HilbertSpace(lattice=lat, sym_gen={"translation": k}) for k in (0, 3, 5)]
+    hams = [XXZ(lat, hilbert_space=hs, jxy=1.0, jz=0.7, hz=0.3) for hs in spaces]
+    assert hams[0].n_couplings == 4          # jxy, jz, hx (absent: no instructions), hz
+    sets = [(0.7, 0.0), (0.7, 0.3), (0.7, 1.1), (1.2, 0.5)]
+    scales = [[1.0, jz / 0.7, 1.0, hz / 0.3] for jz, hz in sets]
+    S = CouplingSweep(hams, scales, max_steps=300)
+    e = S.ground_states(tol=1e-11).e0.reshape(len(hams), len(sets))
+    S.close()
+    for p, hs in enumerate(spaces):
+        for r, (jz, hz) in enumerate(sets):
+            ref = XXZ(lat, hilbert_space=hs, jxy=1.0, jz=jz, hz=hz).diagonalize(k=1, tol=1e-12, maxiter=300)[0]
+            assert abs(e[p, r] - ref) <= 1e-10 * max(1.0, abs(ref)), (p, jz, hz)
