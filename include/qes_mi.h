@@ -194,6 +194,12 @@ int qes_op_table_coo(qes_op_t op, int32_t* rows_dev, int32_t* cols_dev, double* 
 
 /* Number of off-diagonal gathers of one matvec (E_off) -- stats pass. */
 int qes_op_count_offdiag(qes_op_t op, int64_t* e_off);
+/* Debug pass: nonzero off-diagonal elements whose target lies outside the
+ * sector (another U(1) / Z2 filling, or a representative of zero norm) --
+ * the targets the reference's kernel skips silently
+ * (Algebra/Hilbert/matrix_builder.py:1366-1371); nonzero for an operator
+ * that does not conserve the sector's quantum numbers. */
+int qes_op_count_outside(qes_op_t op, int64_t* outside);
 
 /*
  * Tuning and diagnostics (no reference counterpart; the reference has one

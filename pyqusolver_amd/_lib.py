@@ -61,6 +61,7 @@ SIGNATURES = {
     "qes_projected_matvec": [_P, _P, _P, _P, _P],
     "qes_op_n_couplings": [_P, _PI32],
     "qes_op_count_offdiag": [_P, _PI64],
+    "qes_op_count_outside": [_P, _PI64],
     "qes_op_assemble": [_P, _I32, C.c_double, _P],
     "qes_op_assembly_info": [_P, _PI64, _PI64, _PI32, _PI32, _PI32],
     "qes_op_table_coo": [_P, _P, _P, _P, _I64, _PI64, _P],

@@ -540,6 +540,12 @@ int qes_op_count_offdiag(qes_op_t h, int64_t* e_off) {
   return count_offdiag(&h->op, e_off, nullptr);
 }
 
+int qes_op_count_outside(qes_op_t h, int64_t* outside) {
+  if (!h || !outside) return fail("null argument");
+  int64_t e = 0;
+  return count_offdiag(&h->op, &e, nullptr, outside);
+}
+
 int qes_lanczos_create(qes_op_t op, int32_t max_steps, int32_t store_basis, int32_t dtype, qes_lanczos_t* out) {
   if (!op || !out) return fail("null argument");
   Lanczos* L = nullptr;

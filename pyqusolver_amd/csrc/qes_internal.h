@@ -118,7 +118,9 @@ int map_states(const Basis* B, const uint64_t* d_states, int64_t n, int64_t* d_i
 int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int accumulate,
            hipStream_t st);
 int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_t st);
-int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st);
+// off-diagonal elements inside the sector; *outside (optional): nonzero
+// elements whose target lies outside it (skipped by the product)
+int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st, int64_t* outside = nullptr);
 int op_diagonal(const Op* op, double* dre, double* dim, hipStream_t st);
 int projected_matvec(const Op* op, const Basis* out, const double* x, double* y, hipStream_t st);
 int build_plan(Op* op, int canon_override);   // -1: automatic

@@ -183,20 +183,27 @@ __global__ void __launch_bounds__(256) count_offdiag_kernel(QesGroup G, QesLooku
                                                             unsigned long long* __restrict__ out) {
   extern __shared__ char smem[];
   const RowCtx C = load_ctx<true>(G, L, P, chi_re, chi_im, groups, n_diag, exval, lut, smem);
-  __shared__ unsigned long long s_sum[256];
+  __shared__ unsigned long long s_sum[2][256];
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long cnt = 0;
+  unsigned long long cnt = 0, miss = 0;
   if (a < n) {
     const W r = load_rep<W>(L, a);
-    visit_row<W, true, CK>(C, r, [&](int64_t, double, double, int) { ++cnt; });
+    visit_row_m<W, true, CK>(C, r, [&](int64_t, double, double, int) { ++cnt; }, [&]() { ++miss; });
   }
-  s_sum[threadIdx.x] = cnt;
+  s_sum[0][threadIdx.x] = cnt;
+  s_sum[1][threadIdx.x] = miss;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < (unsigned)off) s_sum[threadIdx.x] += s_sum[threadIdx.x + off];
+    if (threadIdx.x < (unsigned)off) {
+      s_sum[0][threadIdx.x] += s_sum[0][threadIdx.x + off];
+      s_sum[1][threadIdx.x] += s_sum[1][threadIdx.x + off];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(out, s_sum[0]);
+  if (threadIdx.x == 0) {
+    atomicAdd(out, s_sum[0][0]);
+    atomicAdd(out + 1, s_sum[1][0]);
+  }
 }
 
 // Fast canonicalisers against the brute-force one on given states.
@@ -453,14 +460,15 @@ int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_
   return 0;
 }
 
-int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st) {
+int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st, int64_t* outside) {
   const Basis* B = op->basis;
   const int64_t n = B->n;
   *e_off = 0;
+  if (outside) *outside = 0;
   if (n == 0) return 0;
   unsigned long long* d = nullptr;
-  QES_HIP(hipMalloc(&d, sizeof(unsigned long long)));
-  QES_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
+  QES_HIP(hipMalloc(&d, 2 * sizeof(unsigned long long)));
+  QES_HIP(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), st));
   dim3 grid((unsigned)((n + 255) / 256));
   const QesPlan& P = op->plan;
 #define QES_CO(W_, CK)                                                                                              \
@@ -475,11 +483,12 @@ int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st) {
   }
 #undef QES_CO
   QES_HIP(hipGetLastError());
-  unsigned long long h = 0;
-  QES_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, st));
+  unsigned long long h[2] = {0, 0};
+  QES_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st));
   QES_HIP(hipStreamSynchronize(st));
   QES_HIP(hipFree(d));
-  *e_off = (int64_t)h;
+  *e_off = (int64_t)h[0];
+  if (outside) *outside = (int64_t)h[1];
   return 0;
 }
 

@@ -55,8 +55,11 @@ __device__ __forceinline__ Canon<W> canon_dispatch(const RowCtx& C, W s) {
 
 // Visit every valid off-diagonal element of row r: f(b, fr, fi, coupling)
 // with (fr, fi) = conj(val) * chi(g*) * N_b (the caller divides by N_a).
-template <typename W, bool CPLX, int CK, class F>
-__device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
+// miss() is called for every nonzero element whose target lies outside the
+// sector (another U(1)/Z2 filling, or an orbit of zero norm): the targets
+// the reference's kernel skips (Algebra/Hilbert/matrix_builder.py:1366-1371).
+template <typename W, bool CPLX, int CK, class F, class M>
+__device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& miss) {
   const int ns = C.G.ns;
   const W mask = (W)C.G.mask;
   auto emit = [&](W sp, double vr, double vi, int cpl) {
@@ -64,7 +67,10 @@ __device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
     int cnt_unused = 0;
     const int64_t b = (C.L.packed && !C.L.hash) ? lookup_packed(C.L, (uint64_t)cn.rep, &cnt_unused)
                                                 : lookup_rep<W>(C.L, cn.rep);
-    if (b < 0) return;
+    if (b < 0) {
+      miss();
+      return;
+    }
     const double nb = sqrt((double)cn.count);
     const double cr = C.s_chr[cn.g];
     double fr, fi;
@@ -130,10 +136,17 @@ __device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
     const double vi = CPLX ? sel4(g.im, k) : 0.0;
     if (vr == 0.0 && vi == 0.0) continue;
     const W sp = r ^ (W)g.mask;
-    if (C.G.n_up >= 0 && popc<W>(sp) != C.G.n_up) continue;
-    if (C.G.z2 != 0 && ((popc<W>(sp) & 1) ? -1 : 1) != C.G.z2) continue;
+    if ((C.G.n_up >= 0 && popc<W>(sp) != C.G.n_up) || (C.G.z2 != 0 && ((popc<W>(sp) & 1) ? -1 : 1) != C.G.z2)) {
+      miss();
+      continue;
+    }
     emit(sp, vr, vi, -1 - t);
   }
+}
+
+template <typename W, bool CPLX, int CK, class F>
+__device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
+  visit_row_m<W, CPLX, CK>(C, r, f, []() {});
 }
 
 // Load the per-block tables into LDS; returns the context.

@@ -278,6 +278,14 @@ class Hamiltonian:
         self._hamil_sparse = A
         return A.toarray() if use_numpy else A
 
+    def count_outside(self) -> int:
+        """Nonzero off-diagonal elements whose target lies outside the sector
+        (skipped by the product, as the reference's kernel does): 0 for an
+        operator that conserves the sector's quantum numbers."""
+        v = C.c_int64()
+        _lib.check(_lib.lib().qes_op_count_outside(self._device_op(), C.byref(v)), "qes_op_count_outside")
+        return int(v.value)
+
     def count_offdiag(self) -> int:
         v = C.c_int64()
         _lib.check(_lib.lib().qes_op_count_offdiag(self._device_op(), C.byref(v)), "qes_op_count_offdiag")

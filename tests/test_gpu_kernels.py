@@ -213,3 +213,17 @@ def test_assembled_multi_rhs_equals_single_columns(case, fmt):
         H.matvec_device(X, out=Y2, accumulate=True)
         assert torch.allclose(Y2, 2 * Y, rtol=1e-14, atol=1e-14 * Y.abs().max().item())
     H.assemble(False)
+
+
+def test_count_outside_sector():
+    """Debug count of nonzero elements whose target leaves the sector (the
+    reference skips them silently): zero for a U(1)-conserving model at k=0;
+    for a transverse field in a U(1) sector every sigma^x_i leaves it, so the
+    count is N * ns exactly; the gathers inside are unchanged by the field."""
+    from pyqusolver_amd.models import J1J2Model
+    lat, hs = _space("chain", 12, 1, {"translation": 0}, 6)
+    H0 = J1J2Model(lat, J1=-1.0, J2=-0.4, hilbert_space=hs)
+    assert H0.count_outside() == 0
+    Hx = J1J2Model(lat, J1=-1.0, J2=-0.4, hilbert_space=hs, hx=0.2)
+    assert Hx.count_outside() == hs.nh * 12
+    assert Hx.count_offdiag() == H0.count_offdiag()
