@@ -120,6 +120,7 @@ def bench_batched(a):
         "sector_by_sector": "one device Lanczos per problem, each operator assembled for its own solve",
         "e0_sectors_L24": [float(v) for v in res.e0[:len(specs)]],
         "max_rel_dev_batched_vs_sequential": float(dev.max()),
+        "schedule": res.schedule,
         "max_rel_dev_E0_k_vs_minus_k": pair_dev,
         "stopping": "per problem: Ritz residual |beta_m s_m| <= 1e-10 max(1,|E0|), checked every 10 steps",
     }

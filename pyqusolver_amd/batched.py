@@ -27,6 +27,7 @@ class BatchResult:
     e0: np.ndarray
     steps: np.ndarray
     seconds: float
+    schedule: Optional[list] = None     # autotuned schedule of each batch (batched_ground_states)
 
 
 _DIAGONAL_CODES = (13, 22)   # sigma_z local / sigma_z correlation (operators_spin.py:1084-1118)
@@ -139,6 +140,56 @@ class BatchedLanczos:
     @property
     def torch_dtype(self):
         return torch.complex128 if self.dtype == 1 else torch.float64
+
+    # candidate values of the persistent matvec schedule, tried in this order
+    SCHEDULE_SPACE = (("tile_order", (1, 0)), ("grid_mult", (8, 4, 16, 2)), ("block_rows", (16, 8, 32)))
+
+    def autotune(self, reps: int = 3, space=None) -> dict:
+        """Choose the schedule of the persistent batched matvec (tile order,
+        blocks per CU, rows per block tile) for this batch by timing it:
+        coordinate search over ``SCHEDULE_SPACE``, each candidate timed as the
+        median of ``reps`` products on a fixed vector with device events; the
+        fastest value of each knob is kept before the next knob is tried.
+        The products are identical under every schedule, so tuning changes
+        only speed.  Returns {option: value, ..., "ms": best time}."""
+        space = self.SCHEDULE_SPACE if space is None else space
+        total = int(self.offsets[-1])
+        g = torch.Generator(device="cuda").manual_seed(0)
+        x = torch.randn(total, dtype=torch.float64, device="cuda", generator=g)
+        if self.dtype == 1:
+            x = torch.complex(x, torch.randn(total, dtype=torch.float64, device="cuda", generator=g))
+        y = torch.empty_like(x)
+        stream = torch.cuda.current_stream()
+        L = _lib.lib()
+
+        def timed() -> float:
+            L.qes_batch_matvec(self._h, _lib.dptr(x), _lib.dptr(y), _lib.stream_ptr())    # warm
+            ts = []
+            for _ in range(max(1, reps)):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                _lib.check(L.qes_batch_matvec(self._h, _lib.dptr(x), _lib.dptr(y), _lib.stream_ptr()),
+                           "qes_batch_matvec")
+                b.record(stream)
+                b.synchronize()
+                ts.append(a.elapsed_time(b))
+            return float(np.median(ts))
+        has_blocks = bool(self.blocks)
+        chosen, best_ms = {}, None
+        for name, values in space:
+            if name == "block_rows" and not has_blocks:
+                continue
+            times = {}
+            for v in values:
+                self.set_option(name, v)
+                times[v] = timed()
+            v_best = min(times, key=times.get)
+            self.set_option(name, v_best)
+            chosen[name] = v_best
+            best_ms = times[v_best]
+        chosen["ms"] = best_ms
+        self.schedule = chosen
+        return chosen
 
     def set_option(self, name: str, value: int):
         """Schedule knob: tile_order (0/1), grid_mult (blocks per CU),
@@ -348,16 +399,19 @@ def sector_by_sector(hams: Sequence, tol: float = 1e-10, max_steps: int = 300, c
 
 
 def batched_ground_states(hams: Sequence, tol: float = 1e-10, max_steps: int = 300, check_every: int = 10,
-                          seeds=None, split_dtypes: bool = True, concurrent: bool = True, warmup: bool = False):
+                          seeds=None, split_dtypes: bool = True, concurrent: bool = True, warmup: bool = False,
+                          autotune: bool = True):
     """Ground states of many problems with batched Lanczos.
 
     Problems that need complex128 vectors (generic momenta, complex models)
     and the real ones go to two batches, so the real problems (and their
     realisation blocks) move half the bytes; the two batches run on their
     own streams from two host threads and overlap on the device.  Seeds
-    default to the problem's position, as in ``sector_by_sector``.  Returns
-    (BatchResult in the order of ``hams``, seconds spent creating the
-    batches: tables, diagonals)."""
+    default to the problem's position, as in ``sector_by_sector``.  With
+    ``autotune`` each batch's matvec schedule is chosen by timing
+    (``BatchedLanczos.autotune``).  Returns (BatchResult in the order of
+    ``hams``, seconds spent creating the batches: tables, diagonals,
+    tuning)."""
     import threading
     seeds = list(range(len(hams))) if seeds is None else list(seeds)
     groups = {}
@@ -367,6 +421,7 @@ def batched_ground_states(hams: Sequence, tol: float = 1e-10, max_steps: int = 3
     t0 = time.perf_counter()
     batches = [(idx, BatchedLanczos([hams[i] for i in idx], max_steps=max_steps, complex_=c))
                for c, idx in sorted(groups.items(), key=lambda kv: -len(kv[1]))]
+    schedules = [B.autotune() if autotune else None for _, B in batches]
     torch.cuda.synchronize()
     t_create = time.perf_counter() - t0
     results = [None] * len(batches)
@@ -413,4 +468,4 @@ def batched_ground_states(hams: Sequence, tol: float = 1e-10, max_steps: int = 3
         e0[idx] = r.e0
         steps[idx] = r.steps
         B.close()
-    return BatchResult(e0=e0, steps=steps, seconds=seconds), t_create
+    return BatchResult(e0=e0, steps=steps, seconds=seconds, schedule=schedules), t_create

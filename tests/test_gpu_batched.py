@@ -283,3 +283,31 @@ def test_coupling_sweep_field_scan_against_separate_models():
         for r, (jz, hz) in enumerate(sets):
             ref = XXZ(lat, hilbert_space=hs, jxy=1.0, jz=jz, hz=hz).diagonalize(k=1, tol=1e-12, maxiter=300)[0]
             assert abs(e[p, r] - ref) <= 1e-10 * max(1.0, abs(ref)), (p, jz, hz)
+
+
+def test_autotuned_schedule_changes_speed_only():
+    """The schedule autotuner picks tile order / blocks per CU / block rows
+    by timing the batch's own product; products and ground states are the
+    same under the tuned schedule as under the default one."""
+    import torch
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.batched import BatchedLanczos
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import XXZ
+    lat, specs, spaces, hams = _xxz_sectors(12)
+    real = [h for h in hams if not h.needs_complex]
+    lat10 = ChainLattice(10)
+    hs10 = HilbertSpace(lattice=lat10, global_syms=[get_u1_sym(lat10, 5)])
+    real += [XXZ(lat10, hilbert_space=hs10, jxy=1.0, jz=0.7, hz=np.random.default_rng(r).uniform(-2, 2, 10))
+             for r in range(8)]
+    B = BatchedLanczos(real, max_steps=120)
+    x = B.concat([np.random.default_rng(i).standard_normal(h.nh) for i, h in enumerate(real)])
+    y0 = B.matvec(x)
+    r0 = B.ground_states(tol=1e-10)
+    sched = B.autotune(reps=2)
+    assert set(sched) >= {"tile_order", "grid_mult", "block_rows", "ms"} and sched["ms"] > 0
+    assert torch.equal(B.matvec(x), y0)
+    r1 = B.ground_states(tol=1e-10)
+    np.testing.assert_array_equal(r1.e0, r0.e0)
+    np.testing.assert_array_equal(r1.steps, r0.steps)
+    B.close()
