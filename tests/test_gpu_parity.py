@@ -339,6 +339,50 @@ def test_square6x6_scale_properties():
     H.assemble(False)
 
 
+@pytest.mark.parametrize("which", ["honeycomb3x4_hk", "chain24_xxz_k5"])
+def test_workload_scale_sampled_rows(which):
+    """Full-size sectors of the Kitaev-Heisenberg workload (honeycomb 3x4
+    cells, k=0, no U(1), fields along [111]: N = 1,398,500, complex) and of
+    the batched sweep (XXZ L=24, Sz=0, k=5: complex characters): closed-form
+    dimension, 300 sampled rows against the oracle's row formula, and
+    hermiticity <z, Hx> = <Hz, x>."""
+    import torch
+    from pyqusolver_amd import ChainLattice, HoneycombLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import HeisenbergKitaev, XXZ
+    if which == "honeycomb3x4_hk":
+        lat, olat = HoneycombLattice(3, 4), O.OLattice("honeycomb", 3, 4)
+        hs = HilbertSpace(lattice=lat, sym_gen={"translation": {"kx": 0, "ky": 0}})
+        h = 0.05 / np.sqrt(3.0)
+        H = HeisenbergKitaev(lat, K=-1.0, J=0.2, hilbert_space=hs, hx=h, hy=h, hz=h)
+        t = O.terms_heisenberg_kitaev(olat, K=-1.0, J=0.2, hx=h, hy=h, hz=h)
+        grp = O.Group(olat, [O.Generator("tx", 0), O.Generator("ty", 0)])
+        assert hs.nh == 1_398_500
+    else:
+        lat, olat = ChainLattice(24), O.OLattice("chain", 24)
+        hs = HilbertSpace(lattice=lat, sym_gen={"translation": 5}, global_syms=[get_u1_sym(lat, 12)])
+        H = XXZ(lat, hilbert_space=hs, jxy=1.0, jz=0.7)
+        t = O.terms_xxz(olat, 1.0, 0.7)
+        grp = O.Group(olat, [O.Generator("tx", 5)], n_up=12)
+    n = hs.nh
+    assert n == O.sector_dimension(grp)
+    c, s, co = t.arrays()
+    g = torch.Generator(device="cuda").manual_seed(4)
+    x = torch.complex(torch.randn(n, dtype=torch.float64, device="cuda", generator=g),
+                      torch.randn(n, dtype=torch.float64, device="cuda", generator=g))
+    z = torch.complex(torch.randn(n, dtype=torch.float64, device="cuda", generator=g),
+                      torch.randn(n, dtype=torch.float64, device="cuda", generator=g))
+    y = H.matvec_device(x)
+    lhs = torch.vdot(z, y).item()
+    rhs = torch.vdot(H.matvec_device(z), x).item()
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    reps = hs.representative_list
+    sample = np.sort(np.random.default_rng(6).choice(n, 300, replace=False))
+    ys = O.sampled_rows(grp, reps, hs.normalization, sample, lat.ns, c, s, co, True, x.cpu().numpy())
+    yg = y.cpu().numpy()[sample]
+    assert np.linalg.norm(yg - ys) <= MATVEC_TOL * np.linalg.norm(ys)
+
+
 def test_index_map_on_random_states_chain32():
     from pyqusolver_amd import ChainLattice, get_u1_sym
     from pyqusolver_amd.hilbert import HilbertSpace
