@@ -493,6 +493,20 @@ def bench_plain(a):
                               f"({r.second_passes} of {r.steps} steps)") if reorth else
                              "none (three-term recurrence)"}
         log(f"lanczos E0={e0:.14f} steps={r.steps} {dt:.2f}s residual<={r.residual_bound:.2e}", a.quiet)
+        if a.config == "large":
+            # accuracy witness outside the timed solve: the ground-state Ritz
+            # vector (second run of the recurrence, no stored basis) and its
+            # true residual ||H psi - E0 psi|| / ||psi||
+            from pyqusolver_amd.lanczos import lanczos_ground_state
+            H.assemble(True, max_gb=min(64.0, 0.25 * torch.cuda.mem_get_info()[0] / 1e9))
+            rv = lanczos_ground_state(H, k=1, tol=1e-10, maxiter=a.lanczos_steps, seed=0, reorth=False,
+                                      store_basis=False, compute_eigvec=True)
+            psi = rv.eigenvectors
+            res = torch.linalg.vector_norm(H.matvec_device(psi) - float(rv.eigenvalues[0]) * psi)
+            lanczos["ritz_vector_residual"] = float(res / torch.linalg.vector_norm(psi))
+            H.assemble(False)
+            del psi
+            log(f"ritz vector residual {lanczos['ritz_vector_residual']:.2e}", a.quiet)
 
     cpu = None
     if a.cpu_sample > 0:

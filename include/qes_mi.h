@@ -241,6 +241,11 @@ int qes_lanczos_start(qes_lanczos_t L, const void* v0_dev, void* stream);
 int qes_lanczos_run(qes_lanczos_t L, int32_t nsteps, int32_t reorth, int32_t use_graph);
 int qes_lanczos_get(qes_lanczos_t L, double* alpha_host, double* beta_host, int32_t* steps_done);
 int qes_lanczos_ritz(qes_lanczos_t L, const double* coeffs_host, int32_t k, void* out_dev, void* stream);
+/* out[i] += coeffs[i] * v_j for i < nvec (1..16; out holds nvec consecutive
+ * vectors), v_j = the current Lanczos vector (v_0 after start, v_{j+1} after
+ * step j).  With a second run from the same start vector (the steps are
+ * bitwise reproducible) this forms Ritz vectors without storing the basis. */
+int qes_lanczos_accumulate(qes_lanczos_t L, const double* coeffs_host, int32_t nvec, void* out_dev, void* stream);
 int qes_lanczos_destroy(qes_lanczos_t L);
 /* Number of second Gram-Schmidt passes run since the last start. */
 int qes_lanczos_stats(qes_lanczos_t L, int32_t* second_passes);

@@ -81,6 +81,7 @@ SIGNATURES = {
     "qes_lanczos_run": [_P, _I32, _I32, _I32],
     "qes_lanczos_get": [_P, _PD, _PD, _PI32],
     "qes_lanczos_ritz": [_P, _PD, _I32, _P, _P],
+    "qes_lanczos_accumulate": [_P, _PD, _I32, _P, _P],
     "qes_lanczos_destroy": [_P],
     "qes_lanczos_stats": [_P, _PI32],
     "qes_lanczos_ground_state": [_P, _P, _I32, _I32, _I32, C.c_double, _I32, _PD, _PD, _PI32, _PD, _P],

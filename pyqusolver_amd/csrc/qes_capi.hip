@@ -585,6 +585,12 @@ int qes_lanczos_ritz(qes_lanczos_t h, const double* coeffs_host, int32_t k, void
   return lanczos_combine(h->L, coeffs_host, k, out_dev, (hipStream_t)stream);
 }
 
+int qes_lanczos_accumulate(qes_lanczos_t h, const double* coeffs_host, int32_t nvec, void* out_dev, void* stream) {
+  if (!h || !coeffs_host || !out_dev) return fail("null argument");
+  (void)stream;
+  return lanczos_accumulate(h->L, coeffs_host, nvec, out_dev);
+}
+
 int qes_lanczos_destroy(qes_lanczos_t h) {
   if (!h) return 0;
   lanczos_destroy(h->L);

@@ -553,6 +553,32 @@ int lanczos_combine(Lanczos* L, const double* c_host, int k, void* out, hipStrea
   return 0;
 }
 
+// out[i] += c[i] * v for i < nvec (out: nvec consecutive vectors of len doubles;
+// complex vectors take real coefficients: the Ritz vectors of T_m are real)
+struct AccumCoef {
+  double c[LANCZOS_MAX_ACCUM];
+};
+
+__global__ void __launch_bounds__(RB) accumulate_kernel(const double* __restrict__ v, int64_t len, AccumCoef cf,
+                                                        int nvec, double* __restrict__ out) {
+  for (int64_t e = (int64_t)blockIdx.x * RB + threadIdx.x; e < len; e += (int64_t)gridDim.x * RB) {
+    const double x = v[e];
+    for (int i = 0; i < nvec; ++i) out[(int64_t)i * len + e] += cf.c[i] * x;
+  }
+}
+
+int lanczos_accumulate(Lanczos* L, const double* c_host, int nvec, void* out) {
+  if (nvec < 1 || nvec > LANCZOS_MAX_ACCUM) return fail("accumulate takes 1.." + std::to_string(LANCZOS_MAX_ACCUM) +
+                                                        " vectors");
+  AccumCoef cf{};
+  for (int i = 0; i < nvec; ++i) cf.c[i] = c_host[i];
+  const int g = grid_for(L->len);
+  hipLaunchKernelGGL(accumulate_kernel, dim3(g), dim3(RB), 0, L->stream, L->v, L->len, cf, nvec, (double*)out);
+  QES_HIP(hipGetLastError());
+  QES_HIP(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
 int lanczos_steps_done(const Lanczos* L) { return L->steps_done; }
 
 // eigenvalue idx (0 = lowest) of the symmetric tridiagonal T_k with

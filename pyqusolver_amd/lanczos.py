@@ -88,6 +88,12 @@ class DeviceLanczos:
         _lib.check(_lib.lib().qes_lanczos_ritz(self._h, c.ctypes.data_as(_lib._PD), len(c), _lib.dptr(out),
                                                _lib.stream_ptr()), "qes_lanczos_ritz")
 
+    def accumulate(self, coeffs: np.ndarray, out: torch.Tensor):
+        """out[i] += coeffs[i] * (current Lanczos vector) for the rows of out."""
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        _lib.check(_lib.lib().qes_lanczos_accumulate(self._h, c.ctypes.data_as(_lib._PD), len(c), _lib.dptr(out),
+                                                     _lib.stream_ptr()), "qes_lanczos_accumulate")
+
     def close(self):
         if self._h is not None and self._h.value:
             _lib.lib().qes_lanczos_destroy(self._h)
@@ -145,7 +151,8 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
     single start vector finds one copy of a degenerate eigenvalue; without
     reorthogonalisation only the lowest value is reliable.  With
     ``compute_eigvec`` the Ritz vectors are formed on the device: shape (n,)
-    for k = 1, (n, k) otherwise."""
+    for k = 1, (n, k) otherwise; without a stored basis (store_basis=False)
+    by a second run of the recurrence from the same start vector."""
     n = ham.nh
     complex_ = bool(ham.needs_complex)
     if v0 is None:
@@ -196,9 +203,27 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
     kk = min(k, len(a))
     res_bound = float(np.max(np.abs(b[len(a) - 1] * V[-1, :kk]))) if len(a) else float("nan")
     vecs = None
+    if compute_eigvec and not store_basis:
+        # second pass from the same start vector: the steps are bitwise
+        # reproducible, so v_j is regenerated and sum_j S[j, i] v_j formed
+        # without keeping the basis (memory: four vectors + the k outputs)
+        m = len(a)
+        kk = min(k, m, 16)
+        second = L.second_passes if reorth else 0
+        L.close()
+        L = DeviceLanczos(ham, max(m, 1), False, complex_)
+        L.start(v0t)
+        cols = torch.zeros((kk, n), dtype=v0t.dtype, device="cuda")
+        L.accumulate(V[0, :kk], cols)
+        for j in range(1, m):
+            L.run(1, False, use_graph)
+            L.accumulate(V[j, :kk], cols)
+        vecs = cols[0].clone() if k == 1 else cols.transpose(0, 1).contiguous()
+        L.close()
+        return LanczosResult(eigenvalues=w[:k].copy(), alpha=a, beta=b, steps=len(a), converged=converged,
+                             history=hist, eigenvectors=vecs, seconds=time.perf_counter() - t0,
+                             residual_bound=res_bound, second_passes=second)
     if compute_eigvec:
-        if not store_basis:
-            raise ValueError("eigenvectors need store_basis=True")
         if k == 1:
             vecs = torch.empty((n,), dtype=v0t.dtype, device="cuda")
             L.ritz(V[:, 0], vecs)

@@ -672,3 +672,26 @@ def test_lowest_k_eigenpairs_with_reorthogonalisation():
         v = V[:, i].contiguous()
         r = H.matvec_device(v) - float(w[i]) * v
         assert torch.linalg.vector_norm(r).item() <= 1e-8 * torch.linalg.vector_norm(v).item()
+
+
+def test_two_pass_ritz_vector_without_stored_basis():
+    """Ritz vectors without keeping the Krylov basis: the recurrence is run
+    a second time from the same start vector (bitwise reproducible steps)
+    and sum_j s_j v_j accumulated on the device.  Equal to the stored-basis
+    Ritz vector of the same recurrence, an eigenvector of H to the residual,
+    for k = 1 and k = 2."""
+    import torch
+    from pyqusolver_amd.lanczos import lanczos_ground_state
+    hs, H, grp, b, t = build_pair("chain", 16, 1, [("tx", 0)], 8, "j1j2")
+    for k in (1, 2):
+        r2 = lanczos_ground_state(H, k=k, tol=1e-12, maxiter=200, seed=4, reorth=False, store_basis=False,
+                                  compute_eigvec=True)
+        r1 = lanczos_ground_state(H, k=k, tol=1e-12, maxiter=200, seed=4, reorth=False, store_basis=True,
+                                  compute_eigvec=True)
+        np.testing.assert_array_equal(r1.alpha, r2.alpha)
+        v1, v2 = r1.eigenvectors, r2.eigenvectors
+        assert tuple(v2.shape) == tuple(v1.shape)
+        assert torch.allclose(v2, v1, rtol=0, atol=1e-12)
+        psi = v2 if k == 1 else v2[:, 0].contiguous()
+        res = H.matvec_device(psi) - float(r2.eigenvalues[0]) * psi
+        assert torch.linalg.vector_norm(res).item() <= 1e-9 * torch.linalg.vector_norm(psi).item()
