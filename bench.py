@@ -59,6 +59,9 @@ def parse():
                         "J1-J2 6x6 square, every momentum sector (flip+) at J2 in {0.4, 0.5, 0.6}, one coupling "
                         "sweep per sector (several minutes)")
     p.add_argument("--realisations", type=int, default=256)
+    p.add_argument("--check-min", type=int, default=None,
+                   help="batched/sweep: 0 = stopping test every 10 steps; n >= 1 = adaptive, >= n steps apart; "
+                        "default: the library's choice (adaptive only for a few large problems)")
     p.add_argument("--sectors", type=int, default=36, help="sweep: momentum sectors to run (of 36)")
     p.add_argument("--sector-start", type=int, default=0, help="sweep: first sector (row-major (kx, ky) order)")
     p.add_argument("--baseline-sectors", type=int, default=1,
@@ -93,7 +96,8 @@ def bench_batched(a):
     log(f"{len(hams)} problems, {rows} rows, setup {t_setup:.2f}s", a.quiet)
     # real and complex problems in two batches that overlap on the device;
     # one warm-up sweep (graph capture, code objects) before the timed one
-    res, t_create = batched_ground_states(hams, tol=1e-10, max_steps=a.lanczos_steps, warmup=True)
+    res, t_create = batched_ground_states(hams, tol=1e-10, max_steps=a.lanczos_steps, warmup=True,
+                                          check_min=a.check_min)
     log(f"batched: {res.seconds:.3f}s (+{t_create:.3f}s batch creation: tables, diagonals), "
         f"steps max {res.steps.max()}", a.quiet)
     seq = sector_by_sector(hams, tol=1e-10, max_steps=a.lanczos_steps)
@@ -239,7 +243,7 @@ def bench_sweep(a):
         S = CouplingSweep([H], scales, max_steps=a.lanczos_steps, max_gb=160.0)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        res = S.ground_states(tol=1e-10)
+        res = S.ground_states(tol=1e-10, check_min=a.check_min)
         S.close()
         torch.cuda.synchronize()
         t3 = time.perf_counter()
@@ -282,7 +286,8 @@ def bench_sweep(a):
         "config": {"workload": "J1-J2 square 6x6 PBC J1=1 Sz=0 flip+, momentum sectors x J2 in {0.4, 0.5, 0.6}, "
                                "one coupling sweep per sector", "sectors": len(ks), "first_sector": a.sector_start},
         "seconds_total": t_total, "seconds_tables": t_tables, "sectors": results, "sequential_baseline": base,
-        "stopping": "per (sector, J2): Ritz residual |beta_m s_m| <= 1e-10 max(1,|E0|), checked every 10 steps",
+        "stopping": "per (sector, J2): Ritz residual |beta_m s_m| <= 1e-10 max(1,|E0|), tested at most 10 steps "
+                    "apart, where the residual's decay predicts the crossing",
     }
     print(json.dumps(out), flush=True)
 
@@ -290,6 +295,18 @@ def bench_sweep(a):
 def log(msg, quiet=False):
     if not quiet:
         print(msg, file=sys.stderr, flush=True)
+
+
+def _cpu_model() -> str:
+    """Host CPU model name (the lscpu "Model name" line, from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(wl, J2, reps, norms, sample, n):
@@ -326,7 +343,7 @@ def cpu_baseline(wl, J2, reps, norms, sample, n):
     dtm = time.perf_counter() - t0
     cols_m = sum(b - a for a, b in ranges) / dtm
     return {"value": cols_m / n, "unit": "matvec/s", "cores": len(ranges), "kind": "port",
-            "single_thread_value": cols_1 / n,
+            "single_thread_value": cols_1 / n, "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"push-form columns of the same {wl['name']} sector: 0..{k1} on 1 thread ({dt:.1f} s, "
                       f"{cols_1:.0f} columns/s), then {len(ranges)} threads x {per} columns ({dtm:.1f} s wall, "
                       f"{cols_m:.0f} columns/s), extrapolated to N={n}; reference group action per element and "
@@ -485,7 +502,8 @@ def bench_plain(a):
                    "steps": int(r.steps), "steps_to_1e-10": hit,
                    "steps_to_1e-10_reference": "exact E0" if e_exact is not None else "final E0 (self)",
                    "residual_bound": r.residual_bound, "converged": bool(r.converged),
-                   "stopping": "Ritz residual |beta_m s_m| <= 1e-10 max(1,|E0|), checked every 10 steps",
+                   "stopping": "Ritz residual |beta_m s_m| <= 1e-10 max(1,|E0|), tested at most 10 steps apart, "
+                               "where the residual's decay predicts the crossing",
                    "seconds": round(dt, 4),
                    "seconds_assembly": round(t_asm, 4), "seconds_solve": round(dt - t_asm, 4),
                    "assembled_operator": bool(r.assembled),

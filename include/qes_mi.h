@@ -252,7 +252,10 @@ int qes_lanczos_stats(qes_lanczos_t L, int32_t* second_passes);
 
 /*
  * All-in-one ground state: run up to max_steps Lanczos steps from v0 (device),
- * checking the lowest Ritz value theta every check_every steps; stop on
+ * checking the lowest Ritz value theta every check_every steps (for N >=
+ * 2^20 rows at most check_every steps apart: the next test goes where the
+ * residual's geometric decay between the last two tests reaches tol, if that
+ * is sooner); stop on
  * breakdown or when the residual norm of the Ritz pair r = |beta_k s_k|
  * (s: unit eigenvector of T_k) is <= tol*max(1,|theta|) -- the stopping test
  * of the reference's eigsh/ARPACK path (Algebra/Hamil/hamil_diag_engine.py:
@@ -329,7 +332,11 @@ int qes_batch_matvec(qes_batch_t b, const void* x_dev, void* y_dev, void* stream
  * all steps; 0: canonicalise and look up on the fly like qes_matvec),
  * "op_tables" (1, default: ordinary problems whose operator was assembled with
  * qes_op_assemble before qes_batch_create stream that table; the batch keeps
- * pointers to it, so the assembly must outlive the batch). */
+ * pointers to it, so the assembly must outlive the batch), "check_min"
+ * (qes_batch_ground_states: 0 tests every check_every steps; n >= 1 places
+ * each test where the residual decay of the active problems predicts the
+ * first crossing, at least n and at most check_every steps apart; -1,
+ * default: 1 for batches of at most 4 problems with >= 2^20 rows, else 0). */
 int qes_batch_set_option(qes_batch_t b, const char* name, int64_t value);
 /* Lanczos on every problem of the batch; a problem stops on breakdown, at
  * its dimension, or under the residual test of qes_lanczos_ground_state. */

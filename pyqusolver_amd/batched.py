@@ -195,7 +195,9 @@ class BatchedLanczos:
         """Schedule knob: tile_order (0/1), grid_mult (blocks per CU),
         block_rows (rows per tile of a block problem), block_table (0:
         matrix-free block rows, 1: cached neighbour table) or op_tables (0:
-        matrix-free ordinary problems, 1: their assembled tables)."""
+        matrix-free ordinary problems, 1: their assembled tables); check_min
+        (ground states: 0 fixed test interval, n >= 1 adaptive, see
+        ``ground_states``)."""
         _lib.check(_lib.lib().qes_batch_set_option(self._h, name.encode(), int(value)), "qes_batch_set_option")
 
     def concat(self, vecs: Sequence) -> torch.Tensor:
@@ -245,7 +247,18 @@ class BatchedLanczos:
         _lib.check(_lib.lib().qes_batch_run(self._h, 0, 1), "qes_batch_run")
 
     def ground_states(self, tol: float = 1e-10, check_every: int = 10, v0=None, seeds=None,
-                      sync_device: bool = True) -> BatchResult:
+                      sync_device: bool = True, check_min: Optional[int] = None) -> BatchResult:
+        """Lanczos on every problem to the residual test (qes_batch_ground_states).
+        ``check_min`` (the batch option of that name): 0 tests every
+        ``check_every`` steps; n >= 1 places each test where the active
+        problems' residual decay predicts the first crossing, n..check_every
+        steps apart; -1 (the batch's default): 1 for at most 4 problems of
+        >= 2^20 rows (a coupling sweep of one large sector), else 0 -- with
+        many problems the earliest predicted crossing keeps the interval
+        short and every test is a host round trip; None keeps the batch's
+        setting."""
+        if check_min is not None:
+            self.set_option("check_min", int(check_min))
         if sync_device:
             torch.cuda.synchronize()
         else:
@@ -348,12 +361,13 @@ class CouplingSweep(BatchedLanczos):
         self.offsets = offs
 
     def ground_states(self, tol: float = 1e-10, check_every: int = 10, v0=None, seeds=None,
-                      sync_device: bool = True) -> BatchResult:
+                      sync_device: bool = True, check_min: Optional[int] = None) -> BatchResult:
         """Seeds default to sector index (the same start vector for every
         column of a sector, as separate solves with seed = sector would use)."""
         if seeds is None and v0 is None:
             seeds = [p for p in range(len(self.sectors)) for _ in range(self.ncol)]
-        return super().ground_states(tol=tol, check_every=check_every, v0=v0, seeds=seeds, sync_device=sync_device)
+        return super().ground_states(tol=tol, check_every=check_every, v0=v0, seeds=seeds, sync_device=sync_device,
+                                     check_min=check_min)
 
 
 def momentum_sectors(L: int, reflection: bool = True):
@@ -400,7 +414,7 @@ def sector_by_sector(hams: Sequence, tol: float = 1e-10, max_steps: int = 300, c
 
 def batched_ground_states(hams: Sequence, tol: float = 1e-10, max_steps: int = 300, check_every: int = 10,
                           seeds=None, split_dtypes: bool = True, concurrent: bool = True, warmup: bool = False,
-                          autotune: bool = True):
+                          autotune: bool = True, check_min: Optional[int] = None):
     """Ground states of many problems with batched Lanczos.
 
     Problems that need complex128 vectors (generic momenta, complex models)
@@ -411,7 +425,7 @@ def batched_ground_states(hams: Sequence, tol: float = 1e-10, max_steps: int = 3
     ``autotune`` each batch's matvec schedule is chosen by timing
     (``BatchedLanczos.autotune``).  Returns (BatchResult in the order of
     ``hams``, seconds spent creating the batches: tables, diagonals,
-    tuning)."""
+    tuning).  ``check_min``: see ``BatchedLanczos.ground_states``."""
     import threading
     seeds = list(range(len(hams))) if seeds is None else list(seeds)
     groups = {}
@@ -438,7 +452,7 @@ def batched_ground_states(hams: Sequence, tol: float = 1e-10, max_steps: int = 3
             # work while another batch's graph runs
             with torch.cuda.stream(streams[k]):
                 results[k] = B.ground_states(tol=tol, check_every=check_every, seeds=[seeds[i] for i in idx],
-                                             sync_device=False)
+                                             sync_device=False, check_min=check_min)
         except Exception as ex:   # re-raised on the calling thread
             errors.append(ex)
 

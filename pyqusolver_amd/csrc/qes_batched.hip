@@ -767,6 +767,8 @@ struct Batch {
   int grid_matvec = 0, grid_vec = 0;
   int tile_order = 1;     // 0: problem-contiguous tiles, 1: largest problems first, round-robin
   int grid_mult = 8;      // persistent matvec blocks per CU
+  int check_min = -1;     // ground states: 0 = test every check_every steps; n >= 1 = adaptive, >= n apart;
+                          // -1 = adaptive (n = 1) for at most 4 problems of >= 2^20 rows, else 0
   int block_rows = 16;    // rows per tile of a block problem (at least one per lane group)
   int block_table = 1;    // block problems use their neighbour table
   bool use_tables = true; // ordinary problems stream their operator's assembled table when it has one
@@ -1229,12 +1231,16 @@ int batch_matvec(Batch* B, const void* x, void* y, hipStream_t st) {
 }
 
 int batch_set_option(Batch* B, const std::string& name, int64_t value) {
+  if (name == "check_min") {   // no effect on the step graph or the tiles
+    B->check_min = (int)std::max<int64_t>(-1, std::min<int64_t>(value, 1 << 20));
+    return 0;
+  }
   if (name == "tile_order") B->tile_order = (int)value;
   else if (name == "grid_mult") B->grid_mult = (int)std::max<int64_t>(1, value);
   else if (name == "block_rows") B->block_rows = (int)std::max<int64_t>(1, std::min<int64_t>(value, 1 << 20));
   else if (name == "block_table") B->block_table = value ? 1 : 0;
   else if (name == "op_tables") B->use_tables = value != 0;
-  else return fail("unknown batch option " + name + " (tile_order, grid_mult, block_rows, block_table, op_tables)");
+  else return fail("unknown batch option " + name + " (tile_order, grid_mult, block_rows, block_table, op_tables, check_min)");
   batch_drop_graph(B);
   if (batch_upload_probs(B)) return -1;
   return batch_build_tiles(B);
@@ -1244,18 +1250,26 @@ int batch_ground_states(Batch* B, double tol, int check_every, double* e0, int* 
   const int V = B->nvp;
   std::vector<double> a((size_t)V * B->maxs), b((size_t)V * B->maxs);
   std::vector<int> steps(V), active(V, 1);
-  std::vector<double> cur(V, 0.0);
+  std::vector<double> cur(V, 0.0), r_prev(V, 0.0);
+  std::vector<int> m_prev(V, 0);
   std::vector<int64_t> dim(V);
   for (int p = 0; p < B->nprob; ++p)
     for (int r = 0; r < B->nrhs[p]; ++r) dim[B->vp0[p] + r] = B->ops[p]->basis->n;
   if (check_every < 1) check_every = 10;
-  int done_steps = 0;
+  int check_min = B->check_min;
+  if (check_min < 0) {   // a test costs a host round trip: worth placing only when steps are long and few problems set it
+    int64_t nmax = 0;
+    for (int p = 0; p < V; ++p) nmax = std::max(nmax, dim[p]);
+    check_min = (V <= 4 && nmax >= ((int64_t)1 << 20)) ? 1 : 0;
+  }
+  int done_steps = 0, interval = check_every;
   while (done_steps < B->maxs) {
-    const int ns = std::min(check_every, B->maxs - done_steps);
+    const int ns = std::min(interval, B->maxs - done_steps);
     if (batch_run(B, ns, 1)) return -1;
     done_steps += ns;
     if (batch_get(B, a.data(), b.data(), steps.data())) return -1;
     bool any = false;
+    int next_iv = check_every;
     for (int p = 0; p < V; ++p) {
       if (!active[p]) continue;
       const int k = steps[p];
@@ -1265,15 +1279,21 @@ int batch_ground_states(Batch* B, double tol, int check_every, double* e0, int* 
       bool brk = false;
       for (int i = 0; i < k; ++i)
         if (!(std::fabs(bp[i]) > 1e-14 * std::max(1.0, std::fabs(cur[p])))) brk = true;
-      if (brk || k >= dim[p] || ritz_error(ap, bp, k, cur[p]) <= tol * std::max(1.0, std::fabs(cur[p])) ||
-          k >= B->maxs) {
+      const double target = tol * std::max(1.0, std::fabs(cur[p]));
+      const double r = ritz_error(ap, bp, k, cur[p]);
+      if (brk || k >= dim[p] || r <= target || k >= B->maxs) {
         active[p] = 0;
       } else {
         any = true;
+        if (check_min > 0)
+          next_iv = std::min(next_iv, next_check(m_prev[p], r_prev[p], k, r, target, check_min, check_every));
+        m_prev[p] = k;
+        r_prev[p] = r;
       }
     }
     if (batch_set_active(B, active.data())) return -1;
     if (!any) break;
+    interval = next_iv;
   }
   if (batch_get(B, nullptr, nullptr, steps.data())) return -1;
   for (int p = 0; p < V; ++p) {

@@ -617,8 +617,13 @@ int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int
   int rc = lanczos_start(L, v0_dev, (hipStream_t)stream);
   double cur = 0.0;
   std::vector<double> a(max_steps), b(max_steps);
+  // tests placed by the residual's decay when steps are long (N >= 2^20);
+  // for small N a test's host round trip outweighs the steps it saves
+  const bool adaptive = op->op.basis->n >= ((int64_t)1 << 20);
+  int interval = check_every, m_prev = 0;
+  double r_prev = 0.0;
   while (!rc && lanczos_steps_done(L) < max_steps) {
-    rc = lanczos_run(L, check_every, reorth, 1);
+    rc = lanczos_run(L, std::min(interval, max_steps - lanczos_steps_done(L)), reorth, 1);
     if (rc) break;
     const int k = lanczos_steps_done(L);
     rc = lanczos_get(L, a.data(), b.data());
@@ -627,7 +632,12 @@ int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int
     bool breakdown = false;
     for (int i = 0; i < k; ++i)
       if (!(std::fabs(b[i]) > 1e-14 * std::max(1.0, std::fabs(cur)))) breakdown = true;
-    if (breakdown || ritz_error(a.data(), b.data(), k, cur) <= tol * std::max(1.0, std::fabs(cur))) break;
+    const double target = tol * std::max(1.0, std::fabs(cur));
+    const double r = ritz_error(a.data(), b.data(), k, cur);
+    if (breakdown || r <= target) break;
+    if (adaptive) interval = next_check(m_prev, r_prev, k, r, target, 1, check_every);
+    m_prev = k;
+    r_prev = r;
   }
   const int k = lanczos_steps_done(L);
   if (!rc) {

@@ -19,4 +19,8 @@ double tridiag_lowest(const double* a, const double* b, int k);
 double tridiag_eig(const double* a, const double* b, int k, int idx);
 // residual norm r of the lowest Ritz pair (b[k-1] = beta_k); *quad = min(r, r^2/gap)
 double ritz_error(const double* a, const double* b, int k, double theta, double* quad = nullptr);
+// steps to run before the next stopping test, given the residuals r0, r1 of
+// the tests after m0 < m1 steps: where the geometric decay between them
+// reaches target, clamped to [lo, hi]; hi when r is not decreasing
+int next_check(int m0, double r0, int m1, double r1, double target, int lo, int hi);
 }  // namespace qes

@@ -607,6 +607,16 @@ double tridiag_eig(const double* a, const double* b, int k, int idx) {
 
 double tridiag_lowest(const double* a, const double* b, int k) { return tridiag_eig(a, b, k, 0); }
 
+int next_check(int m0, double r0, int m1, double r1, double target, int lo, int hi) {
+  if (lo < 1) lo = 1;
+  if (hi < lo) hi = lo;
+  if (!(m1 > m0 && r0 > 0.0 && r1 > 0.0 && r1 < r0 && target > 0.0 && r1 > target)) return hi;
+  const double rate = (std::log(r1) - std::log(r0)) / (double)(m1 - m0);   // < 0
+  const double need = std::ceil((std::log(target) - std::log(r1)) / rate);
+  if (!(need < (double)hi)) return hi;
+  return need < (double)lo ? lo : (int)need;
+}
+
 // Residual norm r = ||H y - theta y|| = |beta_k s_k| of the lowest Ritz pair
 // after k Lanczos steps (b[k-1] = beta_k, the coefficient of the next basis
 // vector; s = unit eigenvector of T_k for theta).  |theta - lambda| <= r for

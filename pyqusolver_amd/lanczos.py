@@ -142,14 +142,35 @@ def _k_residuals(alpha, beta, k: int):
     return w[:kk], np.abs(float(beta[m - 1]) * S[-1, :kk])
 
 
+def next_check(m0: int, r0: float, m1: int, r1: float, target: float, lo: int, hi: int) -> int:
+    """Steps to run before the next stopping test: where the geometric decay
+    of the residual between the tests after m0 < m1 steps (r0, r1) reaches
+    target, clamped to [lo, hi]; hi while r is not decreasing.  Once the
+    lowest Ritz pair is isolated its residual falls about geometrically, so
+    the test lands near the step where it first passes instead of up to
+    hi - 1 steps after it."""
+    lo = max(1, int(lo))
+    hi = max(lo, int(hi))
+    if not (m1 > m0 and r0 > 0.0 and r1 > 0.0 and r1 < r0 and target > 0.0 and r1 > target):
+        return hi
+    rate = (math.log(r1) - math.log(r0)) / (m1 - m0)
+    need = math.ceil((math.log(target) - math.log(r1)) / rate)
+    return int(min(hi, max(lo, need)))
+
+
 def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300, v0=None, seed: int = 0,
                          reorth: bool = True, store_basis: Optional[bool] = None, compute_eigvec: bool = False,
-                         check_every: int = 10, use_graph: bool = True) -> LanczosResult:
+                         check_every: int = 10, use_graph: bool = True,
+                         adaptive: Optional[bool] = None) -> LanczosResult:
     """Lowest eigenvalue(s) by device Lanczos.  Stops when the Ritz residual
     |beta_m s_m| of the lowest pair is <= tol*max(1,|theta|); with k > 1 and
     the basis kept (reorthogonalised), all k lowest pairs must pass.  A
     single start vector finds one copy of a degenerate eigenvalue; without
-    reorthogonalisation only the lowest value is reliable.  With
+    reorthogonalisation only the lowest value is reliable.  Tests are at
+    most ``check_every`` steps apart; the next one goes where the residual's
+    decay between the last two predicts the crossing (``next_check``;
+    ``adaptive=False``: every ``check_every`` steps; None: adaptive for
+    N >= 2^20, where a step outlasts a test's host round trip).  With
     ``compute_eigvec`` the Ritz vectors are formed on the device: shape (n,)
     for k = 1, (n, k) otherwise; without a stored basis (store_basis=False)
     by a second run of the recurrence from the same start vector."""
@@ -163,6 +184,8 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
         v0t = torch.from_numpy(np.ascontiguousarray(v0)).to("cuda")
     v0t = v0t.to(torch.complex128 if complex_ else torch.float64).contiguous()
     maxiter = int(min(maxiter, max(n, 1)))
+    if adaptive is None:
+        adaptive = n >= (1 << 20)
     if store_basis is None:
         store_basis = bool(reorth or compute_eigvec)
     reorth = (2 if reorth in (2, "twice") else bool(reorth)) if store_basis else False
@@ -172,9 +195,10 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
     hist = []
     converged = False
     a = b = np.zeros(0)
+    interval, m_prev, r_prev = check_every, 0, 0.0
     while True:
         a_done = len(a)
-        nsteps = min(check_every, maxiter - a_done)
+        nsteps = min(interval, maxiter - a_done)
         if nsteps <= 0:
             break
         L.run(nsteps, reorth, use_graph)
@@ -190,14 +214,23 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
             converged = True
             break
         if k > 1 and reorth:
-            if len(a) >= k:
-                wk, rk = _k_residuals(a, b, k)
-                if np.all(rk <= tol * np.maximum(1.0, np.abs(wk))):
-                    converged = True
-                    break
-        elif ritz_residual(a, b)[0] <= tol * scale:
-            converged = True
-            break
+            if len(a) < k:
+                continue
+            wk, rk = _k_residuals(a, b, k)
+            tgt = tol * np.maximum(1.0, np.abs(wk))
+            if np.all(rk <= tgt):
+                converged = True
+                break
+            worst = int(np.argmax(rk / tgt))
+            r_now, target = float(rk[worst]), float(tgt[worst])
+        else:
+            r_now, target = ritz_residual(a, b)[0], tol * scale
+            if r_now <= target:
+                converged = True
+                break
+        if adaptive:
+            interval = next_check(m_prev, r_prev, len(a), r_now, target, 1, check_every)
+        m_prev, r_prev = len(a), r_now
     torch.cuda.synchronize()
     w, V = tridiag_eigs(a, b)
     kk = min(k, len(a))

@@ -130,3 +130,27 @@ def test_breakdown_and_one_step_cases():
     w, S = np.linalg.eigh(T)
     assert th == pytest.approx(w[0])
     assert r == pytest.approx(1e-3 * abs(S[-1, 0]), rel=1e-8)
+
+
+def test_next_check_interval():
+    """The next stopping test goes where the geometric decay between the last
+    two tests reaches the target, within [lo, hi]."""
+    from pyqusolver_amd.lanczos import next_check
+    # r falls by 10x per 10 steps: from 1e-8 at step 90, 1e-10 is 20 steps on
+    assert next_check(80, 1e-7, 90, 1e-8, 1e-10, 1, 10) == 10
+    assert next_check(80, 1e-7, 90, 1e-8, 1e-10, 1, 30) == 20
+    # 1e-9.5 is 5 steps on (ceil of the exact 5.0 up to rounding)
+    assert next_check(80, 1e-7, 90, 1e-8, 10 ** -8.5, 1, 10) in (5, 6)
+    assert next_check(80, 1e-7, 90, 1e-8, 9.99e-9, 1, 10) == 1
+    assert next_check(80, 1e-7, 90, 1e-8, 9.99e-9, 3, 10) == 3
+    # not decreasing, no history, or already below target: the longest interval
+    assert next_check(80, 1e-8, 90, 1e-8, 1e-10, 1, 10) == 10
+    assert next_check(80, 1e-9, 90, 1e-8, 1e-10, 1, 10) == 10
+    assert next_check(0, 0.0, 10, 1e-3, 1e-10, 1, 10) == 10
+    assert next_check(80, 1e-7, 90, 1e-11, 1e-10, 1, 10) == 10
+    # along an exact geometric sequence the predicted step is the first that passes
+    rho, target = 0.7, 1e-10
+    r = lambda m: 3.0 * rho ** m
+    first = next(m for m in range(1000) if r(m) <= target)
+    m0, m1 = first - 25, first - 15
+    assert m1 + next_check(m0, r(m0), m1, r(m1), target, 1, 100) == first
