@@ -480,14 +480,29 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
 #define QES_ASM_COUNT(CK) QES_ASM(CK, PASS_COUNT)
 #define QES_ASM_DICT(CK) QES_ASM(CK, PASS_DICT)
 #define QES_ASM_FILL(CK) QES_ASM(CK, PASS_FILL)
+  // scratch buffers of the passes, freed on every return path (the table
+  // itself lives in op and is freed by the caller when a pass fails)
+  struct Scratch {
+    void* p = nullptr;
+    hipStream_t st = nullptr;
+    ~Scratch() {
+      if (p) {
+        (void)hipStreamSynchronize(st);   // no pass may still be using it
+        hipFree(p);
+      }
+    }
+  } s_rowlen, s_dict;
+  s_rowlen.st = s_dict.st = st;
   // pass A: row lengths (diagonal included)
-  QES_HIP(hipMalloc(&O.rowlen, sizeof(int32_t) * n));
+  QES_HIP(hipMalloc(&s_rowlen.p, sizeof(int32_t) * n));
+  O.rowlen = (int32_t*)s_rowlen.p;
   QES_ASM_CK_SWITCH(P.canon, QES_ASM_COUNT)
   QES_HIP(hipGetLastError());
   std::vector<int32_t> rowlen(n);
   QES_HIP(hipMemcpyAsync(rowlen.data(), O.rowlen, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
   QES_HIP(hipStreamSynchronize(st));
-  hipFree(O.rowlen);
+  hipFree(s_rowlen.p);   // before the table is allocated
+  s_rowlen.p = nullptr;
   O.rowlen = nullptr;
 
   // pass B: value dictionary of the off-diagonal elements
@@ -498,7 +513,8 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
   const size_t nslot = size_t(1) << DICT_HBITS;
   void* dbuf = nullptr;
   if (want_dict) {
-    QES_HIP(hipMalloc(&dbuf, nslot * (8 + 8 + 8 + 4 + 2) + 64));
+    QES_HIP(hipMalloc(&s_dict.p, nslot * (8 + 8 + 8 + 4 + 2) + 64));
+    dbuf = s_dict.p;
     char* p = (char*)dbuf;
     O.D.key = (unsigned long long*)p;
     O.D.vr = (double*)(p + nslot * 8);
@@ -547,7 +563,6 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
       }
     }
     if (fmt == ASM_PLAIN && tagged) {
-      hipFree(dbuf);
       return fail("the operator's values do not fit a dictionary: no coupling-tagged table");
     }
   }
@@ -571,7 +586,6 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
   const double diag_bytes = fmt == ASM_PLAIN ? 0.0 : 8.0 * cw * n * (1 + (tagged ? op->n_coupling : 0));
   const double bytes = (double)acc * per_entry + diag_bytes + 12.0 * ns;
   if (bytes > op->asm_max_bytes) {
-    if (dbuf) hipFree(dbuf);
     return fail("assembled table would take " + std::to_string(bytes / 1e9) + " GB (limit " +
                 std::to_string(op->asm_max_bytes / 1e9) + " GB)");
   }
@@ -623,7 +637,6 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
 #undef QES_ASM_DICT
 #undef QES_ASM_FILL
 #undef QES_ASM
-  if (dbuf) hipFree(dbuf);
   if (bad) return fail("dictionary fingerprint collision");   // caller retries without a dictionary
   op->asm_entries = acc;
   op->asm_slices = ns;

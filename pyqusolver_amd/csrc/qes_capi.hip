@@ -22,6 +22,9 @@ int fail(const std::string& msg) {
 }
 int check_hip(hipError_t e, const char* what) {
   g_err = std::string(what) + ": " + hipGetErrorString(e);
+  // a failed allocation also sets the runtime's last-error state; clear it
+  // so that the next launch check of an unrelated call does not report it
+  (void)hipGetLastError();
   return -1;
 }
 
@@ -206,7 +209,7 @@ const char* qes_last_error(void) { return g_err.c_str(); }
 const char* qes_version(void) { return "qes_mi 0.1 gfx950"; }
 
 int qes_basis_create(const qes_group_spec* spec, qes_basis_t* out, int64_t* n_out) {
-  if (!spec || !out) return fail("null argument");
+  if (!spec || !out || !spec->chi_re || !spec->chi_im) return fail("null argument");
   if (spec->ns < 1 || spec->ns > 64) return fail("ns must be in 1..64");
   if (spec->nbase < 1 || spec->nbase > QES_MAX_BASE) return fail("nbase must be in 1..4");
   if (spec->ntx < 1 || spec->nty < 1 || spec->ntx > QES_MAX_TX) return fail("bad translation extents");
@@ -280,18 +283,20 @@ int qes_basis_device_ptrs(qes_basis_t h, const uint64_t** reps_dev, const double
 int qes_basis_map_states(qes_basis_t h, const uint64_t* states_host, int64_t n, int64_t* idx_out, int32_t* g_out) {
   if (!h) return fail("null basis");
   if (n <= 0) return 0;
+  if (!states_host || !idx_out) return fail("null argument");
   uint64_t* ds = nullptr;
   int64_t* di = nullptr;
   int32_t* dg = nullptr;
-  QES_HIP(hipMalloc(&ds, n * 8));
-  QES_HIP(hipMalloc(&di, n * 8));
-  QES_HIP(hipMalloc(&dg, n * 4));
-  QES_HIP(hipMemcpy(ds, states_host, n * 8, hipMemcpyHostToDevice));
-  int rc = map_states(&h->b, ds, n, di, dg, nullptr, nullptr);
+  hipError_t e = hipMalloc(&ds, n * 8);
+  if (e == hipSuccess) e = hipMalloc(&di, n * 8);
+  if (e == hipSuccess) e = hipMalloc(&dg, n * 4);
+  if (e == hipSuccess) e = hipMemcpy(ds, states_host, n * 8, hipMemcpyHostToDevice);
+  int rc = e == hipSuccess ? map_states(&h->b, ds, n, di, dg, nullptr, nullptr) : check_hip(e, "qes_basis_map_states");
   if (!rc) {
-    QES_HIP(hipDeviceSynchronize());
-    QES_HIP(hipMemcpy(idx_out, di, n * 8, hipMemcpyDeviceToHost));
-    if (g_out) QES_HIP(hipMemcpy(g_out, dg, n * 4, hipMemcpyDeviceToHost));
+    e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(idx_out, di, n * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && g_out) e = hipMemcpy(g_out, dg, n * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = check_hip(e, "qes_basis_map_states");
   }
   hipFree(ds);
   hipFree(di);
@@ -488,10 +493,12 @@ int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_ho
     *mismatches = 0;
     return 0;
   }
+  if (!states_host) return fail("null argument");
   uint64_t* ds = nullptr;
   QES_HIP(hipMalloc(&ds, n * 8));
-  QES_HIP(hipMemcpy(ds, states_host, n * 8, hipMemcpyHostToDevice));
-  int rc = canon_check(&b->b, kind, nullptr, ds, n, mismatches, nullptr);
+  const hipError_t e = hipMemcpy(ds, states_host, n * 8, hipMemcpyHostToDevice);
+  const int rc = e == hipSuccess ? canon_check(&b->b, kind, nullptr, ds, n, mismatches, nullptr)
+                                 : check_hip(e, "qes_basis_canon_check");
   hipFree(ds);
   return rc;
 }

@@ -428,20 +428,34 @@ int lanczos_create(Op* op, int max_steps, int store, int dtype, Lanczos** out) {
   L->nred = grid_for(L->n);
   L->nchunk = (int)std::min<int64_t>(std::max<int64_t>((L->n + RB - 1) / RB, 1), 512);
   L->nvec_blocks = (L->maxv + VB - 1) / VB;
-  QES_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
-  QES_HIP(hipMalloc(&L->v, len * 8));
-  QES_HIP(hipMalloc(&L->vp, len * 8));
-  QES_HIP(hipMalloc(&L->w, len * 8));
-  if (store) QES_HIP(hipMalloc(&L->V, (size_t)std::max<int64_t>(L->ldv, 2) * 8 * L->maxv));
-  QES_HIP(hipMalloc(&L->alpha, sizeof(double) * L->maxv));
-  QES_HIP(hipMalloc(&L->beta, sizeof(double) * L->maxv));
-  QES_HIP(hipMalloc(&L->scal, sizeof(double) * 4));
-  QES_HIP(hipMalloc(&L->part, sizeof(double) * 2 * L->nred));
-  QES_HIP(hipMalloc(&L->coef, sizeof(double) * 2 * L->maxv));
-  QES_HIP(hipMalloc(&L->ppart, sizeof(double) * 2 * (size_t)L->nchunk * L->nvec_blocks * VB));
-  QES_HIP(hipMalloc(&L->d_j, sizeof(int)));
-  QES_HIP(hipMalloc(&L->dg, sizeof(double) * 4));
-  QES_HIP(hipMalloc(&L->flag, sizeof(int) * 2));
+  const size_t vbytes = store ? (size_t)std::max<int64_t>(L->ldv, 2) * 8 * L->maxv : 0;
+  hipError_t e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
+  const char* what = "stream";
+  auto grab = [&](auto** p, size_t bytes, const char* name) {
+    if (e == hipSuccess) {
+      e = hipMalloc((void**)p, bytes);
+      what = name;
+    }
+  };
+  grab(&L->v, len * 8, "Lanczos vectors");
+  grab(&L->vp, len * 8, "Lanczos vectors");
+  grab(&L->w, len * 8, "Lanczos vectors");
+  if (store) grab(&L->V, vbytes, "stored Krylov basis");
+  grab(&L->alpha, sizeof(double) * L->maxv, "coefficients");
+  grab(&L->beta, sizeof(double) * L->maxv, "coefficients");
+  grab(&L->scal, sizeof(double) * 4, "scalars");
+  grab(&L->part, sizeof(double) * 2 * L->nred, "partial sums");
+  grab(&L->coef, sizeof(double) * 2 * L->maxv, "coefficients");
+  grab(&L->ppart, sizeof(double) * 2 * (size_t)L->nchunk * L->nvec_blocks * VB, "partial sums");
+  grab(&L->d_j, sizeof(int), "scalars");
+  grab(&L->dg, sizeof(double) * 4, "scalars");
+  grab(&L->flag, sizeof(int) * 2, "scalars");
+  if (e != hipSuccess) {   // nothing of the handle survives a failed create
+    lanczos_destroy(L);
+    std::string msg = std::string("lanczos_create (") + what;
+    if (store) msg += "; the stored basis alone takes " + std::to_string(vbytes / 1e9) + " GB";
+    return check_hip(e, (msg + ")").c_str());
+  }
   *out = L;
   return 0;
 }

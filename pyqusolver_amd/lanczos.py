@@ -221,7 +221,8 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
             if np.all(rk <= tgt):
                 converged = True
                 break
-            worst = int(np.argmax(rk / tgt))
+            with np.errstate(divide="ignore", invalid="ignore"):   # tol = 0: no pair ever passes
+                worst = int(np.argmax(np.where(tgt > 0, rk / tgt, np.inf)))
             r_now, target = float(rk[worst]), float(tgt[worst])
         else:
             r_now, target = ritz_residual(a, b)[0], tol * scale

@@ -337,6 +337,16 @@ def test_square6x6_scale_properties():
     H.assemble(True, max_gb=64)
     assert torch.equal(H.matvec_device(x), y)
     H.assemble(False)
+    # a stored basis of 1024 vectors (1 TB) cannot be allocated: the create
+    # fails with a message naming it, frees what it had taken, and leaves no
+    # error behind for the next call
+    from pyqusolver_amd import _lib
+    from pyqusolver_amd.lanczos import DeviceLanczos
+    free0 = torch.cuda.mem_get_info()[0]
+    with pytest.raises(_lib.QesError, match="stored Krylov basis"):
+        DeviceLanczos(H, 1023, True, False)
+    assert torch.cuda.mem_get_info()[0] >= free0 - (64 << 20)
+    assert torch.equal(H.matvec_device(x), y)
 
 
 @pytest.mark.parametrize("which", ["honeycomb3x4_hk", "chain24_xxz_k5"])
