@@ -8,6 +8,12 @@
  * vectors, and hipStream_t passed as void*.  Every function returns 0 on
  * success and -1 on error; qes_last_error() then holds a message.
  *
+ * Handles borrow what they are built from: an operator keeps a pointer to its
+ * basis, a Lanczos handle or batch to its operators (and their assembled
+ * tables), so destroy them in the reverse order of creation.  Device vectors
+ * passed in are borrowed for the call (its work is ordered on the stream
+ * argument where there is one) and are never freed by the library.
+ *
  * Reference interfaces each entry point replaces are cited as
  * file:line relative to the reference package root Python/QES/.
  */
