@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import warnings
 import time
 from dataclasses import dataclass, field
 from typing import Optional
@@ -134,12 +135,37 @@ def ritz_residual(alpha, beta):
     return r, (min(r, r * r / gap) if gap > 0 else r)
 
 
-def _k_residuals(alpha, beta, k: int):
-    """Residual norms |beta_m S[m-1, i]| of the k lowest Ritz pairs and their values."""
+_WHICH = {"smallest": "SA", "sa": "SA", "largest": "LA", "la": "LA", "both": "BE", "be": "BE"}
+
+
+def which_code(which: str) -> str:
+    """Normalised spectrum end: 'SA' (smallest algebraic, the default), 'LA'
+    (largest) or 'BE' (both ends) -- the reference's which='smallest' /
+    'largest' / 'both' (hamil_diag_engine.py:183) and eigsh's codes."""
+    code = _WHICH.get(str(which).lower())
+    if code is None:
+        raise ValueError(f"which={which!r}: the Lanczos path computes 'smallest', 'largest' or 'both'")
+    return code
+
+
+def select_ritz(m: int, k: int, which: str = "SA") -> np.ndarray:
+    """Indices (ascending) of the k wanted Ritz values among m sorted ones:
+    the k lowest (SA), the k highest (LA), or k//2 lowest and the rest
+    highest (BE, one more from the top when k is odd, as eigsh does)."""
+    k = max(0, min(int(k), m))
+    if which == "SA":
+        return np.arange(k)
+    if which == "LA":
+        return np.arange(m - k, m)
+    lo = k // 2
+    return np.concatenate([np.arange(lo), np.arange(m - (k - lo), m)])
+
+
+def _pair_residuals(alpha, beta, idx):
+    """Ritz values and residual norms |beta_m S[m-1, i]| of the pairs idx."""
     w, S = tridiag_eigs(alpha, beta)
     m = len(alpha)
-    kk = min(k, m)
-    return w[:kk], np.abs(float(beta[m - 1]) * S[-1, :kk])
+    return w[idx], np.abs(float(beta[m - 1]) * S[-1, idx])
 
 
 def next_check(m0: int, r0: float, m1: int, r1: float, target: float, lo: int, hi: int) -> int:
@@ -161,12 +187,15 @@ def next_check(m0: int, r0: float, m1: int, r1: float, target: float, lo: int, h
 def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300, v0=None, seed: int = 0,
                          reorth: bool = True, store_basis: Optional[bool] = None, compute_eigvec: bool = False,
                          check_every: int = 10, use_graph: bool = True,
-                         adaptive: Optional[bool] = None) -> LanczosResult:
-    """Lowest eigenvalue(s) by device Lanczos.  Stops when the Ritz residual
-    |beta_m s_m| of the lowest pair is <= tol*max(1,|theta|); with k > 1 and
-    the basis kept (reorthogonalised), all k lowest pairs must pass.  A
-    single start vector finds one copy of a degenerate eigenvalue; without
-    reorthogonalisation only the lowest value is reliable.  Tests are at
+                         adaptive: Optional[bool] = None, which: str = "smallest") -> LanczosResult:
+    """Extreme eigenvalue(s) by device Lanczos, the lowest by default.  Stops
+    when the Ritz residual |beta_m s_m| of the extreme pair is <=
+    tol*max(1,|theta|); with k > 1 and the basis kept (reorthogonalised), all
+    k wanted pairs must pass.  ``which``: 'smallest' (k lowest), 'largest'
+    (k highest) or 'both' (k//2 lowest, the rest highest); eigenvalues come
+    back in ascending order.  A single start vector finds one copy of a
+    degenerate eigenvalue; without reorthogonalisation only the extreme
+    value(s) at the wanted end(s) are reliable.  Tests are at
     most ``check_every`` steps apart; the next one goes where the residual's
     decay between the last two predicts the crossing (``next_check``;
     ``adaptive=False``: every ``check_every`` steps; None: adaptive for
@@ -176,6 +205,7 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
     by a second run of the recurrence from the same start vector."""
     n = ham.nh
     complex_ = bool(ham.needs_complex)
+    code = which_code(which)
     if v0 is None:
         v0 = start_vector(n, complex_, seed)
     if isinstance(v0, torch.Tensor):
@@ -189,6 +219,9 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
     if store_basis is None:
         store_basis = bool(reorth or compute_eigvec)
     reorth = (2 if reorth in (2, "twice") else bool(reorth)) if store_basis else False
+    if k > (2 if code == "BE" else 1) and not reorth:
+        warnings.warn("without reorthogonalisation only the extreme Ritz value(s) are converged; "
+                      "the other values returned are not", stacklevel=2)
     t0 = time.perf_counter()
     L = DeviceLanczos(ham, maxiter, store_basis, complex_)
     L.start(v0t)
@@ -213,10 +246,20 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
             w, _ = tridiag_eigs(a, b)
             converged = True
             break
-        if k > 1 and reorth:
-            if len(a) < k:
-                continue
-            wk, rk = _k_residuals(a, b, k)
+        if code == "SA" and not (k > 1 and reorth):
+            r_now, target = ritz_residual(a, b)[0], tol * scale
+            if r_now <= target:
+                converged = True
+                break
+        else:
+            m = len(a)
+            if k > 1 and reorth:
+                if m < k:
+                    continue
+                idx = select_ritz(m, k, code)
+            else:   # no stored basis: the extreme pair(s) of the wanted end(s)
+                idx = select_ritz(m, 1 if code != "BE" else 2, code)
+            wk, rk = _pair_residuals(a, b, idx)
             tgt = tol * np.maximum(1.0, np.abs(wk))
             if np.all(rk <= tgt):
                 converged = True
@@ -224,50 +267,47 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
             with np.errstate(divide="ignore", invalid="ignore"):   # tol = 0: no pair ever passes
                 worst = int(np.argmax(np.where(tgt > 0, rk / tgt, np.inf)))
             r_now, target = float(rk[worst]), float(tgt[worst])
-        else:
-            r_now, target = ritz_residual(a, b)[0], tol * scale
-            if r_now <= target:
-                converged = True
-                break
         if adaptive:
             interval = next_check(m_prev, r_prev, len(a), r_now, target, 1, check_every)
         m_prev, r_prev = len(a), r_now
     torch.cuda.synchronize()
     w, V = tridiag_eigs(a, b)
-    kk = min(k, len(a))
-    res_bound = float(np.max(np.abs(b[len(a) - 1] * V[-1, :kk]))) if len(a) else float("nan")
+    sel = select_ritz(len(a), k, code)     # wanted pairs, ascending
+    kk = len(sel)
+    res_bound = float(np.max(np.abs(b[len(a) - 1] * V[-1, sel]))) if kk else float("nan")
     vecs = None
     if compute_eigvec and not store_basis:
         # second pass from the same start vector: the steps are bitwise
         # reproducible, so v_j is regenerated and sum_j S[j, i] v_j formed
         # without keeping the basis (memory: four vectors + the k outputs)
         m = len(a)
-        kk = min(k, m, 16)
+        if kk > 16:
+            raise ValueError("Ritz vectors without a stored basis: at most 16 (keep the basis for more)")
         second = L.second_passes if reorth else 0
         L.close()
         L = DeviceLanczos(ham, max(m, 1), False, complex_)
         L.start(v0t)
         cols = torch.zeros((kk, n), dtype=v0t.dtype, device="cuda")
-        L.accumulate(V[0, :kk], cols)
+        L.accumulate(V[0, sel], cols)
         for j in range(1, m):
             L.run(1, False, use_graph)
-            L.accumulate(V[j, :kk], cols)
+            L.accumulate(V[j, sel], cols)
         vecs = cols[0].clone() if k == 1 else cols.transpose(0, 1).contiguous()
         L.close()
-        return LanczosResult(eigenvalues=w[:k].copy(), alpha=a, beta=b, steps=len(a), converged=converged,
+        return LanczosResult(eigenvalues=w[sel].copy(), alpha=a, beta=b, steps=len(a), converged=converged,
                              history=hist, eigenvectors=vecs, seconds=time.perf_counter() - t0,
                              residual_bound=res_bound, second_passes=second)
     if compute_eigvec:
         if k == 1:
             vecs = torch.empty((n,), dtype=v0t.dtype, device="cuda")
-            L.ritz(V[:, 0], vecs)
+            L.ritz(V[:, sel[0]], vecs)
         else:
             cols = torch.empty((kk, n), dtype=v0t.dtype, device="cuda")
-            for i in range(kk):
-                L.ritz(V[:, i], cols[i])
+            for i, s in enumerate(sel):
+                L.ritz(V[:, s], cols[i])
             vecs = cols.transpose(0, 1).contiguous()
     second = L.second_passes if reorth else 0
     L.close()
-    return LanczosResult(eigenvalues=w[:k].copy(), alpha=a, beta=b, steps=len(a), converged=converged,
+    return LanczosResult(eigenvalues=w[sel].copy(), alpha=a, beta=b, steps=len(a), converged=converged,
                          history=hist, eigenvectors=vecs, seconds=time.perf_counter() - t0,
                          residual_bound=res_bound, second_passes=second)

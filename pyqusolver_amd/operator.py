@@ -498,22 +498,33 @@ class Hamiltonian:
         return self.as_linear_operator()
 
     # ---- diagonalisation ------------------------------------------------
-    def diagonalize(self, method: str = "lanczos", k: int = 1, tol: float = 1e-10, maxiter: int = 300,
+    def diagonalize(self, method: str = "lanczos", k: Optional[int] = None, tol: float = 1e-10, maxiter: int = 300,
                     v0=None, seed: int = 0, reorth: bool = True, store_basis: Optional[bool] = None,
                     compute_eigvec: bool = False, check_every: int = 10, assemble: Optional[bool] = None,
-                    **kwargs):
-        """Lowest eigenvalue(s) by device Lanczos (hamil.py:1809-2114 with
-        method='lanczos').  With the basis kept (reorth, the default) all k
-        lowest Ritz pairs are converged to the residual test; without it only
-        E0 is.  One start vector finds one copy of a degenerate level.
+                    which: str = "smallest", **kwargs):
+        """Extreme eigenvalue(s) by device Lanczos (hamil.py:1809-2114 with
+        method='lanczos'; k defaults to 6 and which to 'smallest' as in
+        hamil_diag_engine.py:183,254-256; 'largest' and 'both' as there).
+        With the basis kept (reorth, the default) all k wanted Ritz pairs are
+        converged to the residual test; without it only the extreme one(s).
+        One start vector finds one copy of a degenerate level.
         ``compute_eigvec``: eig_vec is (n,) for k = 1, (n, k) otherwise.
+        Keyword arguments of the reference's other solvers are accepted and
+        ignored, except those that would change the result: ``sigma``
+        (shift-invert) and ``hermitian=False`` raise.
 
         ``assemble``: tabulate H for the solve (see ``assemble``); None picks
         it when the table fits in a quarter of the free device memory (at
         most 64 GB) and N >= 65536.  A table built here is freed afterwards."""
-        from .lanczos import lanczos_ground_state
+        from .lanczos import lanczos_ground_state, which_code
         if method not in ("lanczos", "auto"):
             raise NotImplementedError("only method='lanczos' is implemented on the device")
+        if kwargs.get("sigma") is not None:
+            raise NotImplementedError("shift-invert (sigma) is not part of the device Lanczos path")
+        if not kwargs.get("hermitian", True):
+            raise NotImplementedError("the device Lanczos path is for Hermitian operators")
+        which_code(which)
+        k = 6 if k is None else int(k)
         if self.nh == 0:
             self._eig_val = np.zeros(0)
             return self._eig_val
@@ -529,7 +540,7 @@ class Hamiltonian:
         try:
             res = lanczos_ground_state(self, k=k, tol=tol, maxiter=maxiter, v0=v0, seed=seed, reorth=reorth,
                                        store_basis=store_basis, compute_eigvec=compute_eigvec,
-                                       check_every=check_every)
+                                       check_every=check_every, which=which)
         finally:
             if built:
                 self.assemble(False)

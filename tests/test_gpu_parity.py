@@ -684,6 +684,33 @@ def test_lowest_k_eigenpairs_with_reorthogonalisation():
         assert torch.linalg.vector_norm(r).item() <= 1e-8 * torch.linalg.vector_norm(v).item()
 
 
+@pytest.mark.parametrize("which", ["largest", "both"])
+def test_largest_and_both_ends(which):
+    """which='largest' / 'both' (the reference's diagonalize options,
+    hamil_diag_engine.py:183): the k highest, or k//2 lowest and the rest
+    highest, eigenvalues of the dense reduced matrix to 1e-10, ascending,
+    with Ritz vectors that satisfy H v = E v; the default k is 6."""
+    import torch
+    hs, H, grp, b, t = build_pair("chain", 14, 1, [("tx", 0), ("R", 1), ("Px", -1)], 7, "j1j2")
+    c, s, co = t.arrays()
+    ev = np.linalg.eigvalsh(O.reduced_dense(b, 14, c, s, co, True))
+    want = ev[-6:] if which == "largest" else np.concatenate([ev[:3], ev[-3:]])
+    assert np.min(np.diff(want)) > 1e-3             # separated levels at both ends here
+    w = H.diagonalize(which=which, tol=1e-11, maxiter=min(300, hs.nh), compute_eigvec=True)
+    assert len(w) == 6
+    np.testing.assert_allclose(w, want, rtol=0, atol=1e-10 * np.max(np.abs(ev)))
+    assert H.last_lanczos.converged
+    V = H.eig_vec
+    for i in range(6):
+        v = V[:, i].contiguous()
+        r = H.matvec_device(v) - float(w[i]) * v
+        assert torch.linalg.vector_norm(r).item() <= 1e-8 * torch.linalg.vector_norm(v).item()
+    with pytest.raises(ValueError):
+        H.diagonalize(which="LM")
+    with pytest.raises(NotImplementedError):
+        H.diagonalize(sigma=0.5)
+
+
 def test_two_pass_ritz_vector_without_stored_basis():
     """Ritz vectors without keeping the Krylov basis: the recurrence is run
     a second time from the same start vector (bitwise reproducible steps)

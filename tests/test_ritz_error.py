@@ -154,3 +154,21 @@ def test_next_check_interval():
     first = next(m for m in range(1000) if r(m) <= target)
     m0, m1 = first - 25, first - 15
     assert m1 + next_check(m0, r(m0), m1, r(m1), target, 1, 100) == first
+
+
+def test_ritz_selection_by_spectrum_end():
+    """which='smallest'/'largest'/'both' select the k lowest, k highest, or
+    k//2 lowest and the rest highest Ritz values (eigsh's SA/LA/BE), in
+    ascending order; unknown ends raise."""
+    from pyqusolver_amd.lanczos import select_ritz, which_code
+    assert [which_code(w) for w in ("smallest", "SA", "largest", "la", "both", "BE")] == \
+        ["SA", "SA", "LA", "LA", "BE", "BE"]
+    with pytest.raises(ValueError):
+        which_code("LM")
+    assert select_ritz(10, 3, "SA").tolist() == [0, 1, 2]
+    assert select_ritz(10, 3, "LA").tolist() == [7, 8, 9]
+    assert select_ritz(10, 3, "BE").tolist() == [0, 8, 9]
+    assert select_ritz(10, 4, "BE").tolist() == [0, 1, 8, 9]
+    assert select_ritz(2, 6, "SA").tolist() == [0, 1]        # k clipped to the Krylov dimension
+    assert select_ritz(3, 6, "BE").tolist() == [0, 1, 2]
+    assert select_ritz(5, 1, "BE").tolist() == [4]
