@@ -498,27 +498,39 @@ class Hamiltonian:
         return self.as_linear_operator()
 
     # ---- diagonalisation ------------------------------------------------
-    def diagonalize(self, method: str = "lanczos", k: Optional[int] = None, tol: float = 1e-10, maxiter: int = 300,
+    EXACT_AUTO_MAX = 4096      # method='auto': dense below, Lanczos above
+
+    def diagonalize(self, method: str = "exact", k: Optional[int] = None, tol: float = 1e-10, maxiter: int = 300,
                     v0=None, seed: int = 0, reorth: bool = True, store_basis: Optional[bool] = None,
                     compute_eigvec: bool = False, check_every: int = 10, assemble: Optional[bool] = None,
                     which: str = "smallest", **kwargs):
-        """Extreme eigenvalue(s) by device Lanczos (hamil.py:1809-2114 with
-        method='lanczos'; k defaults to 6 and which to 'smallest' as in
-        hamil_diag_engine.py:183,254-256; 'largest' and 'both' as there).
-        With the basis kept (reorth, the default) all k wanted Ritz pairs are
-        converged to the residual test; without it only the extreme one(s).
-        One start vector finds one copy of a degenerate level.
-        ``compute_eigvec``: eig_vec is (n,) for k = 1, (n, k) otherwise.
-        Keyword arguments of the reference's other solvers are accepted and
-        ignored, except those that would change the result: ``sigma``
-        (shift-invert) and ``hermitian=False`` raise.
+        """Eigenvalues of the reduced H (hamil.py:1809-2114).
+
+        method='exact' (the reference's default): the whole spectrum of the
+        dense reduced matrix, built on the device from the assembled table
+        and diagonalised there (FP64 ``torch.linalg.eigh``); N^2 doubles
+        must fit in a quarter of the free device memory.  'auto': exact up
+        to N = 4096, Lanczos above.
+
+        method='lanczos': extreme eigenvalue(s) by device Lanczos, k
+        defaulting to 6 and which to 'smallest' as in hamil_diag_engine.py:
+        183,254-256 ('largest' and 'both' as there).  With the basis kept
+        (reorth, the default) all k wanted Ritz pairs are converged to the
+        residual test; without it only the extreme one(s).  One start vector
+        finds one copy of a degenerate level.
+
+        ``compute_eigvec``: eig_vec is (n,) for one Lanczos pair, (n, k)
+        otherwise, (n, n) for 'exact'.  Keyword arguments of the reference's
+        other solvers are accepted and ignored, except those that would
+        change the result: ``sigma`` (shift-invert) and ``hermitian=False``
+        raise.
 
         ``assemble``: tabulate H for the solve (see ``assemble``); None picks
         it when the table fits in a quarter of the free device memory (at
         most 64 GB) and N >= 65536.  A table built here is freed afterwards."""
         from .lanczos import lanczos_ground_state, which_code
-        if method not in ("lanczos", "auto"):
-            raise NotImplementedError("only method='lanczos' is implemented on the device")
+        if method not in ("lanczos", "exact", "auto"):
+            raise NotImplementedError(f"method={method!r}: 'exact', 'lanczos' and 'auto' run on the device")
         if kwargs.get("sigma") is not None:
             raise NotImplementedError("shift-invert (sigma) is not part of the device Lanczos path")
         if not kwargs.get("hermitian", True):
@@ -528,6 +540,8 @@ class Hamiltonian:
         if self.nh == 0:
             self._eig_val = np.zeros(0)
             return self._eig_val
+        if method == "exact" or (method == "auto" and self.nh <= self.EXACT_AUTO_MAX):
+            return self._diagonalize_exact(compute_eigvec)
         built = False
         if self.assembly_bytes == 0 and (assemble or (assemble is None and self.nh >= 65536)):
             free, _ = torch.cuda.mem_get_info()
@@ -548,6 +562,53 @@ class Hamiltonian:
         self.last_lanczos = res
         self._eig_val = res.eigenvalues
         self._eig_vec = res.eigenvectors
+        return self._eig_val
+
+    def dense_device(self) -> torch.Tensor:
+        """The reduced H as a dense (N, N) device matrix (float64 or
+        complex128), filled from the assembled table on the device."""
+        n = self.nh
+        cplx = bool(self.needs_complex)
+        itemsize = 16 if cplx else 8
+        free, _ = torch.cuda.mem_get_info()
+        if n * n * itemsize > 0.25 * free:
+            raise _lib.QesError(f"dense H of N={n} takes {n * n * itemsize / 1e9:.1f} GB, more than a quarter "
+                                f"of the free device memory; use method='lanczos'")
+        built = False
+        if self.assembly_bytes == 0:
+            self.assemble(True)
+            built = True
+        try:
+            L = _lib.lib()
+            cnt = C.c_int64()
+            _lib.check(L.qes_op_table_coo(self._device_op(), None, None, None, 0, C.byref(cnt), None),
+                       "qes_op_table_coo")
+            m = int(cnt.value)
+            rows = torch.full((m,), -1, dtype=torch.int32, device="cuda")
+            cols = torch.zeros((m,), dtype=torch.int32, device="cuda")
+            vals = torch.zeros((m, 2) if cplx else (m,), dtype=torch.float64, device="cuda")
+            _lib.check(L.qes_op_table_coo(self._device_op(), _lib.dptr(rows), _lib.dptr(cols), _lib.dptr(vals), m,
+                                          C.byref(cnt), _lib.stream_ptr()), "qes_op_table_coo")
+        finally:
+            if built:
+                self.assemble(False)
+        v = torch.view_as_complex(vals) if cplx else vals
+        keep = rows >= 0                       # padding slots carry row -1
+        A = torch.zeros((n, n), dtype=v.dtype, device="cuda")
+        A.index_put_((rows[keep].long(), cols[keep].long()), v[keep], accumulate=True)
+        return A
+
+    def _diagonalize_exact(self, compute_eigvec: bool):
+        A = self.dense_device()
+        if compute_eigvec:
+            w, V = torch.linalg.eigh(A)
+            self._eig_vec = V
+        else:
+            w = torch.linalg.eigvalsh(A)
+            self._eig_vec = None
+        del A
+        self._eig_val = w.cpu().numpy()
+        self.last_lanczos = None
         return self._eig_val
 
     @property

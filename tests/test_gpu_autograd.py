@@ -162,7 +162,7 @@ def test_structure_factor_continued_fraction_against_dense():
     hsq = HilbertSpace(lattice=lat, sym_gen={"translation": kq}, global_syms=[get_u1_sym(lat, 6)])
     H0 = J1J2Model(lat, J1=-1.0, J2=0.0, hilbert_space=hs0)
     Hq = J1J2Model(lat, J1=-1.0, J2=0.0, hilbert_space=hsq, dtype=np.complex128)
-    e0 = H0.diagonalize(k=1, tol=1e-14, maxiter=hs0.nh, compute_eigvec=True)[0]
+    e0 = H0.diagonalize(method="lanczos", k=1, tol=1e-14, maxiter=hs0.nh, compute_eigvec=True)[0]
     psi0 = H0.eig_vec.cpu().numpy().astype(np.complex128)
     psi0 /= np.linalg.norm(psi0)
     q = 2 * np.pi * kq / L

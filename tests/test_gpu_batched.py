@@ -258,7 +258,7 @@ def test_coupling_sweep_square_against_separate_solves():
     S.close()
     for p, hs in enumerate(spaces):
         for r, j2 in enumerate(j2s):
-            ref = J1J2Model(lat, J1=-1.0, J2=-j2, hilbert_space=hs).diagonalize(k=1, tol=1e-12, maxiter=300)[0]
+            ref = J1J2Model(lat, J1=-1.0, J2=-j2, hilbert_space=hs).diagonalize(method="lanczos", k=1, tol=1e-12, maxiter=300)[0]
             assert abs(e[p, r] - ref) <= 1e-10 * abs(ref), (ks[p], j2)
 
 
@@ -281,7 +281,7 @@ def test_coupling_sweep_field_scan_against_separate_models():
     S.close()
     for p, hs in enumerate(spaces):
         for r, (jz, hz) in enumerate(sets):
-            ref = XXZ(lat, hilbert_space=hs, jxy=1.0, jz=jz, hz=hz).diagonalize(k=1, tol=1e-12, maxiter=300)[0]
+            ref = XXZ(lat, hilbert_space=hs, jxy=1.0, jz=jz, hz=hz).diagonalize(method="lanczos", k=1, tol=1e-12, maxiter=300)[0]
             assert abs(e[p, r] - ref) <= 1e-10 * max(1.0, abs(ref)), (p, jz, hz)
 
 

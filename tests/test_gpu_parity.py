@@ -223,7 +223,7 @@ def test_lanczos_e0_against_dense(case):
     c, s, co = t.arrays()
     ns = lx * ly * (2 if kind == "honeycomb" else 1)
     e_ref = np.linalg.eigvalsh(O.reduced_dense(b, ns, c, s, co, t.is_complex))[0]
-    e0 = H.diagonalize(k=1, tol=1e-13, maxiter=min(300, hs.nh), compute_eigvec=True)[0]
+    e0 = H.diagonalize(method="lanczos", k=1, tol=1e-13, maxiter=min(300, hs.nh), compute_eigvec=True)[0]
     assert abs(e0 - e_ref) <= E0_TOL * max(1.0, abs(e_ref))
     psi = H.eig_vec
     r = H.matvec_device(psi) - e0 * psi
@@ -258,7 +258,7 @@ def test_majumdar_ghosh_chain24_exact_energy():
     lat = ChainLattice(24)
     hs = HilbertSpace(lattice=lat, sym_gen={"translation": 0, "parity": {"x": 1}}, global_syms=[get_u1_sym(lat, 12)])
     H = J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=hs)
-    e0 = H.diagonalize(k=1, tol=1e-13, maxiter=300)[0]
+    e0 = H.diagonalize(method="lanczos", k=1, tol=1e-13, maxiter=300)[0]
     assert abs(e0 - (-9.0)) <= E0_TOL * 9.0
 
 
@@ -463,7 +463,7 @@ def test_expand_and_reduce_against_oracle(case):
     np.testing.assert_allclose(back, x * np.sqrt(grp.n_group) / b.norms ** 2, rtol=1e-12, atol=0)
     # eigenvector of the full-space Hamiltonian
     c, s, co = t.arrays()
-    e0 = H.diagonalize(k=1, tol=1e-13, maxiter=min(300, hs.nh), compute_eigvec=True)[0]
+    e0 = H.diagonalize(method="lanczos", k=1, tol=1e-13, maxiter=min(300, hs.nh), compute_eigvec=True)[0]
     v = H.eig_vec.cpu().numpy()
     vf = hs.expand_state(v)
     Hf = O.full_sparse_hamiltonian(ns, c, s, co, t.is_complex)
@@ -590,7 +590,7 @@ def test_boundary_flux_kitaev_sector_parity(gens):
     assert np.abs(A - A.conj().T).max() <= 1e-12
     Hd = O.reduced_dense(b, lat.ns, c, s, co, True)
     e_ref = np.linalg.eigvalsh(Hd)[0]
-    e0 = H.diagonalize(k=1, tol=1e-12, maxiter=400)[0]
+    e0 = H.diagonalize(method="lanczos", k=1, tol=1e-12, maxiter=400)[0]
     assert abs(e0 - e_ref) <= E0_TOL * abs(e_ref)
     with pytest.raises(NotImplementedError):
         HilbertSpace(lattice=lat, sym_gen={"translation": {"ky": 0}})
@@ -642,7 +642,7 @@ def test_expectation_value_and_overlap():
     evt = H.expectation_value(Xt[:, 0], Yt[:, 0])
     assert isinstance(evt, torch.Tensor) and abs(evt.item() - ref) <= 1e-12 * abs(ref)
     np.testing.assert_allclose(H.overlap(X, Y), np.einsum("ij,ij->j", X.conj(), Y), rtol=1e-13)
-    H.diagonalize(k=1, tol=1e-12, maxiter=300, compute_eigvec=True)
+    H.diagonalize(method="lanczos", k=1, tol=1e-12, maxiter=300, compute_eigvec=True)
     psi = H.eig_vec
     e = (H.expectation_value(psi) / H.overlap(psi)).real.item()
     e_ref = np.linalg.eigvalsh(Hd)[0]
@@ -673,7 +673,7 @@ def test_lowest_k_eigenpairs_with_reorthogonalisation():
     c, s, co = t.arrays()
     ev = np.linalg.eigvalsh(O.reduced_dense(b, 14, c, s, co, True))
     assert np.min(np.diff(ev[:5])) > 1e-3          # lowest levels non-degenerate here
-    w = H.diagonalize(k=4, tol=1e-11, maxiter=min(300, hs.nh), reorth=True, compute_eigvec=True)
+    w = H.diagonalize(method="lanczos", k=4, tol=1e-11, maxiter=min(300, hs.nh), reorth=True, compute_eigvec=True)
     np.testing.assert_allclose(w, ev[:4], rtol=0, atol=1e-10 * abs(ev[0]))
     assert H.last_lanczos.converged and H.last_lanczos.residual_bound <= 1e-11 * max(1.0, abs(ev[3]))
     V = H.eig_vec
@@ -682,6 +682,34 @@ def test_lowest_k_eigenpairs_with_reorthogonalisation():
         v = V[:, i].contiguous()
         r = H.matvec_device(v) - float(w[i]) * v
         assert torch.linalg.vector_norm(r).item() <= 1e-8 * torch.linalg.vector_norm(v).item()
+
+
+@pytest.mark.parametrize("case", [("chain", 14, 1, [("tx", 0), ("R", 1)], 7, "j1j2", False),
+                                  ("chain", 12, 1, [("tx", 5)], 6, "xxz", True),
+                                  ("honeycomb", 2, 2, [("tx", 0), ("ty", 0)], None, "hk", True)])
+def test_exact_diagonalisation_on_device(case):
+    """method='exact' (the reference's default): the whole spectrum of the
+    dense reduced matrix built on the device, against the oracle's dense
+    matrix on the CPU to 1e-12 relative; eigenvectors (n, n) with
+    H v = E v; 'auto' takes the same path at this size."""
+    import torch
+    kind, lx, ly, gens, n_up, model, cplx = case
+    hs, H, grp, b, t = build_pair(kind, lx, ly, gens, n_up, model)
+    c, s, co = t.arrays()
+    ev = np.linalg.eigvalsh(O.reduced_dense(b, t.ns, c, s, co, cplx))
+    assert bool(H.needs_complex) == cplx
+    w = H.diagonalize()
+    assert len(w) == hs.nh and H.eig_vec is None
+    scale = np.max(np.abs(ev))
+    np.testing.assert_allclose(w, ev, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(H.diagonalize(method="auto"), ev, rtol=0, atol=1e-12 * scale)
+    w = H.diagonalize(method="exact", compute_eigvec=True)
+    V = H.eig_vec
+    assert tuple(V.shape) == (hs.nh, hs.nh)
+    for i in (0, hs.nh // 2, hs.nh - 1):
+        v = V[:, i].contiguous()
+        r = H.matvec_device(v) - float(w[i]) * v
+        assert torch.linalg.vector_norm(r).item() <= 1e-10 * scale
 
 
 @pytest.mark.parametrize("which", ["largest", "both"])
@@ -696,7 +724,7 @@ def test_largest_and_both_ends(which):
     ev = np.linalg.eigvalsh(O.reduced_dense(b, 14, c, s, co, True))
     want = ev[-6:] if which == "largest" else np.concatenate([ev[:3], ev[-3:]])
     assert np.min(np.diff(want)) > 1e-3             # separated levels at both ends here
-    w = H.diagonalize(which=which, tol=1e-11, maxiter=min(300, hs.nh), compute_eigvec=True)
+    w = H.diagonalize(method="lanczos", which=which, tol=1e-11, maxiter=min(300, hs.nh), compute_eigvec=True)
     assert len(w) == 6
     np.testing.assert_allclose(w, want, rtol=0, atol=1e-10 * np.max(np.abs(ev)))
     assert H.last_lanczos.converged
@@ -706,9 +734,9 @@ def test_largest_and_both_ends(which):
         r = H.matvec_device(v) - float(w[i]) * v
         assert torch.linalg.vector_norm(r).item() <= 1e-8 * torch.linalg.vector_norm(v).item()
     with pytest.raises(ValueError):
-        H.diagonalize(which="LM")
+        H.diagonalize(method="lanczos", which="LM")
     with pytest.raises(NotImplementedError):
-        H.diagonalize(sigma=0.5)
+        H.diagonalize(method="lanczos", sigma=0.5)
 
 
 def test_two_pass_ritz_vector_without_stored_basis():
