@@ -422,7 +422,7 @@ void free_assembly(Op* op) {
   for (void* p : {(void*)op->asm_off, (void*)op->asm_len, (void*)op->asm_col, (void*)op->asm_val,
                   (void*)op->asm_packed, (void*)op->asm_vidx, (void*)op->asm_dict, (void*)op->asm_diag,
                   (void*)op->asm_diag_c, (void*)op->asm_dict_cpl, (void*)op->asm_dict_s, (void*)op->asm_part})
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   op->asm_off = nullptr;
   op->asm_len = nullptr;
   op->asm_col = nullptr;
@@ -488,7 +488,7 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
     ~Scratch() {
       if (p) {
         (void)hipStreamSynchronize(st);   // no pass may still be using it
-        hipFree(p);
+        (void)hipFree(p);
       }
     }
   } s_rowlen, s_dict;
@@ -501,7 +501,7 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
   std::vector<int32_t> rowlen(n);
   QES_HIP(hipMemcpyAsync(rowlen.data(), O.rowlen, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
   QES_HIP(hipStreamSynchronize(st));
-  hipFree(s_rowlen.p);   // before the table is allocated
+  (void)hipFree(s_rowlen.p);   // before the table is allocated
   s_rowlen.p = nullptr;
   O.rowlen = nullptr;
 

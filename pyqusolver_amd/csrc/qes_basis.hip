@@ -277,7 +277,7 @@ int build_bucket(Basis* B, int rows_log2, hipStream_t st) {
                                         : std::max<int64_t>(n << std::min(-rows_log2, 4), 1);
   int mbits = 0;
   while (mbits < 24 && ((int64_t)ne << (mbits + 1)) <= target) ++mbits;   // bucket_of keys stay 32-bit
-  if (B->bucket) hipFree(B->bucket);
+  if (B->bucket) (void)hipFree(B->bucket);
   B->bucket = nullptr;
   B->bucket_rows_log2 = rows_log2;
   B->mbits = mbits;

@@ -285,8 +285,8 @@ __device__ __forceinline__ void block_row_sell(const QesProb& P, int64_t a, unsi
     dcr[c] = dci[c] = 0.0;
     if (c < T.ncpl) {
       const double* dc = T.diag_c + ((int64_t)c * T.nrows + a) * VW;
-      dcr[c] = dc[0];
-      if (CV) dci[c] = dc[1];
+      dcr[c] = stream_load(dc);
+      if (CV) dci[c] = stream_load(dc + 1);
     }
   }
   double ar[RC], ai[RC];
@@ -318,12 +318,12 @@ __device__ __forceinline__ void block_row_sell(const QesProb& P, int64_t a, unsi
   const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
   auto entry = [&](int64_t e, int64_t& b, uint32_t& idx) {
     if (FMT == ASM_P4) {
-      const uint32_t p = T.packed[e];
+      const uint32_t p = stream_load(T.packed + e);
       b = p & cmask;
       idx = p >> T.colbits;
     } else {
-      b = T.col[e];
-      idx = T.vidx[e];
+      b = stream_load(T.col + e);
+      idx = stream_load(T.vidx + e);
     }
   };
   auto acc = [&](int64_t b, uint32_t idx) {
@@ -814,14 +814,14 @@ static int batch_build_tiles(Batch* B) {
       for (int p : order)
         if (i < per[p].size()) tiles.push_back(per[p][i]);
   }
-  if (B->d_tiles) hipFree(B->d_tiles);
+  if (B->d_tiles) (void)hipFree(B->d_tiles);
   B->d_tiles = nullptr;
   B->ntiles = (int)tiles.size();
   QES_HIP(hipMalloc(&B->d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(Tile)));
   if (!tiles.empty()) QES_HIP(batch_copy(B, B->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
   int dev = 0, ncu = 256;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  QES_HIP(hipGetDevice(&dev));
+  QES_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   B->grid_matvec = std::max(1, std::min(B->ntiles, ncu * B->grid_mult));
   return 0;
 }
@@ -892,28 +892,28 @@ static int batch_upload_probs(Batch* B) {
 }
 
 static void batch_free(Batch* B) {
-  for (auto p : B->t_rowptr) hipFree(p);
-  for (auto p : B->t_col) hipFree(p);
-  for (auto p : B->t_val) hipFree(p);
-  for (auto p : B->s_buf) hipFree(p);
-  if (B->exec) hipGraphExecDestroy(B->exec);
-  if (B->graph) hipGraphDestroy(B->graph);
-  hipFree(B->d_probs);
-  hipFree(B->d_tiles);
-  hipFree(B->d_chunks);
-  hipFree(B->d_vslot0);
-  hipFree(B->d_vnslot);
-  hipFree(B->v);
-  hipFree(B->vp);
-  hipFree(B->w);
-  hipFree(B->alpha);
-  hipFree(B->beta);
-  hipFree(B->scal);
-  hipFree(B->part);
-  hipFree(B->d_active);
-  hipFree(B->d_pactive);
-  hipFree(B->d_j);
-  if (B->stream) hipStreamDestroy(B->stream);
+  for (auto p : B->t_rowptr) (void)hipFree(p);
+  for (auto p : B->t_col) (void)hipFree(p);
+  for (auto p : B->t_val) (void)hipFree(p);
+  for (auto p : B->s_buf) (void)hipFree(p);
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  (void)hipFree(B->d_probs);
+  (void)hipFree(B->d_tiles);
+  (void)hipFree(B->d_chunks);
+  (void)hipFree(B->d_vslot0);
+  (void)hipFree(B->d_vnslot);
+  (void)hipFree(B->v);
+  (void)hipFree(B->vp);
+  (void)hipFree(B->w);
+  (void)hipFree(B->alpha);
+  (void)hipFree(B->beta);
+  (void)hipFree(B->scal);
+  (void)hipFree(B->part);
+  (void)hipFree(B->d_active);
+  (void)hipFree(B->d_pactive);
+  (void)hipFree(B->d_j);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
 }
 
 int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, int max_steps, int dtype,
@@ -999,12 +999,17 @@ int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, 
             dc[((size_t)r * nd + i) * vw + w] = op->asm_h_dict[(size_t)i * vw + w] * CS[p][r * ASM_MAX_CPL + op->asm_h_dict_cpl[i]];
       double* d = nullptr;
       if (hipMalloc(&d, buf.size() * sizeof(double)) != hipSuccess) {
-        for (auto b : B->s_buf) hipFree(b);
+        for (auto b : B->s_buf) (void)hipFree(b);
         delete B;
         return fail("hipMalloc failed in batch_create");
       }
-      hipMemcpy(d, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice);
       B->s_buf.push_back(d);
+      const hipError_t ce = hipMemcpy(d, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice);
+      if (ce != hipSuccess) {
+        for (auto b : B->s_buf) (void)hipFree(b);
+        delete B;
+        return check_hip(ce, "batch_create: copy of the column scales");
+      }
       q.cscale = d;
       q.dcols = d + (size_t)R * ASM_MAX_CPL;
       q.ndict = nd;
@@ -1065,15 +1070,17 @@ int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, 
 #undef QES_BALLOC
   B->h_probs = h;
   if (!rc) {
-    hipMemcpy(B->d_probs, h.data(), B->nprob * sizeof(QesProb), hipMemcpyHostToDevice);
-    if (!chunks.empty()) hipMemcpy(B->d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice);
-    hipMemcpy(B->d_vslot0, vslot0.data(), B->nvp * sizeof(int), hipMemcpyHostToDevice);
-    hipMemcpy(B->d_vnslot, vnslot.data(), B->nvp * sizeof(int), hipMemcpyHostToDevice);
     B->h_active.assign(B->nvp, 1);
     std::vector<int> pa(B->nprob, 1);
-    hipMemcpy(B->d_active, B->h_active.data(), sizeof(int) * B->nvp, hipMemcpyHostToDevice);
-    hipMemcpy(B->d_pactive, pa.data(), sizeof(int) * B->nprob, hipMemcpyHostToDevice);
-    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) rc = fail("stream creation failed");
+    hipError_t e = hipMemcpy(B->d_probs, h.data(), B->nprob * sizeof(QesProb), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !chunks.empty())
+      e = hipMemcpy(B->d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_vslot0, vslot0.data(), B->nvp * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_vnslot, vnslot.data(), B->nvp * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_active, B->h_active.data(), sizeof(int) * B->nvp, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_pactive, pa.data(), sizeof(int) * B->nprob, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) rc = check_hip(e, "batch_create: problem tables");
   }
   B->grid_vec = std::max(1, std::min(B->nchunks, 4096));
   if (!rc) rc = batch_build_tables(B);
@@ -1089,8 +1096,8 @@ int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, 
 }
 
 static void batch_drop_graph(Batch* B) {
-  if (B->exec) hipGraphExecDestroy(B->exec);
-  if (B->graph) hipGraphDestroy(B->graph);
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
   B->exec = nullptr;
   B->graph = nullptr;
 }

@@ -298,23 +298,23 @@ int qes_basis_map_states(qes_basis_t h, const uint64_t* states_host, int64_t n, 
     if (e == hipSuccess && g_out) e = hipMemcpy(g_out, dg, n * 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = check_hip(e, "qes_basis_map_states");
   }
-  hipFree(ds);
-  hipFree(di);
-  hipFree(dg);
+  (void)hipFree(ds);
+  (void)hipFree(di);
+  (void)hipFree(dg);
   return rc;
 }
 
 int qes_basis_destroy(qes_basis_t h) {
   if (!h) return 0;
   Basis& B = h->b;
-  if (B.chi_re) hipFree(B.chi_re);
-  if (B.chi_im) hipFree(B.chi_im);
-  if (B.reps) hipFree(B.reps);
-  if (B.keys32) hipFree(B.keys32);
-  if (B.norms) hipFree(B.norms);
-  if (B.bucket) hipFree(B.bucket);
-  if (B.hash) hipFree(B.hash);
-  if (B.keyc) hipFree(B.keyc);
+  if (B.chi_re) (void)hipFree(B.chi_re);
+  if (B.chi_im) (void)hipFree(B.chi_im);
+  if (B.reps) (void)hipFree(B.reps);
+  if (B.keys32) (void)hipFree(B.keys32);
+  if (B.norms) (void)hipFree(B.norms);
+  if (B.bucket) (void)hipFree(B.bucket);
+  if (B.hash) (void)hipFree(B.hash);
+  if (B.keyc) (void)hipFree(B.keyc);
   delete h;
   return 0;
 }
@@ -365,12 +365,12 @@ int qes_op_create(qes_basis_t bh, const int32_t* codes, const int32_t* sites, co
   if (!op.h_groups.empty() &&
       hipMemcpy(op.d_groups, op.h_groups.data(), op.h_groups.size() * sizeof(QesTermGroup), hipMemcpyHostToDevice) !=
           hipSuccess) {
-    hipFree(op.d_groups);
+    (void)hipFree(op.d_groups);
     delete h;
     return fail("copy of term groups failed");
   }
   if (build_plan(&op, -1)) {
-    hipFree(op.d_groups);
+    (void)hipFree(op.d_groups);
     free_plan(&op);
     delete h;
     return -1;
@@ -499,14 +499,14 @@ int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_ho
   const hipError_t e = hipMemcpy(ds, states_host, n * 8, hipMemcpyHostToDevice);
   const int rc = e == hipSuccess ? canon_check(&b->b, kind, nullptr, ds, n, mismatches, nullptr)
                                  : check_hip(e, "qes_basis_canon_check");
-  hipFree(ds);
+  (void)hipFree(ds);
   return rc;
 }
 
 int qes_op_destroy(qes_op_t h) {
   if (!h) return 0;
   free_plan(&h->op);
-  if (h->op.d_groups) hipFree(h->op.d_groups);
+  if (h->op.d_groups) (void)hipFree(h->op.d_groups);
   delete h;
   return 0;
 }

@@ -462,22 +462,22 @@ int lanczos_create(Op* op, int max_steps, int store, int dtype, Lanczos** out) {
 
 int lanczos_destroy(Lanczos* L) {
   if (!L) return 0;
-  if (L->exec) hipGraphExecDestroy(L->exec);
-  if (L->graph) hipGraphDestroy(L->graph);
-  hipFree(L->v);
-  hipFree(L->vp);
-  hipFree(L->w);
-  if (L->V) hipFree(L->V);
-  hipFree(L->alpha);
-  hipFree(L->beta);
-  hipFree(L->scal);
-  hipFree(L->part);
-  hipFree(L->coef);
-  hipFree(L->ppart);
-  hipFree(L->d_j);
-  hipFree(L->dg);
-  hipFree(L->flag);
-  if (L->stream) hipStreamDestroy(L->stream);
+  if (L->exec) (void)hipGraphExecDestroy(L->exec);
+  if (L->graph) (void)hipGraphDestroy(L->graph);
+  (void)hipFree(L->v);
+  (void)hipFree(L->vp);
+  (void)hipFree(L->w);
+  if (L->V) (void)hipFree(L->V);
+  (void)hipFree(L->alpha);
+  (void)hipFree(L->beta);
+  (void)hipFree(L->scal);
+  (void)hipFree(L->part);
+  (void)hipFree(L->coef);
+  (void)hipFree(L->ppart);
+  (void)hipFree(L->d_j);
+  (void)hipFree(L->dg);
+  (void)hipFree(L->flag);
+  if (L->stream) (void)hipStreamDestroy(L->stream);
   delete L;
   return 0;
 }
@@ -510,8 +510,8 @@ int lanczos_run(Lanczos* L, int nsteps, int reorth, int use_graph) {
   if (use_graph) {
     if (!L->exec || L->reorth_graph != reorth) {
       if (L->exec) {
-        hipGraphExecDestroy(L->exec);
-        hipGraphDestroy(L->graph);
+        (void)hipGraphExecDestroy(L->exec);
+        (void)hipGraphDestroy(L->graph);
         L->exec = nullptr;
         L->graph = nullptr;
       }

@@ -277,10 +277,10 @@ static std::vector<uint32_t> row_lut(const QesGroup& G) {
 
 void free_plan(Op* op) {
   free_assembly(op);   // the table follows the plan's visit order
-  if (op->d_exval) hipFree(op->d_exval);
-  if (op->d_excpl) hipFree(op->d_excpl);
-  if (op->d_lut) hipFree(op->d_lut);
-  if (op->d_gen) hipFree(op->d_gen);
+  if (op->d_exval) (void)hipFree(op->d_exval);
+  if (op->d_excpl) (void)hipFree(op->d_excpl);
+  if (op->d_lut) (void)hipFree(op->d_lut);
+  if (op->d_gen) (void)hipFree(op->d_gen);
   op->d_exval = nullptr;
   op->d_excpl = nullptr;
   op->d_lut = nullptr;
@@ -653,8 +653,8 @@ int canon_check(const Basis* B, int kind, const uint32_t* d_lut_in, const uint64
   unsigned long long h = 0;
   QES_HIP(hipMemcpyAsync(&h, d_bad, sizeof(h), hipMemcpyDeviceToHost, st));
   QES_HIP(hipStreamSynchronize(st));
-  hipFree(d_bad);
-  hipFree(d_lut);
+  (void)hipFree(d_bad);
+  (void)hipFree(d_lut);
   *bad_out = (int64_t)h;
   return 0;
 }

@@ -12,6 +12,22 @@ static constexpr int SELL_C = 64;
 #define QES_SELL_UNROLL 4   // slots per lane in flight in sell_row_t
 #endif
 static constexpr int ASM_MAX_CPL = 8;   // coupling ids of a coupling-tagged table
+#ifndef QES_SELL_NT
+#define QES_SELL_NT 1
+#endif
+
+// load of a table word that is read once per product: with QES_SELL_NT the
+// load carries the streaming (non-temporal) hint, so the table passing
+// through leaves the caches to the gathered x lines (measured: 0.98 -> 0.95
+// ms at L=32, 62.0 -> 60.7 ms at 6x6)
+template <typename T>
+__device__ __forceinline__ T stream_load(const T* p) {
+#if QES_SELL_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
 enum { ASM_PLAIN = 0, ASM_P4 = 1, ASM_P6 = 2 };
 
 struct AsmTable {
@@ -37,13 +53,13 @@ struct AsmTable {
 // entry e of a dictionary table -> (column, dictionary index)
 __device__ __forceinline__ void sell_entry(const AsmTable& T, int64_t e, int64_t& b, uint32_t& idx) {
   if (T.fmt == ASM_P4) {
-    const uint32_t p = T.packed[e];
+    const uint32_t p = stream_load(T.packed + e);
     const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
     b = p & cmask;
     idx = p >> T.colbits;
   } else {
-    b = T.col[e];
-    idx = T.vidx[e];
+    b = stream_load(T.col + e);
+    idx = stream_load(T.vidx + e);
   }
 }
 
@@ -72,13 +88,13 @@ __device__ __forceinline__ void sell_row_t(const AsmTable& T, int64_t a, const d
       for (int c = 0; c < ASM_MAX_CPL; ++c) {
         if (c < T.ncpl) {
           const double* dc = T.diag_c + ((int64_t)c * T.nrows + a) * VW;
-          dr += T.dscale[c] * dc[0];
-          if (CV) di += T.dscale[c] * dc[1];
+          dr += T.dscale[c] * stream_load(dc);
+          if (CV) di += T.dscale[c] * stream_load(dc + 1);
         }
       }
     } else {
-      dr = T.diag[a * VW];
-      if (CV) di = T.diag[a * VW + 1];
+      dr = stream_load(T.diag + a * VW);
+      if (CV) di = stream_load(T.diag + a * VW + 1);
     }
     if (XC) {
       const double xr = x[2 * a], xi = x[2 * a + 1];
@@ -91,20 +107,20 @@ __device__ __forceinline__ void sell_row_t(const AsmTable& T, int64_t a, const d
   const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
   auto entry = [&](int64_t e, int64_t& b, double& vr, double& vi) {
     if (FMT == ASM_P4) {
-      const uint32_t p = T.packed[e];
+      const uint32_t p = stream_load(T.packed + e);
       b = p & cmask;
       const uint32_t idx = p >> T.colbits;
       vr = T.dict[idx * VW];
       vi = CV ? T.dict[idx * VW + 1] : 0.0;
     } else if (FMT == ASM_P6) {
-      b = T.col[e];
-      const uint32_t idx = T.vidx[e];
+      b = stream_load(T.col + e);
+      const uint32_t idx = stream_load(T.vidx + e);
       vr = T.dict[idx * VW];
       vi = CV ? T.dict[idx * VW + 1] : 0.0;
     } else {
-      b = T.col[e];
-      vr = T.val[e * VW];
-      vi = CV ? T.val[e * VW + 1] : 0.0;
+      b = stream_load(T.col + e);
+      vr = stream_load(T.val + e * VW);
+      vi = CV ? stream_load(T.val + e * VW + 1) : 0.0;
     }
   };
   auto accumulate = [&](int64_t b, double vr, double vi) {
