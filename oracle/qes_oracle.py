@@ -491,7 +491,7 @@ def normalization(group: Group, reps: np.ndarray) -> np.ndarray:
 
 
 def sector_dimension(group: Group) -> int:
-    """Closed form N = (1/|G|) sum_g conj(chi_g) Tr_{U(1)}(g) from cycle
+    """Closed form N = (1/|G|) sum_g conj(chi_g) Tr_{U(1)/Z2}(g) from cycle
     structures -- independent of the orbit scan.  Only for groups without
     intrinsic phases (parity x/z)."""
     ns = group.lat.ns
@@ -535,8 +535,13 @@ def sector_dimension(group: Group) -> int:
                 else:
                     new[c // 2:] += 2 * poly[:ns + 1 - c // 2]
             poly = new
+        # fixed states in the U(1) and/or Z2 popcount-parity sector
+        # (z2 = +1: even popcount, as candidate_states)
+        want = None if group.z2 is None else (0 if int(group.z2) == 1 else 1)
         if group.n_up is not None:
-            tr = poly[group.n_up]
+            tr = poly[group.n_up] if (want is None or group.n_up % 2 == want) else 0.0
+        elif want is not None:
+            tr = poly[want::2].sum()
         else:
             tr = poly.sum()
         total += np.conj(group.chi[g]) * tr

@@ -95,6 +95,30 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("kind,lx,ly,gens,z2", [("chain", 10, 1, [("tx", 0), ("R", 1)], 1),
+                                                ("chain", 10, 1, [("tx", 3)], -1),
+                                                ("chain", 9, 1, [("tx", 2)], 1),
+                                                ("chain", 8, 1, [("tx", 0), ("Px", 1)], 1),
+                                                ("square", 4, 2, [("tx", 1), ("ty", 0)], -1),
+                                                ("honeycomb", 2, 2, [("tx", 0), ("ty", 1)], 1)])
+def test_z2_scan_dimension_matches_character_formula(kind, lx, ly, gens, z2):
+    """Z2 popcount-parity sectors without U(1): the orbit scan's size equals
+    the character formula restricted to fixed states of that popcount parity,
+    the two parities add up to the unfiltered sector, and every
+    representative has the requested parity."""
+    lat = O.OLattice(kind, lx, ly)
+    gs = [O.Generator(k, s) for k, s in gens]
+    grp = O.Group(lat, gs, z2=z2)
+    b = O.scan_basis(grp)
+    assert len(b.reps) == O.sector_dimension(grp)
+    par = np.array([bin(int(r)).count("1") % 2 for r in b.reps])
+    assert np.all(np.where(par == 0, 1, -1) == z2)
+    other = O.sector_dimension(O.Group(lat, gs, z2=-z2))
+    assert len(b.reps) + other == O.sector_dimension(O.Group(lat, gs))
+    assert O.sector_dimension(O.Group(lat, gs, n_up=3, z2=1)) == 0          # odd popcount, even parity
+    assert O.sector_dimension(O.Group(lat, gs, n_up=3, z2=-1)) == O.sector_dimension(O.Group(lat, gs, n_up=3))
+
+
 @pytest.mark.parametrize("kind,lx,ly,gens,n_up", CASES)
 def test_scan_dimension_matches_character_formula(kind, lx, ly, gens, n_up):
     lat = O.OLattice(kind, lx, ly)
