@@ -123,6 +123,9 @@ def bench_batched(a):
         "speedup_including_creation": seq.seconds / (res.seconds + t_create),
         "sector_by_sector": "one device Lanczos per problem, each operator assembled for its own solve",
         "e0_sectors_L24": [float(v) for v in res.e0[:len(specs)]],
+        "residuals_sectors_L24": [float(v) for v in res.residuals[:len(specs)]],
+        "residual_bound_max": float(res.residuals.max()),
+        "rel_error_bound_max": float(np.max(res.residuals / np.maximum(1.0, np.abs(res.e0)))),
         "max_rel_dev_batched_vs_sequential": float(dev.max()),
         "schedule": res.schedule,
         "max_rel_dev_E0_k_vs_minus_k": pair_dev,
@@ -244,15 +247,17 @@ def bench_sweep(a):
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         res = S.ground_states(tol=1e-10, check_min=a.check_min)
-        S.close()
         torch.cuda.synchronize()
         t3 = time.perf_counter()
+        resid = S.residuals()
+        S.close()
         del H, hs
         t_total += t3 - t0
         t_tables += t2 - t1
         steps_max = max(steps_max, int(res.steps.max()))
         results.append({"k": [kx, ky], "N": int(n_sec), "e0": [float(v) for v in res.e0],
-                        "steps": [int(v) for v in res.steps], "seconds": round(t3 - t0, 3)})
+                        "steps": [int(v) for v in res.steps], "residuals": [float(v) for v in resid],
+                        "seconds": round(t3 - t0, 3)})
         log(f"sector ({kx},{ky}): E0 {' '.join(f'{v:.12f}' for v in res.e0)} steps {list(res.steps)} "
             f"{t3 - t0:.2f}s (table {t2 - t1:.2f}s, solve {t3 - t2:.2f}s)", a.quiet)
     base = None

@@ -28,6 +28,9 @@ class BatchResult:
     steps: np.ndarray
     seconds: float
     schedule: Optional[list] = None     # autotuned schedule of each batch (batched_ground_states)
+    # residual norm |beta_m s_m| of each lowest Ritz pair: |E0 - lambda| <= residual
+    # for an eigenvalue lambda (the FP64 error bound of each result)
+    residuals: Optional[np.ndarray] = None
 
 
 _DIAGONAL_CODES = (13, 22)   # sigma_z local / sigma_z correlation (operators_spin.py:1084-1118)
@@ -273,6 +276,19 @@ class BatchedLanczos:
                    "qes_batch_ground_states")
         return BatchResult(e0=e0[self.vp_of], steps=steps[self.vp_of], seconds=time.perf_counter() - t0)
 
+    def residuals(self) -> np.ndarray:
+        """Residual norm |beta_m s_m| of the lowest Ritz pair of every problem
+        after the last run (order of hams), from the recurrence coefficients;
+        0 after a breakdown (the Krylov space is invariant)."""
+        from .lanczos import ritz_residual
+        a, b, k = self.coefficients()
+        out = np.zeros(len(k))
+        for i, m in enumerate(k):
+            m = int(m)
+            if m > 0:
+                out[i] = ritz_residual(a[i, :m], b[i, :m])[0]
+        return out
+
     def coefficients(self):
         """alpha, beta (one row per Hamiltonian, in the order of hams) and steps."""
         P, M = len(self.hams), self.max_steps
@@ -478,8 +494,10 @@ def batched_ground_states(hams: Sequence, tol: float = 1e-10, max_steps: int = 3
     seconds = time.perf_counter() - t0
     e0 = np.zeros(len(hams))
     steps = np.zeros(len(hams), dtype=np.int32)
+    resid = np.zeros(len(hams))
     for (idx, B), r in zip(batches, results):
         e0[idx] = r.e0
         steps[idx] = r.steps
+        resid[idx] = B.residuals()          # after the timed region
         B.close()
-    return BatchResult(e0=e0, steps=steps, seconds=seconds, schedule=schedules), t_create
+    return BatchResult(e0=e0, steps=steps, seconds=seconds, schedule=schedules, residuals=resid), t_create

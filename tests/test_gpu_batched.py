@@ -54,6 +54,7 @@ def test_batched_momentum_sweep_e0_against_oracle():
     lat, specs, spaces, hams = _xxz_sectors(L)
     B = BatchedLanczos(hams, max_steps=200)
     res = B.ground_states(tol=1e-13)
+    resid = B.residuals()
     olat = O.OLattice("chain", L)
     t = O.terms_xxz(olat, 1.0, 0.7)
     c, s, co = t.arrays()
@@ -62,8 +63,13 @@ def test_batched_momentum_sweep_e0_against_oracle():
         if "reflection" in spec:
             gens.append(O.Generator("R", spec["reflection"]))
         b = O.scan_basis(O.Group(olat, gens, n_up=L // 2))
-        e_ref = np.linalg.eigvalsh(O.reduced_dense(b, L, c, s, co, True))[0]
+        ev = np.linalg.eigvalsh(O.reduced_dense(b, L, c, s, co, True))
+        e_ref = ev[0]
         assert abs(res.e0[p] - e_ref) <= 1e-10 * max(1.0, abs(e_ref)), (spec, res.e0[p], e_ref)
+        # the stated FP64 error bound: |E0 - lambda| <= residual for an eigenvalue
+        # lambda, and every problem stopped under the residual test (or at its dimension)
+        assert np.min(np.abs(ev - res.e0[p])) <= resid[p] * (1 + 1e-6) + 1e-14
+        assert resid[p] <= 1e-13 * max(1.0, abs(res.e0[p])) or res.steps[p] >= len(b.reps)
     # the lowest sector energy is the Sz=0 ground state of the full space
     Hf = O.full_dense_hamiltonian(L, c, s, co, True)
     mask = np.array([bin(i).count("1") == L // 2 for i in range(1 << L)])
@@ -311,3 +317,26 @@ def test_autotuned_schedule_changes_speed_only():
     np.testing.assert_array_equal(r1.e0, r0.e0)
     np.testing.assert_array_equal(r1.steps, r0.steps)
     B.close()
+
+
+def test_batched_ground_states_split_batches_with_error_bounds():
+    """batched_ground_states: real and complex problems in two batches on two
+    streams (threads), realisation blocks inside the real one; E0 equal to
+    separate solves to 1e-10, and each result carries its residual bound
+    (|E0 - lambda| <= r), all under the stopping tolerance."""
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.batched import batched_ground_states, sector_by_sector
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import XXZ
+    lat, specs, spaces, hams = _xxz_sectors(12)
+    lat10 = ChainLattice(10)
+    hs10 = HilbertSpace(lattice=lat10, global_syms=[get_u1_sym(lat10, 5)])
+    hams = list(hams) + [XXZ(lat10, hilbert_space=hs10, jxy=1.0, jz=0.7,
+                             hz=np.random.default_rng(50 + r).uniform(-3, 3, 10)) for r in range(6)]
+    assert any(h.needs_complex for h in hams) and not all(h.needs_complex for h in hams)
+    res, t_create = batched_ground_states(hams, tol=1e-11, max_steps=250, autotune=False)
+    ref = sector_by_sector(hams, tol=1e-11, max_steps=250)
+    np.testing.assert_allclose(res.e0, ref.e0, rtol=0, atol=1e-10 * np.max(np.abs(ref.e0)))
+    assert res.residuals is not None and res.residuals.shape == (len(hams),)
+    assert np.all(res.residuals <= 1e-11 * np.maximum(1.0, np.abs(res.e0)) * (1 + 1e-9))
+    assert t_create >= 0.0 and res.seconds > 0.0
