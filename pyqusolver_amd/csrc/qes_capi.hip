@@ -315,6 +315,9 @@ int qes_basis_destroy(qes_basis_t h) {
   if (B.bucket) (void)hipFree(B.bucket);
   if (B.hash) (void)hipFree(B.hash);
   if (B.keyc) (void)hipFree(B.keyc);
+  if (B.rmap) (void)hipFree(B.rmap);
+  if (B.rbase) (void)hipFree(B.rbase);
+  if (B.rlo) (void)hipFree(B.rlo);
   delete h;
   return 0;
 }
@@ -459,15 +462,22 @@ int qes_op_get_plan(qes_op_t h, int32_t* canon, int32_t* n_classes, int32_t* n_g
 
 int qes_basis_set_lookup(qes_basis_t b, int32_t mode) {
   if (!b) return fail("null basis");
-  if (mode < 0 || mode > 2)
-    return fail("mode must be 0 (sorted keys + bucket table), 1 (hash table) or 2 (packed keys + bucket table)");
+  if (mode < 0 || mode > 3)
+    return fail("mode must be 0 (sorted keys + bucket table), 1 (hash table), 2 (packed keys + bucket table) "
+                "or 3 (rank map)");
   if (mode == 1 && !b->b.hash) {
     if (build_hash(&b->b, nullptr)) return -1;
     if (!b->b.hash) return fail("no hash table for this basis (index does not fit beside the key)");
   }
   if (mode == 2 && !b->b.keyc) return fail("no packed keys for this basis (ns > 56 or intrinsic phases)");
+  if (mode == 3) {
+    int ok = 0;
+    if (build_rank_map(&b->b, nullptr, &ok)) return -1;
+    if (!ok) return fail("no rank map for this basis (needs ns <= 32 and a U(1) filling)");
+  }
   b->b.use_hash = mode == 1;
   b->b.use_packed = mode == 2;
+  b->b.use_rank = mode == 3;
   return 0;
 }
 

@@ -274,6 +274,16 @@ struct QesLookup {
   // packed keys (ns <= 56): key | (|Stab(rep)| - 1) << 56, sorted by key;
   // a hit on the packed array also yields the stabiliser size
   const uint64_t* keyc;
+  // rank map (ns <= 32 with a U(1) filling n_up; used when non-null).  The
+  // colex rank of v among the states of that filling is
+  // c = rbase[v >> 16] + rlo[v & 0xffff]; rmap[c / 64] holds the bit mask
+  // of the representatives among ranks 64*(c/64) ... +63 and the number of
+  // representatives of lower rank, so one 16-byte load resolves a lookup.
+  const uint64_t* rmap;     // pairs {bits, count before}
+  const uint32_t* rbase;    // 65536: states of the filling with a smaller high half
+  const uint16_t* rlo;      // 65536: states with the same low-half popcount and a smaller low half
+  int32_t n_up;
+  int32_t pad_r;
 };
 
 #define QES_KEY_BITS 56
@@ -339,8 +349,21 @@ __device__ __forceinline__ int64_t lookup_packed(const QesLookup& L, uint64_t v,
   return -1;
 }
 
+// rank-map lookup of a 32-bit state; -1 if it is not a representative
+__device__ __forceinline__ int64_t lookup_rank(const QesLookup& L, uint64_t v) {
+  if (L.ns < 64 && (v >> L.ns)) return -1;   // outside the ns-bit states (ns <= 32 here)
+  const uint32_t u = (uint32_t)v;
+  if (__popc(u) != L.n_up) return -1;
+  const uint32_t c = L.rbase[u >> 16] + (uint32_t)L.rlo[u & 0xffffu];
+  const ulonglong2 e = reinterpret_cast<const ulonglong2*>(L.rmap)[c >> 6];
+  const uint32_t bit = c & 63u;
+  if (!((e.x >> bit) & 1ull)) return -1;
+  return (int64_t)e.y + (int64_t)__popcll(e.x & ((1ull << bit) - 1ull));
+}
+
 template <typename W>
 __device__ __forceinline__ int64_t lookup_rep(const QesLookup& L, W v) {
+  if (L.rmap) return lookup_rank(L, (uint64_t)v);
   if (L.hash) {
     uint64_t slot;
     const uint64_t e0 = hash_first(L, (uint64_t)v, &slot);

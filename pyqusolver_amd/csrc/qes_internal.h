@@ -48,8 +48,19 @@ struct Basis {
     L.hbits = hbits;
     L.packed = (use_packed && keyc) ? 1 : 0;
     L.keyc = keyc;
+    L.rmap = (use_rank && rmap) ? rmap : nullptr;
+    L.rbase = rbase;
+    L.rlo = rlo;
+    L.n_up = group.n_up;
+    L.pad_r = 0;
     return L;
   }
+  // rank map (ns <= 32 with a U(1) filling): see QesLookup
+  uint64_t* rmap = nullptr;
+  uint32_t* rbase = nullptr;
+  uint16_t* rlo = nullptr;
+  int64_t rmap_words = 0;
+  bool use_rank = false;
   uint64_t* keyc = nullptr;   // device packed keys (key | (|Stab|-1) << 56), ns <= 56
   uint64_t* hash = nullptr;   // device hash table (representative -> index)
   int hbits = 0;
@@ -107,6 +118,9 @@ struct Op {
 // launchers (qes_basis.hip)
 int build_basis(Basis* B, hipStream_t st);
 int build_hash(Basis* B, hipStream_t st);
+// rank map of a sector with ns <= 32 and a U(1) filling; *eligible = 0 (and
+// nothing built) otherwise
+int build_rank_map(Basis* B, hipStream_t st, int* eligible);
 int build_bucket(Basis* B, int rows_log2, hipStream_t st);
 // reduce == 0: full = expand(x_red) (2^ns entries); 1: x_red = reduce(full)
 int expand_reduce(const Basis* B, int reduce, const double* x_red, double* full, int dtype, hipStream_t st);

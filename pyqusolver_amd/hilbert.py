@@ -126,11 +126,12 @@ class HilbertSpace:
                                                     C.byref(bad)), "qes_basis_canon_check")
         return int(bad.value)
 
-    LOOKUP_MODES = {"sorted": 0, "hash": 1, "packed": 2}
+    LOOKUP_MODES = {"sorted": 0, "hash": 1, "packed": 2, "rank": 3}
 
     def set_lookup(self, mode: str = "packed", bucket_rows_log2: Optional[int] = None):
         """Representative lookup of the matvec: 'sorted' (32/64-bit keys),
-        'hash' or 'packed' (64-bit key | stabiliser size); optionally rebuild
+        'hash', 'packed' (64-bit key | stabiliser size) or 'rank' (bitmap
+        over the colex ranks of a U(1) filling, ns <= 32); optionally rebuild
         the bucket table for about 2^bucket_rows_log2 representatives per bucket."""
         _lib.check(_lib.lib().qes_basis_set_lookup(self._h, self.LOOKUP_MODES[mode]), "qes_basis_set_lookup")
         if bucket_rows_log2 is not None:

@@ -166,6 +166,41 @@ def test_z2_parity_sectors_without_u1(case):
     assert abs(e0 - ev[0]) <= 1e-10 * abs(ev[0])
 
 
+@pytest.mark.parametrize("case", [c for c in CASES if c[4] is not None and c[1] * c[2] * (2 if c[0] == "honeycomb" else 1) <= 32
+                                  and all(k != "Py" for k, _ in c[3])], ids=_ids)
+def test_rank_map_lookup_matches_sorted_keys(case):
+    """The rank-map lookup (bitmap over the colex ranks of the U(1) filling)
+    finds the same representatives as the sorted keys: products bitwise
+    equal, index maps equal, E_off equal; out-of-sector states miss."""
+    import torch
+    kind, lx, ly, gens, n_up, model = case
+    hs, H, grp, b, t = build_pair(kind, lx, ly, gens, n_up, model)
+    dt = torch.complex128 if H.needs_complex else torch.float64
+    x = torch.randn(hs.nh, dtype=dt, device="cuda", generator=torch.Generator(device="cuda").manual_seed(5))
+    y0 = H.matvec_device(x)
+    idx0, g0 = hs.map_states(b.states)
+    e0 = H.count_offdiag()
+    hs.set_lookup("rank")
+    assert torch.equal(H.matvec_device(x), y0)
+    idx1, g1 = hs.map_states(b.states)
+    np.testing.assert_array_equal(idx1, idx0)
+    np.testing.assert_array_equal(g1, g0)
+    assert H.count_offdiag() == e0
+    ns = t.ns
+    other = np.array([s for s in range(min(1 << ns, 1 << 16)) if bin(s).count("1") != n_up][:64], dtype=np.uint64)
+    if len(other):
+        assert np.all(hs.map_states(other)[0] < 0)
+    hs.set_lookup("sorted")
+    assert torch.equal(H.matvec_device(x), y0)
+
+
+def test_rank_map_refused_without_u1():
+    from pyqusolver_amd._lib import QesError
+    hs, H, grp, b, t = build_pair("chain", 12, 1, [("tx", 0), ("R", 1)], None, "j1j2_field")
+    with pytest.raises(QesError, match="rank map"):
+        hs.set_lookup("rank")
+
+
 @pytest.mark.parametrize("nb", [2, 3, 5])
 def test_multi_rhs_matches_single(nb):
     hs, H, grp, b, t = build_pair("chain", 14, 1, [("tx", 3)], 7, "xxz")
