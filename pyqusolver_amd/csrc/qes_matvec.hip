@@ -25,18 +25,27 @@
 
 namespace qes {
 
+// minimum resident waves per SIMD the register allocation must allow (0: the
+// compiler's choice)
+#ifndef QES_MV_WAVES
+#define QES_MV_WAVES 8
+#endif
+#if QES_MV_WAVES > 0
+#define QES_MV_ATTR __attribute__((amdgpu_waves_per_eu(QES_MV_WAVES)))
+#else
+#define QES_MV_ATTR
+#endif
+
 // NB = number of right-hand sides per lane (columns j0 .. j0+NB-1 of an
 // (n, ldx) row-major block).
 template <typename W, bool CPLX, int CK, int NB>
-__global__ void __launch_bounds__(256) matvec_kernel(QesGroup G, QesLookup L, QesPlan P,
-                                                     const double* __restrict__ chi_re,
-                                                     const double* __restrict__ chi_im,
-                                                     const QesTermGroup* __restrict__ groups, int n_diag,
-                                                     const double* __restrict__ exval,
-                                                     const uint32_t* __restrict__ lut,
-                                                     const double* __restrict__ norms, const double* __restrict__ x,
-                                                     double* __restrict__ y, int64_t n, int64_t ldx, int64_t j0,
-                                                     int ncols, int accumulate, int remap) {
+__device__ __forceinline__ void matvec_rows(const QesGroup& G, const QesLookup& L, const QesPlan& P,
+                                            const double* __restrict__ chi_re, const double* __restrict__ chi_im,
+                                            const QesTermGroup* __restrict__ groups, int n_diag,
+                                            const double* __restrict__ exval, const uint32_t* __restrict__ lut,
+                                            const double* __restrict__ norms, const double* __restrict__ x,
+                                            double* __restrict__ y, int64_t n, int64_t ldx, int64_t j0, int ncols,
+                                            int accumulate, int remap) {
   extern __shared__ char smem[];
   const RowCtx C = load_ctx<CPLX>(G, L, P, chi_re, chi_im, groups, n_diag, exval, lut, smem);
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, remap);
@@ -111,6 +120,28 @@ __global__ void __launch_bounds__(256) matvec_kernel(QesGroup G, QesLookup L, Qe
   }
 }
 
+
+#define QES_MV_PARAMS                                                                                            \
+  QesGroup G, QesLookup L, QesPlan P, const double* __restrict__ chi_re, const double* __restrict__ chi_im,       \
+      const QesTermGroup* __restrict__ groups, int n_diag, const double* __restrict__ exval,                      \
+      const uint32_t* __restrict__ lut, const double* __restrict__ norms, const double* __restrict__ x,         \
+      double* __restrict__ y, int64_t n, int64_t ldx, int64_t j0, int ncols, int accumulate, int remap
+#define QES_MV_ARGS G, L, P, chi_re, chi_im, groups, n_diag, exval, lut, norms, x, y, n, ldx, j0, ncols, accumulate, remap
+
+// Up to 32 sites: registers capped so that 8 waves per SIMD stay resident
+// (the lookups and gathers are latency bound; 7 -> 8 waves: 2.88 -> 2.68 ms
+// at L=32, no spills).  Wider states keep the compiler's allocation: under
+// the cap their cyclic canonicaliser spills to scratch.
+template <bool CPLX, int CK, int NB>
+__global__ void __launch_bounds__(256) QES_MV_ATTR matvec_kernel(QES_MV_PARAMS) {
+  matvec_rows<uint32_t, CPLX, CK, NB>(QES_MV_ARGS);
+}
+template <bool CPLX, int CK, int NB>
+__global__ void __launch_bounds__(256) matvec_kernel_wide(QES_MV_PARAMS) {
+  matvec_rows<uint64_t, CPLX, CK, NB>(QES_MV_ARGS);
+}
+#undef QES_MV_PARAMS
+#undef QES_MV_ARGS
 // Per-coupling outputs y[c, a] (single RHS), c = coupling id of each term.
 #define QES_MAX_COUPLINGS 8
 template <typename W, bool CPLX, int CK>
@@ -397,17 +428,21 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   const QesPlan& P = op->plan;
   for (int64_t j0 = 0; j0 < nbatch; j0 += 4) {
     const int ncols = (int)std::min<int64_t>(4, nbatch - j0);
+#define QES_LAUNCH_K(KER)                                                                                        \
+  hipLaunchKernelGGL(KER, dim3((unsigned)nblocks), dim3(block), op->lds_bytes, st, B->group, B->lookup(), P,       \
+                     B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_lut, B->norms, x, y, n, nbatch, \
+                     j0, ncols, accumulate, op->xcd_remap)
 #define QES_LAUNCH(CK)                                                                                           \
-  if (ncols == 1)                                                                                                \
-    hipLaunchKernelGGL((matvec_kernel<W, CPLX, CK, 1>), dim3((unsigned)nblocks), dim3(block), op->lds_bytes, st, \
-                       B->group, B->lookup(), P, B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval,        \
-                       op->d_lut, B->norms, x, y, n, nbatch, j0, ncols, accumulate, op->xcd_remap);              \
-  else                                                                                                           \
-    hipLaunchKernelGGL((matvec_kernel<W, CPLX, CK, 4>), dim3((unsigned)nblocks), dim3(block), op->lds_bytes, st, \
-                       B->group, B->lookup(), P, B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval,        \
-                       op->d_lut, B->norms, x, y, n, nbatch, j0, ncols, accumulate, op->xcd_remap)
+  if (sizeof(W) == 4) {                                                                                          \
+    if (ncols == 1) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1>));                                                  \
+    else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 4>));                                                             \
+  } else {                                                                                                       \
+    if (ncols == 1) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1>));                                             \
+    else QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 4>));                                                        \
+  }
     QES_CK_SWITCH(P.canon, QES_LAUNCH)
 #undef QES_LAUNCH
+#undef QES_LAUNCH_K
     QES_HIP(hipGetLastError());
   }
   return 0;
