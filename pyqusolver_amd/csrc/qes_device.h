@@ -468,7 +468,8 @@ __device__ __forceinline__ int max_zero_run(W v, int ns, W mask, W* starts) {
   for (int j = 5; j >= 0; --j) {
     const int step = 1 << j;
     if (step < ns) {
-      const W cand = cur & rotc<W, FULL>(R[j], kk % ns, ns, mask);
+      // FULL: the rotate takes its count modulo the word width itself
+      const W cand = cur & rotc<W, FULL>(R[j], FULL ? kk : kk % ns, ns, mask);
       if (cand) {
         cur = cand;
         kk += step;
