@@ -242,6 +242,12 @@ int build_basis(Basis* B, hipStream_t st) {
 
   // lookup structures
   const int64_t n = B->n;
+  {
+    double tab[257];
+    for (int k = 0; k <= 256; ++k) tab[k] = std::sqrt((double)k);
+    QES_HIP(hipMalloc(&B->sqrtn, sizeof(tab)));
+    QES_HIP(hipMemcpy(B->sqrtn, tab, sizeof(tab), hipMemcpyHostToDevice));
+  }
   if (w32) {
     QES_HIP(hipMalloc(&B->keys32, nalloc * sizeof(uint32_t)));
     if (n) hipLaunchKernelGGL(keys32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B->reps, B->keys32, n);

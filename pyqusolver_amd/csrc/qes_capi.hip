@@ -318,6 +318,7 @@ int qes_basis_destroy(qes_basis_t h) {
   if (B.rmap) (void)hipFree(B.rmap);
   if (B.rbase) (void)hipFree(B.rbase);
   if (B.rlo) (void)hipFree(B.rlo);
+  if (B.sqrtn) (void)hipFree(B.sqrtn);
   delete h;
   return 0;
 }

@@ -284,6 +284,9 @@ struct QesLookup {
   const uint16_t* rlo;      // 65536: states with the same low-half popcount and a smaller low half
   int32_t n_up;
   int32_t pad_r;
+  // sqrt(k) for k = 0..256 (stabiliser sizes; |G| <= 256): one cached load
+  // per element instead of an FP64 square root
+  const double* sqrtn;
 };
 
 #define QES_KEY_BITS 56

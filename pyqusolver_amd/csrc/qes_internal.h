@@ -53,8 +53,10 @@ struct Basis {
     L.rlo = rlo;
     L.n_up = group.n_up;
     L.pad_r = 0;
+    L.sqrtn = sqrtn;
     return L;
   }
+  double* sqrtn = nullptr;    // device sqrt(0..256)
   // rank map (ns <= 32 with a U(1) filling): see QesLookup
   uint64_t* rmap = nullptr;
   uint32_t* rbase = nullptr;

@@ -71,7 +71,7 @@ __device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& mis
       miss();
       return;
     }
-    const double nb = sqrt((double)cn.count);
+    const double nb = C.L.sqrtn[cn.count];
     const double cr = C.s_chr[cn.g];
     double fr, fi;
     if (CPLX) {
@@ -113,7 +113,7 @@ __device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& mis
         }
         const int b0 = (int)((r >> p) & 1);
         const double* e = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
-        const double nb = sqrt((double)cnt);
+        const double nb = C.L.sqrtn[cnt];
         f(b, e[0] * nb, CPLX ? -e[1] * nb : 0.0, c * ns + p);
       }
     }
