@@ -225,9 +225,10 @@ int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_ho
 /* Representative lookup used by the matvec: 0 sorted keys (32-bit when
  * ns <= 32; default there) + bucket table, 1 open-addressing hash table,
  * 2 packed 64-bit keys (key | stabiliser size) + bucket table (default for
- * 33..56 sites without intrinsic phases), 3 rank map (ns <= 32 with a U(1)
+ * 33..56 sites without intrinsic phases), 3 rank map (ns <= 48 with a U(1)
  * filling: a bitmap over the colex ranks of the filling's states with a
- * running count, C(ns, n_up)/4 bytes; one 16-byte load per lookup). */
+ * running count, C(ns, n_up)/4 bytes, at most 4 GB; one 16-byte load per
+ * lookup; the default wherever it applies). */
 int qes_basis_set_lookup(qes_basis_t b, int32_t mode);
 /* Rebuild the bucket table for about 2^rows_log2 representatives per bucket
  * (-4..20; negative: several buckets per representative; default -3). */

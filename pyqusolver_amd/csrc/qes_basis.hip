@@ -270,10 +270,10 @@ int build_basis(Basis* B, hipStream_t st) {
     QES_HIP(hipStreamSynchronize(st));
   }
   B->use_hash = false;
-  // rank map by default where it applies (ns <= 32 with a U(1) filling:
-  // C(ns, n_up)/4 bytes, 150 MB at 32 sites): one 16-byte load per lookup
-  // instead of a bucket word and a key (L=32 matvec 3.84 -> 2.88 ms); the
-  // sorted keys and bucket table stay for the other modes
+  // rank map by default where it applies (ns <= 48 with a U(1) filling and
+  // C(ns, n_up)/4 bytes <= 4 GB: 150 MB at 32 sites, 2.3 GB at 36): one
+  // 16-byte load per lookup instead of a bucket word and a key (L=32 matvec
+  // 3.84 -> 2.88 ms); the sorted keys and bucket table stay for the other modes
   int ok = 0;
   if (n > 0 && build_rank_map(B, st, &ok) == 0) B->use_rank = ok != 0;
   else B->use_rank = false;   // no memory for it: the bucketed keys serve
@@ -366,16 +366,24 @@ int build_hash(Basis* B, hipStream_t st) {
 // representatives are ascending and each bitmap word is filled from one
 // contiguous range of them (no atomics).
 // ---------------------------------------------------------------------------
-__global__ void rank_of_reps_kernel(const uint32_t* __restrict__ keys, int64_t n, const uint32_t* __restrict__ rbase,
-                                    const uint16_t* __restrict__ rlo, uint32_t* __restrict__ rank) {
+__global__ void rank_of_reps_kernel(QesLookup L, const uint64_t* __restrict__ reps, int64_t n,
+                                    uint64_t* __restrict__ rank) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t u = keys[i];
-  rank[i] = rbase[u >> 16] + (uint32_t)rlo[u & 0xffffu];
+  const uint64_t v = reps[i];
+  if (L.ns <= 32) {
+    const uint32_t u = (uint32_t)v;
+    rank[i] = L.rbase[u >> 16] + (uint64_t)L.rlo[u & 0xffffu];
+  } else {
+    const uint32_t lo32 = (uint32_t)v;
+    const int row = __popc(lo32) - L.k1min;
+    rank[i] = L.rbase2[v >> 32] + (uint64_t)L.rbase1[(uint32_t)row * 65536u + (lo32 >> 16)] +
+              (uint64_t)L.rlo[lo32 & 0xffffu];
+  }
 }
 
 // word w: bits of the ranks in [64w, 64w + 64) and the number of ranks below 64w
-__global__ void rank_map_fill_kernel(const uint32_t* __restrict__ rank, int64_t n, int64_t nwords,
+__global__ void rank_map_fill_kernel(const uint64_t* __restrict__ rank, int64_t n, int64_t nwords,
                                      uint64_t* __restrict__ rmap) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwords) return;
@@ -383,50 +391,90 @@ __global__ void rank_map_fill_kernel(const uint32_t* __restrict__ rank, int64_t 
   int64_t lo = 0, hi = n;   // first index with rank >= base
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
-    if ((uint64_t)rank[mid] < base) lo = mid + 1; else hi = mid;
+    if (rank[mid] < base) lo = mid + 1; else hi = mid;
   }
   uint64_t bits = 0;
-  for (int64_t i = lo; i < n && (uint64_t)rank[i] < base + 64ull; ++i) bits |= 1ull << ((uint64_t)rank[i] - base);
+  for (int64_t i = lo; i < n && rank[i] < base + 64ull; ++i) bits |= 1ull << (rank[i] - base);
   rmap[2 * w] = bits;
   rmap[2 * w + 1] = (uint64_t)lo;
 }
 
-int build_rank_map(Basis* B, hipStream_t st, int* eligible) {
+int build_rank_map(Basis* B, hipStream_t st, int* eligible, double max_bytes) {
   const QesGroup& G = B->group;
   *eligible = 0;
-  if (G.ns > 32 || G.n_up < 0 || !B->keys32) return 0;
+  if (G.ns > 48 || G.n_up < 0 || G.n_up > G.ns) return 0;
+  const int k = G.n_up;
+  const uint64_t nstates = host_binom(G.ns, k);   // ranks 0 .. nstates-1
+  const int64_t nwords = (int64_t)((nstates + 63) / 64);
+  if ((double)nwords * 16.0 > max_bytes) return 0;
   *eligible = 1;
   if (B->rmap) return 0;
-  const int k = G.n_up;
-  // rlo[l] = #{l' < l : popc(l') = popc(l)}; rbase[h] = sum over h' < h of C(16, k - popc(h'))
+  // rlo[l] = #{l' < l : popc(l') = popc(l)}
   std::vector<uint16_t> rlo(65536);
-  std::vector<uint32_t> rbase(65536);
   {
     uint32_t seen[17] = {0};
-    for (uint32_t l = 0; l < 65536; ++l) {
-      const int p = __builtin_popcount(l);
-      rlo[l] = (uint16_t)seen[p]++;
-    }
+    for (uint32_t l = 0; l < 65536; ++l) rlo[l] = (uint16_t)seen[__builtin_popcount(l)]++;
+  }
+  // ns <= 32: rbase[h] = sum over h' < h of C(16, k - popc(h'))
+  // 33..48:   rbase2[h] = sum over h' < h of C(32, k - popc(h'));
+  //           rbase1[k1][h] = sum over h' < h of C(16, k1 - popc(h')), k1 = k1min..
+  std::vector<uint32_t> rbase, rbase1;
+  std::vector<uint64_t> rbase2;
+  int k1min = 0, nrows = 0;
+  if (G.ns <= 32) {
+    rbase.resize(65536);
     uint64_t acc = 0;
     for (uint32_t h = 0; h < 65536; ++h) {
       rbase[h] = (uint32_t)acc;
       const int m = k - __builtin_popcount(h);
       acc += (m >= 0 && m <= 16) ? host_binom(16, m) : 0;
     }
+  } else {
+    rbase2.resize(65536);
+    uint64_t acc = 0;
+    for (uint32_t h = 0; h < 65536; ++h) {
+      rbase2[h] = acc;
+      const int m = k - __builtin_popcount(h);
+      acc += (m >= 0 && m <= 32) ? host_binom(32, m) : 0;
+    }
+    k1min = std::max(0, k - (G.ns - 32));
+    const int k1max = std::min(k, 32);
+    nrows = k1max - k1min + 1;
+    rbase1.resize((size_t)nrows * 65536);
+    for (int r = 0; r < nrows; ++r) {
+      const int k1 = k1min + r;
+      uint64_t a = 0;
+      for (uint32_t h = 0; h < 65536; ++h) {
+        rbase1[(size_t)r * 65536 + h] = (uint32_t)a;
+        const int m = k1 - __builtin_popcount(h);
+        a += (m >= 0 && m <= 16) ? host_binom(16, m) : 0;
+      }
+    }
   }
-  const uint64_t nstates = host_binom(G.ns, k);   // ranks 0 .. nstates-1
-  const int64_t nwords = (int64_t)((nstates + 63) / 64);
   const int64_t n = B->n;
-  uint32_t* d_rank = nullptr;
-  hipError_t e = hipMalloc(&B->rbase, 65536 * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&B->rlo, 65536 * sizeof(uint16_t));
-  if (e == hipSuccess) e = hipMalloc(&B->rmap, (size_t)std::max<int64_t>(nwords, 1) * 16);
-  if (e == hipSuccess) e = hipMalloc(&d_rank, (size_t)std::max<int64_t>(n, 1) * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMemcpy(B->rbase, rbase.data(), 65536 * sizeof(uint32_t), hipMemcpyHostToDevice);
+  uint64_t* d_rank = nullptr;
+  hipError_t e = hipMalloc(&B->rlo, 65536 * sizeof(uint16_t));
   if (e == hipSuccess) e = hipMemcpy(B->rlo, rlo.data(), 65536 * sizeof(uint16_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !rbase.empty()) e = hipMalloc(&B->rbase, rbase.size() * sizeof(uint32_t));
+  if (e == hipSuccess && !rbase.empty())
+    e = hipMemcpy(B->rbase, rbase.data(), rbase.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !rbase2.empty()) e = hipMalloc(&B->rbase2, rbase2.size() * sizeof(uint64_t));
+  if (e == hipSuccess && !rbase2.empty())
+    e = hipMemcpy(B->rbase2, rbase2.data(), rbase2.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !rbase1.empty()) e = hipMalloc(&B->rbase1, rbase1.size() * sizeof(uint32_t));
+  if (e == hipSuccess && !rbase1.empty())
+    e = hipMemcpy(B->rbase1, rbase1.data(), rbase1.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&B->rmap, (size_t)std::max<int64_t>(nwords, 1) * 16);
+  if (e == hipSuccess) e = hipMalloc(&d_rank, (size_t)std::max<int64_t>(n, 1) * sizeof(uint64_t));
   if (e == hipSuccess) {
-    if (n) hipLaunchKernelGGL(rank_of_reps_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B->keys32, n,
-                              B->rbase, B->rlo, d_rank);
+    B->k1min = k1min;
+    QesLookup L = B->lookup();
+    L.rbase = B->rbase;
+    L.rlo = B->rlo;
+    L.rbase2 = B->rbase2;
+    L.rbase1 = B->rbase1;
+    if (n) hipLaunchKernelGGL(rank_of_reps_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, B->reps, n,
+                              d_rank);
     if (nwords) hipLaunchKernelGGL(rank_map_fill_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, st,
                                    d_rank, n, nwords, B->rmap);
     e = hipGetLastError();
@@ -434,11 +482,13 @@ int build_rank_map(Basis* B, hipStream_t st, int* eligible) {
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (d_rank) (void)hipFree(d_rank);
   if (e != hipSuccess) {
-    for (void* p : {(void*)B->rbase, (void*)B->rlo, (void*)B->rmap})
+    for (void* p : {(void*)B->rbase, (void*)B->rlo, (void*)B->rmap, (void*)B->rbase2, (void*)B->rbase1})
       if (p) (void)hipFree(p);
     B->rbase = nullptr;
     B->rlo = nullptr;
     B->rmap = nullptr;
+    B->rbase2 = nullptr;
+    B->rbase1 = nullptr;
     return check_hip(e, "build_rank_map");
   }
   B->rmap_words = nwords;

@@ -319,6 +319,8 @@ int qes_basis_destroy(qes_basis_t h) {
   if (B.rbase) (void)hipFree(B.rbase);
   if (B.rlo) (void)hipFree(B.rlo);
   if (B.sqrtn) (void)hipFree(B.sqrtn);
+  if (B.rbase2) (void)hipFree(B.rbase2);
+  if (B.rbase1) (void)hipFree(B.rbase1);
   delete h;
   return 0;
 }
@@ -474,7 +476,7 @@ int qes_basis_set_lookup(qes_basis_t b, int32_t mode) {
   if (mode == 3) {
     int ok = 0;
     if (build_rank_map(&b->b, nullptr, &ok)) return -1;
-    if (!ok) return fail("no rank map for this basis (needs ns <= 32 and a U(1) filling)");
+    if (!ok) return fail("no rank map for this basis (needs ns <= 48, a U(1) filling and C(ns, n_up)/4 bytes <= 4 GB)");
   }
   b->b.use_hash = mode == 1;
   b->b.use_packed = mode == 2;
@@ -606,7 +608,7 @@ int qes_lanczos_ritz(qes_lanczos_t h, const double* coeffs_host, int32_t k, void
 int qes_lanczos_accumulate(qes_lanczos_t h, const double* coeffs_host, int32_t nvec, void* out_dev, void* stream) {
   if (!h || !coeffs_host || !out_dev) return fail("null argument");
   (void)stream;
-  return lanczos_accumulate(h->L, coeffs_host, nvec, out_dev);
+  return lanczos_accumulate(h->L, coeffs_host, nvec, out_dev, (hipStream_t)stream);
 }
 
 int qes_lanczos_destroy(qes_lanczos_t h) {

@@ -274,16 +274,21 @@ struct QesLookup {
   // packed keys (ns <= 56): key | (|Stab(rep)| - 1) << 56, sorted by key;
   // a hit on the packed array also yields the stabiliser size
   const uint64_t* keyc;
-  // rank map (ns <= 32 with a U(1) filling n_up; used when non-null).  The
-  // colex rank of v among the states of that filling is
-  // c = rbase[v >> 16] + rlo[v & 0xffff]; rmap[c / 64] holds the bit mask
-  // of the representatives among ranks 64*(c/64) ... +63 and the number of
-  // representatives of lower rank, so one 16-byte load resolves a lookup.
+  // rank map (ns <= 48 with a U(1) filling n_up; used when non-null).  The
+  // colex rank of v among the states of that filling is, for ns <= 32,
+  // c = rbase[v >> 16] + rlo[v & 0xffff], and for 33..48 sites
+  // c = rbase2[v >> 32] + rbase1[(k1 - k1min) * 65536 + (v >> 16 & 0xffff)]
+  // + rlo[v & 0xffff] with k1 = popcount of the low 32 bits; rmap[c / 64]
+  // holds the bit mask of the representatives among ranks 64*(c/64) ... +63
+  // and the number of representatives of lower rank, so one 16-byte load
+  // resolves a lookup.
   const uint64_t* rmap;     // pairs {bits, count before}
-  const uint32_t* rbase;    // 65536: states of the filling with a smaller high half
+  const uint32_t* rbase;    // ns <= 32: 65536 states of the filling with a smaller high half
   const uint16_t* rlo;      // 65536: states with the same low-half popcount and a smaller low half
+  const uint64_t* rbase2;   // 33..48 sites: 65536, states with a smaller top chunk
+  const uint32_t* rbase1;   // 33..48 sites: rows k1 = k1min.., 65536 each
   int32_t n_up;
-  int32_t pad_r;
+  int32_t k1min;
   // sqrt(k) for k = 0..256 (stabiliser sizes; |G| <= 256): one cached load
   // per element instead of an FP64 square root
   const double* sqrtn;
@@ -354,12 +359,20 @@ __device__ __forceinline__ int64_t lookup_packed(const QesLookup& L, uint64_t v,
 
 // rank-map lookup of a 32-bit state; -1 if it is not a representative
 __device__ __forceinline__ int64_t lookup_rank(const QesLookup& L, uint64_t v) {
-  if (L.ns < 64 && (v >> L.ns)) return -1;   // outside the ns-bit states (ns <= 32 here)
-  const uint32_t u = (uint32_t)v;
-  if (__popc(u) != L.n_up) return -1;
-  const uint32_t c = L.rbase[u >> 16] + (uint32_t)L.rlo[u & 0xffffu];
+  if (L.ns < 64 && (v >> L.ns)) return -1;   // outside the ns-bit states (ns <= 48 here)
+  if (__popcll(v) != L.n_up) return -1;
+  uint64_t c;
+  if (L.ns <= 32) {
+    const uint32_t u = (uint32_t)v;
+    c = L.rbase[u >> 16] + (uint32_t)L.rlo[u & 0xffffu];
+  } else {
+    const uint32_t lo32 = (uint32_t)v;
+    const int row = __popc(lo32) - L.k1min;   // in 0..16 for a state of the filling
+    c = L.rbase2[v >> 32] + (uint64_t)L.rbase1[(uint32_t)row * 65536u + (lo32 >> 16)] +
+        (uint64_t)L.rlo[lo32 & 0xffffu];
+  }
   const ulonglong2 e = reinterpret_cast<const ulonglong2*>(L.rmap)[c >> 6];
-  const uint32_t bit = c & 63u;
+  const uint32_t bit = (uint32_t)(c & 63u);
   if (!((e.x >> bit) & 1ull)) return -1;
   return (int64_t)e.y + (int64_t)__popcll(e.x & ((1ull << bit) - 1ull));
 }

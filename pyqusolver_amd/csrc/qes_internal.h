@@ -46,21 +46,26 @@ struct Basis {
     L.ns = group.ns;
     L.hash = use_hash ? hash : nullptr;
     L.hbits = hbits;
-    L.packed = (use_packed && keyc) ? 1 : 0;
+    L.packed = (use_packed && keyc && !(use_rank && rmap)) ? 1 : 0;   // the rank map takes precedence
     L.keyc = keyc;
     L.rmap = (use_rank && rmap) ? rmap : nullptr;
     L.rbase = rbase;
     L.rlo = rlo;
+    L.rbase2 = rbase2;
+    L.rbase1 = rbase1;
     L.n_up = group.n_up;
-    L.pad_r = 0;
+    L.k1min = k1min;
     L.sqrtn = sqrtn;
     return L;
   }
   double* sqrtn = nullptr;    // device sqrt(0..256)
-  // rank map (ns <= 32 with a U(1) filling): see QesLookup
+  // rank map (ns <= 48 with a U(1) filling): see QesLookup
   uint64_t* rmap = nullptr;
   uint32_t* rbase = nullptr;
   uint16_t* rlo = nullptr;
+  uint64_t* rbase2 = nullptr;
+  uint32_t* rbase1 = nullptr;
+  int k1min = 0;
   int64_t rmap_words = 0;
   bool use_rank = false;
   uint64_t* keyc = nullptr;   // device packed keys (key | (|Stab|-1) << 56), ns <= 56
@@ -120,9 +125,9 @@ struct Op {
 // launchers (qes_basis.hip)
 int build_basis(Basis* B, hipStream_t st);
 int build_hash(Basis* B, hipStream_t st);
-// rank map of a sector with ns <= 32 and a U(1) filling; *eligible = 0 (and
-// nothing built) otherwise
-int build_rank_map(Basis* B, hipStream_t st, int* eligible);
+// rank map of a sector with ns <= 48 and a U(1) filling whose map takes at
+// most max_bytes; *eligible = 0 (and nothing built) otherwise
+int build_rank_map(Basis* B, hipStream_t st, int* eligible, double max_bytes = 4e9);
 int build_bucket(Basis* B, int rows_log2, hipStream_t st);
 // reduce == 0: full = expand(x_red) (2^ns entries); 1: x_red = reduce(full)
 int expand_reduce(const Basis* B, int reduce, const double* x_red, double* full, int dtype, hipStream_t st);

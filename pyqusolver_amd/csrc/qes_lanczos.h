@@ -10,7 +10,7 @@ int lanczos_start(Lanczos* L, const void* v0, hipStream_t user);
 int lanczos_run(Lanczos* L, int nsteps, int reorth, int use_graph);
 #define LANCZOS_MAX_ACCUM 16
 // out[i] += c[i] * (current Lanczos vector), i < nvec
-int lanczos_accumulate(Lanczos* L, const double* c_host, int nvec, void* out);
+int lanczos_accumulate(Lanczos* L, const double* c_host, int nvec, void* out, hipStream_t user);
 int lanczos_get(Lanczos* L, double* alpha, double* beta);
 int lanczos_second_passes(Lanczos* L, int* count);
 int lanczos_combine(Lanczos* L, const double* c_host, int k, void* out, hipStream_t user);

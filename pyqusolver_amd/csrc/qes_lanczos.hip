@@ -556,12 +556,30 @@ int lanczos_get(Lanczos* L, double* alpha, double* beta) {
   return 0;
 }
 
+// Make the handle's stream wait for the work already queued on the caller's
+// stream (e.g. the zero fill of an output buffer): the handle's stream is a
+// non-blocking one of its own, so nothing orders the two otherwise.
+static int order_after(Lanczos* L, hipStream_t user) {
+  if (user == L->stream) return 0;
+  hipEvent_t ev;
+  QES_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  hipError_t e = hipEventRecord(ev, user);
+  if (e == hipSuccess) e = hipStreamWaitEvent(L->stream, ev, 0);
+  (void)hipEventDestroy(ev);
+  if (e != hipSuccess) return check_hip(e, "ordering after the caller's stream");
+  return 0;
+}
+
 int lanczos_combine(Lanczos* L, const double* c_host, int k, void* out, hipStream_t user) {
   if (!L->store) return fail("Lanczos basis was not stored");
   if (k < 1 || k > L->steps_done + 1 || k > L->maxv) return fail("bad number of Ritz coefficients");
   double* dc = nullptr;
   QES_HIP(hipMalloc(&dc, sizeof(double) * k));
   QES_HIP(hipMemcpy(dc, c_host, sizeof(double) * k, hipMemcpyHostToDevice));
+  if (order_after(L, user)) {
+    (void)hipFree(dc);
+    return -1;
+  }
   const int g = grid_for(L->len);
   if (L->dtype == 1)
     hipLaunchKernelGGL(combine_kernel<true>, dim3(g), dim3(RB), 0, L->stream, L->V, L->ldv, k, dc, (double*)out, L->n);
@@ -570,7 +588,6 @@ int lanczos_combine(Lanczos* L, const double* c_host, int k, void* out, hipStrea
   QES_HIP(hipGetLastError());
   QES_HIP(hipStreamSynchronize(L->stream));
   QES_HIP(hipFree(dc));
-  (void)user;
   return 0;
 }
 
@@ -588,9 +605,10 @@ __global__ void __launch_bounds__(RB) accumulate_kernel(const double* __restrict
   }
 }
 
-int lanczos_accumulate(Lanczos* L, const double* c_host, int nvec, void* out) {
+int lanczos_accumulate(Lanczos* L, const double* c_host, int nvec, void* out, hipStream_t user) {
   if (nvec < 1 || nvec > LANCZOS_MAX_ACCUM) return fail("accumulate takes 1.." + std::to_string(LANCZOS_MAX_ACCUM) +
                                                         " vectors");
+  if (order_after(L, user)) return -1;
   AccumCoef cf{};
   for (int i = 0; i < nvec; ++i) cf.c[i] = c_host[i];
   const int g = grid_for(L->len);

@@ -194,6 +194,34 @@ def test_rank_map_lookup_matches_sorted_keys(case):
     assert torch.equal(H.matvec_device(x), y0)
 
 
+@pytest.mark.parametrize("case", [("chain", 36, 1, [("tx", 0), ("R", 1)], 3, "xxz"),
+                                  ("chain", 34, 1, [("tx", 5)], 4, "xxz"),
+                                  ("chain", 40, 1, [("tx", 0)], 3, "j1j2"),
+                                  ("square", 6, 6, [("tx", 1), ("ty", 0)], 3, "j1j2")], ids=_ids)
+def test_rank_map_beyond_32_sites(case):
+    """33..48 sites: the rank map's three-chunk colex rank (top 16 bits, then
+    a row per popcount of the low 32 bits, then the low 16 bits) finds the
+    oracle's representatives: products against the oracle and bitwise equal
+    to the sorted 64-bit keys, index maps equal."""
+    import torch
+    kind, lx, ly, gens, n_up, model = case
+    hs, H, grp, b, t = build_pair(kind, lx, ly, gens, n_up, model)
+    np.testing.assert_array_equal(hs.representative_list, b.reps)
+    c, s, co = t.arrays()
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(hs.nh) + 1j * rng.standard_normal(hs.nh)
+    yo = O.push_matvec(b, t.ns, c, s, co, t.is_complex, x)
+    y = H.matvec_fun(x)
+    assert np.linalg.norm(y - yo) <= MATVEC_TOL * np.linalg.norm(yo)
+    idx0, g0 = hs.map_states(b.states)
+    np.testing.assert_array_equal(idx0, b.state_rep)
+    hs.set_lookup("sorted")
+    np.testing.assert_array_equal(H.matvec_fun(x), y)
+    np.testing.assert_array_equal(hs.map_states(b.states)[0], idx0)
+    hs.set_lookup("rank")
+    np.testing.assert_array_equal(H.matvec_fun(x), y)
+
+
 def test_rank_map_refused_without_u1():
     from pyqusolver_amd._lib import QesError
     hs, H, grp, b, t = build_pair("chain", 12, 1, [("tx", 0), ("R", 1)], None, "j1j2_field")
@@ -401,6 +429,22 @@ def test_square6x6_scale_properties():
     np.testing.assert_allclose(hs.normalization[sample] ** 2, cnt, rtol=1e-14)
     H.assemble(True, max_gb=64)
     assert torch.equal(H.matvec_device(x), y)
+    # Ritz vector without a stored basis at full size (the accumulation must
+    # wait for the zero fill of its output, queued on another stream): equal
+    # to the stored-basis Ritz vector, residual equal to |beta_m s_m|
+    from pyqusolver_amd.lanczos import lanczos_ground_state
+    del x, y
+    r2 = lanczos_ground_state(H, k=1, tol=1e-10, maxiter=20, seed=1, reorth=False, store_basis=False,
+                              compute_eigvec=True)
+    r1 = lanczos_ground_state(H, k=1, tol=1e-10, maxiter=20, seed=1, reorth=False, store_basis=True,
+                              compute_eigvec=True)
+    v1, v2 = r1.eigenvectors, r2.eigenvectors
+    assert (torch.linalg.vector_norm(v1 - v2) / torch.linalg.vector_norm(v1)).item() <= 1e-12
+    res = torch.linalg.vector_norm(H.matvec_device(v2) - float(r2.eigenvalues[0]) * v2) / torch.linalg.vector_norm(v2)
+    assert abs(res.item() - r2.residual_bound) <= 1e-6 * r2.residual_bound + 1e-12
+    del v1, v2, r1, r2
+    x = torch.randn(n, dtype=torch.float64, device="cuda", generator=torch.Generator(device="cuda").manual_seed(3))
+    y = H.matvec_device(x)
     H.assemble(False)
     # a stored basis of 1024 vectors (1 TB) cannot be allocated: the create
     # fails with a message naming it, frees what it had taken, and leaves no
