@@ -570,7 +570,7 @@ def bench_plain(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); this build runs no multi-GPU path.",
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the library has no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
     if a.config == "batched":

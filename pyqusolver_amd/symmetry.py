@@ -215,7 +215,7 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
             if n_up != ns // 2 or ns % 2 != 0:
                 continue          # X/Y need half filling
         if gen == SymmetryGenerators.Translation_z:
-            raise NotImplementedError(f"{gen.name} is not part of this build")
+            raise NotImplementedError(f"{gen.name} is not supported (spin chains and 2D lattices only)")
         if gen == SymmetryGenerators.Inversion:
             if lattice is None:
                 raise ValueError("inversion symmetry requires a lattice")

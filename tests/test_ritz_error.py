@@ -6,7 +6,7 @@ r = |beta_m s_m| (s: unit eigenvector of T_m), falls below tol*max(1,|theta|)
 Pins: r equals ||H y - theta y|| of the Ritz vector built from an explicit
 Krylov basis (the identity the test rests on); the library's host routine
 (qes_tridiag_ritz, inverse iteration on T_m) agrees with the dense
-eigendecomposition of T_m that the Python driver uses; r bounds the true
+eigendecomposition of T_m that the Python solver uses; r bounds the true
 error of a converging run; with an unresolved near-degenerate pair, where a
 test on successive Ritz values stops early, r stays above tol.
 """
