@@ -217,7 +217,11 @@ int qes_op_count_outside(qes_op_t op, int64_t* outside);
  * from the brute-force one (representative, stabiliser size, character).
  */
 int qes_op_set_canonicalizer(qes_op_t op, int32_t kind);
-/* Kernel options: "bond_order_uniform" (0/1), "direct_lookup" (0/1), "xcd_remap" (0/1). */
+/* Kernel options: "direct_lookup" (look each exchanged state up before
+ * canonicalising it; hits skip the canonicalisation), "bond_order_uniform"
+ * (the wavefront walks the bonds in one order in that pass): 1 on, 0 off,
+ * -1 (default) on where the basis uses a rank map; "xcd_remap" (0/1).  The
+ * options change the order of the sums (results agree to rounding). */
 int qes_op_set_option(qes_op_t op, const char* name, int64_t value);
 int qes_op_get_plan(qes_op_t op, int32_t* canon, int32_t* n_classes, int32_t* n_generic);
 int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_host, int64_t n, int64_t* mismatches);

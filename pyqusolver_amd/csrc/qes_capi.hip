@@ -395,8 +395,8 @@ int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
   if (!h || !name) return fail("null argument");
   Op& op = h->op;
   const std::string n(name);
-  if (n == "bond_order_uniform") op.bond_order_uniform = value ? 1 : 0;
-  else if (n == "direct_lookup") op.direct_lookup = value ? 1 : 0;
+  if (n == "bond_order_uniform") op.bond_order_uniform = value < 0 ? -1 : (value ? 1 : 0);
+  else if (n == "direct_lookup") op.direct_lookup = value < 0 ? -1 : (value ? 1 : 0);
   else if (n == "xcd_remap") op.xcd_remap = value ? 1 : 0;
   else return fail("unknown option " + n + " (bond_order_uniform, direct_lookup, xcd_remap)");
   return build_plan(&op, op.plan.canon == pick_canon_public(op.basis->group) ? -1 : op.plan.canon);

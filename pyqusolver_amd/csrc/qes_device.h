@@ -292,6 +292,7 @@ struct QesLookup {
   // sqrt(k) for k = 0..256 (stabiliser sizes; |G| <= 256): one cached load
   // per element instead of an FP64 square root
   const double* sqrtn;
+  const double* norms;      // N_b of every representative (direct rank-map hits)
 };
 
 #define QES_KEY_BITS 56

@@ -94,10 +94,14 @@ __device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& mis
     const int d = C.s_cls_d[c];
     const W pm = (W)C.s_cls_pm[c];
     const W act = (r ^ rotl_ns<W>(r, d, ns, mask)) & pm;
-    const bool direct = C.L.keyc && C.direct;
+    // direct hits: packed keys carry |Stab|; a rank-map hit takes N_b from
+    // the norms (the first minimiser of a representative has character 1)
+    // direct / uniform: 1 on, 0 off, -1 on exactly where a rank map serves
+    const bool direct = C.direct > 0 ? (C.L.keyc || C.L.rmap) : (C.direct < 0 && C.L.rmap);
+    const bool uniform = C.uniform > 0 || (C.uniform < 0 && C.L.rmap);
     W fail = direct ? (W)0 : act;
     if (direct) {
-      W m = C.uniform ? pm : act;   // uniform order, or each lane's active bonds
+      W m = uniform ? pm : act;   // uniform order, or each lane's active bonds
       while (m) {
         const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
         m &= m - 1;
@@ -106,14 +110,14 @@ __device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& mis
         if (p1 < 0) p1 += ns;
         const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
         int cnt = 0;
-        const int64_t b = lookup_packed(C.L, (uint64_t)sp, &cnt);
+        const int64_t b = C.L.rmap ? lookup_rank(C.L, (uint64_t)sp) : lookup_packed(C.L, (uint64_t)sp, &cnt);
         if (b < 0) {
           fail |= (W)1 << p;
           continue;
         }
         const int b0 = (int)((r >> p) & 1);
         const double* e = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
-        const double nb = C.L.sqrtn[cnt];
+        const double nb = C.L.rmap ? C.L.norms[b] : C.L.sqrtn[cnt];
         f(b, e[0] * nb, CPLX ? -e[1] * nb : 0.0, c * ns + p);
       }
     }

@@ -85,7 +85,10 @@ def test_matvec_plan_variants_agree(case):
                 hs.set_lookup(mode, bucket_rows_log2=rows_log2)
             except QesError:
                 continue   # mode not available for this basis
-            assert torch.equal(H.matvec_device(x), y_fast), (mode, rows_log2)
+            # the rank map's direct pass sums in another order: equal to rounding
+            y = H.matvec_device(x)
+            rel = (torch.linalg.vector_norm(y - y_fast) / torch.linalg.vector_norm(y_fast)).item()
+            assert rel <= 1e-14, (mode, rows_log2, rel)
 
 
 def test_per_coupling_outputs_sum_to_matvec():
