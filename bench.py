@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # (KiB per dispatch, FETCH scaled x2.0 and WRITE x1.0 by a 1 GiB device copy
 # measured in the same runs); filled from the committed profiles
 TRAFFIC = {
-    # profiles/matvec_kernel_chain32_j1j2.md: 4,236,076 KiB x 2.0 + 73,384 KiB x 1.0
-    "plain": 4236076 * 1024 * 2 + 73384 * 1024,
+    # profiles/matvec_kernel_chain32_j1j2.md: 4,764,474 KiB x 2.0 + 73,384 KiB x 1.0
+    "plain": 4764474 * 1024 * 2 + 73384 * 1024,
 }
 
 
