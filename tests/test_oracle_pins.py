@@ -321,3 +321,193 @@ def test_inversion_sectors_split_the_momentum_sector(kind, lx, ly, n_up):
     assert len(e_pm) == len(e_all)
     np.testing.assert_allclose(np.sort(e_pm), e_all, atol=1e-12)
 
+
+
+# ------------------------------------------------- XXZ term list (xxz.py)
+def _pauli_string(ns, ops):
+    """Sparse 2^ns matrix of a product of Pauli matrices {site: 'x'|'y'|'z'}
+    with site i on bit ns-1-i (leftmost Kronecker factor = site 0)."""
+    import scipy.sparse as sp
+    P = {"x": np.array([[0, 1], [1, 0]], dtype=np.complex128),
+         "y": np.array([[0, -1j], [1j, 0]], dtype=np.complex128),
+         "z": np.array([[1, 0], [0, -1]], dtype=np.complex128)}
+    M = sp.identity(1, dtype=np.complex128, format="csr")
+    for site in range(ns):
+        f = sp.csr_matrix(P[ops[site]]) if site in ops else sp.identity(2, dtype=np.complex128, format="csr")
+        M = sp.kron(M, f, format="csr")
+    return M
+
+
+def test_xxz_two_site_closed_form_fixes_signs():
+    """Chain L=2 (PBC: the bond is listed twice, 2 H_bond).  With
+    H_bond = -jxy (SxSx + SySy) - jz SzSz (S = sigma/2) the spectrum is fixed
+    in closed form: triplet m = +-1 at -jz/4, the symmetric m = 0 triplet at
+    -jxy/2 + jz/4 and the singlet at jxy/2 + jz/4 -- so the sign of jxy
+    decides which of the two m = 0 states is lower."""
+    rng = np.random.default_rng(7)
+    for _ in range(4):
+        jxy, jz = rng.uniform(-2, 2, 2)
+        c, s, co = O.terms_xxz(O.OLattice("chain", 2), jxy=jxy, jz=jz).arrays()
+        H = O.full_dense_hamiltonian(2, c, s, co, True)
+        up, dn = 3, 0                       # bit 1 = spin up (+1/2)
+        trip0 = np.zeros(4, complex)
+        trip0[[1, 2]] = 1 / np.sqrt(2)
+        sing = np.zeros(4, complex)
+        sing[1], sing[2] = 1 / np.sqrt(2), -1 / np.sqrt(2)
+        for vec, e in ((np.eye(4)[up], -jz / 4), (np.eye(4)[dn], -jz / 4),
+                       (trip0, -jxy / 2 + jz / 4), (sing, jxy / 2 + jz / 4)):
+            np.testing.assert_allclose(H @ vec, 2 * e * vec, atol=1e-14)
+
+
+def test_xxz_fields_fix_signs():
+    """-hz Sz_i and -hx Sx_i per site: the saturated state (all bits 1, Sz =
+    +1/2 each) has energy -jz n_bonds/4 - sum_i hz_i/2, and the product of
+    +x eigenstates has <H> = -sum hx/2 at jxy = jz = 0; site i is bit ns-1-i,
+    checked with a random field on one saturated-but-one state."""
+    L = 6
+    lat = O.OLattice("chain", L)
+    rng = np.random.default_rng(3)
+    hz = rng.uniform(-5, 5, L)
+    c, s, co = O.terms_xxz(lat, jxy=0.6, jz=0.9, hz=hz).arrays()
+    H = O.full_dense_hamiltonian(L, c, s, co, True)
+    full = (1 << L) - 1
+    assert abs(H[full, full] - (-0.9 * L / 4 - hz.sum() / 2)) < 1e-13
+    for site in range(L):        # spin of `site` down: its field term flips sign
+        st = full ^ (1 << (L - 1 - site))
+        ez = -0.9 * (L - 2) / 4 + 0.9 * 2 / 4     # two of the L bonds now antiparallel
+        assert abs(H[st, st].real - (ez - hz.sum() / 2 + hz[site])) < 1e-13
+    hx = rng.uniform(-2, 2, L)
+    c, s, co = O.terms_xxz(lat, jxy=0.0, jz=0.0, hx=hx).arrays()
+    H = O.full_dense_hamiltonian(L, c, s, co, True)
+    plus = np.ones(1 << L) / np.sqrt(1 << L)
+    assert abs(plus @ H @ plus - (-hx.sum() / 2)) < 1e-13
+
+
+def test_xxz_at_isotropy_equals_heisenberg_and_known_e0():
+    """XXZ(jxy = jz = J) is -J sum S.S, the same operator as J1J2Model(J1=J,
+    J2=0) (random bond-resolved J, L=10); XXZ(jxy=-1, jz=-1) on the L=16
+    ring at Sz=0, k=0 has the antiferromagnet's E0 = -7.14229636061679.  The
+    sign of jxy alone is fixed by the L=14 ring, where the AF ground state
+    sits at k = pi: per-momentum spectra of XXZ(-1, -1) and of the
+    Heisenberg form must agree sector by sector (a flipped jxy moves the
+    ground state to k = 0 when the number of down spins is odd)."""
+    lat = O.OLattice("chain", 10)
+    J = np.random.default_rng(11).uniform(-1, 1, 10)
+    c1, s1, co1 = O.terms_xxz(lat, jxy=J, jz=J).arrays()
+    c2, s2, co2 = O.terms_j1j2(lat, J1=J, J2=0.0).arrays()
+    H1 = O.full_dense_hamiltonian(10, c1, s1, co1, True)
+    H2 = O.full_dense_hamiltonian(10, c2, s2, co2, False)
+    np.testing.assert_allclose(H1, H2, atol=1e-14)
+    lat16 = O.OLattice("chain", 16)
+    b = O.scan_basis(O.Group(lat16, [O.Generator("tx", 0)], n_up=8))
+    c, s, co = O.terms_xxz(lat16, jxy=-1.0, jz=-1.0).arrays()
+    e0 = np.linalg.eigvalsh(O.reduced_dense(b, 16, c, s, co, True))[0]
+    assert abs(e0 - (-7.14229636061679)) < 1e-11
+    lat14 = O.OLattice("chain", 14)
+    cx, sx, cox = O.terms_xxz(lat14, jxy=-1.0, jz=-1.0).arrays()
+    cf, sf, cof = O.terms_xxz(lat14, jxy=1.0, jz=-1.0).arrays()
+    e_af, e_flip = [], []
+    for k in range(14):
+        bk = O.scan_basis(O.Group(lat14, [O.Generator("tx", k)], n_up=7))
+        e_af.append(np.linalg.eigvalsh(O.reduced_dense(bk, 14, cx, sx, cox, True))[0])
+        e_flip.append(np.linalg.eigvalsh(O.reduced_dense(bk, 14, cf, sf, cof, True))[0])
+    assert int(np.argmin(e_af)) == 7                   # AF ring, L/2 odd: k = pi
+    np.testing.assert_allclose(e_flip, np.roll(e_af, 7), atol=1e-11)   # jxy -> -jxy: k -> k + pi
+    assert int(np.argmin(e_flip)) == 0
+
+
+# --------------------------- Kitaev-Heisenberg bond labels (heisenberg_kitaev.py)
+def _hk_bond_labels(lat):
+    """Bond -> Kitaev label read back from the term list at J = h = 0 with
+    distinct K per direction."""
+    c, s, co = O.terms_heisenberg_kitaev(lat, K=(1.0, 2.0, 3.0), J=0.0).arrays()
+    lab = {O.SIG_X_C: "x", O.SIG_Y_C: "y", O.SIG_Z_C: "z"}
+    out = {}
+    for code, (i, j), v in zip(c, s, co):
+        out.setdefault((int(i), int(j)), []).append((lab[int(code)], float(np.real(v))))
+    return out
+
+
+@pytest.mark.parametrize("lx,ly", [(2, 3), (3, 2), (3, 4)])
+def test_hk_each_site_has_one_bond_of_each_label(lx, ly):
+    lat = O.OLattice("honeycomb", lx, ly)
+    labels = _hk_bond_labels(lat)
+    per_site = {i: [] for i in range(lat.ns)}
+    kval = {"x": -4.0, "y": -8.0, "z": -12.0}    # -4 K_alpha (CORR_TERM_MULT = 4, K subtracted)
+    for (i, j), terms in labels.items():
+        assert len(terms) == 1
+        a, v = terms[0]
+        assert abs(v - kval[a]) < 1e-14
+        per_site[i].append(a)
+        per_site[j].append(a)
+    for i, labs in per_site.items():
+        assert sorted(labs) == ["x", "y", "z"], (i, labs)
+
+
+def _six_cycles(ns, bonds):
+    adj = {i: set() for i in range(ns)}
+    for i, j in bonds:
+        adj[i].add(j)
+        adj[j].add(i)
+    cycles = set()
+    for s0 in range(ns):
+        stack = [(s0, [s0])]
+        while stack:
+            v, path = stack.pop()
+            if len(path) == 6:
+                if s0 in adj[v]:
+                    cycles.add(frozenset(path))
+                continue
+            for w in adj[v]:
+                if w not in path and w > s0:
+                    stack.append((w, path + [w]))
+    return [sorted(c) for c in cycles]
+
+
+@pytest.mark.parametrize("lx,ly", [(2, 3), (3, 2)])
+def test_hk_plaquette_operators_commute_with_kitaev_hamiltonian(lx, ly):
+    """At J = h = 0 every closed loop operator W = prod_{j in loop}
+    sigma_j^{alpha_j} (alpha_j: label of the bond at j that leaves the loop)
+    commutes with H (Kitaev's conserved fluxes).  A mislabelled bond breaks
+    it on the loops through that bond."""
+    lat = O.OLattice("honeycomb", lx, ly)
+    labels = _hk_bond_labels(lat)
+    label_of = {}
+    for (i, j), terms in labels.items():
+        label_of[(i, j)] = label_of[(j, i)] = terms[0][0]
+    rng = np.random.default_rng(5)
+    K = tuple(rng.uniform(-1.5, 1.5, 3))
+    c, s, co = O.terms_heisenberg_kitaev(lat, K=K, J=0.0).arrays()
+    H = O.full_sparse_hamiltonian(lat.ns, c, s, co, False)
+    loops = _six_cycles(lat.ns, list(labels))
+    assert len(loops) >= lx * ly
+    nb = {i: [j for (a, j) in label_of if a == i] for i in range(lat.ns)}
+    checked = 0
+    for loop in loops:
+        ops = {}
+        for v in loop:
+            out = [w for w in nb[v] if w not in loop]
+            if len(out) != 1:
+                break
+            ops[v] = label_of[(v, out[0])]
+        else:
+            W = _pauli_string(lat.ns, ops)
+            comm = (W @ H - H @ W).tocsr()
+            assert (np.abs(comm.data).max() if comm.nnz else 0.0) < 1e-12, loop
+            assert len(ops) == 6 and W.diagonal().sum() == 0   # a non-trivial Pauli string
+            checked += 1
+    assert checked >= lx * ly
+
+
+def test_hk_at_zero_kitaev_is_scaled_heisenberg():
+    """K = 0: HeisenbergKitaev(J, h) is 4J sum_<ij> S.S - 2 h.S per site
+    (CORR_TERM_MULT = 4, SINGLE_TERM_MULT = 2, heisenberg_kitaev.py:72-76),
+    i.e. J1J2Model(J1=-4J) with fields 2h on the same bonds."""
+    lat = O.OLattice("honeycomb", 2, 3)
+    J, hx, hy, hz = 0.37, 0.11, -0.07, 0.23
+    c1, s1, co1 = O.terms_heisenberg_kitaev(lat, K=0.0, J=J, hx=hx, hy=hy, hz=hz).arrays()
+    c2, s2, co2 = O.terms_j1j2(lat, J1=-4 * J, J2=None, hx=2 * hx, hy=2 * hy, hz=2 * hz).arrays()
+    H1 = O.full_sparse_hamiltonian(lat.ns, c1, s1, co1, True)
+    H2 = O.full_sparse_hamiltonian(lat.ns, c2, s2, co2, True)
+    d = (H1 - H2).tocsr()
+    assert H1.nnz > 0 and (np.abs(d.data).max() if d.nnz else 0.0) < 1e-13
