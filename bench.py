@@ -233,7 +233,9 @@ def bench_sweep(a):
     lat = SquareLattice(6, 6)
     j2s = [0.4, 0.5, 0.6]
     scales = [[1.0, j / 0.5] for j in j2s]
-    ks = [(kx, ky) for kx in range(6) for ky in range(6)][a.sector_start: a.sector_start + max(1, a.sectors)]
+    from pyqusolver_amd.hilbert import momentum_key, momentum_symmetry_types, sector_specs
+    ks = [momentum_key(lat, sec) for sec, _ in sector_specs(momentum_symmetry_types(lat), lat)]
+    ks = ks[a.sector_start: a.sector_start + max(1, a.sectors)]
     results, t_total, t_tables, steps_max = [], 0.0, 0.0, 0
     for kx, ky in ks:
         torch.cuda.synchronize()

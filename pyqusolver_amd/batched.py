@@ -20,6 +20,7 @@ import torch
 
 from . import _lib
 from .lanczos import start_vector
+from .symmetry import SymmetryGenerators
 
 
 @dataclass
@@ -387,16 +388,18 @@ class CouplingSweep(BatchedLanczos):
 
 
 def momentum_sectors(L: int, reflection: bool = True):
-    """Sector specs of a chain in the order of HilbertSpace.iter_momentum_sectors
-    (hilbert.py:546-652): every k, with reflection parities only where they
-    form a symmetry sector (k = 0 and k = L/2)."""
+    """sym_gen dicts of a chain's momentum sectors in the order of
+    HilbertSpace.iter_momentum_sectors (hilbert.py:546-652), with reflection
+    parities only where they form a symmetry sector (k = 0 and k = L/2)."""
+    from .hilbert import momentum_symmetry_types, sector_specs
+    from .lattice import ChainLattice
+    lat = ChainLattice(L)
     out = []
-    for k in range(L):
-        real = k == 0 or (L % 2 == 0 and k == L // 2)
-        if reflection and real:
-            out += [{"translation": k, "reflection": 1}, {"translation": k, "reflection": -1}]
-        else:
-            out.append({"translation": k})
+    for sector, _ in sector_specs(momentum_symmetry_types(lat, reflection), lat):
+        d = {"translation": sector[SymmetryGenerators.Translation_x]}
+        if sector.get(SymmetryGenerators.Reflection, 0):
+            d["reflection"] = sector[SymmetryGenerators.Reflection]
+        out.append(d)
     return out
 
 

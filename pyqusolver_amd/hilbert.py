@@ -11,13 +11,109 @@ The basis is built by libqes_mi on the device: no 2^Ns map is ever formed.
 from __future__ import annotations
 
 import ctypes as C
-from typing import Optional
+from itertools import product
+from typing import Dict, Iterator, List, Optional, Sequence, Tuple
 
 import numpy as np
 
 from . import _lib
-from .symmetry import (GlobalSymmetry, SectorGroup, build_sector_group, get_u1_sym,
+from .symmetry import (GlobalSymmetry, SectorGroup, SymmetryGenerators, build_sector_group, get_u1_sym,
                        normalize_symmetry_generators)
+
+_TRANSLATIONS = (SymmetryGenerators.Translation_x, SymmetryGenerators.Translation_y, SymmetryGenerators.Translation_z)
+_SPATIAL_PARITIES = (SymmetryGenerators.Reflection, SymmetryGenerators.Inversion)
+
+
+def _lattice_extents(lattice) -> Tuple[int, int, int]:
+    lx = int(getattr(lattice, "lx", None) or getattr(lattice, "Lx", 0) or 0)
+    ly = int(getattr(lattice, "ly", None) or getattr(lattice, "Ly", 1) or 1)
+    lz = int(getattr(lattice, "lz", None) or getattr(lattice, "Lz", 1) or 1)
+    return lx, ly, lz
+
+
+def _momentum_is_real(lattice, sector: Dict) -> bool:
+    """k = -k (mod L) in every translated direction: the momenta at which a
+    spatial parity (P T P^-1 = T^-1) commutes with the translations, i.e. at
+    which (k, parity) is a one-dimensional irrep and the sector block is
+    Hermitian."""
+    ext = _lattice_extents(lattice) if lattice is not None else (0, 0, 0)
+    for gen, L in zip(_TRANSLATIONS, ext):
+        if gen in sector and L:
+            k = int(sector[gen]) % L
+            if (2 * k) % L != 0:
+                return False
+    return True
+
+
+def sector_specs(symmetry_types: Sequence[Tuple[SymmetryGenerators, Sequence]], lattice=None,
+                 physical_only: bool = True) -> List[Tuple[Dict, List[Tuple[SymmetryGenerators, object]]]]:
+    """(sector_dict, sym_gen) of every sector that iter_symmetry_sectors
+    visits, in the reference's order: itertools.product over the sector lists
+    (Algebra/hilbert.py:520-543).  With ``physical_only`` a spatial parity
+    (Reflection, Inversion) is kept only at momenta with k = -k; at other
+    momenta the sector is listed once, without that generator and with
+    parity value 0 in sector_dict (the reference lists both parities there;
+    those blocks are not Hermitian and their dimensions double-count the
+    momentum sector)."""
+    if not symmetry_types:
+        raise ValueError("At least one symmetry type must be provided.")
+    gens = [g for g, _ in symmetry_types]
+    lists = [list(v) for _, v in symmetry_types]
+    out, seen = [], set()
+    for values in product(*lists):
+        sector = dict(zip(gens, values))
+        sym_gen = list(zip(gens, values))
+        if physical_only and any(g in sector for g in _SPATIAL_PARITIES) and not _momentum_is_real(lattice, sector):
+            for g in _SPATIAL_PARITIES:
+                if g in sector:
+                    sector[g] = 0
+            sym_gen = [(g, v) for g, v in sym_gen if g not in _SPATIAL_PARITIES]
+            key = tuple(sector.items())
+            if key in seen:
+                continue
+            seen.add(key)
+        out.append((sector, sym_gen))
+    return out
+
+
+def momentum_symmetry_types(lattice, include_parity: bool = False) -> List[Tuple[SymmetryGenerators, Sequence]]:
+    """Translation sector lists of a lattice (x, then y and z where the
+    lattice extends that way) plus reflection parities on request
+    (Algebra/hilbert.py:626-642)."""
+    lx, ly, lz = _lattice_extents(lattice)
+    if lx <= 0:
+        raise ValueError("Lattice must have positive lx dimension.")
+    types = [(SymmetryGenerators.Translation_x, range(lx))]
+    if ly > 1:
+        types.append((SymmetryGenerators.Translation_y, range(ly)))
+    if lz > 1:
+        types.append((SymmetryGenerators.Translation_z, range(lz)))
+    if include_parity:
+        types.append((SymmetryGenerators.Reflection, [1, -1]))
+    return types
+
+
+def momentum_key(lattice, sector: Dict, include_parity: bool = False) -> Tuple:
+    """(kx[, ky[, kz]][, parity]) as iter_momentum_sectors yields it."""
+    lx, ly, lz = _lattice_extents(lattice)
+    key = [sector[SymmetryGenerators.Translation_x]]
+    if ly > 1:
+        key.append(sector[SymmetryGenerators.Translation_y])
+    if lz > 1:
+        key.append(sector[SymmetryGenerators.Translation_z])
+    if include_parity:
+        key.append(sector[SymmetryGenerators.Reflection])
+    return tuple(key)
+
+
+_ALL_SECTOR_NAMES = {
+    "translation": "translation", "momentum": "translation", "trans": "translation",
+    "reflection": "reflection", "parityspatial": "reflection", "spatial": "reflection", "mirror": "reflection",
+    "inversion": "inversion", "inv": "inversion", "spatialinversion": "inversion",
+    "parityz": "parity_z", "spinparity": "parity_z", "pz": "parity_z", "spinflip": "parity_z",
+    "parityx": "parity_x", "px": "parity_x",
+    "parityy": "parity_y", "py": "parity_y",
+}
 
 
 class HilbertSpace:
@@ -48,6 +144,116 @@ class HilbertSpace:
         self._nh = int(n.value)
         self._reps = None
         self._norms = None
+
+    # ---- sector iteration (Algebra/hilbert.py:412-993) -------------------
+    @staticmethod
+    def iter_symmetry_sectors(symmetry_types, *, lattice=None, ns: Optional[int] = None, gen_mapping: bool = True,
+                              verbose: bool = False, physical_only: bool = True, **hilbert_kwargs) -> Iterator:
+        """Yield (sector_dict, HilbertSpace) for the Cartesian product of
+        the (generator, sector list) pairs (Algebra/hilbert.py:412-543), one
+        space at a time.  ``physical_only`` (default) keeps spatial parities
+        only at momenta k = -k, see :func:`sector_specs`; False gives the
+        reference's literal product."""
+        if lattice is None and ns is None:
+            raise ValueError("Either 'lattice' or 'ns' must be provided.")
+        specs = sector_specs(symmetry_types, lattice, physical_only)
+        for i, (sector, sym_gen) in enumerate(specs):
+            if verbose:
+                txt = ", ".join(f"{g.name}={v}" for g, v in sector.items())
+                print(f"[{i + 1}/{len(specs)}] Building sector: {txt}")
+            yield sector, HilbertSpace(lattice=lattice, ns=ns, sym_gen=sym_gen, gen_mapping=gen_mapping,
+                                       **hilbert_kwargs)
+
+    @staticmethod
+    def iter_momentum_sectors(lattice, *, include_parity: bool = False, gen_mapping: bool = True,
+                              verbose: bool = False, physical_only: bool = True, **hilbert_kwargs) -> Iterator:
+        """Yield ((kx[, ky[, kz]][, parity]), HilbertSpace) over every
+        momentum of the lattice (Algebra/hilbert.py:546-652); with
+        ``include_parity`` the reflection parities +1, -1 at momenta k = -k
+        and parity 0 (no reflection) elsewhere unless ``physical_only`` is
+        False."""
+        types = momentum_symmetry_types(lattice, include_parity)
+        for sector, hs in HilbertSpace.iter_symmetry_sectors(types, lattice=lattice, gen_mapping=gen_mapping,
+                                                             verbose=verbose, physical_only=physical_only,
+                                                             **hilbert_kwargs):
+            yield momentum_key(lattice, sector, include_parity), hs
+
+    @staticmethod
+    def iter_reflection_sectors(lattice, *, momentum_sector: Optional[int] = None, gen_mapping: bool = True,
+                                verbose: bool = False, **hilbert_kwargs) -> Iterator:
+        """Yield (parity, HilbertSpace) for reflection parity +1, -1
+        (Algebra/hilbert.py:655-712)."""
+        types = [] if momentum_sector is None else [(SymmetryGenerators.Translation_x, [momentum_sector])]
+        types.append((SymmetryGenerators.Reflection, [1, -1]))
+        for sector, hs in HilbertSpace.iter_symmetry_sectors(types, lattice=lattice, gen_mapping=gen_mapping,
+                                                             verbose=verbose, physical_only=False, **hilbert_kwargs):
+            yield sector[SymmetryGenerators.Reflection], hs
+
+    @staticmethod
+    def iter_inversion_sectors(lattice, *, momentum_sector: Optional[int] = None, gen_mapping: bool = True,
+                               verbose: bool = False, **hilbert_kwargs) -> Iterator:
+        """Yield (parity, HilbertSpace) for spatial inversion +1, -1
+        (Algebra/hilbert.py:714-783)."""
+        types = [] if momentum_sector is None else [(SymmetryGenerators.Translation_x, [momentum_sector])]
+        types.append((SymmetryGenerators.Inversion, [1, -1]))
+        for sector, hs in HilbertSpace.iter_symmetry_sectors(types, lattice=lattice, gen_mapping=gen_mapping,
+                                                             verbose=verbose, physical_only=False, **hilbert_kwargs):
+            yield sector[SymmetryGenerators.Inversion], hs
+
+    @staticmethod
+    def iter_parity_sectors(lattice, axis: str = "z", *, gen_mapping: bool = True, verbose: bool = False,
+                            **hilbert_kwargs) -> Iterator:
+        """Yield (parity, HilbertSpace) for the spin-flip parity along
+        ``axis`` (Algebra/hilbert.py:785-844)."""
+        gens = {"x": SymmetryGenerators.ParityX, "y": SymmetryGenerators.ParityY, "z": SymmetryGenerators.ParityZ}
+        if axis.lower() not in gens:
+            raise ValueError(f"Unknown parity axis '{axis}'. Use 'x', 'y', or 'z'.")
+        g = gens[axis.lower()]
+        for sector, hs in HilbertSpace.iter_symmetry_sectors([(g, [1, -1])], lattice=lattice, gen_mapping=gen_mapping,
+                                                             verbose=verbose, **hilbert_kwargs):
+            yield sector[g], hs
+
+    @staticmethod
+    def all_sector_types(lattice, symmetries: Optional[List[str]] = None):
+        """(symmetry_types, friendly names) of iter_all_sectors
+        (Algebra/hilbert.py:903-978)."""
+        symmetries = ["translation"] if symmetries is None else list(symmetries)
+        lx, ly, lz = _lattice_extents(lattice)
+        types, names = [], {}
+        for sym in symmetries:
+            kind = _ALL_SECTOR_NAMES.get(sym.lower().replace("_", "").replace("-", ""))
+            if kind == "translation":
+                for gen, L, nm in zip(_TRANSLATIONS, (lx, ly, lz), ("kx", "ky", "kz")):
+                    if gen is SymmetryGenerators.Translation_x or L > 1:
+                        types.append((gen, list(range(L))))
+                        names[gen] = nm
+            elif kind == "reflection":
+                types.append((SymmetryGenerators.Reflection, [1, -1]))
+                names[SymmetryGenerators.Reflection] = "reflection"
+            elif kind == "inversion":
+                types.append((SymmetryGenerators.Inversion, [1, -1]))
+                names[SymmetryGenerators.Inversion] = "inversion"
+            elif kind in ("parity_x", "parity_y", "parity_z"):
+                gen = {"parity_x": SymmetryGenerators.ParityX, "parity_y": SymmetryGenerators.ParityY,
+                       "parity_z": SymmetryGenerators.ParityZ}[kind]
+                types.append((gen, [1, -1]))
+                names[gen] = kind
+            else:
+                raise ValueError(f"Unknown symmetry '{sym}'. Options: translation, reflection, inversion, "
+                                 f"parity_z, parity_x, parity_y")
+        return types, names
+
+    @staticmethod
+    def iter_all_sectors(lattice, symmetries: Optional[List[str]] = None, *, gen_mapping: bool = True,
+                         verbose: bool = False, physical_only: bool = True, **hilbert_kwargs) -> Iterator:
+        """Yield ({name: value}, HilbertSpace) over every combination of the
+        named symmetries' sectors (Algebra/hilbert.py:846-993): names kx,
+        ky, kz, reflection, inversion, parity_x, parity_y, parity_z."""
+        types, names = HilbertSpace.all_sector_types(lattice, symmetries)
+        for sector, hs in HilbertSpace.iter_symmetry_sectors(types, lattice=lattice, gen_mapping=gen_mapping,
+                                                             verbose=verbose, physical_only=physical_only,
+                                                             **hilbert_kwargs):
+            yield {names[g]: v for g, v in sector.items()}, hs
 
     # ---- reference accessors -----------------------------------------
     @property
@@ -197,4 +403,4 @@ class HilbertSpace:
             pass
 
 
-__all__ = ["HilbertSpace", "get_u1_sym"]
+__all__ = ["HilbertSpace", "get_u1_sym", "sector_specs", "momentum_symmetry_types", "momentum_key"]
