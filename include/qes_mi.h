@@ -129,6 +129,18 @@ int qes_op_needs_complex(qes_op_t op, int32_t* needs);
  */
 int qes_matvec(qes_op_t op, const void* x_dev, void* y_dev, int64_t nbatch, int32_t dtype, int32_t accumulate,
                void* stream);
+/*
+ * Same product with caller workspace (device, at least n * 8 bytes for
+ * float64 or n * 16 for complex128 when nbatch == 1; ignored otherwise).
+ * A matrix-free single-vector product first writes N .* x there and
+ * gathers the neighbours from it, so no element loads a neighbour's norm
+ * (one random line fewer per neighbour that is a representative).  The
+ * workspace must not overlap x or y; calls sharing one workspace must be
+ * ordered on one stream.  Same reference interface as qes_matvec
+ * (special_operator.py:593-709, matrix_builder.py:1347-1377).
+ */
+int qes_matvec_ws(qes_op_t op, const void* x_dev, void* y_dev, int64_t nbatch, int32_t dtype, int32_t accumulate,
+                  void* ws_dev, int64_t ws_bytes, void* stream);
 
 /*
  * Sector-changing product y_out = U_out^dagger O U_in x_in (complex128 in and
@@ -220,8 +232,12 @@ int qes_op_set_canonicalizer(qes_op_t op, int32_t kind);
 /* Kernel options: "direct_lookup" (look each exchanged state up before
  * canonicalising it; hits skip the canonicalisation), "bond_order_uniform"
  * (the wavefront walks the bonds in one order in that pass): 1 on, 0 off,
- * -1 (default) on where the basis uses a rank map; "xcd_remap" (0/1).  The
- * options change the order of the sums (results agree to rounding). */
+ * -1 (default) on where the basis uses a rank map; "xcd_remap" (0/1);
+ * "top_row_test" (0 default / 1: row lattices test the top row before a
+ * direct lookup, single-lookup path only); "direct_batch" (-1 default, 1, 2,
+ * 4: rank-map lookups a lane keeps in flight in a qes_matvec_ws product;
+ * same sums in the same order for every value).  The other options change
+ * the order of the sums (results agree to rounding). */
 int qes_op_set_option(qes_op_t op, const char* name, int64_t value);
 int qes_op_get_plan(qes_op_t op, int32_t* canon, int32_t* n_classes, int32_t* n_generic);
 int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_host, int64_t n, int64_t* mismatches);

@@ -57,6 +57,7 @@ SIGNATURES = {
     "qes_op_destroy": [_P],
     "qes_op_needs_complex": [_P, _PI32],
     "qes_matvec": [_P, _P, _P, _I64, _I32, _I32, _P],
+    "qes_matvec_ws": [_P, _P, _P, _I64, _I32, _I32, _P, _I64, _P],
     "qes_matvec_couplings": [_P, _P, _P, _I32, _P],
     "qes_projected_matvec": [_P, _P, _P, _P, _P],
     "qes_op_n_couplings": [_P, _PI32],

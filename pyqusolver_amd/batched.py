@@ -291,14 +291,19 @@ class BatchedLanczos:
         return out
 
     def coefficients(self):
-        """alpha, beta (one row per Hamiltonian, in the order of hams) and steps."""
+        """alpha, beta (one row per Hamiltonian, in the order of hams) and steps
+        (cut after a breakdown: rows past it belong to a zero vector)."""
+        from .lanczos import effective_steps
         P, M = len(self.hams), self.max_steps
         a = np.zeros(P * M)
         b = np.zeros(P * M)
         k = np.zeros(P, dtype=np.int32)
         _lib.check(_lib.lib().qes_batch_get(self._h, a.ctypes.data_as(_lib._PD), b.ctypes.data_as(_lib._PD),
                                             k.ctypes.data_as(_lib._PI32)), "qes_batch_get")
-        return a.reshape(P, M)[self.vp_of], b.reshape(P, M)[self.vp_of], k[self.vp_of]
+        a, b, k = a.reshape(P, M)[self.vp_of], b.reshape(P, M)[self.vp_of], k[self.vp_of]
+        for i in range(len(k)):
+            k[i] = effective_steps(a[i, :k[i]], b[i, :k[i]])
+        return a, b, k
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

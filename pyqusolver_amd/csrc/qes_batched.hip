@@ -747,6 +747,7 @@ struct BlockSpec {
 
 struct Batch {
   std::vector<Op*> ops;           // real problems (ordinary then blocks)
+  bool acquired = false;          // ops counted in op->users
   std::vector<int> nrhs, vp0, blk;
   int nprob = 0, nvp = 0, maxs = 0, dtype = 0;
   bool w32 = true;
@@ -1091,6 +1092,8 @@ int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, 
     delete B;
     return -1;
   }
+  for (Op* op : B->ops) op_acquire(op);
+  B->acquired = true;
   *out = B;
   return 0;
 }
@@ -1104,6 +1107,8 @@ static void batch_drop_graph(Batch* B) {
 
 int batch_destroy(Batch* B) {
   if (!B) return 0;
+  if (B->acquired)
+    for (Op* op : B->ops) op_release(op);
   batch_free(B);
   delete B;
   return 0;
@@ -1256,7 +1261,7 @@ int batch_set_option(Batch* B, const std::string& name, int64_t value) {
 int batch_ground_states(Batch* B, double tol, int check_every, double* e0, int* steps_out) {
   const int V = B->nvp;
   std::vector<double> a((size_t)V * B->maxs), b((size_t)V * B->maxs);
-  std::vector<int> steps(V), active(V, 1);
+  std::vector<int> steps(V), active(V, 1), k_eff(V, 0);
   std::vector<double> cur(V, 0.0), r_prev(V, 0.0);
   std::vector<int> m_prev(V, 0);
   std::vector<int64_t> dim(V);
@@ -1279,13 +1284,14 @@ int batch_ground_states(Batch* B, double tol, int check_every, double* e0, int* 
     int next_iv = check_every;
     for (int p = 0; p < V; ++p) {
       if (!active[p]) continue;
-      const int k = steps[p];
       const double* ap = a.data() + (size_t)p * B->maxs;
       const double* bp = b.data() + (size_t)p * B->maxs;
+      // a problem that broke down since the last test ran on a zero vector:
+      // keep only the rows that span its Krylov space
+      const int k = effective_steps(ap, bp, steps[p]);
+      k_eff[p] = k;
       cur[p] = tridiag_lowest(ap, bp, k);
-      bool brk = false;
-      for (int i = 0; i < k; ++i)
-        if (!(std::fabs(bp[i]) > 1e-14 * std::max(1.0, std::fabs(cur[p])))) brk = true;
+      const bool brk = k < steps[p] || !(std::fabs(bp[k - 1]) > 1e-14 * std::max(1.0, std::fabs(cur[p])));
       const double target = tol * std::max(1.0, std::fabs(cur[p]));
       const double r = ritz_error(ap, bp, k, cur[p]);
       if (brk || k >= dim[p] || r <= target || k >= B->maxs) {
@@ -1302,10 +1308,9 @@ int batch_ground_states(Batch* B, double tol, int check_every, double* e0, int* 
     if (!any) break;
     interval = next_iv;
   }
-  if (batch_get(B, nullptr, nullptr, steps.data())) return -1;
   for (int p = 0; p < V; ++p) {
     if (e0) e0[p] = cur[p];
-    if (steps_out) steps_out[p] = steps[p];
+    if (steps_out) steps_out[p] = k_eff[p];
   }
   return 0;
 }

@@ -201,6 +201,20 @@ struct qes_lanczos_s {
 
 namespace qes {
 Op* qes_op_internal(qes_op_t h) { return &h->op; }
+
+int op_check_unused(const Op* op, const char* what) {
+  if (op->users > 0)
+    return fail(std::string(what) + ": the operator is in use by " + std::to_string(op->users) +
+                " Lanczos handle(s) or batch(es), which hold its plan and assembled table; destroy them first");
+  return 0;
+}
+
+int basis_check_unused(const Basis* b, const char* what) {
+  if (b->users > 0)
+    return fail(std::string(what) + ": the basis is in use by " + std::to_string(b->users) +
+                " Lanczos handle(s) or batch(es), which hold its lookup tables; destroy them first");
+  return 0;
+}
 }  // namespace qes
 
 extern "C" {
@@ -306,6 +320,7 @@ int qes_basis_map_states(qes_basis_t h, const uint64_t* states_host, int64_t n, 
 
 int qes_basis_destroy(qes_basis_t h) {
   if (!h) return 0;
+  if (basis_check_unused(&h->b, "qes_basis_destroy")) return -1;
   Basis& B = h->b;
   if (B.chi_re) (void)hipFree(B.chi_re);
   if (B.chi_im) (void)hipFree(B.chi_im);
@@ -388,6 +403,7 @@ int qes_op_create(qes_basis_t bh, const int32_t* codes, const int32_t* sites, co
 int qes_op_set_canonicalizer(qes_op_t h, int32_t kind) {
   if (!h) return fail("null operator");
   if (kind < -1 || kind > 2) return fail("kind must be -1 (auto), 0 (brute), 1 (cyclic) or 2 (rows)");
+  if (op_check_unused(&h->op, "qes_op_set_canonicalizer")) return -1;
   return build_plan(&h->op, kind);
 }
 
@@ -395,10 +411,15 @@ int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
   if (!h || !name) return fail("null argument");
   Op& op = h->op;
   const std::string n(name);
+  if (op_check_unused(&op, "qes_op_set_option")) return -1;
   if (n == "bond_order_uniform") op.bond_order_uniform = value < 0 ? -1 : (value ? 1 : 0);
   else if (n == "direct_lookup") op.direct_lookup = value < 0 ? -1 : (value ? 1 : 0);
   else if (n == "xcd_remap") op.xcd_remap = value ? 1 : 0;
-  else return fail("unknown option " + n + " (bond_order_uniform, direct_lookup, xcd_remap)");
+  else if (n == "top_row_test") op.top_row_test = value ? 1 : 0;
+  else if (n == "direct_batch") {
+    if (value != -1 && value != 1 && value != 2 && value != 4) return fail("direct_batch must be -1 (auto), 1, 2 or 4");
+    op.direct_batch = (int)value;
+  } else return fail("unknown option " + n + " (bond_order_uniform, direct_lookup, xcd_remap, top_row_test, direct_batch)");
   return build_plan(&op, op.plan.canon == pick_canon_public(op.basis->group) ? -1 : op.plan.canon);
 }
 
@@ -411,6 +432,8 @@ int qes_op_assemble(qes_op_t h, int32_t mode, double max_gb, void* stream) {
   if (!h) return fail("null operator");
   if (mode < 0 || mode > 3)
     return fail("mode must be 0 (free), 1 (auto format), 2 (plain values) or 3 (coupling-tagged dictionary)");
+  if (mode == 0 && h->op.asm_fmt < 0) return 0;   // nothing to free
+  if (op_check_unused(&h->op, "qes_op_assemble")) return -1;
   if (max_gb > 0) h->op.asm_max_bytes = max_gb * 1e9;
   const int want_dict = mode == 1 ? 1 : (mode == 3 ? 2 : 0);
   return op_assemble(&h->op, mode != 0, want_dict, (hipStream_t)stream);
@@ -465,6 +488,7 @@ int qes_op_get_plan(qes_op_t h, int32_t* canon, int32_t* n_classes, int32_t* n_g
 
 int qes_basis_set_lookup(qes_basis_t b, int32_t mode) {
   if (!b) return fail("null basis");
+  if (basis_check_unused(&b->b, "qes_basis_set_lookup")) return -1;
   if (mode < 0 || mode > 3)
     return fail("mode must be 0 (sorted keys + bucket table), 1 (hash table), 2 (packed keys + bucket table) "
                 "or 3 (rank map)");
@@ -496,6 +520,7 @@ int qes_basis_reduce(qes_basis_t b, const void* full_dev, void* x_red_dev, int32
 
 int qes_basis_set_bucket(qes_basis_t b, int32_t rows_log2) {
   if (!b) return fail("null basis");
+  if (basis_check_unused(&b->b, "qes_basis_set_bucket")) return -1;
   if (rows_log2 < -4 || rows_log2 > 20) return fail("rows_log2 must be in -4..20");
   return build_bucket(&b->b, rows_log2, nullptr);
 }
@@ -518,6 +543,7 @@ int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_ho
 
 int qes_op_destroy(qes_op_t h) {
   if (!h) return 0;
+  if (op_check_unused(&h->op, "qes_op_destroy")) return -1;
   free_plan(&h->op);
   if (h->op.d_groups) (void)hipFree(h->op.d_groups);
   delete h;
@@ -541,8 +567,37 @@ int qes_matvec(qes_op_t h, const void* x_dev, void* y_dev, int64_t nbatch, int32
   if (!h) return fail("null operator");
   if (dtype != 0 && dtype != 1) return fail("dtype must be 0 (float64) or 1 (complex128)");
   if (h->op.basis->n > 0 && (!x_dev || !y_dev)) return fail("null vector");
-  if (x_dev == y_dev && h->op.basis->n > 0) return fail("x and y must not alias");
+  if (h->op.basis->n > 0 && nbatch > 0) {
+    // the pull-form kernel gathers x from other rows while it writes y: the
+    // two ranges must not overlap
+    const uintptr_t bytes = (uintptr_t)h->op.basis->n * (uintptr_t)nbatch * (dtype == 1 ? 16u : 8u);
+    const uintptr_t xa = (uintptr_t)x_dev, ya = (uintptr_t)y_dev;
+    if (xa < ya + bytes && ya < xa + bytes) return fail("x and y must not overlap (in-place products are not supported)");
+  }
   return matvec(&h->op, x_dev, y_dev, nbatch, dtype, accumulate, (hipStream_t)stream);
+}
+
+int qes_matvec_ws(qes_op_t h, const void* x_dev, void* y_dev, int64_t nbatch, int32_t dtype, int32_t accumulate,
+                  void* ws_dev, int64_t ws_bytes, void* stream) {
+  if (!h) return fail("null operator");
+  const int64_t need = h->op.basis->n * (dtype == 1 ? 16 : 8);
+  if (nbatch == 1 && h->op.basis->n > 0) {
+    if (!ws_dev) return fail("null workspace");
+    if (ws_bytes < need) return fail("workspace of " + std::to_string(ws_bytes) + " bytes; the product needs " +
+                                     std::to_string(need) + " (n * 8, or n * 16 for complex128)");
+    const uintptr_t wa = (uintptr_t)ws_dev, xa = (uintptr_t)x_dev, ya = (uintptr_t)y_dev, b = (uintptr_t)need;
+    if ((wa < xa + b && xa < wa + b) || (wa < ya + b && ya < wa + b))
+      return fail("the workspace must not overlap x or y");
+  }
+  if (dtype != 0 && dtype != 1) return fail("dtype must be 0 (float64) or 1 (complex128)");
+  if (h->op.basis->n > 0 && (!x_dev || !y_dev)) return fail("null vector");
+  if (h->op.basis->n > 0 && nbatch > 0) {
+    const uintptr_t bytes = (uintptr_t)h->op.basis->n * (uintptr_t)nbatch * (dtype == 1 ? 16u : 8u);
+    const uintptr_t xa = (uintptr_t)x_dev, ya = (uintptr_t)y_dev;
+    if (xa < ya + bytes && ya < xa + bytes) return fail("x and y must not overlap (in-place products are not supported)");
+  }
+  return matvec(&h->op, x_dev, y_dev, nbatch, dtype, accumulate, (hipStream_t)stream,
+                nbatch == 1 ? (double*)ws_dev : nullptr);
 }
 
 int qes_matvec_couplings(qes_op_t h, const void* x_dev, void* y_dev, int32_t dtype, void* stream) {
@@ -640,18 +695,20 @@ int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int
   // tests placed by the residual's decay when steps are long (N >= 2^20);
   // for small N a test's host round trip outweighs the steps it saves
   const bool adaptive = op->op.basis->n >= ((int64_t)1 << 20);
-  int interval = check_every, m_prev = 0;
+  int interval = check_every, m_prev = 0, k_eff = 0;
   double r_prev = 0.0;
   while (!rc && lanczos_steps_done(L) < max_steps) {
     rc = lanczos_run(L, std::min(interval, max_steps - lanczos_steps_done(L)), reorth, 1);
     if (rc) break;
-    const int k = lanczos_steps_done(L);
+    const int kall = lanczos_steps_done(L);
     rc = lanczos_get(L, a.data(), b.data());
     if (rc) break;
+    // after a breakdown the recurrence runs on a zero vector until this test:
+    // keep the rows that span the Krylov space
+    const int k = effective_steps(a.data(), b.data(), kall);
+    k_eff = k;
     cur = tridiag_lowest(a.data(), b.data(), k);
-    bool breakdown = false;
-    for (int i = 0; i < k; ++i)
-      if (!(std::fabs(b[i]) > 1e-14 * std::max(1.0, std::fabs(cur)))) breakdown = true;
+    const bool breakdown = k < kall || !(std::fabs(b[k - 1]) > 1e-14 * std::max(1.0, std::fabs(cur)));
     const double target = tol * std::max(1.0, std::fabs(cur));
     const double r = ritz_error(a.data(), b.data(), k, cur);
     if (breakdown || r <= target) break;
@@ -659,7 +716,7 @@ int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int
     m_prev = k;
     r_prev = r;
   }
-  const int k = lanczos_steps_done(L);
+  const int k = k_eff;
   if (!rc) {
     memcpy(alpha_host, a.data(), sizeof(double) * k);
     memcpy(beta_host, b.data(), sizeof(double) * k);

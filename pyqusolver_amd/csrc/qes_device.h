@@ -358,6 +358,25 @@ __device__ __forceinline__ int64_t lookup_packed(const QesLookup& L, uint64_t v,
   return -1;
 }
 
+// colex rank of a state of the U(1) filling (the rank map's first phase:
+// two or three loads from small cached tables, independent of each other)
+__device__ __forceinline__ uint64_t colex_rank(const QesLookup& L, uint64_t v) {
+  if (L.ns <= 32) {
+    const uint32_t u = (uint32_t)v;
+    return L.rbase[u >> 16] + (uint32_t)L.rlo[u & 0xffffu];
+  }
+  const uint32_t lo32 = (uint32_t)v;
+  const int row = __popc(lo32) - L.k1min;
+  return L.rbase2[v >> 32] + (uint64_t)L.rbase1[(uint32_t)row * 65536u + (lo32 >> 16)] + (uint64_t)L.rlo[lo32 & 0xffffu];
+}
+
+// second phase: the map word of rank c gives the index (-1: no representative)
+__device__ __forceinline__ int64_t rank_resolve(ulonglong2 e, uint64_t c) {
+  const uint32_t bit = (uint32_t)(c & 63u);
+  if (!((e.x >> bit) & 1ull)) return -1;
+  return (int64_t)e.y + (int64_t)__popcll(e.x & ((1ull << bit) - 1ull));
+}
+
 // rank-map lookup of a 32-bit state; -1 if it is not a representative
 __device__ __forceinline__ int64_t lookup_rank(const QesLookup& L, uint64_t v) {
   if (L.ns < 64 && (v >> L.ns)) return -1;   // outside the ns-bit states (ns <= 48 here)
@@ -604,6 +623,40 @@ __device__ __forceinline__ Canon<W> canonicalize_rows(const GV& G, const uint32_
   return c;
 }
 
+// Necessary condition of canonicalize_rows for s to be its own
+// representative: the top row of s must be the smallest row rotation over
+// all rows (that y-translations bring to the top) and base elements.  A
+// state that fails it is not a representative; one that passes usually is.
+#define QES_TOP_ROWS 8   // rows_top_ok reads at most this many rows (others: the test passes them)
+template <typename W>
+__device__ __forceinline__ bool rows_top_ok(const GV& G, const uint32_t* __restrict__ lut, W s) {
+  const int ns = G.ns;
+  const W mask = (W)G.mask;
+  const int Wr = G.row_bits;
+  const int nrows = ns / Wr;
+  const W rowmask = (W)((Wr == 64) ? ~0ull : ((1ull << Wr) - 1ull));
+  const int q_lo = (G.nty > 1) ? 0 : nrows - 1;
+  const uint32_t top = (uint32_t)((s >> ((nrows - 1) * Wr)) & rowmask);
+  // all lookups issued before any result is tested (no early exit: one wait
+  // for the whole batch of LDS reads)
+  uint32_t m = 0xffffu;
+#pragma unroll
+  for (int B = 0; B < QES_MAX_BASE; ++B) {
+    if (B < G.nbase) {
+      int dummy = 0;
+      const W v = B == 0 ? s : apply_base<W>(G, B, s, &dummy);
+#pragma unroll
+      for (int q = 0; q < QES_TOP_ROWS; ++q)
+        if (q >= q_lo && q < nrows) {
+          const uint32_t e = lut[(uint32_t)((v >> (q * Wr)) & rowmask)] & 0xffffu;
+          m = e < m ? e : m;
+        }
+    }
+  }
+  (void)mask;
+  return m >= top;
+}
+
 // --------------------------------------------------------------------------
 // Operator plan: exchange bonds grouped into "shift classes" plus generic
 // term groups.  An exchange group flips bits p0 and p1 = (p0 - d) mod ns and
@@ -624,5 +677,7 @@ struct QesPlan {
   int32_t lut_bits;  // ROWS: row LUT has 1 << lut_bits entries
   int32_t uniform;   // 1: bonds in wavefront-uniform order, 0: each lane's active bonds
   int32_t direct;    // 1: look flipped states up directly before canonicalising
+  int32_t top_test;  // ROWS: skip the direct lookup of states that fail rows_top_ok
+  int32_t pad;
   QesExClass cls[QES_MAX_CLASSES];
 };

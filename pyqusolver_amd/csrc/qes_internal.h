@@ -78,6 +78,10 @@ struct Basis {
   // lookups resolve with no search step) and the narrowest key array.
   bool use_packed = false;    // probe the packed keys (ns <= 56, no phases)
   int bucket_rows_log2 = -3;  // about 2^k representatives per bucket
+  // live Lanczos handles and batches over operators of this basis: they hold
+  // its lookup tables (captured graphs keep the pointers), so the tables
+  // must not be rebuilt while this is nonzero
+  int users = 0;
 };
 
 // Compiled operator (Hamiltonian) bound to a basis.
@@ -105,6 +109,12 @@ struct Op {
   // 146 ms); 0 off; 1 on
   int bond_order_uniform = -1;
   int direct_lookup = -1;
+  int top_row_test = 0;            // row lattices: rows_top_ok before each direct lookup (K = 1 path)
+  // PRE products with a rank map: lookups in flight per lane (1, 2, 4; -1:
+  // 2 for 32-bit states, whose kernel is capped at 8 waves per SIMD and
+  // spills at 4, else 4; measured on MI355X: L=32 chain 2.37 / 1.94 / 1.96 ms,
+  // 6x6 square 140.9 / 122.1 / 117.1 ms for 1 / 2 / 4)
+  int direct_batch = -1;
   // assembled (sliced ELLPACK) table, see qes_assemble.hip
   int64_t* asm_off = nullptr;
   int32_t* asm_len = nullptr;
@@ -124,7 +134,23 @@ struct Op {
   std::vector<double> asm_h_dict;          // host copy of the dictionary
   std::vector<uint8_t> asm_h_dict_cpl;
   std::vector<double> asm_h_scale;         // scales of the dictionary in asm_dict_s
+  // live Lanczos handles and batches over this operator: they keep pointers
+  // to its plan and assembled table (and graphs captured with them), so
+  // neither may change or be freed while this is nonzero
+  int users = 0;
 };
+
+inline void op_acquire(Op* op) {
+  ++op->users;
+  ++op->basis->users;
+}
+inline void op_release(Op* op) {
+  --op->users;
+  --op->basis->users;
+}
+// fail (-1, message) when a live Lanczos handle or batch uses the operator
+int op_check_unused(const Op* op, const char* what);
+int basis_check_unused(const Basis* b, const char* what);
 
 // launchers (qes_basis.hip)
 int build_basis(Basis* B, hipStream_t st);
@@ -139,9 +165,11 @@ int map_states(const Basis* B, const uint64_t* d_states, int64_t n, int64_t* d_i
                int32_t* d_cnt, hipStream_t st);
 
 // launchers (qes_matvec.hip)
-// dtype 0: float64 vectors, 1: complex128 (interleaved re,im)
+// dtype 0: float64 vectors, 1: complex128 (interleaved re,im).  ws (may be
+// null): n * (1 or 2) doubles of device workspace; with it a matrix-free
+// single-vector product gathers from the pre-scaled vector N .* x
 int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int accumulate,
-           hipStream_t st);
+           hipStream_t st, double* ws = nullptr);
 int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_t st);
 // off-diagonal elements inside the sector; *outside (optional): nonzero
 // elements whose target lies outside it (skipped by the product)

@@ -16,6 +16,10 @@ int lanczos_second_passes(Lanczos* L, int* count);
 int lanczos_combine(Lanczos* L, const double* c_host, int k, void* out, hipStream_t user);
 int lanczos_steps_done(const Lanczos* L);
 double tridiag_lowest(const double* a, const double* b, int k);
+// Steps that span the Krylov space: i + 1 for the first |b[i]| <= 1e-14 *
+// max(1, |theta_k|) (an invariant subspace; the rows after it are zero rows of
+// a vector that is zero, whose eigenvalue 0 is spurious), else k.
+int effective_steps(const double* a, const double* b, int k);
 double tridiag_eig(const double* a, const double* b, int k, int idx);
 // residual norm r of the lowest Ritz pair (b[k-1] = beta_k); *quad = min(r, r^2/gap)
 double ritz_error(const double* a, const double* b, int k, double theta, double* quad = nullptr);

@@ -336,13 +336,14 @@ __global__ void __launch_bounds__(RB) combine_kernel(const double* __restrict__ 
 // ---------------------------------------------------------------------------
 struct Lanczos {
   Op* op = nullptr;
+  bool acquired = false;   // counted in op->users
   int dtype = 0;
   int maxv = 0;          // stored vectors (max_steps + 1) when store
   int max_steps = 0;
   int store = 0;
   int64_t n = 0, len = 0;  // len = n * cw
   int64_t ldv = 0;         // leading dimension of V: len rounded up to even (16-byte rows)
-  double *v = nullptr, *vp = nullptr, *w = nullptr, *V = nullptr;
+  double *v = nullptr, *vp = nullptr, *w = nullptr, *V = nullptr, *ws = nullptr;
   double *alpha = nullptr, *beta = nullptr, *scal = nullptr, *part = nullptr, *coef = nullptr, *ppart = nullptr;
   int* d_j = nullptr;
   double* dg = nullptr;    // squared norms before / after the first Gram-Schmidt pass
@@ -373,7 +374,7 @@ static int enqueue_dot(Lanczos* L, const double* a, const double* b, double* out
 
 static int enqueue_step(Lanczos* L, int reorth, hipStream_t st) {
   const bool c = L->dtype == 1;
-  if (matvec(L->op, L->v, L->w, 1, L->dtype, 0, st)) return -1;
+  if (matvec(L->op, L->v, L->w, 1, L->dtype, 0, st, L->ws)) return -1;
   enqueue_dot(L, L->v, L->w, L->scal, 0, st);
   hipLaunchKernelGGL(set_alpha_kernel, dim3(1), dim3(1), 0, st, L->scal, L->alpha, L->d_j);
   const int g = grid_for(L->len);
@@ -447,6 +448,7 @@ int lanczos_create(Op* op, int max_steps, int store, int dtype, Lanczos** out) {
   grab(&L->v, len * 8, "Lanczos vectors");
   grab(&L->vp, len * 8, "Lanczos vectors");
   grab(&L->w, len * 8, "Lanczos vectors");
+  grab(&L->ws, len * 8, "Lanczos vectors");   // gather workspace of the matrix-free product
   if (store) grab(&L->V, vbytes, "stored Krylov basis");
   grab(&L->alpha, sizeof(double) * L->maxv, "coefficients");
   grab(&L->beta, sizeof(double) * L->maxv, "coefficients");
@@ -463,17 +465,21 @@ int lanczos_create(Op* op, int max_steps, int store, int dtype, Lanczos** out) {
     if (store) msg += "; the stored basis alone takes " + std::to_string(vbytes / 1e9) + " GB";
     return check_hip(e, (msg + ")").c_str());
   }
+  op_acquire(op);
+  L->acquired = true;
   *out = L;
   return 0;
 }
 
 int lanczos_destroy(Lanczos* L) {
   if (!L) return 0;
+  if (L->acquired) op_release(L->op);
   if (L->exec) (void)hipGraphExecDestroy(L->exec);
   if (L->graph) (void)hipGraphDestroy(L->graph);
   (void)hipFree(L->v);
   (void)hipFree(L->vp);
   (void)hipFree(L->w);
+  (void)hipFree(L->ws);
   if (L->V) (void)hipFree(L->V);
   (void)hipFree(L->alpha);
   (void)hipFree(L->beta);
@@ -645,6 +651,14 @@ double tridiag_eig(const double* a, const double* b, int k, int idx) {
 }
 
 double tridiag_lowest(const double* a, const double* b, int k) { return tridiag_eig(a, b, k, 0); }
+
+int effective_steps(const double* a, const double* b, int k) {
+  if (k <= 1) return k;
+  const double scale = std::max(1.0, std::fabs(tridiag_lowest(a, b, k)));
+  for (int i = 0; i < k; ++i)
+    if (!(std::fabs(b[i]) > 1e-14 * scale)) return i + 1;
+  return k;
+}
 
 int next_check(int m0, double r0, int m1, double r1, double target, int lo, int hi) {
   if (lo < 1) lo = 1;

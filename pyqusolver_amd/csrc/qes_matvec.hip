@@ -37,15 +37,17 @@ namespace qes {
 #endif
 
 // NB = number of right-hand sides per lane (columns j0 .. j0+NB-1 of an
-// (n, ldx) row-major block).
-template <typename W, bool CPLX, int CK, int NB>
+// (n, ldx) row-major block).  PRE (one column): the neighbours are gathered
+// from xs = N .* x (prescale_kernel), so no element needs its N_b -- a
+// direct rank-map hit loads one line of xs instead of a norm and an x line.
+template <typename W, bool CPLX, int CK, int NB, bool PRE, int K>
 __device__ __forceinline__ void matvec_rows(const QesGroup& G, const QesLookup& L, const QesPlan& P,
                                             const double* __restrict__ chi_re, const double* __restrict__ chi_im,
                                             const QesTermGroup* __restrict__ groups, int n_diag,
                                             const double* __restrict__ exval, const uint32_t* __restrict__ lut,
                                             const double* __restrict__ norms, const double* __restrict__ x,
-                                            double* __restrict__ y, int64_t n, int64_t ldx, int64_t j0, int ncols,
-                                            int accumulate, int remap) {
+                                            const double* __restrict__ xs, double* __restrict__ y, int64_t n,
+                                            int64_t ldx, int64_t j0, int ncols, int accumulate, int remap) {
   extern __shared__ char smem[];
   const RowCtx C = load_ctx<CPLX>(G, L, P, chi_re, chi_im, groups, n_diag, exval, lut, smem);
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, remap);
@@ -84,23 +86,42 @@ __device__ __forceinline__ void matvec_rows(const QesGroup& G, const QesLookup& 
       }
     }
   }
-  visit_row<W, CPLX, CK>(C, r, [&](int64_t b, double fr, double fi, int) {
-    fr *= inv_na;
-    fi *= inv_na;
+  if (PRE) {
+    // off-diagonal sum over N_b x_b, scaled by 1 / N_a once
+    double sr = 0.0, si = 0.0;
+    if (K > 1 && L.rmap) {
+      pre_row_sum<W, CPLX, CK, K>(C, r, xs, sr, si);
+    } else visit_row_t<W, CPLX, CK, false>(C, r, [&](int64_t b, double fr, double fi, int) {
+      const double* xp = xs + b * cw;
+      if (CPLX) {
+        const double xr = xp[0], xi = xp[1];
+        sr += fr * xr - fi * xi;
+        si += fr * xi + fi * xr;
+      } else {
+        sr += fr * xp[0];
+      }
+    }, []() {});
+    acc_r[0] += sr * inv_na;
+    if (CPLX) acc_i[0] += si * inv_na;
+  } else {
+    visit_row<W, CPLX, CK>(C, r, [&](int64_t b, double fr, double fi, int) {
+      fr *= inv_na;
+      fi *= inv_na;
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      if (j < ncols) {
-        const double* xp = x + (b * ldx + j0 + j) * cw;
-        if (CPLX) {
-          double xr = xp[0], xi = xp[1];
-          acc_r[j] += fr * xr - fi * xi;
-          acc_i[j] += fr * xi + fi * xr;
-        } else {
-          acc_r[j] += fr * xp[0];
+      for (int j = 0; j < NB; ++j) {
+        if (j < ncols) {
+          const double* xp = x + (b * ldx + j0 + j) * cw;
+          if (CPLX) {
+            double xr = xp[0], xi = xp[1];
+            acc_r[j] += fr * xr - fi * xi;
+            acc_i[j] += fr * xi + fi * xr;
+          } else {
+            acc_r[j] += fr * xp[0];
+          }
         }
       }
-    }
-  });
+    });
+  }
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     if (j < ncols) {
@@ -125,20 +146,36 @@ __device__ __forceinline__ void matvec_rows(const QesGroup& G, const QesLookup& 
   QesGroup G, QesLookup L, QesPlan P, const double* __restrict__ chi_re, const double* __restrict__ chi_im,       \
       const QesTermGroup* __restrict__ groups, int n_diag, const double* __restrict__ exval,                      \
       const uint32_t* __restrict__ lut, const double* __restrict__ norms, const double* __restrict__ x,         \
-      double* __restrict__ y, int64_t n, int64_t ldx, int64_t j0, int ncols, int accumulate, int remap
-#define QES_MV_ARGS G, L, P, chi_re, chi_im, groups, n_diag, exval, lut, norms, x, y, n, ldx, j0, ncols, accumulate, remap
+      const double* __restrict__ xs, double* __restrict__ y, int64_t n, int64_t ldx, int64_t j0, int ncols,      \
+      int accumulate, int remap
+#define QES_MV_ARGS G, L, P, chi_re, chi_im, groups, n_diag, exval, lut, norms, x, xs, y, n, ldx, j0, ncols, accumulate, remap
 
 // Up to 32 sites: registers capped so that 8 waves per SIMD stay resident
 // (the lookups and gathers are latency bound; 7 -> 8 waves: 2.88 -> 2.68 ms
 // at L=32, no spills).  Wider states keep the compiler's allocation: under
 // the cap their cyclic canonicaliser spills to scratch.
-template <bool CPLX, int CK, int NB>
+template <bool CPLX, int CK, int NB, bool PRE, int K>
 __global__ void __launch_bounds__(256) QES_MV_ATTR matvec_kernel(QES_MV_PARAMS) {
-  matvec_rows<uint32_t, CPLX, CK, NB>(QES_MV_ARGS);
+  matvec_rows<uint32_t, CPLX, CK, NB, PRE, K>(QES_MV_ARGS);
 }
-template <bool CPLX, int CK, int NB>
+template <bool CPLX, int CK, int NB, bool PRE, int K>
 __global__ void __launch_bounds__(256) matvec_kernel_wide(QES_MV_PARAMS) {
-  matvec_rows<uint64_t, CPLX, CK, NB>(QES_MV_ARGS);
+  matvec_rows<uint64_t, CPLX, CK, NB, PRE, K>(QES_MV_ARGS);
+}
+
+// xs = N .* x (cw doubles per row): the gather vector of the PRE products
+__global__ void __launch_bounds__(256) prescale_kernel(const double* __restrict__ x, const double* __restrict__ norms,
+                                                       double* __restrict__ xs, int64_t n, int cw) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double nb = norms[i];
+    if (cw == 2) {
+      const double2 v = reinterpret_cast<const double2*>(x)[i];
+      reinterpret_cast<double2*>(xs)[i] = make_double2(v.x * nb, v.y * nb);
+    } else {
+      xs[i] = x[i] * nb;
+    }
+  }
 }
 #undef QES_MV_PARAMS
 #undef QES_MV_ARGS
@@ -327,6 +364,7 @@ int build_plan(Op* op, int canon_override) {
   P.canon = canon_override >= 0 ? canon_override : pick_canon(G);
   P.uniform = op->bond_order_uniform;
   P.direct = op->direct_lookup;
+  P.top_test = op->top_row_test;
   if (P.canon == QES_CANON_CYCLIC &&
       !(G.nty == 1 && G.row_bits == ns && G.cell_bits == 1 && G.ntx == ns && G.ntx > 1))
     return fail("cyclic canonicaliser needs a chain translation group");
@@ -418,7 +456,8 @@ int build_plan(Op* op, int canon_override) {
   }
 
 template <typename W, bool CPLX>
-static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatch, int accumulate, hipStream_t st) {
+static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatch, int accumulate, double* ws,
+                         hipStream_t st) {
   const Basis* B = op->basis;
   const int64_t n = B->n;
   if (n == 0) return 0;
@@ -426,19 +465,33 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   const int64_t nblocks = (n + block - 1) / block;
   if (nblocks > 0x7fffffffll) return fail("matvec grid too large");
   const QesPlan& P = op->plan;
+  const bool pre = ws != nullptr && nbatch == 1;
+  // lookups in flight per lane in the direct and canonicalising passes
+  const int kb = !B->rmap ? 1 : (op->direct_batch > 0 ? op->direct_batch : (sizeof(W) == 4 ? 2 : 4));
+  if (pre) {
+    const int64_t g = std::min<int64_t>(nblocks, 256 * 64);
+    hipLaunchKernelGGL(prescale_kernel, dim3((unsigned)g), dim3(block), 0, st, x, B->norms, ws, n, CPLX ? 2 : 1);
+    QES_HIP(hipGetLastError());
+  }
   for (int64_t j0 = 0; j0 < nbatch; j0 += 4) {
     const int ncols = (int)std::min<int64_t>(4, nbatch - j0);
 #define QES_LAUNCH_K(KER)                                                                                        \
   hipLaunchKernelGGL(KER, dim3((unsigned)nblocks), dim3(block), op->lds_bytes, st, B->group, B->lookup(), P,       \
-                     B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_lut, B->norms, x, y, n, nbatch, \
-                     j0, ncols, accumulate, op->xcd_remap)
+                     B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_lut, B->norms, x, ws, y, n,   \
+                     nbatch, j0, ncols, accumulate, op->xcd_remap)
 #define QES_LAUNCH(CK)                                                                                           \
   if (sizeof(W) == 4) {                                                                                          \
-    if (ncols == 1) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1>));                                                  \
-    else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 4>));                                                             \
+    if (pre && kb == 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4>));                                     \
+    else if (pre && kb == 2) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2>));                                \
+    else if (pre) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 1>));                                           \
+    else if (ncols == 1) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, false, 1>));                                   \
+    else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 4, false, 1>));                                                   \
   } else {                                                                                                       \
-    if (ncols == 1) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1>));                                             \
-    else QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 4>));                                                        \
+    if (pre && kb == 4) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4>));                                \
+    else if (pre && kb == 2) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 2>));                           \
+    else if (pre) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 1>));                                      \
+    else if (ncols == 1) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, false, 1>));                              \
+    else QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 4, false, 1>));                                              \
   }
     QES_CK_SWITCH(P.canon, QES_LAUNCH)
 #undef QES_LAUNCH
@@ -448,7 +501,8 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   return 0;
 }
 
-int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int accumulate, hipStream_t st) {
+int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int accumulate, hipStream_t st,
+           double* ws) {
   const Basis* B = op->basis;
   const bool cplx = dtype == 1;
   if (!cplx && (op->is_complex || op->chi_complex))
@@ -460,10 +514,10 @@ int matvec(const Op* op, const void* x, void* y, int64_t nbatch, int dtype, int 
   const bool w32 = B->group.ns <= 32;
   const double* xd = (const double*)x;
   double* yd = (double*)y;
-  if (w32) return cplx ? launch_matvec<uint32_t, true>(op, xd, yd, nbatch, accumulate, st)
-                       : launch_matvec<uint32_t, false>(op, xd, yd, nbatch, accumulate, st);
-  return cplx ? launch_matvec<uint64_t, true>(op, xd, yd, nbatch, accumulate, st)
-              : launch_matvec<uint64_t, false>(op, xd, yd, nbatch, accumulate, st);
+  if (w32) return cplx ? launch_matvec<uint32_t, true>(op, xd, yd, nbatch, accumulate, ws, st)
+                       : launch_matvec<uint32_t, false>(op, xd, yd, nbatch, accumulate, ws, st);
+  return cplx ? launch_matvec<uint64_t, true>(op, xd, yd, nbatch, accumulate, ws, st)
+              : launch_matvec<uint64_t, false>(op, xd, yd, nbatch, accumulate, ws, st);
 }
 
 int matvec_couplings(const Op* op, const void* x, void* y, int dtype, hipStream_t st) {

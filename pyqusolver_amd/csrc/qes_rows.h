@@ -35,6 +35,7 @@ struct RowCtx {
   const uint32_t* s_lut;        // LDS row LUT
   int uniform;                  // bond order of the direct-lookup pass
   int direct;                   // run the direct-lookup pass
+  int top_test;                 // row lattices: top-row test before a direct lookup
 };
 
 // bytes of dynamic LDS used by load_ctx before the per-op tables
@@ -54,12 +55,14 @@ __device__ __forceinline__ Canon<W> canon_dispatch(const RowCtx& C, W s) {
 }
 
 // Visit every valid off-diagonal element of row r: f(b, fr, fi, coupling)
-// with (fr, fi) = conj(val) * chi(g*) * N_b (the caller divides by N_a).
-// miss() is called for every nonzero element whose target lies outside the
-// sector (another U(1)/Z2 filling, or an orbit of zero norm): the targets
-// the reference's kernel skips (Algebra/Hilbert/matrix_builder.py:1366-1371).
-template <typename W, bool CPLX, int CK, class F, class M>
-__device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& miss) {
+// with (fr, fi) = conj(val) * chi(g*) * N_b (the caller divides by N_a);
+// NORM = false leaves out N_b (for products that gather N_b x_b from a
+// pre-scaled vector).  miss() is called for every nonzero element whose
+// target lies outside the sector (another U(1)/Z2 filling, or an orbit of
+// zero norm): the targets the reference's kernel skips
+// (Algebra/Hilbert/matrix_builder.py:1366-1371).
+template <typename W, bool CPLX, int CK, bool NORM, class F, class M>
+__device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& miss) {
   const int ns = C.G.ns;
   const W mask = (W)C.G.mask;
   auto emit = [&](W sp, double vr, double vi, int cpl) {
@@ -71,7 +74,7 @@ __device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& mis
       miss();
       return;
     }
-    const double nb = C.L.sqrtn[cn.count];
+    const double nb = NORM ? C.L.sqrtn[cn.count] : 1.0;
     const double cr = C.s_chr[cn.g];
     double fr, fi;
     if (CPLX) {
@@ -109,6 +112,12 @@ __device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& mis
         int p1 = p - d;
         if (p1 < 0) p1 += ns;
         const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
+        // row lattices: a state whose top row is not the smallest row
+        // rotation is no representative -- skip its lookup
+        if (CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
+          fail |= (W)1 << p;
+          continue;
+        }
         int cnt = 0;
         const int64_t b = C.L.rmap ? lookup_rank(C.L, (uint64_t)sp) : lookup_packed(C.L, (uint64_t)sp, &cnt);
         if (b < 0) {
@@ -117,7 +126,7 @@ __device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& mis
         }
         const int b0 = (int)((r >> p) & 1);
         const double* e = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
-        const double nb = C.L.rmap ? C.L.norms[b] : C.L.sqrtn[cnt];
+        const double nb = NORM ? (C.L.rmap ? C.L.norms[b] : C.L.sqrtn[cnt]) : 1.0;
         f(b, e[0] * nb, CPLX ? -e[1] * nb : 0.0, c * ns + p);
       }
     }
@@ -148,9 +157,208 @@ __device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& mis
   }
 }
 
+// Off-diagonal sum of row r over a pre-scaled vector xs = N .* x (PRE
+// products): sum_b conj(val) chi(g*) xs[b], the same elements in the same
+// order as visit_row_t<NORM = false>.  With a rank map the direct pass takes
+// the wavefront-uniform bonds K at a time: K colex ranks, then K map words,
+// then K gathers are in flight together, where visit_row_t waits for each
+// lookup before the next (the product is bound by the latency of these
+// three dependent loads, not by bandwidth).
+template <typename W, bool CPLX, int CK, int K>
+__device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* __restrict__ xs, double& sr,
+                                            double& si) {
+  const int ns = C.G.ns;
+  const W mask = (W)C.G.mask;
+  const int cw = CPLX ? 2 : 1;
+  const bool direct = C.L.rmap != nullptr && C.direct != 0;
+  const bool uniform = C.uniform != 0;
+  auto add = [&](int64_t b, double fr, double fi) {
+    const double* xp = xs + b * cw;
+    if (CPLX) {
+      const double xr = xp[0], xi = xp[1];
+      sr += fr * xr - fi * xi;
+      si += fr * xi + fi * xr;
+    } else {
+      sr += fr * xp[0];
+    }
+  };
+  auto emit = [&](W sp, double vr, double vi) {
+    const Canon<W> cn = canon_dispatch<W, CK>(C, sp);
+    const int64_t b = lookup_rep<W>(C.L, cn.rep);
+    if (b < 0) return;
+    const double cr = C.s_chr[cn.g];
+    if (CPLX) {
+      const double ci = C.s_chi[cn.g];
+      add(b, vr * cr + vi * ci, vr * ci - vi * cr);
+    } else {
+      add(b, vr * cr, 0.0);
+    }
+  };
+  for (int c = 0; c < C.nclass; ++c) {
+    const int d = C.s_cls_d[c];
+    const W pm = (W)C.s_cls_pm[c];
+    const W act = (r ^ rotl_ns<W>(r, d, ns, mask)) & pm;
+    W fail = direct ? (W)0 : act;
+    if (direct) {
+      W m = uniform ? pm : act;
+      while (m) {
+        int pos[K];
+        bool on[K];
+        uint64_t rk[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          pos[k] = 0;
+          on[k] = false;
+          if (m) {
+            pos[k] = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
+            m &= m - 1;
+            on[k] = (act >> pos[k]) & 1;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          rk[k] = 0;
+          if (on[k]) {
+            int p1 = pos[k] - d;
+            if (p1 < 0) p1 += ns;
+            // an exchange keeps the filling, so the state has a colex rank
+            rk[k] = colex_rank(C.L, (uint64_t)(r ^ (((W)1 << pos[k]) | ((W)1 << p1))));
+          }
+        }
+        ulonglong2 e[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
+        int64_t bk[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
+          if (on[k] && bk[k] < 0) fail |= (W)1 << pos[k];
+        }
+        double xr[K], xi[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          xr[k] = 0.0;
+          xi[k] = 0.0;
+          if (bk[k] >= 0) {
+            const double* xp = xs + bk[k] * cw;
+            xr[k] = xp[0];
+            if (CPLX) xi[k] = xp[1];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (bk[k] >= 0) {
+            const int b0 = (int)((r >> pos[k]) & 1);
+            const double* ev = C.s_ex + ((c * ns + pos[k]) * 2 + b0) * 2;
+            const double fr = ev[0];
+            if (CPLX) {
+              const double fi = -ev[1];
+              sr += fr * xr[k] - fi * xi[k];
+              si += fr * xi[k] + fi * xr[k];
+            } else {
+              sr += fr * xr[k];
+            }
+          }
+        }
+      }
+    }
+    // states that are no representative: canonicalise K at a time, then
+    // their ranks, map words and gathers in flight together (rank map only)
+    while (C.L.rmap && fail) {
+      int pos[K];
+      bool on[K];
+      Canon<W> cn[K];
+      uint64_t rk[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        pos[k] = 0;
+        on[k] = false;
+        if (fail) {
+          pos[k] = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
+          fail &= fail - 1;
+          on[k] = true;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        rk[k] = 0;
+        if (on[k]) {
+          int p1 = pos[k] - d;
+          if (p1 < 0) p1 += ns;
+          cn[k] = canon_dispatch<W, CK>(C, r ^ (((W)1 << pos[k]) | ((W)1 << p1)));
+          // outside the filling (not for a group of spin flips at half filling): no rank
+          on[k] = popc<W>(cn[k].rep) == C.L.n_up;
+          if (on[k]) rk[k] = colex_rank(C.L, (uint64_t)cn[k].rep);
+        }
+      }
+      ulonglong2 e[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
+      int64_t bk[K];
+      double xr[K], xi[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
+        xr[k] = 0.0;
+        xi[k] = 0.0;
+        if (bk[k] >= 0) {
+          const double* xp = xs + bk[k] * cw;
+          xr[k] = xp[0];
+          if (CPLX) xi[k] = xp[1];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (bk[k] >= 0) {
+          const int b0 = (int)((r >> pos[k]) & 1);
+          const double* ev = C.s_ex + ((c * ns + pos[k]) * 2 + b0) * 2;
+          const double vr = ev[0], vi = CPLX ? ev[1] : 0.0;
+          const double cr = C.s_chr[cn[k].g];
+          if (CPLX) {
+            const double ci = C.s_chi[cn[k].g];
+            const double fr = vr * cr + vi * ci, fi = vr * ci - vi * cr;
+            sr += fr * xr[k] - fi * xi[k];
+            si += fr * xi[k] + fi * xr[k];
+          } else {
+            sr += (vr * cr) * xr[k];
+          }
+        }
+      }
+    }
+    while (fail) {
+      const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
+      fail &= fail - 1;
+      int p1 = p - d;
+      if (p1 < 0) p1 += ns;
+      const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
+      const int b0 = (int)((r >> p) & 1);
+      const double* ev = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
+      emit(sp, ev[0], CPLX ? ev[1] : 0.0);
+    }
+  }
+  for (int t = C.n_diag; t < C.n_diag + C.n_gen; ++t) {
+    const QesTermGroup& g = C.groups[t];
+    const int k = term_bits<W>(g, r);
+    const double vr = sel4(g.re, k);
+    const double vi = CPLX ? sel4(g.im, k) : 0.0;
+    if (vr == 0.0 && vi == 0.0) continue;
+    const W sp = r ^ (W)g.mask;
+    if ((C.G.n_up >= 0 && popc<W>(sp) != C.G.n_up) || (C.G.z2 != 0 && ((popc<W>(sp) & 1) ? -1 : 1) != C.G.z2))
+      continue;
+    emit(sp, vr, vi);
+  }
+}
+
+template <typename W, bool CPLX, int CK, class F, class M>
+__device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& miss) {
+  visit_row_t<W, CPLX, CK, true>(C, r, f, miss);
+}
+
 template <typename W, bool CPLX, int CK, class F>
 __device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
-  visit_row_m<W, CPLX, CK>(C, r, f, []() {});
+  visit_row_t<W, CPLX, CK, true>(C, r, f, []() {});
 }
 
 // Load the per-block tables into LDS; returns the context.
@@ -192,6 +400,7 @@ __device__ __forceinline__ RowCtx load_ctx(const QesGroup& G, const QesLookup& L
   C.n_gen = P.n_gen;
   C.uniform = P.uniform;
   C.direct = P.direct;
+  C.top_test = P.top_test;
   C.s_cls_d = s_cls_d;
   C.s_cls_pm = s_cls_pm;
   C.groups = groups;

@@ -115,6 +115,18 @@ def tridiag_eigs(alpha, beta, k=1):
     return w, v
 
 
+def effective_steps(alpha, beta) -> int:
+    """Rows of (alpha, beta) that span the Krylov space: i + 1 for the first
+    |beta_i| <= 1e-14 max(1, |theta|) (an invariant subspace: later rows
+    belong to a zero vector and add a spurious eigenvalue 0), else all."""
+    m = len(alpha)
+    if m <= 1:
+        return m
+    scale = max(1.0, abs(float(tridiag_eigs(alpha, beta)[0][0])))
+    small = np.nonzero(~(np.abs(np.asarray(beta[:m])) > 1e-14 * scale))[0]
+    return int(small[0]) + 1 if small.size else m
+
+
 def ritz_residual(alpha, beta):
     """(r, quad) for the lowest Ritz value after m = len(alpha) steps
     (beta[m-1] = beta_m, the coefficient of the next basis vector):
@@ -241,7 +253,7 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
         hist.append((len(a), e0))
         scale = max(1.0, abs(e0))
         if np.any(np.abs(b) <= 1e-14 * scale):          # invariant subspace found
-            cut = int(np.argmax(np.abs(b) <= 1e-14 * scale)) + 1
+            cut = effective_steps(a, b)
             a, b = a[:cut], b[:cut]
             w, _ = tridiag_eigs(a, b)
             converged = True

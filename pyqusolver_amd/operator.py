@@ -206,7 +206,7 @@ class Hamiltonian:
                    "qes_op_set_canonicalizer")
 
     def set_option(self, name: str, value: int):
-        """Kernel tuning knob (bond_order_uniform, direct_lookup, xcd_remap)."""
+        """Kernel tuning knob (bond_order_uniform, direct_lookup, xcd_remap, top_row_test, direct_batch)."""
         _lib.check(_lib.lib().qes_op_set_option(self._device_op(), name.encode(), int(value)), "qes_op_set_option")
 
     @property
@@ -307,7 +307,20 @@ class Hamiltonian:
         y = out if out is not None else torch.empty_like(x)
         if y.shape != x.shape or y.dtype != x.dtype or not y.is_contiguous():
             raise ValueError("out must match x in shape/dtype and be contiguous")
+        nbytes = x.numel() * x.element_size()
+        if x.numel() and (x.data_ptr() < y.data_ptr() + nbytes and y.data_ptr() < x.data_ptr() + nbytes):
+            raise ValueError("out must not overlap x: the product reads x at other rows while it writes out")
         dtype = 1 if x.dtype == torch.complex128 else 0
+        if nb == 1 and self.assembly_bytes == 0:
+            # matrix-free: gather from N .* x in a workspace (one random line
+            # fewer per neighbour that is a representative)
+            ws = torch.empty_like(x)
+            if stream is not None:
+                ws.record_stream(stream)      # freed only after the product on that stream
+            _lib.check(_lib.lib().qes_matvec_ws(self._device_op(), _lib.dptr(x), _lib.dptr(y), 1, dtype,
+                                                1 if accumulate else 0, _lib.dptr(ws), ws.numel() * ws.element_size(),
+                                                _lib.stream_ptr(stream)), "qes_matvec_ws")
+            return y
         _lib.check(_lib.lib().qes_matvec(self._device_op(), _lib.dptr(x), _lib.dptr(y), nb, dtype,
                                          1 if accumulate else 0, _lib.stream_ptr(stream)), "qes_matvec")
         return y
@@ -353,9 +366,35 @@ class Hamiltonian:
         xa = np.asarray(x)
         if xa.dtype not in (np.float64, np.complex128):
             xa = xa.astype(np.complex128 if np.iscomplexobj(xa) else np.float64)
-        xt = torch.from_numpy(np.ascontiguousarray(xa)).to("cuda")
-        y = self.matvec_device(xt)
-        return y.cpu().numpy()
+        return self._matvec_host(np.ascontiguousarray(xa))
+
+    def _matvec_host(self, xa: np.ndarray) -> np.ndarray:
+        """H x for a host array: x is copied (multi-threaded) into a cached
+        page-locked staging buffer and sent to a cached device vector; the
+        result comes back over PCIe straight into a freshly allocated
+        page-locked array, returned as the caller's numpy array (no second
+        host copy).  Pageable numpy memory cannot be registered here, so one
+        host copy of x remains."""
+        import threading
+        if xa.shape[0] != self.nh or xa.ndim > 2:
+            raise ValueError(f"vector shape {xa.shape} does not match nh={self.nh}")
+        lock = self.__dict__.setdefault("_host_lock", threading.Lock())
+        with lock:
+            dt = torch.from_numpy(xa[:0]).dtype
+            key = (tuple(xa.shape), dt)
+            st = self.__dict__.get("_host_stage")
+            if st is None or st[0] != key:
+                st = (key, torch.empty(xa.shape, dtype=dt, pin_memory=True),
+                      torch.empty(xa.shape, dtype=dt, device="cuda"), torch.empty(xa.shape, dtype=dt, device="cuda"))
+                self._host_stage = st
+            _, pin_in, xd, yd = st
+            pin_in.copy_(torch.from_numpy(xa))
+            xd.copy_(pin_in, non_blocking=True)
+            self.matvec_device(xd, out=yd)
+            out = torch.empty(xa.shape, dtype=dt, pin_memory=True)
+            out.copy_(yd, non_blocking=True)
+            torch.cuda.current_stream().synchronize()
+            return out.numpy()
 
     @property
     def matvec_fun(self):

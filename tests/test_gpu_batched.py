@@ -340,3 +340,51 @@ def test_batched_ground_states_split_batches_with_error_bounds():
     assert res.residuals is not None and res.residuals.shape == (len(hams),)
     assert np.all(res.residuals <= 1e-11 * np.maximum(1.0, np.abs(res.e0)) * (1 + 1e-9))
     assert t_create >= 0.0 and res.seconds > 0.0
+
+
+def test_one_state_sector_of_positive_energy_after_breakdown():
+    """The saturated sector of an antiferromagnetic XXZ ring (n_up = L, one
+    state, E = L/4 > 0) breaks down after one step; the batched and the plain
+    C-ABI ground-state solves must not report the spurious 0 of the zero rows
+    that follow (reference: E0 = -jz L/4 with H = -jz sum SzSz)."""
+    import ctypes as C
+    import torch
+    from pyqusolver_amd import ChainLattice, _lib, get_u1_sym
+    from pyqusolver_amd.batched import BatchedLanczos
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import XXZ
+    L = 8
+    lat = ChainLattice(L)
+    hs1 = HilbertSpace(lattice=lat, sym_gen={"translation": 0}, global_syms=[get_u1_sym(lat, L)])
+    assert hs1.nh == 1
+    h1 = XXZ(lat, hilbert_space=hs1, jxy=1.0, jz=-1.0)
+    hs4 = HilbertSpace(lattice=lat, sym_gen={"translation": 0}, global_syms=[get_u1_sym(lat, L // 2)])
+    h4 = XXZ(lat, hilbert_space=hs4, jxy=1.0, jz=-1.0)
+    B = BatchedLanczos([h1, h4], max_steps=60)
+    res = B.ground_states(tol=1e-12, check_every=10)
+    assert abs(res.e0[0] - L / 4) < 1e-13
+    assert res.steps[0] == 1
+    a, b, k = B.coefficients()
+    assert k[0] == 1
+    e_ref = np.linalg.eigvalsh(_dense(h4))[0]
+    assert abs(res.e0[1] - e_ref) <= 1e-10 * abs(e_ref)
+    B.close()
+    # plain device solve through the C ABI
+    v0 = torch.ones(1, dtype=torch.complex128, device="cuda")
+    al = np.zeros(40)
+    be = np.zeros(40)
+    steps = C.c_int32()
+    e0 = C.c_double()
+    _lib.check(_lib.lib().qes_lanczos_ground_state(h1._device_op(), _lib.dptr(v0), 1, 40, 0, 1e-12, 10,
+                                                   al.ctypes.data_as(_lib._PD), be.ctypes.data_as(_lib._PD),
+                                                   C.byref(steps), C.byref(e0), _lib.stream_ptr()),
+               "qes_lanczos_ground_state")
+    assert steps.value == 1 and abs(e0.value - L / 4) < 1e-13
+    # and the Python solver
+    assert abs(h1.diagonalize(method="lanczos", k=1, tol=1e-12, maxiter=40)[0] - L / 4) < 1e-13
+
+
+def _dense(h):
+    n = h.nh
+    cols = [h.matvec_fun(np.eye(n, dtype=np.complex128)[:, j]) for j in range(n)]
+    return np.array(cols).T

@@ -197,9 +197,14 @@ def test_rank_map_lookup_matches_sorted_keys(case):
     assert (torch.linalg.vector_norm(ys - y0) / torch.linalg.vector_norm(y0)).item() <= 1e-14
     H.set_option("direct_lookup", 0)   # the same order as the sorted keys
     hs.set_lookup("rank")
-    assert torch.equal(H.matvec_device(x), ys)
+
+    def same(a, b):   # complex multiply-adds may contract differently in the batched-lookup path
+        if a.is_complex():
+            return (torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item() <= 1e-15
+        return torch.equal(a, b)
+    assert same(H.matvec_device(x), ys)
     H.set_option("direct_lookup", -1)
-    assert torch.equal(H.matvec_device(x), y0)
+    assert same(H.matvec_device(x), y0)
 
 
 @pytest.mark.parametrize("case", [("chain", 36, 1, [("tx", 0), ("R", 1)], 3, "xxz"),
@@ -247,7 +252,9 @@ def test_multi_rhs_matches_single(nb):
     Y = H.matvec_fun(X)
     for j in range(nb):
         yj = H.matvec_fun(np.ascontiguousarray(X[:, j]))
-        np.testing.assert_array_equal(Y[:, j], yj)
+        # one column gathers N_b x_b from a pre-scaled copy; several columns
+        # scale each element by N_b: equal to rounding
+        assert np.linalg.norm(Y[:, j] - yj) <= 1e-14 * np.linalg.norm(yj)
 
 
 def test_matvec_is_deterministic_and_linear():
@@ -406,7 +413,7 @@ def test_square6x6_scale_properties():
     """6x6 J1-J2, Sz=0, k=(0,0), flip+ (126,056,625 states, |G| = 72, 64-bit
     states): closed-form dimension, sorted reps, rows sampled across the
     basis against the oracle's row formula, hermiticity, and the assembled
-    table's product equal to the matrix-free one bit for bit."""
+    table's product equal to the matrix-free one to rounding."""
     import torch
     from pyqusolver_amd import SquareLattice, get_u1_sym
     from pyqusolver_amd.hilbert import HilbertSpace
@@ -438,7 +445,9 @@ def test_square6x6_scale_properties():
     _, _, cnt = O.canonicalize(grp, reps[sample])
     np.testing.assert_allclose(hs.normalization[sample] ** 2, cnt, rtol=1e-14)
     H.assemble(True, max_gb=64)
-    assert torch.equal(H.matvec_device(x), y)
+    ya = H.matvec_device(x)
+    assert (torch.linalg.vector_norm(ya - y) / torch.linalg.vector_norm(y)).item() <= 1e-14
+    del ya
     # Ritz vector without a stored basis at full size (the accumulation must
     # wait for the zero fill of its output, queued on another stream): equal
     # to the stored-basis Ritz vector, residual equal to |beta_m s_m|
@@ -465,7 +474,8 @@ def test_square6x6_scale_properties():
     with pytest.raises(_lib.QesError, match="stored Krylov basis"):
         DeviceLanczos(H, 1023, True, False)
     assert torch.cuda.mem_get_info()[0] >= free0 - (64 << 20)
-    assert torch.equal(H.matvec_device(x), y)
+    # y came from the table; the matrix-free product equals it to rounding
+    assert (torch.linalg.vector_norm(H.matvec_device(x) - y) / torch.linalg.vector_norm(y)).item() <= 1e-14
 
 
 @pytest.mark.parametrize("which", ["honeycomb3x4_hk", "chain24_xxz_k5"])
@@ -879,3 +889,52 @@ def test_two_pass_ritz_vector_without_stored_basis():
         psi = v2 if k == 1 else v2[:, 0].contiguous()
         res = H.matvec_device(psi) - float(r2.eigenvalues[0]) * psi
         assert torch.linalg.vector_norm(res).item() <= 1e-9 * torch.linalg.vector_norm(psi).item()
+
+
+def test_in_place_and_overlapping_products_are_rejected():
+    import torch
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import J1J2Model
+    lat = ChainLattice(12)
+    hs = HilbertSpace(lattice=lat, sym_gen={"translation": 0}, global_syms=[get_u1_sym(lat, 6)])
+    H = J1J2Model(lat, J1=-1.0, J2=-0.3, hilbert_space=hs)
+    x = torch.randn(2 * hs.nh, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        H.matvec_device(x[:hs.nh], out=x[:hs.nh])
+    with pytest.raises(ValueError):
+        H.matvec_device(x[:hs.nh], out=x[1:hs.nh + 1])
+    y = H.matvec_device(x[:hs.nh], out=x[hs.nh:])    # adjacent, not overlapping
+    torch.testing.assert_close(y, H.matvec_device(x[:hs.nh].clone()), rtol=0, atol=0)
+
+
+def test_operator_in_use_refuses_table_and_plan_changes():
+    """A live Lanczos handle or batch holds the operator's plan, assembled
+    table and lookup tables (its captured graphs keep the pointers): changing
+    or freeing them must fail until the handle is destroyed."""
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd._lib import QesError
+    from pyqusolver_amd.batched import BatchedLanczos
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.lanczos import DeviceLanczos
+    from pyqusolver_amd.models import J1J2Model
+    lat = ChainLattice(12)
+    hs = HilbertSpace(lattice=lat, sym_gen={"translation": 0}, global_syms=[get_u1_sym(lat, 6)])
+    H = J1J2Model(lat, J1=-1.0, J2=-0.3, hilbert_space=hs)
+    B = BatchedLanczos([H], max_steps=20)            # assembles H's table for the batch
+    assert H.assembly_bytes > 0
+    with pytest.raises(QesError, match="in use"):
+        H.assemble(False)
+    with pytest.raises(QesError, match="in use"):
+        H.set_option("direct_lookup", 0)
+    with pytest.raises(QesError, match="in use"):
+        hs.set_lookup("sorted")
+    B.close()
+    assert H.assembly_bytes == 0
+    L = DeviceLanczos(H, 10, False, False)
+    with pytest.raises(QesError, match="in use"):
+        H.assemble(True)
+    L.close()
+    H.assemble(True)
+    H.assemble(False)
+    hs.set_lookup("rank")

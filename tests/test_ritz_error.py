@@ -172,3 +172,21 @@ def test_ritz_selection_by_spectrum_end():
     assert select_ritz(2, 6, "SA").tolist() == [0, 1]        # k clipped to the Krylov dimension
     assert select_ritz(3, 6, "BE").tolist() == [0, 1, 2]
     assert select_ritz(5, 1, "BE").tolist() == [4]
+
+
+def test_effective_steps_drops_rows_after_a_breakdown():
+    """A problem that broke down (beta = 0) and kept stepping on a zero
+    vector appends rows (0, 0): its T would carry a spurious eigenvalue 0.
+    Only the rows up to the breakdown span the Krylov space."""
+    from pyqusolver_amd.lanczos import effective_steps, tridiag_eigs
+    a = np.array([2.0, 0.0, 0.0])
+    b = np.array([0.0, 0.0, 0.0])
+    assert tridiag_eigs(a, b)[0][0] == 0.0          # the spurious value
+    k = effective_steps(a, b)
+    assert k == 1 and tridiag_eigs(a[:k], b[:k])[0][0] == 2.0
+    a = np.array([1.0, 3.0, 0.0, 0.0])
+    b = np.array([0.5, 0.0, 0.0, 0.0])
+    assert effective_steps(a, b) == 2
+    a = np.array([1.0, 3.0, 2.0])
+    b = np.array([0.5, 0.4, 0.1])
+    assert effective_steps(a, b) == 3
