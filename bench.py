@@ -1,18 +1,23 @@
 #!/usr/bin/env python
 """Benchmark: symmetric-sector H.v matvec throughput and time-to-E0 on one MI355X.
 
-Workload (default, --config plain): J1-J2 Heisenberg chain L=32, PBC, J1=1,
-J2=0.5 (reference constructor J1J2Model(J1=-1, J2=-0.5)), Sz=0, k=0, spin
-flip +1: N = 9,393,119 reduced states, float64.  A "step" is one H.v on a
-seeded N(0,1) vector already resident in HBM.  After the timed steps a
-Lanczos ground-state solve (full FP64 reorthogonalisation, up to 300 steps)
-reports time-to-E0 against the exact Majumdar-Ghosh energy E0 = -3L/8.
+Workload (default, --config square6x6): J1-J2 Heisenberg 6x6 square
+lattice, PBC, J1=1, J2=0.5 (reference constructor J1J2Model(J1=-1,
+J2=-0.5)), Sz=0, 2D translations k=(0,0), spin flip +1: N = 126,056,625
+reduced states, float64.  A "step" is one matrix-free H.v on a seeded N(0,1)
+vector already resident in HBM.  After the timed steps: the assembled
+operator's product (same H.v through a table), a Lanczos ground-state solve
+(three-term recurrence: one stored vector is 1 GB) to the 1e-10 residual
+test with its time-to-E0, the CPU baseline (the oracle's C restatement of
+the reference push-form kernel on column windows strided across the basis),
+and the J1-J2 chain L=32 (k=0, flip+, N = 9,393,119) as a secondary field.
 
-Other workloads: --config large (J1-J2 6x6 square, k=(0,0), flip +1,
-N = 126,056,625; Lanczos without reorthogonalisation), --config batched
-(XXZ L=24 all momentum sectors + 256 random-field L=20 realisations, one
-batched Lanczos against a sector-by-sector sweep), --config grad
-(Kitaev-Heisenberg honeycomb 24 sites: dE0/dJ and Krylov gradients).
+Other workloads: --config chain32 (the L=32 chain alone, Lanczos with full
+reorthogonalisation against the exact Majumdar-Ghosh E0 = -3L/8), --config
+batched (XXZ L=24 all momentum sectors + 256 random-field L=20
+realisations, one batched Lanczos against a sector-by-sector sweep),
+--config grad (Kitaev-Heisenberg honeycomb 24 sites: dE0/dJ and Krylov
+gradients), --config sweep (6x6 momentum sectors x J2 coupling sweeps).
 
 Prints one JSON line.  Only one GPU is used; --gpus N > 1 exits.
 """
@@ -30,12 +35,15 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per matvec dispatch from rocprofv3 FETCH_SIZE + WRITE_SIZE
 # (KiB per dispatch, FETCH scaled x2.0 and WRITE x1.0 by a 1 GiB device copy
-# measured in the same runs); filled from the committed profiles
+# measured in the same runs); filled from the committed profiles of the
+# current kernels (profiles/matvec_kernel_square6x6_j1j2.md,
+# profiles/matvec_kernel_chain32_j1j2.md)
 TRAFFIC = {
-    # profiles/matvec_kernel_chain32_j1j2.md: 4,764,474 KiB x 2.0 + 73,384 KiB x 1.0
-    "plain": 4764474 * 1024 * 2 + 73384 * 1024,
+    # matvec_kernel_wide<false,2,1,true,4>: 301,958,511 KiB x 2.0 + 984,817 KiB x 1.0
+    "square6x6": 301958511 * 1024 * 2 + 984817 * 1024,
+    # matvec_kernel<false,1,1,true,2>: 3,896,884 KiB x 2.0 + 73,384 KiB x 1.0
+    "chain32": 3896884 * 1024 * 2 + 73384 * 1024,
 }
-
 
 def parse():
     p = argparse.ArgumentParser()
@@ -47,15 +55,21 @@ def parse():
     p.add_argument("--lanczos-steps", type=int, default=300)
     p.add_argument("--no-lanczos", action="store_true")
     p.add_argument("--no-assembled", action="store_true", help="skip the assembled-operator timing")
-    p.add_argument("--cpu-sample", type=int, default=2000, help="columns timed for the CPU baseline (0: skip)")
+    p.add_argument("--cpu-sample", type=int, default=2000,
+                   help="columns timed on one thread for the CPU baseline, in windows strided across the basis "
+                        "(0: skip)")
+    p.add_argument("--no-secondary", action="store_true", help="square6x6: skip the L=32 chain field")
+    p.add_argument("--no-ritz", action="store_true", help="skip the Ritz-vector residual check (three-term runs)")
     p.add_argument("--quiet", action="store_true")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="also run a plain device copy of MIB MiB (profiler counter calibration)")
-    p.add_argument("--config", default="plain", choices=["plain", "batched", "grad", "large", "sweep"],
-                   help="plain: L=32 J1-J2 chain matvec + Lanczos (default); batched: XXZ L=24 all momentum "
+    p.add_argument("--config", default="square6x6",
+                   choices=["square6x6", "chain32", "batched", "grad", "sweep", "large", "plain"],
+                   help="square6x6 (default; alias large): J1-J2 6x6 square k=(0,0) flip+ matvec + Lanczos (no "
+                        "reorthogonalisation) + the L=32 chain as a secondary field; chain32 (alias plain): L=32 "
+                        "J1-J2 chain matvec + Lanczos with full reorthogonalisation; batched: XXZ L=24 all momentum "
                         "sectors + random-field L=20 realisations, batched vs sector-by-sector; grad: "
-                        "Kitaev-Heisenberg honeycomb 24 sites, dE0/dJ and Krylov time-evolution gradients; large: "
-                        "J1-J2 6x6 square k=(0,0) flip+ matvec + Lanczos (no reorthogonalisation); sweep: "
+                        "Kitaev-Heisenberg honeycomb 24 sites, dE0/dJ and Krylov time-evolution gradients; sweep: "
                         "J1-J2 6x6 square, every momentum sector (flip+) at J2 in {0.4, 0.5, 0.6}, one coupling "
                         "sweep per sector (several minutes)")
     p.add_argument("--realisations", type=int, default=256)
@@ -66,7 +80,9 @@ def parse():
     p.add_argument("--sector-start", type=int, default=0, help="sweep: first sector (row-major (kx, ky) order)")
     p.add_argument("--baseline-sectors", type=int, default=1,
                    help="sweep: sectors also solved once per J2 value (the sequential baseline)")
-    return p.parse_args()
+    a = p.parse_args()
+    a.config = {"large": "square6x6", "plain": "chain32"}.get(a.config, a.config)
+    return a
 
 
 def bench_batched(a):
@@ -318,12 +334,15 @@ def _cpu_model() -> str:
 
 def cpu_baseline(wl, J2, reps, norms, sample, n):
     """CPU port of the reference push-form kernel (oracle/qes_oracle.c),
-    extrapolated to matvecs/s from bounded samples: the first ``sample``
-    columns on one thread (the reference matvec_fun is single-threaded,
-    special_operator.py:656-663), then column ranges on every host core the
-    box gives this process (at most 16), each thread with its own output
-    buffer like the reference's multithreaded kernel (matrix_builder.py:
-    1380-1470).  The multi-core rate is the reported value."""
+    extrapolated to matvecs/s from bounded samples: column windows at
+    positions i*N/W (i = 0..W-1, strided across the whole sorted basis, so
+    low and high representatives both count), first ``sample`` columns in
+    all on one thread (the reference matvec_fun is single-threaded,
+    special_operator.py:656-663), then the windows that follow them on every
+    host core the box gives this process (at most 16), each thread with its
+    own output buffer like the reference's multithreaded kernel
+    (matrix_builder.py:1380-1470).  The multi-core rate is the reported
+    value."""
     import os
     from concurrent.futures import ThreadPoolExecutor
     from oracle import qes_oracle as O
@@ -334,32 +353,45 @@ def cpu_baseline(wl, J2, reps, norms, sample, n):
     c, s, co = t.arrays()
     x = np.random.default_rng(0).standard_normal(len(reps))
     cg = OC.CGroup(grp)
-    k1 = min(sample, len(reps))
+    nwin = 32
+    starts = [(i * n) // nwin for i in range(nwin)]
+    w1 = max(1, sample // nwin)
+    r1 = [(st, min(n, st + w1)) for st in starts]
+    y = np.zeros_like(x)
     t0 = time.perf_counter()
-    OC.push_matvec(grp, reps, norms, c, s, co, False, x, k_range=(0, k1), cgroup=cg)
+    for r in r1:
+        OC.push_matvec(grp, reps, norms, c, s, co, False, x, k_range=r, cgroup=cg, y=y)
     dt = time.perf_counter() - t0
-    cols_1 = k1 / dt
+    cols_1 = sum(b - a for a, b in r1) / dt
     nth = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1), os.cpu_count() or 1))
-    per = max(1, sample // 2)
-    ranges = [(k1 + i * per, min(len(reps), k1 + (i + 1) * per)) for i in range(nth)]
-    ranges = [r for r in ranges if r[0] < r[1]]
+    wm = max(1, (sample // 2) * nth // nwin)
+    rm = [(min(n, st + w1), min(n, st + w1 + wm)) for st in starts]
+    per_thread = [rm[i::nth] for i in range(nth)]
+    per_thread = [p for p in per_thread if p]
+    ys = [np.zeros_like(x) for _ in per_thread]
+
+    def work(i):
+        for r in per_thread[i]:
+            OC.push_matvec(grp, reps, norms, c, s, co, False, x, k_range=r, cgroup=cg, y=ys[i])
     t0 = time.perf_counter()
     # ctypes releases the GIL inside the C kernel, so the threads run in parallel
-    with ThreadPoolExecutor(max_workers=len(ranges)) as ex:
-        list(ex.map(lambda r: OC.push_matvec(grp, reps, norms, c, s, co, False, x, k_range=r, cgroup=cg), ranges))
+    with ThreadPoolExecutor(max_workers=len(per_thread)) as ex:
+        list(ex.map(work, range(len(per_thread))))
     dtm = time.perf_counter() - t0
-    cols_m = sum(b - a for a, b in ranges) / dtm
-    return {"value": cols_m / n, "unit": "matvec/s", "cores": len(ranges), "kind": "port",
+    cols_m = sum(b - a for a, b in rm) / dtm
+    return {"value": cols_m / n, "unit": "matvec/s", "cores": len(per_thread), "kind": "port",
             "single_thread_value": cols_1 / n, "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
-            "sample": f"push-form columns of the same {wl['name']} sector: 0..{k1} on 1 thread ({dt:.1f} s, "
-                      f"{cols_1:.0f} columns/s), then {len(ranges)} threads x {per} columns ({dtm:.1f} s wall, "
-                      f"{cols_m:.0f} columns/s), extrapolated to N={n}; reference group action per element and "
-                      f"binary search in place of the 2^{wl['ns']}-entry dense map"}
+            "sample_positions": starts,
+            "sample": f"push-form columns of the same {wl['name']} sector in {nwin} windows at i*N/{nwin} "
+                      f"(i = 0..{nwin - 1}): {w1} columns per window on 1 thread ({dt:.1f} s, {cols_1:.0f} "
+                      f"columns/s), then the next {wm} columns per window on {len(per_thread)} threads "
+                      f"({dtm:.1f} s wall, {cols_m:.0f} columns/s), extrapolated to N={n}; reference group action "
+                      f"per element and binary search in place of the 2^{wl['ns']}-entry dense map"}
 
 
 WORKLOADS = {
     # J1-J2 chain, k=0, flip +1 (the plain-at-scale workload)
-    "plain": lambda a: {"kind": "chain", "lx": a.L, "ly": 1, "ns": a.L, "n_up": a.L // 2,
+    "chain32": lambda a: {"kind": "chain", "lx": a.L, "ly": 1, "ns": a.L, "n_up": a.L // 2,
                         "sym": {"translation": 0, "parity": {"x": 1}},
                         "oracle_gens": [("tx", 0), ("Px", 1)],
                         "name": f"J1-J2 chain L={a.L} PBC J1=1 J2={a.J2} Sz=0 k=0 flip+",
@@ -367,7 +399,7 @@ WORKLOADS = {
                         "reorth": True},
     # J1-J2 6x6 square, k=(0,0), flip +1: no full reorthogonalisation (one
     # basis vector is 1 GB; 300 of them do not fit next to the operator)
-    "large": lambda a: {"kind": "square", "lx": 6, "ly": 6, "ns": 36, "n_up": 18,
+    "square6x6": lambda a: {"kind": "square", "lx": 6, "ly": 6, "ns": 36, "n_up": 18,
                         "sym": {"translation": {"kx": 0, "ky": 0}, "parity": {"x": 1}},
                         "oracle_gens": [("tx", 0), ("ty", 0), ("Px", 1)],
                         "name": f"J1-J2 square 6x6 PBC J1=1 J2={a.J2} Sz=0 k=(0,0) flip+",
@@ -375,7 +407,11 @@ WORKLOADS = {
 }
 
 
-def bench_plain(a):
+def measure_sector(a, config, steps, warmup, primary=True):
+    """One sector workload: basis, E_off, timed matrix-free products (HIP
+    events on the launch stream), the assembled product, host-buffer
+    products (N < 5e7), the Lanczos solve and, for the primary workload,
+    the CPU baseline.  Returns the fields of the JSON line."""
     import torch
     from pyqusolver_amd import ChainLattice, SquareLattice, get_u1_sym
     from pyqusolver_amd.hilbert import HilbertSpace
@@ -384,7 +420,7 @@ def bench_plain(a):
     # the timed time-to-E0
     from pyqusolver_amd import lanczos as _lanczos  # noqa: F401
 
-    wl = WORKLOADS[a.config](a)
+    wl = WORKLOADS[config](a)
     J2 = a.J2
     lat = ChainLattice(wl["lx"]) if wl["kind"] == "chain" else SquareLattice(wl["lx"], wl["ly"])
     torch.cuda.synchronize()
@@ -398,30 +434,46 @@ def bench_plain(a):
     log(f"{wl['name']}: basis N={n} built in {t_basis:.3f}s, E_off={e_off}", a.quiet)
 
     # seeded N(0,1) inputs resident on the device before the timed region
-    nx = min(max(a.steps, 1), 10 if n < 5e7 else 2)
+    nx = min(max(steps, 1), 10 if n < 5e7 else 2)
     xs = [torch.from_numpy(np.random.default_rng(i).standard_normal(n)).to("cuda") for i in range(nx)]
     y = torch.empty(n, dtype=torch.float64, device="cuda")
-    for i in range(a.warmup):
+    for i in range(warmup):
         H.matvec_device(xs[i % nx], out=y)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record(stream)
-    for i in range(a.steps):
+    for i in range(steps):
         evs[i][0].record(stream)
         H.matvec_device(xs[i % nx], out=y)
         evs[i][1].record(stream)
     stop.record(stream)
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
-    kern_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
-    ms_per_step = total_ms / a.steps
+    step_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
+    ms_per_step = total_ms / steps
+    # the dominant kernel alone (the product is a short prescale pass, then
+    # the row kernel): its launch duration from HIP events on the same stream
+    import ctypes as C
+    from pyqusolver_amd import _lib
+    kev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    pev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    ws = torch.empty_like(y)
+    op = H._device_op()
+    for i in range(steps):
+        x = xs[i % nx]
+        pev[i][0].record(stream)
+        _lib.check(_lib.lib().qes_matvec_ws(op, _lib.dptr(x), _lib.dptr(y), 1, 0, 0, _lib.dptr(ws), n * 8,
+                                            C.c_void_p(stream.cuda_stream)), "qes_matvec_ws")
+        pev[i][1].record(stream)
+    torch.cuda.synchronize()
+    del ws
     b_alg = n * (8 + 8 + 8 + 8) + e_off * 8
-    achieved = b_alg / (kern_ms * 1e-3) / 1e9
-    log(f"matvec {ms_per_step:.3f} ms/step (kernel {kern_ms:.3f} ms), B_alg {b_alg/1e9:.3f} GB, "
-        f"{achieved:.1f} GB/s", a.quiet)
+    prod_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in pev]))
+    log(f"matvec {ms_per_step:.3f} ms/step (events {step_ms:.3f} ms, product {prod_ms:.3f} ms), B_alg "
+        f"{b_alg/1e9:.3f} GB, {b_alg / (prod_ms * 1e-3) / 1e9:.1f} GB/s", a.quiet)
     # assembled operator (table built once, streamed per matvec): same H.v,
     # reported beside the matrix-free value
     assembled = None
@@ -436,11 +488,11 @@ def bench_plain(a):
         else:
             torch.cuda.synchronize()
             t_build = time.perf_counter() - t0
-            for i in range(a.warmup):
+            for i in range(warmup):
                 H.matvec_device(xs[i % nx], out=y)
-            ev2 = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+            ev2 = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
             torch.cuda.synchronize()
-            for i in range(a.steps):
+            for i in range(steps):
                 ev2[i][0].record(stream)
                 H.matvec_device(xs[i % nx], out=y)
                 ev2[i][1].record(stream)
@@ -470,7 +522,7 @@ def bench_plain(a):
         log(f"matvec_fun on host numpy arrays: {host_ms:.2f} ms/call (PCIe-inclusive)", a.quiet)
     del xs
 
-    if a.calibrate_copy > 0:
+    if primary and a.calibrate_copy > 0:
         # plain copy of a known size: its FETCH_SIZE / WRITE_SIZE per dispatch
         # calibrate the profiler's byte counters
         nel = a.calibrate_copy * (1 << 20) // 8
@@ -503,25 +555,27 @@ def bench_plain(a):
         r = H.last_lanczos
         target = e_exact if e_exact is not None else float(e0)
         hit = None
-        for steps, e in r.history:
+        for st_, e in r.history:
             if abs(e - target) <= 1e-10 * abs(target):
-                hit = steps
+                hit = st_
                 break
         lanczos = {"E0": float(e0), "E0_exact": e_exact,
                    "rel_err": (abs(e0 - e_exact) / abs(e_exact)) if e_exact else None,
                    "steps": int(r.steps), "steps_to_1e-10": hit,
                    "steps_to_1e-10_reference": "exact E0" if e_exact is not None else "final E0 (self)",
-                   "residual_bound": r.residual_bound, "converged": bool(r.converged),
+                   "residual_bound": r.residual_bound,
+                   "rel_error_bound": r.residual_bound / max(1.0, abs(float(e0))),
+                   "converged": bool(r.converged),
                    "stopping": "Ritz residual |beta_m s_m| <= 1e-10 max(1,|E0|), tested at most 10 steps apart, "
-                               "where the residual's decay predicts the crossing",
-                   "seconds": round(dt, 4),
+                               "where the residual's decay predicts the crossing; |E0 - lambda| <= residual_bound",
+                   "time_to_e0_s": round(dt, 4),
                    "seconds_assembly": round(t_asm, 4), "seconds_solve": round(dt - t_asm, 4),
                    "assembled_operator": bool(r.assembled),
                    "reorth": (f"full FP64 classical Gram-Schmidt against all stored vectors, DGKS second pass "
                               f"({r.second_passes} of {r.steps} steps)") if reorth else
                              "none (three-term recurrence)"}
         log(f"lanczos E0={e0:.14f} steps={r.steps} {dt:.2f}s residual<={r.residual_bound:.2e}", a.quiet)
-        if a.config == "large":
+        if not reorth and not a.no_ritz:
             # accuracy witness outside the timed solve: the ground-state Ritz
             # vector (second run of the recurrence, no stored basis) and its
             # true residual ||H psi - E0 psi|| / ||psi||
@@ -537,35 +591,65 @@ def bench_plain(a):
             log(f"ritz vector residual {lanczos['ritz_vector_residual']:.2e}", a.quiet)
 
     cpu = None
-    if a.cpu_sample > 0:
+    if primary and a.cpu_sample > 0:
         try:
             cpu = cpu_baseline(wl, J2, hs.representative_list, hs.normalization, a.cpu_sample, n)
         except Exception as ex:  # baseline is reported, never required
             cpu = {"value": None, "error": str(ex)}
-
+    achieved = b_alg / (prod_ms * 1e-3) / 1e9
     out = {
-        "metric": "H·v matvecs/s + HBM GB/s frac; time-to-E0 at 1e-10 (batched/plain)",
-        "value": 1000.0 / ms_per_step,
-        "unit": "matvec/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
+        "value": 1000.0 / ms_per_step, "ms_per_step": ms_per_step,
         "config": {"workload": wl["name"], "N": n, "E_off": e_off, "group_order": hs.group.order,
                    "basis_build_s": round(t_basis, 4)},
         "host_buffer_ms_per_matvec": host_ms,
         "assembled": assembled,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(a.config),
-                     "kernel": "matvec_kernel", "kernel_ms": kern_ms, "bytes_alg": b_alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(config),
+                     "kernel": "matvec_kernel_wide" if wl["ns"] > 32 else "matvec_kernel",
+                     "kernel_ms": prod_ms, "bytes_alg": b_alg,
+                     "bytes_alg_formula": "N*(8 rep + 8 norm + 8 x_a + 8 y_a) + E_off*8 x gathers",
+                     "launches": "prescale_kernel (N*24 B, ~0.4% of the time) + the row kernel; kernel_ms is "
+                                 "the product's HIP-event time on its stream"},
         "cpu_baseline": cpu,
         "lanczos": lanczos,
     }
+    del H, hs
+    torch.cuda.synchronize()
+    return out
+
+
+def bench_plain(a):
+    import torch
+    r = measure_sector(a, a.config, a.steps, a.warmup, primary=True)
+    out = {
+        "metric": "H·v matvecs/s + HBM GB/s frac; time-to-E0 at 1e-10 (batched/plain)",
+        "value": r["value"],
+        "unit": "matvec/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": r["config"],
+        "roofline": r["roofline"],
+        "cpu_baseline": r["cpu_baseline"],
+        "assembled": r["assembled"],
+        "lanczos": r["lanczos"],
+        "time_to_e0_s": (r["lanczos"] or {}).get("time_to_e0_s"),
+        "host_buffer_ms_per_matvec": r["host_buffer_ms_per_matvec"],
+    }
+    if a.config == "square6x6" and not a.no_secondary:
+        torch.cuda.empty_cache()
+        sec = measure_sector(a, "chain32", 10, 2, primary=False)
+        out["secondary"] = {"workload": sec["config"]["workload"], "N": sec["config"]["N"],
+                            "E_off": sec["config"]["E_off"], "matvec_per_s": sec["value"],
+                            "ms_per_matvec": sec["ms_per_step"], "roofline": sec["roofline"],
+                            "assembled": sec["assembled"], "host_buffer_ms_per_matvec": sec["host_buffer_ms_per_matvec"],
+                            "lanczos": sec["lanczos"]}
     print(json.dumps(out), flush=True)
 
 

@@ -69,9 +69,10 @@ class CGroup:
                          self.cr.ctypes.data, self.ci.ctypes.data)
 
 
-def push_matvec(group: O.Group, reps, norms, codes, sites, coeffs, is_complex, x, k_range=None, cgroup=None):
-    """Accumulate the push-form matvec over columns k_range (default all).
-    Returns (y, n_gathers)."""
+def push_matvec(group: O.Group, reps, norms, codes, sites, coeffs, is_complex, x, k_range=None, cgroup=None,
+                y=None):
+    """Accumulate the push-form matvec over columns k_range (default all)
+    into y (zeros of x's shape and dtype when None).  Returns (y, n_gathers)."""
     cg = cgroup or CGroup(group)
     reps = np.ascontiguousarray(reps, dtype=np.int64)
     norms = np.ascontiguousarray(norms, dtype=np.float64)
@@ -82,7 +83,10 @@ def push_matvec(group: O.Group, reps, norms, codes, sites, coeffs, is_complex, x
     cre, cim = np.ascontiguousarray(co.real), np.ascontiguousarray(co.imag)
     xc = np.iscomplexobj(x)
     xa = np.ascontiguousarray(x, dtype=np.complex128 if xc else np.float64)
-    y = np.zeros_like(xa)
+    if y is None:
+        y = np.zeros_like(xa)
+    elif y.shape != xa.shape or y.dtype != xa.dtype or not y.flags.c_contiguous:
+        raise ValueError("y must be a C-contiguous array of x's shape and dtype")
     k0, k1 = (0, n) if k_range is None else k_range
     g = lib().oracle_push_matvec(C.byref(cg.s), reps.ctypes.data, norms.ctypes.data, n, codes.ctypes.data,
                                  sites.ctypes.data, cre.ctypes.data, cim.ctypes.data, len(codes),
