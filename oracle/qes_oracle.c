@@ -205,3 +205,87 @@ int64_t oracle_push_matvec(const ogroup* G, const int64_t* reps, const double* n
   free(ov);
   return gathers;
 }
+
+/*
+ * Representatives and normalisations of the states of popcount n_up whose
+ * colex rank lies in [rank0, rank0 + count): the scan of
+ * scan_chunk_find_representatives / compute_normalization
+ * (Algebra/Symmetries/jit/symmetry_container_jit.py:154-255) -- s is a
+ * representative iff no group element maps it below itself, and its
+ * normalisation is sqrt|sum over g with g(s) = s of conj(chi_g)|, kept
+ * when that sum exceeds 1e-7 (symmetry_container.py:55).  Colex order of
+ * the states of one popcount is their integer order, so the output is
+ * sorted.  Each element acts as the composite of its primitive ops
+ * (apply_element), tabulated once per call as a site permutation plus a
+ * complement flag (elements here are translations, reflections and spin
+ * flips, which compose to exactly that).  No intrinsic (parity-y) phases.
+ * Returns the number of representatives written.
+ */
+static uint64_t binom_u64(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  if (k > n - k) k = n - k;
+  unsigned __int128 r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (unsigned)(n - k + i) / (unsigned)i;
+  return (uint64_t)r;
+}
+
+int64_t oracle_scan_window(const ogroup* G, int n_up, uint64_t rank0, int64_t count, int64_t* reps_out,
+                           double* norms_out) {
+  const int ns = G->ns;
+  const int ng = G->n_group;
+  const int64_t mask = ns == 64 ? -1 : (((int64_t)1 << ns) - 1);
+  int32_t* perm = (int32_t*)malloc(sizeof(int32_t) * (size_t)ng * ns);
+  int64_t* flip = (int64_t*)malloc(sizeof(int64_t) * (size_t)ng);
+  for (int g = 0; g < ng; ++g) {
+    flip[g] = apply_element(G, 0, g);   /* 0 or all ones */
+    for (int src = 0; src < ns; ++src) {
+      const int64_t img = apply_element(G, (int64_t)1 << src, g) ^ flip[g];
+      int dst = 0;
+      while (dst < ns && !((img >> dst) & 1)) ++dst;
+      perm[g * ns + src] = dst;
+    }
+  }
+  /* unrank: the state with colex rank rank0 among the n_up-subsets of ns bits */
+  uint64_t r = rank0;
+  int64_t s = 0;
+  for (int i = n_up, c = ns - 1; i >= 1; --i) {
+    while (c >= i - 1 && binom_u64(c, i) > r) --c;
+    s |= (int64_t)1 << c;
+    r -= binom_u64(c, i);
+    --c;
+  }
+  int64_t found = 0;
+  for (int64_t t = 0; t < count; ++t) {
+    int rep = 1;
+    double sre = 0.0, sim = 0.0;
+    for (int g = 0; g < ng && rep; ++g) {
+      int64_t img = 0;
+      const int32_t* pg = perm + g * ns;
+      for (int src = 0; src < ns; ++src)
+        if ((s >> src) & 1) img |= (int64_t)1 << pg[src];
+      img = (img ^ flip[g]) & mask;
+      if (img < s) rep = 0;
+      else if (img == s) {
+        sre += G->chi_re[g];
+        sim -= G->chi_im[g];
+      }
+    }
+    if (rep) {
+      const double a = sqrt(sre * sre + sim * sim);
+      if (a > 1e-7) {
+        reps_out[found] = s;
+        norms_out[found] = sqrt(a);
+        ++found;
+      }
+    }
+    /* next state of the same popcount in increasing order (Gosper) */
+    if (t + 1 < count) {
+      const int64_t lowbit = s & -s;
+      const int64_t ripple = s + lowbit;
+      s = (((ripple ^ s) >> 2) / lowbit) | ripple;
+    }
+  }
+  free(perm);
+  free(flip);
+  return found;
+}

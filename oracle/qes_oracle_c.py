@@ -38,6 +38,8 @@ def lib():
         L.oracle_push_matvec.argtypes = [C.POINTER(_OGroup), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                          C.c_void_p, C.c_int, C.c_int64, C.c_int64]
+        L.oracle_scan_window.restype = C.c_int64
+        L.oracle_scan_window.argtypes = [C.POINTER(_OGroup), C.c_int, C.c_uint64, C.c_int64, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -92,3 +94,24 @@ def push_matvec(group: O.Group, reps, norms, codes, sites, coeffs, is_complex, x
                                  sites.ctypes.data, cre.ctypes.data, cim.ctypes.data, len(codes),
                                  1 if is_complex else 0, xa.ctypes.data, y.ctypes.data, 1 if xc else 0, k0, k1)
     return y, int(g)
+
+
+def scan_window(group: O.Group, rank0: int, count: int, cgroup=None, threads: int = 1):
+    """Representatives and norms among the states of the group's filling
+    with colex rank in [rank0, rank0 + count) (oracle_scan_window), split
+    over ``threads`` host threads (ctypes releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    cg = cgroup or CGroup(group)
+    n_up = int(group.n_up)
+    parts = max(1, int(threads))
+    bounds = [rank0 + (count * i) // parts for i in range(parts + 1)]
+
+    def one(i):
+        r0, c = bounds[i], bounds[i + 1] - bounds[i]
+        reps = np.zeros(max(c, 1), np.int64)
+        norms = np.zeros(max(c, 1), np.float64)
+        m = lib().oracle_scan_window(C.byref(cg.s), n_up, r0, c, reps.ctypes.data, norms.ctypes.data)
+        return reps[:m], norms[:m]
+    with ThreadPoolExecutor(max_workers=parts) as ex:
+        out = list(ex.map(one, range(parts)))
+    return np.concatenate([o[0] for o in out]), np.concatenate([o[1] for o in out])

@@ -388,3 +388,105 @@ def _dense(h):
     n = h.nh
     cols = [h.matvec_fun(np.eye(n, dtype=np.complex128)[:, j]) for j in range(n)]
     return np.array(cols).T
+
+
+def test_config3_batch_at_size():
+    """The batched workload at full size: XXZ L=24 (Jxy=1, Jz=0.7, Sz=0), all
+    24 momenta with reflection parities at k = 0 and 12 (26 sectors, complex
+    at generic k), plus 256 random-field L=20 realisations (h_i ~ U[-5, 5]
+    from seeds 1000 + r) as one (n, 256) block; every ground state by batched
+    Lanczos against one device Lanczos per problem (1e-10 relative), E(k) =
+    E(-k), every residual bound within the 1e-10 test, and one block column's
+    product against the oracle's row formula on sampled rows."""
+    import torch
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.batched import BatchedLanczos, batched_ground_states, momentum_sectors, sector_by_sector
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import XXZ
+    lat24 = ChainLattice(24)
+    specs = momentum_sectors(24, reflection=True)
+    assert len(specs) == 26
+    spaces = [HilbertSpace(lattice=lat24, sym_gen=s, global_syms=[get_u1_sym(lat24, 12)]) for s in specs]
+    assert sum(hs.nh for hs in spaces) == 2_704_156          # C(24, 12)
+    hams = [XXZ(lat24, hilbert_space=h, jxy=1.0, jz=0.7) for h in spaces]
+    lat20 = ChainLattice(20)
+    hs20 = HilbertSpace(lattice=lat20, global_syms=[get_u1_sym(lat20, 10)])
+    assert hs20.nh == 184_756
+    fields = [np.random.default_rng(1000 + r).uniform(-5.0, 5.0, 20) for r in range(256)]
+    hams20 = [XXZ(lat20, hilbert_space=hs20, jxy=1.0, jz=0.7, hz=h) for h in fields]
+    res, _ = batched_ground_states(hams + hams20, tol=1e-10, max_steps=300)
+    assert np.all(res.residuals <= 1e-10 * np.maximum(1.0, np.abs(res.e0)) * 1.0000001)
+    seq = sector_by_sector(hams + hams20, tol=1e-10, max_steps=300)
+    dev = np.abs(res.e0 - seq.e0) / np.maximum(1.0, np.abs(seq.e0))
+    assert dev.max() <= 1e-10, dev.max()
+    ek = {s["translation"]: e for s, e in zip(specs, res.e0[:26]) if "reflection" not in s}
+    for k in range(1, 12):
+        assert abs(ek[k] - ek[24 - k]) <= 1e-10 * abs(ek[k])
+    # one realisation column of the block product against the oracle's rows
+    B = BatchedLanczos(hams20, max_steps=10)
+    assert len(B.blocks) == 1 and not B.regular
+    rng = np.random.default_rng(5)
+    xs = [rng.standard_normal(hs20.nh) for _ in hams20]
+    ys = B.split(B.matvec(B.concat(xs)))
+    B.close()
+    olat = O.OLattice("chain", 20)
+    grp = O.Group(olat, [], n_up=10)
+    reps, norms = hs20.representative_list, hs20.normalization
+    sample = np.sort(rng.choice(hs20.nh, 300, replace=False))
+    for r in (0, 131, 255):
+        c, s, co = O.terms_xxz(olat, 1.0, 0.7, hz=fields[r]).arrays()
+        yo = O.sampled_rows(grp, reps, norms, sample, 20, c, s, co, True, xs[r])
+        yg = ys[r].cpu().numpy()[sample]
+        assert np.linalg.norm(yg - yo.real) <= 1e-12 * np.linalg.norm(yo), r
+
+
+def test_square6x6_coupling_sweep_sector():
+    """The 6x6 sweep at full size on two sectors: a complex one, k = (1, 2)
+    (N = 126.0 M, complex128), whose three coupling columns (J2 = 0.4, 0.5,
+    0.6 by scaling the J2 coupling) must equal the scaled products of its
+    tagged table and, on sampled rows, the oracle's rows of models built at
+    each J2; and k = (0, 0), whose J2 = 0.5 column's E0 from the sweep equals
+    the plain solve's (1e-10 relative)."""
+    import torch
+    from pyqusolver_amd import SquareLattice, get_u1_sym
+    from pyqusolver_amd.batched import CouplingSweep
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import J1J2Model
+    lat = SquareLattice(6, 6)
+    j2s = [0.4, 0.5, 0.6]
+    scales = [[1.0, j / 0.5] for j in j2s]
+    olat = O.OLattice("square", 6, 6)
+    hs = HilbertSpace(lattice=lat, sym_gen={"translation": {"kx": 1, "ky": 2}, "parity": {"x": 1}},
+                      global_syms=[get_u1_sym(lat, 18)])
+    H = J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=hs)
+    assert H.needs_complex
+    S = CouplingSweep([H], scales, max_steps=20, max_gb=160.0)
+    g = torch.Generator(device="cuda").manual_seed(9)
+    x = torch.randn(3 * hs.nh, dtype=torch.complex128, device="cuda", generator=g)
+    xs = [x[i * hs.nh:(i + 1) * hs.nh] for i in range(3)]
+    y = S.split(S.matvec(S.concat(xs)))
+    grp = O.Group(olat, [O.Generator("tx", 1), O.Generator("ty", 2), O.Generator("Px", 1)], n_up=18)
+    reps, norms = hs.representative_list, hs.normalization
+    sample = np.sort(np.random.default_rng(4).choice(hs.nh, 100, replace=False))
+    for r, j2 in enumerate(j2s):
+        ref = H.matvec_scaled_device(scales[r], xs[r].contiguous())
+        assert torch.allclose(y[r], ref, rtol=1e-13, atol=1e-13 * ref.abs().max().item()), j2
+        c, s, co = O.terms_j1j2(olat, -1.0, -j2).arrays()
+        yo = O.sampled_rows(grp, reps, norms, sample, 36, c, s, co, True, xs[r].cpu().numpy())
+        yg = y[r].cpu().numpy()[sample]
+        assert np.linalg.norm(yg - yo) <= 1e-12 * np.linalg.norm(yo), j2
+    S.close()
+    del S, H, hs, x, xs, y, ref
+    torch.cuda.empty_cache()
+    hs0 = HilbertSpace(lattice=lat, sym_gen={"translation": {"kx": 0, "ky": 0}, "parity": {"x": 1}},
+                       global_syms=[get_u1_sym(lat, 18)])
+    H0 = J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=hs0)
+    S0 = CouplingSweep([H0], scales, max_steps=300, max_gb=160.0)
+    e = S0.ground_states(tol=1e-10).e0
+    resid = S0.residuals()
+    S0.close()
+    assert np.all(resid <= 1e-10 * np.abs(e) * 1.0000001)
+    e_plain = H0.diagonalize(method="lanczos", k=1, tol=1e-10, maxiter=300, seed=0, reorth=False,
+                             store_basis=False, assemble=True)[0]
+    assert abs(e[1] - e_plain) <= 1e-10 * abs(e_plain)
+    assert e[0] < e[1] < e[2]          # E0 rises with J2 at k = 0 (frustration)

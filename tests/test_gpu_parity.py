@@ -938,3 +938,139 @@ def test_operator_in_use_refuses_table_and_plan_changes():
     H.assemble(True)
     H.assemble(False)
     hs.set_lookup("rank")
+
+
+def _colex_rank(state: int) -> int:
+    from math import comb
+    r, i, s = 0, 0, int(state)
+    while s:
+        p = (s & -s).bit_length() - 1
+        i += 1
+        r += comb(p, i)
+        s &= s - 1
+    return r
+
+
+def _colex_unrank(ns: int, k: int, r: int) -> int:
+    from math import comb
+    s, c = 0, ns - 1
+    for i in range(k, 0, -1):
+        while c >= i - 1 and comb(c, i) > r:
+            c -= 1
+        s |= 1 << c
+        r -= comb(c, i)
+        c -= 1
+    return s
+
+
+@pytest.mark.parametrize("kind", ["chain32", "square6x6"])
+def test_basis_windows_bit_exact_at_full_size(kind):
+    """At full size (L=32: 9,393,119 representatives; 6x6: 126,056,625) the
+    representative list equals the C oracle's orbit scan bit for bit on
+    three windows of 2^22 consecutive colex ranks of the filling: at the
+    first representative, around the median one and ending at the last one
+    (representatives carry their longest zero run on top, so they sit at
+    low ranks); norms to 1e-14."""
+    from math import comb
+    from oracle import qes_oracle_c as OC
+    from pyqusolver_amd import ChainLattice, SquareLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    if kind == "chain32":
+        lat, ns, n_up = ChainLattice(32), 32, 16
+        sym, gens = {"translation": 0, "parity": {"x": 1}}, [("tx", 0), ("Px", 1)]
+        olat = O.OLattice("chain", 32)
+    else:
+        lat, ns, n_up = SquareLattice(6, 6), 36, 18
+        sym, gens = {"translation": {"kx": 0, "ky": 0}, "parity": {"x": 1}}, [("tx", 0), ("ty", 0), ("Px", 1)]
+        olat = O.OLattice("square", 6, 6)
+    hs = HilbertSpace(lattice=lat, sym_gen=sym, global_syms=[get_u1_sym(lat, n_up)])
+    reps = hs.representative_list
+    norms = hs.normalization
+    grp = O.Group(olat, [O.Generator(g, k) for g, k in gens], n_up=n_up)
+    cg = OC.CGroup(grp)
+    w = 1 << 22
+    total = comb(ns, n_up)
+    n = len(reps)
+    starts = [_colex_rank(reps[0]), max(0, _colex_rank(reps[n // 2]) - w // 2), _colex_rank(reps[-1]) - w + 1]
+    checked = 0
+    for r0 in starts:
+        r0 = min(max(0, r0), total - w)
+        ro, no = OC.scan_window(grp, r0, w, cgroup=cg, threads=16)
+        lo = np.searchsorted(reps, _colex_unrank(ns, n_up, r0))
+        hi = np.searchsorted(reps, _colex_unrank(ns, n_up, r0 + w)) if r0 + w < total else n
+        np.testing.assert_array_equal(reps[lo:hi], ro)
+        np.testing.assert_allclose(norms[lo:hi], no, rtol=1e-14)
+        checked += len(ro)
+    assert checked > 100_000
+
+
+def _all_codes_terms(L, rng, u1):
+    """A Hermitian, translation-invariant instruction list that uses every
+    code the VM accepts (operators_spin.py:1084-1118): each non-Hermitian
+    term comes with its conjugate (S+ / S-, S+S+ / S-S-, S+S- / S-S+, complex
+    coefficients conjugated); the mixed Pauli products XY, YX, XZ, ZX, YZ, ZY
+    are Hermitian with real coefficients.  u1: only terms that keep the
+    popcount (diagonal, XX + YY exchange, S+S- / S-S+)."""
+    c = lambda: complex(rng.uniform(-1, 1), rng.uniform(-1, 1))
+    r = lambda: float(rng.uniform(-1, 1))
+    terms = []   # (code, coefficient, [sites]) as functions of the first site i
+    if u1:
+        cz, cx, cpm = r(), r(), c()
+        terms += [(13, lambda i: r0, lambda i: [i]) for r0 in [r()]]
+        terms += [(22, lambda i, v=cz: v, lambda i: [i, (i + 1) % L]),
+                  (20, lambda i, v=cx: v, lambda i: [i, (i + 1) % L]),
+                  (21, lambda i, v=cx: v, lambda i: [i, (i + 1) % L]),
+                  (31, lambda i, v=cpm: v, lambda i: [i, (i + 2) % L]),
+                  (32, lambda i, v=cpm: np.conj(v), lambda i: [i, (i + 2) % L])]
+        return terms
+    cp, cpp, cpm = c(), c(), c()
+    terms += [(11, lambda i, v=r(): v, lambda i: [i]), (12, lambda i, v=r(): v, lambda i: [i]),
+              (13, lambda i, v=r(): v, lambda i: [i]),
+              (14, lambda i, v=cp: v, lambda i: [i]), (15, lambda i, v=cp: np.conj(v), lambda i: [i]),
+              (20, lambda i, v=r(): v, lambda i: [i, (i + 1) % L]), (21, lambda i, v=r(): v, lambda i: [i, (i + 1) % L]),
+              (22, lambda i, v=r(): v, lambda i: [i, (i + 1) % L]),
+              (23, lambda i, v=cpp: v, lambda i: [i, (i + 1) % L]),
+              (24, lambda i, v=cpp: np.conj(v), lambda i: [i, (i + 1) % L]),
+              (31, lambda i, v=cpm: v, lambda i: [i, (i + 2) % L]),
+              (32, lambda i, v=cpm: np.conj(v), lambda i: [i, (i + 2) % L])]
+    for code in (25, 26, 27, 28, 29, 30):
+        terms.append((code, lambda i, v=r(): v, lambda i: [i, (i + 1) % L]))
+    return terms
+
+
+@pytest.mark.parametrize("L,gens,n_up", [(10, [("tx", 3)], None), (11, [("tx", 0)], None),
+                                         (12, [("tx", 5)], 6), (12, [("tx", 6)], 6)])
+def test_every_instruction_code_against_the_oracle(L, gens, n_up):
+    """Codes 11-15 and 20-32 through the HIP product (qes_op_create ->
+    qes_matvec) against the oracle's push-form restatement of the reference
+    kernel on the same instruction arrays; complex coefficients."""
+    from pyqusolver_amd import ChainLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.operator import Hamiltonian, SpinOperator
+    rng = np.random.default_rng(L + (n_up or 0))
+    lat = ChainLattice(L)
+    hs = HilbertSpace(lattice=lat, sym_gen=_sym(gens), global_syms=[get_u1_sym(lat, n_up)] if n_up else [])
+    H = Hamiltonian(hs, dtype=np.complex128)
+    codes, sites, coeffs = [], [], []
+    for code, coef, st in _all_codes_terms(L, rng, n_up is not None):
+        for i in range(L):
+            H.add(SpinOperator(f"c{code}", code, "local" if code < 20 else "correlation", True), coef(i), sites=st(i))
+            codes.append(code)
+            sites.append(st(i) + [-1] * (2 - len(st(i))))
+            coeffs.append(coef(i))
+    used = set(codes)
+    assert used >= ({11, 12, 13, 14, 15, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32} if n_up is None
+                    else {13, 20, 21, 22, 31, 32})
+    grp = O.Group(O.OLattice("chain", L), [O.Generator(k, s) for k, s in gens], n_up=n_up)
+    b = O.scan_basis(grp)
+    np.testing.assert_array_equal(hs.representative_list, b.reps)
+    c_ = np.asarray(codes, np.int32)
+    s_ = np.asarray(sites, np.int32)
+    co_ = np.asarray(coeffs, np.complex128)
+    x = rng.standard_normal(hs.nh) + 1j * rng.standard_normal(hs.nh)
+    yo = O.push_matvec(b, L, c_, s_, co_, True, x)
+    y = H.matvec_fun(x)
+    assert np.linalg.norm(y - yo) <= MATVEC_TOL * np.linalg.norm(yo)
+    # the reduced operator is Hermitian (the pull form relies on it)
+    z = rng.standard_normal(hs.nh) + 1j * rng.standard_normal(hs.nh)
+    assert abs(np.vdot(z, y) - np.vdot(H.matvec_fun(z), x)) <= 1e-12 * np.linalg.norm(y) * np.linalg.norm(z)

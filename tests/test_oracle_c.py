@@ -26,3 +26,34 @@ def test_c_oracle_matches_numpy_oracle(kind, lx, ly, gens, n_up, model):
     y1 = O.push_matvec(b, lat.ns, c, s, co, t.is_complex, x)
     y2, _ = OC.push_matvec(grp, b.reps, b.norms, c, s, co, t.is_complex, x)
     assert np.linalg.norm(y1 - y2) <= 1e-13 * np.linalg.norm(y1)
+
+
+@pytest.mark.parametrize("kind,lx,ly,gens,n_up", [
+    ("chain", 14, 1, [("tx", 0), ("Px", 1)], 7),
+    ("chain", 15, 1, [("tx", 4)], 7),
+    ("chain", 12, 1, [("tx", 6), ("R", -1), ("Px", -1)], 6),
+    ("square", 4, 3, [("tx", 1), ("ty", 2)], 6),
+    ("square", 4, 4, [("tx", 0), ("ty", 0), ("Px", 1)], 8),
+])
+def test_c_window_scan_matches_numpy_scan(kind, lx, ly, gens, n_up):
+    """The C rank-window scan (composite element permutations) equals the
+    NumPy orbit scan over the whole filling and over sub-windows."""
+    from math import comb
+    lat = O.OLattice(kind, lx, ly)
+    grp = O.Group(lat, [O.Generator(k, s) for k, s in gens], n_up=n_up)
+    b = O.scan_basis(grp)
+    total = comb(lat.ns, n_up)
+    reps, norms = OC.scan_window(grp, 0, total, threads=3)
+    np.testing.assert_array_equal(reps, b.reps)
+    np.testing.assert_allclose(norms, b.norms, rtol=1e-15)
+    r0, cnt = total // 3, total // 4
+    lo, hi = np.searchsorted(b.reps, _unrank(lat.ns, n_up, r0)), np.searchsorted(b.reps, _unrank(lat.ns, n_up, r0 + cnt))
+    reps_w, _ = OC.scan_window(grp, r0, cnt)
+    np.testing.assert_array_equal(reps_w, b.reps[lo:hi])
+
+
+def _unrank(ns, k, r):
+    """State of colex rank r among the k-subsets (brute force over the sorted filling)."""
+    from itertools import combinations
+    states = sorted(sum(1 << i for i in c) for c in combinations(range(ns), k))
+    return states[r] if r < len(states) else 1 << ns
