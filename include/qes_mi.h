@@ -201,6 +201,17 @@ int qes_op_matvec_scaled(qes_op_t op, const double* scale, int32_t nscale, const
  * gradient of <g, H(J) v>.  Blocks the host until done. */
 int qes_op_coupling_dots(qes_op_t op, const void* v_dev, const void* g_dev, int32_t dtype, double* out,
                          void* stream);
+/* No counterpart in the reference: the same sums as qes_op_coupling_dots
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc / hipFree:
+ * the per-block partial sums go to a static device array of 2048 * 8 * 2 + 16
+ * doubles (256 KB, held by every GPU the library runs on until the process
+ * ends); inputs of any length (a fixed grid of at most 2048 blocks strides
+ * over the rows); at most 8 coupling ids, otherwise an error.  Calls that
+ * overlap on one GPU share that array and race.  Blocks the host until done;
+ * sums in another order than qes_op_coupling_dots (equal to rounding). */
+int qes_op_coupling_dots_noalloc(qes_op_t op, const void* v_dev, const void* g_dev, int32_t dtype, double* out,
+                                 void* stream);
 /* Coordinate list (row, col, value) of the assembled table on the device --
  * the explicit sparse matrix of the reference's Hamiltonian.build
  * (Algebra/hamil.py:1434-1557 via matrix_builder.py:151-206).  Padding slots

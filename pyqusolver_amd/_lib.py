@@ -70,6 +70,7 @@ SIGNATURES = {
     "qes_batch_create_scaled": [C.POINTER(_P), _I32, _I32, _PD, _I32, _I32, C.POINTER(_P)],
     "qes_op_matvec_scaled": [_P, _PD, _I32, _P, _P, _I32, _P],
     "qes_op_coupling_dots": [_P, _P, _P, _I32, _PD, _P],
+    "qes_op_coupling_dots_noalloc": [_P, _P, _P, _I32, _PD, _P],
     "qes_op_diagonal": [_P, _P, _P, _P],
     "qes_op_set_canonicalizer": [_P, _I32],
     "qes_op_set_option": [_P, C.c_char_p, _I64],

@@ -26,6 +26,7 @@
 // one diagonal per coupling, which gives H(J) x for any scale of each coupling
 // and the per-coupling products Re<g, H_c v> from one pass over the table.
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -335,67 +336,64 @@ __global__ void __launch_bounds__(256) sell_matvec_multi_kernel(AsmTable T, cons
 }
 
 // d_c = sum_a conj(g_a) (H_c v)_a for every coupling c of a tagged table, in
-// one pass: per row, the table entries are split by the coupling of their
-// dictionary value; per block partial sums part[block][c][re, im].
+// Row a of the coupling-resolved product: (H_c v)_a for every coupling id c
+// (the table entries split by the coupling of their dictionary value, the
+// per-coupling diagonals), and g_a.
 template <bool XC>
-__global__ void __launch_bounds__(256) sell_coupling_dots_kernel(AsmTable T, const uint8_t* __restrict__ dict_cpl,
-                                                                 const double* __restrict__ v,
-                                                                 const double* __restrict__ g, int64_t n,
-                                                                 double* __restrict__ part) {
-  __shared__ double sr[4][ASM_MAX_CPL], si[4][ASM_MAX_CPL];
-  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double hr[ASM_MAX_CPL], hi[ASM_MAX_CPL];
+__device__ __forceinline__ void dots_row(const AsmTable& T, const uint8_t* __restrict__ dict_cpl,
+                                         const double* __restrict__ v, const double* __restrict__ g, int64_t a,
+                                         double* hr, double* hi, double& gr, double& gi) {
+  const int64_t s = a / SELL_C;
+  const int l = (int)(a % SELL_C);
+  const int64_t base = T.off[s] + l;
+  const int len = T.len[s];
+  const int dw = T.dvc ? 2 : 1;
+  double vr_a, vi_a = 0.0;
+  if (XC) {
+    vr_a = v[2 * a];
+    vi_a = v[2 * a + 1];
+    gr = g[2 * a];
+    gi = g[2 * a + 1];
+  } else {
+    vr_a = v[a];
+    gr = g[a];
+    gi = 0.0;
+  }
 #pragma unroll
-  for (int c = 0; c < ASM_MAX_CPL; ++c) hr[c] = hi[c] = 0.0;
-  double gr = 0.0, gi = 0.0;
-  if (a < n) {
-    const int64_t s = a / SELL_C;
-    const int l = (int)(a % SELL_C);
-    const int64_t base = T.off[s] + l;
-    const int len = T.len[s];
-    const int dw = T.dvc ? 2 : 1;
-    double vr_a, vi_a = 0.0;
-    if (XC) {
-      vr_a = v[2 * a];
-      vi_a = v[2 * a + 1];
-      gr = g[2 * a];
-      gi = g[2 * a + 1];
-    } else {
-      vr_a = v[a];
-      gr = g[a];
-    }
-#pragma unroll
-    for (int c = 0; c < ASM_MAX_CPL; ++c) {
-      if (c < T.ncpl) {
-        const double* dc = T.diag_c + ((int64_t)c * T.nrows + a) * dw;
-        const double dr = dc[0], di = T.dvc ? dc[1] : 0.0;
-        hr[c] = dr * vr_a - di * vi_a;
-        hi[c] = dr * vi_a + di * vr_a;
-      }
-    }
-    for (int k = 0; k < len; ++k) {
-      const int64_t e = base + (int64_t)k * SELL_C;
-      int64_t b;
-      uint32_t idx;
-      sell_entry(T, e, b, idx);
-      const double wr = T.dict[idx * dw], wi = T.dvc ? T.dict[idx * dw + 1] : 0.0;
-      const double xr = XC ? v[2 * b] : v[b], xi = XC ? v[2 * b + 1] : 0.0;
-      const double pr = wr * xr - wi * xi, pi = wr * xi + wi * xr;
-      const int cc = dict_cpl[idx];
-#pragma unroll
-      for (int c = 0; c < ASM_MAX_CPL; ++c)
-        if (c == cc) {
-          hr[c] += pr;
-          hi[c] += pi;
-        }
+  for (int c = 0; c < ASM_MAX_CPL; ++c) {
+    hr[c] = hi[c] = 0.0;
+    if (c < T.ncpl) {
+      const double* dc = T.diag_c + ((int64_t)c * T.nrows + a) * dw;
+      const double dr = dc[0], di = T.dvc ? dc[1] : 0.0;
+      hr[c] = dr * vr_a - di * vi_a;
+      hi[c] = dr * vi_a + di * vr_a;
     }
   }
-  // conj(g_a) * (H_c v)_a, reduced over the block
+  for (int k = 0; k < len; ++k) {
+    const int64_t e = base + (int64_t)k * SELL_C;
+    int64_t b;
+    uint32_t idx;
+    sell_entry(T, e, b, idx);
+    const double wr = T.dict[idx * dw], wi = T.dvc ? T.dict[idx * dw + 1] : 0.0;
+    const double xr = XC ? v[2 * b] : v[b], xi = XC ? v[2 * b + 1] : 0.0;
+    const double pr = wr * xr - wi * xi, pi = wr * xi + wi * xr;
+    const int cc = dict_cpl[idx];
+#pragma unroll
+    for (int c = 0; c < ASM_MAX_CPL; ++c)
+      if (c == cc) {
+        hr[c] += pr;
+        hi[c] += pi;
+      }
+  }
+}
+
+// per-block sums of (pr[c], pi[c]) into part[block][c][re, im]
+__device__ __forceinline__ void dots_block_reduce(double* pr_, double* pi_, double* __restrict__ part) {
+  __shared__ double sr[4][ASM_MAX_CPL], si[4][ASM_MAX_CPL];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < ASM_MAX_CPL; ++c) {
-    double pr = gr * hr[c] + gi * hi[c];
-    double pi = gr * hi[c] - gi * hr[c];
+    double pr = pr_[c], pi = pi_[c];
     for (int o = 32; o > 0; o >>= 1) {
       pr += __shfl_down(pr, o, 64);
       pi += __shfl_down(pi, o, 64);
@@ -416,6 +414,59 @@ __global__ void __launch_bounds__(256) sell_coupling_dots_kernel(AsmTable T, con
     part[((int64_t)blockIdx.x * ASM_MAX_CPL + c) * 2] = pr;
     part[((int64_t)blockIdx.x * ASM_MAX_CPL + c) * 2 + 1] = pi;
   }
+}
+
+// one pass: conj(g_a) (H_c v)_a per row, per block partial sums
+// part[block][c][re, im] (one row per thread)
+template <bool XC>
+__global__ void __launch_bounds__(256) sell_coupling_dots_kernel(AsmTable T, const uint8_t* __restrict__ dict_cpl,
+                                                                 const double* __restrict__ v,
+                                                                 const double* __restrict__ g, int64_t n,
+                                                                 double* __restrict__ part) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double hr[ASM_MAX_CPL], hi[ASM_MAX_CPL], pr[ASM_MAX_CPL], pi[ASM_MAX_CPL];
+  double gr = 0.0, gi = 0.0;
+#pragma unroll
+  for (int c = 0; c < ASM_MAX_CPL; ++c) hr[c] = hi[c] = 0.0;
+  if (a < n) dots_row<XC>(T, dict_cpl, v, g, a, hr, hi, gr, gi);
+#pragma unroll
+  for (int c = 0; c < ASM_MAX_CPL; ++c) {
+    pr[c] = gr * hr[c] + gi * hi[c];
+    pi[c] = gr * hi[c] - gi * hr[c];
+  }
+  dots_block_reduce(pr, pi, part);
+}
+
+// Same sums with a fixed grid: each block strides over row blocks, so the
+// partials need gridDim.x * ASM_MAX_CPL * 2 doubles whatever n is.
+template <bool XC>
+__global__ void __launch_bounds__(256) sell_coupling_dots_gs_kernel(AsmTable T, const uint8_t* __restrict__ dict_cpl,
+                                                                    const double* __restrict__ v,
+                                                                    const double* __restrict__ g, int64_t n,
+                                                                    double* __restrict__ part) {
+  double hr[ASM_MAX_CPL], hi[ASM_MAX_CPL], pr[ASM_MAX_CPL], pi[ASM_MAX_CPL];
+#pragma unroll
+  for (int c = 0; c < ASM_MAX_CPL; ++c) pr[c] = pi[c] = 0.0;
+  for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < n; a += (int64_t)gridDim.x * blockDim.x) {
+    double gr, gi;
+    dots_row<XC>(T, dict_cpl, v, g, a, hr, hi, gr, gi);
+#pragma unroll
+    for (int c = 0; c < ASM_MAX_CPL; ++c) {
+      pr[c] += gr * hr[c] + gi * hi[c];
+      pi[c] += gr * hi[c] - gi * hr[c];
+    }
+  }
+  dots_block_reduce(pr, pi, part);
+}
+
+// out[c][re, im] = sum over nblk partials (fixed order), one thread per value
+__global__ void dots_finish_kernel(const double* __restrict__ part, int nblk, int ncpl, double* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t >= 2 * ncpl) return;
+  const int c = t >> 1, q = t & 1;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[((int64_t)b * ASM_MAX_CPL + c) * 2 + q];
+  out[t] = s;
 }
 
 void free_assembly(Op* op) {
@@ -855,6 +906,74 @@ int assembled_coupling_dots(Op* op, const double* v, const double* g, int dtype,
       out_host[2 * c] += part[((size_t)b * ASM_MAX_CPL + c) * 2];
       out_host[2 * c + 1] += part[((size_t)b * ASM_MAX_CPL + c) * 2 + 1];
     }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free coupling dots.  Partial sums live in a file-scope device
+// array, `scratch`: each GPU holds its own copy (found with
+// hipGetSymbolAddress once per device, with that device current) until the
+// process ends.  Layout: DOTS_NOALLOC_BLOCKS * ASM_MAX_CPL * 2 partials, then
+// 2 * ASM_MAX_CPL results.  Every block writes all of its partial slots and
+// the finishing kernel writes every result slot it returns, so nothing needs
+// clearing between calls; calls that overlap on one GPU share the array and
+// race.
+// ---------------------------------------------------------------------------
+#define DOTS_NOALLOC_BLOCKS 2048
+#define DOTS_SCRATCH_DOUBLES (DOTS_NOALLOC_BLOCKS * ASM_MAX_CPL * 2 + 2 * ASM_MAX_CPL)
+static __device__ __align__(256) double scratch[DOTS_SCRATCH_DOUBLES];
+
+#define QES_MAX_DEVICES 64
+static double* g_scratch_dev[QES_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// device address of this GPU's scratch array (hipGetSymbolAddress on the
+// first call with that device current)
+static int scratch_ptr(double** out) {
+  int dev = 0;
+  QES_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= QES_MAX_DEVICES) return fail("device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    double* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return check_hip(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+int assembled_coupling_dots_noalloc(const Op* op, const double* v, const double* g, int dtype, double* out_host,
+                                    hipStream_t st) {
+  if (op->asm_fmt < 0 || !op->asm_tagged) return fail("operator has no coupling-tagged assembled table");
+  const int64_t n = op->basis->n;
+  const int nc = op->n_coupling;
+  if (nc < 1 || nc > ASM_MAX_CPL)
+    return fail("the allocation-free coupling dots hold at most " + std::to_string(ASM_MAX_CPL) +
+                " coupling ids (operator has " + std::to_string(nc) + ")");
+  for (int c = 0; c < 2 * nc; ++c) out_host[c] = 0.0;
+  if (n == 0) return 0;
+  const bool xc = dtype == 1;
+  if (op->asm_dvc && !xc) return fail("complex matrix elements need complex128 vectors");
+  double* sp = nullptr;
+  if (scratch_ptr(&sp)) return -1;
+  double* part = sp;
+  double* res = sp + DOTS_NOALLOC_BLOCKS * ASM_MAX_CPL * 2;
+  const int nblk = (int)std::min<int64_t>(DOTS_NOALLOC_BLOCKS, (n + 255) / 256);
+  AsmTable T = asm_table_of(op);
+  T.ncpl = nc;
+  if (xc)
+    hipLaunchKernelGGL((sell_coupling_dots_gs_kernel<true>), dim3((unsigned)nblk), dim3(256), 0, st, T,
+                       op->asm_dict_cpl, v, g, n, part);
+  else
+    hipLaunchKernelGGL((sell_coupling_dots_gs_kernel<false>), dim3((unsigned)nblk), dim3(256), 0, st, T,
+                       op->asm_dict_cpl, v, g, n, part);
+  QES_HIP(hipGetLastError());
+  hipLaunchKernelGGL(dots_finish_kernel, dim3(1), dim3(64), 0, st, part, nblk, nc, res);
+  QES_HIP(hipGetLastError());
+  QES_HIP(hipMemcpyAsync(out_host, res, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, st));
+  QES_HIP(hipStreamSynchronize(st));
   return 0;
 }
 

@@ -455,6 +455,14 @@ int qes_op_coupling_dots(qes_op_t h, const void* v_dev, const void* g_dev, int32
                                  (hipStream_t)stream);
 }
 
+int qes_op_coupling_dots_noalloc(qes_op_t h, const void* v_dev, const void* g_dev, int32_t dtype, double* out,
+                                 void* stream) {
+  if (!h || !v_dev || !g_dev || !out) return fail("null argument");
+  if (dtype != 0 && dtype != 1) return fail("dtype must be 0 (float64) or 1 (complex128)");
+  return assembled_coupling_dots_noalloc(&h->op, (const double*)v_dev, (const double*)g_dev, dtype, out,
+                                         (hipStream_t)stream);
+}
+
 int qes_op_assembly_info(qes_op_t h, int64_t* entries, int64_t* bytes, int32_t* format, int32_t* n_values,
                          int32_t* tagged) {
   if (!h) return fail("null operator");

@@ -189,6 +189,9 @@ int assembled_coo(const Op* op, int32_t* rows, int32_t* cols, double* vals, int6
 int assembled_matvec_scaled(Op* op, const double* scale, int nscale, const double* x, double* y, int dtype,
                             hipStream_t st);
 int assembled_coupling_dots(Op* op, const double* v, const double* g, int dtype, double* out_host, hipStream_t st);
+// same sums with no allocation: partials in a static device array
+int assembled_coupling_dots_noalloc(const Op* op, const double* v, const double* g, int dtype, double* out_host,
+                                    hipStream_t st);
 int pick_canon_public(const QesGroup& G);
 void free_plan(Op* op);
 int canon_check(const Basis* B, int kind, const uint32_t* d_lut, const uint64_t* d_states, int64_t n,
