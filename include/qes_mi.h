@@ -383,6 +383,27 @@ int qes_batch_set_option(qes_batch_t b, const char* name, int64_t value);
 int qes_batch_ground_states(qes_batch_t b, double tol, int32_t check_every, double* e0_host, int32_t* steps_host);
 int qes_batch_destroy(qes_batch_t b);
 
+/*
+ * Block Lanczos contractions on FP64 matrix cores (v_mfma_f64_16x16x4f64):
+ * the dense parts of Hamiltonian.diagonalize(method='block_lanczos')
+ * (Algebra/Hamil/hamil_diag_engine.py:59, :219 block_size; documented at
+ * Algebra/hamil.py:1826-1828), whose eigensolver lives in the absent
+ * general_python submodule.  Vectors are stored one after another (leading
+ * dimension ld doubles, even, 16-byte aligned device memory); complex
+ * vectors (cplx = 1) as interleaved (re, im) pairs, len counting doubles.
+ * qes_block_dot: C = V^H W for nv vectors V and ncol vectors W (ncol <= 16
+ * real, <= 8 complex); C is nv x ncol row-major on the device (complex:
+ * interleaved); ws: device workspace of qes_block_dot_ws_bytes(nv, len)
+ * bytes for the partial sums (fixed summation order).
+ * qes_block_combine: Y_q = beta Y_q + sum_i V_i C[i][q] (beta = 0: Y is
+ * not read; Y may be the same vectors as V).
+ */
+int qes_block_dot_ws_bytes(int32_t nv, int64_t len, int64_t* bytes);
+int qes_block_dot(const double* V, int64_t ldv, int32_t nv, const double* W, int64_t ldw, int32_t ncol, int64_t len,
+                  int32_t cplx, double* C, double* ws, int64_t ws_bytes, void* stream);
+int qes_block_combine(const double* V, int64_t ldv, int32_t nv, const double* C, double beta, double* Y, int64_t ldy,
+                      int32_t ncol, int64_t len, int32_t cplx, void* stream);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* qes_last_error(void);
 

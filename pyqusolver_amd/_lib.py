@@ -99,6 +99,9 @@ SIGNATURES = {
     "qes_batch_set_option": [_P, C.c_char_p, _I64],
     "qes_batch_ground_states": [_P, C.c_double, _I32, _PD, _PI32],
     "qes_batch_destroy": [_P],
+    "qes_block_dot_ws_bytes": [_I32, _I64, _PI64],
+    "qes_block_dot": [_P, _I64, _I32, _P, _I64, _I32, _I64, _I32, _P, _P, _I64, _P],
+    "qes_block_combine": [_P, _I64, _I32, _P, C.c_double, _P, _I64, _I32, _I64, _I32, _P],
     "qes_last_error": [],
     "qes_version": [],
 }

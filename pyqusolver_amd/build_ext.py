@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqes_mi.so")
 SOURCES = ["qes_capi.hip", "qes_basis.hip", "qes_matvec.hip", "qes_assemble.hip", "qes_lanczos.hip",
-           "qes_batched.hip"]
+           "qes_batched.hip", "qes_block.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

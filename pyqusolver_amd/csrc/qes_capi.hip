@@ -735,4 +735,21 @@ int qes_lanczos_ground_state(qes_op_t op, const void* v0_dev, int32_t dtype, int
   return rc;
 }
 
+int qes_block_dot_ws_bytes(int32_t nv, int64_t len, int64_t* bytes) {
+  if (!bytes) return fail("null argument");
+  if (nv < 1 || len < 1) return fail("block sizes must be >= 1");
+  *bytes = block_dot_ws_bytes(nv, len);
+  return 0;
+}
+
+int qes_block_dot(const double* V, int64_t ldv, int32_t nv, const double* W, int64_t ldw, int32_t ncol, int64_t len,
+                  int32_t cplx, double* C, double* ws, int64_t ws_bytes, void* stream) {
+  return block_dot(V, ldv, nv, W, ldw, ncol, len, cplx ? 1 : 0, C, ws, ws_bytes, (hipStream_t)stream);
+}
+
+int qes_block_combine(const double* V, int64_t ldv, int32_t nv, const double* C, double beta, double* Y, int64_t ldy,
+                      int32_t ncol, int64_t len, int32_t cplx, void* stream) {
+  return block_combine(V, ldv, nv, C, beta, Y, ldy, ncol, len, cplx ? 1 : 0, (hipStream_t)stream);
+}
+
 }  // extern "C"

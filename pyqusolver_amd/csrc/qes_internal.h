@@ -193,6 +193,12 @@ int assembled_coupling_dots(Op* op, const double* v, const double* g, int dtype,
 int assembled_coupling_dots_noalloc(const Op* op, const double* v, const double* g, int dtype, double* out_host,
                                     hipStream_t st);
 int pick_canon_public(const QesGroup& G);
+// block Lanczos contractions on FP64 matrix cores (qes_block.hip)
+int64_t block_dot_ws_bytes(int nv, int64_t len);
+int block_dot(const double* V, int64_t ldv, int nv, const double* W, int64_t ldw, int ncol, int64_t len, int cplx,
+              double* C, double* ws, int64_t ws_bytes, hipStream_t st);
+int block_combine(const double* V, int64_t ldv, int nv, const double* C, double beta, double* Y, int64_t ldy, int ncol,
+                  int64_t len, int cplx, hipStream_t st);
 void free_plan(Op* op);
 int canon_check(const Basis* B, int kind, const uint32_t* d_lut, const uint64_t* d_states, int64_t n,
                 int64_t* d_bad, hipStream_t st);

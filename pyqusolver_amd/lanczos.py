@@ -323,3 +323,176 @@ def lanczos_ground_state(ham, k: int = 1, tol: float = 1e-10, maxiter: int = 300
     return LanczosResult(eigenvalues=w[sel].copy(), alpha=a, beta=b, steps=len(a), converged=converged,
                          history=hist, eigenvectors=vecs, seconds=time.perf_counter() - t0,
                          residual_bound=res_bound, second_passes=second)
+
+
+# ---------------------------------------------------------------------------
+# Block Lanczos (Hamiltonian.diagonalize(method='block_lanczos'),
+# Algebra/Hamil/hamil_diag_engine.py:59, block_size :219)
+# ---------------------------------------------------------------------------
+
+class _BlockStore:
+    """Stored Krylov blocks as consecutive vectors of doubles (complex:
+    interleaved), with the FP64-MFMA contractions of libqes_mi."""
+
+    def __init__(self, n: int, cplx: bool, nvec: int, p: int):
+        self.n, self.cplx, self.p = n, cplx, p
+        self.len = 2 * n if cplx else n
+        self.ld = self.len + (self.len & 1)
+        self.V = torch.zeros((nvec, self.ld), dtype=torch.float64, device="cuda")
+        self.W = torch.zeros((p, self.ld), dtype=torch.float64, device="cuda")
+        nb = C.c_int64()
+        _lib.check(_lib.lib().qes_block_dot_ws_bytes(nvec, self.len, C.byref(nb)), "qes_block_dot_ws_bytes")
+        self.ws = torch.empty(max(1, int(nb.value) // 8), dtype=torch.float64, device="cuda")
+
+    def cview(self, t: torch.Tensor) -> torch.Tensor:
+        """(rows, n) complex or real view of stored rows."""
+        x = t[:, :self.len]
+        return x.view(torch.complex128) if self.cplx else x
+
+    def dot(self, V: torch.Tensor, W: torch.Tensor) -> np.ndarray:
+        """V^H W for row blocks V (nv rows) and W (ncol rows)."""
+        nv, ncol = V.shape[0], W.shape[0]
+        C_ = torch.empty(nv * ncol * (2 if self.cplx else 1), dtype=torch.float64, device="cuda")
+        _lib.check(_lib.lib().qes_block_dot(_lib.dptr(V), self.ld, nv, _lib.dptr(W), self.ld, ncol, self.len,
+                                            1 if self.cplx else 0, _lib.dptr(C_), _lib.dptr(self.ws),
+                                            self.ws.numel() * 8, _lib.stream_ptr()), "qes_block_dot")
+        c = C_.cpu().numpy()
+        return (c[0::2] + 1j * c[1::2]).reshape(nv, ncol) if self.cplx else c.reshape(nv, ncol)
+
+    def combine(self, V: torch.Tensor, coef: np.ndarray, beta: float, Y: torch.Tensor):
+        """Y_q = beta Y_q + sum_i V_i coef[i, q]."""
+        nv, ncol = coef.shape
+        if self.cplx:
+            c = np.empty((nv, ncol, 2))
+            c[..., 0], c[..., 1] = coef.real, coef.imag
+        else:
+            c = np.ascontiguousarray(coef.real)
+        ct = torch.from_numpy(np.ascontiguousarray(c).reshape(-1)).to("cuda")
+        _lib.check(_lib.lib().qes_block_combine(_lib.dptr(V), self.ld, nv, _lib.dptr(ct), float(beta), _lib.dptr(Y),
+                                                self.ld, ncol, self.len, 1 if self.cplx else 0, _lib.stream_ptr()),
+                   "qes_block_combine")
+
+
+def _chol_qr(st: _BlockStore, W: torch.Tensor, out: torch.Tensor):
+    """out B = W with out orthonormal (Cholesky QR, twice); returns B (p x p
+    upper triangular) or None when W is (numerically) rank deficient."""
+    B = np.eye(W.shape[0], dtype=np.complex128 if st.cplx else np.float64)
+    src = W
+    for _ in range(2):
+        G = st.dot(src, src)
+        G = 0.5 * (G + G.conj().T)
+        try:
+            R = np.linalg.cholesky(G).conj().T           # G = R^H R
+        except np.linalg.LinAlgError:
+            return None
+        if np.min(np.abs(np.diag(R))) <= 1e-12 * max(1.0, np.max(np.abs(np.diag(R)))):
+            return None
+        st.combine(src, np.linalg.inv(R), 0.0, out)
+        B = R @ B
+        src = out
+    return B
+
+
+def block_lanczos(ham, k: int = 6, block_size: Optional[int] = None, tol: float = 1e-10, maxiter: int = 100,
+                  v0=None, seed: int = 0, which: str = "smallest", compute_eigvec: bool = False) -> LanczosResult:
+    """k extreme eigenpairs by block Lanczos with full reorthogonalisation.
+
+    Each step applies H to a block of p vectors (one product of the
+    operator's multi-column path: an assembled table streams once per four
+    columns), projects out the previous two blocks and then the whole stored
+    basis twice (classical Gram-Schmidt with a second pass), with the
+    projections C = V^H W and the updates W -= V C on FP64 matrix cores, and
+    orthonormalises the block by Cholesky QR (twice).  The block tridiagonal
+    T is solved on the host in FP64; a Ritz pair (theta_i, s_i) is converged
+    when ||B_m s_i[last block]|| <= tol max(1, |theta_i|), the residual norm
+    of its Ritz vector.  ``maxiter``: block steps; the stored basis holds
+    (maxiter + 1) p vectors.  ``block_size`` defaults to max(4, k) (at most 16
+    for real, 8 for complex vectors).  Several vectors per step find
+    degenerate levels that a single start vector cannot."""
+    code = which_code(which)
+    n = ham.nh
+    cplx = bool(ham.needs_complex)
+    pmax = 8 if cplx else 16
+    p = int(block_size) if block_size else max(4, int(k))
+    if not 1 <= p <= pmax:
+        raise ValueError(f"block_size must be 1..{pmax} for {'complex' if cplx else 'real'} vectors")
+    k = int(k)
+    if k < 1 or k > n:
+        raise ValueError(f"k must be in 1..{n}")
+    p = min(p, n)
+    m_max = max(1, min(int(maxiter), (n + p - 1) // p))
+    t0 = time.perf_counter()
+    st = _BlockStore(n, cplx, (m_max + 1) * p, p)
+    dt = torch.complex128 if cplx else torch.float64
+    if v0 is None:
+        g = torch.Generator(device="cuda").manual_seed(int(seed))
+        X0 = torch.randn((p, n), dtype=dt, device="cuda", generator=g)
+    else:
+        X0 = torch.as_tensor(v0).to(device="cuda", dtype=dt).reshape(-1, n)[:p].contiguous()
+        if X0.shape[0] < p:
+            g = torch.Generator(device="cuda").manual_seed(int(seed))
+            X0 = torch.cat([X0, torch.randn((p - X0.shape[0], n), dtype=dt, device="cuda", generator=g)])
+    st.cview(st.W).copy_(X0)
+    if _chol_qr(st, st.W, st.V[0:p]) is None:
+        raise _lib.QesError("block Lanczos: the start block is rank deficient")
+    As, Bs = [], []
+    hist = []
+    converged = False
+    theta = S = None
+    m = 0
+    for j in range(m_max):
+        Xj = st.V[j * p:(j + 1) * p]
+        # W = H X_j through the multi-column product ((n, p) row-major)
+        Y = ham.matvec_device(st.cview(Xj).transpose(0, 1).contiguous())
+        st.cview(st.W).copy_(Y.transpose(0, 1))
+        A = st.dot(Xj, st.W)
+        lo = max(0, j - 1) * p
+        coef = -A if j == 0 else -np.vstack([Bs[-1].conj().T, A])
+        st.combine(st.V[lo:(j + 1) * p], coef, 1.0, st.W)
+        for _ in range(2):                           # full reorthogonalisation, twice
+            C1 = st.dot(st.V[:(j + 1) * p], st.W)
+            st.combine(st.V[:(j + 1) * p], -C1, 1.0, st.W)
+            A = A + C1[j * p:(j + 1) * p]
+        A = 0.5 * (A + A.conj().T)
+        As.append(A)
+        m = j + 1
+        B = _chol_qr(st, st.W, st.V[(j + 1) * p:(j + 2) * p]) if m < m_max else None
+        # block tridiagonal T_m and its Ritz pairs
+        T = np.zeros((m * p, m * p), dtype=np.complex128 if cplx else np.float64)
+        for i in range(m):
+            T[i * p:(i + 1) * p, i * p:(i + 1) * p] = As[i]
+            if i + 1 < m:
+                T[(i + 1) * p:(i + 2) * p, i * p:(i + 1) * p] = Bs[i]
+                T[i * p:(i + 1) * p, (i + 1) * p:(i + 2) * p] = Bs[i].conj().T
+        theta, S = np.linalg.eigh(T)
+        idx = select_ritz(m * p, min(k, m * p), code)
+        if B is None:        # invariant subspace (or step budget): the Ritz pairs are final
+            res = np.zeros(len(idx)) if m < m_max else np.full(len(idx), np.inf)
+            hist.append((m * p, float(theta[idx[0]])))
+            converged = m < m_max and m * p >= k
+            break
+        res = np.linalg.norm(B @ S[(m - 1) * p:m * p, idx], axis=0)
+        Bs.append(B)
+        hist.append((m * p, float(theta[idx[0]])))
+        if m * p >= k and np.all(res <= tol * np.maximum(1.0, np.abs(theta[idx]))):
+            converged = True
+            break
+    idx = select_ritz(m * p, min(k, m * p), code)
+    vals = theta[idx]
+    vecs = None
+    if compute_eigvec:
+        Sk = S[:, idx]
+        out = torch.zeros((len(idx), st.ld), dtype=torch.float64, device="cuda")
+        step = pmax
+        for c0 in range(0, len(idx), step):
+            st.combine(st.V[:m * p], Sk[:, c0:c0 + step], 0.0, out[c0:c0 + step])
+        vecs = st.cview(out).transpose(0, 1).contiguous()
+        if k == 1:
+            vecs = vecs[:, 0]
+    r = LanczosResult(eigenvalues=np.asarray(vals, dtype=np.float64), alpha=np.zeros(0), beta=np.zeros(0),
+                      steps=int(m * p), converged=bool(converged), history=hist, eigenvectors=vecs,
+                      seconds=time.perf_counter() - t0,
+                      residual_bound=float(np.max(res)) if len(idx) else 0.0)
+    r.block_size = p
+    r.block_steps = m
+    return r

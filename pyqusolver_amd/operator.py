@@ -551,6 +551,11 @@ class Hamiltonian:
         must fit in a quarter of the free device memory.  'auto': exact up
         to N = 4096, Lanczos above.
 
+        method='block_lanczos' (hamil_diag_engine.py:59): the k extreme
+        pairs by block Lanczos with ``block_size`` vectors per step
+        (default max(4, k)) and full reorthogonalisation on FP64 matrix
+        cores; ``maxiter`` counts block steps.
+
         method='lanczos': extreme eigenvalue(s) by device Lanczos, k
         defaulting to 6 and which to 'smallest' as in hamil_diag_engine.py:
         183,254-256 ('largest' and 'both' as there).  With the basis kept
@@ -567,9 +572,10 @@ class Hamiltonian:
         ``assemble``: tabulate H for the solve (see ``assemble``); None picks
         it when the table fits in a quarter of the free device memory (at
         most 64 GB) and N >= 65536.  A table built here is freed afterwards."""
-        from .lanczos import lanczos_ground_state, which_code
-        if method not in ("lanczos", "exact", "auto"):
-            raise NotImplementedError(f"method={method!r}: 'exact', 'lanczos' and 'auto' run on the device")
+        from .lanczos import block_lanczos, lanczos_ground_state, which_code
+        if method not in ("lanczos", "exact", "auto", "block_lanczos"):
+            raise NotImplementedError(f"method={method!r}: 'exact', 'lanczos', 'block_lanczos' and 'auto' run on "
+                                      f"the device")
         if kwargs.get("sigma") is not None:
             raise NotImplementedError("shift-invert (sigma) is not part of the device Lanczos path")
         if not kwargs.get("hermitian", True):
@@ -591,9 +597,13 @@ class Hamiltonian:
                 if assemble:
                     raise
         try:
-            res = lanczos_ground_state(self, k=k, tol=tol, maxiter=maxiter, v0=v0, seed=seed, reorth=reorth,
-                                       store_basis=store_basis, compute_eigvec=compute_eigvec,
-                                       check_every=check_every, which=which)
+            if method == "block_lanczos":
+                res = block_lanczos(self, k=k, block_size=kwargs.get("block_size"), tol=tol, maxiter=maxiter, v0=v0,
+                                    seed=seed, which=which, compute_eigvec=compute_eigvec)
+            else:
+                res = lanczos_ground_state(self, k=k, tol=tol, maxiter=maxiter, v0=v0, seed=seed, reorth=reorth,
+                                           store_basis=store_basis, compute_eigvec=compute_eigvec,
+                                           check_every=check_every, which=which)
         finally:
             if built:
                 self.assemble(False)
