@@ -393,8 +393,9 @@ int qes_batch_destroy(qes_batch_t b);
  * vectors (cplx = 1) as interleaved (re, im) pairs, len counting doubles.
  * qes_block_dot: C = V^H W for nv vectors V and ncol vectors W (ncol <= 16
  * real, <= 8 complex); C is nv x ncol row-major on the device (complex:
- * interleaved); ws: device workspace of qes_block_dot_ws_bytes(nv, len)
- * bytes for the partial sums (fixed summation order).
+ * interleaved); ws: device workspace for the partial sums (fixed
+ * summation order) of qes_block_dot_ws_bytes(nv, len) bytes, enough for
+ * every call with at most nv vectors.
  * qes_block_combine: Y_q = beta Y_q + sum_i V_i C[i][q] (beta = 0: Y is
  * not read; Y may be the same vectors as V).
  */

@@ -374,18 +374,25 @@ class _BlockStore:
 
 
 def _chol_qr(st: _BlockStore, W: torch.Tensor, out: torch.Tensor):
-    """out B = W with out orthonormal (Cholesky QR, twice); returns B (p x p
-    upper triangular) or None when W is (numerically) rank deficient."""
-    B = np.eye(W.shape[0], dtype=np.complex128 if st.cplx else np.float64)
+    """out B = W with out orthonormal: shifted Cholesky QR, then Cholesky QR
+    twice (stable for condition numbers up to ~1/eps, Fukaya et al. 2020);
+    returns B (p x p upper triangular) or None when W vanishes (an
+    invariant subspace).  Near-dependent directions of W come out as
+    orthonormal vectors with tiny couplings in B, which the full
+    reorthogonalisation keeps orthogonal to the basis."""
+    p = W.shape[0]
+    B = np.eye(p, dtype=np.complex128 if st.cplx else np.float64)
     src = W
-    for _ in range(2):
+    for it in range(3):
         G = st.dot(src, src)
         G = 0.5 * (G + G.conj().T)
-        try:
-            R = np.linalg.cholesky(G).conj().T           # G = R^H R
-        except np.linalg.LinAlgError:
+        gmax = float(np.max(np.abs(np.diag(G).real)))
+        if it == 0 and not gmax > 1e-300:
             return None
-        if np.min(np.abs(np.diag(R))) <= 1e-12 * max(1.0, np.max(np.abs(np.diag(R)))):
+        shift = 11.0 * (st.len * p + p * (p + 1)) * np.finfo(np.float64).eps * gmax if it == 0 else 0.0
+        try:
+            R = np.linalg.cholesky(G + shift * np.eye(p)).conj().T           # G = R^H R
+        except np.linalg.LinAlgError:
             return None
         st.combine(src, np.linalg.inv(R), 0.0, out)
         B = R @ B
