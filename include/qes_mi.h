@@ -241,9 +241,10 @@ int qes_op_count_outside(qes_op_t op, int64_t* outside);
  */
 int qes_op_set_canonicalizer(qes_op_t op, int32_t kind);
 /* Kernel options: "direct_lookup" (look each exchanged state up before
- * canonicalising it; hits skip the canonicalisation), "bond_order_uniform"
- * (the wavefront walks the bonds in one order in that pass): 1 on, 0 off,
- * -1 (default) on where the basis uses a rank map; "xcd_remap" (0/1);
+ * canonicalising it; hits skip the canonicalisation): 1 on, 0 off, -1
+ * (default) on where the basis uses a rank map; "bond_order_uniform" (the
+ * wavefront walks the bonds in one order in that pass): 1 on, 0 (default)
+ * off, -1 on where a rank map serves; "xcd_remap" (0/1);
  * "top_row_test" (0 default / 1: row lattices test the top row before a
  * direct lookup, single-lookup path only); "direct_batch" (-1 default, 1, 2,
  * 4: rank-map lookups a lane keeps in flight in a qes_matvec_ws product;

@@ -104,10 +104,13 @@ struct Op {
   // blocks spread over all XCDs keep one compact row window hot in the
   // shared MALL, faster than giving each XCD a contiguous slice
   int xcd_remap = 0;
-  // -1 (default): on where the basis looks representatives up in a rank map
-  // (direct hits skip the canonicalisation: L=32 2.70 -> 2.56 ms, 6x6 169 ->
-  // 146 ms); 0 off; 1 on
-  int bond_order_uniform = -1;
+  // direct_lookup -1 (default): on where the basis looks representatives up
+  // in a rank map (direct hits skip the canonicalisation: L=32 2.70 -> 2.56
+  // ms, 6x6 169 -> 146 ms); 0 off; 1 on.  bond_order_uniform: 0 (default)
+  // each lane walks its own active bonds, which fills every slot of the
+  // batched lookups (L=32 1.95 -> 1.89 ms, 6x6 116.9 -> 113.6 ms against the
+  // wavefront-uniform order); 1 uniform order; -1 uniform where a rank map serves
+  int bond_order_uniform = 0;
   int direct_lookup = -1;
   int top_row_test = 0;            // row lattices: rows_top_ok before each direct lookup (K = 1 path)
   // PRE products with a rank map: lookups in flight per lane (1, 2, 4; -1:

@@ -171,7 +171,7 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
   const W mask = (W)C.G.mask;
   const int cw = CPLX ? 2 : 1;
   const bool direct = C.L.rmap != nullptr && C.direct != 0;
-  const bool uniform = C.uniform != 0;
+  const bool uniform = C.uniform > 0 || (C.uniform < 0 && C.L.rmap);
   auto add = [&](int64_t b, double fr, double fi) {
     const double* xp = xs + b * cw;
     if (CPLX) {
