@@ -56,6 +56,15 @@ typedef struct {
   int32_t cell_bits;      /* bits per unit cell = n_basis                         */
   const double* chi_re;   /* host, ntx*nty*nbase                                  */
   const double* chi_im;   /* host, ntx*nty*nbase                                  */
+  /* Twisted translations (boundary fluxes; NULL for none), the reference's
+   * trans_cross masks and lattice.boundary_phases() table
+   * (symmetry_container_jit.py:91-98, translation.py:80-89, base.py:246):
+   * cross_masks[a] (a < ntx) = bits (bit = site) that cross the boundary
+   * during a x translations, cross_masks[ntx + b] (b < nty) during b y
+   * translations; bphase[(d * (ns + 1) + n) * 2 + {0, 1}] = re, im of the
+   * phase for n crossing up spins along d (0 x, 1 y), per translation step. */
+  const uint64_t* cross_masks;
+  const double* bphase;
 } qes_group_spec;
 
 /*

@@ -244,6 +244,11 @@ class Group:
     refl_perm: list = field(default_factory=list)
     inv_perm: list = field(default_factory=list)
     parity_axis: list = field(default_factory=list)
+    # lattice.boundary_phases(): (3, ns + 1) phase per number of up spins
+    # crossing the boundary in one translation step, per direction; None: 1
+    bound_phase: Optional[np.ndarray] = None
+    trans_cross: list = field(default_factory=list)
+    trans_dir: list = field(default_factory=list)
 
     def __post_init__(self):
         ns = self.lat.ns
@@ -284,7 +289,10 @@ class Group:
         for g in gens:
             if g.kind in ("tx", "ty", "tz"):
                 tidx[id(g)] = len(self.trans_perm)
-                self.trans_perm.append(translation_perm(self.lat, "xyz".index(g.kind[1]))[0])
+                perm, cross = translation_perm(self.lat, "xyz".index(g.kind[1]))
+                self.trans_perm.append(perm)
+                self.trans_cross.append(cross)
+                self.trans_dir.append("xyz".index(g.kind[1]))
             elif g.kind == "R":
                 ridx[id(g)] = len(self.refl_perm)
                 self.refl_perm.append(reflection_perm(ns))
@@ -329,15 +337,23 @@ class Group:
     def apply(self, states: np.ndarray, g: int):
         """symmetry_container_jit.py:63-118 on an int64 array of states.
 
-        Returns (new_states, intrinsic_phase) -- boundary phases are 1 without
-        flux, so translations contribute no phase.
+        Returns (new_states, intrinsic_phase): every translation step
+        multiplies bound_phase[direction, occ] with occ the number of up spins
+        (set bits, bit = site) on the sites that cross the boundary
+        (symmetry_container_jit.py:91-98, translation.py:80-89); 1 without flux.
         """
         ns = self.lat.ns
         cur = states.astype(np.int64, copy=True)
         phase = np.ones(cur.shape, dtype=np.complex128)
         for code, a0, a1 in self.elements[g]:
             if code == OP_TRANSLATION:
+                cmask = 0
+                for site in np.nonzero(self.trans_cross[a0])[0]:
+                    cmask |= 1 << int(site)
                 for _ in range(a1):
+                    if self.bound_phase is not None:
+                        occ = _popcount(cur & np.int64(cmask))
+                        phase = phase * self.bound_phase[self.trans_dir[a0]][occ]
                     cur = _apply_perm(cur, ns, self.trans_perm[a0])
             elif code == OP_REFLECTION:
                 cur = _apply_perm(cur, ns, self.refl_perm[a0])
@@ -428,6 +444,12 @@ class Basis:
     state_gmin: np.ndarray     # first minimising group element
     phase_table: np.ndarray
     g_to_pidx: np.ndarray
+
+    def lookup_g(self, s: np.ndarray) -> np.ndarray:
+        """First minimising group element of states of the sector."""
+        pos = np.searchsorted(self.states, s)
+        pos = np.minimum(pos, len(self.states) - 1)
+        return self.state_gmin[pos]
 
     def lookup(self, s: np.ndarray):
         """repr_map/phase_idx lookup for arbitrary states (-1 if not in sector)."""
@@ -654,12 +676,18 @@ def vm_apply(states: np.ndarray, ns: int, codes, sites, coeffs, is_complex: bool
     return outs
 
 
-def push_matvec(basis: Basis, ns: int, codes, sites, coeffs, is_complex: bool, x: np.ndarray) -> np.ndarray:
+def push_matvec(basis: Basis, ns: int, codes, sites, coeffs, is_complex: bool, x: np.ndarray,
+                group: Optional[Group] = None) -> np.ndarray:
     """_apply_op_batch_compact_seq_jit, Algebra/Hilbert/matrix_builder.py:1347-1377.
 
     y[idx] += val * conj(phase_table[phase_idx[s']]) * N_idx / N_k * x[k]
     (output overwritten, as SpecialOperator.matvec_fun allocates zeros,
-    special_operator.py:656-663)."""
+    special_operator.py:656-663).  With ``group`` each element also carries
+    the intrinsic phase ph of the first minimising element g* on s' (the
+    basis vectors of expand_to_full_state_jit carry it, jit :261-315, so
+    U^dagger H U has val * conj(chi(g*)) * ph_{g*}(s') * N_idx / N_k; the
+    reference's kernel leaves ph out -- the same where no element has a
+    state-dependent phase)."""
     x2 = x[:, None] if x.ndim == 1 else x
     dt = np.result_type(x2.dtype, np.complex128)
     y = np.zeros(x2.shape, dtype=dt)
@@ -672,6 +700,13 @@ def push_matvec(basis: Basis, ns: int, codes, sites, coeffs, is_complex: bool, x
         i = idx[m]
         phase = basis.phase_table[pidx[m]]
         factor = val[m] * np.conj(phase) * norms[i] * inv_nk[k]
+        if group is not None:
+            gmin = basis.lookup_g(st[m])
+            ph = np.ones(len(k), dtype=np.complex128)
+            for g in np.unique(gmin):
+                sel = gmin == g
+                ph[sel] = group.apply(st[m][sel], int(g))[1]
+            factor = factor * ph
         np.add.at(y, i, factor[:, None] * x2[k])
     if not np.iscomplexobj(x):
         if np.abs(y.imag).max(initial=0.0) > 1e-12 * max(1.0, np.abs(y).max(initial=0.0)):
@@ -680,10 +715,10 @@ def push_matvec(basis: Basis, ns: int, codes, sites, coeffs, is_complex: bool, x
     return y[:, 0] if x.ndim == 1 else y
 
 
-def reduced_dense(basis: Basis, ns, codes, sites, coeffs, is_complex) -> np.ndarray:
+def reduced_dense(basis: Basis, ns, codes, sites, coeffs, is_complex, group: Optional[Group] = None) -> np.ndarray:
     """Dense reduced H from the push-form matvec on unit vectors."""
     n = len(basis.reps)
-    return push_matvec(basis, ns, codes, sites, coeffs, is_complex, np.eye(n, dtype=np.complex128))
+    return push_matvec(basis, ns, codes, sites, coeffs, is_complex, np.eye(n, dtype=np.complex128), group=group)
 
 
 # ---------------------------------------------------------------------------
@@ -894,6 +929,45 @@ def terms_xxz(lat: OLattice, jxy=1.0, jz=1.0, hx=0.0, hz=0.0) -> Terms:
             if not _isclose0(jz_[i]):
                 t.add(SIG_Z_C, -jz_[i], [i, j])
     return t
+
+
+def bond_windings(lat: OLattice) -> List[Tuple[int, int, Tuple[int, int]]]:
+    """Forward NN bonds (i, j) of oracle_bonds with their winding (wx, wy):
+    +1 where the forward step leaves the cell range along that direction
+    (chain: i -> i+1 at i = L-1; square: (x+1, y) at x = lx-1, (x, y+1) at
+    y = ly-1)."""
+    out = []
+    for (i, j, n) in oracle_bonds(lat, 1):
+        if lat.kind == "chain":
+            w = (1 if i == lat.lx - 1 else 0, 0)
+        elif lat.kind == "square":
+            x, y, _ = lat.get_coordinates(i)
+            w = (1 if (n == 0 and x == lat.lx - 1) else 0, 1 if (n == 1 and y == lat.ly - 1) else 0)
+        else:
+            raise ValueError("twisted bonds for chains and square lattices")
+        out.append((i, j, w))
+    return out
+
+
+def terms_xxz_twisted(lat: OLattice, jxy, jz, flux) -> Terms:
+    """Twisted-boundary XXZ (no reference model): per forward bond
+    -2 jxy (e^{i theta} S+_i S-_j + e^{-i theta} S-_i S+_j) (codes 31, 32;
+    XX + YY = 2 (PM + MP) in the VM's units) and -jz SzSz, theta = w . flux."""
+    t = Terms(lat.ns, is_complex=True)
+    fx, fy = flux
+    for (i, j, (wx, wy)) in bond_windings(lat):
+        ph = np.exp(1j * (wx * fx + wy * fy))
+        t.add(SIG_PM_C, -2.0 * jxy * ph, [i, j])
+        t.add(SIG_MP_C, -2.0 * jxy * np.conj(ph), [i, j])
+        t.add(SIG_Z_C, -jz, [i, j])
+    return t
+
+
+def boundary_phase_table(ns: int, flux) -> np.ndarray:
+    """(3, ns + 1) table exp(i phi_d n) (the lattice's boundary_phases())."""
+    n = np.arange(ns + 1)
+    fx, fy = flux
+    return np.stack([np.exp(1j * fx * n), np.exp(1j * fy * n), np.ones(ns + 1, dtype=np.complex128)])
 
 
 def honeycomb_bond_flux_phase(lat: OLattice, i: int, n: int, flux) -> float:

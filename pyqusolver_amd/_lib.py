@@ -34,6 +34,8 @@ class GroupSpec(C.Structure):
         ("cell_bits", C.c_int32),
         ("chi_re", C.POINTER(C.c_double)),
         ("chi_im", C.POINTER(C.c_double)),
+        ("cross_masks", C.POINTER(C.c_uint64)),
+        ("bphase", C.POINTER(C.c_double)),
     ]
 
 

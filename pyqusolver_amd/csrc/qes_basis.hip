@@ -261,7 +261,7 @@ int build_basis(Basis* B, hipStream_t st) {
   B->e_min = bitlen(lo_rep);
   B->e_max = bitlen(hi_rep);
   if (build_bucket(B, B->bucket_rows_log2, st)) return -1;
-  if (G.ns <= QES_KEY_BITS && !G.has_phase && n > 0) {
+  if (G.ns <= QES_KEY_BITS && !G.has_phase && !G.has_bphase && n > 0) {
     QES_HIP(hipMalloc(&B->keyc, n * sizeof(uint64_t)));
     B->use_packed = G.ns > 32;   // 32-bit keys are narrower below 33 sites
     hipLaunchKernelGGL(keyc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B->reps, B->norms, n,
@@ -601,7 +601,8 @@ __global__ void reduce_kernel(QesGroup G, const double* __restrict__ chi_re, con
 
 int expand_reduce(const Basis* B, int reduce, const double* x_red, double* full, int dtype, hipStream_t st) {
   const QesGroup& G = B->group;
-  if (G.has_phase) return fail("expansion with parity-y intrinsic phases is not supported");
+  if (G.has_phase || G.has_bphase)
+    return fail("expansion with intrinsic phases (parity y, twisted translations) is not supported");
   if (G.ns > 36) return fail("full-space vectors are limited to 36 sites");
   bool chi_cplx = false;
   for (int g = 0; g < B->gsize; ++g) chi_cplx = chi_cplx || std::fabs(B->h_chi_im[g]) > 1e-12;

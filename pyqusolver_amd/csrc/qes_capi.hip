@@ -259,6 +259,8 @@ int qes_basis_create(const qes_group_spec* spec, qes_basis_t* out, int64_t* n_ou
       lo |= (r ? ((1ull << r) - 1ull) : 0ull) << (row * spec->row_bits);
     G.rowlo[a] = lo;
   }
+  if ((spec->cross_masks == nullptr) != (spec->bphase == nullptr))
+    return fail("twisted translations need both cross_masks and bphase");
   B.gsize = gs;
   B.h_chi_re.assign(spec->chi_re, spec->chi_re + gs);
   B.h_chi_im.assign(spec->chi_im, spec->chi_im + gs);
@@ -268,6 +270,22 @@ int qes_basis_create(const qes_group_spec* spec, qes_basis_t* out, int64_t* n_ou
     rc = fail("hipMalloc failed for characters");
   if (!rc && hipMemcpy(B.chi_re, spec->chi_re, gs * 8, hipMemcpyHostToDevice) != hipSuccess) rc = fail("copy chi");
   if (!rc && hipMemcpy(B.chi_im, spec->chi_im, gs * 8, hipMemcpyHostToDevice) != hipSuccess) rc = fail("copy chi");
+  if (!rc && spec->cross_masks) {
+    const size_t nm = (size_t)spec->ntx + spec->nty, np = 2 * ((size_t)spec->ns + 1) * 2;
+    uint64_t* dm = nullptr;
+    double* dp = nullptr;
+    if (hipMalloc(&dm, nm * 8) != hipSuccess || hipMalloc(&dp, np * 8) != hipSuccess ||
+        hipMemcpy(dm, spec->cross_masks, nm * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dp, spec->bphase, np * 8, hipMemcpyHostToDevice) != hipSuccess) {
+      if (dm) (void)hipFree(dm);
+      if (dp) (void)hipFree(dp);
+      rc = fail("device copy of the boundary phases failed");
+    } else {
+      G.bmask = dm;
+      G.bphase = dp;
+      G.has_bphase = 1;
+    }
+  }
   if (!rc) rc = build_basis(&B, st);
   if (rc) {
     qes_basis_destroy(h);
@@ -322,6 +340,8 @@ int qes_basis_destroy(qes_basis_t h) {
   if (!h) return 0;
   if (basis_check_unused(&h->b, "qes_basis_destroy")) return -1;
   Basis& B = h->b;
+  if (B.group.bmask) (void)hipFree((void*)B.group.bmask);
+  if (B.group.bphase) (void)hipFree((void*)B.group.bphase);
   if (B.chi_re) (void)hipFree(B.chi_re);
   if (B.chi_im) (void)hipFree(B.chi_im);
   if (B.reps) (void)hipFree(B.reps);
@@ -353,16 +373,20 @@ int qes_op_create(qes_basis_t bh, const int32_t* codes, const int32_t* sites, co
     return -1;
   if (diag.size() + off.size() > QES_MAX_TERMS) return fail("too many term groups");
   {
-    // Sectors whose reference matrix is not Hermitian: parity y (its +-i
-    // intrinsic phase enters the norms but not the matrix elements) and
-    // reflection combined with a complex (generic-k) translation character.
+    // Reflection combined with a complex (generic-k) translation character is
+    // not a symmetry sector (non-Hermitian block), nor is parity y.  The
+    // boundary phases of twisted translations enter the matrix elements here
+    // (phase of the minimising element on each target state; the reference's
+    // kernel leaves them out of the elements, which makes those blocks
+    // non-Hermitian).
     const QesGroup& G = bh->b.group;
     bool has_rev = false, chi_cplx = false;
     for (int b = 0; b < G.nbase; ++b)
       for (int k = 0; k < QES_MAX_BASE_OPS; ++k) has_rev = has_rev || G.base_ops[b][k] == QES_BOP_REV;
     for (int g = 0; g < bh->b.gsize; ++g) chi_cplx = chi_cplx || std::fabs(bh->b.h_chi_im[g]) > 1e-12;
     if (G.has_phase)
-      return fail("parity-y sectors are not supported for matvec: the reference block is not Hermitian there");
+      return fail("parity-y sectors are not supported for matvec: the reference's +-i phase does not make the "
+                  "elements a representation (non-Hermitian block)");
     if (has_rev && chi_cplx)
       return fail("reflection at a generic momentum is not a symmetry sector (non-Hermitian reference block)");
   }
@@ -373,7 +397,8 @@ int qes_op_create(qes_basis_t bh, const int32_t* codes, const int32_t* sites, co
   op.n_off = (int)off.size();
   op.n_coupling = ncpl;
   op.is_complex = has_imag;
-  op.chi_complex = false;
+  // complex vectors for complex characters and for intrinsic phases
+  op.chi_complex = bh->b.group.has_phase || bh->b.group.has_bphase;
   for (int g = 0; g < bh->b.gsize; ++g)
     if (std::fabs(bh->b.h_chi_im[g]) > 1e-12) op.chi_complex = true;
   op.h_groups = diag;

@@ -35,10 +35,17 @@ struct QesGroup {
   int32_t has_phase;     // some base element carries an intrinsic (+-i) phase
   int32_t n_up;          // U(1) popcount sector, -1 if none
   int32_t z2;            // popcount-parity sector (+1/-1), 0 if none
-  int32_t pad;
+  int32_t has_bphase;    // translations carry boundary (flux) phases
   uint8_t base_ops[QES_MAX_BASE][QES_MAX_BASE_OPS];
   uint64_t mask;                      // (1<<ns)-1
   uint64_t rowlo[QES_MAX_TX];         // per x-power a: bits of each row below a*cell_bits
+  // twisted translations (device arrays, set when has_bphase): bmask[a] =
+  // bits that cross the boundary during a x steps, bmask[ntx + b] during b
+  // y steps; bphase[(d * (ns + 1) + n) * 2 + {re, im}] = phase of n crossing
+  // up spins along direction d (0: x, 1: y), multiplied per step
+  // (symmetry_container_jit.py:91-98, translation.py:80-89)
+  const uint64_t* bmask;
+  const double* bphase;
 };
 
 // Register-sized view of a QesGroup used inside kernels.  Kernel arguments
@@ -46,10 +53,12 @@ struct QesGroup {
 // so the per-x-power row masks live in LDS and the base ops are packed into
 // one 32-bit word (2 bits per op).
 struct GV {
-  int ns, nbase, ntx, nty, row_bits, cell_bits, has_phase, n_up, z2;
+  int ns, nbase, ntx, nty, row_bits, cell_bits, has_phase, n_up, z2, has_bphase;
   uint32_t bops;
   uint64_t mask;
   const uint64_t* rowlo;   // LDS copy, ntx entries
+  const uint64_t* bmask;   // global, ntx + nty entries (has_bphase)
+  const double* bphase;    // global, 2 (ns + 1) complex entries (has_bphase)
   __device__ __forceinline__ int bop(int B, int k) const { return (int)((bops >> ((B * 4 + k) * 2)) & 3u); }
 };
 
@@ -67,6 +76,9 @@ __device__ __forceinline__ GV make_gv(const QesGroup& G, uint64_t* s_rowlo) {
   v.has_phase = G.has_phase;
   v.n_up = G.n_up;
   v.z2 = G.z2;
+  v.has_bphase = G.has_bphase;
+  v.bmask = G.bmask;
+  v.bphase = G.bphase;
   v.mask = G.mask;
   uint32_t b = 0;
 #pragma unroll
@@ -140,6 +152,38 @@ __device__ __forceinline__ W apply_element(const GV& G, int g, W s) {
   W v = apply_base<W>(G, B, s, &dummy);
   v = rotl_rows<W>(v, a * G.cell_bits, G.row_bits, (W)G.rowlo[a], mask);
   return rotl_ns<W>(v, b * G.row_bits, G.ns, mask);
+}
+
+// Intrinsic phase of element g = (a * nty + b) * nbase + B acting on s: the
+// parity-y quarter turns of its base element, then the boundary phase of
+// every up spin that crosses during the a x steps and the b y steps
+// (symmetry_container_jit.py:63-118; the base element acts first).
+template <typename W>
+__device__ __forceinline__ void element_phase(const GV& G, int g, W s, double* pre, double* pim) {
+  const int B = g % G.nbase;
+  const int ab = g / G.nbase;
+  const int b = ab % G.nty, a = ab / G.nty;
+  int q = 0;
+  const W v = apply_base<W>(G, B, s, &q);
+  double re = 1.0, im = 0.0;
+  switch (q & 3) {
+    case 1: re = 0.0; im = 1.0; break;
+    case 2: re = -1.0; im = 0.0; break;
+    case 3: re = 0.0; im = -1.0; break;
+    default: break;
+  }
+  if (G.has_bphase) {
+    const int nx = popc<W>(v & (W)G.bmask[a]);
+    const int ny = popc<W>(v & (W)G.bmask[G.ntx + b]);
+    const double* px = G.bphase + 2 * nx;
+    const double* py = G.bphase + 2 * ((G.ns + 1) + ny);
+    const double tr = px[0] * py[0] - px[1] * py[1], ti = px[0] * py[1] + px[1] * py[0];
+    const double r2 = re * tr - im * ti, i2 = re * ti + im * tr;
+    re = r2;
+    im = i2;
+  }
+  *pre = re;
+  *pim = im;
 }
 
 // --------------------------------------------------------------------------
@@ -239,6 +283,16 @@ __device__ __forceinline__ bool rep_test(const GV& G, const double* __restrict__
               case 1: tr = -ci; ti = cr; break;
               case 2: tr = -cr; ti = -ci; break;
               default: tr = ci; ti = -cr; break;
+            }
+            if (G.has_bphase) {   // boundary phases of the crossing up spins
+              const int nx = popc<W>(vb[B] & (W)G.bmask[a]);
+              const int ny = popc<W>(vb[B] & (W)G.bmask[G.ntx + b]);
+              const double* px = G.bphase + 2 * nx;
+              const double* py = G.bphase + 2 * ((G.ns + 1) + ny);
+              const double br = px[0] * py[0] - px[1] * py[1], bi = px[0] * py[1] + px[1] * py[0];
+              const double ur = tr * br - ti * bi, ui = tr * bi + ti * br;
+              tr = ur;
+              ti = ui;
             }
             re += tr;
             im += ti;

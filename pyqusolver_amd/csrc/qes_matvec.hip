@@ -648,7 +648,8 @@ int projected_matvec(const Op* op, const Basis* out, const double* x, double* y,
       Gi.cell_bits != Go.cell_bits || memcmp(Gi.base_ops, Go.base_ops, sizeof(Gi.base_ops)) != 0)
     return fail("input and output sectors must share the group elements (same generators); only the sector "
                 "quantum numbers may differ");
-  if (Gi.has_phase || Go.has_phase) return fail("projected products with parity-y phases are not supported");
+  if (Gi.has_phase || Go.has_phase || Gi.has_bphase || Go.has_bphase)
+    return fail("projected products with intrinsic phases (parity y, twisted translations) are not supported");
   if (out->n == 0) return 0;
   const int nterms = op->n_diag + op->n_off;
   const dim3 grid((unsigned)((out->n + 255) / 256));

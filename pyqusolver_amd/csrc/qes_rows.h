@@ -74,13 +74,22 @@ __device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& mis
       miss();
       return;
     }
-    const double nb = NORM ? C.L.sqrtn[cn.count] : 1.0;
+    // with intrinsic (parity-y, boundary) phases N_b^2 is not |Stab|: read it
+    const bool phased = C.G.has_phase || C.G.has_bphase;
+    const double nb = NORM ? (phased ? C.L.norms[b] : C.L.sqrtn[cn.count]) : 1.0;
     const double cr = C.s_chr[cn.g];
     double fr, fi;
     if (CPLX) {
       const double ci = C.s_chi[cn.g];
       fr = (vr * cr + vi * ci) * nb;
       fi = (vr * ci - vi * cr) * nb;
+      if (phased) {   // times conj(phase of the minimising element on s')
+        double pr, pi;
+        element_phase<W>(C.G, cn.g, sp, &pr, &pi);
+        const double ur = fr * pr + fi * pi, ui = fi * pr - fr * pi;
+        fr = ur;
+        fi = ui;
+      }
     } else {
       fr = vr * cr * nb;
       fi = 0.0;
@@ -189,7 +198,15 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
     const double cr = C.s_chr[cn.g];
     if (CPLX) {
       const double ci = C.s_chi[cn.g];
-      add(b, vr * cr + vi * ci, vr * ci - vi * cr);
+      double fr = vr * cr + vi * ci, fi = vr * ci - vi * cr;
+      if (C.G.has_phase || C.G.has_bphase) {   // times conj(phase of the minimising element on s')
+        double pr, pi;
+        element_phase<W>(C.G, cn.g, sp, &pr, &pi);
+        const double ur = fr * pr + fi * pi, ui = fi * pr - fr * pi;
+        fr = ur;
+        fi = ui;
+      }
+      add(b, fr, fi);
     } else {
       add(b, vr * cr, 0.0);
     }
@@ -318,7 +335,16 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
           const double cr = C.s_chr[cn[k].g];
           if (CPLX) {
             const double ci = C.s_chi[cn[k].g];
-            const double fr = vr * cr + vi * ci, fi = vr * ci - vi * cr;
+            double fr = vr * cr + vi * ci, fi = vr * ci - vi * cr;
+            if (C.G.has_phase || C.G.has_bphase) {   // times conj(phase of the minimising element on s')
+              int p1 = pos[k] - d;
+              if (p1 < 0) p1 += ns;
+              double pr, pi;
+              element_phase<W>(C.G, cn[k].g, r ^ (((W)1 << pos[k]) | ((W)1 << p1)), &pr, &pi);
+              const double ur = fr * pr + fi * pi, ui = fi * pr - fr * pi;
+              fr = ur;
+              fi = ui;
+            }
             sr += fr * xr[k] - fi * xi[k];
             si += fr * xi[k] + fi * xr[k];
           } else {

@@ -113,11 +113,21 @@ class J1J2Model(Hamiltonian):
 
 
 class XXZ(Hamiltonian):
-    """H = -sum_<ij> [Jxy (SxSx+SySy) + Jz SzSz] - sum_i (hx Sx_i + hz Sz_i)."""
+    """H = -sum_<ij> [Jxy (SxSx+SySy) + Jz SzSz] - sum_i (hx Sx_i + hz Sz_i).
+
+    ``twist=True`` (no reference counterpart; the reference multiplies the
+    whole bond by the raw flux phase, xxz.py:393-431, which is Hermitian only
+    for phases +-1): a lattice flux enters as a twisted boundary,
+    -Jxy (SxSx + SySy) -> -2 Jxy (e^{i theta} S+_i S-_j + e^{-i theta} S-_i S+_j)
+    in the instruction VM's units (codes 31, 32; theta = w . phi of the
+    forward bond i -> j), which commutes with the twisted translations that
+    multiply exp(i phi) per up spin crossing the boundary
+    (Lattice.boundary_phases), so momentum sectors stay symmetry sectors."""
 
     COUPLINGS = ("jxy", "jz", "hx", "hz")
 
-    def __init__(self, lattice, hilbert_space=None, jxy=1.0, jz=1.0, delta=None, hx=0.0, hz=0.0, **kwargs):
+    def __init__(self, lattice, hilbert_space=None, jxy=1.0, jz=1.0, delta=None, hx=0.0, hz=0.0, twist: bool = False,
+                 **kwargs):
         if lattice is None:
             raise ValueError("XXZ requires a 'lattice' object to define neighbors.")
         super().__init__(hilbert_space, dtype=np.complex128, name="XXZ Model", lattice=lattice)
@@ -137,6 +147,16 @@ class XXZ(Hamiltonian):
             for nidx in range(lattice.get_nn_forward_num(i)):
                 j = lattice.get_nn_forward(i, num=nidx)
                 if lattice.wrong_nei(j):
+                    continue
+                if twist:
+                    ph = complex(lattice.boundary_phase_from_winding(*lattice.bond_winding(i, int(j))))
+                    if _nz(self._jxy[i]):
+                        self.add(spin_op("pm", "correlation"), -2.0 * self._jxy[i] * ph, True, [i, int(j)],
+                                 coupling=0)
+                        self.add(spin_op("mp", "correlation"), -2.0 * self._jxy[i] * np.conj(ph), True,
+                                 [i, int(j)], coupling=0)
+                    if _nz(self._jz[i]):
+                        self.add(szz, -self._jz[i], False, [i, int(j)], coupling=1)
                     continue
                 phase = _bond_phase(lattice, i, int(j), False)
                 cxy = self._jxy[i] * phase

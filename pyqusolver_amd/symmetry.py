@@ -138,6 +138,10 @@ class SectorGroup:
     g_to_pidx: np.ndarray = field(default=None)
     phase_table: np.ndarray = field(default=None)
     matvec_error: Optional[str] = None    # sector where the reference block is not Hermitian
+    # twisted translations: crossing masks (ntx + nty, bit = site) and the
+    # boundary phase table (2, ns + 1) of the x and y directions
+    cross_masks: Optional[np.ndarray] = None
+    bphase: Optional[np.ndarray] = None
 
     @property
     def order(self) -> int:
@@ -159,6 +163,13 @@ class SectorGroup:
         self._chi_im = np.ascontiguousarray(self.chi.imag, dtype=np.float64)
         s.chi_re = self._chi_re.ctypes.data_as(C.POINTER(C.c_double))
         s.chi_im = self._chi_im.ctypes.data_as(C.POINTER(C.c_double))
+        if self.cross_masks is not None:
+            self._cm = np.ascontiguousarray(self.cross_masks, dtype=np.uint64)
+            bp = np.empty((2, self.ns + 1, 2))
+            bp[..., 0], bp[..., 1] = self.bphase.real, self.bphase.imag
+            self._bp = np.ascontiguousarray(bp.reshape(-1))
+            s.cross_masks = self._cm.ctypes.data_as(C.POINTER(C.c_uint64))
+            s.bphase = self._bp.ctypes.data_as(C.POINTER(C.c_double))
         return s
 
 
@@ -192,6 +203,28 @@ def inversion_permutation(lattice, tol: float = 1e-7) -> np.ndarray:
     if len(np.unique(perm)) != ns:
         raise ValueError("inversion is not a bijection of the lattice sites")
     return perm
+
+
+def _twist_tables(lattice, ns: int, ntx: int, nty: int):
+    """Crossing masks and boundary phases of twisted translations.  Bit i is
+    site i for the group action (translation.py:50-89); a site crosses the
+    x boundary during a x steps when its Bravais x >= lx - a (translation.py:
+    316-437 marks x = lx - 1 for one step) and the y boundary during b y
+    steps when y >= ly - b.  Phases: Lattice.boundary_phases(), exp(i phi_d n)
+    for n crossing up spins (rows x and y)."""
+    nb = getattr(lattice, "n_basis", 1)
+    masks = np.zeros(ntx + nty, dtype=np.uint64)
+    for site in range(ns):
+        cell = site // nb
+        x, y = cell % lattice.lx, (cell // lattice.lx) % max(lattice.ly, 1)
+        for a in range(ntx):
+            if ntx > 1 and x >= lattice.lx - a:
+                masks[a] |= np.uint64(1) << np.uint64(site)
+        for b in range(nty):
+            if nty > 1 and y >= lattice.ly - b:
+                masks[ntx + b] |= np.uint64(1) << np.uint64(site)
+    ph = np.asarray(lattice.boundary_phases(), dtype=np.complex128)[:2]
+    return masks, ph
 
 
 def _is_real_momentum(k, L) -> bool:
@@ -231,13 +264,21 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
     nb = getattr(lattice, "n_basis", 1) if lattice is not None else 1
     trans = {g: s for g, s in gens if g in (SymmetryGenerators.Translation_x, SymmetryGenerators.Translation_y)}
     flux = getattr(lattice, "flux", None) or {}
+    twisted = []
     for g in trans:
         d = "x" if g == SymmetryGenerators.Translation_x else "y"
-        if abs(float(flux.get(d, 0.0))) > 0.0:
-            raise NotImplementedError(
-                f"translation along {d} with a boundary flux along {d}: the flux phase on the boundary bonds breaks "
-                "the translation invariance of the couplings; the twisted translation (boundary_phases table in the "
-                "group action) is not supported -- drop that translation")
+        phi = float(flux.get(d, 0.0))
+        if phi != 0.0:
+            # the twisted translation T' = (boundary phase per crossing up
+            # spin) T (symmetry_container_jit.py:91-98): T'^L multiplies by
+            # exp(i phi N_up), so the momenta exp(2 pi i k / L) label sectors
+            # only when phi N_up is a multiple of 2 pi
+            if n_up < 0 or abs(np.exp(1j * phi * n_up) - 1.0) > 1e-9:
+                raise NotImplementedError(
+                    f"translation along {d} with a boundary flux phi={phi} along {d} needs a U(1) filling N_up with "
+                    f"phi * N_up a multiple of 2 pi (got N_up={n_up}): otherwise the twisted translation's L-th "
+                    "power is not the identity and exp(2 pi i k / L) does not label its sectors")
+            twisted.append(d)
     others = [(g, s) for g, s in gens if g not in trans]
     spatial = [g for g, _ in others if g in (SymmetryGenerators.Reflection, SymmetryGenerators.Inversion)]
     matvec_error = None
@@ -249,10 +290,13 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
                 name = spatial[0].name.lower()
                 matvec_error = (f"{name} is only a symmetry of momentum sectors k=0 or k=L/2 (got k={k}, L={L}); "
                                 "the reference block is not Hermitian there -- use the translation sector alone")
-    if any(g == SymmetryGenerators.ParityY for g, _ in others):
-        matvec_error = ("parity-y sectors: the reference's +-i intrinsic phase enters the normalisation but not "
-                        "the matrix elements, so its block is not Hermitian -- use parity x")
     # base elements: identity + combinations of the other generators in order (:918-928)
+    if any(g == SymmetryGenerators.ParityY for g, _ in others):
+        # the reference's parity-y phase (+i for an even number of down spins,
+        # parity.py:55-63) does not square to one on every state, so the
+        # elements do not form a representation: no consistent block
+        matvec_error = ("parity-y sectors: the reference's +-i intrinsic phase (parity.py:55-63) does not make the "
+                        "group elements a representation, so the block is not Hermitian -- use parity x")
     base = [()]
     for r in range(1, len(others) + 1):
         for combo in itertools.combinations(range(len(others)), r):
@@ -305,6 +349,8 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
     grp = SectorGroup(ns=ns, n_up=n_up, z2=z2, generators=gens, nbase=len(base), base_ops=base_ops, ntx=ntx,
                       nty=nty, row_bits=row_bits, cell_bits=cell_bits, chi=chi)
     grp.matvec_error = matvec_error
+    if twisted:
+        grp.cross_masks, grp.bphase = _twist_tables(lattice, ns, ntx, nty)
     grp.g_to_pidx, grp.phase_table = phases_map(chi)
     return grp
 

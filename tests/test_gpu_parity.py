@@ -1074,3 +1074,63 @@ def test_every_instruction_code_against_the_oracle(L, gens, n_up):
     # the reduced operator is Hermitian (the pull form relies on it)
     z = rng.standard_normal(hs.nh) + 1j * rng.standard_normal(hs.nh)
     assert abs(np.vdot(z, y) - np.vdot(H.matvec_fun(z), x)) <= 1e-12 * np.linalg.norm(y) * np.linalg.norm(z)
+
+
+@pytest.mark.parametrize("kind,lx,ly,n_up,flux,axis", [("chain", 12, 1, 6, np.pi / 3, "x"),
+                                                      ("chain", 16, 1, 8, 3 * np.pi / 4, "x"),
+                                                      ("square", 4, 3, 6, np.pi / 3, "x"),
+                                                      ("square", 3, 4, 6, 2 * np.pi / 3, "y")])
+def test_twisted_translation_sectors_against_the_oracle(kind, lx, ly, n_up, flux, axis):
+    """Boundary flux phi along a translated direction: the twisted
+    translations (exp(i phi) per up spin crossing, per step;
+    symmetry_container_jit.py:91-98) give the basis of the oracle's orbit scan
+    with the same phases (representatives bit-exact, norms 1e-14); the
+    twisted-boundary XXZ model (XXZ(twist=True)) has the oracle's
+    instruction list, its products equal the oracle's push form with the
+    element phases, every block is Hermitian, and the lowest energies of all
+    momentum sectors contain the filling's ground state."""
+    from pyqusolver_amd import ChainLattice, SquareLattice, get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.models import XXZ
+    fl = {axis: flux}
+    lat = ChainLattice(lx, flux=flux) if kind == "chain" else SquareLattice(lx, ly, flux=fl)
+    olat = O.OLattice(kind, lx, ly)
+    fxy = (flux, 0.0) if axis == "x" else (0.0, flux)
+    bound = O.boundary_phase_table(olat.ns, fxy)
+    t = O.terms_xxz_twisted(olat, 1.0, 0.6, fxy)
+    c, s, co = t.arrays()
+    L = lx if axis == "x" else ly
+    gname = "tx" if axis == "x" else "ty"
+    e_min = []
+    for k in range(L):
+        sym = {"translation": {"kx": k} if axis == "x" else {"ky": k}} if kind != "chain" else {"translation": k}
+        hs = HilbertSpace(lattice=lat, sym_gen=sym, global_syms=[get_u1_sym(lat, n_up)])
+        grp = O.Group(olat, [O.Generator(gname, k)], n_up=n_up, bound_phase=bound)
+        b = O.scan_basis(grp)
+        np.testing.assert_array_equal(hs.representative_list, b.reps)
+        np.testing.assert_allclose(hs.normalization, b.norms, rtol=1e-14)
+        if hs.nh == 0:
+            continue
+        H = XXZ(lat, hilbert_space=hs, jxy=1.0, jz=0.6, twist=True)
+        sites_h, co_h, codes_h = H.instruction_arrays()
+        np.testing.assert_array_equal(codes_h, c)
+        np.testing.assert_array_equal(sites_h, s)
+        np.testing.assert_allclose(co_h, co, rtol=0, atol=1e-15)
+        rng = np.random.default_rng(k)
+        x = rng.standard_normal(hs.nh) + 1j * rng.standard_normal(hs.nh)
+        yo = O.push_matvec(b, olat.ns, c, s, co, True, x, group=grp)
+        y = H.matvec_fun(x)
+        assert np.linalg.norm(y - yo) <= MATVEC_TOL * np.linalg.norm(yo), k
+        z = rng.standard_normal(hs.nh) + 1j * rng.standard_normal(hs.nh)
+        assert abs(np.vdot(z, y) - np.vdot(H.matvec_fun(z), x)) <= 1e-12 * np.linalg.norm(y) * np.linalg.norm(z)
+        Hd = O.reduced_dense(b, olat.ns, c, s, co, True, group=grp)
+        e_ref = np.linalg.eigvalsh(Hd)[0]
+        e_min.append(e_ref)
+        if hs.nh > 20:
+            e0 = H.diagonalize(method="lanczos", k=1, tol=1e-12, maxiter=300)[0]
+            assert abs(e0 - e_ref) <= 1e-10 * max(1.0, abs(e_ref))
+    Hf = O.full_sparse_hamiltonian(olat.ns, c, s, co, True)
+    idx = np.array([i for i in range(1 << olat.ns) if bin(i).count("1") == n_up])
+    import scipy.sparse.linalg as spla
+    e_full = spla.eigsh(Hf[idx][:, idx], k=1, which="SA", tol=1e-13)[0][0]
+    assert abs(min(e_min) - e_full) <= 1e-10 * abs(e_full)

@@ -511,3 +511,57 @@ def test_hk_at_zero_kitaev_is_scaled_heisenberg():
     H2 = O.full_sparse_hamiltonian(lat.ns, c2, s2, co2, True)
     d = (H1 - H2).tocsr()
     assert H1.nnz > 0 and (np.abs(d.data).max() if d.nnz else 0.0) < 1e-13
+
+
+# -------------------- twisted translations and intrinsic phases in elements
+def _sector_checks(lat, gens, n_up, terms, bound, check_literal):
+    """For every momentum sector: the oracle's reduced matrix with element
+    phases equals U^dagger H U of the reference's basis vectors (rotation
+    matrix with phases, expand_to_full_state_jit) and is Hermitian; the
+    sectors' spectra together are the full spectrum of the filling."""
+    c, s, co = terms.arrays()
+    Hf = O.full_dense_hamiltonian(lat.ns, c, s, co, True)
+    fill = np.array([bin(i).count("1") == n_up for i in range(1 << lat.ns)])
+    E_full = np.linalg.eigvalsh(Hf[np.ix_(fill, fill)])
+    E = []
+    literal_nonherm = 0.0
+    for gl in gens:
+        grp = O.Group(lat, [O.Generator(k, v) for k, v in gl], n_up=n_up, bound_phase=bound)
+        b = O.scan_basis(grp)
+        if len(b.reps) == 0:
+            continue
+        U = O.rotation_matrix(grp, b)
+        np.testing.assert_allclose(U.conj().T @ U, np.eye(len(b.reps)), atol=1e-12)
+        Hr = O.reduced_dense(b, lat.ns, c, s, co, True, group=grp)
+        np.testing.assert_allclose(Hr, U.conj().T @ Hf @ U, atol=1e-12)
+        np.testing.assert_allclose(Hr, Hr.conj().T, atol=1e-12)
+        E += list(np.linalg.eigvalsh(Hr))
+        if check_literal:
+            Hl = O.reduced_dense(b, lat.ns, c, s, co, True)
+            literal_nonherm = max(literal_nonherm, np.abs(Hl - Hl.conj().T).max())
+    np.testing.assert_allclose(np.sort(E), E_full, atol=1e-11)
+    return literal_nonherm
+
+
+@pytest.mark.parametrize("kind,lx,ly,n_up,flux", [("chain", 8, 1, 4, (2 * np.pi / 4, 0.0)),
+                                                  ("chain", 9, 1, 3, (4 * np.pi / 3, 0.0)),
+                                                  ("square", 3, 2, 3, (2 * np.pi / 3, 0.0)),
+                                                  ("square", 2, 3, 3, (0.0, 2 * np.pi / 3))])
+def test_twisted_translations_decompose_the_twisted_model(kind, lx, ly, n_up, flux):
+    """Boundary phases in the group action (symmetry_container_jit.py:91-98:
+    exp(i phi) per up spin crossing, per translation step) make the twisted
+    translations commute with the twisted XXZ model; its momentum sectors
+    (phi N_up a multiple of 2 pi) are Hermitian blocks whose spectra add up
+    to the filling's.  Without the element phase (the reference's kernel
+    takes only the character) the blocks are not Hermitian."""
+    lat = O.OLattice(kind, lx, ly)
+    bound = O.boundary_phase_table(lat.ns, flux)
+    terms = O.terms_xxz_twisted(lat, 1.0, 0.6, flux)
+    if kind == "chain":
+        gens = [[("tx", k)] for k in range(lx)]
+    elif flux[1] == 0.0:
+        gens = [[("tx", k)] for k in range(lx)]
+    else:
+        gens = [[("ty", k)] for k in range(ly)]
+    literal = _sector_checks(lat, gens, n_up, terms, bound, check_literal=True)
+    assert literal > 1e-3
