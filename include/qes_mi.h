@@ -195,6 +195,13 @@ int qes_op_diagonal(qes_op_t op, void* re_dev, void* im_dev, void* stream);
 int qes_op_assemble(qes_op_t op, int32_t mode, double max_gb, void* stream);
 int qes_op_assembly_info(qes_op_t op, int64_t* entries, int64_t* bytes, int32_t* format, int32_t* n_values,
                          int32_t* tagged);
+/* Column panels of the assembled table (option "asm_panel_cols"): number of
+ * panels (0: single-vector products make one pass over the table), columns
+ * per panel, entries and bytes of the panel tables (beside the one-pass
+ * table, which the multi-vector, batched, coupling-dot and export paths use).
+ * No reference counterpart: the reference's products have no blocked layout
+ * (Algebra/Hilbert/matrix_builder.py:1347-1377 walks representatives once). */
+int qes_op_panel_info(qes_op_t op, int32_t* npanel, int64_t* cols, int64_t* entries, int64_t* bytes);
 /* With a coupling-tagged table (mode 3): y = sum_c scale[c] H_c x, one pass
  * over the table, for H(J) = sum_c (J_c / J0_c) H_c (the differentiable
  * forward; the reference builds a fresh operator per coupling set,
@@ -257,8 +264,13 @@ int qes_op_set_canonicalizer(qes_op_t op, int32_t kind);
  * "top_row_test" (0 default / 1: row lattices test the top row before a
  * direct lookup, single-lookup path only); "direct_batch" (-1 default, 1, 2,
  * 4: rank-map lookups a lane keeps in flight in a qes_matvec_ws product;
- * same sums in the same order for every value).  The other options change
- * the order of the sums (results agree to rounding). */
+ * same sums in the same order for every value); "asm_panel_cols" (-1
+ * default: automatic; 0 off; > 0 columns per panel): the next qes_op_assemble
+ * also splits the table into column panels for single-vector products (pass
+ * p gathers only from x[p*cols, (p+1)*cols), a window that stays in the
+ * cache; at most 32 panels; automatic: 128 MB windows where x exceeds 192 MB,
+ * kept only while table and panels fit the max_gb limit).  The other options
+ * change the order of the sums (results agree to rounding). */
 int qes_op_set_option(qes_op_t op, const char* name, int64_t value);
 int qes_op_get_plan(qes_op_t op, int32_t* canon, int32_t* n_classes, int32_t* n_generic);
 int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_host, int64_t n, int64_t* mismatches);
@@ -382,7 +394,13 @@ int qes_batch_matvec(qes_batch_t b, const void* x_dev, void* y_dev, void* stream
  * all steps; 0: canonicalise and look up on the fly like qes_matvec),
  * "op_tables" (1, default: ordinary problems whose operator was assembled with
  * qes_op_assemble before qes_batch_create stream that table; the batch keeps
- * pointers to it, so the assembly must outlive the batch), "check_min"
+ * pointers to it, so the assembly must outlive the batch), "scaled_kernel"
+ * (a batch of coupling-scaled blocks only -- qes_batch_create_scaled --: 1
+ * runs its products in a kernel of their own, 0 in the combined kernel; the
+ * same sums either way; default 1 for real, 0 for complex vectors),
+ * "scaled_lds" (bytes of LDS reserved per block of that kernel, which caps
+ * the resident blocks per CU; default 54272 for real vectors: two blocks),
+ * such batches default to one block per tile (grid_mult 2^20), "check_min"
  * (qes_batch_ground_states: 0 tests every check_every steps; n >= 1 places
  * each test where the residual decay of the active problems predicts the
  * first crossing, at least n and at most check_every steps apart; -1,

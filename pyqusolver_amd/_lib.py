@@ -67,6 +67,7 @@ SIGNATURES = {
     "qes_op_count_outside": [_P, _PI64],
     "qes_op_assemble": [_P, _I32, C.c_double, _P],
     "qes_op_assembly_info": [_P, _PI64, _PI64, _PI32, _PI32, _PI32],
+    "qes_op_panel_info": [_P, _PI32, _PI64, _PI64, _PI64],
     "qes_op_table_coo": [_P, _P, _P, _P, _I64, _PI64, _P],
     "qes_tridiag_ritz": [_PD, _PD, _I32, _PD, _PD, _PD],
     "qes_batch_create_scaled": [C.POINTER(_P), _I32, _I32, _PD, _I32, _I32, C.POINTER(_P)],

@@ -93,8 +93,15 @@ class BatchedLanczos:
                 if h.assembly_bytes:
                     continue
                 try:
+                    # the batch streams the one-pass table; column panels
+                    # serve single-vector products only (restored by close())
+                    h.set_option("asm_panel_cols", 0)
+                except _lib.QesError:   # in use by another batch or handle: its table cannot change
+                    continue
+                try:
                     nb = h.assemble(True, max_gb=budget / 1e9)
                 except _lib.QesError:
+                    h.set_option("asm_panel_cols", -1)
                     continue
                 budget -= nb
                 self._assembled_here.append(h)
@@ -312,6 +319,7 @@ class BatchedLanczos:
         for h in getattr(self, "_assembled_here", []):
             try:
                 h.assemble(False)
+                h.set_option("asm_panel_cols", -1)
             except Exception:
                 pass
         self._assembled_here = []
@@ -364,6 +372,9 @@ class CouplingSweep(BatchedLanczos):
                 raise ValueError(f"sector {p} has {nc} coupling ids, scales give {sc.shape[2]}")
             packed[p, :, :nc] = sc[p, :, :nc]
             if not h.assembly["couplings"]:
+                # the sweep streams the one-pass table; column panels serve
+                # single-vector products only (restored by close())
+                h.set_option("asm_panel_cols", 0)
                 h.assemble(True, max_gb=max_gb, couplings=True)
                 self._assembled_here.append(h)
         self.scales = packed

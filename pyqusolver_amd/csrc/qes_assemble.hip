@@ -237,20 +237,22 @@ __global__ void __launch_bounds__(256) asm_rows_kernel(QesGroup G, QesLookup L, 
 template <bool XC, int FMT, bool CV>
 __global__ void __launch_bounds__(256) sell_matvec_kernel(AsmTable T, const double* __restrict__ x,
                                                           double* __restrict__ y, int64_t n, int accumulate) {
-  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
-  double ar, ai;
-  sell_row_t<FMT, XC, CV>(T, a, x, ar, ai);
-  if (XC) {
-    if (accumulate) {
-      y[2 * a] += ar;
-      y[2 * a + 1] += ai;
+  // grid-stride over rows (one pass when the grid covers n)
+  for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < n; a += (int64_t)gridDim.x * blockDim.x) {
+    if (T.nodiag && T.len[a / SELL_C] == 0) continue;   // a later panel with nothing in this slice
+    double ar, ai;
+    sell_row_t<FMT, XC, CV>(T, a, x, ar, ai);
+    if (XC) {
+      if (accumulate) {
+        y[2 * a] += ar;
+        y[2 * a + 1] += ai;
+      } else {
+        y[2 * a] = ar;
+        y[2 * a + 1] = ai;
+      }
     } else {
-      y[2 * a] = ar;
-      y[2 * a + 1] = ai;
+      if (accumulate) y[a] += ar; else y[a] = ar;
     }
-  } else {
-    if (accumulate) y[a] += ar; else y[a] = ar;
   }
 }
 
@@ -469,7 +471,10 @@ __global__ void dots_finish_kernel(const double* __restrict__ part, int nblk, in
   out[t] = s;
 }
 
+void free_panels(Op* op);
+
 void free_assembly(Op* op) {
+  free_panels(op);
   for (void* p : {(void*)op->asm_off, (void*)op->asm_len, (void*)op->asm_col, (void*)op->asm_val,
                   (void*)op->asm_packed, (void*)op->asm_vidx, (void*)op->asm_dict, (void*)op->asm_diag,
                   (void*)op->asm_diag_c, (void*)op->asm_dict_cpl, (void*)op->asm_dict_s, (void*)op->asm_part})
@@ -703,6 +708,264 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
   return 0;
 }
 
+// ---- column panels ---------------------------------------------------------
+// The 6x6 J1-J2 sector's x vector (1 GB) is four times the MALL, and every
+// gathered x element of the one-pass product is a random HBM line.  The
+// panelled table splits each row's entries by column range: pass p of the
+// product streams panel p's slices and gathers only from x[p*pw, (p+1)*pw),
+// a window that stays cache resident, and adds into y.  Entries keep their
+// slot order inside a panel; zero entries (the padding of the one-pass
+// table) are dropped and each panel pads its own slices with (column p*pw,
+// value 0).
+
+static constexpr int PANEL_MAX = 32;
+
+struct PanelSrc {
+  AsmTable T;          // the one-pass table
+  int64_t n, ns, pw;
+  int npanel;
+  uint32_t zidx;       // dictionary index of the value 0 (dictionary formats)
+};
+
+// entry e of the one-pass table: column, and whether it carries a nonzero value
+__device__ __forceinline__ bool panel_src_entry(const PanelSrc& S, int64_t e, int64_t& b, uint32_t& w0, uint32_t& w1,
+                                                double& v0, double& v1) {
+  const AsmTable& T = S.T;
+  if (T.fmt == ASM_P4) {
+    w0 = T.packed[e];
+    const uint32_t cmask = (T.colbits >= 32) ? ~0u : ((1u << T.colbits) - 1u);
+    b = w0 & cmask;
+    return (w0 >> T.colbits) != S.zidx;
+  }
+  b = T.col[e];
+  if (T.fmt == ASM_P6) {
+    w1 = T.vidx[e];
+    return w1 != S.zidx;
+  }
+  const int vw = T.vc ? 2 : 1;
+  v0 = T.val[e * vw];
+  v1 = T.vc ? T.val[e * vw + 1] : 0.0;
+  return v0 != 0.0 || v1 != 0.0;
+}
+
+// slice lengths of every panel: entries per row counted in LDS, the maximum
+// over the slice's 64 rows (one wavefront) written as plen[p][s]
+__global__ void __launch_bounds__(256) panel_count_kernel(PanelSrc S, int32_t* __restrict__ plen) {
+  __shared__ int32_t cnt[PANEL_MAX * 256];
+  const int tid = threadIdx.x;
+  for (int p = 0; p < S.npanel; ++p) cnt[p * 256 + tid] = 0;
+  const int64_t a = (int64_t)blockIdx.x * 256 + tid;
+  const int64_t s = a / SELL_C;
+  if (a < S.n) {
+    const int64_t base = S.T.off[s] + a % SELL_C;
+    const int len = S.T.len[s];
+    for (int k = 0; k < len; ++k) {
+      int64_t b;
+      uint32_t w0, w1;
+      double v0, v1;
+      if (panel_src_entry(S, base + (int64_t)k * SELL_C, b, w0, w1, v0, v1)) ++cnt[(int)(b / S.pw) * 256 + tid];
+    }
+  }
+  for (int p = 0; p < S.npanel; ++p) {
+    int v = cnt[p * 256 + tid];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    if ((tid & (SELL_C - 1)) == 0 && s < S.ns) plen[p * S.ns + s] = v;
+  }
+}
+
+struct PanelDst {
+  const int64_t* poff;   // [p][ns+1]
+  const int32_t* plen;   // [p][ns]
+  int32_t* col;          // PLAIN / P6
+  double* val;           // PLAIN
+  uint32_t* packed;      // P4
+  uint16_t* vidx;        // P6
+};
+
+__global__ void __launch_bounds__(256) panel_fill_kernel(PanelSrc S, PanelDst D) {
+  __shared__ int32_t cnt[PANEL_MAX * 256];
+  const int tid = threadIdx.x;
+  for (int p = 0; p < S.npanel; ++p) cnt[p * 256 + tid] = 0;
+  const int64_t a = (int64_t)blockIdx.x * 256 + tid;
+  if (a >= S.n) return;   // no wavefront-wide operation follows
+  const int64_t s = a / SELL_C;
+  const int l = (int)(a % SELL_C);
+  const AsmTable& T = S.T;
+  const int vw = T.vc ? 2 : 1;
+  const int64_t base = T.off[s] + l;
+  const int len = T.len[s];
+  for (int k = 0; k < len; ++k) {
+    int64_t b;
+    uint32_t w0 = 0, w1 = 0;
+    double v0 = 0.0, v1 = 0.0;
+    if (!panel_src_entry(S, base + (int64_t)k * SELL_C, b, w0, w1, v0, v1)) continue;
+    const int p = (int)(b / S.pw);
+    const int kp = cnt[p * 256 + tid]++;
+    const int64_t e = D.poff[p * (S.ns + 1) + s] + (int64_t)kp * SELL_C + l;
+    if (T.fmt == ASM_P4) {
+      D.packed[e] = w0;
+    } else {
+      D.col[e] = (int32_t)b;
+      if (T.fmt == ASM_P6) {
+        D.vidx[e] = (uint16_t)w1;
+      } else {
+        D.val[e * vw] = v0;
+        if (T.vc) D.val[e * vw + 1] = v1;
+      }
+    }
+  }
+  for (int p = 0; p < S.npanel; ++p) {
+    const int64_t b = (int64_t)p * S.pw;   // padding gathers x[p*pw], inside the panel's window
+    const int64_t o = D.poff[p * (S.ns + 1) + s];
+    for (int kp = cnt[p * 256 + tid]; kp < D.plen[p * S.ns + s]; ++kp) {
+      const int64_t e = o + (int64_t)kp * SELL_C + l;
+      if (T.fmt == ASM_P4) {
+        D.packed[e] = (uint32_t)b | (S.zidx << T.colbits);
+      } else {
+        D.col[e] = (int32_t)b;
+        if (T.fmt == ASM_P6) {
+          D.vidx[e] = (uint16_t)S.zidx;
+        } else {
+          D.val[e * vw] = 0.0;
+          if (T.vc) D.val[e * vw + 1] = 0.0;
+        }
+      }
+    }
+  }
+}
+
+void free_panels(Op* op) {
+  for (void* p : {(void*)op->asm_poff, (void*)op->asm_plen, (void*)op->asm_pcol, (void*)op->asm_pval,
+                  (void*)op->asm_ppacked, (void*)op->asm_pvidx})
+    if (p) (void)hipFree(p);
+  op->asm_poff = nullptr;
+  op->asm_plen = nullptr;
+  op->asm_pcol = nullptr;
+  op->asm_pval = nullptr;
+  op->asm_ppacked = nullptr;
+  op->asm_pvidx = nullptr;
+  op->asm_npanel = 0;
+  op->asm_pw = 0;
+  op->asm_pentries = 0;
+  op->asm_pbytes = 0;
+}
+
+// columns per panel: the option, or automatically a 128 MB window of complex
+// or real x where x itself exceeds 192 MB (panels cost one more read and
+// write of y per pass, which smaller vectors do not win back).  Measured on
+// MI355X, 6x6 J1-J2 k=(0,0) (x = 1 GB, one-pass product 61.1 ms): windows of
+// 96 / 128 / 160 / 192 MB gave 44.7 / 44.7 / 46.5 / 47.8 ms; the panel
+// tables hold 15.3 / 14.4 / 13.5 / 13.1 G slots against 10.1 G (more
+// slices padded)
+static int64_t panel_cols(const Op* op) {
+  const int64_t n = op->basis->n;
+  if (op->asm_panel_cols >= 0) return op->asm_panel_cols >= n ? 0 : op->asm_panel_cols;
+  const int64_t xs = (op->asm_vc || op->asm_dvc) ? 16 : 8;
+  if (n * xs <= (int64_t)192 << 20) return 0;
+  return ((int64_t)128 << 20) / xs;
+}
+
+static int build_panels(Op* op, hipStream_t st) {
+  free_panels(op);
+  const int64_t pw = panel_cols(op);
+  if (pw <= 0) return 0;
+  const int64_t n = op->basis->n, ns = op->asm_slices;
+  const int64_t np = (n + pw - 1) / pw;
+  if (np < 2) return 0;
+  if (np > PANEL_MAX) return fail("asm_panel_cols gives " + std::to_string(np) + " panels (at most " +
+                                  std::to_string(PANEL_MAX) + ")");
+  PanelSrc S{};
+  S.T = asm_table_of(op);
+  S.n = n;
+  S.ns = ns;
+  S.pw = pw;
+  S.npanel = (int)np;
+  S.zidx = 0;
+  if (op->asm_fmt != ASM_PLAIN) {
+    const int dw = op->asm_dvc ? 2 : 1;
+    bool found = false;
+    for (size_t i = 0; i < op->asm_h_dict_cpl.size() && !found; ++i) {
+      if (op->asm_h_dict_cpl[i] == 0 && op->asm_h_dict[i * dw] == 0.0 && (dw == 1 || op->asm_h_dict[i * dw + 1] == 0.0)) {
+        S.zidx = (uint32_t)i;
+        found = true;
+      }
+    }
+    if (!found) return fail("assembled dictionary holds no zero value");
+  }
+  const dim3 grid((unsigned)((n + 255) / 256));
+  int32_t* d_plen = nullptr;
+  QES_HIP(hipMalloc(&d_plen, sizeof(int32_t) * np * ns));
+  op->asm_plen = d_plen;
+  hipLaunchKernelGGL(panel_count_kernel, grid, dim3(256), 0, st, S, d_plen);
+  QES_HIP(hipGetLastError());
+  std::vector<int32_t> plen(np * ns);
+  QES_HIP(hipMemcpyAsync(plen.data(), d_plen, sizeof(int32_t) * np * ns, hipMemcpyDeviceToHost, st));
+  QES_HIP(hipStreamSynchronize(st));
+  std::vector<int64_t> poff(np * (ns + 1));
+  int64_t acc = 0;
+  for (int64_t p = 0; p < np; ++p) {
+    for (int64_t s = 0; s < ns; ++s) {
+      poff[p * (ns + 1) + s] = acc;
+      acc += (int64_t)plen[p * ns + s] * SELL_C;
+    }
+    poff[p * (ns + 1) + ns] = acc;
+  }
+  const int cw = op->asm_vc ? 2 : 1;
+  const double per_entry = op->asm_fmt == ASM_PLAIN ? 4.0 + 8.0 * cw : (op->asm_fmt == ASM_P4 ? 4.0 : 6.0);
+  const double bytes = (double)acc * per_entry + 12.0 * np * ns;
+  if (op->asm_bytes + bytes > op->asm_max_bytes) {   // the one-pass table serves alone
+    free_panels(op);
+    return 0;
+  }
+  QES_HIP(hipMalloc(&op->asm_poff, sizeof(int64_t) * poff.size()));
+  QES_HIP(hipMemcpy(op->asm_poff, poff.data(), sizeof(int64_t) * poff.size(), hipMemcpyHostToDevice));
+  const size_t ne = (size_t)std::max<int64_t>(acc, 1);
+  PanelDst D{};
+  D.poff = op->asm_poff;
+  D.plen = op->asm_plen;
+  if (op->asm_fmt == ASM_P4) {
+    QES_HIP(hipMalloc(&op->asm_ppacked, sizeof(uint32_t) * ne));
+    QES_HIP(hipMemsetAsync(op->asm_ppacked, 0, sizeof(uint32_t) * ne, st));
+  } else {
+    QES_HIP(hipMalloc(&op->asm_pcol, sizeof(int32_t) * ne));
+    QES_HIP(hipMemsetAsync(op->asm_pcol, 0, sizeof(int32_t) * ne, st));
+    if (op->asm_fmt == ASM_P6) {
+      QES_HIP(hipMalloc(&op->asm_pvidx, sizeof(uint16_t) * ne));
+      QES_HIP(hipMemsetAsync(op->asm_pvidx, 0, sizeof(uint16_t) * ne, st));
+    } else {
+      QES_HIP(hipMalloc(&op->asm_pval, sizeof(double) * cw * ne));
+      QES_HIP(hipMemsetAsync(op->asm_pval, 0, sizeof(double) * cw * ne, st));
+    }
+  }
+  D.col = op->asm_pcol;
+  D.val = op->asm_pval;
+  D.packed = op->asm_ppacked;
+  D.vidx = op->asm_pvidx;
+  hipLaunchKernelGGL(panel_fill_kernel, grid, dim3(256), 0, st, S, D);
+  QES_HIP(hipGetLastError());
+  QES_HIP(hipStreamSynchronize(st));
+  op->asm_npanel = (int)np;
+  op->asm_pw = pw;
+  op->asm_pentries = acc;
+  op->asm_pbytes = (int64_t)bytes;
+  return 0;
+}
+
+// table of panel p (arrays of the panels, the dictionary and diagonals of T0)
+static AsmTable panel_table(const Op* op, const AsmTable& T0, int p) {
+  AsmTable T = T0;
+  const int64_t ns = op->asm_slices;
+  T.off = op->asm_poff + (int64_t)p * (ns + 1);
+  T.len = op->asm_plen + (int64_t)p * ns;
+  T.col = op->asm_pcol;
+  T.val = op->asm_pval;
+  T.packed = op->asm_ppacked;
+  T.vidx = op->asm_pvidx;
+  T.nodiag = p > 0 ? 1 : 0;
+  return T;
+}
+
 int op_assemble(Op* op, int enable, int want_dict, hipStream_t st) {
   free_assembly(op);
   if (!enable || op->basis->n == 0) return 0;
@@ -718,6 +981,7 @@ int op_assemble(Op* op, int enable, int want_dict, hipStream_t st) {
     free_assembly(op);
     rc = run(0);
   }
+  if (!rc) rc = build_panels(op, st);
   if (rc) free_assembly(op);
   return rc;
 }
@@ -772,12 +1036,25 @@ int assembled_coo(const Op* op, int32_t* rows, int32_t* cols, double* vals, int6
   return 0;
 }
 
+// blocks of the single-vector product: one per 256 rows, or (sell_grid > 0)
+// sell_grid blocks per CU striding over the rows
+static dim3 sell_grid_of(int sell_grid, int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (sell_grid > 0) {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    g = std::min<int64_t>(g, (int64_t)ncu * sell_grid);
+  }
+  return dim3((unsigned)std::max<int64_t>(g, 1));
+}
+
 template <bool XC>
-static void launch_sell_x(const AsmTable& T, const double* x, double* y, int64_t n, int accumulate, hipStream_t st) {
-  const dim3 grid((unsigned)((n + 255) / 256));
+static void launch_sell_x(const AsmTable& T, const double* x, double* y, int64_t n, int accumulate, hipStream_t st,
+                          int sell_grid, int sell_lds) {
+  const dim3 grid = sell_grid_of(sell_grid, n);
   const bool cv = T.fmt == ASM_PLAIN ? T.vc != 0 : T.dvc != 0;
 #define QES_SELL(F, C) \
-  hipLaunchKernelGGL((sell_matvec_kernel<XC, F, C>), grid, dim3(256), 0, st, T, x, y, n, accumulate)
+  hipLaunchKernelGGL((sell_matvec_kernel<XC, F, C>), grid, dim3(256), sell_lds, st, T, x, y, n, accumulate)
   if (T.fmt == ASM_P4) {
     if (cv) QES_SELL(ASM_P4, true); else QES_SELL(ASM_P4, false);
   } else if (T.fmt == ASM_P6) {
@@ -789,10 +1066,22 @@ static void launch_sell_x(const AsmTable& T, const double* x, double* y, int64_t
 }
 
 static int launch_sell(const AsmTable& T, const double* x, double* y, int64_t n, bool xc, int accumulate,
-                       hipStream_t st) {
-  if (xc) launch_sell_x<true>(T, x, y, n, accumulate, st);
-  else launch_sell_x<false>(T, x, y, n, accumulate, st);
+                       hipStream_t st, int sell_grid = 0, int sell_lds = 0) {
+  if (xc) launch_sell_x<true>(T, x, y, n, accumulate, st, sell_grid, sell_lds);
+  else launch_sell_x<false>(T, x, y, n, accumulate, st, sell_grid, sell_lds);
   QES_HIP(hipGetLastError());
+  return 0;
+}
+
+// single-vector product over the column panels when the operator has them
+// (pass 0 adds the diagonal and honours accumulate, later passes add), else
+// one pass over the table
+static int launch_sell_panels(const Op* op, const AsmTable& T0, const double* x, double* y, int64_t n, bool xc,
+                              int accumulate, hipStream_t st) {
+  if (op->asm_npanel <= 0) return launch_sell(T0, x, y, n, xc, accumulate, st, op->sell_grid, op->sell_lds);
+  for (int p = 0; p < op->asm_npanel; ++p)
+    if (launch_sell(panel_table(op, T0, p), x, y, n, xc, p > 0 ? 1 : accumulate, st, op->sell_grid, op->sell_lds))
+      return -1;
   return 0;
 }
 
@@ -801,7 +1090,7 @@ int assembled_matvec(const Op* op, const double* x, double* y, int dtype, int ac
   if (n == 0) return 0;
   const bool xc = dtype == 1;
   if ((op->asm_vc || op->asm_dvc) && !xc) return fail("complex matrix elements need complex128 vectors");
-  return launch_sell(asm_table_of(op), x, y, n, xc, accumulate, st);
+  return launch_sell_panels(op, asm_table_of(op), x, y, n, xc, accumulate, st);
 }
 
 template <bool XC, int FMT, bool CV>
@@ -876,7 +1165,7 @@ int assembled_matvec_scaled(Op* op, const double* scale, int nscale, const doubl
   T.dict = op->asm_dict_s;
   T.ncpl = op->n_coupling;
   for (int c = 0; c < op->n_coupling; ++c) T.dscale[c] = scale[c];
-  return launch_sell(T, x, y, n, xc, 0, st);
+  return launch_sell_panels(op, T, x, y, n, xc, 0, st);
 }
 
 int assembled_coupling_dots(Op* op, const double* v, const double* g, int dtype, double* out_host, hipStream_t st) {

@@ -279,35 +279,35 @@ __device__ __forceinline__ void block_row_sell(const QesProb& P, int64_t a, unsi
   const int cw = CPLX ? 2 : 1;
   const double* xp0 = x + P.off * cw;
   double* yp0 = y + P.off * cw;
-  double dcr[ASM_MAX_CPL], dci[ASM_MAX_CPL];
+  // diagonal of column r: sum over c of cscale[r][c] diag_c[a], summed in c
+  // order (coupling outer, column inner keeps RC sums live instead of the
+  // eight diagonals)
+  double dr[RC], di[RC];
+#pragma unroll
+  for (int r = 0; r < RC; ++r) dr[r] = di[r] = 0.0;
 #pragma unroll
   for (int c = 0; c < ASM_MAX_CPL; ++c) {
-    dcr[c] = dci[c] = 0.0;
     if (c < T.ncpl) {
       const double* dc = T.diag_c + ((int64_t)c * T.nrows + a) * VW;
-      dcr[c] = stream_load(dc);
-      if (CV) dci[c] = stream_load(dc + 1);
+      const double dcr = stream_load(dc), dci = CV ? stream_load(dc + 1) : 0.0;
+#pragma unroll
+      for (int r = 0; r < RC; ++r) {
+        const double s = P.cscale[r * ASM_MAX_CPL + c];
+        dr[r] += s * dcr;
+        if (CV) di[r] += s * dci;
+      }
     }
   }
   double ar[RC], ai[RC];
 #pragma unroll
   for (int r = 0; r < RC; ++r) {
-    double dr = 0.0, di = 0.0;
-#pragma unroll
-    for (int c = 0; c < ASM_MAX_CPL; ++c) {
-      if (c < T.ncpl) {
-        const double s = P.cscale[r * ASM_MAX_CPL + c];
-        dr += s * dcr[c];
-        if (CV) di += s * dci[c];
-      }
-    }
     const int64_t e = a * RC + r;
     if (CPLX) {
       const double xr = xp0[e * 2], xi = xp0[e * 2 + 1];
-      ar[r] = dr * xr - di * xi;
-      ai[r] = dr * xi + di * xr;
+      ar[r] = dr[r] * xr - di[r] * xi;
+      ai[r] = dr[r] * xi + di[r] * xr;
     } else {
-      ar[r] = dr * xp0[e];
+      ar[r] = dr[r] * xp0[e];
       ai[r] = 0.0;
     }
   }
@@ -459,6 +459,30 @@ __global__ void __launch_bounds__(BRB) batch_matvec_kernel(const QesProb* __rest
         }
       }
     }
+  }
+}
+
+// Batches whose problems are all coupling-scaled blocks (coupling sweeps):
+// the same tile loop as batch_matvec_kernel with only the table-streaming
+// branch, so the kernel is not sized by the matrix-free row visitor's
+// registers and LDS (the combined kernel held 103-133 VGPRs and spilled
+// SGPRs; this one keeps more waves resident for the gathers)
+template <bool CPLX, int FMT, bool CV, int RC>
+__global__ void __launch_bounds__(BRB) batch_scaled_kernel(const QesProb* __restrict__ probs,
+                                                           const Tile* __restrict__ tiles, int ntiles,
+                                                           const int* __restrict__ prob_active,
+                                                           const int* __restrict__ vp_active,
+                                                           const double* __restrict__ x, double* __restrict__ y) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile T = tiles[t];
+    if (prob_active && !prob_active[T.prob]) continue;   // uniform per block
+    const QesProb& P = probs[T.prob];
+    unsigned cm = 0;
+#pragma unroll
+    for (int r = 0; r < RC; ++r)
+      if (!vp_active || vp_active[P.vp0 + r]) cm |= 1u << r;
+    const int64_t a = T.row0 + threadIdx.x;
+    if (a < P.n && a < T.row0 + T.rows) block_row_sell<FMT, CPLX, CV, RC>(P, a, cm, x, y);
   }
 }
 
@@ -773,6 +797,10 @@ struct Batch {
   int block_rows = 16;    // rows per tile of a block problem (at least one per lane group)
   int block_table = 1;    // block problems use their neighbour table
   bool use_tables = true; // ordinary problems stream their operator's assembled table when it has one
+  int scaled_sig = -1;     // every problem a coupling-scaled block of one table format, value type and
+                           // width: fmt | cv << 2 | R << 3; else -1
+  int scaled_kernel = 1;   // all_scaled batches: 1 their own kernel, 0 the combined one
+  int scaled_lds = 0;      // dynamic LDS bytes per block of that kernel (caps resident blocks per CU)
   std::vector<QesProb> h_probs;
   std::vector<int64_t*> t_rowptr;   // per real problem (null for ordinary problems)
   std::vector<int32_t*> t_col;
@@ -823,7 +851,7 @@ static int batch_build_tiles(Batch* B) {
   int dev = 0, ncu = 256;
   QES_HIP(hipGetDevice(&dev));
   QES_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  B->grid_matvec = std::max(1, std::min(B->ntiles, ncu * B->grid_mult));
+  B->grid_matvec = (int)std::max<int64_t>(1, std::min<int64_t>(B->ntiles, (int64_t)ncu * B->grid_mult));
   return 0;
 }
 
@@ -1070,6 +1098,23 @@ int batch_create(Op* const* ops, int nreg, const BlockSpec* blocks, int nblock, 
   QES_BALLOC(B->d_j, sizeof(int) * B->nvp);
 #undef QES_BALLOC
   B->h_probs = h;
+  for (size_t i = 0; i < h.size(); ++i) {
+    const QesProb& q = h[i];
+    const int sig = q.cscale && q.nrhs <= 4 ? (q.T.fmt | (q.T.dvc ? 4 : 0) | (q.nrhs << 3)) : -2;
+    B->scaled_sig = (i == 0 || sig == B->scaled_sig) ? sig : -2;
+  }
+  if (B->scaled_sig < 0) B->scaled_sig = -1;
+  if (B->scaled_sig >= 0) {
+    // coupling sweeps, measured on MI355X (6x6 J1-J2, three columns, one
+    // tagged table): a block per tile beats the persistent grid (real
+    // 111 -> 96 ms, complex 173 -> 145 ms per product); real sweeps then run
+    // their own kernel at two resident blocks per CU (87 ms: fewer gathers
+    // in flight thrash the caches less), complex ones stay in the combined
+    // kernel (145 ms against 172 ms in their own)
+    B->grid_mult = 1 << 20;
+    B->scaled_kernel = dtype == 1 ? 0 : 1;
+    B->scaled_lds = dtype == 1 ? 0 : 54272;
+  }
   if (!rc) {
     B->h_active.assign(B->nvp, 1);
     std::vector<int> pa(B->nprob, 1);
@@ -1117,7 +1162,31 @@ int batch_destroy(Batch* B) {
 static int enqueue_bmatvec(Batch* B, const double* x, double* y, const int* pactive, const int* vactive,
                            hipStream_t st) {
   const bool c = B->dtype == 1;
-  if (B->w32) {
+  if (B->scaled_sig >= 0 && B->scaled_kernel) {
+    const size_t lds = B->scaled_lds;
+    // every problem: coupling-scaled block of the same format, value type and width
+    const int fmt = B->scaled_sig & 3, cv = (B->scaled_sig >> 2) & 1, R = B->scaled_sig >> 3;
+#define QES_SK(XC, F, CVV, RR)                                                                                     \
+  hipLaunchKernelGGL((batch_scaled_kernel<XC, F, CVV, RR>), dim3(B->grid_matvec), dim3(BRB), lds, st, B->d_probs, \
+                     B->d_tiles, B->ntiles, pactive, vactive, x, y)
+#define QES_SK_R(XC, F, CVV)              \
+  switch (R) {                            \
+    case 1: QES_SK(XC, F, CVV, 1); break; \
+    case 2: QES_SK(XC, F, CVV, 2); break; \
+    case 3: QES_SK(XC, F, CVV, 3); break; \
+    default: QES_SK(XC, F, CVV, 4); break; \
+  }
+#define QES_SK_F(XC)                                                \
+  if (fmt == ASM_P4) {                                              \
+    if (cv) { QES_SK_R(XC, ASM_P4, true) } else { QES_SK_R(XC, ASM_P4, false) } \
+  } else {                                                          \
+    if (cv) { QES_SK_R(XC, ASM_P6, true) } else { QES_SK_R(XC, ASM_P6, false) } \
+  }
+    if (c) { QES_SK_F(true) } else { QES_SK_F(false) }
+#undef QES_SK_F
+#undef QES_SK_R
+#undef QES_SK
+  } else if (B->w32) {
     if (c) hipLaunchKernelGGL((batch_matvec_kernel<uint32_t, true>), dim3(B->grid_matvec), dim3(BRB), B->lds, st,
                                B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y);
     else hipLaunchKernelGGL((batch_matvec_kernel<uint32_t, false>), dim3(B->grid_matvec), dim3(BRB), B->lds, st,
@@ -1248,11 +1317,15 @@ int batch_set_option(Batch* B, const std::string& name, int64_t value) {
     return 0;
   }
   if (name == "tile_order") B->tile_order = (int)value;
-  else if (name == "grid_mult") B->grid_mult = (int)std::max<int64_t>(1, value);
+  else if (name == "grid_mult") B->grid_mult = (int)std::max<int64_t>(1, std::min<int64_t>(value, 1 << 20));
   else if (name == "block_rows") B->block_rows = (int)std::max<int64_t>(1, std::min<int64_t>(value, 1 << 20));
   else if (name == "block_table") B->block_table = value ? 1 : 0;
   else if (name == "op_tables") B->use_tables = value != 0;
-  else return fail("unknown batch option " + name + " (tile_order, grid_mult, block_rows, block_table, op_tables, check_min)");
+  else if (name == "scaled_kernel") B->scaled_kernel = value ? 1 : 0;
+  else if (name == "scaled_lds") B->scaled_lds = (int)std::max<int64_t>(0, std::min<int64_t>(value, 65536));
+  else return fail("unknown batch option " + name +
+                   " (tile_order, grid_mult, block_rows, block_table, op_tables, scaled_kernel, scaled_lds, "
+                   "check_min)");
   batch_drop_graph(B);
   if (batch_upload_probs(B)) return -1;
   return batch_build_tiles(B);

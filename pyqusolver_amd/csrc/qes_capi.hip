@@ -444,7 +444,20 @@ int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
   else if (n == "direct_batch") {
     if (value != -1 && value != 1 && value != 2 && value != 4) return fail("direct_batch must be -1 (auto), 1, 2 or 4");
     op.direct_batch = (int)value;
-  } else return fail("unknown option " + n + " (bond_order_uniform, direct_lookup, xcd_remap, top_row_test, direct_batch)");
+  } else if (n == "sell_grid") {
+    if (value < 0 || value > 64) return fail("sell_grid must be 0 (a block per 256 rows) or 1..64 blocks per CU");
+    op.sell_grid = (int)value;
+    return 0;   // launch shape only: the plan and the table stay
+  } else if (n == "sell_lds") {
+    if (value < 0 || value > 65536) return fail("sell_lds must be 0..65536 bytes");
+    op.sell_lds = (int)value;
+    return 0;
+  } else if (n == "asm_panel_cols") {
+    if (value < -1) return fail("asm_panel_cols must be -1 (auto), 0 (off) or a column count");
+    op.asm_panel_cols = value;
+  } else
+    return fail("unknown option " + n +
+                " (bond_order_uniform, direct_lookup, xcd_remap, top_row_test, direct_batch, asm_panel_cols, sell_grid)");
   return build_plan(&op, op.plan.canon == pick_canon_public(op.basis->group) ? -1 : op.plan.canon);
 }
 
@@ -498,6 +511,16 @@ int qes_op_assembly_info(qes_op_t h, int64_t* entries, int64_t* bytes, int32_t* 
   if (format) *format = op.asm_fmt;
   if (n_values) *n_values = op.asm_ndict;
   if (tagged) *tagged = on ? op.asm_tagged : 0;
+  return 0;
+}
+
+int qes_op_panel_info(qes_op_t h, int32_t* npanel, int64_t* cols, int64_t* entries, int64_t* bytes) {
+  if (!h) return fail("null operator");
+  const Op& op = h->op;
+  if (npanel) *npanel = op.asm_npanel;
+  if (cols) *cols = op.asm_pw;
+  if (entries) *entries = op.asm_pentries;
+  if (bytes) *bytes = op.asm_pbytes;
   return 0;
 }
 

@@ -133,6 +133,22 @@ struct Op {
   double* asm_part = nullptr;        // per-block partial sums of assembled_coupling_dots
   int64_t asm_entries = 0, asm_slices = 0, asm_bytes = 0;
   int asm_vc = 0, asm_dvc = 0, asm_fmt = -1, asm_colbits = 32, asm_ndict = 0, asm_tagged = 0;
+  // column panels of the table (single-vector products): panel p holds the
+  // entries whose column lies in [p*pw, (p+1)*pw) in its own sliced layout,
+  // so one pass gathers from an x window that stays in the MALL.  Arrays of
+  // all panels are concatenated: slice offsets [p][ns+1] (absolute), lengths
+  // [p][ns].  asm_panel_cols: -1 automatic, 0 off, > 0 columns per panel
+  int64_t asm_panel_cols = -1;
+  int sell_grid = 0;   // single-vector table product: 0 one block per 256 rows, k > 0 k blocks per CU (grid stride)
+  int sell_lds = 0;    // dynamic LDS bytes per block of that product (caps resident blocks per CU)
+  int asm_npanel = 0;
+  int64_t asm_pw = 0, asm_pentries = 0, asm_pbytes = 0;
+  int64_t* asm_poff = nullptr;
+  int32_t* asm_plen = nullptr;
+  int32_t* asm_pcol = nullptr;
+  double* asm_pval = nullptr;
+  uint32_t* asm_ppacked = nullptr;
+  uint16_t* asm_pvidx = nullptr;
   double asm_max_bytes = 64e9;
   std::vector<double> asm_h_dict;          // host copy of the dictionary
   std::vector<uint8_t> asm_h_dict_cpl;

@@ -36,7 +36,7 @@ struct AsmTable {
   int32_t dvc;          // P4 / P6: complex dictionary values and diagonal
   int32_t colbits;      // P4 column field width
   int32_t ncpl;         // > 0: diagonal = sum_c dscale[c] * diag_c[c] (per-coupling diagonals)
-  int32_t pad;
+  int32_t nodiag;       // column panel after the first: no diagonal term, rows of empty slices untouched
   int64_t nrows;        // row count (stride of diag_c)
   const int64_t* off;
   const int32_t* len;
@@ -80,7 +80,7 @@ __device__ __forceinline__ void sell_row_t(const AsmTable& T, int64_t a, const d
   const int len = T.len[s];
   ar = 0.0;
   ai = 0.0;
-  if (FMT != ASM_PLAIN) {
+  if (FMT != ASM_PLAIN && !T.nodiag) {
     double dr, di = 0.0;
     if (T.ncpl > 0) {
       dr = 0.0;
@@ -168,7 +168,7 @@ inline AsmTable asm_table_of(const Op* op) {
   T.dvc = op->asm_dvc;
   T.colbits = op->asm_colbits;
   T.ncpl = 0;
-  T.pad = 0;
+  T.nodiag = 0;
   T.nrows = op->basis->n;
   T.off = op->asm_off;
   T.len = op->asm_len;

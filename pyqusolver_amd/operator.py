@@ -206,7 +206,8 @@ class Hamiltonian:
                    "qes_op_set_canonicalizer")
 
     def set_option(self, name: str, value: int):
-        """Kernel tuning knob (bond_order_uniform, direct_lookup, xcd_remap, top_row_test, direct_batch)."""
+        """Kernel tuning knob (bond_order_uniform, direct_lookup, xcd_remap, top_row_test, direct_batch,
+        asm_panel_cols: columns per panel of the next assembled table, -1 automatic, 0 off)."""
         _lib.check(_lib.lib().qes_op_set_option(self._device_op(), name.encode(), int(value)), "qes_op_set_option")
 
     @property
@@ -237,8 +238,13 @@ class Hamiltonian:
         _lib.check(_lib.lib().qes_op_assembly_info(self._device_op(), C.byref(e), C.byref(b), C.byref(f),
                                                    C.byref(nv), C.byref(tg)))
         names = {-1: None, 0: "plain", 1: "packed32", 2: "index16"}
+        np_, pc, pe, pb = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64()
+        _lib.check(_lib.lib().qes_op_panel_info(self._device_op(), C.byref(np_), C.byref(pc), C.byref(pe),
+                                                C.byref(pb)))
         return {"entries": int(e.value), "bytes": int(b.value), "format": names[int(f.value)],
-                "n_values": int(nv.value), "couplings": bool(tg.value)}
+                "n_values": int(nv.value), "couplings": bool(tg.value),
+                "panels": int(np_.value), "panel_cols": int(pc.value), "panel_entries": int(pe.value),
+                "panel_bytes": int(pb.value)}
 
     @property
     def assembly_bytes(self) -> int:
@@ -570,8 +576,9 @@ class Hamiltonian:
         raise.
 
         ``assemble``: tabulate H for the solve (see ``assemble``); None picks
-        it when the table fits in a quarter of the free device memory (at
-        most 64 GB) and N >= 65536.  A table built here is freed afterwards."""
+        it when the table (with its column panels, where x exceeds the
+        cache) fits in 40 % of the free device memory (at most 128 GB) and
+        N >= 65536.  A table built here is freed afterwards."""
         from .lanczos import block_lanczos, lanczos_ground_state, which_code
         if method not in ("lanczos", "exact", "auto", "block_lanczos"):
             raise NotImplementedError(f"method={method!r}: 'exact', 'lanczos', 'block_lanczos' and 'auto' run on "
@@ -591,7 +598,7 @@ class Hamiltonian:
         if self.assembly_bytes == 0 and (assemble or (assemble is None and self.nh >= 65536)):
             free, _ = torch.cuda.mem_get_info()
             try:
-                self.assemble(True, max_gb=min(64.0, 0.25 * free / 1e9))
+                self.assemble(True, max_gb=min(128.0, 0.4 * free / 1e9))
                 built = True
             except _lib.QesError:
                 if assemble:

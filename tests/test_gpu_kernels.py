@@ -204,6 +204,8 @@ def test_assembled_multi_rhs_equals_single_columns(case, fmt):
         H = HeisenbergKitaev(lat, K=-1.0, J=0.2, hilbert_space=hs, hx=0.03, hy=0.02, hz=0.05)
     else:
         H = J1J2Model(lat, J1=-1.0, J2=-0.4, hilbert_space=hs)
+    # one-pass table: the single-vector product then sums in the multi-vector kernel's order
+    H.set_option("asm_panel_cols", 0)
     H.assemble(True, dictionary=(fmt == "auto"))
     dt = torch.complex128 if H.needs_complex else torch.float64
     g = torch.Generator(device="cuda").manual_seed(8)
@@ -216,6 +218,66 @@ def test_assembled_multi_rhs_equals_single_columns(case, fmt):
         H.matvec_device(X, out=Y2, accumulate=True)
         assert torch.allclose(Y2, 2 * Y, rtol=1e-14, atol=1e-14 * Y.abs().max().item())
     H.assemble(False)
+
+
+@pytest.mark.parametrize("case", [SPACES[2], SPACES[8], SPACES[11], SPACES[12]],
+                         ids=lambda c: f"{c[0]}{c[1]}x{c[2]}-{c[3]}")
+@pytest.mark.parametrize("fmt", ["auto", "plain", "couplings"])
+def test_column_panels_match_one_pass_table(case, fmt):
+    """Column panels (forced narrow, so that small sectors take five passes):
+    the panelled single-vector product equals the one-pass table's to
+    rounding (each row's sum is split by column range), accumulate adds,
+    the scaled product of a coupling-tagged table agrees, a Lanczos solve
+    through the panels gives the same ground state, and more than 32 panels
+    are refused."""
+    import torch
+    from pyqusolver_amd import _lib
+    from pyqusolver_amd.lanczos import lanczos_ground_state
+    from pyqusolver_amd.models import HeisenbergKitaev, J1J2Model
+    kind, lx, ly, sym, n_up, ck = case
+    lat, hs = _space(kind, lx, ly, sym, n_up)
+    if kind == "honeycomb":
+        H = HeisenbergKitaev(lat, K=-1.0, J=0.2, hilbert_space=hs, hx=0.03, hy=0.02, hz=0.05, split_couplings=True)
+    else:
+        H = J1J2Model(lat, J1=-1.0, J2=-0.4, hilbert_space=hs)
+    dt = torch.complex128 if H.needs_complex else torch.float64
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn(hs.nh, dtype=dt, device="cuda", generator=g)
+    kw = {"dictionary": fmt != "plain", "couplings": fmt == "couplings"}
+    nc = H.n_couplings
+    scales = np.linspace(0.6, 1.4, nc)
+    H.set_option("asm_panel_cols", 0)
+    H.assemble(True, **kw)
+    assert H.assembly["panels"] == 0
+    y1 = H.matvec_device(x)
+    ys1 = H.matvec_scaled_device(scales, x) if fmt == "couplings" else None
+    e1 = lanczos_ground_state(H, k=1, tol=1e-11, maxiter=200, seed=2).eigenvalues[0]
+    H.assemble(False)
+    cols = hs.nh // 5 + 1
+    H.set_option("asm_panel_cols", cols)
+    H.assemble(True, **kw)
+    info = H.assembly
+    assert info["panels"] == 5 and info["panel_cols"] == cols and info["panel_entries"] > 0, info
+    y2 = H.matvec_device(x)
+    tol = 1e-14 * y1.abs().max().item()
+    assert torch.allclose(y2, y1, rtol=1e-14, atol=tol)
+    assert torch.equal(H.matvec_device(x), y2)   # deterministic
+    y3 = y2.clone()
+    H.matvec_device(x, out=y3, accumulate=True)
+    assert torch.allclose(y3, 2 * y1, rtol=1e-14, atol=2 * tol)
+    if fmt == "couplings":
+        ys2 = H.matvec_scaled_device(scales, x)
+        assert torch.allclose(ys2, ys1, rtol=1e-14, atol=1e-14 * ys1.abs().max().item())
+    e2 = lanczos_ground_state(H, k=1, tol=1e-11, maxiter=200, seed=2).eigenvalues[0]
+    assert abs(e2 - e1) <= 1e-10 * abs(e1)
+    H.assemble(False)
+    assert H.assembly["panels"] == 0
+    assert hs.nh >= 40
+    H.set_option("asm_panel_cols", hs.nh // 40)
+    with pytest.raises(_lib.QesError, match="panels"):
+        H.assemble(True, **kw)
+    assert H.assembly_bytes == 0
+    H.set_option("asm_panel_cols", -1)
 
 
 def test_count_outside_sector():
