@@ -240,3 +240,47 @@ def test_coupling_tagged_table_autograd_matches_matrix_free(kind):
     np.testing.assert_allclose(Jg.grad.numpy(), g_hf.numpy(), rtol=2e-6, atol=1e-8)
     assert abs(float((g_hf * Jg.detach()).sum()) - e_t.item()) <= 1e-9 * abs(e_t.item())
     ham.assemble(False)
+
+
+def test_config_gradients_at_full_size():
+    """The differentiable workload at its full size: Kitaev-Heisenberg
+    honeycomb 3x4 cells (24 sites, N = 1,398,500, complex128), K=-1, J=0.2,
+    h=0.05 along [111].  dE0/d(K,J,h) through the Lanczos recurrence against
+    Hellmann-Feynman and central differences (step 1e-5); the Hellmann-
+    Feynman sum rule sum_c J_c dE0/dJ_c = E0 (H is linear in J); the
+    gradient of |<psi0|exp(-iHt)|phi>|^2 (t=1, 30-step Krylov) against
+    central differences."""
+    import torch
+    from pyqusolver_amd.autograd import (central_difference, hellmann_feynman, krylov_overlap, lanczos_energy)
+    from pyqusolver_amd.lanczos import lanczos_ground_state, start_vector
+    lat, hs, ham = _honeycomb(3, 4)
+    assert hs.nh == 1_398_500
+    H = _dh(ham)
+    J0 = H.J0.clone()
+    res = lanczos_ground_state(ham, tol=1e-13, maxiter=300, seed=0, reorth=True, compute_eigvec=True)
+    e0 = float(res.eigenvalues[0])
+    psi0 = res.eigenvectors / torch.linalg.vector_norm(res.eigenvectors)
+    g_hf = hellmann_feynman(H, psi0)
+    assert abs(float((g_hf * J0).sum()) - e0) <= 1e-9 * abs(e0)
+    v0 = start_vector(hs.nh, True, seed=0)
+    m = min(int(res.steps) + 40, 300)
+    J = J0.clone().requires_grad_(True)
+    e = lanczos_energy(H, J, v0, m)
+    e.backward()
+    g_ad = J.grad.clone()
+    assert abs(e.item() - e0) <= 1e-10 * abs(e0)
+    g_fd = central_difference(lambda Jp: lanczos_energy(H, Jp, v0, m), J0, h=1e-5)
+    np.testing.assert_allclose(g_ad.numpy(), g_hf.numpy(), rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(g_ad.numpy(), g_fd.numpy(), rtol=1e-6, atol=1e-9)
+    g = torch.Generator(device="cuda").manual_seed(1234)
+    phi = torch.randn(hs.nh, dtype=torch.complex128, device="cuda", generator=g)
+    phi = phi / torch.linalg.vector_norm(phi) + 0.5 * psi0
+    J.grad = None
+    f = krylov_overlap(H, J, psi0, phi, t=1.0, m=30)
+    f.backward()
+    g_k = J.grad.clone()
+    g_kfd = central_difference(lambda Jp: krylov_overlap(H, Jp, psi0, phi, t=1.0, m=30), J0, h=1e-5)
+    # psi0 is an eigenvector, so f is flat to first order and its gradient
+    # small (~1e-4); the objective itself carries ~1e-13 of rounding at this
+    # N, which central differences at h = 1e-5 turn into ~1e-8 absolute
+    np.testing.assert_allclose(g_k.numpy(), g_kfd.numpy(), rtol=1e-6, atol=1e-8)
