@@ -10,7 +10,10 @@ operator's product (same H.v through a table), a Lanczos ground-state solve
 (three-term recurrence: one stored vector is 1 GB) to the 1e-10 residual
 test with its time-to-E0, the CPU baseline (the oracle's C restatement of
 the reference push-form kernel on column windows strided across the basis),
-and the J1-J2 chain L=32 (k=0, flip+, N = 9,393,119) as a secondary field.
+the J1-J2 chain L=32 (k=0, flip+, N = 9,393,119) as a secondary field, the
+batched workload (XXZ L=24 momentum sectors + 256 random-field L=20
+realisations: batched against sector by sector) and a one-sector 6x6
+coupling sweep (k=(0,0) at J2 in {0.4, 0.5, 0.6} against three solves).
 
 Other workloads: --config chain32 (the L=32 chain alone, Lanczos with full
 reorthogonalisation against the exact Majumdar-Ghosh E0 = -3L/8), --config
@@ -59,6 +62,10 @@ def parse():
                    help="columns timed on one thread for the CPU baseline, in windows strided across the basis "
                         "(0: skip)")
     p.add_argument("--no-secondary", action="store_true", help="square6x6: skip the L=32 chain field")
+    p.add_argument("--no-batched-field", action="store_true",
+                   help="square6x6: skip the batched field (XXZ L=24 sectors + L=20 realisations)")
+    p.add_argument("--no-sweep-field", action="store_true",
+                   help="square6x6: skip the coupling-sweep field (6x6 k=(0,0) at three J2 values)")
     p.add_argument("--no-ritz", action="store_true", help="skip the Ritz-vector residual check (three-term runs)")
     p.add_argument("--quiet", action="store_true")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -86,6 +93,10 @@ def parse():
 
 
 def bench_batched(a):
+    print(json.dumps(run_batched(a)), flush=True)
+
+
+def run_batched(a):
     """XXZ chain L=24 (Jxy=1, Jz=0.7, Sz=0): all 24 momenta, reflection parity
     at k=0 and k=12 (26 sectors), plus random-field XXZ L=20 Sz=0 realisations
     with h_i ~ U[-5, 5] (seed 1000+r), all ground states by batched Lanczos
@@ -147,7 +158,10 @@ def bench_batched(a):
         "max_rel_dev_E0_k_vs_minus_k": pair_dev,
         "stopping": "per problem: Ritz residual |beta_m s_m| <= 1e-10 max(1,|E0|), checked every 10 steps",
     }
-    print(json.dumps(out), flush=True)
+    del hams, spaces
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_grad(a):
@@ -234,6 +248,10 @@ def bench_grad(a):
 
 
 def bench_sweep(a):
+    print(json.dumps(run_sweep(a)), flush=True)
+
+
+def run_sweep(a):
     """J1-J2 6x6 square, Sz=0, spin flip +1, momentum sectors (kx, ky) in
     row-major order, each at J2 in {0.4, 0.5, 0.6}: one coupling sweep per
     sector (its coupling-tagged table built once, streamed once per step for
@@ -312,7 +330,9 @@ def bench_sweep(a):
         "stopping": "per (sector, J2): Ritz residual |beta_m s_m| <= 1e-10 max(1,|E0|), tested at most 10 steps "
                     "apart, where the residual's decay predicts the crossing",
     }
-    print(json.dumps(out), flush=True)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_block(a):
@@ -750,6 +770,25 @@ def bench_plain(a):
                             "ms_per_matvec": sec["ms_per_step"], "roofline": sec["roofline"],
                             "assembled": sec["assembled"], "host_buffer_ms_per_matvec": sec["host_buffer_ms_per_matvec"],
                             "lanczos": sec["lanczos"]}
+    if a.config == "square6x6" and not a.no_batched_field:
+        # the batched half of the metric: XXZ L=24 sectors + 256 random-field
+        # L=20 realisations, batched against sector by sector
+        torch.cuda.empty_cache()
+        b = run_batched(a)
+        out["batched"] = {k: b[k] for k in ("config", "time_to_e0_batched_s", "time_to_e0_sector_by_sector_s",
+                                            "batch_creation_s", "speedup", "speedup_including_creation",
+                                            "residual_bound_max", "rel_error_bound_max",
+                                            "max_rel_dev_batched_vs_sequential", "max_rel_dev_E0_k_vs_minus_k")}
+        out["batched"]["matvec_per_s"] = b["value"]
+    if a.config == "square6x6" and not a.no_sweep_field:
+        # the 6x6 coupling sweep on its first momentum sector, k = (0,0), at
+        # J2 in {0.4, 0.5, 0.6}, against three separate solves of that sector
+        torch.cuda.empty_cache()
+        a.sector_start, a.sectors, a.baseline_sectors = 0, 1, 1
+        w = run_sweep(a)
+        out["sweep"] = {"workload": "J1-J2 square 6x6 k=(0,0) flip+ at J2 in {0.4, 0.5, 0.6}, one coupling sweep",
+                        "sectors": w["sectors"], "sequential_baseline": w["sequential_baseline"],
+                        "seconds_tables": w["seconds_tables"]}
     print(json.dumps(out), flush=True)
 
 
