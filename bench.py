@@ -42,10 +42,10 @@ HBM_PEAK_GBS = 8000.0
 # current kernels (profiles/matvec_kernel_square6x6_j1j2.md,
 # profiles/matvec_kernel_chain32_j1j2.md)
 TRAFFIC = {
-    # matvec_kernel_wide<false,2,1,true,4>: 301,958,511 KiB x 2.0 + 984,817 KiB x 1.0
-    "square6x6": 301958511 * 1024 * 2 + 984817 * 1024,
-    # matvec_kernel<false,1,1,true,2>: 3,896,884 KiB x 2.0 + 73,384 KiB x 1.0
-    "chain32": 3896884 * 1024 * 2 + 73384 * 1024,
+    # matvec_kernel_wide<false,2,1,true,4>: 309,592,613 KiB x 2.0 + 984,817 KiB x 1.0
+    "square6x6": 309592613 * 1024 * 2 + 984817 * 1024,
+    # matvec_kernel<false,1,1,true,2>: 4,067,051 KiB x 2.0 + 73,384 KiB x 1.0
+    "chain32": 4067051 * 1024 * 2 + 73384 * 1024,
 }
 
 def parse():
