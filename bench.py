@@ -42,7 +42,9 @@ HBM_PEAK_GBS = 8000.0
 # current kernels (profiles/matvec_kernel_square6x6_j1j2.md,
 # profiles/matvec_kernel_chain32_j1j2.md)
 TRAFFIC = {
-    # matvec_kernel_wide<false,2,1,true,4>: 309,592,613 KiB x 2.0 + 984,817 KiB x 1.0
+    # matvec_kernel_wide<false,2,1,true,4,true> (rank gather): 224,735,084 KiB x 2.0 + 984,817 KiB x 1.0
+    "square6x6_rank_gather": 224735084 * 1024 * 2 + 984817 * 1024,
+    # matvec_kernel_wide<false,2,1,true,4,false>: 309,592,613 KiB x 2.0 + 984,817 KiB x 1.0
     "square6x6": 309592613 * 1024 * 2 + 984817 * 1024,
     # matvec_kernel<false,1,1,true,2>: 4,067,051 KiB x 2.0 + 73,384 KiB x 1.0
     "chain32": 4067051 * 1024 * 2 + 73384 * 1024,
@@ -62,6 +64,8 @@ def parse():
                    help="columns timed on one thread for the CPU baseline, in windows strided across the basis "
                         "(0: skip)")
     p.add_argument("--no-secondary", action="store_true", help="square6x6: skip the L=32 chain field")
+    p.add_argument("--no-rank-gather", action="store_true",
+                   help="time only the default dense gathers (no rank-gather variant for the value)")
     p.add_argument("--no-batched-field", action="store_true",
                    help="square6x6: skip the batched field (XXZ L=24 sectors + L=20 realisations)")
     p.add_argument("--no-sweep-field", action="store_true",
@@ -566,10 +570,42 @@ def measure_sector(a, config, steps, warmup, primary=True):
     total_ms = start.elapsed_time(stop)
     step_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
     ms_per_step = total_ms / steps
-    # the dominant kernel alone (the product is a short prescale pass, then
-    # the row kernel): its launch duration from HIP events on the same stream
     import ctypes as C
     from pyqusolver_amd import _lib
+    # the rank-gather variant (the library option rank_gather: gathers from a
+    # vector indexed by the colex rank of every state of the filling, built
+    # once per basis); used for the value where it builds, the default gathers
+    # reported beside it
+    default_gather = {"ms_per_step": ms_per_step, "matvec_per_s": 1000.0 / ms_per_step}
+    rank_gather = None
+    if primary and not a.no_rank_gather:
+        try:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            H.set_option("rank_gather", 1)
+            torch.cuda.synchronize()
+            t_rg = time.perf_counter() - t0
+        except _lib.QesError as ex:   # no rank map, or the vector does not fit: the default gathers stay
+            rank_gather = {"error": str(ex)}
+        else:
+            for i in range(warmup):
+                H.matvec_device(xs[i % nx], out=y)
+            torch.cuda.synchronize()
+            start.record(stream)
+            for i in range(steps):
+                evs[i][0].record(stream)
+                H.matvec_device(xs[i % nx], out=y)
+                evs[i][1].record(stream)
+            stop.record(stream)
+            torch.cuda.synchronize()
+            ms_per_step = start.elapsed_time(stop) / steps
+            step_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
+            import math
+            words = -(-math.comb(wl["ns"], wl["n_up"]) // 64)
+            rank_gather = {"ms_per_step": ms_per_step, "build_s": round(t_rg, 4),
+                           "vector_bytes": words * 64 * 8}
+    # the dominant kernel alone (the product is a short prescale pass, then
+    # the row kernel): its launch duration from HIP events on the same stream
     kev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     pev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     ws = torch.empty_like(y)
@@ -582,6 +618,9 @@ def measure_sector(a, config, steps, warmup, primary=True):
         pev[i][1].record(stream)
     torch.cuda.synchronize()
     del ws
+    rg_on = rank_gather is not None and "error" not in rank_gather
+    if rg_on:
+        H.set_option("rank_gather", 0)   # frees the basis's vector before the table is built
     b_alg = n * (8 + 8 + 8 + 8) + e_off * 8
     prod_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in pev]))
     log(f"matvec {ms_per_step:.3f} ms/step (events {step_ms:.3f} ms, product {prod_ms:.3f} ms), B_alg "
@@ -724,12 +763,18 @@ def measure_sector(a, config, steps, warmup, primary=True):
         "host_buffer_ms_per_matvec": host_ms,
         "assembled": assembled,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(config),
-                     "kernel": "matvec_kernel_wide" if wl["ns"] > 32 else "matvec_kernel",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC.get(config + ("_rank_gather" if rg_on else "")),
+                     "kernel": ("matvec_kernel_wide" if wl["ns"] > 32 else "matvec_kernel") +
+                               (" (rank gather)" if rg_on else ""),
                      "kernel_ms": prod_ms, "bytes_alg": b_alg,
                      "bytes_alg_formula": "N*(8 rep + 8 norm + 8 x_a + 8 y_a) + E_off*8 x gathers",
-                     "launches": "prescale_kernel (N*24 B, ~0.4% of the time) + the row kernel; kernel_ms is "
-                                 "the product's HIP-event time on its stream"},
+                     "launches": ("rank_scatter_kernel (N*24 B read, N 8-byte stores at the representatives' ranks)"
+                                  if rg_on else "prescale_kernel (N*24 B, ~0.4% of the time)") +
+                                 " + the row kernel; kernel_ms is the product's HIP-event time on its stream"},
+        "matvec_variant": "rank gather (option rank_gather=1)" if rg_on else "dense pre-scaled gathers (default)",
+        "default_gather": default_gather,
+        "rank_gather": rank_gather,
         "cpu_baseline": cpu,
         "lanczos": lanczos,
     }
@@ -756,6 +801,9 @@ def bench_plain(a):
         "data": "synthetic",
         "config": r["config"],
         "roofline": r["roofline"],
+        "matvec_variant": r["matvec_variant"],
+        "default_gather": r["default_gather"],
+        "rank_gather": r["rank_gather"],
         "cpu_baseline": r["cpu_baseline"],
         "assembled": r["assembled"],
         "lanczos": r["lanczos"],

@@ -263,14 +263,23 @@ int qes_op_set_canonicalizer(qes_op_t op, int32_t kind);
  * off, -1 on where a rank map serves; "xcd_remap" (0/1);
  * "top_row_test" (0 default / 1: row lattices test the top row before a
  * direct lookup, single-lookup path only); "direct_batch" (-1 default, 1, 2,
- * 4: rank-map lookups a lane keeps in flight in a qes_matvec_ws product;
- * same sums in the same order for every value); "asm_panel_cols" (-1
+ * 4, 8: rank-map lookups a lane keeps in flight in a qes_matvec_ws product,
+ * 8 for rank-gather products of more than 32 sites only; same sums in the
+ * same order for every value); "asm_panel_cols" (-1
  * default: automatic; 0 off; > 0 columns per panel): the next qes_op_assemble
  * also splits the table into column panels for single-vector products (pass
  * p gathers only from x[p*cols, (p+1)*cols), a window that stays in the
  * cache; at most 32 panels; automatic: 128 MB windows where x exceeds 192 MB,
- * kept only while table and panels fit the max_gb limit).  The other options
- * change the order of the sums (results agree to rounding). */
+ * kept only while table and panels fit the max_gb limit); "rank_gather" (0
+ * default, 1): qes_matvec_ws products of one column gather N_b x_b from a
+ * vector indexed by the colex rank of each state of the U(1) filling (the
+ * empty states hold a NaN marker), one load per looked-up state instead of
+ * a rank-map word and then an x element; needs the rank map; the vector
+ * belongs to the basis (8 or 16 bytes per state of the filling: 72.6 GB at
+ * 6x6 J1-J2 Sz=0 real, built when the option is set, freed by setting 0),
+ * so products that use it on one basis must not overlap in time; same sums
+ * in the same order as without it (6x6: 113.6 -> 105.8 ms).  The other
+ * options change the order of the sums (results agree to rounding). */
 int qes_op_set_option(qes_op_t op, const char* name, int64_t value);
 int qes_op_get_plan(qes_op_t op, int32_t* canon, int32_t* n_classes, int32_t* n_generic);
 int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_host, int64_t n, int64_t* mismatches);

@@ -495,6 +495,49 @@ int build_rank_map(Basis* B, hipStream_t st, int* eligible, double max_bytes) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Rank-indexed gather vector: one slot of cw doubles per state of the U(1)
+// filling, the empty marker everywhere until a product writes N_b x_b at the
+// ranks of the representatives (rank_scatter_kernel).
+// ---------------------------------------------------------------------------
+__global__ void rxs_fill_kernel(uint64_t* __restrict__ p, int64_t words) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = QES_RXS_EMPTY;
+}
+
+void free_rank_gather(Basis* B) {
+  if (B->rxs) (void)hipFree(B->rxs);
+  B->rxs = nullptr;
+  B->rxs_states = 0;
+  B->rxs_cw = 0;
+}
+
+int build_rank_gather(Basis* B, int cw, hipStream_t st) {
+  if (!B->rmap) return fail("rank gather needs the basis's rank map (a U(1) filling and ns <= 48)");
+  if (B->rxs && B->rxs_cw == cw) return 0;
+  free_rank_gather(B);
+  const int64_t states = B->rmap_words * 64;   // covers every colex rank of the filling
+  const size_t bytes = (size_t)states * cw * sizeof(double);
+  size_t free_b = 0, total_b = 0;
+  QES_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (bytes > free_b / 2)
+    return fail("rank gather vector would take " + std::to_string(bytes / 1e9) + " GB (" +
+                std::to_string(free_b / 1e9) + " GB free; at most half is used)");
+  QES_HIP(hipMalloc(&B->rxs, bytes));
+  const int64_t words = states * cw;
+  hipLaunchKernelGGL(rxs_fill_kernel, dim3((unsigned)std::min<int64_t>((words + 255) / 256, 256 * 64)), dim3(256), 0,
+                     st, (uint64_t*)B->rxs, words);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    free_rank_gather(B);
+    return check_hip(e, "build_rank_gather");
+  }
+  B->rxs_states = states;
+  B->rxs_cw = cw;
+  return 0;
+}
+
 template <typename W>
 __global__ void map_states_kernel(QesGroup G, QesLookup L, const uint64_t* __restrict__ states, int64_t n,
                                   int64_t* __restrict__ idx, int32_t* __restrict__ gmin, int32_t* __restrict__ cnt) {

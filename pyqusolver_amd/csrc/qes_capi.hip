@@ -351,6 +351,7 @@ int qes_basis_destroy(qes_basis_t h) {
   if (B.hash) (void)hipFree(B.hash);
   if (B.keyc) (void)hipFree(B.keyc);
   if (B.rmap) (void)hipFree(B.rmap);
+  free_rank_gather(&B);
   if (B.rbase) (void)hipFree(B.rbase);
   if (B.rlo) (void)hipFree(B.rlo);
   if (B.sqrtn) (void)hipFree(B.sqrtn);
@@ -442,7 +443,8 @@ int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
   else if (n == "xcd_remap") op.xcd_remap = value ? 1 : 0;
   else if (n == "top_row_test") op.top_row_test = value ? 1 : 0;
   else if (n == "direct_batch") {
-    if (value != -1 && value != 1 && value != 2 && value != 4) return fail("direct_batch must be -1 (auto), 1, 2 or 4");
+    if (value != -1 && value != 1 && value != 2 && value != 4 && value != 8)
+      return fail("direct_batch must be -1 (auto), 1, 2, 4 or 8 (8: rank-gather products of 64-bit states)");
     op.direct_batch = (int)value;
   } else if (n == "sell_grid") {
     if (value < 0 || value > 64) return fail("sell_grid must be 0 (a block per 256 rows) or 1..64 blocks per CU");
@@ -452,12 +454,23 @@ int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
     if (value < 0 || value > 65536) return fail("sell_lds must be 0..65536 bytes");
     op.sell_lds = (int)value;
     return 0;
+  } else if (n == "rank_gather") {
+    if (value != 0 && value != 1) return fail("rank_gather must be 0 or 1");
+    if (basis_check_unused(op.basis, "qes_op_set_option(rank_gather)")) return -1;
+    if (value) {
+      if (build_rank_gather(op.basis, (op.is_complex || op.chi_complex) ? 2 : 1, nullptr)) return -1;
+    } else {
+      free_rank_gather(op.basis);   // other operators of the basis fall back to the dense gathers
+    }
+    op.rank_gather = (int)value;
+    return 0;
   } else if (n == "asm_panel_cols") {
     if (value < -1) return fail("asm_panel_cols must be -1 (auto), 0 (off) or a column count");
     op.asm_panel_cols = value;
   } else
     return fail("unknown option " + n +
-                " (bond_order_uniform, direct_lookup, xcd_remap, top_row_test, direct_batch, asm_panel_cols, sell_grid)");
+                " (bond_order_uniform, direct_lookup, xcd_remap, top_row_test, direct_batch, asm_panel_cols, sell_grid, "
+                "sell_lds, rank_gather)");
   return build_plan(&op, op.plan.canon == pick_canon_public(op.basis->group) ? -1 : op.plan.canon);
 }
 

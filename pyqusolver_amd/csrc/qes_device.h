@@ -424,6 +424,11 @@ __device__ __forceinline__ uint64_t colex_rank(const QesLookup& L, uint64_t v) {
   return L.rbase2[v >> 32] + (uint64_t)L.rbase1[(uint32_t)row * 65536u + (lo32 >> 16)] + (uint64_t)L.rlo[lo32 & 0xffffu];
 }
 
+// marker of the states of a rank-indexed gather vector that are no
+// representative (a quiet NaN payload; a product writes any NaN of x with
+// these bits as another NaN)
+#define QES_RXS_EMPTY 0x7FFC5E17A11C0DE5ull
+
 // second phase: the map word of rank c gives the index (-1: no representative)
 __device__ __forceinline__ int64_t rank_resolve(ulonglong2 e, uint64_t c) {
   const uint32_t bit = (uint32_t)(c & 63u);

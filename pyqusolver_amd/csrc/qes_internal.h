@@ -78,6 +78,13 @@ struct Basis {
   // lookups resolve with no search step) and the narrowest key array.
   bool use_packed = false;    // probe the packed keys (ns <= 56, no phases)
   int bucket_rows_log2 = -3;  // about 2^k representatives per bucket
+  // rank-indexed gather vector (rank_gather products): rxs[c * rxs_cw ..] =
+  // N_b x_b for the representative b of colex rank c, QES_RXS_EMPTY bits at
+  // every other rank of the filling; filled once with the marker, the
+  // representatives' slots rewritten by every product that uses it
+  double* rxs = nullptr;
+  int64_t rxs_states = 0;
+  int rxs_cw = 0;
   // live Lanczos handles and batches over operators of this basis: they hold
   // its lookup tables (captured graphs keep the pointers), so the tables
   // must not be rebuilt while this is nonzero
@@ -118,6 +125,10 @@ struct Op {
   // spills at 4, else 4; measured on MI355X: L=32 chain 2.37 / 1.94 / 1.96 ms,
   // 6x6 square 140.9 / 122.1 / 117.1 ms for 1 / 2 / 4)
   int direct_batch = -1;
+  // matrix-free single-vector products with a workspace: 1 gathers from the
+  // basis's rank-indexed vector (one load per looked-up state instead of a
+  // map word and an x element); 0 (default) the pre-scaled dense vector
+  int rank_gather = 0;
   // assembled (sliced ELLPACK) table, see qes_assemble.hip
   int64_t* asm_off = nullptr;
   int32_t* asm_len = nullptr;
@@ -178,6 +189,10 @@ int build_hash(Basis* B, hipStream_t st);
 // most max_bytes; *eligible = 0 (and nothing built) otherwise
 int build_rank_map(Basis* B, hipStream_t st, int* eligible, double max_bytes = 4e9);
 int build_bucket(Basis* B, int rows_log2, hipStream_t st);
+// rank-indexed gather vector of cw doubles per state of the filling (needs a
+// rank map); allocated and filled with the empty marker once
+int build_rank_gather(Basis* B, int cw, hipStream_t st);
+void free_rank_gather(Basis* B);
 // reduce == 0: full = expand(x_red) (2^ns entries); 1: x_red = reduce(full)
 int expand_reduce(const Basis* B, int reduce, const double* x_red, double* full, int dtype, hipStream_t st);
 int map_states(const Basis* B, const uint64_t* d_states, int64_t n, int64_t* d_idx, int32_t* d_g,

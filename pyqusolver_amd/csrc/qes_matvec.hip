@@ -40,7 +40,7 @@ namespace qes {
 // (n, ldx) row-major block).  PRE (one column): the neighbours are gathered
 // from xs = N .* x (prescale_kernel), so no element needs its N_b -- a
 // direct rank-map hit loads one line of xs instead of a norm and an x line.
-template <typename W, bool CPLX, int CK, int NB, bool PRE, int K>
+template <typename W, bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false>
 __device__ __forceinline__ void matvec_rows(const QesGroup& G, const QesLookup& L, const QesPlan& P,
                                             const double* __restrict__ chi_re, const double* __restrict__ chi_im,
                                             const QesTermGroup* __restrict__ groups, int n_diag,
@@ -90,7 +90,7 @@ __device__ __forceinline__ void matvec_rows(const QesGroup& G, const QesLookup& 
     // off-diagonal sum over N_b x_b, scaled by 1 / N_a once
     double sr = 0.0, si = 0.0;
     if (K > 1 && L.rmap) {
-      pre_row_sum<W, CPLX, CK, K>(C, r, xs, sr, si);
+      pre_row_sum<W, CPLX, CK, K, RG>(C, r, xs, sr, si);
     } else visit_row_t<W, CPLX, CK, false>(C, r, [&](int64_t b, double fr, double fi, int) {
       const double* xp = xs + b * cw;
       if (CPLX) {
@@ -154,13 +154,31 @@ __device__ __forceinline__ void matvec_rows(const QesGroup& G, const QesLookup& 
 // (the lookups and gathers are latency bound; 7 -> 8 waves: 2.88 -> 2.68 ms
 // at L=32, no spills).  Wider states keep the compiler's allocation: under
 // the cap their cyclic canonicaliser spills to scratch.
-template <bool CPLX, int CK, int NB, bool PRE, int K>
+template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false>
 __global__ void __launch_bounds__(256) QES_MV_ATTR matvec_kernel(QES_MV_PARAMS) {
-  matvec_rows<uint32_t, CPLX, CK, NB, PRE, K>(QES_MV_ARGS);
+  matvec_rows<uint32_t, CPLX, CK, NB, PRE, K, RG>(QES_MV_ARGS);
 }
-template <bool CPLX, int CK, int NB, bool PRE, int K>
+template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false>
 __global__ void __launch_bounds__(256) matvec_kernel_wide(QES_MV_PARAMS) {
-  matvec_rows<uint64_t, CPLX, CK, NB, PRE, K>(QES_MV_ARGS);
+  matvec_rows<uint64_t, CPLX, CK, NB, PRE, K, RG>(QES_MV_ARGS);
+}
+
+// rank-indexed gather vector: rxs[rank(rep_b) * cw ..] = N_b x_b (the ranks
+// of the sorted representatives ascend, so the stores walk the vector once);
+// an x whose bits equal the empty marker is stored as another NaN
+__global__ void __launch_bounds__(256) rank_scatter_kernel(QesLookup L, const double* __restrict__ x,
+                                                           const double* __restrict__ norms,
+                                                           double* __restrict__ rxs, int64_t n, int cw) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t c = colex_rank(L, L.keys64[i]);
+    const double nb = norms[i];
+    for (int w = 0; w < cw; ++w) {
+      double v = x[i * cw + w] * nb;
+      if (__double_as_longlong(v) == (long long)QES_RXS_EMPTY) v = __longlong_as_double(0x7FF8000000000001ll);
+      rxs[c * cw + w] = v;
+    }
+  }
 }
 
 // xs = N .* x (cw doubles per row): the gather vector of the PRE products
@@ -468,7 +486,15 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   const bool pre = ws != nullptr && nbatch == 1;
   // lookups in flight per lane in the direct and canonicalising passes
   const int kb = !B->rmap ? 1 : (op->direct_batch > 0 ? op->direct_batch : (sizeof(W) == 4 ? 2 : 4));
-  if (pre) {
+  // rank gather: the basis's rank-indexed vector replaces the dense pre-scaled one
+  const bool rg = pre && op->rank_gather && B->rmap && B->rxs && B->rxs_cw == (CPLX ? 2 : 1) && kb > 1;
+  if (rg) {
+    const int64_t g = std::min<int64_t>(nblocks, 256 * 64);
+    hipLaunchKernelGGL(rank_scatter_kernel, dim3((unsigned)g), dim3(block), 0, st, B->lookup(), x, B->norms, B->rxs,
+                       n, CPLX ? 2 : 1);
+    QES_HIP(hipGetLastError());
+    ws = B->rxs;
+  } else if (pre) {
     const int64_t g = std::min<int64_t>(nblocks, 256 * 64);
     hipLaunchKernelGGL(prescale_kernel, dim3((unsigned)g), dim3(block), 0, st, x, B->norms, ws, n, CPLX ? 2 : 1);
     QES_HIP(hipGetLastError());
@@ -480,14 +506,21 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
                      B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_lut, B->norms, x, ws, y, n,   \
                      nbatch, j0, ncols, accumulate, op->xcd_remap)
 #define QES_LAUNCH(CK)                                                                                           \
-  if (sizeof(W) == 4) {                                                                                          \
-    if (pre && kb == 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4>));                                     \
+  if (rg && sizeof(W) == 4) {                                                                                    \
+    if (kb == 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4, true>));                                      \
+    else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2, true>));                                              \
+  } else if (rg) {                                                                                               \
+    if (kb == 8) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 8, true>));                                 \
+    else if (kb == 2) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 2, true>));                            \
+    else QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4, true>));                                         \
+  } else if (sizeof(W) == 4) {                                                                                   \
+    if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4>));                                     \
     else if (pre && kb == 2) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2>));                                \
     else if (pre) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 1>));                                           \
     else if (ncols == 1) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, false, 1>));                                   \
     else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 4, false, 1>));                                                   \
   } else {                                                                                                       \
-    if (pre && kb == 4) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4>));                                \
+    if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4>));                                \
     else if (pre && kb == 2) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 2>));                           \
     else if (pre) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 1>));                                      \
     else if (ncols == 1) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, false, 1>));                              \

@@ -173,7 +173,12 @@ __device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& mis
 // then K gathers are in flight together, where visit_row_t waits for each
 // lookup before the next (the product is bound by the latency of these
 // three dependent loads, not by bandwidth).
-template <typename W, bool CPLX, int CK, int K>
+//
+// RG (rank gather): xs is the basis's rank-indexed vector (N_b x_b at the
+// colex rank of representative b, QES_RXS_EMPTY elsewhere), so a lookup is
+// one load at the state's rank instead of a map word and then an x element;
+// the same elements are summed in the same order.
+template <typename W, bool CPLX, int CK, int K, bool RG = false>
 __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* __restrict__ xs, double& sr,
                                             double& si) {
   const int ns = C.G.ns;
@@ -193,7 +198,14 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
   };
   auto emit = [&](W sp, double vr, double vi) {
     const Canon<W> cn = canon_dispatch<W, CK>(C, sp);
-    const int64_t b = lookup_rep<W>(C.L, cn.rep);
+    int64_t b;
+    if (RG) {   // the element of xs sits at the representative's rank (the marker: outside the sector)
+      if (popc<W>(cn.rep) != C.L.n_up) return;
+      b = (int64_t)colex_rank(C.L, (uint64_t)cn.rep);
+      if (__double_as_longlong(xs[b * cw]) == (long long)QES_RXS_EMPTY) return;
+    } else {
+      b = lookup_rep<W>(C.L, cn.rep);
+    }
     if (b < 0) return;
     const double cr = C.s_chr[cn.g];
     if (CPLX) {
@@ -242,25 +254,47 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
             rk[k] = colex_rank(C.L, (uint64_t)(r ^ (((W)1 << pos[k]) | ((W)1 << p1))));
           }
         }
-        ulonglong2 e[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-          e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
         int64_t bk[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
-          if (on[k] && bk[k] < 0) fail |= (W)1 << pos[k];
-        }
         double xr[K], xi[K];
+        if (RG) {
+          // one load per state at its rank; the marker: no representative
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          xr[k] = 0.0;
-          xi[k] = 0.0;
-          if (bk[k] >= 0) {
-            const double* xp = xs + bk[k] * cw;
-            xr[k] = xp[0];
-            if (CPLX) xi[k] = xp[1];
+          for (int k = 0; k < K; ++k) {
+            xr[k] = 0.0;
+            xi[k] = 0.0;
+            bk[k] = -1;
+            if (on[k]) {
+              const double* xp = xs + rk[k] * cw;
+              xr[k] = xp[0];
+              if (CPLX) xi[k] = xp[1];
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (on[k]) {
+              bk[k] = __double_as_longlong(xr[k]) == (long long)QES_RXS_EMPTY ? -1 : 0;
+              if (bk[k] < 0) fail |= (W)1 << pos[k];
+            }
+          }
+        } else {
+          ulonglong2 e[K];
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
+            if (on[k] && bk[k] < 0) fail |= (W)1 << pos[k];
+          }
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            xr[k] = 0.0;
+            xi[k] = 0.0;
+            if (bk[k] >= 0) {
+              const double* xp = xs + bk[k] * cw;
+              xr[k] = xp[0];
+              if (CPLX) xi[k] = xp[1];
+            }
           }
         }
 #pragma unroll
@@ -309,21 +343,40 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
           if (on[k]) rk[k] = colex_rank(C.L, (uint64_t)cn[k].rep);
         }
       }
-      ulonglong2 e[K];
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-        e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
       int64_t bk[K];
       double xr[K], xi[K];
+      if (RG) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
-        xr[k] = 0.0;
-        xi[k] = 0.0;
-        if (bk[k] >= 0) {
-          const double* xp = xs + bk[k] * cw;
-          xr[k] = xp[0];
-          if (CPLX) xi[k] = xp[1];
+        for (int k = 0; k < K; ++k) {
+          xr[k] = 0.0;
+          xi[k] = 0.0;
+          bk[k] = -1;
+          if (on[k]) {
+            const double* xp = xs + rk[k] * cw;
+            xr[k] = xp[0];
+            if (CPLX) xi[k] = xp[1];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (on[k]) bk[k] = __double_as_longlong(xr[k]) == (long long)QES_RXS_EMPTY ? -1 : 0;
+          if (bk[k] < 0) xr[k] = xi[k] = 0.0;   // orbit outside the sector: skipped
+        }
+      } else {
+        ulonglong2 e[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
+          xr[k] = 0.0;
+          xi[k] = 0.0;
+          if (bk[k] >= 0) {
+            const double* xp = xs + bk[k] * cw;
+            xr[k] = xp[0];
+            if (CPLX) xi[k] = xp[1];
+          }
         }
       }
 #pragma unroll

@@ -280,6 +280,57 @@ def test_column_panels_match_one_pass_table(case, fmt):
     H.set_option("asm_panel_cols", -1)
 
 
+@pytest.mark.parametrize("case", [SPACES[0], SPACES[1], SPACES[2], SPACES[8], SPACES[9]],
+                         ids=lambda c: f"{c[0]}{c[1]}x{c[2]}-{c[3]}")
+def test_rank_gather_equals_dense_gather(case):
+    """Matrix-free products that gather from the basis's rank-indexed vector
+    (N_b x_b at the colex rank of each representative, a NaN marker
+    elsewhere) equal the dense pre-scaled gathers bit for bit (same
+    elements, same order); an x element whose bits equal the marker is
+    still a NaN of the output, not a hole; accumulate adds; the option is
+    refused where the basis has no rank map."""
+    import torch
+    from pyqusolver_amd import _lib
+    from pyqusolver_amd.models import J1J2Model
+    kind, lx, ly, sym, n_up, ck = case
+    lat, hs = _space(kind, lx, ly, sym, n_up)
+    H = J1J2Model(lat, J1=-1.0, J2=-0.4, hilbert_space=hs, hx=0.0)
+    dt = torch.complex128 if H.needs_complex else torch.float64
+    g = torch.Generator(device="cuda").manual_seed(17)
+    x = torch.randn(hs.nh, dtype=dt, device="cuda", generator=g)
+    y0 = H.matvec_device(x)
+    H.set_option("rank_gather", 1)
+    y1 = H.matvec_device(x)
+    assert torch.equal(y1, y0)
+    x2 = torch.randn(hs.nh, dtype=dt, device="cuda", generator=g)   # slots rewritten, no stale values
+    y2 = H.matvec_device(x2)
+    H.set_option("rank_gather", 0)
+    assert torch.equal(y2, H.matvec_device(x2))
+    H.set_option("rank_gather", 1)
+    y3 = y1.clone()
+    H.matvec_device(x, out=y3, accumulate=True)
+    assert torch.allclose(y3, 2 * y0, rtol=1e-14, atol=1e-14 * y0.abs().max().item())
+    # an element carrying the marker's bits: the products still see a NaN there
+    xm = x.clone()
+    xv = xm.view(torch.float64) if dt == torch.float64 else torch.view_as_real(xm).reshape(-1)
+    xv.view(torch.int64)[0] = 0x7FFC5E17A11C0DE5
+    ym = H.matvec_device(xm)
+    H.set_option("rank_gather", 0)
+    yd = H.matvec_device(xm)
+    assert torch.equal(torch.isnan(ym), torch.isnan(yd)) and bool(torch.isnan(yd).any())
+    fin = ~torch.isnan(yd)
+    assert torch.equal(ym[fin], yd[fin])
+
+
+def test_rank_gather_refused_without_rank_map():
+    from pyqusolver_amd import _lib
+    from pyqusolver_amd.models import J1J2Model
+    lat, hs = _space("chain", 20, 1, {"translation": 3}, None)
+    H = J1J2Model(lat, J1=-1.0, J2=-0.4, hilbert_space=hs)
+    with pytest.raises(_lib.QesError, match="rank map"):
+        H.set_option("rank_gather", 1)
+
+
 def test_count_outside_sector():
     """Debug count of nonzero elements whose target leaves the sector (the
     reference skips them silently): zero for a U(1)-conserving model at k=0;
