@@ -35,6 +35,7 @@ typedef struct qes_lanczos_s* qes_lanczos_t;
 #define QES_BASE_OP_REFLECT 1 /* reflection.py:143, full bit reversal        */
 #define QES_BASE_OP_FLIP 2    /* parity x/z, parity.py:46-54, complement     */
 #define QES_BASE_OP_FLIP_Y 3  /* parity y, parity.py:55-63, complement, +-i  */
+#define QES_BASE_OP_PERM 4    /* inversion.py:53-86, bit i -> bit perm[i]     */
 
 /*
  * Symmetry sector description (one irrep of G = translations x base).
@@ -65,6 +66,11 @@ typedef struct {
    * phase for n crossing up spins along d (0 x, 1 y), per translation step. */
   const uint64_t* cross_masks;
   const double* bphase;
+  /* Site permutation of QES_BASE_OP_PERM (an inversion whose permutation is
+   * not the full site reversal: InversionSymmetry's op.perm,
+   * inversion.py:135-230): bit i goes to bit perm[i]; ns entries, an
+   * involution; NULL when no base element uses it. */
+  const int32_t* perm;
 } qes_group_spec;
 
 /*

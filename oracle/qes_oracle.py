@@ -223,6 +223,9 @@ def inversion_perm(lat: OLattice, tol: float = 1e-7) -> np.ndarray:
 class Generator:
     kind: str      # "tx" | "ty" | "tz" | "R" | "I" | "Px" | "Py" | "Pz"
     sector: object
+    # "I" only: an explicit site permutation (InversionSymmetry's op.perm,
+    # inversion.py:135-230) in place of the lattice's bounding-box inversion
+    perm: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -298,7 +301,8 @@ class Group:
                 self.refl_perm.append(reflection_perm(ns))
             elif g.kind == "I":
                 iidx[id(g)] = len(self.inv_perm)
-                self.inv_perm.append(inversion_perm(self.lat))
+                self.inv_perm.append(np.asarray(g.perm, dtype=np.int64) if g.perm is not None
+                                     else inversion_perm(self.lat))
             else:
                 pidx[id(g)] = len(self.parity_axis)
                 self.parity_axis.append("xyz".index(g.kind[1]))

@@ -36,6 +36,7 @@ class GroupSpec(C.Structure):
         ("chi_im", C.POINTER(C.c_double)),
         ("cross_masks", C.POINTER(C.c_uint64)),
         ("bphase", C.POINTER(C.c_double)),
+        ("perm", C.POINTER(C.c_int32)),
     ]
 
 

@@ -1,4 +1,5 @@
 // extern "C" boundary of libqes_mi (declarations in include/qes_mi.h).
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -233,6 +234,38 @@ int qes_basis_create(const qes_group_spec* spec, qes_basis_t* out, int64_t* n_ou
     return fail("row_bits must divide ns (and row_bits * nty == ns with y translations)");
   if (spec->cell_bits < 1 || spec->ntx * spec->cell_bits > spec->row_bits) return fail("bad cell_bits");
   if (spec->n_up > spec->ns) return fail("n_up > ns");
+  bool uses_perm = false;
+  for (int b = 0; b < spec->nbase; ++b)
+    for (int k = 0; k < QES_MAX_BASE_OPS; ++k) {
+      if (spec->base_ops[b][k] > QES_BOP_PERM) return fail("unknown base-op code");
+      uses_perm = uses_perm || spec->base_ops[b][k] == QES_BOP_PERM;
+    }
+  // the permutation as (mask, shift) pairs: bits that move by the same
+  // distance share one mask
+  std::vector<uint64_t> pairs;
+  if (uses_perm) {
+    if (!spec->perm) return fail("a QES_BASE_OP_PERM base element needs spec->perm");
+    std::vector<int> seen(spec->ns, 0);
+    for (int i = 0; i < spec->ns; ++i) {
+      const int j = spec->perm[i];
+      if (j < 0 || j >= spec->ns || seen[j]++) return fail("perm is not a permutation of 0..ns-1");
+    }
+    for (int i = 0; i < spec->ns; ++i)
+      if (spec->perm[spec->perm[i]] != i) return fail("perm must be an involution (inversion)");
+    std::vector<std::pair<int, uint64_t>> by_shift;
+    for (int i = 0; i < spec->ns; ++i) {
+      const int d = spec->perm[i] - i;
+      auto it = std::find_if(by_shift.begin(), by_shift.end(), [&](const std::pair<int, uint64_t>& e) {
+        return e.first == d;
+      });
+      if (it == by_shift.end()) by_shift.emplace_back(d, 1ull << i);
+      else it->second |= 1ull << i;
+    }
+    for (const auto& e : by_shift) {
+      pairs.push_back(e.second);
+      pairs.push_back((uint64_t)(int64_t)e.first);
+    }
+  }
   auto* h = new qes_basis_s();
   Basis& B = h->b;
   QesGroup& G = B.group;
@@ -284,6 +317,17 @@ int qes_basis_create(const qes_group_spec* spec, qes_basis_t* out, int64_t* n_ou
       G.bmask = dm;
       G.bphase = dp;
       G.has_bphase = 1;
+    }
+  }
+  if (!rc && !pairs.empty()) {
+    uint64_t* dp = nullptr;
+    if (hipMalloc(&dp, pairs.size() * 8) != hipSuccess ||
+        hipMemcpy(dp, pairs.data(), pairs.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+      if (dp) (void)hipFree(dp);
+      rc = fail("device copy of the site permutation failed");
+    } else {
+      G.pperm = dp;
+      G.npperm = (int32_t)(pairs.size() / 2);
     }
   }
   if (!rc) rc = build_basis(&B, st);
@@ -342,6 +386,7 @@ int qes_basis_destroy(qes_basis_t h) {
   Basis& B = h->b;
   if (B.group.bmask) (void)hipFree((void*)B.group.bmask);
   if (B.group.bphase) (void)hipFree((void*)B.group.bphase);
+  if (B.group.pperm) (void)hipFree((void*)B.group.pperm);
   if (B.chi_re) (void)hipFree(B.chi_re);
   if (B.chi_im) (void)hipFree(B.chi_im);
   if (B.reps) (void)hipFree(B.reps);
@@ -383,13 +428,15 @@ int qes_op_create(qes_basis_t bh, const int32_t* codes, const int32_t* sites, co
     const QesGroup& G = bh->b.group;
     bool has_rev = false, chi_cplx = false;
     for (int b = 0; b < G.nbase; ++b)
-      for (int k = 0; k < QES_MAX_BASE_OPS; ++k) has_rev = has_rev || G.base_ops[b][k] == QES_BOP_REV;
+      for (int k = 0; k < QES_MAX_BASE_OPS; ++k)
+        has_rev = has_rev || G.base_ops[b][k] == QES_BOP_REV || G.base_ops[b][k] == QES_BOP_PERM;
     for (int g = 0; g < bh->b.gsize; ++g) chi_cplx = chi_cplx || std::fabs(bh->b.h_chi_im[g]) > 1e-12;
     if (G.has_phase)
       return fail("parity-y sectors are not supported for matvec: the reference's +-i phase does not make the "
                   "elements a representation (non-Hermitian block)");
     if (has_rev && chi_cplx)
-      return fail("reflection at a generic momentum is not a symmetry sector (non-Hermitian reference block)");
+      return fail("reflection or inversion at a generic momentum is not a symmetry sector (non-Hermitian reference "
+                  "block)");
   }
   auto* h = new qes_op_s();
   Op& op = h->op;

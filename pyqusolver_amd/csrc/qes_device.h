@@ -25,6 +25,7 @@
 #define QES_BOP_REV 1      // reflection: bit i -> bit ns-1-i (reflection.py:143)
 #define QES_BOP_FLIP 2     // parity x or z: complement, phase 1 (parity.py:42-54)
 #define QES_BOP_FLIPY 3    // parity y: complement, phase +-i (parity.py:55-63)
+#define QES_BOP_PERM 4     // site permutation (inversion.py:53-86): bit i -> bit perm[i]
 
 struct QesGroup {
   int32_t ns;
@@ -46,20 +47,26 @@ struct QesGroup {
   // (symmetry_container_jit.py:91-98, translation.py:80-89)
   const uint64_t* bmask;
   const double* bphase;
+  // QES_BOP_PERM: the permutation as npperm (mask, shift) pairs -- the bits
+  // of mask move by shift (left if >= 0, right by -shift otherwise); device
+  // array of 2 * npperm words, read with uniform indices
+  const uint64_t* pperm;
+  int32_t npperm;
 };
 
 // Register-sized view of a QesGroup used inside kernels.  Kernel arguments
 // indexed with run-time indices would be copied to private (scratch) memory,
 // so the per-x-power row masks live in LDS and the base ops are packed into
-// one 32-bit word (2 bits per op).
+// one 64-bit word (3 bits per op).
 struct GV {
-  int ns, nbase, ntx, nty, row_bits, cell_bits, has_phase, n_up, z2, has_bphase;
-  uint32_t bops;
+  int ns, nbase, ntx, nty, row_bits, cell_bits, has_phase, n_up, z2, has_bphase, npperm;
+  uint64_t bops;
   uint64_t mask;
   const uint64_t* rowlo;   // LDS copy, ntx entries
   const uint64_t* bmask;   // global, ntx + nty entries (has_bphase)
   const double* bphase;    // global, 2 (ns + 1) complex entries (has_bphase)
-  __device__ __forceinline__ int bop(int B, int k) const { return (int)((bops >> ((B * 4 + k) * 2)) & 3u); }
+  const uint64_t* pperm;   // global, 2 npperm words (QES_BOP_PERM)
+  __device__ __forceinline__ int bop(int B, int k) const { return (int)((bops >> ((B * 4 + k) * 3)) & 7u); }
 };
 
 // Stage the row masks in LDS (s_rowlo must hold QES_MAX_TX entries); every
@@ -79,12 +86,14 @@ __device__ __forceinline__ GV make_gv(const QesGroup& G, uint64_t* s_rowlo) {
   v.has_bphase = G.has_bphase;
   v.bmask = G.bmask;
   v.bphase = G.bphase;
+  v.pperm = G.pperm;
+  v.npperm = G.npperm;
   v.mask = G.mask;
-  uint32_t b = 0;
+  uint64_t b = 0;
 #pragma unroll
   for (int B = 0; B < QES_MAX_BASE; ++B)
 #pragma unroll
-    for (int k = 0; k < QES_MAX_BASE_OPS; ++k) b |= ((uint32_t)G.base_ops[B][k] & 3u) << ((B * 4 + k) * 2);
+    for (int k = 0; k < QES_MAX_BASE_OPS; ++k) b |= ((uint64_t)G.base_ops[B][k] & 7u) << ((B * 4 + k) * 3);
   v.bops = b;
   v.rowlo = s_rowlo;
   return v;
@@ -135,6 +144,14 @@ __device__ __forceinline__ W apply_base(const GV& G, int B, W s, int* ph_quarter
       int downs = ns - popc<W>(v);
       *ph_quarter += (downs & 1) ? 3 : 1;   // -i : +i  (quarter turns)
       v = (~v) & mask;
+    } else if (op == QES_BOP_PERM) {
+      W out = 0;
+      for (int j = 0; j < G.npperm; ++j) {
+        const W part = v & (W)G.pperm[2 * j];
+        const int sh = (int)(int64_t)G.pperm[2 * j + 1];
+        out |= sh >= 0 ? (W)(part << sh) : (W)(part >> -sh);
+      }
+      v = out;
     }
   }
   return v;

@@ -30,7 +30,7 @@ import numpy as np
 
 SYM_NORM_THRESHOLD = 1e-7
 
-BOP_NONE, BOP_REFLECT, BOP_FLIP, BOP_FLIP_Y = 0, 1, 2, 3
+BOP_NONE, BOP_REFLECT, BOP_FLIP, BOP_FLIP_Y, BOP_PERM = 0, 1, 2, 3, 4
 
 
 class SymmetryGenerators(Enum):
@@ -142,6 +142,9 @@ class SectorGroup:
     # boundary phase table (2, ns + 1) of the x and y directions
     cross_masks: Optional[np.ndarray] = None
     bphase: Optional[np.ndarray] = None
+    # inversion that is not the full site reversal: its site permutation,
+    # applied to bits as the reference does (bit i -> bit perm[i])
+    perm: Optional[np.ndarray] = None
 
     @property
     def order(self) -> int:
@@ -170,6 +173,9 @@ class SectorGroup:
             self._bp = np.ascontiguousarray(bp.reshape(-1))
             s.cross_masks = self._cm.ctypes.data_as(C.POINTER(C.c_uint64))
             s.bphase = self._bp.ctypes.data_as(C.POINTER(C.c_double))
+        if self.perm is not None:
+            self._perm = np.ascontiguousarray(self.perm, dtype=np.int32)
+            s.perm = self._perm.ctypes.data_as(C.POINTER(C.c_int32))
         return s
 
 
@@ -304,6 +310,7 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
     if len(base) > 4:
         raise NotImplementedError("at most two non-translation generators")
     base_ops, base_chi = [], []
+    inv_perm = None
     for elem in base:
         ops, chi = [], 1.0
         for g, sec in elem:
@@ -311,10 +318,15 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
                 ops.append(BOP_REFLECT)
             elif g == SymmetryGenerators.Inversion:
                 perm = inversion_permutation(lattice)
-                if not np.array_equal(perm, np.arange(ns)[::-1]):
-                    raise NotImplementedError("inversion permutations other than the full site reversal "
-                                              "(chain, square and honeycomb lattices here) are not supported")
-                ops.append(BOP_REFLECT)      # bit i -> bit perm[i] = ns-1-i
+                if np.array_equal(perm, np.arange(ns)[::-1]):
+                    ops.append(BOP_REFLECT)      # bit i -> bit perm[i] = ns-1-i
+                else:
+                    # any other inversion: the permutation itself, on bits
+                    # (inversion.py:53-86 moves bit i to bit perm[i])
+                    if np.any(perm[perm] != np.arange(ns)):
+                        raise ValueError("the inversion permutation is not an involution")
+                    inv_perm = perm
+                    ops.append(BOP_PERM)
             elif g == SymmetryGenerators.ParityY:
                 ops.append(BOP_FLIP_Y)
             else:
@@ -349,6 +361,7 @@ def build_sector_group(lattice, ns: int, specs, global_syms: Sequence[GlobalSymm
     grp = SectorGroup(ns=ns, n_up=n_up, z2=z2, generators=gens, nbase=len(base), base_ops=base_ops, ntx=ntx,
                       nty=nty, row_bits=row_bits, cell_bits=cell_bits, chi=chi)
     grp.matvec_error = matvec_error
+    grp.perm = inv_perm
     if twisted:
         grp.cross_masks, grp.bphase = _twist_tables(lattice, ns, ntx, nty)
     grp.g_to_pidx, grp.phase_table = phases_map(chi)

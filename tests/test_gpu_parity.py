@@ -1134,3 +1134,54 @@ def test_twisted_translation_sectors_against_the_oracle(kind, lx, ly, n_up, flux
     import scipy.sparse.linalg as spla
     e_full = spla.eigsh(Hf[idx][:, idx], k=1, which="SA", tol=1e-13)[0][0]
     assert abs(min(e_min) - e_full) <= 1e-10 * abs(e_full)
+
+
+@pytest.mark.parametrize("flip", [False, True])
+def test_general_inversion_permutation_against_the_oracle(flip):
+    """An inversion whose permutation is not the full site reversal (a ring
+    whose sites are numbered in a shuffled order) runs as a permutation base
+    element on the device: representatives bit-exact and products against
+    the oracle's restatement with the same permutation, for a Hamiltonian
+    that commutes with it; the (I, P) blocks' spectra add up to the U(1)
+    sector's."""
+    from pyqusolver_amd import get_u1_sym
+    from pyqusolver_amd.hilbert import HilbertSpace
+    from pyqusolver_amd.operator import Hamiltonian, SpinOperator
+    import test_host_logic as HL
+    import test_oracle_pins as OP
+    ns, n_up = 10, 5
+    rng = np.random.default_rng(13 + flip)
+    order, perm = OP.general_inversion_perm(ns, rng)
+    c, s, co = OP.perm_symmetric_bonds(ns, perm, rng)
+    lat = HL._ShuffledRing(ns, order)
+    e_blocks = []
+    for iv in (1, -1):
+        for pv in ((1, -1) if flip else (None,)):
+            sym = {"inversion": iv}
+            if pv is not None:
+                sym["parity"] = {"x": pv}
+            hs = HilbertSpace(lattice=lat, sym_gen=sym, global_syms=[get_u1_sym(lat, n_up)])
+            assert hs.group.perm is not None
+            gens = [O.Generator("I", iv, perm=perm)] + ([O.Generator("Px", pv)] if pv is not None else [])
+            b = O.scan_basis(O.Group(O.OLattice("chain", ns), gens, n_up=n_up))
+            np.testing.assert_array_equal(hs.representative_list, b.reps)
+            np.testing.assert_allclose(hs.normalization, b.norms, rtol=1e-14)
+            if hs.nh == 0:
+                continue
+            H = Hamiltonian(hs, dtype=np.float64)
+            for code, st, v in zip(c, s, co):
+                H.add(SpinOperator(f"c{code}", int(code), "correlation", code != 22), float(v.real),
+                      sites=[int(st[0]), int(st[1])])
+            x = rng.standard_normal(hs.nh)
+            yo = O.push_matvec(b, ns, c, s, co, False, x)
+            y = H.matvec_fun(x)
+            assert np.linalg.norm(y - yo) <= MATVEC_TOL * np.linalg.norm(yo)
+            Hr = O.reduced_dense(b, ns, c, s, co, False)
+            e_blocks.append(np.linalg.eigvalsh(Hr))
+            if hs.nh > 8:
+                e0 = H.diagonalize(method="lanczos", k=1, tol=1e-12, maxiter=200)[0]
+                assert abs(e0 - e_blocks[-1][0]) <= 1e-10 * abs(e_blocks[-1][0])
+    Hf = O.full_dense_hamiltonian(ns, c, s, co, False)
+    mask = np.array([bin(i).count("1") == n_up for i in range(1 << ns)])
+    np.testing.assert_allclose(np.sort(np.concatenate(e_blocks)), np.linalg.eigvalsh(Hf[np.ix_(mask, mask)]),
+                               atol=1e-12)

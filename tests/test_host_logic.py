@@ -213,3 +213,38 @@ def test_overlap_column_conventions():
         H.overlap(a, b[:4])
     with pytest.raises(ValueError):
         H.overlap(a, rng.standard_normal((6, 2)))
+
+
+class _ShuffledRing(ChainLattice):
+    """A ring of L positions whose sites are numbered in a shuffled order
+    (site s at position order[s]): its inversion through the ring's centre
+    is an involution other than the full site reversal."""
+
+    def __init__(self, L, order):
+        super().__init__(L)
+        self._order = np.asarray(order)
+
+    @property
+    def rvectors(self):
+        out = np.zeros((self.ns, 3))
+        out[:, 0] = self._order
+        return out
+
+
+def test_general_inversion_compiles_to_a_permutation_base_op():
+    """An inversion that is not the full reversal becomes a permutation base
+    element carrying its site permutation (applied to bits, as the reference
+    does); the full reversal stays the reflection op."""
+    from pyqusolver_amd.symmetry import BOP_PERM, BOP_REFLECT, inversion_permutation
+    L = 10
+    order = np.random.default_rng(13).permutation(L)
+    lat = _ShuffledRing(L, order)
+    perm = inversion_permutation(lat)
+    at = np.argsort(order)
+    np.testing.assert_array_equal(perm, at[L - 1 - order])
+    grp = build_sector_group(lat, L, [(SymmetryGenerators.Inversion, -1)], [get_u1_sym(lat, 5)])
+    assert grp.base_ops[1] == [BOP_PERM]
+    np.testing.assert_array_equal(grp.perm, perm)
+    assert np.allclose(grp.chi, [1, -1])
+    g2 = build_sector_group(ChainLattice(L), L, [(SymmetryGenerators.Inversion, 1)], [])
+    assert g2.base_ops[1] == [BOP_REFLECT] and g2.perm is None

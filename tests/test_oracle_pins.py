@@ -323,6 +323,71 @@ def test_inversion_sectors_split_the_momentum_sector(kind, lx, ly, n_up):
 
 
 
+def perm_symmetric_bonds(ns, perm, rng, nb=8):
+    """Random bonds (VM sites) closed under the bit-level action of an
+    involution perm: the reference moves bit i to bit perm[i]
+    (inversion.py:53-86) and the spin VM reads site s on bit ns-1-s, so a
+    bond (s, t) maps to (ns-1-perm[ns-1-s], ns-1-perm[ns-1-t]).  Each bond
+    and its image carry one coupling (ZZ, XX, YY)."""
+    img = lambda s: ns - 1 - int(perm[ns - 1 - s])
+    out = {}
+    while len(out) < nb:
+        s, t = sorted(int(v) for v in rng.choice(ns, 2, replace=False))
+        if (s, t) in out:
+            continue
+        jz, jxy = rng.uniform(0.3, 1.2), rng.uniform(0.3, 1.2)
+        out[(s, t)] = (jz, jxy)
+        out[tuple(sorted((img(s), img(t))))] = (jz, jxy)
+    codes, sites, coeffs = [], [], []
+    for (s, t), (jz, jxy) in sorted(out.items()):
+        for code, v in ((22, jz), (20, jxy), (21, jxy)):
+            codes.append(code)
+            sites.append([s, t])
+            coeffs.append(v)
+    return np.asarray(codes, np.int32), np.asarray(sites, np.int32), np.asarray(coeffs, np.complex128)
+
+
+def general_inversion_perm(ns, rng):
+    """An involution of the sites that is not the full reversal: sites
+    numbered in a shuffled order along a ring of ns positions, inverted
+    through the ring's centre (position p -> ns-1-p)."""
+    order = rng.permutation(ns)              # site s sits at position order[s]
+    at = np.argsort(order)                   # the site at each position
+    perm = at[ns - 1 - order]
+    assert not np.array_equal(perm, np.arange(ns)[::-1])
+    return order, perm
+
+
+@pytest.mark.parametrize("flip", [False, True])
+def test_general_inversion_sectors_split_the_u1_sector(flip):
+    """An inversion whose permutation is not the full site reversal (applied
+    to bits as the reference does): the I = +1 and I = -1 blocks (with the
+    spin flip: all four (I, P) blocks) of a Hamiltonian that commutes with it
+    are Hermitian, their dimensions add up to C(10, 5), and their spectra to
+    the U(1) sector's spectrum of the full-space matrix."""
+    ns, n_up = 10, 5
+    rng = np.random.default_rng(13 + flip)
+    _, perm = general_inversion_perm(ns, rng)
+    c, s, co = perm_symmetric_bonds(ns, perm, rng)
+    Hf = O.full_dense_hamiltonian(ns, c, s, co, False)
+    mask = np.array([bin(i).count("1") == n_up for i in range(1 << ns)])
+    e_full = np.linalg.eigvalsh(Hf[np.ix_(mask, mask)])
+    lat = O.OLattice("chain", ns)
+    es = []
+    for iv in (1, -1):
+        for pv in ((1, -1) if flip else (None,)):
+            gens = [O.Generator("I", iv, perm=perm)] + ([O.Generator("Px", pv)] if pv is not None else [])
+            b = O.scan_basis(O.Group(lat, gens, n_up=n_up))
+            if len(b.reps) == 0:
+                continue
+            Hr = O.reduced_dense(b, ns, c, s, co, False)
+            assert np.allclose(Hr, Hr.conj().T, atol=1e-12)
+            es.append(np.linalg.eigvalsh(Hr))
+    e = np.sort(np.concatenate(es))
+    assert len(e) == len(e_full)
+    np.testing.assert_allclose(e, e_full, atol=1e-12)
+
+
 # ------------------------------------------------- XXZ term list (xxz.py)
 def _pauli_string(ns, ops):
     """Sparse 2^ns matrix of a product of Pauli matrices {site: 'x'|'y'|'z'}
