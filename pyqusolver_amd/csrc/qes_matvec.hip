@@ -36,6 +36,7 @@ namespace qes {
 #define QES_MV_ATTR
 #endif
 
+
 // NB = number of right-hand sides per lane (columns j0 .. j0+NB-1 of an
 // (n, ldx) row-major block).  PRE (one column): the neighbours are gathered
 // from xs = N .* x (prescale_kernel), so no element needs its N_b -- a
@@ -160,6 +161,17 @@ __global__ void __launch_bounds__(256) QES_MV_ATTR matvec_kernel(QES_MV_PARAMS) 
 }
 template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false>
 __global__ void __launch_bounds__(256) matvec_kernel_wide(QES_MV_PARAMS) {
+  matvec_rows<uint64_t, CPLX, CK, NB, PRE, K, RG>(QES_MV_ARGS);
+}
+// The same capped at 8 resident waves per SIMD, for the row-LUT canonicaliser
+// with four lookups in flight: measured on MI355X at 6x6 J1-J2 (|G| = 72),
+// rank gather real 107.3 -> 102.7 ms and complex 132.7 -> 124.0 ms, dense
+// gathers real 115.2 -> 108.8 ms; complex dense gathers spill 76 B per lane
+// under the cap and slow down (146 -> 195 ms), as do the cyclic
+// canonicaliser's variants (112-384 B), so those keep the uncapped kernel.
+template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+matvec_kernel_wide8(QES_MV_PARAMS) {
   matvec_rows<uint64_t, CPLX, CK, NB, PRE, K, RG>(QES_MV_ARGS);
 }
 
@@ -512,6 +524,7 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   } else if (rg) {                                                                                               \
     if (kb == 8) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 8, true>));                                 \
     else if (kb == 2) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 2, true>));                            \
+    else if (CK == QES_CANON_ROWS) QES_LAUNCH_K((matvec_kernel_wide8<CPLX, CK, 1, true, 4, true>));              \
     else QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4, true>));                                         \
   } else if (sizeof(W) == 4) {                                                                                   \
     if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4>));                                     \
@@ -520,7 +533,8 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
     else if (ncols == 1) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, false, 1>));                                   \
     else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 4, false, 1>));                                                   \
   } else {                                                                                                       \
-    if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4>));                                \
+    if (pre && kb >= 4 && !CPLX && CK == QES_CANON_ROWS) QES_LAUNCH_K((matvec_kernel_wide8<CPLX, CK, 1, true, 4>)); \
+    else if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4>));                           \
     else if (pre && kb == 2) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 2>));                           \
     else if (pre) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 1>));                                      \
     else if (ncols == 1) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, false, 1>));                              \
