@@ -42,10 +42,10 @@ HBM_PEAK_GBS = 8000.0
 # current kernels (profiles/matvec_kernel_square6x6_j1j2.md,
 # profiles/matvec_kernel_chain32_j1j2.md)
 TRAFFIC = {
-    # matvec_kernel_wide<false,2,1,true,4,true> (rank gather): 224,735,084 KiB x 2.0 + 984,817 KiB x 1.0
-    "square6x6_rank_gather": 224735084 * 1024 * 2 + 984817 * 1024,
-    # matvec_kernel_wide<false,2,1,true,4,false>: 309,592,613 KiB x 2.0 + 984,817 KiB x 1.0
-    "square6x6": 309592613 * 1024 * 2 + 984817 * 1024,
+    # matvec_kernel_wide8<false,2,1,true,4,true> (rank gather): 241,582,958 KiB x 2.0 + 984,817 KiB x 1.0
+    "square6x6_rank_gather": 241582958 * 1024 * 2 + 984817 * 1024,
+    # matvec_kernel_wide8<false,2,1,true,4,false>: 329,314,499 KiB x 2.0 + 4,924,087 KiB x 1.0
+    "square6x6": 329314499 * 1024 * 2 + 4924087 * 1024,
     # matvec_kernel<false,1,1,true,2>: 4,067,051 KiB x 2.0 + 73,384 KiB x 1.0
     "chain32": 4067051 * 1024 * 2 + 73384 * 1024,
 }
@@ -765,7 +765,7 @@ def measure_sector(a, config, steps, warmup, primary=True):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": TRAFFIC.get(config + ("_rank_gather" if rg_on else "")),
-                     "kernel": ("matvec_kernel_wide" if wl["ns"] > 32 else "matvec_kernel") +
+                     "kernel": ("matvec_kernel_wide8" if wl["ns"] > 32 else "matvec_kernel") +
                                (" (rank gather)" if rg_on else ""),
                      "kernel_ms": prod_ms, "bytes_alg": b_alg,
                      "bytes_alg_formula": "N*(8 rep + 8 norm + 8 x_a + 8 y_a) + E_off*8 x gathers",
