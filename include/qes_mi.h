@@ -401,7 +401,10 @@ int qes_batch_run(qes_batch_t b, int32_t nsteps, int32_t use_graph);
 int qes_batch_get(qes_batch_t b, double* alpha_host, double* beta_host, int32_t* steps_host);
 int qes_batch_set_active(qes_batch_t b, const int32_t* active_host);
 int qes_batch_matvec(qes_batch_t b, const void* x_dev, void* y_dev, void* stream);
-/* Schedule options: "tile_order" (0 problem-contiguous, 1 largest-first
+/* Schedule options: "tile_queue" (0, default: each persistent block takes
+ * every gridDim-th tile; 1: blocks take the next tile from an atomic counter,
+ * reset by a kernel at every product, so blocks that finish early take
+ * more), "tile_order" (0 problem-contiguous, 1 largest-first
  * round-robin), "grid_mult" (persistent matvec blocks per CU), "block_rows"
  * (rows per tile of a block problem), "block_table" (1, default: block
  * problems read a neighbour table built once at creation -- column index and

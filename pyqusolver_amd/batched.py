@@ -153,11 +153,13 @@ class BatchedLanczos:
         return torch.complex128 if self.dtype == 1 else torch.float64
 
     # candidate values of the persistent matvec schedule, tried in this order
-    SCHEDULE_SPACE = (("tile_order", (1, 0)), ("grid_mult", (8, 4, 16, 2)), ("block_rows", (16, 8, 32)))
+    SCHEDULE_SPACE = (("tile_queue", (0, 1)), ("tile_order", (1, 0)), ("grid_mult", (8, 4, 16, 2)),
+                      ("block_rows", (16, 8, 32)))
 
     def autotune(self, reps: int = 3, space=None) -> dict:
-        """Choose the schedule of the persistent batched matvec (tile order,
-        blocks per CU, rows per block tile) for this batch by timing it:
+        """Choose the schedule of the persistent batched matvec (static or
+        dynamic tile queue, tile order, blocks per CU, rows per block tile)
+        for this batch by timing it:
         coordinate search over ``SCHEDULE_SPACE``, each candidate timed as the
         median of ``reps`` products on a fixed vector with device events; the
         fastest value of each knob is kept before the next knob is tried.

@@ -377,14 +377,28 @@ __device__ __forceinline__ void block_row_sell_r(const QesProb& P, int64_t a, un
   }
 }
 
+// next tile of a block: with a queue counter, taken by one lane with an
+// atomic add and shared through LDS (blocks that finish early take more
+// tiles); without, the static grid stride.  Every block leaves the loop once
+// the counter passes ntiles.
+__device__ __forceinline__ int next_tile(int* __restrict__ queue, int t, int* s_t) {
+  if (!queue) return t < 0 ? (int)blockIdx.x : t + (int)gridDim.x;
+  __syncthreads();   // every lane has read the previous tile index
+  if (threadIdx.x == 0) *s_t = atomicAdd(queue, 1);
+  __syncthreads();
+  return *s_t;
+}
+
 template <typename W, bool CPLX>
 __global__ void __launch_bounds__(BRB) batch_matvec_kernel(const QesProb* __restrict__ probs,
                                                            const Tile* __restrict__ tiles, int ntiles,
                                                            const int* __restrict__ prob_active,
                                                            const int* __restrict__ vp_active,
-                                                           const double* __restrict__ x, double* __restrict__ y) {
+                                                           const double* __restrict__ x, double* __restrict__ y,
+                                                           int* __restrict__ queue) {
   extern __shared__ char smem[];
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  __shared__ int s_t;
+  for (int t = next_tile(queue, -1, &s_t); t < ntiles; t = next_tile(queue, t, &s_t)) {
     const Tile T = tiles[t];
     if (prob_active && !prob_active[T.prob]) continue;   // uniform per block
     const QesProb& P = probs[T.prob];
@@ -472,8 +486,10 @@ __global__ void __launch_bounds__(BRB) batch_scaled_kernel(const QesProb* __rest
                                                            const Tile* __restrict__ tiles, int ntiles,
                                                            const int* __restrict__ prob_active,
                                                            const int* __restrict__ vp_active,
-                                                           const double* __restrict__ x, double* __restrict__ y) {
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+                                                           const double* __restrict__ x, double* __restrict__ y,
+                                                           int* __restrict__ queue) {
+  __shared__ int s_t;
+  for (int t = next_tile(queue, -1, &s_t); t < ntiles; t = next_tile(queue, t, &s_t)) {
     const Tile T = tiles[t];
     if (prob_active && !prob_active[T.prob]) continue;   // uniform per block
     const QesProb& P = probs[T.prob];
@@ -801,6 +817,8 @@ struct Batch {
                            // width: fmt | cv << 2 | R << 3; else -1
   int scaled_kernel = 1;   // all_scaled batches: 1 their own kernel, 0 the combined one
   int scaled_lds = 0;      // dynamic LDS bytes per block of that kernel (caps resident blocks per CU)
+  int tile_queue = 0;      // 1: blocks take tiles from an atomic counter (dynamic queue); 0: static grid stride
+  int* d_queue = nullptr;  // the counter
   std::vector<QesProb> h_probs;
   std::vector<int64_t*> t_rowptr;   // per real problem (null for ordinary problems)
   std::vector<int32_t*> t_col;
@@ -942,6 +960,7 @@ static void batch_free(Batch* B) {
   (void)hipFree(B->d_active);
   (void)hipFree(B->d_pactive);
   (void)hipFree(B->d_j);
+  if (B->d_queue) (void)hipFree(B->d_queue);
   if (B->stream) (void)hipStreamDestroy(B->stream);
 }
 
@@ -1159,16 +1178,25 @@ int batch_destroy(Batch* B) {
   return 0;
 }
 
+__global__ void queue_reset_kernel(int* q) { *q = 0; }
+
 static int enqueue_bmatvec(Batch* B, const double* x, double* y, const int* pactive, const int* vactive,
                            hipStream_t st) {
   const bool c = B->dtype == 1;
+  // dynamic tile queue: the counter starts at zero for every product (a
+  // kernel, not a memset, so that captured graphs replay it)
+  int* queue = nullptr;
+  if (B->tile_queue && B->d_queue) {
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, st, B->d_queue);
+    queue = B->d_queue;
+  }
   if (B->scaled_sig >= 0 && B->scaled_kernel) {
     const size_t lds = B->scaled_lds;
     // every problem: coupling-scaled block of the same format, value type and width
     const int fmt = B->scaled_sig & 3, cv = (B->scaled_sig >> 2) & 1, R = B->scaled_sig >> 3;
 #define QES_SK(XC, F, CVV, RR)                                                                                     \
   hipLaunchKernelGGL((batch_scaled_kernel<XC, F, CVV, RR>), dim3(B->grid_matvec), dim3(BRB), lds, st, B->d_probs, \
-                     B->d_tiles, B->ntiles, pactive, vactive, x, y)
+                     B->d_tiles, B->ntiles, pactive, vactive, x, y, queue)
 #define QES_SK_R(XC, F, CVV)              \
   switch (R) {                            \
     case 1: QES_SK(XC, F, CVV, 1); break; \
@@ -1188,14 +1216,14 @@ static int enqueue_bmatvec(Batch* B, const double* x, double* y, const int* pact
 #undef QES_SK
   } else if (B->w32) {
     if (c) hipLaunchKernelGGL((batch_matvec_kernel<uint32_t, true>), dim3(B->grid_matvec), dim3(BRB), B->lds, st,
-                               B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y);
+                               B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y, queue);
     else hipLaunchKernelGGL((batch_matvec_kernel<uint32_t, false>), dim3(B->grid_matvec), dim3(BRB), B->lds, st,
-                            B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y);
+                            B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y, queue);
   } else {
     if (c) hipLaunchKernelGGL((batch_matvec_kernel<uint64_t, true>), dim3(B->grid_matvec), dim3(BRB), B->lds, st,
-                               B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y);
+                               B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y, queue);
     else hipLaunchKernelGGL((batch_matvec_kernel<uint64_t, false>), dim3(B->grid_matvec), dim3(BRB), B->lds, st,
-                            B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y);
+                            B->d_probs, B->d_tiles, B->ntiles, pactive, vactive, x, y, queue);
   }
   QES_HIP(hipGetLastError());
   return 0;
@@ -1323,9 +1351,13 @@ int batch_set_option(Batch* B, const std::string& name, int64_t value) {
   else if (name == "op_tables") B->use_tables = value != 0;
   else if (name == "scaled_kernel") B->scaled_kernel = value ? 1 : 0;
   else if (name == "scaled_lds") B->scaled_lds = (int)std::max<int64_t>(0, std::min<int64_t>(value, 65536));
+  else if (name == "tile_queue") {
+    if (value && !B->d_queue) QES_HIP(hipMalloc(&B->d_queue, sizeof(int)));
+    B->tile_queue = value ? 1 : 0;
+  }
   else return fail("unknown batch option " + name +
                    " (tile_order, grid_mult, block_rows, block_table, op_tables, scaled_kernel, scaled_lds, "
-                   "check_min)");
+                   "tile_queue, check_min)");
   batch_drop_graph(B);
   if (batch_upload_probs(B)) return -1;
   return batch_build_tiles(B);

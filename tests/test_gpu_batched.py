@@ -35,9 +35,12 @@ def test_batched_matvec_equals_individual():
         np.testing.assert_array_equal(ys[p].numpy(), yp)
     for order in (0, 1):
         for gm in (1, 3):
-            B.set_option("tile_order", order)
-            B.set_option("grid_mult", gm)
-            np.testing.assert_array_equal(B.matvec(x).cpu().numpy(), y)
+            for q in (0, 1):   # static grid stride or dynamic tile queue
+                B.set_option("tile_order", order)
+                B.set_option("grid_mult", gm)
+                B.set_option("tile_queue", q)
+                np.testing.assert_array_equal(B.matvec(x).cpu().numpy(), y)
+                np.testing.assert_array_equal(B.matvec(x).cpu().numpy(), y)   # the counter restarts
     # sector tables (assembled operators) against on-the-fly rows: same
     # elements in the same order; complex multiply-adds may contract
     # differently in the two kernels
