@@ -284,8 +284,14 @@ int qes_op_set_canonicalizer(qes_op_t op, int32_t kind);
  * belongs to the basis (8 or 16 bytes per state of the filling: 72.6 GB at
  * 6x6 J1-J2 Sz=0 real, built when the option is set, freed by setting 0),
  * so products that use it on one basis must not overlap in time; same sums
- * in the same order as without it (6x6: 113.6 -> 105.8 ms).  The other
- * options change the order of the sums (results agree to rounding). */
+ * in the same order as without it (6x6: 113.6 -> 105.8 ms); products fall
+ * back to the dense gathers while the basis looks representatives up by
+ * another method.  Launch-shape knobs of the single-vector table product
+ * (measured slower than the defaults, kept for measurements): "sell_grid"
+ * (0 default: a block per 256 rows; 1..64: that many blocks per CU striding
+ * over the rows) and "sell_lds" (bytes of LDS reserved per block, capping
+ * the resident blocks).  The other options change the order of the sums
+ * (results agree to rounding). */
 int qes_op_set_option(qes_op_t op, const char* name, int64_t value);
 int qes_op_get_plan(qes_op_t op, int32_t* canon, int32_t* n_classes, int32_t* n_generic);
 int qes_basis_canon_check(qes_basis_t b, int32_t kind, const uint64_t* states_host, int64_t n, int64_t* mismatches);

@@ -499,7 +499,8 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   // lookups in flight per lane in the direct and canonicalising passes
   const int kb = !B->rmap ? 1 : (op->direct_batch > 0 ? op->direct_batch : (sizeof(W) == 4 ? 2 : 4));
   // rank gather: the basis's rank-indexed vector replaces the dense pre-scaled one
-  const bool rg = pre && op->rank_gather && B->rmap && B->rxs && B->rxs_cw == (CPLX ? 2 : 1) && kb > 1;
+  // (the lookup must be the rank map: the row sum without it gathers by index)
+  const bool rg = pre && op->rank_gather && B->lookup().rmap && B->rxs && B->rxs_cw == (CPLX ? 2 : 1) && kb > 1;
   if (rg) {
     const int64_t g = std::min<int64_t>(nblocks, 256 * 64);
     hipLaunchKernelGGL(rank_scatter_kernel, dim3((unsigned)g), dim3(block), 0, st, B->lookup(), x, B->norms, B->rxs,

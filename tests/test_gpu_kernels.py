@@ -320,6 +320,14 @@ def test_rank_gather_equals_dense_gather(case):
     assert torch.equal(torch.isnan(ym), torch.isnan(yd)) and bool(torch.isnan(yd).any())
     fin = ~torch.isnan(yd)
     assert torch.equal(ym[fin], yd[fin])
+    # with the rank map switched off the option falls back to the dense gathers
+    H.set_option("rank_gather", 1)
+    hs.set_lookup("sorted")
+    y4 = H.matvec_device(x)
+    assert torch.allclose(y4, y0, rtol=1e-14, atol=1e-14 * y0.abs().max().item())
+    hs.set_lookup("rank")
+    assert torch.equal(H.matvec_device(x), y0)
+    H.set_option("rank_gather", 0)
 
 
 def test_rank_gather_refused_without_rank_map():
