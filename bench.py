@@ -88,6 +88,9 @@ def parse():
                    help="batched/sweep: 0 = stopping test every 10 steps; n >= 1 = adaptive, >= n steps apart; "
                         "default: the library's choice (adaptive only for a few large problems)")
     p.add_argument("--sectors", type=int, default=36, help="sweep: momentum sectors to run (of 36)")
+    p.add_argument("--classes", action="store_true",
+                   help="sweep: solve one sector per point-group class of momenta (classes checked on the "
+                        "Hamiltonian's term list); --sectors and --sector-start then count classes")
     p.add_argument("--sector-start", type=int, default=0, help="sweep: first sector (row-major (kx, ky) order)")
     p.add_argument("--baseline-sectors", type=int, default=1,
                    help="sweep: sectors also solved once per J2 value (the sequential baseline)")
@@ -262,10 +265,17 @@ def run_sweep(a):
     the three J2 columns; E0 of each column to the 1e-10 residual test).
     The sequential baseline solves the first ``--baseline-sectors`` sectors
     once per J2 value with an assembled operator each, the way a caller of
-    diagonalize would.  value = (sector, J2) ground states per second."""
+    diagonalize would.  value = (sector, J2) ground states per second.
+
+    ``--classes``: sectors whose momenta are related by a point-group
+    element of the lattice that the Hamiltonian's term list also has (D4 for
+    the 6x6 J1-J2 model: 10 classes of the 36 momenta) hold blocks with the
+    same spectrum; only the first sector of each class is solved and its
+    results stand for the class (the sectors listed are then the classes'
+    first members)."""
     import torch
     from pyqusolver_amd import SquareLattice, get_u1_sym
-    from pyqusolver_amd.batched import CouplingSweep
+    from pyqusolver_amd.batched import CouplingSweep, momentum_classes
     from pyqusolver_amd.hilbert import HilbertSpace
     from pyqusolver_amd.models import J1J2Model
     lat = SquareLattice(6, 6)
@@ -273,6 +283,14 @@ def run_sweep(a):
     scales = [[1.0, j / 0.5] for j in j2s]
     from pyqusolver_amd.hilbert import momentum_key, momentum_symmetry_types, sector_specs
     ks = [momentum_key(lat, sec) for sec, _ in sector_specs(momentum_symmetry_types(lat), lat)]
+    classes = None
+    if a.classes:
+        hs0 = HilbertSpace(lattice=lat, sym_gen={"translation": {"kx": 0, "ky": 0}, "parity": {"x": 1}},
+                           global_syms=[get_u1_sym(lat, 18)])
+        classes = momentum_classes(lat, J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=hs0), ks)
+        del hs0
+        members_of = dict(classes)
+        ks = [rep for rep, _ in classes]
     ks = ks[a.sector_start: a.sector_start + max(1, a.sectors)]
     results, t_total, t_tables, steps_max = [], 0.0, 0.0, 0
     for kx, ky in ks:
@@ -298,6 +316,8 @@ def run_sweep(a):
         results.append({"k": [kx, ky], "N": int(n_sec), "e0": [float(v) for v in res.e0],
                         "steps": [int(v) for v in res.steps], "residuals": [float(v) for v in resid],
                         "seconds": round(t3 - t0, 3)})
+        if classes is not None:
+            results[-1]["class_members"] = [list(m) for m in members_of[(kx, ky)]]
         log(f"sector ({kx},{ky}): E0 {' '.join(f'{v:.12f}' for v in res.e0)} steps {list(res.steps)} "
             f"{t3 - t0:.2f}s (table {t2 - t1:.2f}s, solve {t3 - t2:.2f}s)", a.quiet)
     base = None
@@ -322,14 +342,18 @@ def run_sweep(a):
         base = {"sectors": nb, "seconds_sequential": round(t_seq, 3), "seconds_sweep": round(t_sweep, 3),
                 "speedup": t_seq / t_sweep, "max_rel_dev_sweep_vs_sequential": dev}
         log(f"sequential ({nb} sector(s) x 3 J2): {t_seq:.2f}s vs sweep {t_sweep:.2f}s", a.quiet)
-    n_gs = len(ks) * len(j2s)
+    n_gs = (sum(len(members_of[tuple(k)]) for k in ks) if classes is not None else len(ks)) * len(j2s)
     out = {
         "metric": "H·v matvecs/s + HBM GB/s frac; time-to-E0 at 1e-10 (batched/plain)",
         "value": n_gs / t_total, "unit": "ground states/s ((sector, J2) pairs, 6x6 J1-J2, E0 at 1e-10)",
         "n_gpus": 1, "steps": steps_max, "warmup": 0, "ms_per_step": None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+c128", "data": "synthetic",
         "config": {"workload": "J1-J2 square 6x6 PBC J1=1 Sz=0 flip+, momentum sectors x J2 in {0.4, 0.5, 0.6}, "
-                               "one coupling sweep per sector", "sectors": len(ks), "first_sector": a.sector_start},
+                               "one coupling sweep per sector" +
+                               (", one sector per point-group class of momenta" if classes is not None else ""),
+                   "sectors": len(ks), "first_sector": a.sector_start,
+                   "sectors_covered": (sum(len(members_of[tuple(k)]) for k in ks) if classes is not None
+                                       else len(ks))},
         "seconds_total": t_total, "seconds_tables": t_tables, "sectors": results, "sequential_baseline": base,
         "stopping": "per (sector, J2): Ritz residual |beta_m s_m| <= 1e-10 max(1,|E0|), tested at most 10 steps "
                     "apart, where the residual's decay predicts the crossing",

@@ -522,3 +522,87 @@ def batched_ground_states(hams: Sequence, tol: float = 1e-10, max_steps: int = 3
         resid[idx] = B.residuals()          # after the timed region
         B.close()
     return BatchResult(e0=e0, steps=steps, seconds=seconds, schedule=schedules, residuals=resid), t_create
+
+
+# ---------------------------------------------------------------------------
+# Momentum classes of a sweep: sectors related by a point-group symmetry of
+# the lattice that the Hamiltonian's term list also has hold unitarily
+# equivalent blocks (the same spectra), so a sweep may solve one sector per
+# class.  No reference counterpart: Algebra/hilbert.py:546-652 iterates every
+# momentum; this is an option of the sweep, never implied.
+# ---------------------------------------------------------------------------
+
+def _point_group(lattice):
+    """(matrix, site permutation) of the lattice's point-group elements that
+    map the Bravais lattice onto itself: x -> -x on chains; the 8 elements of
+    D4 on square lattices with lx == ly, the 4 reflections otherwise.  Other
+    lattices: none (a class is a single sector)."""
+    kind = getattr(lattice, "kind", "")
+    lx, ly = int(getattr(lattice, "lx", 0)), int(getattr(lattice, "ly", 1))
+    if getattr(lattice, "n_basis", 1) != 1 or kind not in ("chain", "square"):
+        return []
+    if kind == "chain" or ly == 1:
+        mats = [np.array([[1, 0], [0, 1]]), np.array([[-1, 0], [0, 1]])]
+    else:
+        mats = [np.array([[a, 0], [0, b]]) for a in (1, -1) for b in (1, -1)]
+        if lx == ly:
+            mats += [np.array([[0, a], [b, 0]]) for a in (1, -1) for b in (1, -1)]
+    out = []
+    for M in mats:
+        perm = np.empty(lattice.ns, dtype=np.int64)
+        for s in range(lattice.ns):
+            x, y, _ = lattice.get_coordinates(s)
+            xx, yy = M @ np.array([x, y])
+            perm[s] = lattice.site_index(int(xx) % lx, int(yy) % ly)
+        out.append((M, perm))
+    return out
+
+
+def _term_multiset(codes, sites, coeffs, perm=None):
+    """Canonical multiset of a term list, optionally with permuted sites:
+    two-site XX / YY / ZZ terms (codes 20-22) take an unordered pair, and
+    S-_i S+_j (code 32) is written as S+_j S-_i (code 31)."""
+    out = {}
+    for c, st, v in zip(codes, sites, coeffs):
+        c = int(c)
+        a, b = int(st[0]), int(st[1])
+        if perm is not None:
+            a = int(perm[a]) if a >= 0 else a
+            b = int(perm[b]) if b >= 0 else b
+        if c in (20, 21, 22) and b >= 0:
+            a, b = min(a, b), max(a, b)
+        elif c == 32 and b >= 0:
+            c, a, b = 31, b, a
+        v = complex(v)
+        key = (c, a, b, round(v.real, 12), round(v.imag, 12))
+        out[key] = out.get(key, 0) + 1
+    return out
+
+
+def momentum_classes(lattice, ham, keys):
+    """Group the momentum keys ``keys`` ((kx,) or (kx, ky)) of a sweep into
+    classes of sectors whose blocks are unitarily equivalent: k ~ M k for
+    every point-group element M of the lattice (``_point_group``) under
+    which the Hamiltonian's instruction list is invariant.  Returns
+    [(representative key, [member keys])] in the order of first appearance."""
+    codes, sites, coeffs = ham.instruction_arrays()
+    ref = _term_multiset(codes, sites, coeffs)
+    mats = [M for M, perm in _point_group(lattice) if _term_multiset(codes, sites, coeffs, perm) == ref]
+    lx, ly = int(getattr(lattice, "lx", 0)), int(getattr(lattice, "ly", 1))
+    keys = [tuple(k) for k in keys]
+    index = {k: i for i, k in enumerate(keys)}
+    cls = [-1] * len(keys)
+    out = []
+    for i, k in enumerate(keys):
+        if cls[i] >= 0:
+            continue
+        members = []
+        for M in mats or [np.eye(2, dtype=int)]:
+            kk = M @ np.array([k[0], k[1] if len(k) > 1 else 0])
+            img = (int(kk[0]) % lx,) + ((int(kk[1]) % ly,) if len(k) > 1 else ())
+            j = index.get(img)
+            if j is not None and cls[j] < 0:
+                cls[j] = len(out)
+                members.append(img)
+        out.append((k, sorted(members, key=lambda m: index[m])))
+    return out

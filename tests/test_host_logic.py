@@ -248,3 +248,61 @@ def test_general_inversion_compiles_to_a_permutation_base_op():
     assert np.allclose(grp.chi, [1, -1])
     g2 = build_sector_group(ChainLattice(L), L, [(SymmetryGenerators.Inversion, 1)], [])
     assert g2.base_ops[1] == [BOP_REFLECT] and g2.perm is None
+
+
+
+class _Terms:
+    """Instruction arrays of an oracle term list, in the Hamiltonian's form."""
+
+    def __init__(self, t):
+        self._a = t.arrays()
+
+    def instruction_arrays(self):
+        return self._a
+
+
+def test_momentum_classes_follow_the_model_symmetry():
+    """Point-group classes of momentum sectors: the 6x6 J1-J2 model (D4)
+    has 10 classes of the 36 momenta; a site-dependent field leaves every
+    momentum its own class; a 4x6 rectangle keeps the 4 reflections only; a
+    chain pairs k with -k."""
+    from pyqusolver_amd.batched import momentum_classes
+    keys = [(kx, ky) for kx in range(6) for ky in range(6)]
+    cl = momentum_classes(SquareLattice(6, 6), _Terms(O.terms_j1j2(O.OLattice("square", 6, 6), -1.0, -0.5)), keys)
+    assert [len(m) for _, m in cl] == [1, 4, 4, 2, 4, 8, 4, 4, 4, 1]
+    assert sorted(k for _, m in cl for k in m) == sorted(keys)
+    assert dict(cl)[(1, 2)] == [(1, 2), (1, 4), (2, 1), (2, 5), (4, 1), (4, 5), (5, 2), (5, 4)]
+    hz = np.random.default_rng(3).uniform(-1, 1, 36)
+    cl = momentum_classes(SquareLattice(6, 6), _Terms(O.terms_j1j2(O.OLattice("square", 6, 6), -1.0, -0.5,
+                                                                       hz=hz)), keys)
+    assert len(cl) == 36
+    keys46 = [(kx, ky) for kx in range(4) for ky in range(6)]
+    cl = momentum_classes(SquareLattice(4, 6), _Terms(O.terms_j1j2(O.OLattice("square", 4, 6), -1.0, -0.5)), keys46)
+    assert max(len(m) for _, m in cl) == 4 and len(cl) == 3 * 4
+    cl = momentum_classes(ChainLattice(8), _Terms(O.terms_xxz(O.OLattice("chain", 8), 1.0, 0.7)), [(k,) for k in range(8)])
+    assert [m for _, m in cl] == [[(0,)], [(1,), (7,)], [(2,), (6,)], [(3,), (5,)], [(4,)]]
+
+
+def test_momentum_class_members_have_equal_spectra():
+    """The equivalence the classes rest on: on the 4x4 J1-J2 model at
+    half filling, every momentum sector of a class has the same spectrum
+    (dense blocks of the oracle), and sectors of different classes differ."""
+    from pyqusolver_amd.batched import momentum_classes
+    olat = O.OLattice("square", 4, 4)
+    t = O.terms_j1j2(olat, -1.0, -0.45)
+    c, s, co = t.arrays()
+    keys = [(kx, ky) for kx in range(4) for ky in range(4)]
+    spec = {}
+    for k in keys:
+        grp = O.Group(olat, [O.Generator("tx", k[0]), O.Generator("ty", k[1])], n_up=8)
+        b = O.scan_basis(grp)
+        spec[k] = np.linalg.eigvalsh(O.reduced_dense(b, 16, c, s, co, True))
+    cl = momentum_classes(SquareLattice(4, 4), _Terms(t), keys)
+    reps = []
+    for rep, members in cl:
+        for m in members:
+            np.testing.assert_allclose(spec[m], spec[rep], atol=1e-11)
+        reps.append(rep)
+    for i, a in enumerate(reps):
+        for b_ in reps[i + 1:]:
+            assert len(spec[a]) != len(spec[b_]) or not np.allclose(spec[a], spec[b_], atol=1e-8)
