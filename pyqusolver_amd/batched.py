@@ -585,7 +585,7 @@ def momentum_classes(lattice, ham, keys):
     every point-group element M of the lattice (``_point_group``) under
     which the Hamiltonian's instruction list is invariant.  Returns
     [(representative key, [member keys])] in the order of first appearance."""
-    codes, sites, coeffs = ham.instruction_arrays()
+    sites, coeffs, codes = ham.instruction_arrays()
     ref = _term_multiset(codes, sites, coeffs)
     mats = [M for M, perm in _point_group(lattice) if _term_multiset(codes, sites, coeffs, perm) == ref]
     lx, ly = int(getattr(lattice, "lx", 0)), int(getattr(lattice, "ly", 1))

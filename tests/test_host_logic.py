@@ -255,10 +255,10 @@ class _Terms:
     """Instruction arrays of an oracle term list, in the Hamiltonian's form."""
 
     def __init__(self, t):
-        self._a = t.arrays()
+        self._codes, self._sites, self._coeffs = t.arrays()
 
     def instruction_arrays(self):
-        return self._a
+        return self._sites, self._coeffs, self._codes
 
 
 def test_momentum_classes_follow_the_model_symmetry():
@@ -281,6 +281,19 @@ def test_momentum_classes_follow_the_model_symmetry():
     assert max(len(m) for _, m in cl) == 4 and len(cl) == 3 * 4
     cl = momentum_classes(ChainLattice(8), _Terms(O.terms_xxz(O.OLattice("chain", 8), 1.0, 0.7)), [(k,) for k in range(8)])
     assert [m for _, m in cl] == [[(0,)], [(1,), (7,)], [(2,), (6,)], [(3,), (5,)], [(4,)]]
+
+
+def test_momentum_classes_of_the_model_classes():
+    """The same classes from the J1-J2 model class's own instruction list
+    (the sweep's caller), with and without a uniform field."""
+    from pyqusolver_amd.batched import momentum_classes
+    from pyqusolver_amd.models import J1J2Model
+    lat = SquareLattice(6, 6)
+    keys = [(kx, ky) for kx in range(6) for ky in range(6)]
+    for hz in (None, 0.3):
+        H = J1J2Model(lat, J1=-1.0, J2=-0.5, hilbert_space=_dummy_space(lat), hz=hz)
+        cl = momentum_classes(lat, H, keys)
+        assert [len(m) for _, m in cl] == [1, 4, 4, 2, 4, 8, 4, 4, 4, 1]
 
 
 def test_momentum_class_members_have_equal_spectra():
