@@ -250,8 +250,19 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
           if (on[k]) {
             int p1 = pos[k] - d;
             if (p1 < 0) p1 += ns;
+            const W sp = r ^ (((W)1 << pos[k]) | ((W)1 << p1));
+            // row lattices: a state whose top row exceeds the smallest
+            // rotation of one of its rows (or of their images under the base
+            // operations) is no representative -- no lookup; it goes to the
+            // canonicalising pass as a failed lookup would, so the sums keep
+            // their order
+            if (CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
+              on[k] = false;
+              fail |= (W)1 << pos[k];
+              continue;
+            }
             // an exchange keeps the filling, so the state has a colex rank
-            rk[k] = colex_rank(C.L, (uint64_t)(r ^ (((W)1 << pos[k]) | ((W)1 << p1))));
+            rk[k] = colex_rank(C.L, (uint64_t)sp);
           }
         }
         int64_t bk[K];
