@@ -512,10 +512,15 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
     hipLaunchKernelGGL(prescale_kernel, dim3((unsigned)g), dim3(block), 0, st, x, B->norms, ws, n, CPLX ? 2 : 1);
     QES_HIP(hipGetLastError());
   }
+  // complex pre-scaled products with several lookups in flight (pre_row_sum)
+  // keep each lane's misses per exchange class in LDS after the operator tables
+  const size_t lds = (pre && kb > 1 && CPLX) ? ((op->lds_bytes + 7) & ~(size_t)7) + 8 + (size_t)P.nclass * block * 8
+                                     : op->lds_bytes;
+  if (lds > 65536) return fail("matvec: operator tables and miss slots exceed 64 KB of LDS");
   for (int64_t j0 = 0; j0 < nbatch; j0 += 4) {
     const int ncols = (int)std::min<int64_t>(4, nbatch - j0);
 #define QES_LAUNCH_K(KER)                                                                                        \
-  hipLaunchKernelGGL(KER, dim3((unsigned)nblocks), dim3(block), op->lds_bytes, st, B->group, B->lookup(), P,       \
+  hipLaunchKernelGGL(KER, dim3((unsigned)nblocks), dim3(block), lds, st, B->group, B->lookup(), P,                 \
                      B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_lut, B->norms, x, ws, y, n,   \
                      nbatch, j0, ncols, accumulate, op->xcd_remap)
 #define QES_LAUNCH(CK)                                                                                           \

@@ -33,6 +33,7 @@ struct RowCtx {
   const double* s_chi;
   const double* s_ex;           // LDS exchange values
   const uint32_t* s_lut;        // LDS row LUT
+  uint64_t* s_fail;             // LDS miss slots of pre_row_sum (after the tables, 8-byte aligned)
   int uniform;                  // bond order of the direct-lookup pass
   int direct;                   // run the direct-lookup pass
   int top_test;                 // row lattices: top-row test before a direct lookup
@@ -96,12 +97,18 @@ __device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& mis
     }
     f(b, fr, fi, cpl);
   };
-  // Exchange classes.  Pass 1 walks the bonds in wave-uniform order (all
-  // lanes of a wavefront handle the same bond together, so their targets and
-  // key probes cluster in a few cache lines) and looks the flipped state up
-  // directly: most flipped states are representatives already, and a hit on
-  // the packed key array gives the stabiliser size (character of the
-  // identity = 1).  Pass 2 canonicalises only the flipped states that missed.
+  // Exchange classes.  Pass 1 looks a class's flipped states up directly
+  // (optionally walking the bonds in wave-uniform order, so that the lanes'
+  // targets and key probes cluster in a few cache lines): most flipped
+  // states are representatives already, and a hit on the packed key array
+  // gives the stabiliser size (character of the identity = 1).  Pass 2
+  // canonicalises the flipped states that missed.  Order of the elements,
+  // the order of pre_row_sum: real elements class by class (the class's
+  // hits by bond position, then its misses); complex elements every class's
+  // hits (class, then bond position), then every class's misses.
+  W fails[QES_MAX_CLASSES];
+#pragma unroll
+  for (int cc = 0; cc < QES_MAX_CLASSES; ++cc) fails[cc] = 0;
   for (int c = 0; c < C.nclass; ++c) {
     const int d = C.s_cls_d[c];
     const W pm = (W)C.s_cls_pm[c];
@@ -139,6 +146,29 @@ __device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& mis
         f(b, e[0] * nb, CPLX ? -e[1] * nb : 0.0, c * ns + p);
       }
     }
+    if (CPLX) {
+#pragma unroll
+      for (int cc = 0; cc < QES_MAX_CLASSES; ++cc)
+        if (cc == c) fails[cc] = fail;
+      continue;
+    }
+    while (fail) {
+      const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
+      fail &= fail - 1;
+      int p1 = p - d;
+      if (p1 < 0) p1 += ns;
+      const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
+      const int b0 = (int)((r >> p) & 1);
+      const double* e = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
+      emit(sp, e[0], CPLX ? e[1] : 0.0, c * ns + p);
+    }
+  }
+  for (int c = 0; CPLX && c < C.nclass; ++c) {
+    const int d = C.s_cls_d[c];
+    W fail = 0;
+#pragma unroll
+    for (int cc = 0; cc < QES_MAX_CLASSES; ++cc)
+      if (cc == c) fail = fails[cc];
     while (fail) {
       const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
       fail &= fail - 1;
@@ -168,11 +198,11 @@ __device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& mis
 
 // Off-diagonal sum of row r over a pre-scaled vector xs = N .* x (PRE
 // products): sum_b conj(val) chi(g*) xs[b], the same elements in the same
-// order as visit_row_t<NORM = false>.  With a rank map the direct pass takes
-// the wavefront-uniform bonds K at a time: K colex ranks, then K map words,
-// then K gathers are in flight together, where visit_row_t waits for each
-// lookup before the next (the product is bound by the latency of these
-// three dependent loads, not by bandwidth).
+// order as visit_row_t<NORM = false>.  With a rank map each pass takes K
+// bonds at a time: K colex ranks, then K map words, then K gathers are in
+// flight together, where visit_row_t waits for each lookup before the next.
+// Complex elements need the LDS miss slots C.s_fail (nclass 8-byte words per
+// lane of the block).
 //
 // RG (rank gather): xs is the basis's rank-indexed vector (N_b x_b at the
 // colex rank of representative b, QES_RXS_EMPTY elsewhere), so a lookup is
@@ -185,7 +215,6 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
   const W mask = (W)C.G.mask;
   const int cw = CPLX ? 2 : 1;
   const bool direct = C.L.rmap != nullptr && C.direct != 0;
-  const bool uniform = C.uniform > 0 || (C.uniform < 0 && C.L.rmap);
   auto add = [&](int64_t b, double fr, double fi) {
     const double* xp = xs + b * cw;
     if (CPLX) {
@@ -223,53 +252,146 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
       add(b, vr * cr, 0.0);
     }
   };
-  for (int c = 0; c < C.nclass; ++c) {
-    const int d = C.s_cls_d[c];
-    const W pm = (W)C.s_cls_pm[c];
-    const W act = (r ^ rotl_ns<W>(r, d, ns, mask)) & pm;
-    W fail = direct ? (W)0 : act;
-    if (direct) {
-      W m = uniform ? pm : act;
-      while (m) {
+  if (!CPLX) {
+    // real elements: class by class, each class's direct hits, then its
+    // misses (no LDS slots: the occupancy of the real kernels is worth more
+    // than the steps the merged miss stream saves them)
+    const bool uniform = C.uniform > 0 || (C.uniform < 0 && C.L.rmap);
+    for (int c = 0; c < C.nclass; ++c) {
+      const int d = C.s_cls_d[c];
+      const W pm = (W)C.s_cls_pm[c];
+      const W act = (r ^ rotl_ns<W>(r, d, ns, mask)) & pm;
+      W fail = direct ? (W)0 : act;
+      if (direct) {
+        W m = uniform ? pm : act;
+        while (m) {
+          int pos[K];
+          bool on[K];
+          uint64_t rk[K];
+  #pragma unroll
+          for (int k = 0; k < K; ++k) {
+            pos[k] = 0;
+            on[k] = false;
+            if (m) {
+              pos[k] = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
+              m &= m - 1;
+              on[k] = (act >> pos[k]) & 1;
+            }
+          }
+  #pragma unroll
+          for (int k = 0; k < K; ++k) {
+            rk[k] = 0;
+            if (on[k]) {
+              int p1 = pos[k] - d;
+              if (p1 < 0) p1 += ns;
+              const W sp = r ^ (((W)1 << pos[k]) | ((W)1 << p1));
+              // row lattices: a state whose top row exceeds the smallest
+              // rotation of one of its rows (or of their images under the base
+              // operations) is no representative -- no lookup; it goes to the
+              // canonicalising pass as a failed lookup would, so the sums keep
+              // their order
+              if (CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
+                on[k] = false;
+                fail |= (W)1 << pos[k];
+                continue;
+              }
+              // an exchange keeps the filling, so the state has a colex rank
+              rk[k] = colex_rank(C.L, (uint64_t)sp);
+            }
+          }
+          int64_t bk[K];
+          double xr[K], xi[K];
+          if (RG) {
+            // one load per state at its rank; the marker: no representative
+  #pragma unroll
+            for (int k = 0; k < K; ++k) {
+              xr[k] = 0.0;
+              xi[k] = 0.0;
+              bk[k] = -1;
+              if (on[k]) {
+                const double* xp = xs + rk[k] * cw;
+                xr[k] = xp[0];
+                if (CPLX) xi[k] = xp[1];
+              }
+            }
+  #pragma unroll
+            for (int k = 0; k < K; ++k) {
+              if (on[k]) {
+                bk[k] = __double_as_longlong(xr[k]) == (long long)QES_RXS_EMPTY ? -1 : 0;
+                if (bk[k] < 0) fail |= (W)1 << pos[k];
+              }
+            }
+          } else {
+            ulonglong2 e[K];
+  #pragma unroll
+            for (int k = 0; k < K; ++k)
+              e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
+  #pragma unroll
+            for (int k = 0; k < K; ++k) {
+              bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
+              if (on[k] && bk[k] < 0) fail |= (W)1 << pos[k];
+            }
+  #pragma unroll
+            for (int k = 0; k < K; ++k) {
+              xr[k] = 0.0;
+              xi[k] = 0.0;
+              if (bk[k] >= 0) {
+                const double* xp = xs + bk[k] * cw;
+                xr[k] = xp[0];
+                if (CPLX) xi[k] = xp[1];
+              }
+            }
+          }
+  #pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (bk[k] >= 0) {
+              const int b0 = (int)((r >> pos[k]) & 1);
+              const double* ev = C.s_ex + ((c * ns + pos[k]) * 2 + b0) * 2;
+              const double fr = ev[0];
+              if (CPLX) {
+                const double fi = -ev[1];
+                sr += fr * xr[k] - fi * xi[k];
+                si += fr * xi[k] + fi * xr[k];
+              } else {
+                sr += fr * xr[k];
+              }
+            }
+          }
+        }
+      }
+      // states that are no representative: canonicalise K at a time, then
+      // their ranks, map words and gathers in flight together (rank map only)
+      while (C.L.rmap && fail) {
         int pos[K];
         bool on[K];
+        Canon<W> cn[K];
         uint64_t rk[K];
-#pragma unroll
+  #pragma unroll
         for (int k = 0; k < K; ++k) {
           pos[k] = 0;
           on[k] = false;
-          if (m) {
-            pos[k] = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
-            m &= m - 1;
-            on[k] = (act >> pos[k]) & 1;
+          if (fail) {
+            pos[k] = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
+            fail &= fail - 1;
+            on[k] = true;
           }
         }
-#pragma unroll
+  #pragma unroll
         for (int k = 0; k < K; ++k) {
           rk[k] = 0;
           if (on[k]) {
             int p1 = pos[k] - d;
             if (p1 < 0) p1 += ns;
-            const W sp = r ^ (((W)1 << pos[k]) | ((W)1 << p1));
-            // row lattices: a state whose top row exceeds the smallest
-            // rotation of one of its rows (or of their images under the base
-            // operations) is no representative -- no lookup; it goes to the
-            // canonicalising pass as a failed lookup would, so the sums keep
-            // their order
-            if (CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
-              on[k] = false;
-              fail |= (W)1 << pos[k];
-              continue;
-            }
-            // an exchange keeps the filling, so the state has a colex rank
-            rk[k] = colex_rank(C.L, (uint64_t)sp);
+            cn[k] = canon_dispatch<W, CK>(C, r ^ (((W)1 << pos[k]) | ((W)1 << p1)));
+            // outside the filling (not for a group of spin flips at half filling): no rank
+            on[k] = popc<W>(cn[k].rep) == C.L.n_up;
+            if (on[k]) rk[k] = colex_rank(C.L, (uint64_t)cn[k].rep);
           }
         }
         int64_t bk[K];
         double xr[K], xi[K];
         if (RG) {
-          // one load per state at its rank; the marker: no representative
-#pragma unroll
+  #pragma unroll
           for (int k = 0; k < K; ++k) {
             xr[k] = 0.0;
             xi[k] = 0.0;
@@ -280,25 +402,19 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
               if (CPLX) xi[k] = xp[1];
             }
           }
-#pragma unroll
+  #pragma unroll
           for (int k = 0; k < K; ++k) {
-            if (on[k]) {
-              bk[k] = __double_as_longlong(xr[k]) == (long long)QES_RXS_EMPTY ? -1 : 0;
-              if (bk[k] < 0) fail |= (W)1 << pos[k];
-            }
+            if (on[k]) bk[k] = __double_as_longlong(xr[k]) == (long long)QES_RXS_EMPTY ? -1 : 0;
+            if (bk[k] < 0) xr[k] = xi[k] = 0.0;   // orbit outside the sector: skipped
           }
         } else {
           ulonglong2 e[K];
-#pragma unroll
+  #pragma unroll
           for (int k = 0; k < K; ++k)
             e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
-#pragma unroll
+  #pragma unroll
           for (int k = 0; k < K; ++k) {
             bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
-            if (on[k] && bk[k] < 0) fail |= (W)1 << pos[k];
-          }
-#pragma unroll
-          for (int k = 0; k < K; ++k) {
             xr[k] = 0.0;
             xi[k] = 0.0;
             if (bk[k] >= 0) {
@@ -308,124 +424,278 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
             }
           }
         }
-#pragma unroll
+  #pragma unroll
         for (int k = 0; k < K; ++k) {
           if (bk[k] >= 0) {
             const int b0 = (int)((r >> pos[k]) & 1);
             const double* ev = C.s_ex + ((c * ns + pos[k]) * 2 + b0) * 2;
-            const double fr = ev[0];
+            const double vr = ev[0], vi = CPLX ? ev[1] : 0.0;
+            const double cr = C.s_chr[cn[k].g];
             if (CPLX) {
-              const double fi = -ev[1];
+              const double ci = C.s_chi[cn[k].g];
+              double fr = vr * cr + vi * ci, fi = vr * ci - vi * cr;
+              if (C.G.has_phase || C.G.has_bphase) {   // times conj(phase of the minimising element on s')
+                int p1 = pos[k] - d;
+                if (p1 < 0) p1 += ns;
+                double pr, pi;
+                element_phase<W>(C.G, cn[k].g, r ^ (((W)1 << pos[k]) | ((W)1 << p1)), &pr, &pi);
+                const double ur = fr * pr + fi * pi, ui = fi * pr - fr * pi;
+                fr = ur;
+                fi = ui;
+              }
               sr += fr * xr[k] - fi * xi[k];
               si += fr * xi[k] + fi * xr[k];
             } else {
-              sr += fr * xr[k];
+              sr += (vr * cr) * xr[k];
             }
           }
         }
       }
+      while (fail) {
+        const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
+        fail &= fail - 1;
+        int p1 = p - d;
+        if (p1 < 0) p1 += ns;
+        const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
+        const int b0 = (int)((r >> p) & 1);
+        const double* ev = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
+        emit(sp, ev[0], CPLX ? ev[1] : 0.0);
+      }
     }
-    // states that are no representative: canonicalise K at a time, then
-    // their ranks, map words and gathers in flight together (rank map only)
-    while (C.L.rmap && fail) {
-      int pos[K];
-      bool on[K];
-      Canon<W> cn[K];
-      uint64_t rk[K];
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        pos[k] = 0;
-        on[k] = false;
-        if (fail) {
-          pos[k] = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
+  } else
+  {
+    if (!C.L.rmap) {   // (callers pass a rank map; without one every state is canonicalised, class by class)
+      for (int c = 0; c < C.nclass; ++c) {
+        const int d = C.s_cls_d[c];
+        W fail = (r ^ rotl_ns<W>(r, d, ns, mask)) & (W)C.s_cls_pm[c];
+        while (fail) {
+          const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
           fail &= fail - 1;
-          on[k] = true;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        rk[k] = 0;
-        if (on[k]) {
-          int p1 = pos[k] - d;
+          int p1 = p - d;
           if (p1 < 0) p1 += ns;
-          cn[k] = canon_dispatch<W, CK>(C, r ^ (((W)1 << pos[k]) | ((W)1 << p1)));
-          // outside the filling (not for a group of spin flips at half filling): no rank
-          on[k] = popc<W>(cn[k].rep) == C.L.n_up;
-          if (on[k]) rk[k] = colex_rank(C.L, (uint64_t)cn[k].rep);
+          const int b0 = (int)((r >> p) & 1);
+          const double* ev = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
+          emit(r ^ (((W)1 << p) | ((W)1 << p1)), ev[0], CPLX ? ev[1] : 0.0);
         }
       }
-      int64_t bk[K];
-      double xr[K], xi[K];
-      if (RG) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          xr[k] = 0.0;
-          xi[k] = 0.0;
-          bk[k] = -1;
-          if (on[k]) {
-            const double* xp = xs + rk[k] * cw;
-            xr[k] = xp[0];
-            if (CPLX) xi[k] = xp[1];
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          if (on[k]) bk[k] = __double_as_longlong(xr[k]) == (long long)QES_RXS_EMPTY ? -1 : 0;
-          if (bk[k] < 0) xr[k] = xi[k] = 0.0;   // orbit outside the sector: skipped
-        }
-      } else {
-        ulonglong2 e[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-          e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
-          xr[k] = 0.0;
-          xi[k] = 0.0;
-          if (bk[k] >= 0) {
-            const double* xp = xs + bk[k] * cw;
-            xr[k] = xp[0];
-            if (CPLX) xi[k] = xp[1];
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        if (bk[k] >= 0) {
-          const int b0 = (int)((r >> pos[k]) & 1);
-          const double* ev = C.s_ex + ((c * ns + pos[k]) * 2 + b0) * 2;
-          const double vr = ev[0], vi = CPLX ? ev[1] : 0.0;
-          const double cr = C.s_chr[cn[k].g];
-          if (CPLX) {
-            const double ci = C.s_chi[cn[k].g];
-            double fr = vr * cr + vi * ci, fi = vr * ci - vi * cr;
-            if (C.G.has_phase || C.G.has_bphase) {   // times conj(phase of the minimising element on s')
-              int p1 = pos[k] - d;
-              if (p1 < 0) p1 += ns;
-              double pr, pi;
-              element_phase<W>(C.G, cn[k].g, r ^ (((W)1 << pos[k]) | ((W)1 << p1)), &pr, &pi);
-              const double ur = fr * pr + fi * pi, ui = fi * pr - fr * pi;
-              fr = ur;
-              fi = ui;
+    } else if (C.nclass > 0) {
+      // Each lane walks one stream of (class, bond) items over all exchange
+      // classes, K items per step, so a wavefront runs as many steps as its
+      // longest row needs rather than the sum over classes of the longest
+      // class.  Pass 1 looks the flipped states up directly and records the
+      // misses in the lane's LDS slot of each class (C.s_fail, nclass words per
+      // lane at a stride of blockDim.x); pass 2 walks the misses the same way.
+      uint64_t* const sf = C.s_fail + threadIdx.x;
+      const int bd = blockDim.x;
+      auto act_of = [&](int cc, int dd) -> W { return (r ^ rotl_ns<W>(r, dd, ns, mask)) & (W)C.s_cls_pm[cc]; };
+      if (direct) {
+        int c = 0, dc = C.s_cls_d[0];
+        W m = act_of(0, dc);
+        sf[0] = 0;
+        while (true) {
+          int pos[K], cl[K], dk[K];
+          bool on[K];
+          uint64_t rk[K];
+  #pragma unroll
+          for (int k = 0; k < K; ++k) {
+            while (!m && c + 1 < C.nclass) {   // next class with an active bond (every class's slot is cleared)
+              ++c;
+              dc = C.s_cls_d[c];
+              m = act_of(c, dc);
+              sf[c * bd] = 0;
             }
-            sr += fr * xr[k] - fi * xi[k];
-            si += fr * xi[k] + fi * xr[k];
+            pos[k] = 0;
+            cl[k] = c;
+            dk[k] = dc;
+            on[k] = m != 0;
+            if (on[k]) {
+              pos[k] = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
+              m &= m - 1;
+            }
+          }
+          if (!on[0]) break;   // slots fill in stream order: an empty first slot ends the stream
+  #pragma unroll
+          for (int k = 0; k < K; ++k) {
+            rk[k] = 0;
+            if (on[k]) {
+              int p1 = pos[k] - dk[k];
+              if (p1 < 0) p1 += ns;
+              const W sp = r ^ (((W)1 << pos[k]) | ((W)1 << p1));
+              // row lattices: a state whose top row exceeds the smallest
+              // rotation of one of its rows (or of their images under the base
+              // operations) is no representative -- no lookup; it goes to the
+              // canonicalising pass as a failed lookup would
+              if (CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
+                on[k] = false;
+                __hip_atomic_fetch_or(sf + cl[k] * bd, 1ull << pos[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                continue;
+              }
+              // an exchange keeps the filling, so the state has a colex rank
+              rk[k] = colex_rank(C.L, (uint64_t)sp);
+            }
+          }
+          bool hit[K];
+          double xr[K], xi[K];
+          if (RG) {
+            // one load per state at its rank; the marker: no representative
+  #pragma unroll
+            for (int k = 0; k < K; ++k) {
+              xr[k] = 0.0;
+              xi[k] = 0.0;
+              if (on[k]) {
+                const double* xp = xs + rk[k] * cw;
+                xr[k] = xp[0];
+                if (CPLX) xi[k] = xp[1];
+              }
+            }
+  #pragma unroll
+            for (int k = 0; k < K; ++k) hit[k] = on[k] && __double_as_longlong(xr[k]) != (long long)QES_RXS_EMPTY;
           } else {
-            sr += (vr * cr) * xr[k];
+            ulonglong2 e[K];
+            int64_t bk[K];
+  #pragma unroll
+            for (int k = 0; k < K; ++k)
+              e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
+  #pragma unroll
+            for (int k = 0; k < K; ++k) {
+              bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
+              hit[k] = bk[k] >= 0;
+            }
+  #pragma unroll
+            for (int k = 0; k < K; ++k) {
+              xr[k] = 0.0;
+              xi[k] = 0.0;
+              if (hit[k]) {
+                const double* xp = xs + bk[k] * cw;
+                xr[k] = xp[0];
+                if (CPLX) xi[k] = xp[1];
+              }
+            }
+          }
+  #pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (on[k] && !hit[k])
+              __hip_atomic_fetch_or(sf + cl[k] * bd, 1ull << pos[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+  #pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (hit[k]) {
+              const int b0 = (int)((r >> pos[k]) & 1);
+              const double* ev = C.s_ex + ((cl[k] * ns + pos[k]) * 2 + b0) * 2;
+              const double fr = ev[0];
+              if (CPLX) {
+                const double fi = -ev[1];
+                sr += fr * xr[k] - fi * xi[k];
+                si += fr * xi[k] + fi * xr[k];
+              } else {
+                sr += fr * xr[k];
+              }
+            }
           }
         }
       }
-    }
-    while (fail) {
-      const int p = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)fail) : __builtin_ctzll((uint64_t)fail);
-      fail &= fail - 1;
-      int p1 = p - d;
-      if (p1 < 0) p1 += ns;
-      const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
-      const int b0 = (int)((r >> p) & 1);
-      const double* ev = C.s_ex + ((c * ns + p) * 2 + b0) * 2;
-      emit(sp, ev[0], CPLX ? ev[1] : 0.0);
+      // states that are no representative (every active bond's state without
+      // the direct pass): canonicalise K at a time, then their ranks, map words
+      // and gathers in flight together
+      int c = 0, dc = C.s_cls_d[0];
+      W m = direct ? (W)sf[0] : act_of(0, dc);
+      while (true) {
+        int pos[K], cl[K], dk[K];
+        bool on[K];
+        Canon<W> cn[K];
+        uint64_t rk[K];
+  #pragma unroll
+        for (int k = 0; k < K; ++k) {
+          while (!m && c + 1 < C.nclass) {
+            ++c;
+            dc = C.s_cls_d[c];
+            m = direct ? (W)sf[c * bd] : act_of(c, dc);
+          }
+          pos[k] = 0;
+          cl[k] = c;
+          dk[k] = dc;
+          on[k] = m != 0;
+          if (on[k]) {
+            pos[k] = (sizeof(W) == 4) ? __builtin_ctz((uint32_t)m) : __builtin_ctzll((uint64_t)m);
+            m &= m - 1;
+          }
+        }
+        if (!on[0]) break;
+  #pragma unroll
+        for (int k = 0; k < K; ++k) {
+          rk[k] = 0;
+          if (on[k]) {
+            int p1 = pos[k] - dk[k];
+            if (p1 < 0) p1 += ns;
+            cn[k] = canon_dispatch<W, CK>(C, r ^ (((W)1 << pos[k]) | ((W)1 << p1)));
+            // outside the filling (not for a group of spin flips at half filling): no rank
+            on[k] = popc<W>(cn[k].rep) == C.L.n_up;
+            if (on[k]) rk[k] = colex_rank(C.L, (uint64_t)cn[k].rep);
+          }
+        }
+        int64_t bk[K];
+        double xr[K], xi[K];
+        if (RG) {
+  #pragma unroll
+          for (int k = 0; k < K; ++k) {
+            xr[k] = 0.0;
+            xi[k] = 0.0;
+            bk[k] = -1;
+            if (on[k]) {
+              const double* xp = xs + rk[k] * cw;
+              xr[k] = xp[0];
+              if (CPLX) xi[k] = xp[1];
+            }
+          }
+  #pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (on[k]) bk[k] = __double_as_longlong(xr[k]) == (long long)QES_RXS_EMPTY ? -1 : 0;
+            if (bk[k] < 0) xr[k] = xi[k] = 0.0;   // orbit outside the sector: skipped
+          }
+        } else {
+          ulonglong2 e[K];
+  #pragma unroll
+          for (int k = 0; k < K; ++k)
+            e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
+  #pragma unroll
+          for (int k = 0; k < K; ++k) {
+            bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
+            xr[k] = 0.0;
+            xi[k] = 0.0;
+            if (bk[k] >= 0) {
+              const double* xp = xs + bk[k] * cw;
+              xr[k] = xp[0];
+              if (CPLX) xi[k] = xp[1];
+            }
+          }
+        }
+  #pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (bk[k] >= 0) {
+            const int b0 = (int)((r >> pos[k]) & 1);
+            const double* ev = C.s_ex + ((cl[k] * ns + pos[k]) * 2 + b0) * 2;
+            const double vr = ev[0], vi = CPLX ? ev[1] : 0.0;
+            const double cr = C.s_chr[cn[k].g];
+            if (CPLX) {
+              const double ci = C.s_chi[cn[k].g];
+              double fr = vr * cr + vi * ci, fi = vr * ci - vi * cr;
+              if (C.G.has_phase || C.G.has_bphase) {   // times conj(phase of the minimising element on s')
+                int p1 = pos[k] - dk[k];
+                if (p1 < 0) p1 += ns;
+                double pr, pi;
+                element_phase<W>(C.G, cn[k].g, r ^ (((W)1 << pos[k]) | ((W)1 << p1)), &pr, &pi);
+                const double ur = fr * pr + fi * pi, ui = fi * pr - fr * pi;
+                fr = ur;
+                fi = ui;
+              }
+              sr += fr * xr[k] - fi * xi[k];
+              si += fr * xi[k] + fi * xr[k];
+            } else {
+              sr += (vr * cr) * xr[k];
+            }
+          }
+        }
+      }
     }
   }
   for (int t = C.n_diag; t < C.n_diag + C.n_gen; ++t) {
@@ -499,6 +769,7 @@ __device__ __forceinline__ RowCtx load_ctx(const QesGroup& G, const QesLookup& L
   C.s_chi = s_chi;
   C.s_ex = s_ex;
   C.s_lut = s_lut;
+  C.s_fail = (uint64_t*)(((uintptr_t)(s_lut + nlut) + 7) & ~(uintptr_t)7);
   return C;
 }
 
