@@ -268,7 +268,8 @@ int qes_op_set_canonicalizer(qes_op_t op, int32_t kind);
  * wavefront walks the bonds in one order in that pass): 1 on, 0 (default)
  * off, -1 on where a rank map serves; "xcd_remap" (0/1);
  * "top_row_test" (0 default / 1: row lattices test the top row before a
- * direct lookup, single-lookup path only); "direct_batch" (-1 default, 1, 2,
+ * direct lookup; a state that fails it is no representative and goes
+ * straight to canonicalisation); "direct_batch" (-1 default, 1, 2,
  * 4, 8: rank-map lookups a lane keeps in flight in a qes_matvec_ws product,
  * 8 for rank-gather products of more than 32 sites only; same sums in the
  * same order for every value); "asm_panel_cols" (-1

@@ -119,7 +119,7 @@ struct Op {
   // wavefront-uniform order); 1 uniform order; -1 uniform where a rank map serves
   int bond_order_uniform = 0;
   int direct_lookup = -1;
-  int top_row_test = 0;            // row lattices: rows_top_ok before each direct lookup (K = 1 path)
+  int top_row_test = 0;            // row lattices: rows_top_ok before each direct lookup (off: on the 6x6 J1-J2 product its VALU costs more than the skipped lookups save)
   // PRE products with a rank map: lookups in flight per lane (1, 2, 4; -1:
   // 2 for 32-bit states, whose kernel is capped at 8 waves per SIMD and
   // spills at 4, else 4; measured on MI355X: L=32 chain 2.37 / 1.94 / 1.96 ms,
