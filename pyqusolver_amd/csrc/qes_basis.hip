@@ -9,7 +9,7 @@
 // each tested for "s == min_g g(s)" with an early exit, and the survivors
 // are stream-compacted in order, so the list comes out sorted without a sort.
 #include <algorithm>
-#include <cmath>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -370,16 +370,7 @@ __global__ void rank_of_reps_kernel(QesLookup L, const uint64_t* __restrict__ re
                                     uint64_t* __restrict__ rank) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint64_t v = reps[i];
-  if (L.ns <= 32) {
-    const uint32_t u = (uint32_t)v;
-    rank[i] = L.rbase[u >> 16] + (uint64_t)L.rlo[u & 0xffffu];
-  } else {
-    const uint32_t lo32 = (uint32_t)v;
-    const int row = __popc(lo32) - L.k1min;
-    rank[i] = L.rbase2[v >> 32] + (uint64_t)L.rbase1[(uint32_t)row * 65536u + (lo32 >> 16)] +
-              (uint64_t)L.rlo[lo32 & 0xffffu];
-  }
+  rank[i] = colex_rank(L, reps[i]);
 }
 
 // word w: bits of the ranks in [64w, 64w + 64) and the number of ranks below 64w
@@ -451,28 +442,28 @@ int build_rank_map(Basis* B, hipStream_t st, int* eligible, double max_bytes) {
       }
     }
   }
+  // one buffer, fixed offsets (QesLookup::rtab)
+  std::vector<uint64_t> tab;
+  if (G.ns <= 32) {
+    tab.assign(QES_RTAB_NARROW_LO + 16384, 0);
+    std::memcpy(tab.data(), rbase.data(), rbase.size() * sizeof(uint32_t));
+    std::memcpy(tab.data() + QES_RTAB_NARROW_LO, rlo.data(), rlo.size() * sizeof(uint16_t));
+  } else {
+    tab.assign(QES_RTAB_WIDE_B1 + (rbase1.size() + 1) / 2, 0);
+    std::memcpy(tab.data(), rbase2.data(), rbase2.size() * sizeof(uint64_t));
+    std::memcpy(tab.data() + QES_RTAB_WIDE_LO, rlo.data(), rlo.size() * sizeof(uint16_t));
+    std::memcpy(tab.data() + QES_RTAB_WIDE_B1, rbase1.data(), rbase1.size() * sizeof(uint32_t));
+  }
   const int64_t n = B->n;
   uint64_t* d_rank = nullptr;
-  hipError_t e = hipMalloc(&B->rlo, 65536 * sizeof(uint16_t));
-  if (e == hipSuccess) e = hipMemcpy(B->rlo, rlo.data(), 65536 * sizeof(uint16_t), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !rbase.empty()) e = hipMalloc(&B->rbase, rbase.size() * sizeof(uint32_t));
-  if (e == hipSuccess && !rbase.empty())
-    e = hipMemcpy(B->rbase, rbase.data(), rbase.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !rbase2.empty()) e = hipMalloc(&B->rbase2, rbase2.size() * sizeof(uint64_t));
-  if (e == hipSuccess && !rbase2.empty())
-    e = hipMemcpy(B->rbase2, rbase2.data(), rbase2.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !rbase1.empty()) e = hipMalloc(&B->rbase1, rbase1.size() * sizeof(uint32_t));
-  if (e == hipSuccess && !rbase1.empty())
-    e = hipMemcpy(B->rbase1, rbase1.data(), rbase1.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  hipError_t e = hipMalloc(&B->rtab, tab.size() * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMemcpy(B->rtab, tab.data(), tab.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&B->rmap, (size_t)std::max<int64_t>(nwords, 1) * 16);
   if (e == hipSuccess) e = hipMalloc(&d_rank, (size_t)std::max<int64_t>(n, 1) * sizeof(uint64_t));
   if (e == hipSuccess) {
     B->k1min = k1min;
     QesLookup L = B->lookup();
-    L.rbase = B->rbase;
-    L.rlo = B->rlo;
-    L.rbase2 = B->rbase2;
-    L.rbase1 = B->rbase1;
+    L.rtab = B->rtab;
     if (n) hipLaunchKernelGGL(rank_of_reps_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, B->reps, n,
                               d_rank);
     if (nwords) hipLaunchKernelGGL(rank_map_fill_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, st,
@@ -482,13 +473,10 @@ int build_rank_map(Basis* B, hipStream_t st, int* eligible, double max_bytes) {
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (d_rank) (void)hipFree(d_rank);
   if (e != hipSuccess) {
-    for (void* p : {(void*)B->rbase, (void*)B->rlo, (void*)B->rmap, (void*)B->rbase2, (void*)B->rbase1})
+    for (void* p : {(void*)B->rtab, (void*)B->rmap})
       if (p) (void)hipFree(p);
-    B->rbase = nullptr;
-    B->rlo = nullptr;
+    B->rtab = nullptr;
     B->rmap = nullptr;
-    B->rbase2 = nullptr;
-    B->rbase1 = nullptr;
     return check_hip(e, "build_rank_map");
   }
   B->rmap_words = nwords;

@@ -397,11 +397,8 @@ int qes_basis_destroy(qes_basis_t h) {
   if (B.keyc) (void)hipFree(B.keyc);
   if (B.rmap) (void)hipFree(B.rmap);
   free_rank_gather(&B);
-  if (B.rbase) (void)hipFree(B.rbase);
-  if (B.rlo) (void)hipFree(B.rlo);
+  if (B.rtab) (void)hipFree(B.rtab);
   if (B.sqrtn) (void)hipFree(B.sqrtn);
-  if (B.rbase2) (void)hipFree(B.rbase2);
-  if (B.rbase1) (void)hipFree(B.rbase1);
   delete h;
   return 0;
 }
@@ -501,6 +498,10 @@ int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
     if (value < 0 || value > 65536) return fail("sell_lds must be 0..65536 bytes");
     op.sell_lds = (int)value;
     return 0;
+  } else if (n == "lean") {
+    if (value != 0 && value != 1) return fail("lean must be 0 or 1");
+    op.lean = (int)value;
+    return 0;   // kernel choice only
   } else if (n == "rank_gather") {
     if (value != 0 && value != 1) return fail("rank_gather must be 0 or 1");
     if (basis_check_unused(op.basis, "qes_op_set_option(rank_gather)")) return -1;
@@ -517,7 +518,7 @@ int qes_op_set_option(qes_op_t h, const char* name, int64_t value) {
   } else
     return fail("unknown option " + n +
                 " (bond_order_uniform, direct_lookup, xcd_remap, top_row_test, direct_batch, asm_panel_cols, sell_grid, "
-                "sell_lds, rank_gather)");
+                "sell_lds, rank_gather, lean)");
   return build_plan(&op, op.plan.canon == pick_canon_public(op.basis->group) ? -1 : op.plan.canon);
 }
 

@@ -157,6 +157,15 @@ __device__ __forceinline__ W apply_base(const GV& G, int B, W s, int* ph_quarter
   return v;
 }
 
+// base element B of a lean group (identity, then at most one spin flip; the
+// host checks the group): no run-time switch over the operation codes
+template <typename W, bool LEAN>
+__device__ __forceinline__ W base_of(const GV& G, int B, W s) {
+  if (LEAN) return B == 0 ? s : (W)((~s) & (W)G.mask);
+  int dummy = 0;
+  return apply_base<W>(G, B, s, &dummy);
+}
+
 // apply group element g = (a * nty + b) * nbase + B (reference order): base
 // element B first, then a cell shifts along x and b row shifts along y
 template <typename W>
@@ -352,12 +361,12 @@ struct QesLookup {
   // + rlo[v & 0xffff] with k1 = popcount of the low 32 bits; rmap[c / 64]
   // holds the bit mask of the representatives among ranks 64*(c/64) ... +63
   // and the number of representatives of lower rank, so one 16-byte load
-  // resolves a lookup.
+  // resolves a lookup.  The tables share one buffer, rtab, at fixed offsets
+  // (one base pointer in the kernels): ns <= 32 rbase (u32[65536]) then rlo
+  // (u16[65536]); 33..48 sites rbase2 (u64[65536]), rlo, then rbase1
+  // (u32[rows * 65536]).
   const uint64_t* rmap;     // pairs {bits, count before}
-  const uint32_t* rbase;    // ns <= 32: 65536 states of the filling with a smaller high half
-  const uint16_t* rlo;      // 65536: states with the same low-half popcount and a smaller low half
-  const uint64_t* rbase2;   // 33..48 sites: 65536, states with a smaller top chunk
-  const uint32_t* rbase1;   // 33..48 sites: rows k1 = k1min.., 65536 each
+  const uint64_t* rtab;     // the colex tables
   int32_t n_up;
   int32_t k1min;
   // sqrt(k) for k = 0..256 (stabiliser sizes; |G| <= 256): one cached load
@@ -429,16 +438,35 @@ __device__ __forceinline__ int64_t lookup_packed(const QesLookup& L, uint64_t v,
   return -1;
 }
 
+// offsets of the colex tables in rtab, in 8-byte words
+#define QES_RTAB_NARROW_LO 32768ull            // after rbase u32[65536]
+#define QES_RTAB_WIDE_LO 65536ull              // after rbase2 u64[65536]
+#define QES_RTAB_WIDE_B1 (65536ull + 16384ull)  // after rlo u16[65536]
+
 // colex rank of a state of the U(1) filling (the rank map's first phase:
-// two or three loads from small cached tables, independent of each other)
-__device__ __forceinline__ uint64_t colex_rank(const QesLookup& L, uint64_t v) {
-  if (L.ns <= 32) {
-    const uint32_t u = (uint32_t)v;
-    return L.rbase[u >> 16] + (uint32_t)L.rlo[u & 0xffffu];
-  }
+// two or three loads from small cached tables, independent of each other);
+// narrow: ns <= 32, wide: 33..48 sites
+__device__ __forceinline__ uint64_t colex_rank_narrow(const QesLookup& L, uint64_t v) {
+  const uint32_t u = (uint32_t)v;
+  const uint32_t* rbase = reinterpret_cast<const uint32_t*>(L.rtab);
+  const uint16_t* rlo = reinterpret_cast<const uint16_t*>(L.rtab + QES_RTAB_NARROW_LO);
+  return rbase[u >> 16] + (uint32_t)rlo[u & 0xffffu];
+}
+__device__ __forceinline__ uint64_t colex_rank_wide(const QesLookup& L, uint64_t v) {
   const uint32_t lo32 = (uint32_t)v;
-  const int row = __popc(lo32) - L.k1min;
-  return L.rbase2[v >> 32] + (uint64_t)L.rbase1[(uint32_t)row * 65536u + (lo32 >> 16)] + (uint64_t)L.rlo[lo32 & 0xffffu];
+  const int row = __popc(lo32) - L.k1min;   // in 0..16 for a state of the filling
+  const uint16_t* rlo = reinterpret_cast<const uint16_t*>(L.rtab + QES_RTAB_WIDE_LO);
+  const uint32_t* rbase1 = reinterpret_cast<const uint32_t*>(L.rtab + QES_RTAB_WIDE_B1);
+  return L.rtab[v >> 32] + (uint64_t)rbase1[(uint32_t)row * 65536u + (lo32 >> 16)] + (uint64_t)rlo[lo32 & 0xffffu];
+}
+__device__ __forceinline__ uint64_t colex_rank(const QesLookup& L, uint64_t v) {
+  return L.ns <= 32 ? colex_rank_narrow(L, v) : colex_rank_wide(L, v);
+}
+// the same for kernels whose state word W follows the site count (32-bit
+// states exactly for ns <= 32): no run-time branch
+template <typename W>
+__device__ __forceinline__ uint64_t colex_rank_w(const QesLookup& L, uint64_t v) {
+  return sizeof(W) == 4 ? colex_rank_narrow(L, v) : colex_rank_wide(L, v);
 }
 
 // marker of the states of a rank-indexed gather vector that are no
@@ -457,16 +485,7 @@ __device__ __forceinline__ int64_t rank_resolve(ulonglong2 e, uint64_t c) {
 __device__ __forceinline__ int64_t lookup_rank(const QesLookup& L, uint64_t v) {
   if (L.ns < 64 && (v >> L.ns)) return -1;   // outside the ns-bit states (ns <= 48 here)
   if (__popcll(v) != L.n_up) return -1;
-  uint64_t c;
-  if (L.ns <= 32) {
-    const uint32_t u = (uint32_t)v;
-    c = L.rbase[u >> 16] + (uint32_t)L.rlo[u & 0xffffu];
-  } else {
-    const uint32_t lo32 = (uint32_t)v;
-    const int row = __popc(lo32) - L.k1min;   // in 0..16 for a state of the filling
-    c = L.rbase2[v >> 32] + (uint64_t)L.rbase1[(uint32_t)row * 65536u + (lo32 >> 16)] +
-        (uint64_t)L.rlo[lo32 & 0xffffu];
-  }
+  const uint64_t c = colex_rank(L, v);
   const ulonglong2 e = reinterpret_cast<const ulonglong2*>(L.rmap)[c >> 6];
   const uint32_t bit = (uint32_t)(c & 63u);
   if (!((e.x >> bit) & 1ull)) return -1;
@@ -592,21 +611,21 @@ __device__ __forceinline__ int max_zero_run(W v, int ns, W mask, W* starts) {
   return kk;
 }
 
-template <typename W, bool FULL>
+template <typename W, bool FULL, bool LEAN = false>
 __device__ __forceinline__ Canon<W> canonicalize_cyclic(const GV& G, W s) {
   const int ns = G.ns;
   const W mask = (W)G.mask;
+  constexpr int NB = LEAN ? 2 : QES_MAX_BASE;
   W vb[QES_MAX_BASE], st[QES_MAX_BASE];
   int run[QES_MAX_BASE];
   int best_run = -1;
 #pragma unroll
-  for (int B = 0; B < QES_MAX_BASE; ++B) {
+  for (int B = 0; B < NB; ++B) {
     run[B] = -1;
     st[B] = 0;
     vb[B] = 0;
     if (B < G.nbase) {
-      int dummy = 0;
-      vb[B] = apply_base<W>(G, B, s, &dummy);
+      vb[B] = base_of<W, LEAN>(G, B, s);
       run[B] = max_zero_run<W, FULL>(vb[B], ns, mask, &st[B]);
       best_run = run[B] > best_run ? run[B] : best_run;
     }
@@ -616,7 +635,7 @@ __device__ __forceinline__ Canon<W> canonicalize_cyclic(const GV& G, W s) {
   c.count = 0;
   c.g = 0;
 #pragma unroll
-  for (int B = 0; B < QES_MAX_BASE; ++B) {
+  for (int B = 0; B < NB; ++B) {
     if (B < G.nbase && run[B] == best_run) {
       W m = st[B];
       while (m) {
@@ -643,7 +662,7 @@ __device__ __forceinline__ Canon<W> canonicalize_cyclic(const GV& G, W s) {
 // top).  lut[rv] = minval | (amask << 16) over x powers a (a*cell_bits bit
 // rotation of the row).  Only candidates whose top row attains it are
 // evaluated in full.
-template <typename W>
+template <typename W, bool LEAN = false>
 __device__ __forceinline__ Canon<W> canonicalize_rows(const GV& G, const uint32_t* __restrict__ lut, W s) {
   const int ns = G.ns;
   const W mask = (W)G.mask;
@@ -651,15 +670,13 @@ __device__ __forceinline__ Canon<W> canonicalize_rows(const GV& G, const uint32_
   const int nrows = ns / Wr;
   const W rowmask = (W)((Wr == 64) ? ~0ull : ((1ull << Wr) - 1ull));
   const int q_lo = (G.nty > 1) ? 0 : nrows - 1;
+  constexpr int NB = LEAN ? 2 : QES_MAX_BASE;
   W vb[QES_MAX_BASE];
 #pragma unroll
-  for (int B = 0; B < QES_MAX_BASE; ++B) {
-    int dummy = 0;
-    vb[B] = (B < G.nbase) ? apply_base<W>(G, B, s, &dummy) : (W)0;
-  }
+  for (int B = 0; B < NB; ++B) vb[B] = (B < G.nbase) ? base_of<W, LEAN>(G, B, s) : (W)0;
   uint32_t mtop = 0xffffu;
 #pragma unroll
-  for (int B = 0; B < QES_MAX_BASE; ++B) {
+  for (int B = 0; B < NB; ++B) {
     if (B < G.nbase) {
       for (int q = q_lo; q < nrows; ++q) {
         const uint32_t e = lut[(uint32_t)((vb[B] >> (q * Wr)) & rowmask)] & 0xffffu;
@@ -672,7 +689,7 @@ __device__ __forceinline__ Canon<W> canonicalize_rows(const GV& G, const uint32_
   c.count = 0;
   c.g = 0;
 #pragma unroll
-  for (int B = 0; B < QES_MAX_BASE; ++B) {
+  for (int B = 0; B < NB; ++B) {
     if (B < G.nbase) {
       for (int q = q_lo; q < nrows; ++q) {
         const uint32_t e = lut[(uint32_t)((vb[B] >> (q * Wr)) & rowmask)];

@@ -49,10 +49,7 @@ struct Basis {
     L.packed = (use_packed && keyc && !(use_rank && rmap)) ? 1 : 0;   // the rank map takes precedence
     L.keyc = keyc;
     L.rmap = (use_rank && rmap) ? rmap : nullptr;
-    L.rbase = rbase;
-    L.rlo = rlo;
-    L.rbase2 = rbase2;
-    L.rbase1 = rbase1;
+    L.rtab = rtab;
     L.n_up = group.n_up;
     L.k1min = k1min;
     L.sqrtn = sqrtn;
@@ -62,10 +59,7 @@ struct Basis {
   double* sqrtn = nullptr;    // device sqrt(0..256)
   // rank map (ns <= 48 with a U(1) filling): see QesLookup
   uint64_t* rmap = nullptr;
-  uint32_t* rbase = nullptr;
-  uint16_t* rlo = nullptr;
-  uint64_t* rbase2 = nullptr;
-  uint32_t* rbase1 = nullptr;
+  uint64_t* rtab = nullptr;   // colex tables (layout: QesLookup)
   int k1min = 0;
   int64_t rmap_words = 0;
   bool use_rank = false;
@@ -129,6 +123,7 @@ struct Op {
   // basis's rank-indexed vector (one load per looked-up state instead of a
   // map word and an x element); 0 (default) the pre-scaled dense vector
   int rank_gather = 0;
+  int lean = 1;                    // lean row-sum kernels where the group and plan allow them (0: never)
   // assembled (sliced ELLPACK) table, see qes_assemble.hip
   int64_t* asm_off = nullptr;
   int32_t* asm_len = nullptr;

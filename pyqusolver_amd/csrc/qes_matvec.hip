@@ -41,7 +41,7 @@ namespace qes {
 // (n, ldx) row-major block).  PRE (one column): the neighbours are gathered
 // from xs = N .* x (prescale_kernel), so no element needs its N_b -- a
 // direct rank-map hit loads one line of xs instead of a norm and an x line.
-template <typename W, bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false>
+template <typename W, bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false, bool LEAN = false>
 __device__ __forceinline__ void matvec_rows(const QesGroup& G, const QesLookup& L, const QesPlan& P,
                                             const double* __restrict__ chi_re, const double* __restrict__ chi_im,
                                             const QesTermGroup* __restrict__ groups, int n_diag,
@@ -91,7 +91,7 @@ __device__ __forceinline__ void matvec_rows(const QesGroup& G, const QesLookup& 
     // off-diagonal sum over N_b x_b, scaled by 1 / N_a once
     double sr = 0.0, si = 0.0;
     if (K > 1 && L.rmap) {
-      pre_row_sum<W, CPLX, CK, K, RG>(C, r, xs, sr, si);
+      pre_row_sum<W, CPLX, CK, K, RG, LEAN>(C, r, xs, sr, si);
     } else visit_row_t<W, CPLX, CK, false>(C, r, [&](int64_t b, double fr, double fi, int) {
       const double* xp = xs + b * cw;
       if (CPLX) {
@@ -155,9 +155,9 @@ __device__ __forceinline__ void matvec_rows(const QesGroup& G, const QesLookup& 
 // (the lookups and gathers are latency bound; 7 -> 8 waves: 2.88 -> 2.68 ms
 // at L=32, no spills).  Wider states keep the compiler's allocation: under
 // the cap their cyclic canonicaliser spills to scratch.
-template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false>
+template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false, bool LEAN = false>
 __global__ void __launch_bounds__(256) QES_MV_ATTR matvec_kernel(QES_MV_PARAMS) {
-  matvec_rows<uint32_t, CPLX, CK, NB, PRE, K, RG>(QES_MV_ARGS);
+  matvec_rows<uint32_t, CPLX, CK, NB, PRE, K, RG, LEAN>(QES_MV_ARGS);
 }
 template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false>
 __global__ void __launch_bounds__(256) matvec_kernel_wide(QES_MV_PARAMS) {
@@ -169,10 +169,10 @@ __global__ void __launch_bounds__(256) matvec_kernel_wide(QES_MV_PARAMS) {
 // gathers real 115.2 -> 108.8 ms; complex dense gathers spill 76 B per lane
 // under the cap and slow down (146 -> 195 ms), as do the cyclic
 // canonicaliser's variants (112-384 B), so those keep the uncapped kernel.
-template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false>
+template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false, bool LEAN = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 matvec_kernel_wide8(QES_MV_PARAMS) {
-  matvec_rows<uint64_t, CPLX, CK, NB, PRE, K, RG>(QES_MV_ARGS);
+  matvec_rows<uint64_t, CPLX, CK, NB, PRE, K, RG, LEAN>(QES_MV_ARGS);
 }
 
 // rank-indexed gather vector: rxs[rank(rep_b) * cw ..] = N_b x_b (the ranks
@@ -495,12 +495,25 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   const int64_t nblocks = (n + block - 1) / block;
   if (nblocks > 0x7fffffffll) return fail("matvec grid too large");
   const QesPlan& P = op->plan;
+  // the row sums take the colex tables' layout from the state word (32-bit
+  // states exactly for ns <= 32)
+  if ((sizeof(W) == 4) != (B->group.ns <= 32)) return fail("matvec: state word does not match the site count");
   const bool pre = ws != nullptr && nbatch == 1;
   // lookups in flight per lane in the direct and canonicalising passes
   const int kb = !B->rmap ? 1 : (op->direct_batch > 0 ? op->direct_batch : (sizeof(W) == 4 ? 2 : 4));
   // rank gather: the basis's rank-indexed vector replaces the dense pre-scaled one
   // (the lookup must be the rank map: the row sum without it gathers by index)
   const bool rg = pre && op->rank_gather && B->lookup().rmap && B->rxs && B->rxs_cw == (CPLX ? 2 : 1) && kb > 1;
+  // lean kernels (real row sums, identity and at most one plain spin flip as
+  // base elements, no generic term groups, no top-row test): no run-time
+  // switch over base operations or term kinds, far fewer uniform values
+  // held across the row loop
+  const QesGroup& G = B->group;
+  bool lean = !CPLX && pre && kb > 1 && B->lookup().rmap && P.n_gen == 0 && !P.top_test && !G.has_phase &&
+              !G.has_bphase && G.nbase >= 1 && G.nbase <= 2 && op->lean != 0;
+  for (int b = 0; lean && b < G.nbase; ++b)
+    for (int k = 0; k < QES_MAX_BASE_OPS; ++k)
+      if (G.base_ops[b][k] != ((b == 1 && k == 0) ? QES_BOP_FLIP : QES_BOP_NONE)) lean = false;
   if (rg) {
     const int64_t g = std::min<int64_t>(nblocks, 256 * 64);
     hipLaunchKernelGGL(rank_scatter_kernel, dim3((unsigned)g), dim3(block), 0, st, B->lookup(), x, B->norms, B->rxs,
@@ -530,16 +543,21 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   } else if (rg) {                                                                                               \
     if (kb == 8) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 8, true>));                                 \
     else if (kb == 2) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 2, true>));                            \
+    else if (CK == QES_CANON_ROWS && lean) QES_LAUNCH_K((matvec_kernel_wide8<false, CK, 1, true, 4, true, true>)); \
     else if (CK == QES_CANON_ROWS) QES_LAUNCH_K((matvec_kernel_wide8<CPLX, CK, 1, true, 4, true>));              \
     else QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4, true>));                                         \
   } else if (sizeof(W) == 4) {                                                                                   \
     if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4>));                                     \
+    else if (pre && kb == 2 && lean && CK != QES_CANON_BRUTE)                                                    \
+      QES_LAUNCH_K((matvec_kernel<false, CK, 1, true, 2, false, true>));                                         \
     else if (pre && kb == 2) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2>));                                \
     else if (pre) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 1>));                                           \
     else if (ncols == 1) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, false, 1>));                                   \
     else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 4, false, 1>));                                                   \
   } else {                                                                                                       \
-    if (pre && kb >= 4 && !CPLX && CK == QES_CANON_ROWS) QES_LAUNCH_K((matvec_kernel_wide8<CPLX, CK, 1, true, 4>)); \
+    if (pre && kb >= 4 && lean && CK == QES_CANON_ROWS)                                                          \
+      QES_LAUNCH_K((matvec_kernel_wide8<false, CK, 1, true, 4, false, true>));                                   \
+    else if (pre && kb >= 4 && !CPLX && CK == QES_CANON_ROWS) QES_LAUNCH_K((matvec_kernel_wide8<CPLX, CK, 1, true, 4>)); \
     else if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4>));                           \
     else if (pre && kb == 2) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 2>));                           \
     else if (pre) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 1>));                                      \

@@ -42,15 +42,17 @@ struct RowCtx {
 // bytes of dynamic LDS used by load_ctx before the per-op tables
 #define QES_CTX_FIXED_LDS (QES_MAX_TX * 8 + QES_MAX_CLASSES * 8 + QES_MAX_CLASSES * 4 + 32)
 
-template <typename W, int CK>
+// LEAN: the group's base elements are the identity and at most one spin
+// flip (no phases), checked by the host before it launches a lean kernel
+template <typename W, int CK, bool LEAN = false>
 __device__ __forceinline__ Canon<W> canon_dispatch(const RowCtx& C, W s) {
   if (CK == QES_CANON_CYCLIC) {
     if (sizeof(W) * 8 == 32 || sizeof(W) * 8 == 64) {
-      if (C.G.ns == (int)(sizeof(W) * 8)) return canonicalize_cyclic<W, true>(C.G, s);
+      if (C.G.ns == (int)(sizeof(W) * 8)) return canonicalize_cyclic<W, true, LEAN>(C.G, s);
     }
-    return canonicalize_cyclic<W, false>(C.G, s);
+    return canonicalize_cyclic<W, false, LEAN>(C.G, s);
   } else if (CK == QES_CANON_ROWS) {
-    return canonicalize_rows<W>(C.G, C.s_lut, s);
+    return canonicalize_rows<W, LEAN>(C.G, C.s_lut, s);
   }
   return canonicalize<W>(C.G, s);
 }
@@ -208,9 +210,13 @@ __device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& mis
 // colex rank of representative b, QES_RXS_EMPTY elsewhere), so a lookup is
 // one load at the state's rank instead of a map word and then an x element;
 // the same elements are summed in the same order.
-template <typename W, bool CPLX, int CK, int K, bool RG = false>
+//
+// LEAN (real elements only): the group is lean (canon_dispatch), the plan
+// has no generic term groups and no top-row test -- the host checks.
+template <typename W, bool CPLX, int CK, int K, bool RG = false, bool LEAN = false>
 __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* __restrict__ xs, double& sr,
                                             double& si) {
+  static_assert(!(LEAN && CPLX), "lean row sums are real");
   const int ns = C.G.ns;
   const W mask = (W)C.G.mask;
   const int cw = CPLX ? 2 : 1;
@@ -226,11 +232,11 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
     }
   };
   auto emit = [&](W sp, double vr, double vi) {
-    const Canon<W> cn = canon_dispatch<W, CK>(C, sp);
+    const Canon<W> cn = canon_dispatch<W, CK, LEAN>(C, sp);
     int64_t b;
     if (RG) {   // the element of xs sits at the representative's rank (the marker: outside the sector)
       if (popc<W>(cn.rep) != C.L.n_up) return;
-      b = (int64_t)colex_rank(C.L, (uint64_t)cn.rep);
+      b = (int64_t)colex_rank_w<W>(C.L, (uint64_t)cn.rep);
       if (__double_as_longlong(xs[b * cw]) == (long long)QES_RXS_EMPTY) return;
     } else {
       b = lookup_rep<W>(C.L, cn.rep);
@@ -290,13 +296,13 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
               // operations) is no representative -- no lookup; it goes to the
               // canonicalising pass as a failed lookup would, so the sums keep
               // their order
-              if (CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
+              if (!LEAN && CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
                 on[k] = false;
                 fail |= (W)1 << pos[k];
                 continue;
               }
               // an exchange keeps the filling, so the state has a colex rank
-              rk[k] = colex_rank(C.L, (uint64_t)sp);
+              rk[k] = colex_rank_w<W>(C.L, (uint64_t)sp);
             }
           }
           int64_t bk[K];
@@ -382,10 +388,10 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
           if (on[k]) {
             int p1 = pos[k] - d;
             if (p1 < 0) p1 += ns;
-            cn[k] = canon_dispatch<W, CK>(C, r ^ (((W)1 << pos[k]) | ((W)1 << p1)));
+            cn[k] = canon_dispatch<W, CK, LEAN>(C, r ^ (((W)1 << pos[k]) | ((W)1 << p1)));
             // outside the filling (not for a group of spin flips at half filling): no rank
             on[k] = popc<W>(cn[k].rep) == C.L.n_up;
-            if (on[k]) rk[k] = colex_rank(C.L, (uint64_t)cn[k].rep);
+            if (on[k]) rk[k] = colex_rank_w<W>(C.L, (uint64_t)cn[k].rep);
           }
         }
         int64_t bk[K];
@@ -525,13 +531,13 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
               // rotation of one of its rows (or of their images under the base
               // operations) is no representative -- no lookup; it goes to the
               // canonicalising pass as a failed lookup would
-              if (CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
+              if (!LEAN && CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
                 on[k] = false;
                 __hip_atomic_fetch_or(sf + cl[k] * bd, 1ull << pos[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 continue;
               }
               // an exchange keeps the filling, so the state has a colex rank
-              rk[k] = colex_rank(C.L, (uint64_t)sp);
+              rk[k] = colex_rank_w<W>(C.L, (uint64_t)sp);
             }
           }
           bool hit[K];
@@ -627,10 +633,10 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
           if (on[k]) {
             int p1 = pos[k] - dk[k];
             if (p1 < 0) p1 += ns;
-            cn[k] = canon_dispatch<W, CK>(C, r ^ (((W)1 << pos[k]) | ((W)1 << p1)));
+            cn[k] = canon_dispatch<W, CK, LEAN>(C, r ^ (((W)1 << pos[k]) | ((W)1 << p1)));
             // outside the filling (not for a group of spin flips at half filling): no rank
             on[k] = popc<W>(cn[k].rep) == C.L.n_up;
-            if (on[k]) rk[k] = colex_rank(C.L, (uint64_t)cn[k].rep);
+            if (on[k]) rk[k] = colex_rank_w<W>(C.L, (uint64_t)cn[k].rep);
           }
         }
         int64_t bk[K];
@@ -698,7 +704,7 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
       }
     }
   }
-  for (int t = C.n_diag; t < C.n_diag + C.n_gen; ++t) {
+  for (int t = C.n_diag; !LEAN && t < C.n_diag + C.n_gen; ++t) {
     const QesTermGroup& g = C.groups[t];
     const int k = term_bits<W>(g, r);
     const double vr = sel4(g.re, k);

@@ -388,3 +388,37 @@ def test_prescaled_product_lookup_batches_agree(case):
         _lib.check(_lib.lib().qes_matvec_ws(H._device_op(), _lib.dptr(x), _lib.dptr(y_plain), 1, 1 if cplx else 0, 0,
                                             _lib.dptr(ws), ws.numel() * ws.element_size(), _lib.stream_ptr()),
                    "qes_matvec_ws")
+
+
+@pytest.mark.parametrize("case", [("chain", 24, 1, {"translation": 0, "parity": {"x": 1}}, 12),
+                                  ("chain", 34, 1, {"translation": 0, "parity": {"x": 1}}, 17),
+                                  ("square", 4, 4, {"translation": {"kx": 0, "ky": 0}, "parity": {"x": 1}}, 8),
+                                  SPACES[6][:5], SPACES[2][:5]],
+                         ids=lambda c: f"{c[0]}{c[1]}x{c[2]}")
+def test_lean_kernels_equal_the_general_ones(case):
+    """Real pre-scaled products of groups whose base elements are the
+    identity and one spin flip run kernels compiled for that case (option
+    lean): bitwise the products of the general kernels, with the dense and
+    the rank gathers, and of the one-lookup path; a group with a reflection
+    (not lean) is unaffected."""
+    import torch
+    from pyqusolver_amd.models import J1J2Model
+    kind, lx, ly, sym, n_up = case
+    lat, hs = _space(kind, lx, ly, sym, n_up)
+    H = J1J2Model(lat, J1=-1.0, J2=-0.45, hilbert_space=hs)
+    assert not H.needs_complex
+    x = torch.randn(hs.nh, dtype=torch.float64, device="cuda", generator=torch.Generator(device="cuda").manual_seed(5))
+    H.set_option("lean", 0)
+    y0 = H.matvec_device(x)
+    H.set_option("lean", 1)
+    assert torch.equal(H.matvec_device(x), y0)
+    H.set_option("direct_batch", 1)
+    assert torch.equal(H.matvec_device(x), y0)
+    H.set_option("direct_batch", -1)
+    if hs.nh > 10**6:
+        H.set_option("rank_gather", 1)
+        assert torch.equal(H.matvec_device(x), y0)
+        H.set_option("lean", 0)
+        assert torch.equal(H.matvec_device(x), y0)
+        H.set_option("rank_gather", 0)
+    H.set_option("lean", 1)
