@@ -39,6 +39,10 @@ struct RowCtx {
   int top_test;                 // row lattices: top-row test before a direct lookup
 };
 
+#ifndef QES_LEAN_KC
+#define QES_LEAN_KC 2
+#endif
+
 // bytes of dynamic LDS used by load_ctx before the per-op tables
 #define QES_CTX_FIXED_LDS (QES_MAX_TX * 8 + QES_MAX_CLASSES * 8 + QES_MAX_CLASSES * 4 + 32)
 
@@ -217,6 +221,10 @@ template <typename W, bool CPLX, int CK, int K, bool RG = false, bool LEAN = fal
 __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* __restrict__ xs, double& sr,
                                             double& si) {
   static_assert(!(LEAN && CPLX), "lean row sums are real");
+  // states canonicalised together in the real canonicalising pass (each
+  // canonicalisation holds its own candidate state; lean kernels keep fewer
+  // to stay within the scalar and vector register budget)
+  constexpr int KC = (LEAN && K > QES_LEAN_KC) ? QES_LEAN_KC : K;
   const int ns = C.G.ns;
   const W mask = (W)C.G.mask;
   const int cw = CPLX ? 2 : 1;
@@ -365,15 +373,15 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
           }
         }
       }
-      // states that are no representative: canonicalise K at a time, then
+      // states that are no representative: canonicalise KC at a time, then
       // their ranks, map words and gathers in flight together (rank map only)
       while (C.L.rmap && fail) {
-        int pos[K];
-        bool on[K];
-        Canon<W> cn[K];
-        uint64_t rk[K];
+        int pos[KC];
+        bool on[KC];
+        Canon<W> cn[KC];
+        uint64_t rk[KC];
   #pragma unroll
-        for (int k = 0; k < K; ++k) {
+        for (int k = 0; k < KC; ++k) {
           pos[k] = 0;
           on[k] = false;
           if (fail) {
@@ -383,7 +391,7 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
           }
         }
   #pragma unroll
-        for (int k = 0; k < K; ++k) {
+        for (int k = 0; k < KC; ++k) {
           rk[k] = 0;
           if (on[k]) {
             int p1 = pos[k] - d;
@@ -394,11 +402,11 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
             if (on[k]) rk[k] = colex_rank_w<W>(C.L, (uint64_t)cn[k].rep);
           }
         }
-        int64_t bk[K];
-        double xr[K], xi[K];
+        int64_t bk[KC];
+        double xr[KC], xi[KC];
         if (RG) {
   #pragma unroll
-          for (int k = 0; k < K; ++k) {
+          for (int k = 0; k < KC; ++k) {
             xr[k] = 0.0;
             xi[k] = 0.0;
             bk[k] = -1;
@@ -409,17 +417,17 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
             }
           }
   #pragma unroll
-          for (int k = 0; k < K; ++k) {
+          for (int k = 0; k < KC; ++k) {
             if (on[k]) bk[k] = __double_as_longlong(xr[k]) == (long long)QES_RXS_EMPTY ? -1 : 0;
             if (bk[k] < 0) xr[k] = xi[k] = 0.0;   // orbit outside the sector: skipped
           }
         } else {
-          ulonglong2 e[K];
+          ulonglong2 e[KC];
   #pragma unroll
-          for (int k = 0; k < K; ++k)
+          for (int k = 0; k < KC; ++k)
             e[k] = on[k] ? reinterpret_cast<const ulonglong2*>(C.L.rmap)[rk[k] >> 6] : make_ulonglong2(0ull, 0ull);
   #pragma unroll
-          for (int k = 0; k < K; ++k) {
+          for (int k = 0; k < KC; ++k) {
             bk[k] = on[k] ? rank_resolve(e[k], rk[k]) : -1;
             xr[k] = 0.0;
             xi[k] = 0.0;
@@ -431,7 +439,7 @@ __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* 
           }
         }
   #pragma unroll
-        for (int k = 0; k < K; ++k) {
+        for (int k = 0; k < KC; ++k) {
           if (bk[k] >= 0) {
             const int b0 = (int)((r >> pos[k]) & 1);
             const double* ev = C.s_ex + ((c * ns + pos[k]) * 2 + b0) * 2;
