@@ -539,6 +539,7 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
 #define QES_LAUNCH(CK)                                                                                           \
   if (rg && sizeof(W) == 4) {                                                                                    \
     if (kb == 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4, true>));                                      \
+    else if (lean && CK != QES_CANON_BRUTE) QES_LAUNCH_K((matvec_kernel<false, CK, 1, true, 2, true, true>));   \
     else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2, true>));                                              \
   } else if (rg) {                                                                                               \
     if (kb == 8) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 8, true>));                                 \

@@ -393,8 +393,8 @@ def test_prescaled_product_lookup_batches_agree(case):
 @pytest.mark.parametrize("case", [("chain", 24, 1, {"translation": 0, "parity": {"x": 1}}, 12),
                                   ("chain", 34, 1, {"translation": 0, "parity": {"x": 1}}, 17),
                                   ("square", 4, 4, {"translation": {"kx": 0, "ky": 0}, "parity": {"x": 1}}, 8),
-                                  SPACES[6][:5], SPACES[2][:5]],
-                         ids=lambda c: f"{c[0]}{c[1]}x{c[2]}")
+                                  SPACES[0][:5], SPACES[6][:5], SPACES[2][:5]],
+                         ids=lambda c: f"{c[0]}{c[1]}x{c[2]}-{c[4]}")
 def test_lean_kernels_equal_the_general_ones(case):
     """Real pre-scaled products of groups whose base elements are the
     identity and one spin flip run kernels compiled for that case (option
