@@ -42,12 +42,14 @@ HBM_PEAK_GBS = 8000.0
 # current kernels (profiles/matvec_kernel_square6x6_j1j2.md,
 # profiles/matvec_kernel_chain32_j1j2.md)
 TRAFFIC = {
-    # matvec_kernel_wide8<false,2,1,true,4,true> (rank gather): 241,582,958 KiB x 2.0 + 984,817 KiB x 1.0
-    "square6x6_rank_gather": 241582958 * 1024 * 2 + 984817 * 1024,
-    # matvec_kernel_wide8<false,2,1,true,4,false>: 329,314,499 KiB x 2.0 + 4,924,087 KiB x 1.0
-    "square6x6": 329314499 * 1024 * 2 + 4924087 * 1024,
-    # matvec_kernel<false,1,1,true,2>: 4,067,051 KiB x 2.0 + 73,384 KiB x 1.0
-    "chain32": 4067051 * 1024 * 2 + 73384 * 1024,
+    # matvec_kernel_wide8<false,2,1,true,4,true,true> (lean, rank gather): 250,813,510 KiB x 2.0 + 984,817 KiB x 1.0
+    "square6x6_rank_gather": 250813510 * 1024 * 2 + 984817 * 1024,
+    # matvec_kernel_wide8<false,2,1,true,4,false,true> (lean): 338,536,079 KiB x 2.0 + 1,969,635 KiB x 1.0
+    "square6x6": 338536079 * 1024 * 2 + 1969635 * 1024,
+    # matvec_kernel<false,1,1,true,2,true,true> (lean, rank gather): 4,991,532 KiB x 2.0 + 73,384 KiB x 1.0
+    "chain32_rank_gather": 4991532 * 1024 * 2 + 73384 * 1024,
+    # matvec_kernel<false,1,1,true,2,false,true> (lean): 4,059,637 KiB x 2.0 + 73,384 KiB x 1.0
+    "chain32": 4059637 * 1024 * 2 + 73384 * 1024,
 }
 
 def parse():
@@ -790,7 +792,9 @@ def measure_sector(a, config, steps, warmup, primary=True):
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": TRAFFIC.get(config + ("_rank_gather" if rg_on else "")),
                      "kernel": ("matvec_kernel_wide8" if wl["ns"] > 32 else "matvec_kernel") +
-                               (" (rank gather)" if rg_on else ""),
+                               # the profiled workloads run the lean kernels (spin flip only)
+                               ((" (lean, rank gather)" if rg_on else " (lean)") if config in ("square6x6", "chain32")
+                                else (" (rank gather)" if rg_on else "")),
                      "kernel_ms": prod_ms, "bytes_alg": b_alg,
                      "bytes_alg_formula": "N*(8 rep + 8 norm + 8 x_a + 8 y_a) + E_off*8 x gathers",
                      "launches": ("rank_scatter_kernel (N*24 B read, N 8-byte stores at the representatives' ranks)"
