@@ -671,6 +671,58 @@ __device__ __forceinline__ Canon<W> canonicalize_rows(const GV& G, const uint32_
   const W rowmask = (W)((Wr == 64) ? ~0ull : ((1ull << Wr) - 1ull));
   const int q_lo = (G.nty > 1) ? 0 : nrows - 1;
   constexpr int NB = LEAN ? 2 : QES_MAX_BASE;
+  if (LEAN) {
+    // identity and at most one spin flip: a row of the flipped state is the
+    // row of s complemented, so each row is extracted once and both LUT
+    // entries read; the (base, row) pairs attaining the smallest top row are
+    // collected while the minimum is found (bit 32 B + q, at most 32 rows:
+    // the host checks) and then evaluated in the order of the general loop
+    // (base element, then row), so the first minimiser is the same
+    const uint32_t rm = (uint32_t)rowmask;
+    const W flip = (~s) & mask;
+    uint32_t mtop = 0xffffu;
+    uint64_t cand = 0;
+    for (int q = q_lo; q < nrows; ++q) {
+      const uint32_t row = (uint32_t)(s >> (q * Wr)) & rm;
+      const uint32_t e0 = lut[row] & 0xffffu;
+      const uint64_t b0 = 1ull << q;
+      cand = e0 < mtop ? b0 : (e0 == mtop ? (cand | b0) : cand);
+      mtop = e0 < mtop ? e0 : mtop;
+      if (G.nbase > 1) {
+        const uint32_t e1 = lut[row ^ rm] & 0xffffu;
+        const uint64_t b1 = 1ull << (32 + q);
+        cand = e1 < mtop ? b1 : (e1 == mtop ? (cand | b1) : cand);
+        mtop = e1 < mtop ? e1 : mtop;
+      }
+    }
+    Canon<W> c;
+    c.rep = ~(W)0;
+    c.count = 0;
+    c.g = 0;
+    while (cand) {
+      const int bi = __builtin_ctzll(cand);
+      cand &= cand - 1;
+      const int B = bi >> 5, q = bi & 31;
+      const W vbB = B ? flip : s;
+      uint32_t am = lut[(uint32_t)(vbB >> (q * Wr)) & rm] >> 16;
+      const int b = nrows - 1 - q;
+      while (am) {
+        const int a = __builtin_ctz(am);
+        am &= am - 1;
+        const W t = rotl_rows<W>(vbB, a * G.cell_bits, Wr, (W)G.rowlo[a], mask);
+        const W v = rotl_ns<W>(t, b * Wr, ns, mask);
+        const int g = (a * G.nty + (G.nty > 1 ? b : 0)) * G.nbase + B;
+        if (v < c.rep) {
+          c.rep = v;
+          c.count = 1;
+          c.g = g;
+        } else if (v == c.rep) {
+          c.count += 1;
+        }
+      }
+    }
+    return c;
+  }
   W vb[QES_MAX_BASE];
 #pragma unroll
   for (int B = 0; B < NB; ++B) vb[B] = (B < G.nbase) ? base_of<W, LEAN>(G, B, s) : (W)0;

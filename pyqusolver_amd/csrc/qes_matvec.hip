@@ -510,7 +510,8 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   // held across the row loop
   const QesGroup& G = B->group;
   bool lean = !CPLX && pre && kb > 1 && B->lookup().rmap && P.n_gen == 0 && !P.top_test && !G.has_phase &&
-              !G.has_bphase && G.nbase >= 1 && G.nbase <= 2 && op->lean != 0;
+              !G.has_bphase && G.nbase >= 1 && G.nbase <= 2 && op->lean != 0 &&
+              (P.canon != QES_CANON_ROWS || G.ns / G.row_bits <= 32);
   for (int b = 0; lean && b < G.nbase; ++b)
     for (int k = 0; k < QES_MAX_BASE_OPS; ++k)
       if (G.base_ops[b][k] != ((b == 1 && k == 0) ? QES_BOP_FLIP : QES_BOP_NONE)) lean = false;
