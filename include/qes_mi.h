@@ -291,9 +291,9 @@ int qes_op_set_canonicalizer(qes_op_t op, int32_t kind);
  * (measured slower than the defaults, kept for measurements): "sell_grid"
  * (0 default: a block per 256 rows; 1..64: that many blocks per CU striding
  * over the rows) and "sell_lds" (bytes of LDS reserved per block, capping
- * the resident blocks).  "lean" (1 default, 0): real pre-scaled products
- * whose group has no base elements but the identity and one plain spin flip
- * and whose operator has no generic term groups run kernels compiled for
+ * the resident blocks).  "lean" (1 default, 0): pre-scaled products whose
+ * group has no base elements but the identity and one plain spin flip and
+ * whose operator has no generic term groups run kernels compiled for
  * exactly that (same sums in the same order).  The other options change the
  * order of the sums (results agree to rounding). */
 int qes_op_set_option(qes_op_t op, const char* name, int64_t value);

@@ -159,9 +159,9 @@ template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false, bool LEAN
 __global__ void __launch_bounds__(256) QES_MV_ATTR matvec_kernel(QES_MV_PARAMS) {
   matvec_rows<uint32_t, CPLX, CK, NB, PRE, K, RG, LEAN>(QES_MV_ARGS);
 }
-template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false>
+template <bool CPLX, int CK, int NB, bool PRE, int K, bool RG = false, bool LEAN = false>
 __global__ void __launch_bounds__(256) matvec_kernel_wide(QES_MV_PARAMS) {
-  matvec_rows<uint64_t, CPLX, CK, NB, PRE, K, RG>(QES_MV_ARGS);
+  matvec_rows<uint64_t, CPLX, CK, NB, PRE, K, RG, LEAN>(QES_MV_ARGS);
 }
 // The same capped at 8 resident waves per SIMD, for the row-LUT canonicaliser
 // with four lookups in flight: measured on MI355X at 6x6 J1-J2 (|G| = 72),
@@ -504,12 +504,12 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   // rank gather: the basis's rank-indexed vector replaces the dense pre-scaled one
   // (the lookup must be the rank map: the row sum without it gathers by index)
   const bool rg = pre && op->rank_gather && B->lookup().rmap && B->rxs && B->rxs_cw == (CPLX ? 2 : 1) && kb > 1;
-  // lean kernels (real row sums, identity and at most one plain spin flip as
+  // lean kernels (identity and at most one plain spin flip as
   // base elements, no generic term groups, no top-row test): no run-time
   // switch over base operations or term kinds, far fewer uniform values
   // held across the row loop
   const QesGroup& G = B->group;
-  bool lean = !CPLX && pre && kb > 1 && B->lookup().rmap && P.n_gen == 0 && !P.top_test && !G.has_phase &&
+  bool lean = pre && kb > 1 && B->lookup().rmap && P.n_gen == 0 && !P.top_test && !G.has_phase &&
               !G.has_bphase && G.nbase >= 1 && G.nbase <= 2 && op->lean != 0 &&
               (P.canon != QES_CANON_ROWS || G.ns / G.row_bits <= 32);
   for (int b = 0; lean && b < G.nbase; ++b)
@@ -540,25 +540,27 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
 #define QES_LAUNCH(CK)                                                                                           \
   if (rg && sizeof(W) == 4) {                                                                                    \
     if (kb == 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4, true>));                                      \
-    else if (lean && CK != QES_CANON_BRUTE) QES_LAUNCH_K((matvec_kernel<false, CK, 1, true, 2, true, true>));   \
+    else if (lean && CK != QES_CANON_BRUTE) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2, true, true>));    \
     else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2, true>));                                              \
   } else if (rg) {                                                                                               \
     if (kb == 8) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 8, true>));                                 \
     else if (kb == 2) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 2, true>));                            \
-    else if (CK == QES_CANON_ROWS && lean) QES_LAUNCH_K((matvec_kernel_wide8<false, CK, 1, true, 4, true, true>)); \
+    else if (CK == QES_CANON_ROWS && lean) QES_LAUNCH_K((matvec_kernel_wide8<CPLX, CK, 1, true, 4, true, true>)); \
     else if (CK == QES_CANON_ROWS) QES_LAUNCH_K((matvec_kernel_wide8<CPLX, CK, 1, true, 4, true>));              \
     else QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4, true>));                                         \
   } else if (sizeof(W) == 4) {                                                                                   \
     if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4>));                                     \
     else if (pre && kb == 2 && lean && CK != QES_CANON_BRUTE)                                                    \
-      QES_LAUNCH_K((matvec_kernel<false, CK, 1, true, 2, false, true>));                                         \
+      QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2, false, true>));                                          \
     else if (pre && kb == 2) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2>));                                \
     else if (pre) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 1>));                                           \
     else if (ncols == 1) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, false, 1>));                                   \
     else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 4, false, 1>));                                                   \
   } else {                                                                                                       \
-    if (pre && kb >= 4 && lean && CK == QES_CANON_ROWS)                                                          \
-      QES_LAUNCH_K((matvec_kernel_wide8<false, CK, 1, true, 4, false, true>));                                   \
+    if (pre && kb >= 4 && lean && !CPLX && CK == QES_CANON_ROWS)                                                 \
+      QES_LAUNCH_K((matvec_kernel_wide8<CPLX, CK, 1, true, 4, false, true>));                                    \
+    else if (pre && kb >= 4 && lean && CK == QES_CANON_ROWS)                                                     \
+      QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4, false, true>));                                     \
     else if (pre && kb >= 4 && !CPLX && CK == QES_CANON_ROWS) QES_LAUNCH_K((matvec_kernel_wide8<CPLX, CK, 1, true, 4>)); \
     else if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4>));                           \
     else if (pre && kb == 2) QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 2>));                           \

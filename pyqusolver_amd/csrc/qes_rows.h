@@ -215,12 +215,11 @@ __device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& mis
 // one load at the state's rank instead of a map word and then an x element;
 // the same elements are summed in the same order.
 //
-// LEAN (real elements only): the group is lean (canon_dispatch), the plan
-// has no generic term groups and no top-row test -- the host checks.
+// LEAN: the group is lean (canon_dispatch: no phases), the plan has no
+// generic term groups and no top-row test -- the host checks.
 template <typename W, bool CPLX, int CK, int K, bool RG = false, bool LEAN = false>
 __device__ __forceinline__ void pre_row_sum(const RowCtx& C, W r, const double* __restrict__ xs, double& sr,
                                             double& si) {
-  static_assert(!(LEAN && CPLX), "lean row sums are real");
   // states canonicalised together in the real canonicalising pass (each
   // canonicalisation holds its own candidate state; lean kernels keep fewer
   // to stay within the scalar and vector register budget)

@@ -422,3 +422,37 @@ def test_lean_kernels_equal_the_general_ones(case):
         assert torch.equal(H.matvec_device(x), y0)
         H.set_option("rank_gather", 0)
     H.set_option("lean", 1)
+
+
+@pytest.mark.parametrize("case", [("chain", 20, 1, {"translation": 3, "parity": {"x": 1}}, 10),
+                                  ("square", 6, 6, {"translation": {"kx": 1, "ky": 2}, "parity": {"x": 1}}, 18)],
+                         ids=lambda c: f"{c[0]}{c[1]}x{c[2]}")
+def test_lean_kernels_complex(case):
+    """Complex momentum sectors with a spin flip run the lean kernels too:
+    the same products as the general kernels (the same sums in the same
+    order; complex multiply-adds may contract differently in the two
+    compilations, so to 1e-15 relative), with the dense and the rank
+    gathers."""
+    import torch
+    from pyqusolver_amd.models import J1J2Model
+    kind, lx, ly, sym, n_up = case
+    lat, hs = _space(kind, lx, ly, sym, n_up)
+    H = J1J2Model(lat, J1=-1.0, J2=-0.45, hilbert_space=hs)
+    assert H.needs_complex
+    g = torch.Generator(device="cuda").manual_seed(6)
+    x = torch.complex(torch.randn(hs.nh, dtype=torch.float64, device="cuda", generator=g),
+                      torch.randn(hs.nh, dtype=torch.float64, device="cuda", generator=g))
+
+    def close(a, b):
+        return (torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item() <= 1e-15
+    H.set_option("lean", 0)
+    y0 = H.matvec_device(x)
+    H.set_option("lean", 1)
+    assert close(H.matvec_device(x), y0)
+    if kind == "chain":
+        H.set_option("rank_gather", 1)
+        y1 = H.matvec_device(x)
+        H.set_option("lean", 0)
+        assert close(H.matvec_device(x), y1) and close(y1, y0)
+        H.set_option("rank_gather", 0)
+        H.set_option("lean", 1)
