@@ -127,7 +127,7 @@ struct AsmOut {
   double* diag_c;                       // tagged: [c][row]
 };
 
-template <typename W, bool CPLX, int CK, int PASS>
+template <typename W, bool CPLX, int CK, int PASS, bool LEAN = false>
 __global__ void __launch_bounds__(256) asm_rows_kernel(QesGroup G, QesLookup L, QesPlan P,
                                                        const double* __restrict__ chi_re,
                                                        const double* __restrict__ chi_im,
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) asm_rows_kernel(QesGroup G, QesLookup L, 
     }
   }
   if (plain) ++k;   // entry 0 is the diagonal
-  visit_row<W, CPLX, CK>(C, r, [&](int64_t b, double fr, double fi, int tag) {
+  visit_row<W, CPLX, CK, LEAN>(C, r, [&](int64_t b, double fr, double fi, int tag) {
     const double vr = fr * inv_na;
     const double vi = CPLX ? fi * inv_na : 0.0;
     int cpl = 0;
@@ -529,10 +529,17 @@ static int assemble_t(Op* op, int want_dict, hipStream_t st) {
   O.tagged = tagged ? 1 : 0;
   O.ncpl = op->n_coupling;
   O.excpl = op->d_excpl;
+  // lean table passes where the group and plan allow (lean_group)
+  const bool lean = lean_group(op) && P.canon != QES_CANON_BRUTE;
 #define QES_ASM(CK, PASS)                                                                                           \
-  hipLaunchKernelGGL((asm_rows_kernel<W, CPLX, CK, PASS>), grid, dim3(256), op->lds_bytes, st, B->group,          \
-                     B->lookup(), P, B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_lut, B->norms, \
-                     n, O)
+  if (lean)                                                                                                         \
+    hipLaunchKernelGGL((asm_rows_kernel<W, CPLX, CK, PASS, true>), grid, dim3(256), op->lds_bytes, st, B->group,  \
+                       B->lookup(), P, B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_lut,       \
+                       B->norms, n, O);                                                                             \
+  else                                                                                                              \
+    hipLaunchKernelGGL((asm_rows_kernel<W, CPLX, CK, PASS>), grid, dim3(256), op->lds_bytes, st, B->group,        \
+                       B->lookup(), P, B->chi_re, B->chi_im, op->d_gen, op->n_diag, op->d_exval, op->d_lut,       \
+                       B->norms, n, O)
 #define QES_ASM_COUNT(CK) QES_ASM(CK, PASS_COUNT)
 #define QES_ASM_DICT(CK) QES_ASM(CK, PASS_DICT)
 #define QES_ASM_FILL(CK) QES_ASM(CK, PASS_FILL)

@@ -206,6 +206,7 @@ int count_offdiag(const Op* op, int64_t* e_off, hipStream_t st, int64_t* outside
 int op_diagonal(const Op* op, double* dre, double* dim, hipStream_t st);
 int projected_matvec(const Op* op, const Basis* out, const double* x, double* y, hipStream_t st);
 int build_plan(Op* op, int canon_override);   // -1: automatic
+bool lean_group(const Op* op);                 // the lean kernels' conditions on group and plan
 // assembled operator (qes_assemble.hip)
 int op_assemble(Op* op, int enable, int want_dict, hipStream_t st);
 void free_assembly(Op* op);

@@ -485,6 +485,20 @@ int build_plan(Op* op, int canon_override) {
     default: MACRO(QES_CANON_BRUTE); break;                \
   }
 
+// The lean kernels' conditions on the group and the plan: base elements the
+// identity and at most one plain spin flip, no phases, no generic term
+// groups, no top-row test, at most 32 rows for the row canonicaliser.
+bool lean_group(const Op* op) {
+  const QesGroup& G = op->basis->group;
+  const QesPlan& P = op->plan;
+  bool lean = op->lean != 0 && P.n_gen == 0 && !P.top_test && !G.has_phase && !G.has_bphase && G.nbase >= 1 &&
+              G.nbase <= 2 && (P.canon != QES_CANON_ROWS || G.ns / G.row_bits <= 32);
+  for (int b = 0; lean && b < G.nbase; ++b)
+    for (int k = 0; k < QES_MAX_BASE_OPS; ++k)
+      if (G.base_ops[b][k] != ((b == 1 && k == 0) ? QES_BOP_FLIP : QES_BOP_NONE)) lean = false;
+  return lean;
+}
+
 template <typename W, bool CPLX>
 static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatch, int accumulate, double* ws,
                          hipStream_t st) {
@@ -508,13 +522,7 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   // base elements, no generic term groups, no top-row test): no run-time
   // switch over base operations or term kinds, far fewer uniform values
   // held across the row loop
-  const QesGroup& G = B->group;
-  bool lean = pre && kb > 1 && B->lookup().rmap && P.n_gen == 0 && !P.top_test && !G.has_phase &&
-              !G.has_bphase && G.nbase >= 1 && G.nbase <= 2 && op->lean != 0 &&
-              (P.canon != QES_CANON_ROWS || G.ns / G.row_bits <= 32);
-  for (int b = 0; lean && b < G.nbase; ++b)
-    for (int k = 0; k < QES_MAX_BASE_OPS; ++k)
-      if (G.base_ops[b][k] != ((b == 1 && k == 0) ? QES_BOP_FLIP : QES_BOP_NONE)) lean = false;
+  const bool lean = pre && kb > 1 && B->lookup().rmap && lean_group(op);
   if (rg) {
     const int64_t g = std::min<int64_t>(nblocks, 256 * 64);
     hipLaunchKernelGGL(rank_scatter_kernel, dim3((unsigned)g), dim3(block), 0, st, B->lookup(), x, B->norms, B->rxs,

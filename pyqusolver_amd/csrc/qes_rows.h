@@ -68,12 +68,12 @@ __device__ __forceinline__ Canon<W> canon_dispatch(const RowCtx& C, W s) {
 // target lies outside the sector (another U(1)/Z2 filling, or an orbit of
 // zero norm): the targets the reference's kernel skips
 // (Algebra/Hilbert/matrix_builder.py:1366-1371).
-template <typename W, bool CPLX, int CK, bool NORM, class F, class M>
+template <typename W, bool CPLX, int CK, bool NORM, bool LEAN = false, class F, class M>
 __device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& miss) {
   const int ns = C.G.ns;
   const W mask = (W)C.G.mask;
   auto emit = [&](W sp, double vr, double vi, int cpl) {
-    const Canon<W> cn = canon_dispatch<W, CK>(C, sp);
+    const Canon<W> cn = canon_dispatch<W, CK, LEAN>(C, sp);
     int cnt_unused = 0;
     const int64_t b = (C.L.packed && !C.L.hash) ? lookup_packed(C.L, (uint64_t)cn.rep, &cnt_unused)
                                                 : lookup_rep<W>(C.L, cn.rep);
@@ -136,7 +136,7 @@ __device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& mis
         const W sp = r ^ (((W)1 << p) | ((W)1 << p1));
         // row lattices: a state whose top row is not the smallest row
         // rotation is no representative -- skip its lookup
-        if (CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
+        if (!LEAN && CK == QES_CANON_ROWS && C.top_test && !rows_top_ok<W>(C.G, C.s_lut, sp)) {
           fail |= (W)1 << p;
           continue;
         }
@@ -187,7 +187,7 @@ __device__ __forceinline__ void visit_row_t(const RowCtx& C, W r, F&& f, M&& mis
     }
   }
   // generic groups
-  for (int t = C.n_diag; t < C.n_diag + C.n_gen; ++t) {
+  for (int t = C.n_diag; !LEAN && t < C.n_diag + C.n_gen; ++t) {
     const QesTermGroup& g = C.groups[t];
     const int k = term_bits<W>(g, r);
     const double vr = sel4(g.re, k);
@@ -729,9 +729,11 @@ __device__ __forceinline__ void visit_row_m(const RowCtx& C, W r, F&& f, M&& mis
   visit_row_t<W, CPLX, CK, true>(C, r, f, miss);
 }
 
-template <typename W, bool CPLX, int CK, class F>
+// LEAN: as in canon_dispatch, and the plan has no generic term groups and
+// no top-row test (the host checks)
+template <typename W, bool CPLX, int CK, bool LEAN = false, class F>
 __device__ __forceinline__ void visit_row(const RowCtx& C, W r, F&& f) {
-  visit_row_t<W, CPLX, CK, true>(C, r, f, []() {});
+  visit_row_t<W, CPLX, CK, true, LEAN>(C, r, f, []() {});
 }
 
 // Load the per-block tables into LDS; returns the context.

@@ -422,6 +422,16 @@ def test_lean_kernels_equal_the_general_ones(case):
         assert torch.equal(H.matvec_device(x), y0)
         H.set_option("rank_gather", 0)
     H.set_option("lean", 1)
+    # the table built by the lean passes holds the same elements in the same order
+    if hs.nh < 10**8:
+        H.set_option("lean", 0)
+        H.assemble(True)
+        ya = H.matvec_device(x)
+        H.assemble(False)
+        H.set_option("lean", 1)
+        H.assemble(True)
+        assert torch.equal(H.matvec_device(x), ya)
+        H.assemble(False)
 
 
 @pytest.mark.parametrize("case", [("chain", 20, 1, {"translation": 3, "parity": {"x": 1}}, 10),
@@ -456,3 +466,11 @@ def test_lean_kernels_complex(case):
         assert close(H.matvec_device(x), y1) and close(y1, y0)
         H.set_option("rank_gather", 0)
         H.set_option("lean", 1)
+        H.set_option("lean", 0)
+        H.assemble(True)
+        ya = H.matvec_device(x)
+        H.assemble(False)
+        H.set_option("lean", 1)
+        H.assemble(True)
+        assert torch.equal(H.matvec_device(x), ya)
+        H.assemble(False)
