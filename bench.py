@@ -42,10 +42,10 @@ HBM_PEAK_GBS = 8000.0
 # current kernels (profiles/matvec_kernel_square6x6_j1j2.md,
 # profiles/matvec_kernel_chain32_j1j2.md)
 TRAFFIC = {
-    # matvec_kernel_wide8<false,2,1,true,4,true,true> (lean, rank gather): 247,154,614 KiB x 2.0 + 984,817 KiB x 1.0
-    "square6x6_rank_gather": 247154614 * 1024 * 2 + 984817 * 1024,
-    # matvec_kernel_wide8<false,2,1,true,4,false,true> (lean): 339,840,493 KiB x 2.0 + 1,969,635 KiB x 1.0
-    "square6x6": 339840493 * 1024 * 2 + 1969635 * 1024,
+    # matvec_kernel_wide8<false,2,1,true,4,true,true> (lean, rank gather): 244,586,596 KiB x 2.0 + 984,817 KiB x 1.0
+    "square6x6_rank_gather": 244586596 * 1024 * 2 + 984817 * 1024,
+    # matvec_kernel_wide8<false,2,1,true,4,false,true> (lean): 335,259,233 KiB x 2.0 + 4,924,087 KiB x 1.0
+    "square6x6": 335259233 * 1024 * 2 + 4924087 * 1024,
     # matvec_kernel<false,1,1,true,2,true,true> (lean, rank gather): 4,991,532 KiB x 2.0 + 73,384 KiB x 1.0
     "chain32_rank_gather": 4991532 * 1024 * 2 + 73384 * 1024,
     # matvec_kernel<false,1,1,true,2,false,true> (lean): 4,059,637 KiB x 2.0 + 73,384 KiB x 1.0
