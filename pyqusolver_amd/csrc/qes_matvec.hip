@@ -514,7 +514,11 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   if ((sizeof(W) == 4) != (B->group.ns <= 32)) return fail("matvec: state word does not match the site count");
   const bool pre = ws != nullptr && nbatch == 1;
   // lookups in flight per lane in the direct and canonicalising passes
-  const int kb = !B->rmap ? 1 : (op->direct_batch > 0 ? op->direct_batch : (sizeof(W) == 4 ? 2 : 4));
+  // (32-bit states: two by default -- four spill in the general kernel --
+  // and four in the real lean kernels, L=32 1.91 -> 1.85 ms with rank gather)
+  const bool lean_ok = lean_group(op);
+  const bool four32 = lean_ok && pre && !CPLX;
+  const int kb = !B->rmap ? 1 : (op->direct_batch > 0 ? op->direct_batch : (sizeof(W) == 4 && !four32 ? 2 : 4));
   // rank gather: the basis's rank-indexed vector replaces the dense pre-scaled one
   // (the lookup must be the rank map: the row sum without it gathers by index)
   const bool rg = pre && op->rank_gather && B->lookup().rmap && B->rxs && B->rxs_cw == (CPLX ? 2 : 1) && kb > 1;
@@ -522,7 +526,7 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
   // base elements, no generic term groups, no top-row test): no run-time
   // switch over base operations or term kinds, far fewer uniform values
   // held across the row loop
-  const bool lean = pre && kb > 1 && B->lookup().rmap && lean_group(op);
+  const bool lean = pre && kb > 1 && B->lookup().rmap && lean_ok;
   if (rg) {
     const int64_t g = std::min<int64_t>(nblocks, 256 * 64);
     hipLaunchKernelGGL(rank_scatter_kernel, dim3((unsigned)g), dim3(block), 0, st, B->lookup(), x, B->norms, B->rxs,
@@ -547,7 +551,8 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
                      nbatch, j0, ncols, accumulate, op->xcd_remap)
 #define QES_LAUNCH(CK)                                                                                           \
   if (rg && sizeof(W) == 4) {                                                                                    \
-    if (kb == 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4, true>));                                      \
+    if (kb == 4 && lean && CK != QES_CANON_BRUTE) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4, true, true>)); \
+    else if (kb == 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4, true>));                                 \
     else if (lean && CK != QES_CANON_BRUTE) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2, true, true>));    \
     else QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2, true>));                                              \
   } else if (rg) {                                                                                               \
@@ -557,7 +562,8 @@ static int launch_matvec(const Op* op, const double* x, double* y, int64_t nbatc
     else if (CK == QES_CANON_ROWS) QES_LAUNCH_K((matvec_kernel_wide8<CPLX, CK, 1, true, 4, true>));              \
     else QES_LAUNCH_K((matvec_kernel_wide<CPLX, CK, 1, true, 4, true>));                                         \
   } else if (sizeof(W) == 4) {                                                                                   \
-    if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4>));                                     \
+    if (pre && kb >= 4 && lean && CK != QES_CANON_BRUTE) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4, false, true>)); \
+    else if (pre && kb >= 4) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 4>));                                \
     else if (pre && kb == 2 && lean && CK != QES_CANON_BRUTE)                                                    \
       QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2, false, true>));                                          \
     else if (pre && kb == 2) QES_LAUNCH_K((matvec_kernel<CPLX, CK, 1, true, 2>));                                \
