@@ -46,10 +46,10 @@ TRAFFIC = {
     "square6x6_rank_gather": 244586596 * 1024 * 2 + 984817 * 1024,
     # matvec_kernel_wide8<false,2,1,true,4,false,true> (lean): 335,259,233 KiB x 2.0 + 4,924,087 KiB x 1.0
     "square6x6": 335259233 * 1024 * 2 + 4924087 * 1024,
-    # matvec_kernel<false,1,1,true,2,true,true> (lean, rank gather): 4,991,532 KiB x 2.0 + 73,384 KiB x 1.0
-    "chain32_rank_gather": 4991532 * 1024 * 2 + 73384 * 1024,
-    # matvec_kernel<false,1,1,true,2,false,true> (lean): 4,059,637 KiB x 2.0 + 73,384 KiB x 1.0
-    "chain32": 4059637 * 1024 * 2 + 73384 * 1024,
+    # matvec_kernel<false,1,1,true,4,true,true> (lean, rank gather): 5,143,062 KiB x 2.0 + 73,384 KiB x 1.0
+    "chain32_rank_gather": 5143062 * 1024 * 2 + 73384 * 1024,
+    # matvec_kernel<false,1,1,true,4,false,true> (lean): 4,242,720 KiB x 2.0 + 220,151 KiB x 1.0
+    "chain32": 4242720 * 1024 * 2 + 220151 * 1024,
 }
 
 def parse():
