@@ -208,7 +208,10 @@ class Hamiltonian:
     def set_option(self, name: str, value: int):
         """Kernel tuning knob (bond_order_uniform, direct_lookup, xcd_remap, top_row_test, direct_batch,
         asm_panel_cols: columns per panel of the next assembled table, -1 automatic, 0 off; rank_gather:
-        1 builds the basis's rank-indexed gather vector for the matrix-free product, 0 frees it)."""
+        1 builds the basis's rank-indexed gather vector for the matrix-free product, 0 frees it; lean:
+        1 (default) runs the kernels compiled for groups of translations and one spin flip where they
+        apply, 0 the general ones -- the same products either way; sell_grid, sell_lds: launch shape
+        of the table product)."""
         _lib.check(_lib.lib().qes_op_set_option(self._device_op(), name.encode(), int(value)), "qes_op_set_option")
 
     @property
